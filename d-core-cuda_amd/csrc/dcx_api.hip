@@ -1,0 +1,685 @@
+// C-ABI entry points of the dcx library: contexts, device graphs and the
+// decomposition drivers.  Every entry point returns a dcx_status; HIP
+// failures are caught at each call and reported, never ignored.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <vector>
+
+#include "dcx_internal.hpp"
+
+using namespace dcx;
+
+struct dcx_ctx {
+    dcx_config cfg{};
+    int device = 0;
+    int num_cus = 0;
+    int coop_grid = 0;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    std::string err;
+    dcx_timing timing{};
+    // grown-on-demand device scratch
+    void* scratch = nullptr;
+    size_t scratch_bytes = 0;
+    uint32_t* ctrs = nullptr;  // small counters: 64 words
+    int32_t* d_scalars = nullptr;  // 64 words
+    int32_t* h_scalars = nullptr;  // pinned mirror
+    hipEvent_t ev[8] = {};
+};
+
+struct dcx_graph {
+    dcx_ctx* ctx = nullptr;
+    uint32_t V = 0;
+    uint64_t E = 0;
+    uint64_t out_cap = 0, in_cap = 0;
+    uint32_t *out_off = nullptr, *in_off = nullptr, *out_nbr = nullptr, *in_nbr = nullptr;
+    int32_t *out_deg = nullptr, *in_deg = nullptr;
+    // resident results
+    int32_t* kmaxes = nullptr;
+    int32_t kmax = -1;
+    bool kmax_valid = false;
+    uint32_t* order = nullptr;   // [V] vertices by kmax, descending
+    uint32_t* vk = nullptr;      // [kmax+2] members per k
+    uint32_t* hist = nullptr;    // [kmax+2]
+    size_t vk_cap = 0;
+    std::vector<uint32_t> h_vk;  // host copy of vk
+    int32_t* rows = nullptr;     // (kmax+1) x V
+    size_t rows_cap = 0;         // rows allocated
+    int32_t* lmax_rows = nullptr;
+    size_t lmax_rows_cap = 0;
+    int32_t lmax = -1;
+    bool rows_valid = false;
+};
+
+namespace {
+
+struct Fail {
+    dcx_status st;
+};
+
+void set_err(dcx_ctx* c, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    c->err = buf;
+}
+
+#define HIPX(ctx, call)                                                                      \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess) {                                                              \
+            set_err((ctx), "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, \
+                    __LINE__);                                                               \
+            throw Fail{e_ == hipErrorOutOfMemory ? DCX_ERR_OUT_OF_MEMORY : DCX_ERR_HIP};     \
+        }                                                                                    \
+    } while (0)
+
+template <class F>
+dcx_status guarded(dcx_ctx* c, F&& f) {
+    if (!c) return DCX_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->err.clear();
+    try {
+        if (hipSetDevice(c->device) != hipSuccess) {
+            set_err(c, "hipSetDevice(%d) failed", c->device);
+            return DCX_ERR_NO_DEVICE;
+        }
+        return f();
+    } catch (const Fail& x) {
+        return x.st;
+    } catch (const std::bad_alloc&) {
+        set_err(c, "host allocation failed");
+        return DCX_ERR_OUT_OF_MEMORY;
+    }
+}
+
+template <class T>
+void dev_alloc(dcx_ctx* c, T** p, size_t n) {
+    *p = nullptr;
+    if (n == 0) n = 1;
+    HIPX(c, hipMalloc(reinterpret_cast<void**>(p), n * sizeof(T)));
+}
+
+template <class T>
+void dev_free(T*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+void ensure_scratch(dcx_ctx* c, size_t bytes) {
+    if (c->scratch_bytes >= bytes) return;
+    dev_free(c->scratch);
+    c->scratch_bytes = 0;
+    HIPX(c, hipMalloc(&c->scratch, bytes));
+    c->scratch_bytes = bytes;
+}
+
+void sync(dcx_ctx* c) { HIPX(c, hipStreamSynchronize(c->stream)); }
+
+float elapsed(dcx_ctx* c, int a, int b) {
+    float ms = 0;
+    HIPX(c, hipEventElapsedTime(&ms, c->ev[a], c->ev[b]));
+    return ms;
+}
+
+PassSpec kmax_spec(const dcx_graph* g) {
+    // swapped roles: key = in-lists, constraint = out-lists, k = 0
+    PassSpec p{};
+    p.V = g->V;
+    p.K = Csr{g->in_off, g->in_deg, g->in_nbr};
+    p.C = Csr{g->out_off, g->out_deg, g->out_nbr};
+    p.kmaxes = nullptr;
+    p.order = nullptr;
+    p.nmembers = g->V;
+    p.k = 0;
+    p.lval = g->kmaxes;
+    return p;
+}
+
+PassSpec row_spec(const dcx_graph* g, int32_t k, int32_t* lval, bool filtered) {
+    PassSpec p{};
+    p.V = g->V;
+    p.K = Csr{g->out_off, g->out_deg, g->out_nbr};
+    p.C = Csr{g->in_off, g->in_deg, g->in_nbr};
+    p.kmaxes = filtered ? g->kmaxes : nullptr;
+    p.order = filtered ? g->order : nullptr;
+    p.nmembers = filtered ? (k <= g->kmax ? g->h_vk[k] : 0u) : g->V;
+    p.k = k;
+    p.lval = lval;
+    return p;
+}
+
+// Runs one pass on the whole grid (cooperative launch); returns lmax of it.
+int32_t run_grid_pass(dcx_ctx* c, const PassSpec& p) {
+    ensure_scratch(c, (size_t)kScratchArrays * std::max<uint32_t>(p.V, 1) * 4);
+    PassScratch s = scratch_slot(c->scratch, p.V, 0);
+    HIPX(c, launch_pass_grid(p, s, c->ctrs, c->d_scalars, c->coop_grid, c->stream));
+    HIPX(c, hipMemcpyAsync(c->h_scalars, c->d_scalars, sizeof(int32_t), hipMemcpyDeviceToHost,
+                           c->stream));
+    sync(c);
+    return c->h_scalars[0];
+}
+
+void compute_kmax(dcx_ctx* c, dcx_graph* g) {
+    if (!g->kmaxes) dev_alloc(c, &g->kmaxes, g->V);
+    if (g->V == 0) {
+        g->kmax = -1;
+        g->kmax_valid = true;
+        return;
+    }
+    HIPX(c, hipEventRecord(c->ev[0], c->stream));
+    g->kmax = run_grid_pass(c, kmax_spec(g));
+    HIPX(c, hipEventRecord(c->ev[1], c->stream));
+    // vertex order by kmax, members per k
+    size_t need = (size_t)g->kmax + 2;
+    if (g->vk_cap < need) {
+        dev_free(g->vk);
+        dev_free(g->hist);
+        dev_alloc(c, &g->vk, need);
+        dev_alloc(c, &g->hist, need);
+        g->vk_cap = need;
+    }
+    if (!g->order) dev_alloc(c, &g->order, g->V);
+    HIPX(c, launch_order_by_kmax(g->kmaxes, g->V, g->kmax, g->hist, g->vk, g->order, c->stream));
+    g->h_vk.assign(need, 0);
+    HIPX(c, hipMemcpyAsync(g->h_vk.data(), g->vk, need * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                           c->stream));
+    HIPX(c, hipEventRecord(c->ev[2], c->stream));
+    sync(c);
+    c->timing.kmax_ms = elapsed(c, 0, 1);
+    c->timing.order_ms = elapsed(c, 1, 2);
+    g->kmax_valid = true;
+}
+
+void compute_rows(dcx_ctx* c, dcx_graph* g, int variant) {
+    const uint32_t V = g->V;
+    const int32_t K = g->kmax;
+    g->rows_valid = false;
+    if (K < 0) {
+        g->lmax = 0;
+        g->rows_valid = true;
+        return;
+    }
+    const size_t nrows = (size_t)K + 1;
+    if (g->rows_cap < nrows * V) {
+        dev_free(g->rows);
+        dev_alloc(c, &g->rows, nrows * V);
+        g->rows_cap = nrows * V;
+    }
+    if (g->lmax_rows_cap < nrows) {
+        dev_free(g->lmax_rows);
+        dev_alloc(c, &g->lmax_rows, nrows);
+        g->lmax_rows_cap = nrows;
+    }
+    HIPX(c, hipEventRecord(c->ev[3], c->stream));
+    HIPX(c, hipMemsetAsync(g->rows, 0, nrows * V * sizeof(int32_t), c->stream));
+    int launches = 0;
+    if (variant == DCX_VARIANT_PEEL) {
+        for (int32_t k = 0; k <= K; ++k) {
+            PassSpec p = row_spec(g, k, g->rows + (size_t)k * V, true);
+            ensure_scratch(c, (size_t)kScratchArrays * V * 4);
+            PassScratch s = scratch_slot(c->scratch, V, 0);
+            HIPX(c, launch_pass_grid(p, s, c->ctrs, g->lmax_rows + k, c->coop_grid, c->stream));
+            ++launches;
+        }
+    } else {
+        // batched: one persistent launch, one workgroup per pass
+        const size_t per_slot = (size_t)kScratchArrays * V * 4;
+        size_t budget = c->cfg.scratch_budget;
+        if (budget == 0) {
+            size_t fr = 0, tot = 0;
+            HIPX(c, hipMemGetInfo(&fr, &tot));
+            budget = std::min<size_t>(fr / 2, (size_t)64 << 30);
+        }
+        int slots = c->cfg.rows_slots > 0 ? c->cfg.rows_slots : c->num_cus * 2;
+        slots = (int)std::min<size_t>((size_t)slots, nrows);
+        slots = (int)std::min<size_t>((size_t)slots, std::max<size_t>(budget / per_slot, 1));
+        ensure_scratch(c, per_slot * slots);
+        HIPX(c, hipMemsetAsync(c->ctrs + 32, 0, sizeof(uint32_t), c->stream));
+        PassSpec base = row_spec(g, 0, g->rows, true);
+        HIPX(c, launch_rows_batched(base, c->scratch, slots, g->vk, K, g->rows, g->lmax_rows,
+                                    c->ctrs + 32, c->stream));
+        launches = 1;
+    }
+    std::vector<int32_t> lm(nrows);
+    HIPX(c, hipMemcpyAsync(lm.data(), g->lmax_rows, nrows * sizeof(int32_t),
+                           hipMemcpyDeviceToHost, c->stream));
+    HIPX(c, hipEventRecord(c->ev[4], c->stream));
+    sync(c);
+    c->timing.rows_ms = elapsed(c, 3, 4);
+    c->timing.rows_launches = launches;
+    g->lmax = 0;
+    for (int32_t x : lm) g->lmax = std::max(g->lmax, x);
+    g->rows_valid = true;
+}
+
+__global__ void k_ragged_gather(const int32_t* rows, const int32_t* kmaxes, const int64_t* row_ptr,
+                                uint32_t V, int32_t* values) {
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x) {
+        const int64_t b = row_ptr[v];
+        const int32_t km = kmaxes[v];
+        for (int32_t k = 0; k <= km; ++k) values[b + k] = rows[(size_t)k * V + v];
+    }
+}
+
+dcx_status check_graph(dcx_ctx* ctx, const dcx_graph* g) {
+    if (!ctx || !g) return DCX_ERR_INVALID_ARG;
+    if (g->ctx != ctx) {
+        set_err(ctx, "graph belongs to another context");
+        return DCX_ERR_INVALID_ARG;
+    }
+    return DCX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* dcx_status_str(dcx_status s) {
+    switch (s) {
+        case DCX_OK: return "DCX_OK";
+        case DCX_ERR_INVALID_ARG: return "DCX_ERR_INVALID_ARG";
+        case DCX_ERR_NO_DEVICE: return "DCX_ERR_NO_DEVICE";
+        case DCX_ERR_HIP: return "DCX_ERR_HIP";
+        case DCX_ERR_OUT_OF_MEMORY: return "DCX_ERR_OUT_OF_MEMORY";
+        case DCX_ERR_CAPACITY: return "DCX_ERR_CAPACITY";
+        case DCX_ERR_IO: return "DCX_ERR_IO";
+        case DCX_ERR_FORMAT: return "DCX_ERR_FORMAT";
+        case DCX_ERR_NOT_FOUND: return "DCX_ERR_NOT_FOUND";
+        case DCX_ERR_STATE: return "DCX_ERR_STATE";
+        case DCX_ERR_BUDGET: return "DCX_ERR_BUDGET";
+        case DCX_ERR_TIMEOUT: return "DCX_ERR_TIMEOUT";
+        case DCX_ERR_INTERNAL: return "DCX_ERR_INTERNAL";
+    }
+    return "DCX_ERR_UNKNOWN";
+}
+
+dcx_status dcx_config_init(dcx_config* cfg) {
+    if (!cfg) return DCX_ERR_INVALID_ARG;
+    std::memset(cfg, 0, sizeof *cfg);
+    cfg->variant = DCX_VARIANT_AUTO;
+    return DCX_OK;
+}
+
+dcx_status dcx_device_count(int32_t* n) {
+    if (!n) return DCX_ERR_INVALID_ARG;
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+    *n = c;
+    return DCX_OK;
+}
+
+dcx_status dcx_ctx_create(const dcx_config* cfg, dcx_ctx** out) {
+    if (!out) return DCX_ERR_INVALID_ARG;
+    *out = nullptr;
+    dcx_config def;
+    dcx_config_init(&def);
+    if (!cfg) cfg = &def;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return DCX_ERR_NO_DEVICE;
+    if (cfg->device < 0 || cfg->device >= n) return DCX_ERR_NO_DEVICE;
+    if (cfg->variant < DCX_VARIANT_AUTO || cfg->variant > DCX_VARIANT_HINDEX)
+        return DCX_ERR_INVALID_ARG;
+    std::unique_ptr<dcx_ctx> c(new (std::nothrow) dcx_ctx);
+    if (!c) return DCX_ERR_OUT_OF_MEMORY;
+    c->cfg = *cfg;
+    c->device = cfg->device;
+    if (hipSetDevice(c->device) != hipSuccess) return DCX_ERR_NO_DEVICE;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, c->device) != hipSuccess) return DCX_ERR_HIP;
+    if (!prop.cooperativeLaunch) return DCX_ERR_NO_DEVICE;
+    c->num_cus = prop.multiProcessorCount;
+    c->coop_grid = c->num_cus * pass_grid_blocks_per_cu();
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+        return DCX_ERR_HIP;
+    for (auto& e : c->ev)
+        if (hipEventCreate(&e) != hipSuccess) return DCX_ERR_HIP;
+    if (hipMalloc(&c->ctrs, 64 * sizeof(uint32_t)) != hipSuccess) return DCX_ERR_OUT_OF_MEMORY;
+    if (hipMalloc(&c->d_scalars, 64 * sizeof(int32_t)) != hipSuccess)
+        return DCX_ERR_OUT_OF_MEMORY;
+    if (hipHostMalloc(&c->h_scalars, 64 * sizeof(int32_t), hipHostMallocDefault) != hipSuccess)
+        return DCX_ERR_OUT_OF_MEMORY;
+    *out = c.release();
+    return DCX_OK;
+}
+
+dcx_status dcx_ctx_destroy(dcx_ctx* c) {
+    if (!c) return DCX_ERR_INVALID_ARG;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    dev_free(c->scratch);
+    dev_free(c->ctrs);
+    dev_free(c->d_scalars);
+    if (c->h_scalars) (void)hipHostFree(c->h_scalars);
+    for (auto& e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+    return DCX_OK;
+}
+
+const char* dcx_ctx_last_error(const dcx_ctx* c) { return c ? c->err.c_str() : ""; }
+
+dcx_status dcx_last_timing(const dcx_ctx* c, dcx_timing* t) {
+    if (!c || !t) return DCX_ERR_INVALID_ARG;
+    *t = c->timing;
+    return DCX_OK;
+}
+
+dcx_status dcx_graph_upload_csr(dcx_ctx* ctx, uint32_t V, uint64_t E, const uint32_t* out_off,
+                                const int32_t* out_deg, const uint32_t* out_nbr,
+                                const uint32_t* in_off, const int32_t* in_deg,
+                                const uint32_t* in_nbr, dcx_graph** out) {
+    if (!ctx || !out || !out_off || !in_off || (V && (!out_deg || !in_deg)))
+        return DCX_ERR_INVALID_ARG;
+    return guarded(ctx, [&]() -> dcx_status {
+        *out = nullptr;
+        // validate the layout on the host: monotone offsets, degrees within
+        // capacity, ids in range, sum of degrees equal on both sides
+        uint64_t so = 0, si = 0;
+        if (out_off[0] != 0 || in_off[0] != 0) {
+            set_err(ctx, "offsets must start at 0");
+            return DCX_ERR_INVALID_ARG;
+        }
+        for (uint32_t v = 0; v < V; ++v) {
+            if (out_off[v + 1] < out_off[v] || in_off[v + 1] < in_off[v] || out_deg[v] < 0 ||
+                in_deg[v] < 0 || (uint32_t)out_deg[v] > out_off[v + 1] - out_off[v] ||
+                (uint32_t)in_deg[v] > in_off[v + 1] - in_off[v]) {
+                set_err(ctx, "bad CSR entry at vertex %u", v);
+                return DCX_ERR_INVALID_ARG;
+            }
+            so += (uint32_t)out_deg[v];
+            si += (uint32_t)in_deg[v];
+            for (int32_t j = 0; j < out_deg[v]; ++j)
+                if (out_nbr[out_off[v] + j] >= V) {
+                    set_err(ctx, "out-neighbour id out of range at vertex %u", v);
+                    return DCX_ERR_INVALID_ARG;
+                }
+            for (int32_t j = 0; j < in_deg[v]; ++j)
+                if (in_nbr[in_off[v] + j] >= V) {
+                    set_err(ctx, "in-neighbour id out of range at vertex %u", v);
+                    return DCX_ERR_INVALID_ARG;
+                }
+        }
+        if (so != E || si != E) {
+            set_err(ctx, "degree sums (%llu out, %llu in) differ from E=%llu",
+                    (unsigned long long)so, (unsigned long long)si, (unsigned long long)E);
+            return DCX_ERR_INVALID_ARG;
+        }
+        std::unique_ptr<dcx_graph> g(new dcx_graph);
+        g->ctx = ctx;
+        g->V = V;
+        g->E = E;
+        g->out_cap = out_off[V];
+        g->in_cap = in_off[V];
+        try {
+            dev_alloc(ctx, &g->out_off, (size_t)V + 1);
+            dev_alloc(ctx, &g->in_off, (size_t)V + 1);
+            dev_alloc(ctx, &g->out_deg, V);
+            dev_alloc(ctx, &g->in_deg, V);
+            dev_alloc(ctx, &g->out_nbr, g->out_cap);
+            dev_alloc(ctx, &g->in_nbr, g->in_cap);
+            hipStream_t st = ctx->stream;
+            HIPX(ctx, hipMemcpyAsync(g->out_off, out_off, 4ull * (V + 1), hipMemcpyHostToDevice, st));
+            HIPX(ctx, hipMemcpyAsync(g->in_off, in_off, 4ull * (V + 1), hipMemcpyHostToDevice, st));
+            if (V) {
+                HIPX(ctx, hipMemcpyAsync(g->out_deg, out_deg, 4ull * V, hipMemcpyHostToDevice, st));
+                HIPX(ctx, hipMemcpyAsync(g->in_deg, in_deg, 4ull * V, hipMemcpyHostToDevice, st));
+            }
+            if (g->out_cap)
+                HIPX(ctx, hipMemcpyAsync(g->out_nbr, out_nbr, 4ull * g->out_cap,
+                                         hipMemcpyHostToDevice, st));
+            if (g->in_cap)
+                HIPX(ctx, hipMemcpyAsync(g->in_nbr, in_nbr, 4ull * g->in_cap,
+                                         hipMemcpyHostToDevice, st));
+            sync(ctx);
+        } catch (...) {
+            dcx_graph* raw = g.release();
+            dev_free(raw->out_off);
+            dev_free(raw->in_off);
+            dev_free(raw->out_deg);
+            dev_free(raw->in_deg);
+            dev_free(raw->out_nbr);
+            dev_free(raw->in_nbr);
+            delete raw;
+            throw;
+        }
+        *out = g.release();
+        return DCX_OK;
+    });
+}
+
+dcx_status dcx_graph_upload_edges(dcx_ctx* ctx, uint32_t V, uint64_t E, const uint32_t* src,
+                                  const uint32_t* dst, uint32_t slack, dcx_graph** out) {
+    if (!ctx || !out || (E && (!src || !dst))) return DCX_ERR_INVALID_ARG;
+    std::vector<uint32_t> oo(V + 1), io(V + 1);
+    std::vector<int32_t> od(V), id(V);
+    uint64_t cap = E + (uint64_t)slack * V;
+    std::vector<uint32_t> on(cap ? cap : 1), in(cap ? cap : 1);
+    dcx_status st = dcx_csr_build(V, E, src, dst, slack, oo.data(), od.data(), on.data(),
+                                  io.data(), id.data(), in.data());
+    if (st != DCX_OK) return st;
+    return dcx_graph_upload_csr(ctx, V, E, oo.data(), od.data(), on.data(), io.data(), id.data(),
+                                in.data(), out);
+}
+
+dcx_status dcx_graph_free(dcx_graph* g) {
+    if (!g) return DCX_ERR_INVALID_ARG;
+    dcx_ctx* c = g->ctx;
+    std::lock_guard<std::mutex> lk(c->mu);
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    dev_free(g->out_off);
+    dev_free(g->in_off);
+    dev_free(g->out_deg);
+    dev_free(g->in_deg);
+    dev_free(g->out_nbr);
+    dev_free(g->in_nbr);
+    dev_free(g->kmaxes);
+    dev_free(g->order);
+    dev_free(g->vk);
+    dev_free(g->hist);
+    dev_free(g->rows);
+    dev_free(g->lmax_rows);
+    delete g;
+    return DCX_OK;
+}
+
+dcx_status dcx_graph_info(const dcx_graph* g, uint32_t* V, uint64_t* E, int32_t* kmax,
+                          int32_t* lmax) {
+    if (!g) return DCX_ERR_INVALID_ARG;
+    if (V) *V = g->V;
+    if (E) *E = g->E;
+    if (kmax) *kmax = g->kmax_valid ? g->kmax : -1;
+    if (lmax) *lmax = g->rows_valid ? g->lmax : -1;
+    return DCX_OK;
+}
+
+dcx_status dcx_graph_capacity(const dcx_graph* g, uint64_t* out_cap, uint64_t* in_cap) {
+    if (!g) return DCX_ERR_INVALID_ARG;
+    if (out_cap) *out_cap = g->out_cap;
+    if (in_cap) *in_cap = g->in_cap;
+    return DCX_OK;
+}
+
+dcx_status dcx_graph_download_csr(dcx_ctx* ctx, const dcx_graph* g, uint32_t* out_off,
+                                  int32_t* out_deg, uint32_t* out_nbr, uint32_t* in_off,
+                                  int32_t* in_deg, uint32_t* in_nbr) {
+    dcx_status s = check_graph(ctx, g);
+    if (s != DCX_OK) return s;
+    if (!out_off || !out_deg || !out_nbr || !in_off || !in_deg || !in_nbr)
+        return DCX_ERR_INVALID_ARG;
+    return guarded(ctx, [&]() -> dcx_status {
+        hipStream_t st = ctx->stream;
+        const uint32_t V = g->V;
+        HIPX(ctx, hipMemcpyAsync(out_off, g->out_off, 4ull * (V + 1), hipMemcpyDeviceToHost, st));
+        HIPX(ctx, hipMemcpyAsync(in_off, g->in_off, 4ull * (V + 1), hipMemcpyDeviceToHost, st));
+        if (V) {
+            HIPX(ctx, hipMemcpyAsync(out_deg, g->out_deg, 4ull * V, hipMemcpyDeviceToHost, st));
+            HIPX(ctx, hipMemcpyAsync(in_deg, g->in_deg, 4ull * V, hipMemcpyDeviceToHost, st));
+        }
+        if (g->out_cap)
+            HIPX(ctx, hipMemcpyAsync(out_nbr, g->out_nbr, 4ull * g->out_cap, hipMemcpyDeviceToHost, st));
+        if (g->in_cap)
+            HIPX(ctx, hipMemcpyAsync(in_nbr, g->in_nbr, 4ull * g->in_cap, hipMemcpyDeviceToHost, st));
+        sync(ctx);
+        return DCX_OK;
+    });
+}
+
+dcx_status dcx_kmax(dcx_ctx* ctx, dcx_graph* g, int32_t* kmaxes, int32_t* kmax) {
+    dcx_status s = check_graph(ctx, g);
+    if (s != DCX_OK) return s;
+    return guarded(ctx, [&]() -> dcx_status {
+        compute_kmax(ctx, g);
+        g->rows_valid = false;
+        if (kmaxes && g->V)
+            HIPX(ctx, hipMemcpy(kmaxes, g->kmaxes, 4ull * g->V, hipMemcpyDeviceToHost));
+        if (kmax) *kmax = g->kmax;
+        return DCX_OK;
+    });
+}
+
+dcx_status dcx_klist(dcx_ctx* ctx, dcx_graph* g, int32_t k, int32_t variant, int32_t* lvals,
+                     int32_t* lmax_k) {
+    dcx_status s = check_graph(ctx, g);
+    if (s != DCX_OK) return s;
+    if (k < 0) return DCX_ERR_INVALID_ARG;
+    if (variant != DCX_VARIANT_AUTO && variant != DCX_VARIANT_PEEL) {
+        set_err(ctx, "dcx_klist: variant %d not available for a single row", variant);
+        return DCX_ERR_INVALID_ARG;
+    }
+    return guarded(ctx, [&]() -> dcx_status {
+        const uint32_t V = g->V;
+        if (V == 0) {
+            if (lmax_k) *lmax_k = -1;
+            return DCX_OK;
+        }
+        int32_t* row = nullptr;
+        dev_alloc(ctx, &row, V);
+        int32_t lk = 0;
+        try {
+            // with resident kmaxes the pass is restricted to the (k,0)-core;
+            // otherwise it runs on the whole graph exactly like KList
+            const bool filtered = g->kmax_valid;
+            if (filtered) HIPX(ctx, hipMemsetAsync(row, 0, 4ull * V, ctx->stream));
+            PassSpec p = row_spec(g, k, row, filtered);
+            if (filtered && p.nmembers == 0) {
+                lk = 0;
+            } else {
+                lk = run_grid_pass(ctx, p);
+            }
+            if (lvals) HIPX(ctx, hipMemcpy(lvals, row, 4ull * V, hipMemcpyDeviceToHost));
+        } catch (...) {
+            dev_free(row);
+            throw;
+        }
+        dev_free(row);
+        if (lmax_k) *lmax_k = lk;
+        return DCX_OK;
+    });
+}
+
+dcx_status dcx_decompose(dcx_ctx* ctx, dcx_graph* g, int32_t variant, int32_t* kmax,
+                         int32_t* lmax) {
+    dcx_status s = check_graph(ctx, g);
+    if (s != DCX_OK) return s;
+    if (variant == DCX_VARIANT_AUTO) variant = ctx->cfg.variant;
+    if (variant == DCX_VARIANT_AUTO) variant = DCX_VARIANT_BATCHED;
+    if (variant != DCX_VARIANT_PEEL && variant != DCX_VARIANT_BATCHED) {
+        set_err(ctx, "dcx_decompose: unknown variant %d", variant);
+        return DCX_ERR_INVALID_ARG;
+    }
+    return guarded(ctx, [&]() -> dcx_status {
+        ctx->timing = dcx_timing{};
+        compute_kmax(ctx, g);
+        compute_rows(ctx, g, variant);
+        ctx->timing.total_ms = ctx->timing.kmax_ms + ctx->timing.order_ms + ctx->timing.rows_ms;
+        if (kmax) *kmax = g->kmax;
+        if (lmax) *lmax = g->lmax;
+        return DCX_OK;
+    });
+}
+
+dcx_status dcx_get_kmaxes(dcx_ctx* ctx, const dcx_graph* g, int32_t* kmaxes) {
+    dcx_status s = check_graph(ctx, g);
+    if (s != DCX_OK) return s;
+    if (!kmaxes && g->V) return DCX_ERR_INVALID_ARG;
+    if (!g->kmax_valid) return DCX_ERR_STATE;
+    return guarded(ctx, [&]() -> dcx_status {
+        if (g->V) HIPX(ctx, hipMemcpy(kmaxes, g->kmaxes, 4ull * g->V, hipMemcpyDeviceToHost));
+        return DCX_OK;
+    });
+}
+
+dcx_status dcx_get_rows(dcx_ctx* ctx, const dcx_graph* g, int32_t k0, int32_t nrows,
+                        int32_t* rows) {
+    dcx_status s = check_graph(ctx, g);
+    if (s != DCX_OK) return s;
+    if (!g->rows_valid) return DCX_ERR_STATE;
+    if (k0 < 0 || nrows < 0 || (int64_t)k0 + nrows > (int64_t)g->kmax + 1 || (nrows && !rows))
+        return DCX_ERR_INVALID_ARG;
+    return guarded(ctx, [&]() -> dcx_status {
+        size_t n = (size_t)nrows * g->V;
+        if (n)
+            HIPX(ctx, hipMemcpy(rows, g->rows + (size_t)k0 * g->V, 4 * n, hipMemcpyDeviceToHost));
+        return DCX_OK;
+    });
+}
+
+dcx_status dcx_ragged_size(dcx_ctx* ctx, const dcx_graph* g, uint64_t* n) {
+    dcx_status s = check_graph(ctx, g);
+    if (s != DCX_OK) return s;
+    if (!n) return DCX_ERR_INVALID_ARG;
+    if (!g->kmax_valid) return DCX_ERR_STATE;
+    uint64_t t = 0;
+    for (int32_t k = 0; k <= g->kmax; ++k) t += g->h_vk[k];
+    *n = t;
+    return DCX_OK;
+}
+
+dcx_status dcx_get_ragged(dcx_ctx* ctx, const dcx_graph* g, int64_t* row_ptr, int32_t* values) {
+    dcx_status s = check_graph(ctx, g);
+    if (s != DCX_OK) return s;
+    if (!row_ptr || (g->V && !values)) return DCX_ERR_INVALID_ARG;
+    if (!g->rows_valid) return DCX_ERR_STATE;
+    return guarded(ctx, [&]() -> dcx_status {
+        const uint32_t V = g->V;
+        std::vector<int32_t> km(V);
+        if (V) HIPX(ctx, hipMemcpy(km.data(), g->kmaxes, 4ull * V, hipMemcpyDeviceToHost));
+        row_ptr[0] = 0;
+        for (uint32_t v = 0; v < V; ++v) row_ptr[v + 1] = row_ptr[v] + km[v] + 1;
+        const uint64_t n = (uint64_t)row_ptr[V];
+        if (n == 0) return DCX_OK;
+        int64_t* d_rp = nullptr;
+        int32_t* d_val = nullptr;
+        dev_alloc(ctx, &d_rp, (size_t)V + 1);
+        try {
+            dev_alloc(ctx, &d_val, n);
+            HIPX(ctx, hipMemcpyAsync(d_rp, row_ptr, 8ull * (V + 1), hipMemcpyHostToDevice,
+                                     ctx->stream));
+            hipLaunchKernelGGL(k_ragged_gather, dim3(std::min<uint32_t>((V + 255) / 256, 4096)),
+                               dim3(256), 0, ctx->stream, g->rows, g->kmaxes, d_rp, V, d_val);
+            HIPX(ctx, hipGetLastError());
+            HIPX(ctx, hipMemcpyAsync(values, d_val, 4ull * n, hipMemcpyDeviceToHost, ctx->stream));
+            sync(ctx);
+        } catch (...) {
+            dev_free(d_rp);
+            dev_free(d_val);
+            throw;
+        }
+        dev_free(d_rp);
+        dev_free(d_val);
+        return DCX_OK;
+    });
+}
+
+}  // extern "C"

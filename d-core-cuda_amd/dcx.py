@@ -1,0 +1,356 @@
+"""ctypes binding of libdcx.so (the C-ABI in include/dcx.h).
+
+This is the host-side mirror used by the tests and bench.py.  It adds no
+compute of its own: every result comes from the HIP library.  Loading fails
+loudly when libdcx.so is missing -- there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdcx.so")
+
+VARIANT_AUTO, VARIANT_PEEL, VARIANT_BATCHED, VARIANT_HINDEX = 0, 1, 2, 3
+STATUS = {
+    0: "DCX_OK", 1: "DCX_ERR_INVALID_ARG", 2: "DCX_ERR_NO_DEVICE", 3: "DCX_ERR_HIP",
+    4: "DCX_ERR_OUT_OF_MEMORY", 5: "DCX_ERR_CAPACITY", 6: "DCX_ERR_IO",
+    7: "DCX_ERR_FORMAT", 8: "DCX_ERR_NOT_FOUND", 9: "DCX_ERR_STATE",
+    10: "DCX_ERR_BUDGET", 11: "DCX_ERR_TIMEOUT", 12: "DCX_ERR_INTERNAL",
+}
+
+
+class DcxError(RuntimeError):
+    def __init__(self, status, where, detail=""):
+        self.status = status
+        super().__init__(f"{where}: {STATUS.get(status, status)} {detail}".strip())
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int32), ("variant", ctypes.c_int32),
+                ("rows_slots", ctypes.c_int32), ("reserved0", ctypes.c_int32),
+                ("scratch_budget", ctypes.c_uint64), ("reserved", ctypes.c_uint64 * 4)]
+
+
+class Timing(ctypes.Structure):
+    _fields_ = [("kmax_ms", ctypes.c_double), ("order_ms", ctypes.c_double),
+                ("rows_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
+                ("rows_launches", ctypes.c_int32), ("reserved", ctypes.c_int32 * 7)]
+
+
+_lib = None
+
+
+def lib():
+    """Loads libdcx.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it first "
+                               "(python -c 'import __graft_entry__ as g; g.build()')")
+        _lib = ctypes.CDLL(LIB_PATH)
+        _lib.dcx_status_str.restype = ctypes.c_char_p
+        _lib.dcx_ctx_last_error.restype = ctypes.c_char_p
+        _lib.dcx_ctx_last_error.argtypes = [ctypes.c_void_p]
+    return _lib
+
+
+def P(a):
+    """Pointer to a numpy array's data (None passes NULL)."""
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def check(status, where, ctx=None):
+    if status != 0:
+        detail = ""
+        if ctx is not None:
+            detail = lib().dcx_ctx_last_error(ctx).decode(errors="replace")
+        raise DcxError(status, where, detail)
+
+
+u32, u64, i32, f64 = ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int32, ctypes.c_double
+
+
+# ---- host-side graph building ------------------------------------------
+
+def gen_er(V, E, seed):
+    src = np.empty(E, np.uint32)
+    dst = np.empty(E, np.uint32)
+    check(lib().dcx_gen_er(u32(V), u64(E), u64(seed), P(src), P(dst)), "dcx_gen_er")
+    return src, dst
+
+
+def gen_rmat(scale, samples, seed, a=0.57, b=0.19, c=0.19):
+    src = np.empty(samples, np.uint32)
+    dst = np.empty(samples, np.uint32)
+    E = ctypes.c_uint64()
+    check(lib().dcx_gen_rmat(u32(scale), u64(samples), f64(a), f64(b), f64(c), u64(seed),
+                             P(src), P(dst), u64(samples), ctypes.byref(E)), "dcx_gen_rmat")
+    return src[:E.value].copy(), dst[:E.value].copy()
+
+
+def gen_chunglu(V, samples, seed, gamma=2.1):
+    src = np.empty(samples, np.uint32)
+    dst = np.empty(samples, np.uint32)
+    E = ctypes.c_uint64()
+    check(lib().dcx_gen_chunglu(u32(V), u64(samples), f64(gamma), u64(seed), P(src), P(dst),
+                                u64(samples), ctypes.byref(E)), "dcx_gen_chunglu")
+    return src[:E.value].copy(), dst[:E.value].copy()
+
+
+def csr_build(V, src, dst, slack=0):
+    src = np.ascontiguousarray(src, np.uint32)
+    dst = np.ascontiguousarray(dst, np.uint32)
+    E = src.size
+    cap = E + slack * V
+    oo = np.empty(V + 1, np.uint32)
+    io = np.empty(V + 1, np.uint32)
+    od = np.empty(V, np.int32)
+    idg = np.empty(V, np.int32)
+    on = np.empty(max(cap, 1), np.uint32)
+    inn = np.empty(max(cap, 1), np.uint32)
+    check(lib().dcx_csr_build(u32(V), u64(E), P(src), P(dst), u32(slack), P(oo), P(od), P(on),
+                              P(io), P(idg), P(inn)), "dcx_csr_build")
+    return dict(out_off=oo, out_deg=od, out_nbr=on[:cap], in_off=io, in_deg=idg,
+                in_nbr=inn[:cap])
+
+
+def text_read(path, prune=False):
+    Vh = ctypes.c_uint32()
+    n = ctypes.c_uint64()
+    check(lib().dcx_text_count(path.encode(), ctypes.byref(Vh), ctypes.byref(n)),
+          "dcx_text_count")
+    src = np.empty(max(n.value, 1), np.uint32)
+    dst = np.empty(max(n.value, 1), np.uint32)
+    V = ctypes.c_uint32()
+    E = ctypes.c_uint64()
+    check(lib().dcx_text_read(path.encode(), i32(1 if prune else 0), P(src), P(dst),
+                              u64(n.value), ctypes.byref(V), ctypes.byref(E)), "dcx_text_read")
+    return V.value, src[:E.value].copy(), dst[:E.value].copy()
+
+
+def text_write(path, V, src, dst):
+    src = np.ascontiguousarray(src, np.uint32)
+    dst = np.ascontiguousarray(dst, np.uint32)
+    check(lib().dcx_text_write(path.encode(), u32(V), u64(src.size), P(src), P(dst)),
+          "dcx_text_write")
+
+
+def binary_read(path, gap):
+    V = ctypes.c_uint32()
+    E = ctypes.c_uint32()
+    check(lib().dcx_binary_header(path.encode(), ctypes.byref(V), ctypes.byref(E)),
+          "dcx_binary_header")
+    V, E = V.value, E.value
+    cap = E + gap * V
+    g = dict(out_off=np.empty(V + 1, np.uint32), out_deg=np.empty(V, np.int32),
+             out_nbr=np.empty(max(cap, 1), np.uint32), in_off=np.empty(V + 1, np.uint32),
+             in_deg=np.empty(V, np.int32), in_nbr=np.empty(max(cap, 1), np.uint32))
+    check(lib().dcx_binary_read(path.encode(), u32(gap), P(g["out_off"]), P(g["out_deg"]),
+                                P(g["out_nbr"]), P(g["in_off"]), P(g["in_deg"]),
+                                P(g["in_nbr"])), "dcx_binary_read")
+    g["out_nbr"] = g["out_nbr"][:cap]
+    g["in_nbr"] = g["in_nbr"][:cap]
+    return V, E, g
+
+
+def binary_write(path, V, E, gap, g):
+    check(lib().dcx_binary_write(path.encode(), u32(V), u32(E), u32(gap), P(g["out_off"]),
+                                 P(g["out_deg"]), P(g["out_nbr"]), P(g["in_off"]),
+                                 P(g["in_deg"]), P(g["in_nbr"])), "dcx_binary_write")
+
+
+def decomp_write(path, V, kmax, lmax, kmaxes, rows):
+    kmaxes = np.ascontiguousarray(kmaxes, np.int32)
+    rows = np.ascontiguousarray(rows, np.int32)
+    check(lib().dcx_decomp_write(path.encode(), u32(V), i32(kmax), i32(lmax), P(kmaxes),
+                                 P(rows)), "dcx_decomp_write")
+
+
+def decomp_read(path, V):
+    K = ctypes.c_int32()
+    Lm = ctypes.c_int32()
+    check(lib().dcx_decomp_header(path.encode(), ctypes.byref(K), ctypes.byref(Lm)),
+          "dcx_decomp_header")
+    km = np.empty(V, np.int32)
+    rows = np.empty((max(K.value + 1, 0), V), np.int32)
+    check(lib().dcx_decomp_read(path.encode(), u32(V), P(km), P(rows)), "dcx_decomp_read")
+    return K.value, Lm.value, km, rows
+
+
+def decompcpu_write(path, V, row_ptr, values):
+    row_ptr = np.ascontiguousarray(row_ptr, np.int64)
+    values = np.ascontiguousarray(values, np.int32)
+    check(lib().dcx_decompcpu_write(path.encode(), u32(V), P(row_ptr), P(values)),
+          "dcx_decompcpu_write")
+
+
+# ---- device ---------------------------------------------------------------
+
+def device_count():
+    n = ctypes.c_int32()
+    check(lib().dcx_device_count(ctypes.byref(n)), "dcx_device_count")
+    return n.value
+
+
+class Context:
+    """A dcx_ctx on one device (library-owned stream and scratch)."""
+
+    def __init__(self, device=0, variant=VARIANT_AUTO, rows_slots=0, scratch_budget=0):
+        cfg = Config()
+        check(lib().dcx_config_init(ctypes.byref(cfg)), "dcx_config_init")
+        cfg.device = device
+        cfg.variant = variant
+        cfg.rows_slots = rows_slots
+        cfg.scratch_budget = scratch_budget
+        h = ctypes.c_void_p()
+        check(lib().dcx_ctx_create(ctypes.byref(cfg), ctypes.byref(h)), "dcx_ctx_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().dcx_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def timing(self):
+        t = Timing()
+        check(lib().dcx_last_timing(self.h, ctypes.byref(t)), "dcx_last_timing", self.h)
+        return t
+
+    def upload_edges(self, V, src, dst, slack=0):
+        return Graph(self, V, src, dst, slack)
+
+    def upload_csr(self, V, E, csr):
+        return Graph(self, V, None, None, 0, csr=(E, csr))
+
+
+class Graph:
+    """A device-resident graph (dcx_graph) with its resident decomposition."""
+
+    def __init__(self, ctx, V, src, dst, slack=0, csr=None):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        if csr is None:
+            src = np.ascontiguousarray(src, np.uint32)
+            dst = np.ascontiguousarray(dst, np.uint32)
+            check(lib().dcx_graph_upload_edges(ctx.h, u32(V), u64(src.size), P(src), P(dst),
+                                               u32(slack), ctypes.byref(h)),
+                  "dcx_graph_upload_edges", ctx.h)
+        else:
+            E, g = csr
+            check(lib().dcx_graph_upload_csr(ctx.h, u32(V), u64(E), P(g["out_off"]),
+                                             P(g["out_deg"]), P(g["out_nbr"]), P(g["in_off"]),
+                                             P(g["in_deg"]), P(g["in_nbr"]), ctypes.byref(h)),
+                  "dcx_graph_upload_csr", ctx.h)
+        self.h = h
+        self.V = V
+
+    def free(self):
+        if self.h:
+            lib().dcx_graph_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def info(self):
+        V = ctypes.c_uint32()
+        E = ctypes.c_uint64()
+        K = ctypes.c_int32()
+        Lm = ctypes.c_int32()
+        check(lib().dcx_graph_info(self.h, ctypes.byref(V), ctypes.byref(E), ctypes.byref(K),
+                                   ctypes.byref(Lm)), "dcx_graph_info")
+        return V.value, E.value, K.value, Lm.value
+
+    def kmax(self):
+        km = np.empty(self.V, np.int32)
+        K = ctypes.c_int32()
+        check(lib().dcx_kmax(self.ctx.h, self.h, P(km), ctypes.byref(K)), "dcx_kmax", self.ctx.h)
+        return K.value, km
+
+    def klist(self, k, variant=VARIANT_AUTO):
+        l = np.empty(self.V, np.int32)
+        lk = ctypes.c_int32()
+        check(lib().dcx_klist(self.ctx.h, self.h, i32(k), i32(variant), P(l), ctypes.byref(lk)),
+              "dcx_klist", self.ctx.h)
+        return lk.value, l
+
+    def decompose(self, variant=VARIANT_AUTO):
+        K = ctypes.c_int32()
+        Lm = ctypes.c_int32()
+        check(lib().dcx_decompose(self.ctx.h, self.h, i32(variant), ctypes.byref(K),
+                                  ctypes.byref(Lm)), "dcx_decompose", self.ctx.h)
+        return K.value, Lm.value
+
+    def kmaxes(self):
+        km = np.empty(self.V, np.int32)
+        check(lib().dcx_get_kmaxes(self.ctx.h, self.h, P(km)), "dcx_get_kmaxes", self.ctx.h)
+        return km
+
+    def rows(self, k0=0, nrows=None):
+        K = self.info()[2]
+        if nrows is None:
+            nrows = K + 1 - k0
+        out = np.empty((nrows, self.V), np.int32)
+        check(lib().dcx_get_rows(self.ctx.h, self.h, i32(k0), i32(nrows), P(out)),
+              "dcx_get_rows", self.ctx.h)
+        return out
+
+    def ragged(self):
+        n = ctypes.c_uint64()
+        check(lib().dcx_ragged_size(self.ctx.h, self.h, ctypes.byref(n)), "dcx_ragged_size",
+              self.ctx.h)
+        rp = np.empty(self.V + 1, np.int64)
+        vals = np.empty(max(n.value, 1), np.int32)
+        check(lib().dcx_get_ragged(self.ctx.h, self.h, P(rp), P(vals)), "dcx_get_ragged",
+              self.ctx.h)
+        return rp, vals[:n.value]
+
+    def download_csr(self):
+        oc = ctypes.c_uint64()
+        ic = ctypes.c_uint64()
+        check(lib().dcx_graph_capacity(self.h, ctypes.byref(oc), ctypes.byref(ic)),
+              "dcx_graph_capacity")
+        V = self.V
+        g = dict(out_off=np.empty(V + 1, np.uint32), out_deg=np.empty(V, np.int32),
+                 out_nbr=np.empty(max(oc.value, 1), np.uint32),
+                 in_off=np.empty(V + 1, np.uint32), in_deg=np.empty(V, np.int32),
+                 in_nbr=np.empty(max(ic.value, 1), np.uint32))
+        check(lib().dcx_graph_download_csr(self.ctx.h, self.h, P(g["out_off"]), P(g["out_deg"]),
+                                           P(g["out_nbr"]), P(g["in_off"]), P(g["in_deg"]),
+                                           P(g["in_nbr"])), "dcx_graph_download_csr",
+              self.ctx.h)
+        g["out_nbr"] = g["out_nbr"][:oc.value]
+        g["in_nbr"] = g["in_nbr"][:ic.value]
+        return g
+
+
+def compact_csr(g):
+    """Compact (no slack) out/in lists from a CSR-with-slack dict, as
+    (out_off, out_nbr, in_off, in_nbr) with uint64 offsets."""
+    res = []
+    for side in ("out", "in"):
+        off = g[side + "_off"].astype(np.int64)
+        deg = g[side + "_deg"].astype(np.int64)
+        V = deg.size
+        coff = np.zeros(V + 1, np.uint64)
+        np.cumsum(deg, out=coff[1:])
+        idx = np.repeat(off[:-1], deg) + (np.arange(int(coff[-1])) - np.repeat(coff[:-1].astype(np.int64), deg))
+        res += [coff, g[side + "_nbr"][idx].astype(np.uint32)]
+    return res

@@ -1,0 +1,206 @@
+/*
+ * dcx.h -- C-ABI of the MI355X-native D-core ((k,l)-core) library.
+ *
+ * The drop-in boundary for the reference's decomposition and maintenance
+ * path (llysdal/d-core-cuda).  Plain pointers and sizes only; every call
+ * returns a dcx_status.  Host buffers are owned by the caller; device memory
+ * and the HIP stream are owned by the library (a dcx_ctx per device).
+ *
+ * Types follow the reference (src/common.h:59-62): vertex ids and CSR
+ * offsets are uint32, degrees and (k,l) values are int32.
+ *
+ * Each declaration cites the reference interface it replaces.
+ */
+#ifndef DCX_H
+#define DCX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum dcx_status {
+    DCX_OK = 0,
+    DCX_ERR_INVALID_ARG = 1,  /* bad pointer, size or value */
+    DCX_ERR_NO_DEVICE = 2,    /* no HIP device / ordinal out of range */
+    DCX_ERR_HIP = 3,          /* a HIP runtime call failed */
+    DCX_ERR_OUT_OF_MEMORY = 4,/* device or host allocation failed */
+    DCX_ERR_CAPACITY = 5,     /* a bounded buffer or slack is exhausted */
+    DCX_ERR_IO = 6,           /* file could not be opened or written */
+    DCX_ERR_FORMAT = 7,       /* malformed file */
+    DCX_ERR_NOT_FOUND = 8,    /* e.g. delete of an absent edge */
+    DCX_ERR_STATE = 9,        /* call order violated (e.g. no decomposition yet) */
+    DCX_ERR_BUDGET = 10,      /* request does not fit the memory budget */
+    DCX_ERR_TIMEOUT = 11,     /* a bounded wait expired */
+    DCX_ERR_INTERNAL = 12
+} dcx_status;
+
+/* Human-readable status name.  Replaces nothing: the reference has no error
+ * reporting at all (no CUDA call is checked anywhere in main.cu). */
+const char* dcx_status_str(dcx_status s);
+
+/* ------------------------------------------------------------------ */
+/* Host-side graph building (no GPU needed).                          */
+/* ------------------------------------------------------------------ */
+
+/* Seeded directed G(n,m): E distinct pairs, no self-loops, uniform endpoints,
+ * sorted by (src,dst).  Replaces generateGraph (src/graph_gen.h:8,
+ * src/graph_gen.cpp:9-34), which seeds from time() and writes a file. */
+dcx_status dcx_gen_er(uint32_t V, uint64_t E, uint64_t seed, uint32_t* src, uint32_t* dst);
+
+/* Seeded R-MAT: V = 2^scale, `samples` draws with quadrant probabilities
+ * (a,b,c,1-a-b-c), self-loops and duplicates removed, sorted by (src,dst).
+ * *E_out receives the edge count; returns DCX_ERR_CAPACITY if it exceeds
+ * `cap` (pass cap = samples to be safe).  No reference counterpart
+ * (the reference only loads SNAP files). */
+dcx_status dcx_gen_rmat(uint32_t scale, uint64_t samples, double a, double b, double c,
+                        uint64_t seed, uint32_t* src, uint32_t* dst, uint64_t cap,
+                        uint64_t* E_out);
+
+/* Seeded Chung-Lu digraph with power-law out/in weights (exponent gamma),
+ * `samples` draws, deduplicated, sorted.  No reference counterpart. */
+dcx_status dcx_gen_chunglu(uint32_t V, uint64_t samples, double gamma, uint64_t seed,
+                           uint32_t* src, uint32_t* dst, uint64_t cap, uint64_t* E_out);
+
+/* Edge list -> in/out CSR with `slack` free slots per vertex:
+ * off[v+1]-off[v] = deg(v)+slack, neighbours in input order, free slots
+ * 0xFFFFFFFF.  Arrays: off[V+1], deg[V], nbr[E + slack*V].
+ * Replaces the CSR build of Graph::readFile (src/graph.cpp:180-243). */
+dcx_status dcx_csr_build(uint32_t V, uint64_t E, const uint32_t* src, const uint32_t* dst,
+                         uint32_t slack, uint32_t* out_off, int32_t* out_deg, uint32_t* out_nbr,
+                         uint32_t* in_off, int32_t* in_deg, uint32_t* in_nbr);
+
+/* Text edge list "# V" + "s t" lines: count, then read.  `prune` = 0 keeps
+ * every id (GraphCPU::readFile, src/graph_cpu.cpp:38-102); prune = 1 drops
+ * vertices without edges and renumbers (Graph::readFile + pruneEmptyVertices,
+ * src/graph.cpp:110-176, with V taken from the header instead of the
+ * hard-coded 7414865 at src/graph.cpp:158).  Self-loops are dropped. */
+dcx_status dcx_text_count(const char* path, uint32_t* V_header, uint64_t* lines);
+dcx_status dcx_text_read(const char* path, int32_t prune, uint32_t* src, uint32_t* dst,
+                         uint64_t cap, uint32_t* V_out, uint64_t* E_out);
+/* Writes the same format (src/graph_gen.cpp:27-29, Graph::writeFile src/graph.cpp:252-259). */
+dcx_status dcx_text_write(const char* path, uint32_t V, uint64_t E, const uint32_t* src,
+                          const uint32_t* dst);
+
+/* Binary CSR cache, byte-compatible with <file>-binary-g=N
+ * (Graph::writeBinary / readBinary, src/graph.cpp:261-325). */
+dcx_status dcx_binary_header(const char* path, uint32_t* V, uint32_t* E);
+dcx_status dcx_binary_read(const char* path, uint32_t gap, uint32_t* out_off, int32_t* out_deg,
+                           uint32_t* out_nbr, uint32_t* in_off, int32_t* in_deg,
+                           uint32_t* in_nbr);
+dcx_status dcx_binary_write(const char* path, uint32_t V, uint32_t E, uint32_t gap,
+                            const uint32_t* out_off, const int32_t* out_deg,
+                            const uint32_t* out_nbr, const uint32_t* in_off,
+                            const int32_t* in_deg, const uint32_t* in_nbr);
+
+/* .decomp result cache, byte-compatible (readDecompFile / writeDecompFile,
+ * src/utils.h:76-109): i32 kmax, i32 lmax, i32 kmaxes[V], (kmax+1) x i32[V]. */
+dcx_status dcx_decomp_write(const char* path, uint32_t V, int32_t kmax, int32_t lmax,
+                            const int32_t* kmaxes, const int32_t* rows);
+dcx_status dcx_decomp_header(const char* path, int32_t* kmax, int32_t* lmax);
+dcx_status dcx_decomp_read(const char* path, uint32_t V, int32_t* kmaxes, int32_t* rows);
+
+/* Ragged per-vertex text result (writeDecompFileCPU, src/utils.h:110-124). */
+dcx_status dcx_decompcpu_write(const char* path, uint32_t V, const int64_t* row_ptr,
+                               const int32_t* values);
+
+/* ------------------------------------------------------------------ */
+/* Device context and device-resident graphs.                         */
+/* ------------------------------------------------------------------ */
+
+typedef struct dcx_ctx dcx_ctx;
+typedef struct dcx_graph dcx_graph;
+
+/* Exact kernel variants; every variant gives bit-identical results. */
+typedef enum dcx_variant {
+    DCX_VARIANT_AUTO = 0,    /* library choice (batched rows) */
+    DCX_VARIANT_PEEL = 1,    /* plain: one k per launch, whole grid per pass */
+    DCX_VARIANT_BATCHED = 2, /* all k-passes in one persistent launch */
+    DCX_VARIANT_HINDEX = 3   /* h-index fixpoint per k */
+} dcx_variant;
+
+typedef struct dcx_config {
+    int32_t device;          /* HIP device ordinal */
+    int32_t variant;         /* default dcx_variant for dcx_decompose */
+    int32_t rows_slots;      /* concurrent passes in batched mode (0 = auto) */
+    int32_t reserved0;
+    uint64_t scratch_budget; /* bytes of device scratch for batched passes (0 = auto) */
+    uint64_t reserved[4];
+} dcx_config;
+
+/* Timings of the last decomposition on a context, from HIP events on the
+ * library's stream.  Mirrors the "kmax_ms lmax_ms total_ms" print of dcore
+ * (main.cu:256). */
+typedef struct dcx_timing {
+    double kmax_ms;   /* in-core (kmax) pass */
+    double order_ms;  /* kmax-descending vertex order */
+    double rows_ms;   /* all l-rows */
+    double total_ms;
+    int32_t rows_launches; /* kernel launches spent on the rows */
+    int32_t reserved[7];
+} dcx_timing;
+
+dcx_status dcx_config_init(dcx_config* cfg);
+dcx_status dcx_device_count(int32_t* n);
+/* Creates a context on cfg->device: one HIP stream (default priority),
+ * device scratch grown on demand.  Replaces the allocate*Memory family
+ * (src/cuda_memory.h:5-106). */
+dcx_status dcx_ctx_create(const dcx_config* cfg, dcx_ctx** out);
+dcx_status dcx_ctx_destroy(dcx_ctx* ctx);
+/* Detail of the last failing call on this context ("" if none). */
+const char* dcx_ctx_last_error(const dcx_ctx* ctx);
+dcx_status dcx_last_timing(const dcx_ctx* ctx, dcx_timing* t);
+
+/* Uploads a CSR-with-slack graph (layout of dcx_csr_build).  Replaces
+ * allocateDeviceGraphMemory + moveGraphToDevice (src/cuda_memory.h:5-41)
+ * and the GraphGPU constructor (src/graph_gpu.cu:25-47). */
+dcx_status dcx_graph_upload_csr(dcx_ctx* ctx, uint32_t V, uint64_t E, const uint32_t* out_off,
+                                const int32_t* out_deg, const uint32_t* out_nbr,
+                                const uint32_t* in_off, const int32_t* in_deg,
+                                const uint32_t* in_nbr, dcx_graph** out);
+/* Builds the CSR (dcx_csr_build) and uploads it. */
+dcx_status dcx_graph_upload_edges(dcx_ctx* ctx, uint32_t V, uint64_t E, const uint32_t* src,
+                                  const uint32_t* dst, uint32_t slack, dcx_graph** out);
+dcx_status dcx_graph_free(dcx_graph* g);
+/* V, current E, and kmax/lmax of the resident decomposition (-1 if none). */
+dcx_status dcx_graph_info(const dcx_graph* g, uint32_t* V, uint64_t* E, int32_t* kmax,
+                          int32_t* lmax);
+/* Capacity of each adjacency array (E + free slots). */
+dcx_status dcx_graph_capacity(const dcx_graph* g, uint64_t* out_cap, uint64_t* in_cap);
+/* Copies the device CSR back (arrays sized V+1, V, capacity). */
+dcx_status dcx_graph_download_csr(dcx_ctx* ctx, const dcx_graph* g, uint32_t* out_off,
+                                  int32_t* out_deg, uint32_t* out_nbr, uint32_t* in_off,
+                                  int32_t* in_deg, uint32_t* in_nbr);
+
+/* In-core numbers (the kmax pass: KListInplace on the in/out-swapped graph
+ * with k = 0, dcore main.cu:218-220; checkKmax main.cu:264-274).  kmaxes
+ * (host, V entries) may be NULL; the values stay resident on the graph. */
+dcx_status dcx_kmax(dcx_ctx* ctx, dcx_graph* g, int32_t* kmaxes, int32_t* kmax);
+
+/* One l-row for a given k (KList / KListInplace, main.cu:144-195): lvals[v]
+ * = max{l : v in D_{k,l}}, 0 outside the (k,0)-core; *lmax_k = last level.
+ * variant: DCX_VARIANT_PEEL/AUTO (grid peel) or DCX_VARIANT_HINDEX. */
+dcx_status dcx_klist(dcx_ctx* ctx, dcx_graph* g, int32_t k, int32_t variant, int32_t* lvals,
+                     int32_t* lmax_k);
+
+/* Full decomposition (dcore, main.cu:197-262; checkDCore main.cu:276-293):
+ * kmaxes and rows k = 0..kmax stay resident on the device graph. */
+dcx_status dcx_decompose(dcx_ctx* ctx, dcx_graph* g, int32_t variant, int32_t* kmax,
+                         int32_t* lmax);
+
+/* Result download: kmaxes[V]; dense rows k0..k0+nrows-1 (row-major, V
+ * each, the .decomp row layout); ragged per-vertex layout (row_ptr[V+1],
+ * values = lmaxes[0..kmaxes[v]][v], the .decompCPU order). */
+dcx_status dcx_get_kmaxes(dcx_ctx* ctx, const dcx_graph* g, int32_t* kmaxes);
+dcx_status dcx_get_rows(dcx_ctx* ctx, const dcx_graph* g, int32_t k0, int32_t nrows,
+                        int32_t* rows);
+/* Number of ragged values (sum over v of kmaxes[v]+1). */
+dcx_status dcx_ragged_size(dcx_ctx* ctx, const dcx_graph* g, uint64_t* n);
+dcx_status dcx_get_ragged(dcx_ctx* ctx, const dcx_graph* g, int64_t* row_ptr, int32_t* values);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DCX_H */

@@ -1,0 +1,310 @@
+/*
+ * dcore_oracle.c -- CPU oracle for D-core ((k,l)-core) decomposition.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP product path.  Only the
+ * tests, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.
+ *
+ * Semantics (restated from the reference, llysdal/d-core-cuda):
+ *   - One peel pass for a fixed k (scan main.cu:27-62, process main.cu:72-141,
+ *     level loop KListInplace main.cu:171-195): start at level L = 0 with all
+ *     vertices alive.  At level L repeatedly remove every alive v with
+ *     out-degree(alive) <= L or in-degree(alive) < k, giving it l[v] = L, until
+ *     none is left; then L := L + 1; stop when nothing is alive.  The return
+ *     value is the last level (KListInplace returns level - 1).
+ *     The reference tests out_deg == L (main.cu:44); that is the same test
+ *     because every alive vertex has out-degree >= L when level L starts.
+ *   - kmax pass: the same peel with in and out swapped and k = 0
+ *     (swapInOut, src/common.h:114-120; dcore main.cu:218-220).
+ *   - Rows k = 0..kmax (dcore main.cu:238-244), lmax = max over rows.
+ *
+ * Implementation: per pass a bucket queue keyed by the current out-degree
+ * with lazy entries (a vertex is re-filed into a lower bucket each time its
+ * key drops above the current level), so one pass costs O(V + E + maxdeg)
+ * instead of the reference's O(levels * V) scans.
+ *
+ * Pinning (each is a test under tests/):
+ *   dco_klist / dco_kmax / dco_lrows
+ *     - the reference's golden vector dataset/digraph(.decomp)
+ *       (tests/test_oracle.py::test_fixture_decomp);
+ *     - brute force from the definition (iterative removal for every (k,l))
+ *       on random and exhaustive tiny digraphs;
+ *     - textbook reductions: kmaxes = in-degree core numbers and
+ *       lmaxes[0] = out-degree core numbers by an independent
+ *       Batagelj-Zaversnik implementation, and on symmetric digraphs both
+ *       equal networkx.core_number of the undirected graph;
+ *     - closed forms (complete digraph, directed cycle, DAGs, stars,
+ *       bidirected K_{a,b}, circulants);
+ *     - reverse duality lmaxes_G[k][v] = max{l : v in D_{l,k}(G^R)};
+ *     - invariants (monotone in k, zero above kmaxes[v], degree bounds,
+ *       core property and maximality of every level set).
+ *   dco_klist_hindex / dco_kmax_hindex
+ *     - equality with the peel on every test graph (two different
+ *       algorithms, fixpoint argument in the A.3 design, main.cu:982-1242).
+ */
+#include "dcore_oracle.h"
+
+#include <stdlib.h>
+#include <string.h>
+
+/* DCO_PLANT (tests only): compiles a deliberately wrong variant so the
+ * tests can show that their checks catch plausible mistakes.
+ *   1: constraint threshold off by one;  2: key decrements sent to the
+ *   wrong side;  3: levels assigned one too high.  Unset in every build
+ *   that is used as the checker. */
+#ifndef DCO_PLANT
+#define DCO_PLANT 0
+#endif
+#if DCO_PLANT == 1
+#define CFAIL(x, k) ((x) <= (k))
+#else
+#define CFAIL(x, k) ((x) < (k))
+#endif
+#if DCO_PLANT == 3
+#define LEVEL(L) ((L) + 1)
+#else
+#define LEVEL(L) (L)
+#endif
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* Generic peel.  Key lists K(v) decide the level (out-lists for an l-pass);
+ * constraint lists C(v) decide membership with threshold k (in-lists for an
+ * l-pass).  Removing v lowers the key degree of every w in C(v) (v was in
+ * K(w)) and the constraint degree of every u in K(v). */
+static int32_t peel(uint32_t V, const uint64_t* koff, const uint32_t* knbr,
+                    const uint64_t* coff, const uint32_t* cnbr, int32_t k,
+                    int32_t* l) {
+    if (V == 0) return -1;
+    int32_t* kd = (int32_t*)malloc(sizeof(int32_t) * V);
+    int32_t* cd = (int32_t*)malloc(sizeof(int32_t) * V);
+    uint8_t* state = (uint8_t*)calloc(V, 1); /* 0 alive, 1 removed */
+    uint32_t* stack = (uint32_t*)malloc(sizeof(uint32_t) * V);
+    int32_t maxkd = 0;
+    for (uint32_t v = 0; v < V; ++v) {
+        kd[v] = (int32_t)(koff[v + 1] - koff[v]);
+        cd[v] = (int32_t)(coff[v + 1] - coff[v]);
+        if (kd[v] > maxkd) maxkd = kd[v];
+    }
+    /* lazy buckets: at most V initial entries plus one per key decrement */
+    uint64_t cap = (uint64_t)V + coff[V];
+    uint32_t* pv = (uint32_t*)malloc(sizeof(uint32_t) * cap);
+    uint64_t* pnext = (uint64_t*)malloc(sizeof(uint64_t) * cap);
+    uint64_t* head = (uint64_t*)malloc(sizeof(uint64_t) * ((size_t)maxkd + 1));
+    const uint64_t NIL = ~(uint64_t)0;
+    for (int32_t d = 0; d <= maxkd; ++d) head[d] = NIL;
+    uint64_t np = 0;
+    for (uint32_t v = 0; v < V; ++v) {
+        pv[np] = v;
+        pnext[np] = head[kd[v]];
+        head[kd[v]] = np++;
+    }
+    uint32_t removed = 0, sp = 0;
+    int32_t L = 0, last = -1;
+    /* level 0 also takes every vertex failing the constraint */
+    for (uint32_t v = 0; v < V; ++v)
+        if (CFAIL(cd[v], k)) {
+            state[v] = 1;
+            l[v] = LEVEL(0);
+            stack[sp++] = v;
+        }
+    for (;; ++L) {
+        if (L <= maxkd) {
+            for (uint64_t e = head[L]; e != NIL; e = pnext[e]) {
+                uint32_t v = pv[e];
+                if (state[v] == 0 && kd[v] <= L) {
+                    state[v] = 1;
+                    l[v] = LEVEL(L);
+                    stack[sp++] = v;
+                }
+            }
+        }
+        while (sp > 0) {
+            uint32_t v = stack[--sp];
+            ++removed;
+            last = L;
+#if DCO_PLANT == 2
+            for (uint64_t o = koff[v]; o < koff[v + 1]; ++o) {
+                uint32_t w = knbr[o];
+#else
+            for (uint64_t o = coff[v]; o < coff[v + 1]; ++o) {
+                uint32_t w = cnbr[o];
+#endif
+                if (state[w]) continue;
+                if (--kd[w] <= L) {
+                    state[w] = 1;
+                    l[w] = LEVEL(L);
+                    stack[sp++] = w;
+                } else {
+                    pv[np] = w;
+                    pnext[np] = head[kd[w]];
+                    head[kd[w]] = np++;
+                }
+            }
+            for (uint64_t o = koff[v]; o < koff[v + 1]; ++o) {
+                uint32_t u = knbr[o];
+                if (state[u]) continue;
+                if (CFAIL(--cd[u], k)) {
+                    state[u] = 1;
+                    l[u] = LEVEL(L);
+                    stack[sp++] = u;
+                }
+            }
+        }
+        if (removed == V) break;
+    }
+    free(kd);
+    free(cd);
+    free(state);
+    free(stack);
+    free(pv);
+    free(pnext);
+    free(head);
+    return last;
+}
+
+int32_t dco_klist(uint32_t V, const uint64_t* out_off, const uint32_t* out_nbr,
+                  const uint64_t* in_off, const uint32_t* in_nbr, int32_t k,
+                  int32_t* l_out) {
+    return peel(V, out_off, out_nbr, in_off, in_nbr, k, l_out);
+}
+
+int32_t dco_kmax(uint32_t V, const uint64_t* out_off, const uint32_t* out_nbr,
+                 const uint64_t* in_off, const uint32_t* in_nbr,
+                 int32_t* kmaxes) {
+    return peel(V, in_off, in_nbr, out_off, out_nbr, 0, kmaxes);
+}
+
+int32_t dco_lrows(uint32_t V, const uint64_t* out_off, const uint32_t* out_nbr,
+                  const uint64_t* in_off, const uint32_t* in_nbr, int32_t kmax,
+                  int32_t* lmaxes, int nthreads) {
+    int32_t lmax = 0;
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1) reduction(max : lmax) num_threads(nthreads)
+#endif
+    for (int32_t k = 0; k <= kmax; ++k) {
+        int32_t lk = peel(V, out_off, out_nbr, in_off, in_nbr, k,
+                          lmaxes + (size_t)k * V);
+        if (lk > lmax) lmax = lk;
+    }
+    (void)nthreads;
+    return lmax;
+}
+
+/* h-index of the multiset {min(cap, x_i)}: the largest h with at least h
+ * entries >= h.  hist must hold cap+1 zeroed counters; it is re-zeroed. */
+static int32_t hindex_capped(const int32_t* vals, const uint32_t* ids,
+                             uint64_t n, int32_t cap, int32_t* hist,
+                             const uint8_t* member, int32_t need) {
+    /* need == 0: h-index; need > 0: largest h with >= need entries >= h */
+    for (uint64_t i = 0; i < n; ++i) {
+        uint32_t w = ids[i];
+        if (member && !member[w]) continue;
+        int32_t x = vals[w];
+        hist[x < cap ? x : cap]++;
+    }
+    int32_t cnt = 0, h = 0;
+    for (int32_t i = cap; i >= 0; --i) {
+        cnt += hist[i];
+        if (cnt >= (need ? need : i)) {
+            h = i;
+            break;
+        }
+    }
+    for (int32_t i = 0; i <= cap; ++i) hist[i] = 0;
+    return h;
+}
+
+int32_t dco_klist_hindex(uint32_t V, const uint64_t* out_off,
+                         const uint32_t* out_nbr, const uint64_t* in_off,
+                         const uint32_t* in_nbr, const int32_t* kmaxes,
+                         int32_t k, int32_t* l_out, int32_t* iters) {
+    uint8_t* member = (uint8_t*)malloc(V ? V : 1);
+    int32_t* cur = (int32_t*)malloc(sizeof(int32_t) * (V ? V : 1));
+    int32_t* nxt = (int32_t*)malloc(sizeof(int32_t) * (V ? V : 1));
+    int32_t maxd = 0;
+    for (uint32_t v = 0; v < V; ++v) member[v] = kmaxes[v] >= k;
+    for (uint32_t v = 0; v < V; ++v) {
+        int32_t d = 0;
+        if (member[v])
+            for (uint64_t o = out_off[v]; o < out_off[v + 1]; ++o)
+                d += member[out_nbr[o]];
+        cur[v] = d;
+        if (d > maxd) maxd = d;
+    }
+    int32_t* hist = (int32_t*)calloc((size_t)maxd + 1, sizeof(int32_t));
+    int32_t it = 0;
+    for (;;) {
+        int changed = 0;
+        ++it;
+        for (uint32_t v = 0; v < V; ++v) {
+            if (!member[v]) {
+                nxt[v] = 0;
+                continue;
+            }
+            int32_t lv = cur[v];
+            int32_t ho = hindex_capped(cur, out_nbr + out_off[v],
+                                       out_off[v + 1] - out_off[v], lv, hist,
+                                       member, 0);
+            int32_t hi = k == 0 ? lv
+                                : hindex_capped(cur, in_nbr + in_off[v],
+                                                in_off[v + 1] - in_off[v], lv,
+                                                hist, member, k);
+            int32_t nv = lv;
+            if (ho < nv) nv = ho;
+            if (hi < nv) nv = hi;
+            nxt[v] = nv;
+            changed |= nv != lv;
+        }
+        int32_t* t = cur;
+        cur = nxt;
+        nxt = t;
+        if (!changed) break;
+    }
+    int32_t lmax = 0;
+    for (uint32_t v = 0; v < V; ++v) {
+        l_out[v] = cur[v];
+        if (cur[v] > lmax) lmax = cur[v];
+    }
+    if (iters) *iters = it;
+    free(member);
+    free(cur);
+    free(nxt);
+    free(hist);
+    return lmax;
+}
+
+int32_t dco_kmax_hindex(uint32_t V, const uint64_t* in_off,
+                        const uint32_t* in_nbr, int32_t* kmaxes,
+                        int32_t* iters) {
+    int32_t* nxt = (int32_t*)malloc(sizeof(int32_t) * (V ? V : 1));
+    int32_t maxd = 0;
+    for (uint32_t v = 0; v < V; ++v) {
+        kmaxes[v] = (int32_t)(in_off[v + 1] - in_off[v]);
+        if (kmaxes[v] > maxd) maxd = kmaxes[v];
+    }
+    int32_t* hist = (int32_t*)calloc((size_t)maxd + 1, sizeof(int32_t));
+    int32_t it = 0;
+    for (;;) {
+        int changed = 0;
+        ++it;
+        for (uint32_t v = 0; v < V; ++v) {
+            int32_t kv = kmaxes[v];
+            int32_t h = hindex_capped(kmaxes, in_nbr + in_off[v],
+                                      in_off[v + 1] - in_off[v], kv, hist,
+                                      NULL, 0);
+            nxt[v] = h < kv ? h : kv;
+            changed |= nxt[v] != kv;
+        }
+        memcpy(kmaxes, nxt, sizeof(int32_t) * V);
+        if (!changed) break;
+    }
+    int32_t kmax = V ? 0 : -1;
+    for (uint32_t v = 0; v < V; ++v)
+        if (kmaxes[v] > kmax) kmax = kmaxes[v];
+    if (iters) *iters = it;
+    free(nxt);
+    free(hist);
+    return kmax;
+}

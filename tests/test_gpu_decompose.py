@@ -1,0 +1,142 @@
+"""GPU parity of the decomposition against the CPU oracle (bit-exact).
+
+Every case prints its name before touching the GPU so a failing run names
+it.  Variants: the plain one-k-per-launch grid peel and the batched
+persistent launch (one workgroup per k-pass) must both equal the oracle.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import dcx
+import graphs
+import oracle
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+VARIANTS = [dcx.VARIANT_PEEL, dcx.VARIANT_BATCHED]
+
+
+def say(msg):
+    print(f"[case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = dcx.Context(0)
+    yield c
+    c.close()
+
+
+def gpu_decompose(ctx, V, src, dst, variant, slack=0):
+    g = ctx.upload_edges(V, src, dst, slack)
+    K, lmax = g.decompose(variant)
+    km = g.kmaxes()
+    rows = g.rows() if K >= 0 else np.zeros((0, V), np.int32)
+    rp, vals = g.ragged()
+    g.free()
+    return K, km, lmax, rows, rp, vals
+
+
+def check_case(ctx, name, V, src, dst, slack=0, variants=VARIANTS):
+    ref = oracle.decompose(oracle.csr_from_edges(V, src, dst))
+    for var in variants:
+        say(f"{name} V={V} E={len(src)} variant={var} slack={slack}")
+        K, km, lmax, rows, rp, vals = gpu_decompose(ctx, V, src, dst, var, slack)
+        assert K == ref[0], (name, var)
+        assert np.array_equal(km, ref[1]), (name, var)
+        assert lmax == ref[2], (name, var)
+        assert np.array_equal(rows, ref[3]), (name, var)
+        orp, ovals = oracle.ragged(ref[1], ref[3])
+        assert np.array_equal(rp, orp) and np.array_equal(vals, ovals), (name, var)
+
+
+def test_fixture_digraph(ctx):
+    V, s, d = dcx.text_read(os.path.join(GOLDEN, "digraph"), prune=False)
+    K0, L0, km0, rows0 = dcx.decomp_read(os.path.join(GOLDEN, "digraph.decomp"), V)
+    for var in VARIANTS:
+        say(f"fixture digraph variant={var}")
+        K, km, lmax, rows, _, _ = gpu_decompose(ctx, V, s, d, var, slack=1)
+        assert (K, lmax) == (K0, L0)
+        assert np.array_equal(km, km0)
+        assert np.array_equal(rows, rows0)
+
+
+@pytest.mark.parametrize("name,maker", [
+    ("complete6", lambda: graphs.complete(6)),
+    ("cycle9", lambda: graphs.cycle(9)),
+    ("dag30", lambda: graphs.dag(30, 0.3, 1)),
+    ("outstar", lambda: graphs.out_star(12)),
+    ("instar", lambda: graphs.in_star(12)),
+    ("biK35", lambda: graphs.bi_kab(3, 5)),
+    ("circ20_4", lambda: graphs.circulant(20, 4)),
+    ("isolated", lambda: (7, np.array([1]), np.array([2]))),
+    ("noedges", lambda: (5, np.array([], np.int64), np.array([], np.int64))),
+])
+def test_closed_forms(ctx, name, maker):
+    V, s, d = maker()
+    check_case(ctx, name, V, s, d)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_small(ctx, seed):
+    rng = np.random.default_rng(seed)
+    V = int(rng.integers(20, 400))
+    V, s, d = graphs.random_digraph(V, float(rng.uniform(0.01, 0.2)), seed)
+    check_case(ctx, f"random{seed}", V, s, d, slack=seed % 3)
+
+
+@pytest.mark.parametrize("scale", [8, 11, 14])
+def test_rmat_small(ctx, scale):
+    s, d = dcx.gen_rmat(scale, 16 << scale, seed=scale)
+    check_case(ctx, f"rmat{scale}", 1 << scale, s, d)
+
+
+def test_er_16k(ctx):
+    s, d = dcx.gen_er(16384, 131072, seed=1)
+    check_case(ctx, "er16k", 16384, s, d)
+
+
+def test_chunglu_small(ctx):
+    s, d = dcx.gen_chunglu(5000, 40000, seed=1001)
+    check_case(ctx, "chunglu5k", 5000, s, d)
+
+
+@pytest.mark.parametrize("k", [0, 1, 2, 5, 40])
+def test_klist_unfiltered_and_filtered(ctx, k):
+    """dcx_klist before kmax is known runs on the whole graph (KList
+    semantics); afterwards it is restricted to the (k,0)-core."""
+    s, d = dcx.gen_rmat(12, 16 << 12, seed=7)
+    V = 1 << 12
+    ref_lk, ref_l = oracle.klist(oracle.csr_from_edges(V, s, d), k)
+    g = ctx.upload_edges(V, s, d)
+    say(f"klist unfiltered k={k}")
+    lk, l = g.klist(k)
+    assert lk == ref_lk and np.array_equal(l, ref_l)
+    K, km = g.kmax()
+    say(f"klist filtered k={k}")
+    lk2, l2 = g.klist(k)
+    assert np.array_equal(l2, ref_l)
+    if k <= K:
+        assert lk2 == ref_lk
+    g.free()
+
+
+def test_rmat20_full(ctx):
+    """The headline graph: R-MAT scale 20, 16*2^20 samples, seed 1."""
+    s, d = dcx.gen_rmat(20, 16 << 20, seed=1)
+    V = 1 << 20
+    say(f"rmat20 oracle E={len(s)}")
+    ref = oracle.decompose(oracle.csr_from_edges(V, s, d))
+    for var in [dcx.VARIANT_BATCHED]:
+        say(f"rmat20 gpu variant={var}")
+        g = ctx.upload_edges(V, s, d)
+        K, lmax = g.decompose(var)
+        assert K == ref[0] and lmax == ref[2]
+        assert np.array_equal(g.kmaxes(), ref[1])
+        assert np.array_equal(g.rows(), ref[3])
+        g.free()
