@@ -20,6 +20,8 @@ struct dcx_ctx {
     int device = 0;
     int num_cus = 0;
     int coop_grid = 0;
+    int hindex_grid = 0;
+    uint64_t timeout_ticks = 0;  // wall-clock ticks one grid barrier may take
     hipStream_t stream = nullptr;
     std::mutex mu;
     std::string err;
@@ -27,19 +29,41 @@ struct dcx_ctx {
     // grown-on-demand device scratch
     void* scratch = nullptr;
     size_t scratch_bytes = 0;
-    uint32_t* ctrs = nullptr;  // small counters: 64 words
-    int32_t* d_scalars = nullptr;  // 64 words
-    int32_t* h_scalars = nullptr;  // pinned mirror
+    // small device words: [0,12) grid-pass phase slots, [16,20) grid sync
+    // words, [20,23) h-index sweep flags, [32] batched queue counter,
+    // [40] batched error word
+    uint32_t* ctrs = nullptr;
+    int32_t* d_scalars = nullptr;  // 64 words of kernel results
+    int32_t* h_scalars = nullptr;  // pinned mirror (words 32.. hold error words)
     hipEvent_t ev[8] = {};
+    // staging for update batches (device + pinned host), grown on demand
+    uint32_t* upd_dev = nullptr;
+    size_t upd_dev_words = 0;
+    uint32_t* upd_host = nullptr;
+    size_t upd_host_words = 0;
 };
+
+namespace {
+constexpr int kCtrSync = 16, kCtrFlags = 20, kCtrQueue = 32, kCtrErr = 40;
+// Workgroups per CU for cooperative launches: far below the occupancy
+// limit, so residency of the whole grid is never in question.
+constexpr int kCoopBlocksPerCu = 2;
+constexpr double kBarrierTimeoutS = 30.0;
+}  // namespace
 
 struct dcx_graph {
     dcx_ctx* ctx = nullptr;
     uint32_t V = 0;
     uint64_t E = 0;
+    // Adjacency arrays: lists start at off[v] with room for capv[v] entries;
+    // *_cap is the used prefix (a bump pointer for relocated lists), *_alloc
+    // the allocated length.  Free slots hold 0xFFFFFFFF.
     uint64_t out_cap = 0, in_cap = 0;
+    uint64_t out_alloc = 0, in_alloc = 0;
     uint32_t *out_off = nullptr, *in_off = nullptr, *out_nbr = nullptr, *in_nbr = nullptr;
     int32_t *out_deg = nullptr, *in_deg = nullptr;
+    uint32_t *out_capv = nullptr, *in_capv = nullptr;
+    uint64_t relocations = 0;
     // resident results
     int32_t* kmaxes = nullptr;
     int32_t kmax = -1;
@@ -49,8 +73,10 @@ struct dcx_graph {
     uint32_t* hist = nullptr;    // [kmax+2]
     size_t vk_cap = 0;
     std::vector<uint32_t> h_vk;  // host copy of vk
+    int32_t* kmax_old = nullptr; // [V] previous kmaxes during an update
     int32_t* rows = nullptr;     // (kmax+1) x V
     size_t rows_cap = 0;         // rows allocated
+    int32_t rows_k = 0;          // rows currently held (kmax+1 when valid)
     int32_t* lmax_rows = nullptr;
     size_t lmax_rows_cap = 0;
     int32_t lmax = -1;
@@ -157,27 +183,81 @@ PassSpec row_spec(const dcx_graph* g, int32_t k, int32_t* lval, bool filtered) {
     return p;
 }
 
-// Runs one pass on the whole grid (cooperative launch); returns lmax of it.
-int32_t run_grid_pass(dcx_ctx* c, const PassSpec& p) {
-    ensure_scratch(c, (size_t)kScratchArrays * std::max<uint32_t>(p.V, 1) * 4);
-    PassScratch s = scratch_slot(c->scratch, p.V, 0);
-    HIPX(c, launch_pass_grid(p, s, c->ctrs, c->d_scalars, c->coop_grid, c->stream));
-    HIPX(c, hipMemcpyAsync(c->h_scalars, c->d_scalars, sizeof(int32_t), hipMemcpyDeviceToHost,
+GridSync gsync(dcx_ctx* c, int grid) {
+    GridSync g;
+    g.w = c->ctrs + kCtrSync;
+    g.nblocks = (uint32_t)grid;
+    g.timeout_ticks = c->timeout_ticks;
+    return g;
+}
+
+// Clears the kernel error words before a series of launches.
+void clear_kernel_errors(dcx_ctx* c) {
+    HIPX(c, hipMemsetAsync(c->ctrs + kCtrSync + kSyncError, 0, sizeof(uint32_t), c->stream));
+    HIPX(c, hipMemsetAsync(c->ctrs + kCtrErr, 0, sizeof(uint32_t), c->stream));
+}
+
+// Reads the kernel error words (synchronises) and turns them into a status.
+void check_kernel_errors(dcx_ctx* c, const char* what) {
+    HIPX(c, hipMemcpyAsync(c->h_scalars + 32, c->ctrs + kCtrSync + kSyncError, 4,
+                           hipMemcpyDeviceToHost, c->stream));
+    HIPX(c, hipMemcpyAsync(c->h_scalars + 33, c->ctrs + kCtrErr, 4, hipMemcpyDeviceToHost,
                            c->stream));
     sync(c);
+    const int32_t e = std::max(c->h_scalars[32], c->h_scalars[33]);
+    if (e == (int32_t)kErrTimeout) {
+        set_err(c, "%s: a grid barrier timed out (workgroups not co-resident?)", what);
+        throw Fail{DCX_ERR_TIMEOUT};
+    }
+    if (e != 0) {
+        set_err(c, "%s: kernel counters inconsistent (code %d)", what, e);
+        throw Fail{DCX_ERR_INTERNAL};
+    }
+}
+
+// Runs one pass on the whole grid (cooperative launch); returns lmax of it.
+// Events ev[a], ev[b] (if a >= 0) bracket the kernel.
+int32_t run_grid_pass(dcx_ctx* c, const PassSpec& p, int a = -1, int b = -1) {
+    ensure_scratch(c, (size_t)kScratchArrays * std::max<uint32_t>(p.V, 1) * 4);
+    PassScratch s = scratch_slot(c->scratch, p.V, 0);
+    clear_kernel_errors(c);
+    if (a >= 0) HIPX(c, hipEventRecord(c->ev[a], c->stream));
+    HIPX(c, launch_pass_grid(p, s, c->ctrs, c->d_scalars, gsync(c, c->coop_grid), c->stream));
+    if (b >= 0) HIPX(c, hipEventRecord(c->ev[b], c->stream));
+    HIPX(c, hipMemcpyAsync(c->h_scalars, c->d_scalars, sizeof(int32_t), hipMemcpyDeviceToHost,
+                           c->stream));
+    check_kernel_errors(c, "grid peel pass");
     return c->h_scalars[0];
 }
 
-void compute_kmax(dcx_ctx* c, dcx_graph* g) {
+// In-core numbers.  cap >= 0 (valid resident kmaxes required): only levels
+// 0..cap are peeled and vertices left alive keep max(old, cap+1) -- exact
+// when no (k,0)-core with k > cap can have changed.
+void compute_kmax(dcx_ctx* c, dcx_graph* g, int32_t cap = -1) {
     if (!g->kmaxes) dev_alloc(c, &g->kmaxes, g->V);
     if (g->V == 0) {
         g->kmax = -1;
         g->kmax_valid = true;
         return;
     }
-    HIPX(c, hipEventRecord(c->ev[0], c->stream));
-    g->kmax = run_grid_pass(c, kmax_spec(g));
-    HIPX(c, hipEventRecord(c->ev[1], c->stream));
+    if (cap >= 0 && g->kmax_valid) {
+        if (!g->kmax_old) dev_alloc(c, &g->kmax_old, g->V);
+        HIPX(c, hipMemcpyAsync(g->kmax_old, g->kmaxes, 4ull * g->V, hipMemcpyDeviceToDevice,
+                               c->stream));
+        PassSpec p = kmax_spec(g);
+        p.level_cap = cap;
+        (void)run_grid_pass(c, p, 0, 1);
+        HIPX(c, launch_capped_fixup(g->kmaxes, g->kmax_old, g->V, cap, c->stream));
+        HIPX(c, hipMemsetAsync(c->d_scalars + 1, 0, 4, c->stream));
+        HIPX(c, launch_max_i32(g->kmaxes, g->V, c->d_scalars + 1, c->stream));
+        HIPX(c, hipMemcpyAsync(c->h_scalars + 1, c->d_scalars + 1, 4, hipMemcpyDeviceToHost,
+                               c->stream));
+        sync(c);
+        g->kmax = c->h_scalars[1];
+    } else {
+        g->kmax = run_grid_pass(c, kmax_spec(g), 0, 1);
+    }
+    HIPX(c, hipEventRecord(c->ev[7], c->stream));
     // vertex order by kmax, members per k
     size_t need = (size_t)g->kmax + 2;
     if (g->vk_cap < need) {
@@ -195,39 +275,65 @@ void compute_kmax(dcx_ctx* c, dcx_graph* g) {
     HIPX(c, hipEventRecord(c->ev[2], c->stream));
     sync(c);
     c->timing.kmax_ms = elapsed(c, 0, 1);
-    c->timing.order_ms = elapsed(c, 1, 2);
+    c->timing.order_ms = elapsed(c, 7, 2);
     g->kmax_valid = true;
 }
 
-void compute_rows(dcx_ctx* c, dcx_graph* g, int variant) {
+// Grows a device array to n elements keeping its first `keep` elements.
+template <class T>
+void grow_keep(dcx_ctx* c, T** p, size_t* cap, size_t n, size_t keep) {
+    if (*cap >= n) return;
+    T* q = nullptr;
+    dev_alloc(c, &q, n);
+    if (*p && keep)
+        HIPX(c, hipMemcpyAsync(q, *p, keep * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
+    sync(c);
+    dev_free(*p);
+    *p = q;
+    *cap = n;
+}
+
+// Computes rows k = 0..kr (kr < 0: all rows 0..kmax) with the given
+// variant; rows above kr keep their values (they must be valid already).
+void compute_rows(dcx_ctx* c, dcx_graph* g, int variant, int32_t kr = -1) {
     const uint32_t V = g->V;
     const int32_t K = g->kmax;
+    const bool had_rows = g->rows_valid;
+    const int32_t old_rows = had_rows ? g->rows_k : 0;
     g->rows_valid = false;
     if (K < 0) {
         g->lmax = 0;
+        g->rows_k = 0;
         g->rows_valid = true;
         return;
     }
+    if (kr < 0 || kr > K || !had_rows) kr = K;
     const size_t nrows = (size_t)K + 1;
-    if (g->rows_cap < nrows * V) {
-        dev_free(g->rows);
-        dev_alloc(c, &g->rows, nrows * V);
-        g->rows_cap = nrows * V;
-    }
-    if (g->lmax_rows_cap < nrows) {
-        dev_free(g->lmax_rows);
-        dev_alloc(c, &g->lmax_rows, nrows);
-        g->lmax_rows_cap = nrows;
-    }
+    const size_t keep_rows = std::min<size_t>((size_t)old_rows, nrows);
+    grow_keep(c, &g->rows, &g->rows_cap, nrows * V, keep_rows * V);
+    grow_keep(c, &g->lmax_rows, &g->lmax_rows_cap, nrows, keep_rows);
+    const size_t nre = (size_t)kr + 1;
+    clear_kernel_errors(c);
     HIPX(c, hipEventRecord(c->ev[3], c->stream));
-    HIPX(c, hipMemsetAsync(g->rows, 0, nrows * V * sizeof(int32_t), c->stream));
-    int launches = 0;
-    if (variant == DCX_VARIANT_PEEL) {
-        for (int32_t k = 0; k <= K; ++k) {
+    HIPX(c, hipMemsetAsync(g->rows, 0, nre * V * sizeof(int32_t), c->stream));
+    int launches = 0, slots_used = 1;
+    HIPX(c, hipEventRecord(c->ev[5], c->stream));
+    if (variant == DCX_VARIANT_HINDEX) {
+        // rows in k order, each started from the row above (l_k <= l_{k-1})
+        for (int32_t k = 0; k <= kr; ++k) {
             PassSpec p = row_spec(g, k, g->rows + (size_t)k * V, true);
-            ensure_scratch(c, (size_t)kScratchArrays * V * 4);
+            HIPX(c, launch_hindex_row(p, k ? g->rows + (size_t)(k - 1) * V : nullptr,
+                                      c->ctrs + kCtrFlags, g->lmax_rows + k, c->d_scalars + 8,
+                                      gsync(c, c->hindex_grid), c->stream));
+            ++launches;
+        }
+    } else if (variant == DCX_VARIANT_PEEL) {
+        ensure_scratch(c, (size_t)kScratchArrays * V * 4);
+        for (int32_t k = 0; k <= kr; ++k) {
+            PassSpec p = row_spec(g, k, g->rows + (size_t)k * V, true);
             PassScratch s = scratch_slot(c->scratch, V, 0);
-            HIPX(c, launch_pass_grid(p, s, c->ctrs, g->lmax_rows + k, c->coop_grid, c->stream));
+            HIPX(c, launch_pass_grid(p, s, c->ctrs, g->lmax_rows + k, gsync(c, c->coop_grid),
+                                     c->stream));
             ++launches;
         }
     } else {
@@ -240,24 +346,30 @@ void compute_rows(dcx_ctx* c, dcx_graph* g, int variant) {
             budget = std::min<size_t>(fr / 2, (size_t)64 << 30);
         }
         int slots = c->cfg.rows_slots > 0 ? c->cfg.rows_slots : c->num_cus * 2;
-        slots = (int)std::min<size_t>((size_t)slots, nrows);
+        slots = (int)std::min<size_t>((size_t)slots, nre);
         slots = (int)std::min<size_t>((size_t)slots, std::max<size_t>(budget / per_slot, 1));
         ensure_scratch(c, per_slot * slots);
-        HIPX(c, hipMemsetAsync(c->ctrs + 32, 0, sizeof(uint32_t), c->stream));
+        HIPX(c, hipMemsetAsync(c->ctrs + kCtrQueue, 0, sizeof(uint32_t), c->stream));
         PassSpec base = row_spec(g, 0, g->rows, true);
-        HIPX(c, launch_rows_batched(base, c->scratch, slots, g->vk, K, g->rows, g->lmax_rows,
-                                    c->ctrs + 32, c->stream));
+        HIPX(c, hipEventRecord(c->ev[5], c->stream));
+        HIPX(c, launch_rows_batched(base, c->scratch, slots, g->vk, kr, g->rows, g->lmax_rows,
+                                    c->ctrs + kCtrQueue, c->ctrs + kCtrErr, c->stream));
         launches = 1;
+        slots_used = slots;
     }
+    HIPX(c, hipEventRecord(c->ev[6], c->stream));
     std::vector<int32_t> lm(nrows);
     HIPX(c, hipMemcpyAsync(lm.data(), g->lmax_rows, nrows * sizeof(int32_t),
                            hipMemcpyDeviceToHost, c->stream));
     HIPX(c, hipEventRecord(c->ev[4], c->stream));
-    sync(c);
+    check_kernel_errors(c, "l-rows");
     c->timing.rows_ms = elapsed(c, 3, 4);
+    c->timing.rows_kernel_ms = elapsed(c, 5, 6);
     c->timing.rows_launches = launches;
+    c->timing.rows_slots = slots_used;
     g->lmax = 0;
     for (int32_t x : lm) g->lmax = std::max(g->lmax, x);
+    g->rows_k = K + 1;
     g->rows_valid = true;
 }
 
@@ -337,7 +449,14 @@ dcx_status dcx_ctx_create(const dcx_config* cfg, dcx_ctx** out) {
     if (hipGetDeviceProperties(&prop, c->device) != hipSuccess) return DCX_ERR_HIP;
     if (!prop.cooperativeLaunch) return DCX_ERR_NO_DEVICE;
     c->num_cus = prop.multiProcessorCount;
-    c->coop_grid = c->num_cus * pass_grid_blocks_per_cu();
+    c->coop_grid = c->num_cus * std::min(kCoopBlocksPerCu, pass_grid_blocks_per_cu());
+    c->hindex_grid = c->num_cus * std::min(kCoopBlocksPerCu, hindex_blocks_per_cu());
+    int clock_khz = 0;
+    if (hipDeviceGetAttribute(&clock_khz, hipDeviceAttributeWallClockRate, c->device) !=
+            hipSuccess ||
+        clock_khz <= 0)
+        clock_khz = 100000;  // 100 MHz, the CDNA constant clock
+    c->timeout_ticks = (uint64_t)(kBarrierTimeoutS * 1000.0 * clock_khz);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
         return DCX_ERR_HIP;
     for (auto& e : c->ev)
@@ -358,7 +477,9 @@ dcx_status dcx_ctx_destroy(dcx_ctx* c) {
     dev_free(c->scratch);
     dev_free(c->ctrs);
     dev_free(c->d_scalars);
+    dev_free(c->upd_dev);
     if (c->h_scalars) (void)hipHostFree(c->h_scalars);
+    if (c->upd_host) (void)hipHostFree(c->upd_host);
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -420,20 +541,37 @@ dcx_status dcx_graph_upload_csr(dcx_ctx* ctx, uint32_t V, uint64_t E, const uint
         g->E = E;
         g->out_cap = out_off[V];
         g->in_cap = in_off[V];
+        // headroom for lists relocated by later inserts
+        auto headroom = [](uint64_t n) { return n + n / 16 + 4096; };
+        g->out_alloc = std::min<uint64_t>(headroom(g->out_cap), 0xffffffffull);
+        g->in_alloc = std::min<uint64_t>(headroom(g->in_cap), 0xffffffffull);
+        std::vector<uint32_t> capo(V), capi(V);
+        for (uint32_t v = 0; v < V; ++v) {
+            capo[v] = out_off[v + 1] - out_off[v];
+            capi[v] = in_off[v + 1] - in_off[v];
+        }
         try {
             dev_alloc(ctx, &g->out_off, (size_t)V + 1);
             dev_alloc(ctx, &g->in_off, (size_t)V + 1);
             dev_alloc(ctx, &g->out_deg, V);
             dev_alloc(ctx, &g->in_deg, V);
-            dev_alloc(ctx, &g->out_nbr, g->out_cap);
-            dev_alloc(ctx, &g->in_nbr, g->in_cap);
+            dev_alloc(ctx, &g->out_capv, V);
+            dev_alloc(ctx, &g->in_capv, V);
+            dev_alloc(ctx, &g->out_nbr, g->out_alloc);
+            dev_alloc(ctx, &g->in_nbr, g->in_alloc);
             hipStream_t st = ctx->stream;
             HIPX(ctx, hipMemcpyAsync(g->out_off, out_off, 4ull * (V + 1), hipMemcpyHostToDevice, st));
             HIPX(ctx, hipMemcpyAsync(g->in_off, in_off, 4ull * (V + 1), hipMemcpyHostToDevice, st));
             if (V) {
                 HIPX(ctx, hipMemcpyAsync(g->out_deg, out_deg, 4ull * V, hipMemcpyHostToDevice, st));
                 HIPX(ctx, hipMemcpyAsync(g->in_deg, in_deg, 4ull * V, hipMemcpyHostToDevice, st));
+                HIPX(ctx, hipMemcpyAsync(g->out_capv, capo.data(), 4ull * V, hipMemcpyHostToDevice,
+                                         st));
+                HIPX(ctx, hipMemcpyAsync(g->in_capv, capi.data(), 4ull * V, hipMemcpyHostToDevice,
+                                         st));
             }
+            HIPX(ctx, hipMemsetAsync(g->out_nbr, 0xff, 4ull * g->out_alloc, st));
+            HIPX(ctx, hipMemsetAsync(g->in_nbr, 0xff, 4ull * g->in_alloc, st));
             if (g->out_cap)
                 HIPX(ctx, hipMemcpyAsync(g->out_nbr, out_nbr, 4ull * g->out_cap,
                                          hipMemcpyHostToDevice, st));
@@ -447,6 +585,8 @@ dcx_status dcx_graph_upload_csr(dcx_ctx* ctx, uint32_t V, uint64_t E, const uint
             dev_free(raw->in_off);
             dev_free(raw->out_deg);
             dev_free(raw->in_deg);
+            dev_free(raw->out_capv);
+            dev_free(raw->in_capv);
             dev_free(raw->out_nbr);
             dev_free(raw->in_nbr);
             delete raw;
@@ -483,6 +623,9 @@ dcx_status dcx_graph_free(dcx_graph* g) {
     dev_free(g->in_deg);
     dev_free(g->out_nbr);
     dev_free(g->in_nbr);
+    dev_free(g->out_capv);
+    dev_free(g->in_capv);
+    dev_free(g->kmax_old);
     dev_free(g->kmaxes);
     dev_free(g->order);
     dev_free(g->vk);
@@ -503,11 +646,48 @@ dcx_status dcx_graph_info(const dcx_graph* g, uint32_t* V, uint64_t* E, int32_t*
     return DCX_OK;
 }
 
+namespace {
+// Per-vertex capacities of one direction, summed (the length of a repacked
+// adjacency array).
+uint64_t capacity_sum(dcx_ctx* c, const uint32_t* capv, uint32_t V) {
+    std::vector<uint32_t> cv(V);
+    if (V) HIPX(c, hipMemcpy(cv.data(), capv, 4ull * V, hipMemcpyDeviceToHost));
+    uint64_t t = 0;
+    for (uint32_t x : cv) t += x;
+    return t;
+}
+
+// Device lists of one direction -> reference layout (off[v+1]-off[v] =
+// capacity of v, lists in place, free slots 0xFFFFFFFF).
+void download_side(dcx_ctx* c, uint32_t V, const uint32_t* d_off, const int32_t* d_deg,
+                   const uint32_t* d_capv, const uint32_t* d_nbr, uint64_t used, uint32_t* off,
+                   int32_t* deg, uint32_t* nbr) {
+    std::vector<uint32_t> o(V + 1), cv(V), all(used ? used : 1);
+    HIPX(c, hipMemcpy(o.data(), d_off, 4ull * (V + 1), hipMemcpyDeviceToHost));
+    if (V) {
+        HIPX(c, hipMemcpy(deg, d_deg, 4ull * V, hipMemcpyDeviceToHost));
+        HIPX(c, hipMemcpy(cv.data(), d_capv, 4ull * V, hipMemcpyDeviceToHost));
+    }
+    if (used) HIPX(c, hipMemcpy(all.data(), d_nbr, 4ull * used, hipMemcpyDeviceToHost));
+    uint64_t pos = 0;
+    for (uint32_t v = 0; v < V; ++v) {
+        off[v] = (uint32_t)pos;
+        for (uint32_t j = 0; j < cv[v]; ++j)
+            nbr[pos + j] = j < (uint32_t)deg[v] ? all[o[v] + j] : 0xffffffffu;
+        pos += cv[v];
+    }
+    off[V] = (uint32_t)pos;
+}
+}  // namespace
+
 dcx_status dcx_graph_capacity(const dcx_graph* g, uint64_t* out_cap, uint64_t* in_cap) {
     if (!g) return DCX_ERR_INVALID_ARG;
-    if (out_cap) *out_cap = g->out_cap;
-    if (in_cap) *in_cap = g->in_cap;
-    return DCX_OK;
+    dcx_ctx* ctx = g->ctx;
+    return guarded(ctx, [&]() -> dcx_status {
+        if (out_cap) *out_cap = capacity_sum(ctx, g->out_capv, g->V);
+        if (in_cap) *in_cap = capacity_sum(ctx, g->in_capv, g->V);
+        return DCX_OK;
+    });
 }
 
 dcx_status dcx_graph_download_csr(dcx_ctx* ctx, const dcx_graph* g, uint32_t* out_off,
@@ -518,19 +698,11 @@ dcx_status dcx_graph_download_csr(dcx_ctx* ctx, const dcx_graph* g, uint32_t* ou
     if (!out_off || !out_deg || !out_nbr || !in_off || !in_deg || !in_nbr)
         return DCX_ERR_INVALID_ARG;
     return guarded(ctx, [&]() -> dcx_status {
-        hipStream_t st = ctx->stream;
-        const uint32_t V = g->V;
-        HIPX(ctx, hipMemcpyAsync(out_off, g->out_off, 4ull * (V + 1), hipMemcpyDeviceToHost, st));
-        HIPX(ctx, hipMemcpyAsync(in_off, g->in_off, 4ull * (V + 1), hipMemcpyDeviceToHost, st));
-        if (V) {
-            HIPX(ctx, hipMemcpyAsync(out_deg, g->out_deg, 4ull * V, hipMemcpyDeviceToHost, st));
-            HIPX(ctx, hipMemcpyAsync(in_deg, g->in_deg, 4ull * V, hipMemcpyDeviceToHost, st));
-        }
-        if (g->out_cap)
-            HIPX(ctx, hipMemcpyAsync(out_nbr, g->out_nbr, 4ull * g->out_cap, hipMemcpyDeviceToHost, st));
-        if (g->in_cap)
-            HIPX(ctx, hipMemcpyAsync(in_nbr, g->in_nbr, 4ull * g->in_cap, hipMemcpyDeviceToHost, st));
         sync(ctx);
+        download_side(ctx, g->V, g->out_off, g->out_deg, g->out_capv, g->out_nbr, g->out_cap,
+                      out_off, out_deg, out_nbr);
+        download_side(ctx, g->V, g->in_off, g->in_deg, g->in_capv, g->in_nbr, g->in_cap, in_off,
+                      in_deg, in_nbr);
         return DCX_OK;
     });
 }
@@ -553,7 +725,8 @@ dcx_status dcx_klist(dcx_ctx* ctx, dcx_graph* g, int32_t k, int32_t variant, int
     dcx_status s = check_graph(ctx, g);
     if (s != DCX_OK) return s;
     if (k < 0) return DCX_ERR_INVALID_ARG;
-    if (variant != DCX_VARIANT_AUTO && variant != DCX_VARIANT_PEEL) {
+    if (variant != DCX_VARIANT_AUTO && variant != DCX_VARIANT_PEEL &&
+        variant != DCX_VARIANT_HINDEX) {
         set_err(ctx, "dcx_klist: variant %d not available for a single row", variant);
         return DCX_ERR_INVALID_ARG;
     }
@@ -568,12 +741,23 @@ dcx_status dcx_klist(dcx_ctx* ctx, dcx_graph* g, int32_t k, int32_t variant, int
         int32_t lk = 0;
         try {
             // with resident kmaxes the pass is restricted to the (k,0)-core;
-            // otherwise it runs on the whole graph exactly like KList
+            // otherwise it runs on the whole graph exactly like KList.  The
+            // h-index fixpoint needs the core, so it computes kmaxes first.
+            if (variant == DCX_VARIANT_HINDEX && !g->kmax_valid) compute_kmax(ctx, g);
             const bool filtered = g->kmax_valid;
             if (filtered) HIPX(ctx, hipMemsetAsync(row, 0, 4ull * V, ctx->stream));
             PassSpec p = row_spec(g, k, row, filtered);
             if (filtered && p.nmembers == 0) {
                 lk = 0;
+            } else if (variant == DCX_VARIANT_HINDEX) {
+                clear_kernel_errors(ctx);
+                HIPX(ctx, launch_hindex_row(p, nullptr, ctx->ctrs + kCtrFlags, ctx->d_scalars,
+                                            ctx->d_scalars + 8, gsync(ctx, ctx->hindex_grid),
+                                            ctx->stream));
+                HIPX(ctx, hipMemcpyAsync(ctx->h_scalars, ctx->d_scalars, 4,
+                                         hipMemcpyDeviceToHost, ctx->stream));
+                check_kernel_errors(ctx, "h-index row");
+                lk = ctx->h_scalars[0];
             } else {
                 lk = run_grid_pass(ctx, p);
             }
@@ -594,7 +778,8 @@ dcx_status dcx_decompose(dcx_ctx* ctx, dcx_graph* g, int32_t variant, int32_t* k
     if (s != DCX_OK) return s;
     if (variant == DCX_VARIANT_AUTO) variant = ctx->cfg.variant;
     if (variant == DCX_VARIANT_AUTO) variant = DCX_VARIANT_BATCHED;
-    if (variant != DCX_VARIANT_PEEL && variant != DCX_VARIANT_BATCHED) {
+    if (variant != DCX_VARIANT_PEEL && variant != DCX_VARIANT_BATCHED &&
+        variant != DCX_VARIANT_HINDEX) {
         set_err(ctx, "dcx_decompose: unknown variant %d", variant);
         return DCX_ERR_INVALID_ARG;
     }
@@ -683,3 +868,7 @@ dcx_status dcx_get_ragged(dcx_ctx* ctx, const dcx_graph* g, int64_t* row_ptr, in
 }
 
 }  // extern "C"
+
+// Edge-update maintenance (same translation unit: it uses the context and
+// graph internals above).
+#include "dcx_update_impl.hpp"

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "dcx.h"
+#include "grid_sync.hpp"
 
 namespace dcx {
 
@@ -35,6 +36,9 @@ struct PassSpec {
     uint32_t nmembers;      // |{v : kmaxes[v] >= k}| (V without filter)
     int32_t k;
     int32_t* lval;          // [V] output row; non-members must already hold 0
+    // Levels above this are not peeled: vertices still alive afterwards keep
+    // lval = -1 (bounded recompute after an update).
+    int32_t level_cap = INT32_MAX;
 };
 
 // Per-pass device scratch (one slot per concurrently running pass).
@@ -62,16 +66,46 @@ __host__ __device__ inline PassScratch scratch_slot(void* base, uint32_t V, int 
 
 // Kernel launchers (peel_kernels.hip).  All enqueue on `st` and return the
 // HIP status of the launch.
+// One pass on the whole grid: cooperative launch (co-residency) with the
+// bounded barrier of grid_sync.hpp; gs.w is zeroed by the launcher.
 hipError_t launch_pass_grid(const PassSpec& p, const PassScratch& s, uint32_t* ctrs,
-                            int32_t* lmax_out, int grid, hipStream_t st);
+                            int32_t* lmax_out, const GridSync& gs, hipStream_t st);
 int pass_grid_blocks_per_cu();
-// Rows k = 0..kmax of one graph, one workgroup per pass pulled from a queue.
+// Rows k = 0..kmax of one graph, one workgroup per pass pulled from a queue;
+// a pass whose counters do not add up raises *err (kErrInconsistent).
 hipError_t launch_rows_batched(const PassSpec& base, void* scratch, int slots,
                                const uint32_t* vk, int32_t kmax, int32_t* rows,
-                               int32_t* lmax_rows, uint32_t* queue_ctr, hipStream_t st);
+                               int32_t* lmax_rows, uint32_t* queue_ctr, uint32_t* err,
+                               hipStream_t st);
 hipError_t launch_order_by_kmax(const int32_t* kmaxes, uint32_t V, int32_t kmax,
                                 uint32_t* hist /*[kmax+2]*/, uint32_t* vk /*[kmax+2]*/,
                                 uint32_t* order, hipStream_t st);
+// h-index fixpoint for one row (hindex_kernels.hip): initialises the row
+// from prev_row (or the out-degree) on the members, then refines in place.
+int hindex_blocks_per_cu();
+hipError_t launch_hindex_row(const PassSpec& p, const int32_t* prev_row, uint32_t* flags,
+                             int32_t* lmax_out, int32_t* sweeps_out, const GridSync& gs,
+                             hipStream_t st);
+// Graph updates (maint_kernels.hip).  pairs are (u, v) interleaved.
+hipError_t launch_find_edges(const uint32_t* off, const int32_t* deg, const uint32_t* nbr,
+                             const uint32_t* pairs, uint32_t n, uint32_t* found, hipStream_t st);
+hipError_t launch_check_room(const int32_t* deg, const uint32_t* capv, const uint32_t* gkey,
+                             const uint32_t* gcnt, uint32_t ngroups, uint32_t* overflow,
+                             hipStream_t st);
+hipError_t launch_relocate(uint32_t* off, const int32_t* deg, uint32_t* capv, uint32_t* nbr,
+                           const uint32_t* rkey, const uint32_t* new_off, const uint32_t* new_cap,
+                           uint32_t nr, hipStream_t st);
+hipError_t launch_apply_inserts(const uint32_t* off, int32_t* deg, uint32_t* nbr,
+                                const uint32_t* gkey, const uint32_t* gstart, const uint32_t* gcnt,
+                                const uint32_t* gval, uint32_t ngroups, hipStream_t st);
+hipError_t launch_apply_deletes(const uint32_t* off, int32_t* deg, uint32_t* nbr,
+                                const uint32_t* gkey, const uint32_t* gstart, const uint32_t* gcnt,
+                                const uint32_t* gval, uint32_t ngroups, uint32_t* missing,
+                                hipStream_t st);
+hipError_t launch_mvalue(const int32_t* kmaxes, const uint32_t* pairs, uint32_t n, int32_t* m,
+                         hipStream_t st);
+hipError_t launch_capped_fixup(int32_t* kmaxes, const int32_t* old, uint32_t V, int32_t cap,
+                               hipStream_t st);
 hipError_t launch_fill_i32(int32_t* p, int32_t v, size_t n, hipStream_t st);
 hipError_t launch_max_i32(const int32_t* p, size_t n, int32_t* out, hipStream_t st);
 

@@ -1,0 +1,368 @@
+// Edge-update maintenance: included at the end of dcx_api.hip.
+//
+// Mirrors the reference's maintenance drivers (maintenance /
+// maintenanceFast, main.cu:1536-1751): apply a batch to the device graph,
+// then bring kmaxes and the l-rows back to the values a full recomputation
+// would give.  The reference raises kmax by h-index refinement and then
+// recomputes rows k = 0..M+1 (main.cu:1731-1741); here:
+//   * deletions are applied first, then insertions, in place on the device
+//     (capacity is checked before anything changes; lists that would
+//     overflow are moved to the tail region);
+//   * kmaxes: a deletion-only batch re-peels levels 0..M_del only
+//     (M_del = max over deleted (u,v) of min(old kmax u, old kmax v), the
+//     getMValue quantity, main.cu:344-364); a batch with insertions
+//     re-peels fully;
+//   * rows: D_{k,l} can change only if a deleted edge had both endpoints in
+//     the old core or an inserted edge has both in the new one, so rows
+//     k <= R = max(M_del(old kmaxes), M_ins(new kmaxes)) are recomputed
+//     (all of them in one batched launch) and rows above R are kept.
+// The bound uses the new kmaxes for insertions because a batch of inserts
+// can raise kmaxes by more than one (e.g. a clique inserted among isolated
+// vertices), which a bound from the old kmaxes would miss.
+#pragma once
+
+#include <chrono>
+#include <unordered_set>
+
+namespace {
+
+struct Groups {
+    std::vector<uint32_t> key, start, cnt, val;
+};
+
+// Groups pairs by source (out-lists) or by target (in-lists).
+Groups group_pairs(const uint32_t* pairs, uint32_t n, bool by_src) {
+    std::vector<uint64_t> kv(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint64_t a = pairs[2 * i], b = pairs[2 * i + 1];
+        kv[i] = by_src ? (a << 32 | b) : (b << 32 | a);
+    }
+    std::sort(kv.begin(), kv.end());
+    Groups g;
+    g.val.resize(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t key = (uint32_t)(kv[i] >> 32);
+        if (g.key.empty() || g.key.back() != key) {
+            g.key.push_back(key);
+            g.start.push_back(i);
+            g.cnt.push_back(0);
+        }
+        g.cnt.back()++;
+        g.val[i] = (uint32_t)kv[i];
+    }
+    return g;
+}
+
+dcx_status validate_pairs(dcx_ctx* c, uint32_t V, const uint32_t* p, uint32_t n,
+                          const char* what) {
+    std::vector<uint64_t> keys(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t u = p[2 * i], v = p[2 * i + 1];
+        if (u >= V || v >= V) {
+            set_err(c, "%s edge %u: vertex id out of range (V=%u)", what, i, V);
+            return DCX_ERR_INVALID_ARG;
+        }
+        if (u == v) {
+            set_err(c, "%s edge %u: self-loop %u->%u", what, i, u, v);
+            return DCX_ERR_INVALID_ARG;
+        }
+        keys[i] = (uint64_t)u << 32 | v;
+    }
+    std::sort(keys.begin(), keys.end());
+    for (uint32_t i = 1; i < n; ++i)
+        if (keys[i] == keys[i - 1]) {
+            set_err(c, "%s batch lists %u->%u twice", what, (uint32_t)(keys[i] >> 32),
+                    (uint32_t)keys[i]);
+            return DCX_ERR_INVALID_ARG;
+        }
+    return DCX_OK;
+}
+
+// Device + pinned host staging of at least `words` uint32 words.
+void ensure_staging(dcx_ctx* c, size_t words) {
+    if (c->upd_dev_words < words) {
+        dev_free(c->upd_dev);
+        c->upd_dev_words = 0;
+        const size_t n = std::max<size_t>(words, 1 << 16);
+        dev_alloc(c, &c->upd_dev, n);
+        c->upd_dev_words = n;
+    }
+    if (c->upd_host_words < words) {
+        if (c->upd_host) (void)hipHostFree(c->upd_host);
+        c->upd_host = nullptr;
+        c->upd_host_words = 0;
+        const size_t n = std::max<size_t>(words, 1 << 16);
+        HIPX(c, hipHostMalloc(reinterpret_cast<void**>(&c->upd_host), n * 4,
+                              hipHostMallocDefault));
+        c->upd_host_words = n;
+    }
+}
+
+struct DevGroups {
+    uint32_t *key, *start, *cnt, *val;
+    uint32_t n;
+};
+
+// Grows one adjacency array to hold `need` words, keeping [0, used).
+void grow_adjacency(dcx_ctx* c, uint32_t** nbr, uint64_t* alloc, uint64_t used, uint64_t need) {
+    if (need <= *alloc) return;
+    uint64_t n = std::max<uint64_t>(need + need / 4, *alloc + *alloc / 2);
+    n = std::min<uint64_t>(n, 0xffffffffull);
+    uint32_t* q = nullptr;
+    dev_alloc(c, &q, n);
+    HIPX(c, hipMemsetAsync(q, 0xff, 4ull * n, c->stream));
+    if (used) HIPX(c, hipMemcpyAsync(q, *nbr, 4ull * used, hipMemcpyDeviceToDevice, c->stream));
+    sync(c);
+    dev_free(*nbr);
+    *nbr = q;
+    *alloc = n;
+}
+
+// Moves lists whose free room is below the batch's demand; `ovf` holds
+// deg+1 for such groups (0 otherwise).  Returns the lists moved.
+uint32_t relocate_side(dcx_ctx* c, dcx_graph* g, bool out, const Groups& G,
+                       const uint32_t* ovf) {
+    std::vector<uint32_t> rk, ro, rc;
+    uint64_t& used = out ? g->out_cap : g->in_cap;
+    uint64_t end = used;
+    for (size_t i = 0; i < G.key.size(); ++i) {
+        if (!ovf[i]) continue;
+        const uint64_t need = (uint64_t)(ovf[i] - 1) + G.cnt[i];
+        const uint64_t cap = std::max<uint64_t>(2 * need, need + 8);
+        rk.push_back(G.key[i]);
+        ro.push_back((uint32_t)end);
+        rc.push_back((uint32_t)cap);
+        end += cap;
+    }
+    if (rk.empty()) return 0;
+    if (end > 0xffffffffull) {
+        set_err(c, "adjacency array would exceed 2^32 slots");
+        throw Fail{DCX_ERR_CAPACITY};
+    }
+    if (out)
+        grow_adjacency(c, &g->out_nbr, &g->out_alloc, used, end);
+    else
+        grow_adjacency(c, &g->in_nbr, &g->in_alloc, used, end);
+    const uint32_t nr = (uint32_t)rk.size();
+    uint32_t* d = nullptr;
+    dev_alloc(c, &d, 3ull * nr);
+    try {
+        HIPX(c, hipMemcpyAsync(d, rk.data(), 4ull * nr, hipMemcpyHostToDevice, c->stream));
+        HIPX(c, hipMemcpyAsync(d + nr, ro.data(), 4ull * nr, hipMemcpyHostToDevice, c->stream));
+        HIPX(c, hipMemcpyAsync(d + 2 * nr, rc.data(), 4ull * nr, hipMemcpyHostToDevice, c->stream));
+        if (out)
+            HIPX(c, launch_relocate(g->out_off, g->out_deg, g->out_capv, g->out_nbr, d, d + nr,
+                                    d + 2 * nr, nr, c->stream));
+        else
+            HIPX(c, launch_relocate(g->in_off, g->in_deg, g->in_capv, g->in_nbr, d, d + nr,
+                                    d + 2 * nr, nr, c->stream));
+        sync(c);
+    } catch (...) {
+        dev_free(d);
+        throw;
+    }
+    dev_free(d);
+    used = end;
+    return nr;
+}
+
+int32_t batch_mvalue(dcx_ctx* c, const int32_t* kmaxes, const uint32_t* d_pairs, uint32_t n) {
+    if (!n) return -1;
+    HIPX(c, launch_fill_i32(c->d_scalars + 2, -1, 1, c->stream));
+    HIPX(c, launch_mvalue(kmaxes, d_pairs, n, c->d_scalars + 2, c->stream));
+    HIPX(c, hipMemcpyAsync(c->h_scalars + 2, c->d_scalars + 2, 4, hipMemcpyDeviceToHost,
+                           c->stream));
+    sync(c);
+    return c->h_scalars[2];
+}
+
+using Clock = std::chrono::steady_clock;
+double ms_since(Clock::time_point t) {
+    return std::chrono::duration<double, std::milli>(Clock::now() - t).count();
+}
+
+dcx_status do_update(dcx_ctx* c, dcx_graph* g, const uint32_t* ins, uint32_t n_ins,
+                     const uint32_t* del, uint32_t n_del, bool maintain, dcx_update_stats* st) {
+    const auto t0 = Clock::now();
+    dcx_update_stats S{};
+    const uint32_t V = g->V;
+    dcx_status s = validate_pairs(c, V, del, n_del, "delete");
+    if (s != DCX_OK) return s;
+    s = validate_pairs(c, V, ins, n_ins, "insert");
+    if (s != DCX_OK) return s;
+    if ((uint64_t)n_ins > g->E + (uint64_t)n_del + 0xffffffffull) return DCX_ERR_CAPACITY;
+
+    const Groups Gdo = group_pairs(del, n_del, true), Gdi = group_pairs(del, n_del, false);
+    const Groups Gio = group_pairs(ins, n_ins, true), Gii = group_pairs(ins, n_ins, false);
+
+    // ---- stage everything in one upload
+    std::vector<size_t> at;
+    size_t words = 0;
+    auto reserve = [&](size_t n) {
+        at.push_back(words);
+        words += n;
+        return at.size() - 1;
+    };
+    const size_t i_del = reserve(2ull * n_del), i_ins = reserve(2ull * n_ins);
+    const Groups* gs[4] = {&Gdo, &Gdi, &Gio, &Gii};
+    size_t i_grp[4][4];
+    for (int q = 0; q < 4; ++q) {
+        i_grp[q][0] = reserve(gs[q]->key.size());
+        i_grp[q][1] = reserve(gs[q]->start.size());
+        i_grp[q][2] = reserve(gs[q]->cnt.size());
+        i_grp[q][3] = reserve(gs[q]->val.size());
+    }
+    const size_t in_words = words;
+    const size_t i_fdel = reserve(n_del), i_fins = reserve(n_ins);
+    const size_t i_ovo = reserve(Gio.key.size()), i_ovi = reserve(Gii.key.size());
+    const size_t i_miss = reserve(1);
+    ensure_staging(c, words);
+    uint32_t* H = c->upd_host;
+    auto put = [&](size_t idx, const uint32_t* src, size_t n) {
+        if (n) std::memcpy(H + at[idx], src, 4 * n);
+    };
+    put(i_del, del, 2ull * n_del);
+    put(i_ins, ins, 2ull * n_ins);
+    for (int q = 0; q < 4; ++q) {
+        put(i_grp[q][0], gs[q]->key.data(), gs[q]->key.size());
+        put(i_grp[q][1], gs[q]->start.data(), gs[q]->start.size());
+        put(i_grp[q][2], gs[q]->cnt.data(), gs[q]->cnt.size());
+        put(i_grp[q][3], gs[q]->val.data(), gs[q]->val.size());
+    }
+    uint32_t* D = c->upd_dev;
+    auto dptr = [&](size_t idx) { return D + at[idx]; };
+    if (in_words)
+        HIPX(c, hipMemcpyAsync(D, H, 4 * in_words, hipMemcpyHostToDevice, c->stream));
+    HIPX(c, hipMemsetAsync(D + in_words, 0, 4 * (words - in_words), c->stream));
+    DevGroups dg[4];
+    for (int q = 0; q < 4; ++q)
+        dg[q] = DevGroups{dptr(i_grp[q][0]), dptr(i_grp[q][1]), dptr(i_grp[q][2]),
+                          dptr(i_grp[q][3]), (uint32_t)gs[q]->key.size()};
+
+    // ---- existence: deletions must exist, insertions must not (unless the
+    // same edge is deleted first in this batch)
+    HIPX(c, launch_find_edges(g->out_off, g->out_deg, g->out_nbr, dptr(i_del), n_del,
+                              dptr(i_fdel), c->stream));
+    HIPX(c, launch_find_edges(g->out_off, g->out_deg, g->out_nbr, dptr(i_ins), n_ins,
+                              dptr(i_fins), c->stream));
+    HIPX(c, launch_check_room(g->out_deg, g->out_capv, dg[2].key, dg[2].cnt, dg[2].n,
+                              dptr(i_ovo), c->stream));
+    HIPX(c, launch_check_room(g->in_deg, g->in_capv, dg[3].key, dg[3].cnt, dg[3].n, dptr(i_ovi),
+                              c->stream));
+    if (words > in_words)
+        HIPX(c, hipMemcpyAsync(H + in_words, D + in_words, 4 * (words - in_words),
+                               hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    for (uint32_t i = 0; i < n_del; ++i)
+        if (!H[at[i_fdel] + i]) {
+            set_err(c, "delete: edge %u->%u is not in the graph", del[2 * i], del[2 * i + 1]);
+            return DCX_ERR_NOT_FOUND;
+        }
+    if (n_ins) {
+        std::unordered_set<uint64_t> deleted;
+        for (uint32_t i = 0; i < n_del; ++i)
+            deleted.insert((uint64_t)del[2 * i] << 32 | del[2 * i + 1]);
+        for (uint32_t i = 0; i < n_ins; ++i)
+            if (H[at[i_fins] + i] &&
+                !deleted.count((uint64_t)ins[2 * i] << 32 | ins[2 * i + 1])) {
+                set_err(c, "insert: edge %u->%u is already in the graph", ins[2 * i],
+                        ins[2 * i + 1]);
+                return DCX_ERR_INVALID_ARG;
+            }
+    }
+
+    // ---- M over deletions with the old kmaxes (before anything changes)
+    const bool have_k = maintain && g->kmax_valid;
+    int32_t m_del = have_k ? batch_mvalue(c, g->kmaxes, dptr(i_del), n_del) : -1;
+    S.kmax_before = g->kmax_valid ? g->kmax : -1;
+
+    // ---- structural update
+    std::vector<uint32_t> ovo(H + at[i_ovo], H + at[i_ovo] + Gio.key.size());
+    std::vector<uint32_t> ovi(H + at[i_ovi], H + at[i_ovi] + Gii.key.size());
+    S.relocated = relocate_side(c, g, true, Gio, ovo.data());
+    S.relocated += relocate_side(c, g, false, Gii, ovi.data());
+    g->relocations += S.relocated;
+    HIPX(c, launch_apply_deletes(g->out_off, g->out_deg, g->out_nbr, dg[0].key, dg[0].start,
+                                 dg[0].cnt, dg[0].val, dg[0].n, dptr(i_miss), c->stream));
+    HIPX(c, launch_apply_deletes(g->in_off, g->in_deg, g->in_nbr, dg[1].key, dg[1].start,
+                                 dg[1].cnt, dg[1].val, dg[1].n, dptr(i_miss), c->stream));
+    HIPX(c, launch_apply_inserts(g->out_off, g->out_deg, g->out_nbr, dg[2].key, dg[2].start,
+                                 dg[2].cnt, dg[2].val, dg[2].n, c->stream));
+    HIPX(c, launch_apply_inserts(g->in_off, g->in_deg, g->in_nbr, dg[3].key, dg[3].start,
+                                 dg[3].cnt, dg[3].val, dg[3].n, c->stream));
+    HIPX(c, hipMemcpyAsync(H + at[i_miss], dptr(i_miss), 4, hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    g->E = g->E + n_ins - n_del;
+    if (H[at[i_miss]]) {
+        set_err(c, "delete: %u edges vanished between check and apply", H[at[i_miss]]);
+        throw Fail{DCX_ERR_INTERNAL};
+    }
+    S.inserted = n_ins;
+    S.deleted = n_del;
+    S.insert_ms = ms_since(t0);
+
+    // ---- bring the decomposition back
+    S.m_del = m_del;
+    S.m_ins = -1;
+    S.rows_recomputed = 0;
+    if (!have_k) {
+        g->kmax_valid = false;
+        g->rows_valid = false;
+    } else {
+        const auto t1 = Clock::now();
+        const bool rows = g->rows_valid;
+        if (n_ins == 0 && m_del >= 0)
+            compute_kmax(c, g, m_del);  // levels above M_del cannot change
+        else if (n_ins > 0)
+            compute_kmax(c, g);
+        S.m_ins = batch_mvalue(c, g->kmaxes, dptr(i_ins), n_ins);
+        S.kmax_ms = ms_since(t1);
+        const int32_t R = std::max(m_del, S.m_ins);
+        if (rows) {
+            const auto t2 = Clock::now();
+            int var = c->cfg.variant == DCX_VARIANT_AUTO ? DCX_VARIANT_BATCHED : c->cfg.variant;
+            if (R >= 0) {
+                const int32_t kr = std::min(R, g->kmax);
+                compute_rows(c, g, var, kr);
+                S.rows_recomputed = kr + 1;
+            }
+            S.rows_ms = ms_since(t2);
+        }
+    }
+    S.kmax_after = g->kmax_valid ? g->kmax : -1;
+    S.total_ms = ms_since(t0);
+    if (st) *st = S;
+    return DCX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+dcx_status dcx_update(dcx_ctx* ctx, dcx_graph* g, const uint32_t* ins, uint32_t n_ins,
+                      const uint32_t* del, uint32_t n_del, dcx_update_stats* stats) {
+    dcx_status s = check_graph(ctx, g);
+    if (s != DCX_OK) return s;
+    if ((n_ins && !ins) || (n_del && !del)) return DCX_ERR_INVALID_ARG;
+    return guarded(ctx, [&]() { return do_update(ctx, g, ins, n_ins, del, n_del, true, stats); });
+}
+
+dcx_status dcx_graph_insert_edges(dcx_ctx* ctx, dcx_graph* g, const uint32_t* pairs,
+                                  uint32_t n) {
+    dcx_status s = check_graph(ctx, g);
+    if (s != DCX_OK) return s;
+    if (n && !pairs) return DCX_ERR_INVALID_ARG;
+    return guarded(ctx,
+                   [&]() { return do_update(ctx, g, pairs, n, nullptr, 0, false, nullptr); });
+}
+
+dcx_status dcx_graph_delete_edges(dcx_ctx* ctx, dcx_graph* g, const uint32_t* pairs,
+                                  uint32_t n) {
+    dcx_status s = check_graph(ctx, g);
+    if (s != DCX_OK) return s;
+    if (n && !pairs) return DCX_ERR_INVALID_ARG;
+    return guarded(ctx,
+                   [&]() { return do_update(ctx, g, nullptr, 0, pairs, n, false, nullptr); });
+}
+
+}  // extern "C"

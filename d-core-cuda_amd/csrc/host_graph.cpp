@@ -415,3 +415,73 @@ dcx_status dcx_decompcpu_write(const char* path, uint32_t V, const int64_t* row_
 }
 
 }  // extern "C"
+
+// ---- edge batching (getKmaxEdgeBatches, main.cu:1441-1525) ------------
+
+extern "C" dcx_status dcx_kmax_edge_batches(uint32_t V, const int32_t* kmaxes,
+                                            const uint32_t* pairs, uint32_t n, int32_t mode,
+                                            uint32_t* batch_of_edge, uint32_t* n_batches) {
+    // Edge root: the endpoint with the larger kmax, the target on ties
+    // (main.cu:1449-1451); edges are bucketed by the root's kmax.
+    // mode 0 (KEDGE_BATCHING, the reference's active mode): within each
+    // bucket, in order, repeatedly take a batch of edges with distinct roots.
+    // mode 1 (KMAX_BATCHING): each batch takes the first remaining edge of
+    // successive buckets whose kmax differs from the batch's first edge by
+    // more than one.
+    if (!n_batches || (n && (!kmaxes || !pairs || !batch_of_edge)) || (mode != 0 && mode != 1))
+        return DCX_ERR_INVALID_ARG;
+    std::vector<uint32_t> root(n);
+    std::vector<int32_t> ek(n);
+    int32_t maxk = 0;
+    for (uint32_t e = 0; e < n; ++e) {
+        const uint32_t a = pairs[2 * e], b = pairs[2 * e + 1];
+        if (a >= V || b >= V) return DCX_ERR_INVALID_ARG;
+        root[e] = kmaxes[a] > kmaxes[b] ? a : b;
+        ek[e] = kmaxes[root[e]];
+        if (ek[e] < 0) return DCX_ERR_INVALID_ARG;
+        maxk = std::max(maxk, ek[e]);
+    }
+    std::vector<std::vector<uint32_t>> B(n ? (size_t)maxk + 1 : 0);
+    for (uint32_t e = 0; e < n; ++e) B[ek[e]].push_back(e);
+    uint32_t nb = 0;
+    if (mode == 0) {
+        std::vector<uint32_t> mark(V, 0xffffffffu);  // batch that used the root
+        for (auto& bucket : B) {
+            while (!bucket.empty()) {
+                std::vector<uint32_t> rest;
+                bool any = false;
+                for (uint32_t e : bucket) {
+                    if (!any || mark[root[e]] != nb) {
+                        batch_of_edge[e] = nb;
+                        mark[root[e]] = nb;
+                        any = true;
+                    } else {
+                        rest.push_back(e);
+                    }
+                }
+                bucket.swap(rest);
+                ++nb;
+            }
+        }
+    } else {
+        std::vector<size_t> head(B.size(), 0);
+        for (;;) {
+            bool any = false;
+            int32_t first_k = 0;
+            for (size_t j = 0; j < B.size(); ++j) {
+                if (head[j] >= B[j].size()) continue;
+                const uint32_t e = B[j][head[j]];
+                if (!any || std::abs(ek[e] - first_k) > 1) {
+                    if (!any) first_k = ek[e];
+                    batch_of_edge[e] = nb;
+                    ++head[j];
+                    any = true;
+                }
+            }
+            if (!any) break;
+            ++nb;
+        }
+    }
+    *n_batches = nb;
+    return DCX_OK;
+}

@@ -25,14 +25,12 @@
 //   * a pass runs either on one workgroup (batched mode: many passes in one
 //     persistent launch, workgroup barriers only) or on the whole grid
 //     (cooperative launch, grid barriers).
-#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 
 #include <climits>
 
 #include "dcx_internal.hpp"
-
-namespace cg = cooperative_groups;
+#include "grid_sync.hpp"
 
 namespace dcx {
 namespace {
@@ -153,19 +151,32 @@ __device__ __forceinline__ uint32_t slot_read(const uint32_t* p) {
 }
 
 // One full peel pass.  GRID: all workgroups of a cooperative launch work on
-// the pass and `slots` is in global memory; otherwise the calling workgroup
-// runs the whole pass and `slots` is in LDS.
+// the pass, `slots` is in global memory and `gs` is the bounded grid
+// barrier; otherwise the calling workgroup runs the whole pass and `slots`
+// is in LDS.  Logic errors (counters that do not add up) are reported into
+// *err (and abort a grid launch) instead of looping.
 template <int BLK, bool GRID>
 __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slots,
-                         TileLds<BLK>& T, int32_t* lmax_out) {
+                         TileLds<BLK>& T, int32_t* lmax_out, const GridSync* gs,
+                         uint32_t* err) {
     const uint32_t tid = threadIdx.x;
     const uint32_t wg = GRID ? blockIdx.x : 0u;
     const uint32_t nwg = GRID ? gridDim.x : 1u;
-    auto barrier = [&]() {
-        if constexpr (GRID)
-            cg::this_grid().sync();
-        else
+    auto barrier = [&]() -> bool {
+        if constexpr (GRID) {
+            return grid_barrier(*gs);
+        } else {
             __syncthreads();
+            return true;
+        }
+    };
+    auto fail = [&](uint32_t code) {
+        if (tid == 0) {
+            if constexpr (GRID)
+                sync_fail(*gs, code);
+            else
+                atomicMax(err, code);
+        }
     };
 
     if (wg == 0 && tid == 0) {
@@ -211,16 +222,23 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
             S.a0[i] = v;
         }
     }
-    barrier();
+    if (!barrier()) return;
 
     uint32_t *A = S.a0, *B = S.a1, *F = S.f0, *G = S.f1;
-    uint32_t nA = nm, removed = 0, s = 0;
+    uint32_t nA = nm, removed = 0, s = 0, levels = 0;
     int32_t L = 0;
     const int32_t k = P.k;
     // Without the membership filter the constraint can fail before any
     // removal (KList semantics on the whole graph, main.cu:144-169).
     const bool check_c = P.kmaxes == nullptr && k > 0;
     while (removed < nm) {
+        // Each removal needs at most two level iterations (a level number
+        // taken from a stale minimum seeds nothing, the next one seeds the
+        // true minimum), so more than 2*nm+2 means the counters are wrong.
+        if (++levels > 2 * nm + 2) {
+            fail(kErrInconsistent);
+            break;
+        }
         // ---- seed level L from the alive list; survivors go to B
         uint32_t* cur = slots + 4 * (s % 3);
         if (wg == 0 && tid == 0) slot_reset(slots + 4 * ((s + 1) % 3));
@@ -239,7 +257,7 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
             wave_append(keep, v, B, &cur[1]);
         }
         wave_min_to(mn, &cur[2]);
-        barrier();
+        if (!barrier()) return;
         uint32_t nF = slot_read<GRID>(&cur[0]);
         nA = slot_read<GRID>(&cur[1]);
         uint32_t mnall = slot_read<GRID>(&cur[2]);
@@ -293,7 +311,7 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
                 }
             }
             wave_min_to(mn2, &nxt[2]);
-            barrier();
+            if (!barrier()) return;
             nF = slot_read<GRID>(&nxt[0]);
             mnall = min(mnall, slot_read<GRID>(&nxt[2]));
             {
@@ -304,16 +322,24 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
             ++s;
             removed += nF;
         }
-        if (removed < nm) L = max(L + 1, (int32_t)min(mnall, (uint32_t)INT_MAX));
+        if (removed < nm) {
+            // alive vertices exist, so some degree was recorded
+            if (nA == 0 || mnall == UINT_MAX) {
+                fail(kErrInconsistent);
+                break;
+            }
+            L = max(L + 1, (int32_t)min(mnall, (uint32_t)INT_MAX));
+            if (L > P.level_cap) break;  // survivors keep lval = -1
+        }
     }
     if (wg == 0 && tid == 0 && lmax_out) *lmax_out = nm ? L : 0;
 }
 
 template <int BLK>
 __global__ __launch_bounds__(BLK) void k_pass_grid(PassSpec P, PassScratch S, uint32_t* slots,
-                                                   int32_t* lmax_out) {
+                                                   int32_t* lmax_out, GridSync gs) {
     __shared__ TileLds<BLK> T;
-    run_pass<BLK, true>(P, S, slots, T, lmax_out);
+    run_pass<BLK, true>(P, S, slots, T, lmax_out, &gs, nullptr);
 }
 
 // Persistent rows kernel: workgroup b owns scratch slot b and pulls k
@@ -322,7 +348,7 @@ template <int BLK>
 __global__ __launch_bounds__(BLK) void k_rows_batched(PassSpec base, void* scratch,
                                                       const uint32_t* vk, int32_t kmax,
                                                       int32_t* rows, int32_t* lmax_rows,
-                                                      uint32_t* qctr) {
+                                                      uint32_t* qctr, uint32_t* err) {
     __shared__ TileLds<BLK> T;
     __shared__ uint32_t slots[12];
     __shared__ int32_t s_k;
@@ -337,7 +363,7 @@ __global__ __launch_bounds__(BLK) void k_rows_batched(PassSpec base, void* scrat
         P.k = k;
         P.nmembers = vk[k];
         P.lval = rows + (size_t)k * base.V;
-        run_pass<BLK, false>(P, S, slots, T, lmax_rows + k);
+        run_pass<BLK, false>(P, S, slots, T, lmax_rows + k, nullptr, err);
         __syncthreads();
     }
 }
@@ -439,19 +465,20 @@ int pass_grid_blocks_per_cu() {
 }
 
 hipError_t launch_pass_grid(const PassSpec& p, const PassScratch& s, uint32_t* ctrs,
-                            int32_t* lmax_out, int grid, hipStream_t st) {
-    PassSpec pp = p;
-    PassScratch ss = s;
-    void* args[] = {&pp, &ss, &ctrs, &lmax_out};
-    return hipLaunchCooperativeKernel(reinterpret_cast<const void*>(k_pass_grid<kGridBlk>),
-                                      dim3(grid), dim3(kGridBlk), args, 0, st);
+                            int32_t* lmax_out, const GridSync& gs, hipStream_t st) {
+    // the sync words must start at zero for every launch
+    hipError_t e = hipMemsetAsync(gs.w, 0, kSyncResetWords * sizeof(uint32_t), st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_pass_grid<kGridBlk>, dim3(gs.nblocks), dim3(kGridBlk), 0, st, p, s, ctrs,
+                       lmax_out, gs);
+    return hipGetLastError();
 }
 
 hipError_t launch_rows_batched(const PassSpec& base, void* scratch, int slots, const uint32_t* vk,
                                int32_t kmax, int32_t* rows, int32_t* lmax_rows, uint32_t* qctr,
-                               hipStream_t st) {
+                               uint32_t* err, hipStream_t st) {
     hipLaunchKernelGGL(k_rows_batched<kRowsBlk>, dim3(slots), dim3(kRowsBlk), 0, st, base,
-                       scratch, vk, kmax, rows, lmax_rows, qctr);
+                       scratch, vk, kmax, rows, lmax_rows, qctr, err);
     return hipGetLastError();
 }
 

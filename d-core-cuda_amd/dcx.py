@@ -35,8 +35,9 @@ class Config(ctypes.Structure):
 
 class Timing(ctypes.Structure):
     _fields_ = [("kmax_ms", ctypes.c_double), ("order_ms", ctypes.c_double),
-                ("rows_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
-                ("rows_launches", ctypes.c_int32), ("reserved", ctypes.c_int32 * 7)]
+                ("rows_ms", ctypes.c_double), ("rows_kernel_ms", ctypes.c_double),
+                ("total_ms", ctypes.c_double), ("rows_launches", ctypes.c_int32),
+                ("rows_slots", ctypes.c_int32), ("reserved", ctypes.c_int32 * 6)]
 
 
 _lib = None

@@ -134,12 +134,14 @@ typedef struct dcx_config {
  * library's stream.  Mirrors the "kmax_ms lmax_ms total_ms" print of dcore
  * (main.cu:256). */
 typedef struct dcx_timing {
-    double kmax_ms;   /* in-core (kmax) pass */
-    double order_ms;  /* kmax-descending vertex order */
-    double rows_ms;   /* all l-rows */
-    double total_ms;
+    double kmax_ms;        /* in-core (kmax) pass kernel */
+    double order_ms;       /* kmax-descending vertex order */
+    double rows_ms;        /* all l-rows: clear, kernel launches, lmax readback */
+    double rows_kernel_ms; /* the rows kernel launches alone */
+    double total_ms;       /* kmax_ms + order_ms + rows_ms */
     int32_t rows_launches; /* kernel launches spent on the rows */
-    int32_t reserved[7];
+    int32_t rows_slots;    /* concurrent k-passes (batched variant) */
+    int32_t reserved[6];
 } dcx_timing;
 
 dcx_status dcx_config_init(dcx_config* cfg);
@@ -196,6 +198,51 @@ dcx_status dcx_decompose(dcx_ctx* ctx, dcx_graph* g, int32_t variant, int32_t* k
 dcx_status dcx_get_kmaxes(dcx_ctx* ctx, const dcx_graph* g, int32_t* kmaxes);
 dcx_status dcx_get_rows(dcx_ctx* ctx, const dcx_graph* g, int32_t k0, int32_t nrows,
                         int32_t* rows);
+/* ------------------------------------------------------------------ */
+/* Edge updates and maintenance.                                      */
+/* ------------------------------------------------------------------ */
+
+/* Per-batch report.  The insert/kmax/rows split mirrors the
+ * "insert: kmax: klist:" timings of maintenanceFast (main.cu:1665-1751). */
+typedef struct dcx_update_stats {
+    uint32_t inserted, deleted;
+    uint32_t relocated;      /* vertex lists moved to make room */
+    int32_t m_del;           /* max min(kmax u, kmax v) over deleted edges, old kmaxes (-1: none) */
+    int32_t m_ins;           /* same over inserted edges, new kmaxes (-1: none) */
+    int32_t rows_recomputed; /* l-rows k = 0..rows_recomputed-1 were recomputed */
+    int32_t kmax_before, kmax_after;
+    double insert_ms, kmax_ms, rows_ms, total_ms;
+    int32_t reserved[8];
+} dcx_update_stats;
+
+/* Applies a batch -- deletions first, then insertions; pairs are (u,v)
+ * interleaved like the reference's modified_edges (main.cu:351-352) -- and
+ * maintains the resident kmaxes and l-rows so that they equal a full
+ * recomputation.  Replaces maintenance / maintenanceFast (main.cu:1536-1751)
+ * with kmaxMaintenance(Delete) (main.cu:668-855).  Errors leave the graph
+ * unchanged: an id out of range, a self-loop or a pair listed twice
+ * (DCX_ERR_INVALID_ARG), an inserted edge already present
+ * (DCX_ERR_INVALID_ARG), a deleted edge absent (DCX_ERR_NOT_FOUND). */
+dcx_status dcx_update(dcx_ctx* ctx, dcx_graph* g, const uint32_t* ins, uint32_t n_ins,
+                      const uint32_t* del, uint32_t n_del, dcx_update_stats* stats);
+
+/* Structural edits only (the resident decomposition is dropped):
+ * GraphGPU::insertEdges (src/graph_gpu.cu:71-74, graphInsertEdges :49-69) /
+ * Graph::insertEdgesInPlace (src/graph.cpp:63-70), and the delete methods
+ * that the reference leaves unimplemented (src/graph.cpp:73-108,
+ * src/graph_gpu.h:18-19).  Lists that run out of free slots are moved. */
+dcx_status dcx_graph_insert_edges(dcx_ctx* ctx, dcx_graph* g, const uint32_t* pairs,
+                                  uint32_t n);
+dcx_status dcx_graph_delete_edges(dcx_ctx* ctx, dcx_graph* g, const uint32_t* pairs,
+                                  uint32_t n);
+
+/* Host-side batching of edges by root kmax (getKmaxEdgeBatches,
+ * main.cu:1441-1525): mode 0 = KEDGE_BATCHING (distinct roots per batch),
+ * mode 1 = KMAX_BATCHING.  batch_of_edge[n] receives each edge's batch. */
+dcx_status dcx_kmax_edge_batches(uint32_t V, const int32_t* kmaxes, const uint32_t* pairs,
+                                 uint32_t n, int32_t mode, uint32_t* batch_of_edge,
+                                 uint32_t* n_batches);
+
 /* Number of ragged values (sum over v of kmaxes[v]+1). */
 dcx_status dcx_ragged_size(dcx_ctx* ctx, const dcx_graph* g, uint64_t* n);
 dcx_status dcx_get_ragged(dcx_ctx* ctx, const dcx_graph* g, int64_t* row_ptr, int32_t* values);

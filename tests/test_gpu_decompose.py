@@ -17,7 +17,7 @@ from conftest import GOLDEN
 
 pytestmark = pytest.mark.gpu
 
-VARIANTS = [dcx.VARIANT_PEEL, dcx.VARIANT_BATCHED]
+VARIANTS = [dcx.VARIANT_PEEL, dcx.VARIANT_BATCHED, dcx.VARIANT_HINDEX]
 
 
 def say(msg):
@@ -118,11 +118,12 @@ def test_klist_unfiltered_and_filtered(ctx, k):
     lk, l = g.klist(k)
     assert lk == ref_lk and np.array_equal(l, ref_l)
     K, km = g.kmax()
-    say(f"klist filtered k={k}")
-    lk2, l2 = g.klist(k)
-    assert np.array_equal(l2, ref_l)
-    if k <= K:
-        assert lk2 == ref_lk
+    for var in (dcx.VARIANT_PEEL, dcx.VARIANT_HINDEX):
+        say(f"klist filtered k={k} variant={var}")
+        lk2, l2 = g.klist(k, var)
+        assert np.array_equal(l2, ref_l)
+        if k <= K:
+            assert lk2 == ref_lk
     g.free()
 
 
@@ -132,7 +133,7 @@ def test_rmat20_full(ctx):
     V = 1 << 20
     say(f"rmat20 oracle E={len(s)}")
     ref = oracle.decompose(oracle.csr_from_edges(V, s, d))
-    for var in [dcx.VARIANT_BATCHED]:
+    for var in [dcx.VARIANT_BATCHED, dcx.VARIANT_HINDEX]:
         say(f"rmat20 gpu variant={var}")
         g = ctx.upload_edges(V, s, d)
         K, lmax = g.decompose(var)
