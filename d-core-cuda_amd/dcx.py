@@ -40,6 +40,16 @@ class Timing(ctypes.Structure):
                 ("rows_slots", ctypes.c_int32), ("reserved", ctypes.c_int32 * 6)]
 
 
+class UpdateStats(ctypes.Structure):
+    _fields_ = [("inserted", ctypes.c_uint32), ("deleted", ctypes.c_uint32),
+                ("relocated", ctypes.c_uint32), ("m_del", ctypes.c_int32),
+                ("m_ins", ctypes.c_int32), ("rows_recomputed", ctypes.c_int32),
+                ("kmax_before", ctypes.c_int32), ("kmax_after", ctypes.c_int32),
+                ("insert_ms", ctypes.c_double), ("kmax_ms", ctypes.c_double),
+                ("rows_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
+                ("reserved", ctypes.c_int32 * 8)]
+
+
 _lib = None
 
 
@@ -323,6 +333,26 @@ class Graph:
               self.ctx.h)
         return rp, vals[:n.value]
 
+    def update(self, ins=None, dels=None):
+        """Maintained batch (dcx_update): deletions first, then insertions.
+        ins/dels: arrays of shape (n, 2).  Returns UpdateStats."""
+        ins = _pairs(ins)
+        dels = _pairs(dels)
+        st = UpdateStats()
+        check(lib().dcx_update(self.ctx.h, self.h, P(ins), u32(ins.size // 2), P(dels),
+                               u32(dels.size // 2), ctypes.byref(st)), "dcx_update", self.ctx.h)
+        return st
+
+    def insert_edges(self, pairs):
+        pairs = _pairs(pairs)
+        check(lib().dcx_graph_insert_edges(self.ctx.h, self.h, P(pairs), u32(pairs.size // 2)),
+              "dcx_graph_insert_edges", self.ctx.h)
+
+    def delete_edges(self, pairs):
+        pairs = _pairs(pairs)
+        check(lib().dcx_graph_delete_edges(self.ctx.h, self.h, P(pairs), u32(pairs.size // 2)),
+              "dcx_graph_delete_edges", self.ctx.h)
+
     def download_csr(self):
         oc = ctypes.c_uint64()
         ic = ctypes.c_uint64()
@@ -340,6 +370,24 @@ class Graph:
         g["out_nbr"] = g["out_nbr"][:oc.value]
         g["in_nbr"] = g["in_nbr"][:ic.value]
         return g
+
+
+def _pairs(p):
+    if p is None:
+        return np.zeros(0, np.uint32)
+    return np.ascontiguousarray(np.asarray(p, dtype=np.uint32).reshape(-1))
+
+
+def kmax_edge_batches(V, kmaxes, pairs, mode=0):
+    """getKmaxEdgeBatches restated in C (host only): batch id per edge."""
+    kmaxes = np.ascontiguousarray(kmaxes, np.int32)
+    pairs = _pairs(pairs)
+    n = pairs.size // 2
+    out = np.empty(max(n, 1), np.uint32)
+    nb = ctypes.c_uint32()
+    check(lib().dcx_kmax_edge_batches(u32(V), P(kmaxes), P(pairs), u32(n), i32(mode), P(out),
+                                      ctypes.byref(nb)), "dcx_kmax_edge_batches")
+    return out[:n], nb.value
 
 
 def compact_csr(g):
