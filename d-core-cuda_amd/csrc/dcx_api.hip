@@ -1,228 +1,59 @@
 // C-ABI entry points of the dcx library: contexts, device graphs and the
 // decomposition drivers.  Every entry point returns a dcx_status; HIP
 // failures are caught at each call and reported, never ignored.
-#include <hip/hip_runtime.h>
-
-#include <algorithm>
-#include <cstdarg>
-#include <cstdio>
-#include <cstring>
-#include <memory>
-#include <new>
-#include <vector>
-
-#include "dcx_internal.hpp"
-
-using namespace dcx;
-
-struct dcx_ctx {
-    dcx_config cfg{};
-    int device = 0;
-    int num_cus = 0;
-    int coop_grid = 0;
-    int hindex_grid = 0;
-    uint64_t timeout_ticks = 0;  // wall-clock ticks one grid barrier may take
-    hipStream_t stream = nullptr;
-    std::mutex mu;
-    std::string err;
-    dcx_timing timing{};
-    // grown-on-demand device scratch
-    void* scratch = nullptr;
-    size_t scratch_bytes = 0;
-    // small device words: [0,12) grid-pass phase slots, [16,20) grid sync
-    // words, [20,23) h-index sweep flags, [32] batched queue counter,
-    // [40] batched error word
-    uint32_t* ctrs = nullptr;
-    int32_t* d_scalars = nullptr;  // 64 words of kernel results
-    int32_t* h_scalars = nullptr;  // pinned mirror (words 32.. hold error words)
-    hipEvent_t ev[8] = {};
-    // staging for update batches (device + pinned host), grown on demand
-    uint32_t* upd_dev = nullptr;
-    size_t upd_dev_words = 0;
-    uint32_t* upd_host = nullptr;
-    size_t upd_host_words = 0;
-};
-
-namespace {
-constexpr int kCtrSync = 16, kCtrFlags = 20, kCtrQueue = 32, kCtrErr = 40;
-// Workgroups per CU for cooperative launches: far below the occupancy
-// limit, so residency of the whole grid is never in question.
-constexpr int kCoopBlocksPerCu = 2;
-constexpr double kBarrierTimeoutS = 30.0;
-}  // namespace
-
-struct dcx_graph {
-    dcx_ctx* ctx = nullptr;
-    uint32_t V = 0;
-    uint64_t E = 0;
-    // Adjacency arrays: lists start at off[v] with room for capv[v] entries;
-    // *_cap is the used prefix (a bump pointer for relocated lists), *_alloc
-    // the allocated length.  Free slots hold 0xFFFFFFFF.
-    uint64_t out_cap = 0, in_cap = 0;
-    uint64_t out_alloc = 0, in_alloc = 0;
-    uint32_t *out_off = nullptr, *in_off = nullptr, *out_nbr = nullptr, *in_nbr = nullptr;
-    int32_t *out_deg = nullptr, *in_deg = nullptr;
-    uint32_t *out_capv = nullptr, *in_capv = nullptr;
-    uint64_t relocations = 0;
-    // resident results
-    int32_t* kmaxes = nullptr;
-    int32_t kmax = -1;
-    bool kmax_valid = false;
-    uint32_t* order = nullptr;   // [V] vertices by kmax, descending
-    uint32_t* vk = nullptr;      // [kmax+2] members per k
-    uint32_t* hist = nullptr;    // [kmax+2]
-    size_t vk_cap = 0;
-    std::vector<uint32_t> h_vk;  // host copy of vk
-    int32_t* kmax_old = nullptr; // [V] previous kmaxes during an update
-    int32_t* rows = nullptr;     // (kmax+1) x V
-    size_t rows_cap = 0;         // rows allocated
-    int32_t rows_k = 0;          // rows currently held (kmax+1 when valid)
-    int32_t* lmax_rows = nullptr;
-    size_t lmax_rows_cap = 0;
-    int32_t lmax = -1;
-    bool rows_valid = false;
-};
+#include "dcx_state.hpp"
 
 namespace {
 
-struct Fail {
-    dcx_status st;
-};
-
-void set_err(dcx_ctx* c, const char* fmt, ...) {
-    char buf[512];
-    va_list ap;
-    va_start(ap, fmt);
-    std::vsnprintf(buf, sizeof buf, fmt, ap);
-    va_end(ap);
-    c->err = buf;
-}
-
-#define HIPX(ctx, call)                                                                      \
-    do {                                                                                     \
-        hipError_t e_ = (call);                                                              \
-        if (e_ != hipSuccess) {                                                              \
-            set_err((ctx), "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, \
-                    __LINE__);                                                               \
-            throw Fail{e_ == hipErrorOutOfMemory ? DCX_ERR_OUT_OF_MEMORY : DCX_ERR_HIP};     \
-        }                                                                                    \
-    } while (0)
-
-template <class F>
-dcx_status guarded(dcx_ctx* c, F&& f) {
-    if (!c) return DCX_ERR_INVALID_ARG;
-    std::lock_guard<std::mutex> lk(c->mu);
-    c->err.clear();
-    try {
-        if (hipSetDevice(c->device) != hipSuccess) {
-            set_err(c, "hipSetDevice(%d) failed", c->device);
-            return DCX_ERR_NO_DEVICE;
-        }
-        return f();
-    } catch (const Fail& x) {
-        return x.st;
-    } catch (const std::bad_alloc&) {
-        set_err(c, "host allocation failed");
-        return DCX_ERR_OUT_OF_MEMORY;
-    }
-}
-
-template <class T>
-void dev_alloc(dcx_ctx* c, T** p, size_t n) {
-    *p = nullptr;
-    if (n == 0) n = 1;
-    HIPX(c, hipMalloc(reinterpret_cast<void**>(p), n * sizeof(T)));
-}
-
-template <class T>
-void dev_free(T*& p) {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-}
-
-void ensure_scratch(dcx_ctx* c, size_t bytes) {
-    if (c->scratch_bytes >= bytes) return;
-    dev_free(c->scratch);
-    c->scratch_bytes = 0;
-    HIPX(c, hipMalloc(&c->scratch, bytes));
-    c->scratch_bytes = bytes;
-}
-
-void sync(dcx_ctx* c) { HIPX(c, hipStreamSynchronize(c->stream)); }
-
-float elapsed(dcx_ctx* c, int a, int b) {
-    float ms = 0;
-    HIPX(c, hipEventElapsedTime(&ms, c->ev[a], c->ev[b]));
-    return ms;
-}
-
+// kmax pass: the peel with in and out swapped and k = 0 (swapInOut,
+// src/common.h:114-120), whole graph, output straight into kmaxes.
 PassSpec kmax_spec(const dcx_graph* g) {
-    // swapped roles: key = in-lists, constraint = out-lists, k = 0
     PassSpec p{};
     p.V = g->V;
     p.K = Csr{g->in_off, g->in_deg, g->in_nbr};
     p.C = Csr{g->out_off, g->out_deg, g->out_nbr};
-    p.kmaxes = nullptr;
-    p.order = nullptr;
+    p.core = false;
     p.nmembers = g->V;
     p.k = 0;
     p.lval = g->kmaxes;
     return p;
 }
 
-PassSpec row_spec(const dcx_graph* g, int32_t k, int32_t* lval, bool filtered) {
+// One row on the whole graph (KList semantics, no kmaxes needed).
+PassSpec full_row_spec(const dcx_graph* g, int32_t k, int32_t* lval) {
     PassSpec p{};
     p.V = g->V;
     p.K = Csr{g->out_off, g->out_deg, g->out_nbr};
     p.C = Csr{g->in_off, g->in_deg, g->in_nbr};
-    p.kmaxes = filtered ? g->kmaxes : nullptr;
-    p.order = filtered ? g->order : nullptr;
-    p.nmembers = filtered ? (k <= g->kmax ? g->h_vk[k] : 0u) : g->V;
+    p.core = false;
+    p.nmembers = g->V;
     p.k = k;
     p.lval = lval;
     return p;
 }
 
-GridSync gsync(dcx_ctx* c, int grid) {
-    GridSync g;
-    g.w = c->ctrs + kCtrSync;
-    g.nblocks = (uint32_t)grid;
-    g.timeout_ticks = c->timeout_ticks;
-    return g;
+// One row on the core-ordered graph (needs the resident kmaxes).
+PassSpec core_row_spec(const dcx_graph* g, int32_t k, int32_t* out_row) {
+    PassSpec p{};
+    p.V = g->V;
+    p.K = Csr{g->cout_off, g->cout_deg, g->cout_nbr};
+    p.C = Csr{g->cin_off, g->cin_deg, g->cin_nbr};
+    p.core = true;
+    p.order = g->order;
+    p.nmembers = k <= g->kmax ? g->h_vk[k] : 0u;
+    p.k = k;
+    p.out_row = out_row;
+    return p;
 }
 
-// Clears the kernel error words before a series of launches.
-void clear_kernel_errors(dcx_ctx* c) {
-    HIPX(c, hipMemsetAsync(c->ctrs + kCtrSync + kSyncError, 0, sizeof(uint32_t), c->stream));
-    HIPX(c, hipMemsetAsync(c->ctrs + kCtrErr, 0, sizeof(uint32_t), c->stream));
-}
-
-// Reads the kernel error words (synchronises) and turns them into a status.
-void check_kernel_errors(dcx_ctx* c, const char* what) {
-    HIPX(c, hipMemcpyAsync(c->h_scalars + 32, c->ctrs + kCtrSync + kSyncError, 4,
-                           hipMemcpyDeviceToHost, c->stream));
-    HIPX(c, hipMemcpyAsync(c->h_scalars + 33, c->ctrs + kCtrErr, 4, hipMemcpyDeviceToHost,
-                           c->stream));
-    sync(c);
-    const int32_t e = std::max(c->h_scalars[32], c->h_scalars[33]);
-    if (e == (int32_t)kErrTimeout) {
-        set_err(c, "%s: a grid barrier timed out (workgroups not co-resident?)", what);
-        throw Fail{DCX_ERR_TIMEOUT};
-    }
-    if (e != 0) {
-        set_err(c, "%s: kernel counters inconsistent (code %d)", what, e);
-        throw Fail{DCX_ERR_INTERNAL};
-    }
-}
-
-// Runs one pass on the whole grid (cooperative launch); returns lmax of it.
-// Events ev[a], ev[b] (if a >= 0) bracket the kernel.
+// Runs one pass on the whole grid; returns its last level.  Events ev[a],
+// ev[b] (if a >= 0) bracket the kernel.
 int32_t run_grid_pass(dcx_ctx* c, const PassSpec& p, int a = -1, int b = -1) {
     ensure_scratch(c, (size_t)kScratchArrays * std::max<uint32_t>(p.V, 1) * 4);
-    PassScratch s = scratch_slot(c->scratch, p.V, 0);
+    PassScratch s = scratch_at(c->scratch, std::max<uint32_t>(p.V, 1));
     clear_kernel_errors(c);
     if (a >= 0) HIPX(c, hipEventRecord(c->ev[a], c->stream));
-    HIPX(c, launch_pass_grid(p, s, c->ctrs, c->d_scalars, gsync(c, c->coop_grid), c->stream));
+    HIPX(c, launch_pass_grid(p, s, c->ctrs, c->d_scalars, gsync(c, c->grid_blocks), c->stream));
     if (b >= 0) HIPX(c, hipEventRecord(c->ev[b], c->stream));
     HIPX(c, hipMemcpyAsync(c->h_scalars, c->d_scalars, sizeof(int32_t), hipMemcpyDeviceToHost,
                            c->stream));
@@ -230,11 +61,39 @@ int32_t run_grid_pass(dcx_ctx* c, const PassSpec& p, int a = -1, int b = -1) {
     return c->h_scalars[0];
 }
 
-// In-core numbers.  cap >= 0 (valid resident kmaxes required): only levels
-// 0..cap are peeled and vertices left alive keep max(old, cap+1) -- exact
-// when no (k,0)-core with k > cap can have changed.
+// Relabels by kmax and sorts every list (see core_graph.hip).
+void build_core(dcx_ctx* c, dcx_graph* g) {
+    const uint32_t V = g->V;
+    if (!g->rank) {
+        dev_alloc(c, &g->rank, V);
+        dev_alloc(c, &g->cout_off, (size_t)V + 1);
+        dev_alloc(c, &g->cin_off, (size_t)V + 1);
+        dev_alloc(c, &g->cout_deg, V);
+        dev_alloc(c, &g->cin_deg, V);
+    }
+    if (g->core_e_cap < g->E || !g->cout_nbr) {
+        dev_free(g->cout_nbr);
+        dev_free(g->cin_nbr);
+        const size_t cap = g->E + g->E / 8 + 1024;
+        dev_alloc(c, &g->cout_nbr, cap);
+        dev_alloc(c, &g->cin_nbr, cap);
+        g->core_e_cap = cap;
+    }
+    ensure_bytes(c, &c->tmp, &c->tmp_bytes, core_graph_tmp_bytes(V, g->E));
+    HIPX(c, build_core_graph(V, g->E, g->order, Csr{g->out_off, g->out_deg, g->out_nbr},
+                             Csr{g->in_off, g->in_deg, g->in_nbr}, g->rank, g->cout_off,
+                             g->cout_deg, g->cout_nbr, g->cin_off, g->cin_deg, g->cin_nbr, c->tmp,
+                             c->stream));
+    g->core_valid = true;
+}
+
+// In-core numbers, then the kmax order, V_k, E_k and the core-ordered
+// graph.  cap >= 0 (valid resident kmaxes required): only levels 0..cap
+// are peeled and vertices left alive keep max(old, cap+1) -- exact when no
+// (k,0)-core with k > cap can have changed.
 void compute_kmax(dcx_ctx* c, dcx_graph* g, int32_t cap = -1) {
     if (!g->kmaxes) dev_alloc(c, &g->kmaxes, g->V);
+    g->core_valid = false;
     if (g->V == 0) {
         g->kmax = -1;
         g->kmax_valid = true;
@@ -258,22 +117,35 @@ void compute_kmax(dcx_ctx* c, dcx_graph* g, int32_t cap = -1) {
         g->kmax = run_grid_pass(c, kmax_spec(g), 0, 1);
     }
     HIPX(c, hipEventRecord(c->ev[7], c->stream));
-    // vertex order by kmax, members per k
-    size_t need = (size_t)g->kmax + 2;
+    const size_t need = (size_t)g->kmax + 2;
     if (g->vk_cap < need) {
         dev_free(g->vk);
         dev_free(g->hist);
+        dev_free(g->ekh);
         dev_alloc(c, &g->vk, need);
         dev_alloc(c, &g->hist, need);
+        dev_alloc(c, &g->ekh, need);
         g->vk_cap = need;
     }
     if (!g->order) dev_alloc(c, &g->order, g->V);
     HIPX(c, launch_order_by_kmax(g->kmaxes, g->V, g->kmax, g->hist, g->vk, g->order, c->stream));
+    HIPX(c, launch_core_edges(g->kmaxes, Csr{g->out_off, g->out_deg, g->out_nbr}, g->V, g->kmax,
+                              g->ekh, c->stream));
+    build_core(c, g);
     g->h_vk.assign(need, 0);
+    std::vector<uint64_t> eh(need, 0);
     HIPX(c, hipMemcpyAsync(g->h_vk.data(), g->vk, need * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                           c->stream));
+    HIPX(c, hipMemcpyAsync(eh.data(), g->ekh, need * sizeof(uint64_t), hipMemcpyDeviceToHost,
                            c->stream));
     HIPX(c, hipEventRecord(c->ev[2], c->stream));
     sync(c);
+    g->h_ek.assign((size_t)g->kmax + 1, 0);
+    uint64_t acc = 0;
+    for (int32_t k = g->kmax; k >= 0; --k) {
+        acc += eh[k];
+        g->h_ek[k] = acc;
+    }
     c->timing.kmax_ms = elapsed(c, 0, 1);
     c->timing.order_ms = elapsed(c, 7, 2);
     g->kmax_valid = true;
@@ -291,6 +163,89 @@ void grow_keep(dcx_ctx* c, T** p, size_t* cap, size_t n, size_t keep) {
     dev_free(*p);
     *p = q;
     *cap = n;
+}
+
+// Static schedule of the passes k = 0..kr onto teams of workgroups: a pass
+// whose work (2 E_k + V_k) exceeds the per-workgroup share gets its own
+// team sized to it (up to kTeamMax, using at most 3/4 of the grid); the
+// remaining passes go to single-workgroup teams, longest first, each to the
+// least loaded team.
+struct Sched {
+    std::vector<TeamDesc> teams;
+    std::vector<uint32_t> block_team;
+    std::vector<int32_t> ks;
+    uint64_t scratch_words = 0;
+    uint32_t sync_stride = 0;
+};
+constexpr uint32_t kTeamMax = 32;
+constexpr int kRowsBlocksPerCu = 2;
+
+Sched make_schedule(const dcx_graph* g, int32_t kr, uint32_t G) {
+    const uint32_t np = (uint32_t)kr + 1;
+    std::vector<double> w(np);
+    double W = 0;
+    for (uint32_t k = 0; k < np; ++k) {
+        w[k] = 2.0 * (double)g->h_ek[k] + (double)g->h_vk[k] + 64.0;
+        W += w[k];
+    }
+    const double share = W / G;
+    std::vector<uint32_t> idx(np);
+    for (uint32_t k = 0; k < np; ++k) idx[k] = k;
+    std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return w[a] > w[b]; });
+    Sched s;
+    s.sync_stride = sync_words(kTeamMax);
+    const uint32_t big_budget = G * 3 / 4;
+    uint32_t used = 0;
+    std::vector<uint32_t> small;
+    for (uint32_t k : idx) {
+        const uint32_t t = (uint32_t)std::min<double>(kTeamMax, w[k] / share);
+        if (t >= 2 && used + t <= big_budget) {
+            TeamDesc td{};
+            td.first_block = used;
+            td.size = t;
+            td.pass_begin = (uint32_t)s.ks.size();
+            s.ks.push_back((int32_t)k);
+            td.pass_end = (uint32_t)s.ks.size();
+            td.scratch_cap = std::max<uint32_t>(g->h_vk[k], 1);
+            s.teams.push_back(td);
+            used += t;
+        } else {
+            small.push_back(k);
+        }
+    }
+    const uint32_t nsingle = std::min<uint32_t>(G - used, (uint32_t)small.size());
+    std::vector<std::vector<int32_t>> lists(nsingle);
+    std::vector<double> load(nsingle, 0.0);
+    for (uint32_t k : small) {  // already longest first
+        const uint32_t t = (uint32_t)(std::min_element(load.begin(), load.end()) - load.begin());
+        lists[t].push_back((int32_t)k);
+        load[t] += w[k];
+    }
+    for (uint32_t t = 0; t < nsingle; ++t) {
+        TeamDesc td{};
+        td.first_block = used + t;
+        td.size = 1;
+        td.pass_begin = (uint32_t)s.ks.size();
+        uint32_t capv = 1;
+        for (int32_t k : lists[t]) {
+            s.ks.push_back(k);
+            capv = std::max<uint32_t>(capv, g->h_vk[k]);
+        }
+        td.pass_end = (uint32_t)s.ks.size();
+        td.scratch_cap = capv;
+        s.teams.push_back(td);
+    }
+    uint64_t off = 0;
+    for (uint32_t t = 0; t < s.teams.size(); ++t) {
+        TeamDesc& td = s.teams[t];
+        td.scratch_off = off;
+        off += (uint64_t)kScratchArrays * td.scratch_cap;
+        td.sync_off = t * s.sync_stride;
+        td.slot_off = t * 12;
+        for (uint32_t b = 0; b < td.size; ++b) s.block_team.push_back(t);
+    }
+    s.scratch_words = off;
+    return s;
 }
 
 // Computes rows k = 0..kr (kr < 0: all rows 0..kmax) with the given
@@ -313,56 +268,80 @@ void compute_rows(dcx_ctx* c, dcx_graph* g, int variant, int32_t kr = -1) {
     grow_keep(c, &g->rows, &g->rows_cap, nrows * V, keep_rows * V);
     grow_keep(c, &g->lmax_rows, &g->lmax_rows_cap, nrows, keep_rows);
     const size_t nre = (size_t)kr + 1;
+    uint32_t nteam_err = 0, team_err_stride = 0;
+    const uint32_t* team_err = nullptr;
     clear_kernel_errors(c);
     HIPX(c, hipEventRecord(c->ev[3], c->stream));
     HIPX(c, hipMemsetAsync(g->rows, 0, nre * V * sizeof(int32_t), c->stream));
     int launches = 0, slots_used = 1;
-    HIPX(c, hipEventRecord(c->ev[5], c->stream));
     if (variant == DCX_VARIANT_HINDEX) {
         // rows in k order, each started from the row above (l_k <= l_{k-1})
+        ensure_scratch(c, 2ull * kScratchArrays * V * 4);
+        HIPX(c, hipEventRecord(c->ev[5], c->stream));
         for (int32_t k = 0; k <= kr; ++k) {
-            PassSpec p = row_spec(g, k, g->rows + (size_t)k * V, true);
-            HIPX(c, launch_hindex_row(p, k ? g->rows + (size_t)(k - 1) * V : nullptr,
-                                      c->ctrs + kCtrFlags, g->lmax_rows + k, c->d_scalars + 8,
-                                      gsync(c, c->hindex_grid), c->stream));
+            PassScratch s = scratch_at(static_cast<uint32_t*>(c->scratch) +
+                                           (size_t)(k & 1) * kScratchArrays * V,
+                                       V);
+            PassScratch prev = scratch_at(static_cast<uint32_t*>(c->scratch) +
+                                              (size_t)((k + 1) & 1) * kScratchArrays * V,
+                                          V);
+            PassSpec p = core_row_spec(g, k, g->rows + (size_t)k * V);
+            HIPX(c, launch_hindex_row(p, s, k ? prev.lv : nullptr, c->ctrs + kCtrFlags,
+                                      g->lmax_rows + k, gsync(c, c->hindex_grid), c->stream));
             ++launches;
         }
     } else if (variant == DCX_VARIANT_PEEL) {
         ensure_scratch(c, (size_t)kScratchArrays * V * 4);
+        HIPX(c, hipEventRecord(c->ev[5], c->stream));
         for (int32_t k = 0; k <= kr; ++k) {
-            PassSpec p = row_spec(g, k, g->rows + (size_t)k * V, true);
-            PassScratch s = scratch_slot(c->scratch, V, 0);
-            HIPX(c, launch_pass_grid(p, s, c->ctrs, g->lmax_rows + k, gsync(c, c->coop_grid),
-                                     c->stream));
+            PassSpec p = core_row_spec(g, k, g->rows + (size_t)k * V);
+            HIPX(c, launch_pass_grid(p, scratch_at(c->scratch, V), c->ctrs, g->lmax_rows + k,
+                                     gsync(c, c->grid_blocks), c->stream));
             ++launches;
         }
     } else {
-        // batched: one persistent launch, one workgroup per pass
-        const size_t per_slot = (size_t)kScratchArrays * V * 4;
-        size_t budget = c->cfg.scratch_budget;
-        if (budget == 0) {
-            size_t fr = 0, tot = 0;
-            HIPX(c, hipMemGetInfo(&fr, &tot));
-            budget = std::min<size_t>(fr / 2, (size_t)64 << 30);
+        // batched: one persistent launch, teams of workgroups per pass
+        const uint32_t G = (uint32_t)(c->cfg.rows_slots > 0 ? c->cfg.rows_slots
+                                                              : c->num_cus * kRowsBlocksPerCu);
+        Sched s = make_schedule(g, kr, G);
+        const size_t nt = s.teams.size(), nb = s.block_team.size(), nk = s.ks.size();
+        const size_t bytes = nt * sizeof(TeamDesc) + 4 * nb + 4 * nk + 64;
+        ensure_bytes(c, &c->sched, &c->sched_bytes, bytes);
+        char* base = static_cast<char*>(c->sched);
+        TeamDesc* d_teams = reinterpret_cast<TeamDesc*>(base);
+        uint32_t* d_bt = reinterpret_cast<uint32_t*>(base + nt * sizeof(TeamDesc));
+        int32_t* d_ks = reinterpret_cast<int32_t*>(d_bt + nb);
+        std::vector<char> host(bytes);
+        std::memcpy(host.data(), s.teams.data(), nt * sizeof(TeamDesc));
+        std::memcpy(host.data() + nt * sizeof(TeamDesc), s.block_team.data(), 4 * nb);
+        std::memcpy(host.data() + nt * sizeof(TeamDesc) + 4 * nb, s.ks.data(), 4 * nk);
+        HIPX(c, hipMemcpyAsync(c->sched, host.data(), bytes, hipMemcpyHostToDevice, c->stream));
+        const size_t pool_words = nt * (s.sync_stride + 12);
+        if (c->pools_words < pool_words) {
+            dev_free(c->pools);
+            dev_alloc(c, &c->pools, pool_words);
+            c->pools_words = pool_words;
         }
-        int slots = c->cfg.rows_slots > 0 ? c->cfg.rows_slots : c->num_cus * 2;
-        slots = (int)std::min<size_t>((size_t)slots, nre);
-        slots = (int)std::min<size_t>((size_t)slots, std::max<size_t>(budget / per_slot, 1));
-        ensure_scratch(c, per_slot * slots);
-        HIPX(c, hipMemsetAsync(c->ctrs + kCtrQueue, 0, sizeof(uint32_t), c->stream));
-        PassSpec base = row_spec(g, 0, g->rows, true);
+        HIPX(c, hipMemsetAsync(c->pools, 0, 4 * pool_words, c->stream));
+        ensure_scratch(c, 4 * s.scratch_words);
+        PassSpec basep = core_row_spec(g, 0, g->rows);
         HIPX(c, hipEventRecord(c->ev[5], c->stream));
-        HIPX(c, launch_rows_batched(base, c->scratch, slots, g->vk, kr, g->rows, g->lmax_rows,
-                                    c->ctrs + kCtrQueue, c->ctrs + kCtrErr, c->stream));
+        HIPX(c, launch_rows_teams(basep, g->vk, g->rows, g->lmax_rows, d_teams, d_bt, d_ks,
+                                  (uint32_t)nb, c->scratch, c->pools,
+                                  c->pools + nt * s.sync_stride, c->timeout_ticks,
+                                  c->ctrs + kCtrErr, c->stream));
         launches = 1;
-        slots_used = slots;
+        slots_used = (int)nb;
+        team_err = c->pools + kSyncError;
+        nteam_err = (uint32_t)nt;
+        team_err_stride = s.sync_stride;
     }
     HIPX(c, hipEventRecord(c->ev[6], c->stream));
     std::vector<int32_t> lm(nrows);
     HIPX(c, hipMemcpyAsync(lm.data(), g->lmax_rows, nrows * sizeof(int32_t),
                            hipMemcpyDeviceToHost, c->stream));
     HIPX(c, hipEventRecord(c->ev[4], c->stream));
-    check_kernel_errors(c, "l-rows");
+    check_kernel_errors(c, "l-rows", team_err, nteam_err, team_err_stride);
     c->timing.rows_ms = elapsed(c, 3, 4);
     c->timing.rows_kernel_ms = elapsed(c, 5, 6);
     c->timing.rows_launches = launches;
@@ -382,13 +361,62 @@ __global__ void k_ragged_gather(const int32_t* rows, const int32_t* kmaxes, cons
     }
 }
 
-dcx_status check_graph(dcx_ctx* ctx, const dcx_graph* g) {
-    if (!ctx || !g) return DCX_ERR_INVALID_ARG;
-    if (g->ctx != ctx) {
-        set_err(ctx, "graph belongs to another context");
-        return DCX_ERR_INVALID_ARG;
+// Per-vertex capacities of one direction, summed (the length of a repacked
+// adjacency array).
+uint64_t capacity_sum(dcx_ctx* c, const uint32_t* capv, uint32_t V) {
+    std::vector<uint32_t> cv(V);
+    if (V) HIPX(c, hipMemcpy(cv.data(), capv, 4ull * V, hipMemcpyDeviceToHost));
+    uint64_t t = 0;
+    for (uint32_t x : cv) t += x;
+    return t;
+}
+
+// Device lists of one direction -> reference layout (off[v+1]-off[v] =
+// capacity of v, lists in place, free slots 0xFFFFFFFF).
+void download_side(dcx_ctx* c, uint32_t V, const uint32_t* d_off, const int32_t* d_deg,
+                   const uint32_t* d_capv, const uint32_t* d_nbr, uint64_t used, uint32_t* off,
+                   int32_t* deg, uint32_t* nbr) {
+    std::vector<uint32_t> o(V + 1), cv(V), all(used ? used : 1);
+    HIPX(c, hipMemcpy(o.data(), d_off, 4ull * (V + 1), hipMemcpyDeviceToHost));
+    if (V) {
+        HIPX(c, hipMemcpy(deg, d_deg, 4ull * V, hipMemcpyDeviceToHost));
+        HIPX(c, hipMemcpy(cv.data(), d_capv, 4ull * V, hipMemcpyDeviceToHost));
     }
-    return DCX_OK;
+    if (used) HIPX(c, hipMemcpy(all.data(), d_nbr, 4ull * used, hipMemcpyDeviceToHost));
+    uint64_t pos = 0;
+    for (uint32_t v = 0; v < V; ++v) {
+        off[v] = (uint32_t)pos;
+        for (uint32_t j = 0; j < cv[v]; ++j)
+            nbr[pos + j] = j < (uint32_t)deg[v] ? all[o[v] + j] : 0xffffffffu;
+        pos += cv[v];
+    }
+    off[V] = (uint32_t)pos;
+}
+
+void free_graph_arrays(dcx_graph* g) {
+    dev_free(g->out_off);
+    dev_free(g->in_off);
+    dev_free(g->out_deg);
+    dev_free(g->in_deg);
+    dev_free(g->out_nbr);
+    dev_free(g->in_nbr);
+    dev_free(g->out_capv);
+    dev_free(g->in_capv);
+    dev_free(g->kmax_old);
+    dev_free(g->kmaxes);
+    dev_free(g->order);
+    dev_free(g->vk);
+    dev_free(g->hist);
+    dev_free(g->ekh);
+    dev_free(g->rank);
+    dev_free(g->cout_off);
+    dev_free(g->cin_off);
+    dev_free(g->cout_deg);
+    dev_free(g->cin_deg);
+    dev_free(g->cout_nbr);
+    dev_free(g->cin_nbr);
+    dev_free(g->rows);
+    dev_free(g->lmax_rows);
 }
 
 }  // namespace
@@ -447,10 +475,11 @@ dcx_status dcx_ctx_create(const dcx_config* cfg, dcx_ctx** out) {
     if (hipSetDevice(c->device) != hipSuccess) return DCX_ERR_NO_DEVICE;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, c->device) != hipSuccess) return DCX_ERR_HIP;
-    if (!prop.cooperativeLaunch) return DCX_ERR_NO_DEVICE;
     c->num_cus = prop.multiProcessorCount;
-    c->coop_grid = c->num_cus * std::min(kCoopBlocksPerCu, pass_grid_blocks_per_cu());
-    c->hindex_grid = c->num_cus * std::min(kCoopBlocksPerCu, hindex_blocks_per_cu());
+    c->grid_blocks = (int)std::min<uint32_t>(
+        kMaxGridBlocks, c->num_cus * std::min(kGridBlocksPerCu, pass_grid_blocks_per_cu()));
+    c->hindex_grid = (int)std::min<uint32_t>(
+        kMaxGridBlocks, c->num_cus * std::min(kGridBlocksPerCu, hindex_blocks_per_cu()));
     int clock_khz = 0;
     if (hipDeviceGetAttribute(&clock_khz, hipDeviceAttributeWallClockRate, c->device) !=
             hipSuccess ||
@@ -461,7 +490,9 @@ dcx_status dcx_ctx_create(const dcx_config* cfg, dcx_ctx** out) {
         return DCX_ERR_HIP;
     for (auto& e : c->ev)
         if (hipEventCreate(&e) != hipSuccess) return DCX_ERR_HIP;
-    if (hipMalloc(&c->ctrs, 64 * sizeof(uint32_t)) != hipSuccess) return DCX_ERR_OUT_OF_MEMORY;
+    if (hipMalloc(&c->ctrs, kCtrWords * sizeof(uint32_t)) != hipSuccess)
+        return DCX_ERR_OUT_OF_MEMORY;
+    if (hipMemset(c->ctrs, 0, kCtrWords * sizeof(uint32_t)) != hipSuccess) return DCX_ERR_HIP;
     if (hipMalloc(&c->d_scalars, 64 * sizeof(int32_t)) != hipSuccess)
         return DCX_ERR_OUT_OF_MEMORY;
     if (hipHostMalloc(&c->h_scalars, 64 * sizeof(int32_t), hipHostMallocDefault) != hipSuccess)
@@ -475,6 +506,9 @@ dcx_status dcx_ctx_destroy(dcx_ctx* c) {
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     dev_free(c->scratch);
+    dev_free(c->tmp);
+    dev_free(c->sched);
+    dev_free(c->pools);
     dev_free(c->ctrs);
     dev_free(c->d_scalars);
     dev_free(c->upd_dev);
@@ -520,13 +554,13 @@ dcx_status dcx_graph_upload_csr(dcx_ctx* ctx, uint32_t V, uint64_t E, const uint
             so += (uint32_t)out_deg[v];
             si += (uint32_t)in_deg[v];
             for (int32_t j = 0; j < out_deg[v]; ++j)
-                if (out_nbr[out_off[v] + j] >= V) {
-                    set_err(ctx, "out-neighbour id out of range at vertex %u", v);
+                if (out_nbr[out_off[v] + j] >= V || out_nbr[out_off[v] + j] == v) {
+                    set_err(ctx, "out-neighbour out of range or self-loop at vertex %u", v);
                     return DCX_ERR_INVALID_ARG;
                 }
             for (int32_t j = 0; j < in_deg[v]; ++j)
-                if (in_nbr[in_off[v] + j] >= V) {
-                    set_err(ctx, "in-neighbour id out of range at vertex %u", v);
+                if (in_nbr[in_off[v] + j] >= V || in_nbr[in_off[v] + j] == v) {
+                    set_err(ctx, "in-neighbour out of range or self-loop at vertex %u", v);
                     return DCX_ERR_INVALID_ARG;
                 }
         }
@@ -581,14 +615,7 @@ dcx_status dcx_graph_upload_csr(dcx_ctx* ctx, uint32_t V, uint64_t E, const uint
             sync(ctx);
         } catch (...) {
             dcx_graph* raw = g.release();
-            dev_free(raw->out_off);
-            dev_free(raw->in_off);
-            dev_free(raw->out_deg);
-            dev_free(raw->in_deg);
-            dev_free(raw->out_capv);
-            dev_free(raw->in_capv);
-            dev_free(raw->out_nbr);
-            dev_free(raw->in_nbr);
+            free_graph_arrays(raw);
             delete raw;
             throw;
         }
@@ -617,21 +644,7 @@ dcx_status dcx_graph_free(dcx_graph* g) {
     std::lock_guard<std::mutex> lk(c->mu);
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
-    dev_free(g->out_off);
-    dev_free(g->in_off);
-    dev_free(g->out_deg);
-    dev_free(g->in_deg);
-    dev_free(g->out_nbr);
-    dev_free(g->in_nbr);
-    dev_free(g->out_capv);
-    dev_free(g->in_capv);
-    dev_free(g->kmax_old);
-    dev_free(g->kmaxes);
-    dev_free(g->order);
-    dev_free(g->vk);
-    dev_free(g->hist);
-    dev_free(g->rows);
-    dev_free(g->lmax_rows);
+    free_graph_arrays(g);
     delete g;
     return DCX_OK;
 }
@@ -645,40 +658,6 @@ dcx_status dcx_graph_info(const dcx_graph* g, uint32_t* V, uint64_t* E, int32_t*
     if (lmax) *lmax = g->rows_valid ? g->lmax : -1;
     return DCX_OK;
 }
-
-namespace {
-// Per-vertex capacities of one direction, summed (the length of a repacked
-// adjacency array).
-uint64_t capacity_sum(dcx_ctx* c, const uint32_t* capv, uint32_t V) {
-    std::vector<uint32_t> cv(V);
-    if (V) HIPX(c, hipMemcpy(cv.data(), capv, 4ull * V, hipMemcpyDeviceToHost));
-    uint64_t t = 0;
-    for (uint32_t x : cv) t += x;
-    return t;
-}
-
-// Device lists of one direction -> reference layout (off[v+1]-off[v] =
-// capacity of v, lists in place, free slots 0xFFFFFFFF).
-void download_side(dcx_ctx* c, uint32_t V, const uint32_t* d_off, const int32_t* d_deg,
-                   const uint32_t* d_capv, const uint32_t* d_nbr, uint64_t used, uint32_t* off,
-                   int32_t* deg, uint32_t* nbr) {
-    std::vector<uint32_t> o(V + 1), cv(V), all(used ? used : 1);
-    HIPX(c, hipMemcpy(o.data(), d_off, 4ull * (V + 1), hipMemcpyDeviceToHost));
-    if (V) {
-        HIPX(c, hipMemcpy(deg, d_deg, 4ull * V, hipMemcpyDeviceToHost));
-        HIPX(c, hipMemcpy(cv.data(), d_capv, 4ull * V, hipMemcpyDeviceToHost));
-    }
-    if (used) HIPX(c, hipMemcpy(all.data(), d_nbr, 4ull * used, hipMemcpyDeviceToHost));
-    uint64_t pos = 0;
-    for (uint32_t v = 0; v < V; ++v) {
-        off[v] = (uint32_t)pos;
-        for (uint32_t j = 0; j < cv[v]; ++j)
-            nbr[pos + j] = j < (uint32_t)deg[v] ? all[o[v] + j] : 0xffffffffu;
-        pos += cv[v];
-    }
-    off[V] = (uint32_t)pos;
-}
-}  // namespace
 
 dcx_status dcx_graph_capacity(const dcx_graph* g, uint64_t* out_cap, uint64_t* in_cap) {
     if (!g) return DCX_ERR_INVALID_ARG;
@@ -740,26 +719,30 @@ dcx_status dcx_klist(dcx_ctx* ctx, dcx_graph* g, int32_t k, int32_t variant, int
         dev_alloc(ctx, &row, V);
         int32_t lk = 0;
         try {
-            // with resident kmaxes the pass is restricted to the (k,0)-core;
+            // with resident kmaxes the pass runs on the core-ordered graph;
             // otherwise it runs on the whole graph exactly like KList.  The
             // h-index fixpoint needs the core, so it computes kmaxes first.
             if (variant == DCX_VARIANT_HINDEX && !g->kmax_valid) compute_kmax(ctx, g);
-            const bool filtered = g->kmax_valid;
-            if (filtered) HIPX(ctx, hipMemsetAsync(row, 0, 4ull * V, ctx->stream));
-            PassSpec p = row_spec(g, k, row, filtered);
-            if (filtered && p.nmembers == 0) {
-                lk = 0;
-            } else if (variant == DCX_VARIANT_HINDEX) {
-                clear_kernel_errors(ctx);
-                HIPX(ctx, launch_hindex_row(p, nullptr, ctx->ctrs + kCtrFlags, ctx->d_scalars,
-                                            ctx->d_scalars + 8, gsync(ctx, ctx->hindex_grid),
-                                            ctx->stream));
-                HIPX(ctx, hipMemcpyAsync(ctx->h_scalars, ctx->d_scalars, 4,
-                                         hipMemcpyDeviceToHost, ctx->stream));
-                check_kernel_errors(ctx, "h-index row");
-                lk = ctx->h_scalars[0];
+            if (g->kmax_valid && g->core_valid) {
+                HIPX(ctx, hipMemsetAsync(row, 0, 4ull * V, ctx->stream));
+                PassSpec p = core_row_spec(g, k, row);
+                if (p.nmembers == 0) {
+                    lk = 0;
+                } else if (variant == DCX_VARIANT_HINDEX) {
+                    ensure_scratch(ctx, (size_t)kScratchArrays * V * 4);
+                    clear_kernel_errors(ctx);
+                    HIPX(ctx, launch_hindex_row(p, scratch_at(ctx->scratch, V), nullptr,
+                                                ctx->ctrs + kCtrFlags, ctx->d_scalars,
+                                                gsync(ctx, ctx->hindex_grid), ctx->stream));
+                    HIPX(ctx, hipMemcpyAsync(ctx->h_scalars, ctx->d_scalars, 4,
+                                             hipMemcpyDeviceToHost, ctx->stream));
+                    check_kernel_errors(ctx, "h-index row");
+                    lk = ctx->h_scalars[0];
+                } else {
+                    lk = run_grid_pass(ctx, p);
+                }
             } else {
-                lk = run_grid_pass(ctx, p);
+                lk = run_grid_pass(ctx, full_row_spec(g, k, row));
             }
             if (lvals) HIPX(ctx, hipMemcpy(lvals, row, 4ull * V, hipMemcpyDeviceToHost));
         } catch (...) {
@@ -785,6 +768,7 @@ dcx_status dcx_decompose(dcx_ctx* ctx, dcx_graph* g, int32_t variant, int32_t* k
     }
     return guarded(ctx, [&]() -> dcx_status {
         ctx->timing = dcx_timing{};
+        g->rows_valid = false;
         compute_kmax(ctx, g);
         compute_rows(ctx, g, variant);
         ctx->timing.total_ms = ctx->timing.kmax_ms + ctx->timing.order_ms + ctx->timing.rows_ms;

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <cstdint>
 #include <mutex>
 #include <string>
@@ -12,9 +13,8 @@
 
 namespace dcx {
 
-// One adjacency direction in device memory, reference layout
-// (src/common.h:81-91): off[V+1] are capacity offsets (degree + slack),
-// deg[V] the used prefix of each list.
+// One adjacency direction in device memory: the list of v starts at off[v]
+// and holds deg[v] entries.
 struct Csr {
     const uint32_t* off;
     const int32_t* deg;
@@ -28,65 +28,112 @@ struct Csr {
 //     its C-neighbours are alive (in-lists for an l-row).
 // The kmax pass is the same with the two directions swapped and k = 0
 // (swapInOut, src/common.h:114-120).
+//
+// Two layouts:
+//   full (core == false): original ids, whole lists, every vertex is a
+//     member; lval is the output array itself (old ids).
+//   core (core == true): ids are ranks in the kmax-descending order, lists
+//     sorted by rank, members are [0, nmembers) and the members inside a
+//     member's list are its prefix; lval is scratch in rank space and the
+//     pass ends by writing out_row[order[i]] = lval[i].
 struct PassSpec {
     uint32_t V;
     Csr K, C;
-    const int32_t* kmaxes;  // membership filter kmaxes[v] >= k (nullptr: all)
-    const uint32_t* order;  // members first (nullptr: identity order)
-    uint32_t nmembers;      // |{v : kmaxes[v] >= k}| (V without filter)
+    bool core;
+    const uint32_t* order;  // core: rank -> original id
+    uint32_t nmembers;
     int32_t k;
-    int32_t* lval;          // [V] output row; non-members must already hold 0
+    int32_t* out_row;       // core: [V] output row in original ids (non-members hold 0)
+    int32_t* lval;          // full: [V] output; core: ignored (scratch used)
     // Levels above this are not peeled: vertices still alive afterwards keep
-    // lval = -1 (bounded recompute after an update).
+    // lval = -1 (bounded recompute after an update; full layout only).
     int32_t level_cap = INT32_MAX;
 };
 
-// Per-pass device scratch (one slot per concurrently running pass).
+// Per-pass device scratch, sized for `cap` vertices.
 struct PassScratch {
-    int32_t* kd;   // [V] live key degree
-    int32_t* cd;   // [V] live constraint degree
-    uint32_t* f0;  // [V] frontier ping
-    uint32_t* f1;  // [V] frontier pong
-    uint32_t* a0;  // [V] alive list ping
-    uint32_t* a1;  // [V] alive list pong
+    int32_t* kd;   // live key degree (poisoned once removed)
+    int32_t* cd;   // live constraint degree (poisoned once removed)
+    int32_t* lv;   // core layout: level per rank (-1 alive)
+    uint32_t* rk;  // core layout: length of the member prefix of the K list
+    uint32_t* rc;  // core layout: length of the member prefix of the C list
+    uint32_t* f0;  // frontier ping
+    uint32_t* f1;  // frontier pong
+    uint32_t* a0;  // alive list ping
+    uint32_t* a1;  // alive list pong
 };
-constexpr int kScratchArrays = 6;
+constexpr int kScratchArrays = 9;
 
-__host__ __device__ inline PassScratch scratch_slot(void* base, uint32_t V, int slot) {
-    uint32_t* p = reinterpret_cast<uint32_t*>(base) + (size_t)slot * kScratchArrays * V;
+__host__ __device__ inline PassScratch scratch_at(void* base, uint64_t cap) {
+    uint32_t* p = reinterpret_cast<uint32_t*>(base);
     PassScratch s;
     s.kd = reinterpret_cast<int32_t*>(p);
-    s.cd = reinterpret_cast<int32_t*>(p + (size_t)V);
-    s.f0 = p + 2ull * V;
-    s.f1 = p + 3ull * V;
-    s.a0 = p + 4ull * V;
-    s.a1 = p + 5ull * V;
+    s.cd = reinterpret_cast<int32_t*>(p + cap);
+    s.lv = reinterpret_cast<int32_t*>(p + 2 * cap);
+    s.rk = p + 3 * cap;
+    s.rc = p + 4 * cap;
+    s.f0 = p + 5 * cap;
+    s.f1 = p + 6 * cap;
+    s.a0 = p + 7 * cap;
+    s.a1 = p + 8 * cap;
     return s;
 }
 
-// Kernel launchers (peel_kernels.hip).  All enqueue on `st` and return the
-// HIP status of the launch.
-// One pass on the whole grid: cooperative launch (co-residency) with the
-// bounded barrier of grid_sync.hpp; gs.w is zeroed by the launcher.
-hipError_t launch_pass_grid(const PassSpec& p, const PassScratch& s, uint32_t* ctrs,
+// Team of workgroups running a list of passes (batched rows).
+struct TeamDesc {
+    uint32_t first_block;  // first workgroup of the team
+    uint32_t size;         // workgroups in the team
+    uint32_t pass_begin;   // passes team_ks[pass_begin .. pass_end)
+    uint32_t pass_end;
+    uint64_t scratch_off;  // uint32 words into the scratch pool
+    uint64_t scratch_cap;  // vertices the team's scratch holds
+    uint32_t sync_off;     // words into the sync pool (sync_words(size))
+    uint32_t slot_off;     // words into the slot pool (12)
+};
+
+// ---- peel kernels (peel_kernels.hip) ----------------------------------
+// One pass on the whole grid (bounded team barrier); `slots` 12 words,
+// gs.w zeroed by the launcher.
+hipError_t launch_pass_grid(const PassSpec& p, const PassScratch& s, uint32_t* slots,
                             int32_t* lmax_out, const GridSync& gs, hipStream_t st);
 int pass_grid_blocks_per_cu();
-// Rows k = 0..kmax of one graph, one workgroup per pass pulled from a queue;
-// a pass whose counters do not add up raises *err (kErrInconsistent).
-hipError_t launch_rows_batched(const PassSpec& base, void* scratch, int slots,
-                               const uint32_t* vk, int32_t kmax, int32_t* rows,
-                               int32_t* lmax_rows, uint32_t* queue_ctr, uint32_t* err,
-                               hipStream_t st);
+// Rows of one graph (core layout): the launch has `blocks` workgroups,
+// block_team[b] names the team of block b.  Pass k of team t is
+// team_ks[..]; base describes the graph (k, nmembers, out_row set per pass).
+hipError_t launch_rows_teams(const PassSpec& base, const uint32_t* vk, int32_t* rows,
+                             int32_t* lmax_rows, const TeamDesc* teams, const uint32_t* block_team,
+                             const int32_t* team_ks, uint32_t blocks, void* scratch,
+                             uint32_t* sync_pool, uint32_t* slot_pool, uint64_t timeout_ticks,
+                             uint32_t* err, hipStream_t st);
 hipError_t launch_order_by_kmax(const int32_t* kmaxes, uint32_t V, int32_t kmax,
                                 uint32_t* hist /*[kmax+2]*/, uint32_t* vk /*[kmax+2]*/,
                                 uint32_t* order, hipStream_t st);
-// h-index fixpoint for one row (hindex_kernels.hip): initialises the row
-// from prev_row (or the out-degree) on the members, then refines in place.
+// ek[k] = edges with both endpoints of kmax >= k, k = 0..kmax (hist [kmax+2]).
+hipError_t launch_core_edges(const int32_t* kmaxes, Csr out, uint32_t V, int32_t kmax,
+                             uint64_t* hist, hipStream_t st);
+
+// ---- h-index fixpoint (hindex_kernels.hip) -----------------------------
+// One row in the core layout: initialises lv from prev (the previous row in
+// rank space) or the member out-degree, refines in place, writes out_row.
 int hindex_blocks_per_cu();
-hipError_t launch_hindex_row(const PassSpec& p, const int32_t* prev_row, uint32_t* flags,
-                             int32_t* lmax_out, int32_t* sweeps_out, const GridSync& gs,
+hipError_t launch_hindex_row(const PassSpec& p, const PassScratch& s, const int32_t* prev_lv,
+                             uint32_t* flags, int32_t* lmax_out, const GridSync& gs,
                              hipStream_t st);
-// Graph updates (maint_kernels.hip).  pairs are (u, v) interleaved.
+
+// ---- core-ordered graph (core_graph.hip, sort_kernels.hip) ------------
+size_t scan_tmp_words(uint64_t n);
+hipError_t launch_excl_scan_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* tmp,
+                                hipStream_t st);
+size_t radix_tmp_words(uint64_t n);
+hipError_t launch_radix_sort_u64(uint64_t* keys, uint64_t* alt, uint64_t n, int bits,
+                                 uint32_t* table, uint64_t** result, hipStream_t st);
+size_t core_graph_tmp_bytes(uint32_t V, uint64_t E);
+hipError_t build_core_graph(uint32_t V, uint64_t E, const uint32_t* order, Csr out, Csr in,
+                            uint32_t* rank, uint32_t* cout_off, int32_t* cout_deg,
+                            uint32_t* cout_nbr, uint32_t* cin_off, int32_t* cin_deg,
+                            uint32_t* cin_nbr, void* tmp, hipStream_t st);
+
+// ---- graph updates (maint_kernels.hip); pairs are (u, v) interleaved ---
 hipError_t launch_find_edges(const uint32_t* off, const int32_t* deg, const uint32_t* nbr,
                              const uint32_t* pairs, uint32_t n, uint32_t* found, hipStream_t st);
 hipError_t launch_check_room(const int32_t* deg, const uint32_t* capv, const uint32_t* gkey,

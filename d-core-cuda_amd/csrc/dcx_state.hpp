@@ -1,0 +1,227 @@
+// Context and graph state behind the dcx C-ABI, and the helpers every entry
+// point uses (error capture, device allocation, events).  Included once, by
+// dcx_api.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <vector>
+
+#include "dcx_internal.hpp"
+
+using namespace dcx;
+
+struct dcx_ctx {
+    dcx_config cfg{};
+    int device = 0;
+    int num_cus = 0;
+    int grid_blocks = 0;    // workgroups of a whole-grid pass / of the rows launch
+    int hindex_grid = 0;
+    uint64_t timeout_ticks = 0;  // wall-clock ticks one team barrier may take
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    std::string err;
+    dcx_timing timing{};
+    // grown-on-demand device scratch
+    void* scratch = nullptr;
+    size_t scratch_bytes = 0;
+    void* tmp = nullptr;  // core-graph build temporaries
+    size_t tmp_bytes = 0;
+    // small device words: [0,12) grid-pass phase slots, [16,20) h-index sweep
+    // flags, [24] batched error word, [64, 1024) grid barrier words
+    uint32_t* ctrs = nullptr;
+    int32_t* d_scalars = nullptr;  // 64 words of kernel results
+    int32_t* h_scalars = nullptr;  // pinned mirror (words 32.. hold error words)
+    hipEvent_t ev[8] = {};
+    // team schedule of the rows launch (device copies, grown on demand)
+    void* sched = nullptr;
+    size_t sched_bytes = 0;
+    uint32_t* pools = nullptr;  // team sync words + team slots
+    size_t pools_words = 0;
+    // staging for update batches (device + pinned host), grown on demand
+    uint32_t* upd_dev = nullptr;
+    size_t upd_dev_words = 0;
+    uint32_t* upd_host = nullptr;
+    size_t upd_host_words = 0;
+};
+
+namespace {
+constexpr int kCtrFlags = 16, kCtrErr = 24, kCtrSync = 64, kCtrWords = 1024;
+constexpr uint32_t kMaxGridBlocks = (kCtrWords - kCtrSync - kSyncGroups) * kSyncFan;
+// Workgroups per CU for grid-wide passes: far below the occupancy limit, so
+// residency of the whole grid is never in question.
+constexpr int kGridBlocksPerCu = 2;
+constexpr double kBarrierTimeoutS = 30.0;
+}  // namespace
+
+struct dcx_graph {
+    dcx_ctx* ctx = nullptr;
+    uint32_t V = 0;
+    uint64_t E = 0;
+    // Adjacency arrays: lists start at off[v] with room for capv[v] entries;
+    // *_cap is the used prefix (a bump pointer for relocated lists), *_alloc
+    // the allocated length.  Free slots hold 0xFFFFFFFF.
+    uint64_t out_cap = 0, in_cap = 0;
+    uint64_t out_alloc = 0, in_alloc = 0;
+    uint32_t *out_off = nullptr, *in_off = nullptr, *out_nbr = nullptr, *in_nbr = nullptr;
+    int32_t *out_deg = nullptr, *in_deg = nullptr;
+    uint32_t *out_capv = nullptr, *in_capv = nullptr;
+    uint64_t relocations = 0;
+    // resident results
+    int32_t* kmaxes = nullptr;
+    int32_t kmax = -1;
+    bool kmax_valid = false;
+    uint32_t* order = nullptr;   // [V] vertices by kmax, descending (rank -> id)
+    uint32_t* vk = nullptr;      // [kmax+2] members per k
+    uint32_t* hist = nullptr;    // [kmax+2]
+    uint64_t* ekh = nullptr;     // [kmax+2] edge histogram by min kmax
+    size_t vk_cap = 0;
+    std::vector<uint32_t> h_vk;  // host copy of vk
+    std::vector<uint64_t> h_ek;  // E_k, k = 0..kmax
+    // core-ordered graph (rebuilt with every kmax computation)
+    uint32_t* rank = nullptr;
+    uint32_t *cout_off = nullptr, *cin_off = nullptr, *cout_nbr = nullptr, *cin_nbr = nullptr;
+    int32_t *cout_deg = nullptr, *cin_deg = nullptr;
+    size_t core_e_cap = 0;
+    bool core_valid = false;
+    int32_t* kmax_old = nullptr; // [V] previous kmaxes during an update
+    int32_t* rows = nullptr;     // (kmax+1) x V
+    size_t rows_cap = 0;         // rows allocated
+    int32_t rows_k = 0;          // rows currently held (kmax+1 when valid)
+    int32_t* lmax_rows = nullptr;
+    size_t lmax_rows_cap = 0;
+    int32_t lmax = -1;
+    bool rows_valid = false;
+};
+
+namespace {
+
+struct Fail {
+    dcx_status st;
+};
+
+void set_err(dcx_ctx* c, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    c->err = buf;
+}
+
+#define HIPX(ctx, call)                                                                      \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess) {                                                              \
+            set_err((ctx), "%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, \
+                    __LINE__);                                                               \
+            throw Fail{e_ == hipErrorOutOfMemory ? DCX_ERR_OUT_OF_MEMORY : DCX_ERR_HIP};     \
+        }                                                                                    \
+    } while (0)
+
+template <class F>
+dcx_status guarded(dcx_ctx* c, F&& f) {
+    if (!c) return DCX_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->err.clear();
+    try {
+        if (hipSetDevice(c->device) != hipSuccess) {
+            set_err(c, "hipSetDevice(%d) failed", c->device);
+            return DCX_ERR_NO_DEVICE;
+        }
+        return f();
+    } catch (const Fail& x) {
+        return x.st;
+    } catch (const std::bad_alloc&) {
+        set_err(c, "host allocation failed");
+        return DCX_ERR_OUT_OF_MEMORY;
+    }
+}
+
+template <class T>
+void dev_alloc(dcx_ctx* c, T** p, size_t n) {
+    *p = nullptr;
+    if (n == 0) n = 1;
+    HIPX(c, hipMalloc(reinterpret_cast<void**>(p), n * sizeof(T)));
+}
+
+template <class T>
+void dev_free(T*& p) {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+}
+
+void ensure_bytes(dcx_ctx* c, void** p, size_t* have, size_t bytes) {
+    if (*have >= bytes) return;
+    if (*p) (void)hipFree(*p);
+    *p = nullptr;
+    *have = 0;
+    HIPX(c, hipMalloc(p, bytes ? bytes : 1));
+    *have = bytes;
+}
+
+void ensure_scratch(dcx_ctx* c, size_t bytes) { ensure_bytes(c, &c->scratch, &c->scratch_bytes, bytes); }
+
+void sync(dcx_ctx* c) { HIPX(c, hipStreamSynchronize(c->stream)); }
+
+float elapsed(dcx_ctx* c, int a, int b) {
+    float ms = 0;
+    HIPX(c, hipEventElapsedTime(&ms, c->ev[a], c->ev[b]));
+    return ms;
+}
+
+GridSync gsync(dcx_ctx* c, int grid) {
+    GridSync g;
+    g.w = c->ctrs + kCtrSync;
+    g.nblocks = (uint32_t)grid;
+    g.rank = 0;
+    g.timeout_ticks = c->timeout_ticks;
+    return g;
+}
+
+// Clears the kernel error words before a series of launches.
+void clear_kernel_errors(dcx_ctx* c) {
+    HIPX(c, hipMemsetAsync(c->ctrs + kCtrSync + kSyncError, 0, sizeof(uint32_t), c->stream));
+    HIPX(c, hipMemsetAsync(c->ctrs + kCtrErr, 0, sizeof(uint32_t), c->stream));
+}
+
+// Reads the kernel error words (synchronises) and turns them into a status.
+// `team_err` is an extra device error word (team pools) or nullptr.
+void check_kernel_errors(dcx_ctx* c, const char* what, const uint32_t* team_err = nullptr,
+                         uint32_t nteam = 0, uint32_t stride = 0) {
+    HIPX(c, hipMemcpyAsync(c->h_scalars + 32, c->ctrs + kCtrSync + kSyncError, 4,
+                           hipMemcpyDeviceToHost, c->stream));
+    HIPX(c, hipMemcpyAsync(c->h_scalars + 33, c->ctrs + kCtrErr, 4, hipMemcpyDeviceToHost,
+                           c->stream));
+    std::vector<uint32_t> te(nteam ? (size_t)nteam * stride : 0);
+    if (team_err && nteam)
+        HIPX(c, hipMemcpyAsync(te.data(), team_err, 4ull * nteam * stride, hipMemcpyDeviceToHost,
+                               c->stream));
+    sync(c);
+    int32_t e = std::max(c->h_scalars[32], c->h_scalars[33]);
+    for (uint32_t t = 0; t < nteam; ++t) e = std::max(e, (int32_t)te[(size_t)t * stride]);
+    if (e == (int32_t)kErrTimeout) {
+        set_err(c, "%s: a team barrier timed out (workgroups not co-resident?)", what);
+        throw Fail{DCX_ERR_TIMEOUT};
+    }
+    if (e != 0) {
+        set_err(c, "%s: kernel counters inconsistent (code %d)", what, e);
+        throw Fail{DCX_ERR_INTERNAL};
+    }
+}
+
+dcx_status check_graph(dcx_ctx* ctx, const dcx_graph* g) {
+    if (!ctx || !g) return DCX_ERR_INVALID_ARG;
+    if (g->ctx != ctx) {
+        set_err(ctx, "graph belongs to another context");
+        return DCX_ERR_INVALID_ARG;
+    }
+    return DCX_OK;
+}
+
+}  // namespace

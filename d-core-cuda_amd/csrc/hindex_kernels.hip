@@ -11,10 +11,12 @@
 // and the true row is a fixpoint), so the updates run in place and in any
 // order; a sweep that changes nothing ends the row.
 //
-// Mapping: one 64-lane wavefront per vertex.  The predicate above is
-// monotone in h, so each wavefront binary-searches h in [0, l(v)]; every
-// probe counts with ballot + popcount over the neighbour values, which
-// stay in registers for lists up to 64*kCache entries.
+// Runs on the core-ordered graph (members are ranks [0, V_k), member
+// neighbours are list prefixes).  Mapping: one 64-lane wavefront per
+// vertex.  The predicate above is monotone in h, so each wavefront
+// binary-searches h in [0, l(v)]; every probe counts with ballot + popcount
+// over the neighbour values, which stay in registers for lists up to
+// 64*kCache entries.
 #include <hip/hip_runtime.h>
 
 #include "dcx_internal.hpp"
@@ -58,18 +60,18 @@ __device__ __forceinline__ void load_cache(int32_t (&vals)[kCache], const uint32
 }
 
 // One vertex: returns its new value (<= lv).  Uniform across the wavefront.
-__device__ __forceinline__ int32_t refine(const PassSpec& P, uint32_t v, int32_t lv) {
+__device__ __forceinline__ int32_t refine(const PassSpec& P, const PassScratch& S, uint32_t v,
+                                          int32_t lv) {
     const uint32_t* on = P.K.nbr + P.K.off[v];
     const uint32_t* in = P.C.nbr + P.C.off[v];
-    const uint32_t no = (uint32_t)P.K.deg[v], ni = (uint32_t)P.C.deg[v];
+    const uint32_t no = S.rk[v], ni = S.rc[v];
     int32_t ov[kCache], iv[kCache];
-    load_cache(ov, on, no, P.lval);
-    load_cache(iv, in, ni, P.lval);
+    load_cache(ov, on, no, S.lv);
+    load_cache(iv, in, ni, S.lv);
     const int32_t k = P.k;
-    // fast accept: the current value already satisfies the predicate
     auto pred = [&](int32_t h) -> bool {
-        if ((int32_t)count_ge(ov, on, no, P.lval, h) < h) return false;
-        return k == 0 || (int32_t)count_ge(iv, in, ni, P.lval, h) >= k;
+        if ((int32_t)count_ge(ov, on, no, S.lv, h) < h) return false;
+        return k == 0 || (int32_t)count_ge(iv, in, ni, S.lv, h) >= k;
     };
     if (lv == 0 || pred(lv)) return lv;
     int32_t lo = 0, hi = lv - 1;  // pred(0) holds for members of C_k
@@ -83,10 +85,37 @@ __device__ __forceinline__ int32_t refine(const PassSpec& P, uint32_t v, int32_t
     return lo;
 }
 
-// Rows are refined in place; flags[0] counts changes of the current sweep
-// (double-buffered by sweep parity, reset one sweep ahead).
-__global__ __launch_bounds__(kBlk) void k_hindex_row(PassSpec P, uint32_t* flags,
-                                                     int32_t* sweeps_out, GridSync gs) {
+// First index of a sorted list whose entry is >= bound.
+__device__ __forceinline__ uint32_t lower_bound(const uint32_t* a, uint32_t n, uint32_t bound) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (a[mid] < bound)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// Upper bound for the row: the previous row (l_k <= l_{k-1}) or the member
+// out-degree; also the member prefix lengths.
+__global__ void k_hindex_init(PassSpec P, PassScratch S, const int32_t* prev_lv) {
+    const uint32_t nm = P.nmembers;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nm; i += gridDim.x * blockDim.x) {
+        const uint32_t rk = lower_bound(P.K.nbr + P.K.off[i], (uint32_t)P.K.deg[i], nm);
+        const uint32_t rc = lower_bound(P.C.nbr + P.C.off[i], (uint32_t)P.C.deg[i], nm);
+        S.rk[i] = rk;
+        S.rc[i] = rc;
+        S.lv[i] = prev_lv ? min(prev_lv[i], (int32_t)rk) : (int32_t)rk;
+    }
+}
+
+// Rows are refined in place; flags count changes per sweep (triple
+// buffered, reset one sweep ahead).
+__global__ __launch_bounds__(kBlk) void k_hindex_row(PassSpec P, PassScratch S, uint32_t* flags,
+                                                     GridSync gs) {
+    gs.rank = blockIdx.x;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / W;
     const uint32_t nwaves = gridDim.x * blockDim.x / W;
     const bool leader = blockIdx.x == 0 && threadIdx.x == 0;
@@ -98,9 +127,8 @@ __global__ __launch_bounds__(kBlk) void k_hindex_row(PassSpec P, uint32_t* flags
     if (!grid_barrier(gs)) return;
     int32_t sweep = 0;
     for (;;) {
-        // values strictly decrease on every sweep that changes anything and
-        // are bounded below, so the sweep count is bounded by the sum of the
-        // initial values; a count past V * (max value) cannot happen
+        // every sweep that changes something lowers a value, so the count
+        // is finite; this bound only guards against a broken counter
         if (sweep > 1 << 20) {
             if (threadIdx.x == 0) sync_fail(gs, kErrInconsistent);
             return;
@@ -108,13 +136,12 @@ __global__ __launch_bounds__(kBlk) void k_hindex_row(PassSpec P, uint32_t* flags
         uint32_t* cur = flags + (sweep % 3);
         if (leader) flags[(sweep + 1) % 3] = 0;
         bool any = false;
-        for (uint32_t i = wave; i < P.nmembers; i += nwaves) {
-            const uint32_t v = P.order ? P.order[i] : i;
-            const int32_t lv = ld_relaxed(P.lval + v);
-            const int32_t nv = refine(P, v, lv);
+        for (uint32_t v = wave; v < P.nmembers; v += nwaves) {
+            const int32_t lv = ld_relaxed(S.lv + v);
+            const int32_t nv = refine(P, S, v, lv);
             if (nv < lv) {
                 if (lane_id() == 0)
-                    __hip_atomic_store(P.lval + v, nv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_store(S.lv + v, nv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 any = true;
             }
         }
@@ -123,30 +150,20 @@ __global__ __launch_bounds__(kBlk) void k_hindex_row(PassSpec P, uint32_t* flags
         ++sweep;
         if (__hip_atomic_load(cur, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) break;
     }
-    if (leader && sweeps_out) *sweeps_out = sweep;
 }
 
-// Upper bound for row k: row k-1 on the members of C_k (zero elsewhere is
-// already in place), or the full out-degree when there is no previous row.
-__global__ void k_hindex_init(PassSpec P, const int32_t* prev_row) {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < P.nmembers;
-         i += gridDim.x * blockDim.x) {
-        const uint32_t v = P.order ? P.order[i] : i;
-        P.lval[v] = prev_row ? prev_row[v] : P.K.deg[v];
-    }
-}
-
-// lmax of a row: max over members.
-__global__ void k_row_max(PassSpec P, int32_t* out) {
+// Writes the row back to original ids and its maximum.
+__global__ void k_hindex_out(PassSpec P, PassScratch S, int32_t* lmax_out) {
     int32_t m = 0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < P.nmembers;
          i += gridDim.x * blockDim.x) {
-        const uint32_t v = P.order ? P.order[i] : i;
-        m = max(m, P.lval[v]);
+        const int32_t x = S.lv[i];
+        P.out_row[P.order[i]] = x;
+        m = max(m, x);
     }
 #pragma unroll
     for (int d = W / 2; d > 0; d >>= 1) m = max(m, __shfl_xor(m, d, W));
-    if (lane_id() == 0) atomicMax(out, m);
+    if (lane_id() == 0) atomicMax(lmax_out, m);
 }
 
 }  // namespace
@@ -158,23 +175,20 @@ int hindex_blocks_per_cu() {
     return n > 0 ? n : 1;
 }
 
-hipError_t launch_hindex_row(const PassSpec& p, const int32_t* prev_row, uint32_t* flags,
-                             int32_t* lmax_out, int32_t* sweeps_out, const GridSync& gs,
+hipError_t launch_hindex_row(const PassSpec& p, const PassScratch& s, const int32_t* prev_lv,
+                             uint32_t* flags, int32_t* lmax_out, const GridSync& gs,
                              hipStream_t st) {
-    if (p.nmembers == 0) return hipMemsetAsync(lmax_out, 0, sizeof(int32_t), st);
+    hipError_t e = hipMemsetAsync(lmax_out, 0, sizeof(int32_t), st);
+    if (e != hipSuccess || p.nmembers == 0) return e;
     const unsigned ib = (unsigned)min((p.nmembers + 255u) / 256u, 4096u);
-    hipLaunchKernelGGL(k_hindex_init, dim3(ib), dim3(256), 0, st, p, prev_row);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_hindex_init, dim3(ib), dim3(256), 0, st, p, s, prev_lv);
     e = hipMemsetAsync(gs.w, 0, kSyncResetWords * sizeof(uint32_t), st);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_hindex_row, dim3(gs.nblocks), dim3(kBlk), 0, st, p, flags, sweeps_out,
-                       gs);
-    e = hipGetLastError();
+    e = hipMemsetAsync(gs.w + kSyncGroups, 0,
+                       (sync_words(gs.nblocks) - kSyncGroups) * sizeof(uint32_t), st);
     if (e != hipSuccess) return e;
-    e = hipMemsetAsync(lmax_out, 0, sizeof(int32_t), st);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_row_max, dim3(ib), dim3(256), 0, st, p, lmax_out);
+    hipLaunchKernelGGL(k_hindex_row, dim3(gs.nblocks), dim3(kBlk), 0, st, p, s, flags, gs);
+    hipLaunchKernelGGL(k_hindex_out, dim3(ib), dim3(256), 0, st, p, s, lmax_out);
     return hipGetLastError();
 }
 

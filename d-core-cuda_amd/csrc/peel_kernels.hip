@@ -8,9 +8,15 @@
 // in-core numbers kmaxes (dcore main.cu:218-220).
 //
 // Work organisation (not the reference's):
-//   * membership and live degrees are set up once per pass, restricted to
-//     the (k,0)-core (members are a prefix of a kmax-descending order, so a
-//     pass over a small core never touches the rest of the graph);
+//   * l-rows run on the core-ordered graph (core_graph.hip): members of the
+//     (k,0)-core are ranks [0, V_k) and the member neighbours of a member
+//     are a prefix of its sorted list, found once per pass by binary
+//     search, so a pass reads exactly the core's edges and never tests
+//     membership;
+//   * a removed vertex gets its two live-degree counters poisoned (set far
+//     below zero), so later decrements can never hit a threshold and the
+//     per-edge work is one adjacency read plus one atomic -- no alive-flag
+//     load;
 //   * each level seeds from a compacted alive list instead of a scan over
 //     all V, and the next level number jumps straight to the smallest live
 //     key degree (levels that would remove nothing are skipped);
@@ -20,11 +26,12 @@
 //     wavefront;
 //   * frontier and alive-list appends are compacted with a 64-lane ballot
 //     and popcount, one atomic per wavefront;
-//   * phase counters are triple-buffered so each phase needs exactly one
-//     barrier;
-//   * a pass runs either on one workgroup (batched mode: many passes in one
-//     persistent launch, workgroup barriers only) or on the whole grid
-//     (cooperative launch, grid barriers).
+//   * phase counters are triple-buffered so each phase needs one barrier;
+//   * a pass runs on a team of workgroups: one workgroup (workgroup
+//     barriers, counters in LDS), or several with the bounded team barrier
+//     of grid_sync.hpp; a team shrinks to its first workgroup once the alive
+//     list is short, so the long tail of small levels costs no team
+//     barriers.  Many passes share one persistent launch (k_rows_teams).
 #include <hip/hip_runtime.h>
 
 #include <climits>
@@ -36,6 +43,8 @@ namespace dcx {
 namespace {
 
 constexpr int W = 64;
+constexpr int kBlk = 512;
+constexpr int32_t kPoison = INT_MIN / 2;
 
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (W - 1)); }
 
@@ -59,15 +68,18 @@ __device__ __forceinline__ void wave_min_to(uint32_t x, uint32_t* dst) {
     if (lane_id() == 0 && x != UINT_MAX) atomicMin(dst, x);
 }
 
+__device__ __forceinline__ void wave_max_to(int32_t x, int32_t* dst) {
+#pragma unroll
+    for (int d = W / 2; d > 0; d >>= 1) x = max(x, __shfl_xor(x, d, W));
+    if (lane_id() == 0) atomicMax(dst, x);
+}
+
 template <int BLK>
 struct TileLds {
-    uint32_t item[BLK];
     uint32_t koff[BLK];
     uint32_t coff[BLK];
     uint32_t klen[BLK];
     uint32_t scan[BLK];
-    int32_t cntk[BLK];
-    int32_t cntc[BLK];
     uint32_t wsum[BLK / W];
     uint32_t total;
     uint32_t nitems;
@@ -102,25 +114,6 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t x, uint32_t* wsum,
     return (w ? wsum[w - 1] : 0u) + inc - x;
 }
 
-// Loads a tile of frontier/member vertices and prefix-sums their list
-// lengths (K list then C list per vertex).
-template <int BLK>
-__device__ __forceinline__ void tile_prepare(TileLds<BLK>& T, const PassSpec& P, bool has,
-                                             uint32_t v, uint32_t n) {
-    const uint32_t tid = threadIdx.x;
-    uint32_t kl = 0, cl = 0;
-    if (has) {
-        T.item[tid] = v;
-        T.koff[tid] = P.K.off[v];
-        T.coff[tid] = P.C.off[v];
-        kl = (uint32_t)P.K.deg[v];
-        cl = (uint32_t)P.C.deg[v];
-    }
-    T.klen[tid] = kl;
-    if (tid == 0) T.nitems = n;
-    T.scan[tid] = block_excl_scan<BLK>(kl + cl, T.wsum, &T.total);
-}
-
 // Item owning edge e: the largest i < nitems with scan[i] <= e.
 template <int BLK>
 __device__ __forceinline__ uint32_t tile_find(const TileLds<BLK>& T, uint32_t e) {
@@ -135,6 +128,19 @@ __device__ __forceinline__ uint32_t tile_find(const TileLds<BLK>& T, uint32_t e)
     return lo;
 }
 
+// First index of a sorted list whose entry is >= bound.
+__device__ __forceinline__ uint32_t lower_bound(const uint32_t* a, uint32_t n, uint32_t bound) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (a[mid] < bound)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
 __device__ __forceinline__ void slot_reset(uint32_t* s) {
     s[0] = 0;         // frontier appends
     s[1] = 0;         // alive-list appends
@@ -142,95 +148,77 @@ __device__ __forceinline__ void slot_reset(uint32_t* s) {
     s[3] = 0;
 }
 
-template <bool GRID>
 __device__ __forceinline__ uint32_t slot_read(const uint32_t* p) {
-    if constexpr (GRID)
-        return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    else
-        return *(volatile const uint32_t*)p;
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// One full peel pass.  GRID: all workgroups of a cooperative launch work on
-// the pass, `slots` is in global memory and `gs` is the bounded grid
-// barrier; otherwise the calling workgroup runs the whole pass and `slots`
-// is in LDS.  Logic errors (counters that do not add up) are reported into
-// *err (and abort a grid launch) instead of looping.
-template <int BLK, bool GRID>
+__device__ __forceinline__ void poison(const PassScratch& S, uint32_t x) {
+    __hip_atomic_store(S.kd + x, kPoison, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(S.cd + x, kPoison, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// One full peel pass.  MULTI: a team of workgroups (gs != nullptr; `slots`
+// in global memory); otherwise the calling workgroup alone (`slots` may be
+// in LDS).  A team shrinks to rank 0 when the alive list falls to
+// `shrink_at` entries; the other members return early (the caller
+// re-synchronises the team before reusing the scratch).  Logic errors
+// (counters that do not add up) are reported into *err (single workgroup)
+// or the team's error word instead of looping.
+template <int BLK, bool MULTI>
 __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slots,
-                         TileLds<BLK>& T, int32_t* lmax_out, const GridSync* gs,
-                         uint32_t* err) {
+                         TileLds<BLK>& T, int32_t* lmax_out, const GridSync* gs, uint32_t* err,
+                         uint32_t shrink_at) {
     const uint32_t tid = threadIdx.x;
-    const uint32_t wg = GRID ? blockIdx.x : 0u;
-    const uint32_t nwg = GRID ? gridDim.x : 1u;
+    uint32_t tw = MULTI ? gs->rank : 0u;
+    uint32_t tn = MULTI ? gs->nblocks : 1u;
+    bool solo = !MULTI || tn == 1;
     auto barrier = [&]() -> bool {
-        if constexpr (GRID) {
-            return grid_barrier(*gs);
-        } else {
-            __syncthreads();
-            return true;
-        }
+        if (MULTI && !solo) return grid_barrier(*gs);
+        __syncthreads();
+        return true;
     };
     auto fail = [&](uint32_t code) {
         if (tid == 0) {
-            if constexpr (GRID)
+            if (MULTI)
                 sync_fail(*gs, code);
             else
                 atomicMax(err, code);
         }
     };
+    int32_t* const lval = P.core ? S.lv : P.lval;
 
-    if (wg == 0 && tid == 0) {
+    if (tw == 0 && tid == 0) {
         slot_reset(slots);
         slot_reset(slots + 4);
         slot_reset(slots + 8);
     }
 
-    // ---- set-up: members, live degrees restricted to members, alive list
+    // ---- set-up: live degrees (member prefixes in the core layout)
     const uint32_t nm = P.nmembers;
-    for (uint32_t base = wg * BLK; base < nm; base += nwg * BLK) {
-        const uint32_t i = base + tid;
-        const bool has = i < nm;
-        const uint32_t v = has ? (P.order ? P.order[i] : i) : 0u;
-        if (P.kmaxes) {
-            __syncthreads();
-            T.cntk[tid] = 0;
-            T.cntc[tid] = 0;
-            tile_prepare<BLK>(T, P, has, v, min(nm - base, (uint32_t)BLK));
-            const uint32_t total = T.total;
-            for (uint32_t eb = 0; eb < total; eb += BLK) {
-                const uint32_t e = eb + tid;
-                if (e < total) {
-                    const uint32_t it = tile_find<BLK>(T, e);
-                    const uint32_t t = e - T.scan[it];
-                    const bool kside = t < T.klen[it];
-                    const uint32_t x = kside ? P.K.nbr[T.koff[it] + t]
-                                             : P.C.nbr[T.coff[it] + (t - T.klen[it])];
-                    if (P.kmaxes[x] >= P.k) atomicAdd(kside ? &T.cntk[it] : &T.cntc[it], 1);
-                }
-            }
-            __syncthreads();
-            if (has) {
-                S.kd[v] = T.cntk[tid];
-                S.cd[v] = T.cntc[tid];
-            }
-        } else if (has) {
-            S.kd[v] = P.K.deg[v];
-            S.cd[v] = P.C.deg[v];
+    for (uint32_t i = tw * BLK + tid; i < nm; i += tn * BLK) {
+        if (P.core) {
+            const uint32_t rk = lower_bound(P.K.nbr + P.K.off[i], (uint32_t)P.K.deg[i], nm);
+            const uint32_t rc = lower_bound(P.C.nbr + P.C.off[i], (uint32_t)P.C.deg[i], nm);
+            S.rk[i] = rk;
+            S.rc[i] = rc;
+            S.kd[i] = (int32_t)rk;
+            S.cd[i] = (int32_t)rc;
+        } else {
+            S.kd[i] = P.K.deg[i];
+            S.cd[i] = P.C.deg[i];
         }
-        if (has) {
-            P.lval[v] = -1;
-            S.a0[i] = v;
-        }
+        lval[i] = -1;
     }
     if (!barrier()) return;
 
-    uint32_t *A = S.a0, *B = S.a1, *F = S.f0, *G = S.f1;
+    const uint32_t* A = nullptr;  // first level: the members in id order
+    uint32_t *B = S.a0, *Bn = S.a1, *F = S.f0, *G = S.f1;
     uint32_t nA = nm, removed = 0, s = 0, levels = 0;
     int32_t L = 0;
     const int32_t k = P.k;
-    // Without the membership filter the constraint can fail before any
+    // Without membership restriction the constraint can fail before any
     // removal (KList semantics on the whole graph, main.cu:144-169).
-    const bool check_c = P.kmaxes == nullptr && k > 0;
+    const bool check_c = !P.core && k > 0;
     while (removed < nm) {
         // Each removal needs at most two level iterations (a level number
         // taken from a stale minimum seeds nothing, the next one seeds the
@@ -239,32 +227,42 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
             fail(kErrInconsistent);
             break;
         }
+        if (MULTI && !solo && nA <= shrink_at) {
+            if (tw != 0) return;
+            solo = true;
+            tw = 0;
+            tn = 1;
+        }
         // ---- seed level L from the alive list; survivors go to B
         uint32_t* cur = slots + 4 * (s % 3);
-        if (wg == 0 && tid == 0) slot_reset(slots + 4 * ((s + 1) % 3));
+        if (tw == 0 && tid == 0) slot_reset(slots + 4 * ((s + 1) % 3));
         uint32_t mn = UINT_MAX;
-        for (uint32_t base = wg * BLK; base < nA; base += nwg * BLK) {
+        for (uint32_t base = tw * BLK; base < nA; base += tn * BLK) {
             const uint32_t i = base + tid;
             const bool has = i < nA;
-            const uint32_t v = has ? A[i] : 0u;
-            const bool alive = has && P.lval[v] == -1;
+            const uint32_t v = has ? (A ? A[i] : i) : 0u;
+            const bool alive = has && lval[v] == -1;
             const int32_t d = alive ? S.kd[v] : 0;
             const bool seed = alive && (d <= L || (check_c && S.cd[v] < k));
             const bool keep = alive && !seed;
-            if (seed) P.lval[v] = L;
+            if (seed) {
+                lval[v] = L;
+                poison(S, v);
+            }
             if (keep) mn = min(mn, (uint32_t)d);
             wave_append(seed, v, F, &cur[0]);
             wave_append(keep, v, B, &cur[1]);
         }
         wave_min_to(mn, &cur[2]);
         if (!barrier()) return;
-        uint32_t nF = slot_read<GRID>(&cur[0]);
-        nA = slot_read<GRID>(&cur[1]);
-        uint32_t mnall = slot_read<GRID>(&cur[2]);
+        uint32_t nF = slot_read(&cur[0]);
+        nA = slot_read(&cur[1]);
+        uint32_t mnall = slot_read(&cur[2]);
+        A = B;
         {
-            uint32_t* t = A;
-            A = B;
-            B = t;
+            uint32_t* t = B;
+            B = Bn;
+            Bn = t;
         }
         ++s;
         removed += nF;
@@ -272,14 +270,23 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
         // ---- cascade inside level L until no vertex crosses a threshold
         while (nF > 0) {
             uint32_t* nxt = slots + 4 * (s % 3);
-            if (wg == 0 && tid == 0) slot_reset(slots + 4 * ((s + 1) % 3));
+            if (tw == 0 && tid == 0) slot_reset(slots + 4 * ((s + 1) % 3));
             uint32_t mn2 = UINT_MAX;
-            for (uint32_t base = wg * BLK; base < nF; base += nwg * BLK) {
+            for (uint32_t base = tw * BLK; base < nF; base += tn * BLK) {
                 const uint32_t i = base + tid;
                 const bool has = i < nF;
                 const uint32_t v = has ? F[i] : 0u;
                 __syncthreads();
-                tile_prepare<BLK>(T, P, has, v, min(nF - base, (uint32_t)BLK));
+                uint32_t kl = 0, cl = 0;
+                if (has) {
+                    T.koff[tid] = P.K.off[v];
+                    T.coff[tid] = P.C.off[v];
+                    kl = P.core ? S.rk[v] : (uint32_t)P.K.deg[v];
+                    cl = P.core ? S.rc[v] : (uint32_t)P.C.deg[v];
+                }
+                T.klen[tid] = kl;
+                if (tid == 0) T.nitems = min(nF - base, (uint32_t)BLK);
+                T.scan[tid] = block_excl_scan<BLK>(kl + cl, T.wsum, &T.total);
                 const uint32_t total = T.total;
                 for (uint32_t eb = 0; eb < total; eb += BLK) {
                     const uint32_t e = eb + tid;
@@ -291,20 +298,21 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
                         const bool kside = t < T.klen[it];
                         x = kside ? P.K.nbr[T.koff[it] + t]
                                   : P.C.nbr[T.coff[it] + (t - T.klen[it])];
-                        if (P.lval[x] == -1) {
-                            if (kside) {
-                                // x loses a C-neighbour: constraint degree
-                                const int32_t old = atomicSub(&S.cd[x], 1);
-                                claim = k > 0 && old == k;
-                            } else {
-                                // x loses a K-neighbour: key degree
-                                const int32_t old = atomicSub(&S.kd[x], 1);
-                                if (old == L + 1)
-                                    claim = true;
-                                else if (old - 1 > L)
-                                    mn2 = min(mn2, (uint32_t)(old - 1));
-                            }
-                            if (claim) claim = atomicCAS(&P.lval[x], -1, L) == -1;
+                        if (kside) {
+                            // x loses a C-neighbour: constraint degree
+                            const int32_t old = atomicSub(&S.cd[x], 1);
+                            claim = k > 0 && old == k;
+                        } else {
+                            // x loses a K-neighbour: key degree
+                            const int32_t old = atomicSub(&S.kd[x], 1);
+                            if (old == L + 1)
+                                claim = true;
+                            else if (old - 1 > L)
+                                mn2 = min(mn2, (uint32_t)(old - 1));
+                        }
+                        if (claim) {
+                            claim = atomicCAS(&lval[x], -1, L) == -1;
+                            if (claim) poison(S, x);
                         }
                     }
                     wave_append(claim, x, G, &nxt[0]);
@@ -312,8 +320,8 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
             }
             wave_min_to(mn2, &nxt[2]);
             if (!barrier()) return;
-            nF = slot_read<GRID>(&nxt[0]);
-            mnall = min(mnall, slot_read<GRID>(&nxt[2]));
+            nF = slot_read(&nxt[0]);
+            mnall = min(mnall, slot_read(&nxt[2]));
             {
                 uint32_t* t = F;
                 F = G;
@@ -332,39 +340,55 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
             if (L > P.level_cap) break;  // survivors keep lval = -1
         }
     }
-    if (wg == 0 && tid == 0 && lmax_out) *lmax_out = nm ? L : 0;
+    if (tw == 0 && tid == 0 && lmax_out) *lmax_out = nm ? L : 0;
+    if (P.core) {
+        // levels back to original ids (rows hold 0 for non-members already)
+        for (uint32_t i = tw * BLK + tid; i < nm; i += tn * BLK) P.out_row[P.order[i]] = S.lv[i];
+    }
 }
 
-template <int BLK>
-__global__ __launch_bounds__(BLK) void k_pass_grid(PassSpec P, PassScratch S, uint32_t* slots,
-                                                   int32_t* lmax_out, GridSync gs) {
-    __shared__ TileLds<BLK> T;
-    run_pass<BLK, true>(P, S, slots, T, lmax_out, &gs, nullptr);
+__global__ __launch_bounds__(kBlk) void k_pass_grid(PassSpec P, PassScratch S, uint32_t* slots,
+                                                    int32_t* lmax_out, GridSync gs,
+                                                    uint32_t shrink_at) {
+    __shared__ TileLds<kBlk> T;
+    gs.rank = blockIdx.x;
+    run_pass<kBlk, true>(P, S, slots, T, lmax_out, &gs, nullptr, shrink_at);
 }
 
-// Persistent rows kernel: workgroup b owns scratch slot b and pulls k
-// values from a device counter (largest passes first) until k > kmax.
-template <int BLK>
-__global__ __launch_bounds__(BLK) void k_rows_batched(PassSpec base, void* scratch,
-                                                      const uint32_t* vk, int32_t kmax,
-                                                      int32_t* rows, int32_t* lmax_rows,
-                                                      uint32_t* qctr, uint32_t* err) {
-    __shared__ TileLds<BLK> T;
-    __shared__ uint32_t slots[12];
-    __shared__ int32_t s_k;
-    const PassScratch S = scratch_slot(scratch, base.V, (int)blockIdx.x);
-    for (;;) {
-        if (threadIdx.x == 0) s_k = (int32_t)atomicAdd(qctr, 1u);
-        __syncthreads();
-        const int32_t k = s_k;
-        __syncthreads();
-        if (k > kmax) break;
+// Persistent rows kernel: block b belongs to team block_team[b]; a team
+// runs its passes one after another, re-synchronising between them.
+__global__ __launch_bounds__(kBlk) void k_rows_teams(PassSpec base, const uint32_t* vk,
+                                                     int32_t* rows, int32_t* lmax_rows,
+                                                     const TeamDesc* teams,
+                                                     const uint32_t* block_team,
+                                                     const int32_t* team_ks, void* scratch,
+                                                     uint32_t* sync_pool, uint32_t* slot_pool,
+                                                     uint64_t timeout_ticks, uint32_t* err,
+                                                     uint32_t shrink_at) {
+    __shared__ TileLds<kBlk> T;
+    __shared__ uint32_t lslots[12];
+    const TeamDesc td = teams[block_team[blockIdx.x]];
+    const PassScratch S =
+        scratch_at(static_cast<uint32_t*>(scratch) + td.scratch_off, td.scratch_cap);
+    GridSync gs;
+    gs.w = sync_pool + td.sync_off;
+    gs.nblocks = td.size;
+    gs.rank = blockIdx.x - td.first_block;
+    gs.timeout_ticks = timeout_ticks;
+    for (uint32_t p = td.pass_begin; p < td.pass_end; ++p) {
+        const int32_t k = team_ks[p];
         PassSpec P = base;
         P.k = k;
         P.nmembers = vk[k];
-        P.lval = rows + (size_t)k * base.V;
-        run_pass<BLK, false>(P, S, slots, T, lmax_rows + k, nullptr, err);
-        __syncthreads();
+        P.out_row = rows + (size_t)k * base.V;
+        if (td.size == 1) {
+            run_pass<kBlk, false>(P, S, lslots, T, lmax_rows + k, nullptr, err, 0);
+            __syncthreads();
+        } else {
+            run_pass<kBlk, true>(P, S, slot_pool + td.slot_off, T, lmax_rows + k, &gs, err,
+                                 shrink_at);
+            if (!grid_barrier(gs)) return;
+        }
     }
 }
 
@@ -435,6 +459,33 @@ __global__ void k_kmax_scatter(const int32_t* kmaxes, uint32_t V, int32_t kmax, 
     }
 }
 
+// hist[m] += edges whose endpoints' smaller kmax is m (one wave per vertex).
+__global__ void k_core_edges(const int32_t* kmaxes, Csr out, uint32_t V, int32_t kmax,
+                             unsigned long long* hist) {
+    __shared__ unsigned long long h[kHistLds];
+    const bool lds = kmax + 1 <= kHistLds;
+    if (lds)
+        for (int i = threadIdx.x; i <= kmax; i += blockDim.x) h[i] = 0;
+    __syncthreads();
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / W;
+    const uint32_t nw = gridDim.x * blockDim.x / W;
+    for (uint32_t v = wave; v < V; v += nw) {
+        const int32_t kv = kmaxes[v];
+        const uint32_t o = out.off[v], n = (uint32_t)out.deg[v];
+        for (uint32_t j = lane_id(); j < n; j += W) {
+            const int32_t m = min(kv, kmaxes[out.nbr[o + j]]);
+            if (lds)
+                atomicAdd(&h[m], 1ull);
+            else
+                atomicAdd(&hist[m], 1ull);
+        }
+    }
+    __syncthreads();
+    if (lds)
+        for (int i = threadIdx.x; i <= kmax; i += blockDim.x)
+            if (h[i]) atomicAdd(&hist[i], h[i]);
+}
+
 __global__ void k_fill_i32(int32_t* p, int32_t v, size_t n) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x)
@@ -446,39 +497,41 @@ __global__ void k_max_i32(const int32_t* p, size_t n, int32_t* out) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x)
         m = max(m, p[i]);
-#pragma unroll
-    for (int d = W / 2; d > 0; d >>= 1) m = max(m, __shfl_xor(m, d, W));
-    if (lane_id() == 0) atomicMax(out, m);
+    wave_max_to(m, out);
 }
-
-constexpr int kGridBlk = 256;
-constexpr int kRowsBlk = 512;
 
 }  // namespace
 
 int pass_grid_blocks_per_cu() {
     int n = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_pass_grid<kGridBlk>, kGridBlk, 0) !=
-        hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_pass_grid, kBlk, 0) != hipSuccess)
         return 1;
     return n > 0 ? n : 1;
 }
 
-hipError_t launch_pass_grid(const PassSpec& p, const PassScratch& s, uint32_t* ctrs,
+hipError_t launch_pass_grid(const PassSpec& p, const PassScratch& s, uint32_t* slots,
                             int32_t* lmax_out, const GridSync& gs, hipStream_t st) {
-    // the sync words must start at zero for every launch
+    // the sync words must start at zero for every launch (error word kept)
     hipError_t e = hipMemsetAsync(gs.w, 0, kSyncResetWords * sizeof(uint32_t), st);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_pass_grid<kGridBlk>, dim3(gs.nblocks), dim3(kGridBlk), 0, st, p, s, ctrs,
-                       lmax_out, gs);
+    e = hipMemsetAsync(gs.w + kSyncGroups, 0,
+                       (sync_words(gs.nblocks) - kSyncGroups) * sizeof(uint32_t), st);
+    if (e != hipSuccess) return e;
+    const uint32_t shrink_at = 16 * kBlk;
+    hipLaunchKernelGGL(k_pass_grid, dim3(gs.nblocks), dim3(kBlk), 0, st, p, s, slots, lmax_out, gs,
+                       shrink_at);
     return hipGetLastError();
 }
 
-hipError_t launch_rows_batched(const PassSpec& base, void* scratch, int slots, const uint32_t* vk,
-                               int32_t kmax, int32_t* rows, int32_t* lmax_rows, uint32_t* qctr,
-                               uint32_t* err, hipStream_t st) {
-    hipLaunchKernelGGL(k_rows_batched<kRowsBlk>, dim3(slots), dim3(kRowsBlk), 0, st, base,
-                       scratch, vk, kmax, rows, lmax_rows, qctr, err);
+hipError_t launch_rows_teams(const PassSpec& base, const uint32_t* vk, int32_t* rows,
+                             int32_t* lmax_rows, const TeamDesc* teams, const uint32_t* block_team,
+                             const int32_t* team_ks, uint32_t blocks, void* scratch,
+                             uint32_t* sync_pool, uint32_t* slot_pool, uint64_t timeout_ticks,
+                             uint32_t* err, hipStream_t st) {
+    const uint32_t shrink_at = 4 * kBlk;
+    hipLaunchKernelGGL(k_rows_teams, dim3(blocks), dim3(kBlk), 0, st, base, vk, rows, lmax_rows,
+                       teams, block_team, team_ks, scratch, sync_pool, slot_pool, timeout_ticks,
+                       err, shrink_at);
     return hipGetLastError();
 }
 
@@ -492,6 +545,16 @@ hipError_t launch_order_by_kmax(const int32_t* kmaxes, uint32_t V, int32_t kmax,
     hipLaunchKernelGGL(k_kmax_suffix, dim3(1), dim3(64), 0, st, hist, vk, kmax);
     hipLaunchKernelGGL(k_kmax_scatter, dim3(blocks ? blocks : 1), dim3(1024), 0, st, kmaxes, V,
                        kmax, hist, order);
+    return hipGetLastError();
+}
+
+hipError_t launch_core_edges(const int32_t* kmaxes, Csr out, uint32_t V, int32_t kmax,
+                             uint64_t* hist, hipStream_t st) {
+    hipError_t e = hipMemsetAsync(hist, 0, sizeof(uint64_t) * (size_t)(kmax + 2), st);
+    if (e != hipSuccess) return e;
+    const unsigned blocks = (unsigned)min((V + 15u) / 16u, 2048u);
+    hipLaunchKernelGGL(k_core_edges, dim3(blocks ? blocks : 1), dim3(1024), 0, st, kmaxes, out, V,
+                       kmax, reinterpret_cast<unsigned long long*>(hist));
     return hipGetLastError();
 }
 
