@@ -88,10 +88,12 @@ void build_core(dcx_ctx* c, dcx_graph* g) {
 }
 
 // In-core numbers, then the kmax order, V_k, E_k and the core-ordered
-// graph.  cap >= 0 (valid resident kmaxes required): only levels 0..cap
-// are peeled and vertices left alive keep max(old, cap+1) -- exact when no
-// (k,0)-core with k > cap can have changed.
-void compute_kmax(dcx_ctx* c, dcx_graph* g, int32_t cap = -1) {
+// graph.  The plain variant peels (the reference's kmax pass); the others
+// run the h-index fixpoint over in-neighbours, from the in-degree or --
+// `warm`, valid only when no kmax can have grown (a deletion-only update) --
+// from the resident kmaxes.
+void compute_kmax(dcx_ctx* c, dcx_graph* g, int variant = DCX_VARIANT_BATCHED,
+                  bool warm = false) {
     if (!g->kmaxes) dev_alloc(c, &g->kmaxes, g->V);
     g->core_valid = false;
     if (g->V == 0) {
@@ -99,22 +101,27 @@ void compute_kmax(dcx_ctx* c, dcx_graph* g, int32_t cap = -1) {
         g->kmax_valid = true;
         return;
     }
-    if (cap >= 0 && g->kmax_valid) {
-        if (!g->kmax_old) dev_alloc(c, &g->kmax_old, g->V);
-        HIPX(c, hipMemcpyAsync(g->kmax_old, g->kmaxes, 4ull * g->V, hipMemcpyDeviceToDevice,
-                               c->stream));
-        PassSpec p = kmax_spec(g);
-        p.level_cap = cap;
-        (void)run_grid_pass(c, p, 0, 1);
-        HIPX(c, launch_capped_fixup(g->kmaxes, g->kmax_old, g->V, cap, c->stream));
-        HIPX(c, hipMemsetAsync(c->d_scalars + 1, 0, 4, c->stream));
-        HIPX(c, launch_max_i32(g->kmaxes, g->V, c->d_scalars + 1, c->stream));
+    if (variant == DCX_VARIANT_PEEL) {
+        g->kmax = run_grid_pass(c, kmax_spec(g), 0, 1);
+    } else {
+        const int32_t* prev = nullptr;
+        if (warm && g->kmax_valid) {
+            if (!g->kmax_old) dev_alloc(c, &g->kmax_old, g->V);
+            HIPX(c, hipMemcpyAsync(g->kmax_old, g->kmaxes, 4ull * g->V, hipMemcpyDeviceToDevice,
+                                   c->stream));
+            prev = g->kmax_old;
+        }
+        ensure_scratch(c, (size_t)kScratchArrays * g->V * 4);
+        clear_kernel_errors(c);
+        HIPX(c, hipEventRecord(c->ev[0], c->stream));
+        HIPX(c, launch_hindex_row(kmax_spec(g), scratch_at(c->scratch, g->V), prev,
+                                  c->ctrs + kCtrFlags, c->d_scalars + 1,
+                                  gsync(c, c->hindex_grid), c->stream));
+        HIPX(c, hipEventRecord(c->ev[1], c->stream));
         HIPX(c, hipMemcpyAsync(c->h_scalars + 1, c->d_scalars + 1, 4, hipMemcpyDeviceToHost,
                                c->stream));
-        sync(c);
+        check_kernel_errors(c, "kmax h-index");
         g->kmax = c->h_scalars[1];
-    } else {
-        g->kmax = run_grid_pass(c, kmax_spec(g), 0, 1);
     }
     HIPX(c, hipEventRecord(c->ev[7], c->stream));
     const size_t need = (size_t)g->kmax + 2;
@@ -167,7 +174,7 @@ void grow_keep(dcx_ctx* c, T** p, size_t* cap, size_t n, size_t keep) {
 
 // Static schedule of the passes k = 0..kr onto teams of workgroups: a pass
 // whose work (2 E_k + V_k) exceeds the per-workgroup share gets its own
-// team sized to it (up to kTeamMax, using at most 3/4 of the grid); the
+// team sized to it (up to team_max, using at most 3/4 of the grid); the
 // remaining passes go to single-workgroup teams, longest first, each to the
 // least loaded team.
 struct Sched {
@@ -178,9 +185,10 @@ struct Sched {
     uint32_t sync_stride = 0;
 };
 constexpr uint32_t kTeamMax = 32;
+constexpr uint32_t kShrinkAt = 4096;
 constexpr int kRowsBlocksPerCu = 2;
 
-Sched make_schedule(const dcx_graph* g, int32_t kr, uint32_t G) {
+Sched make_schedule(const dcx_graph* g, int32_t kr, uint32_t G, uint32_t team_max) {
     const uint32_t np = (uint32_t)kr + 1;
     std::vector<double> w(np);
     double W = 0;
@@ -193,12 +201,12 @@ Sched make_schedule(const dcx_graph* g, int32_t kr, uint32_t G) {
     for (uint32_t k = 0; k < np; ++k) idx[k] = k;
     std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return w[a] > w[b]; });
     Sched s;
-    s.sync_stride = sync_words(kTeamMax);
+    s.sync_stride = sync_words(team_max);
     const uint32_t big_budget = G * 3 / 4;
     uint32_t used = 0;
     std::vector<uint32_t> small;
     for (uint32_t k : idx) {
-        const uint32_t t = (uint32_t)std::min<double>(kTeamMax, w[k] / share);
+        const uint32_t t = (uint32_t)std::min<double>(team_max, w[k] / share);
         if (t >= 2 && used + t <= big_budget) {
             TeamDesc td{};
             td.first_block = used;
@@ -303,7 +311,9 @@ void compute_rows(dcx_ctx* c, dcx_graph* g, int variant, int32_t kr = -1) {
         // batched: one persistent launch, teams of workgroups per pass
         const uint32_t G = (uint32_t)(c->cfg.rows_slots > 0 ? c->cfg.rows_slots
                                                               : c->num_cus * kRowsBlocksPerCu);
-        Sched s = make_schedule(g, kr, G);
+        const uint32_t tmax = c->cfg.team_max > 0 ? (uint32_t)c->cfg.team_max : kTeamMax;
+        const uint32_t shrink = c->cfg.shrink_at > 0 ? (uint32_t)c->cfg.shrink_at : kShrinkAt;
+        Sched s = make_schedule(g, kr, G, tmax);
         const size_t nt = s.teams.size(), nb = s.block_team.size(), nk = s.ks.size();
         const size_t bytes = nt * sizeof(TeamDesc) + 4 * nb + 4 * nk + 64;
         ensure_bytes(c, &c->sched, &c->sched_bytes, bytes);
@@ -329,7 +339,7 @@ void compute_rows(dcx_ctx* c, dcx_graph* g, int variant, int32_t kr = -1) {
         HIPX(c, launch_rows_teams(basep, g->vk, g->rows, g->lmax_rows, d_teams, d_bt, d_ks,
                                   (uint32_t)nb, c->scratch, c->pools,
                                   c->pools + nt * s.sync_stride, c->timeout_ticks,
-                                  c->ctrs + kCtrErr, c->stream));
+                                  c->ctrs + kCtrErr, shrink, c->stream));
         launches = 1;
         slots_used = (int)nb;
         team_err = c->pools + kSyncError;
@@ -467,6 +477,9 @@ dcx_status dcx_ctx_create(const dcx_config* cfg, dcx_ctx** out) {
     if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return DCX_ERR_NO_DEVICE;
     if (cfg->device < 0 || cfg->device >= n) return DCX_ERR_NO_DEVICE;
     if (cfg->variant < DCX_VARIANT_AUTO || cfg->variant > DCX_VARIANT_HINDEX)
+        return DCX_ERR_INVALID_ARG;
+    if (cfg->rows_slots < 0 || cfg->rows_slots > (int32_t)kMaxGridBlocks || cfg->team_max < 0 ||
+        cfg->team_max > 256 || cfg->shrink_at < 0)
         return DCX_ERR_INVALID_ARG;
     std::unique_ptr<dcx_ctx> c(new (std::nothrow) dcx_ctx);
     if (!c) return DCX_ERR_OUT_OF_MEMORY;
@@ -769,7 +782,7 @@ dcx_status dcx_decompose(dcx_ctx* ctx, dcx_graph* g, int32_t variant, int32_t* k
     return guarded(ctx, [&]() -> dcx_status {
         ctx->timing = dcx_timing{};
         g->rows_valid = false;
-        compute_kmax(ctx, g);
+        compute_kmax(ctx, g, variant);
         compute_rows(ctx, g, variant);
         ctx->timing.total_ms = ctx->timing.kmax_ms + ctx->timing.order_ms + ctx->timing.rows_ms;
         if (kmax) *kmax = g->kmax;
@@ -853,6 +866,7 @@ dcx_status dcx_get_ragged(dcx_ctx* ctx, const dcx_graph* g, int64_t* row_ptr, in
 
 }  // extern "C"
 
-// Edge-update maintenance (same translation unit: it uses the context and
-// graph internals above).
+// Edge-update maintenance and graph batches (same translation unit: they
+// use the context and graph internals above).
 #include "dcx_update_impl.hpp"
+#include "dcx_batch_impl.hpp"

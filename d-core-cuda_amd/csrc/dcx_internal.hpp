@@ -45,9 +45,6 @@ struct PassSpec {
     int32_t k;
     int32_t* out_row;       // core: [V] output row in original ids (non-members hold 0)
     int32_t* lval;          // full: [V] output; core: ignored (scratch used)
-    // Levels above this are not peeled: vertices still alive afterwards keep
-    // lval = -1 (bounded recompute after an update; full layout only).
-    int32_t level_cap = INT32_MAX;
 };
 
 // Per-pass device scratch, sized for `cap` vertices.
@@ -104,7 +101,7 @@ hipError_t launch_rows_teams(const PassSpec& base, const uint32_t* vk, int32_t* 
                              int32_t* lmax_rows, const TeamDesc* teams, const uint32_t* block_team,
                              const int32_t* team_ks, uint32_t blocks, void* scratch,
                              uint32_t* sync_pool, uint32_t* slot_pool, uint64_t timeout_ticks,
-                             uint32_t* err, hipStream_t st);
+                             uint32_t* err, uint32_t shrink_at, hipStream_t st);
 hipError_t launch_order_by_kmax(const int32_t* kmaxes, uint32_t V, int32_t kmax,
                                 uint32_t* hist /*[kmax+2]*/, uint32_t* vk /*[kmax+2]*/,
                                 uint32_t* order, hipStream_t st);
@@ -151,8 +148,6 @@ hipError_t launch_apply_deletes(const uint32_t* off, int32_t* deg, uint32_t* nbr
                                 hipStream_t st);
 hipError_t launch_mvalue(const int32_t* kmaxes, const uint32_t* pairs, uint32_t n, int32_t* m,
                          hipStream_t st);
-hipError_t launch_capped_fixup(int32_t* kmaxes, const int32_t* old, uint32_t V, int32_t cap,
-                               hipStream_t st);
 hipError_t launch_fill_i32(int32_t* p, int32_t v, size_t n, hipStream_t st);
 hipError_t launch_max_i32(const int32_t* p, size_t n, int32_t* out, hipStream_t st);
 

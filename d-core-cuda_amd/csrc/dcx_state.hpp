@@ -52,7 +52,7 @@ struct dcx_ctx {
 
 namespace {
 constexpr int kCtrFlags = 16, kCtrErr = 24, kCtrSync = 64, kCtrWords = 1024;
-constexpr uint32_t kMaxGridBlocks = (kCtrWords - kCtrSync - kSyncGroups) * kSyncFan;
+constexpr uint32_t kMaxGridBlocks = kCtrWords - kCtrSync - kSyncFlags;
 // Workgroups per CU for grid-wide passes: far below the occupancy limit, so
 // residency of the whole grid is never in question.
 constexpr int kGridBlocksPerCu = 2;
@@ -165,7 +165,14 @@ void ensure_bytes(dcx_ctx* c, void** p, size_t* have, size_t bytes) {
     *have = bytes;
 }
 
-void ensure_scratch(dcx_ctx* c, size_t bytes) { ensure_bytes(c, &c->scratch, &c->scratch_bytes, bytes); }
+void ensure_scratch(dcx_ctx* c, size_t bytes) {
+    if (c->cfg.scratch_budget && bytes > c->cfg.scratch_budget) {
+        set_err(c, "pass scratch of %zu bytes exceeds the budget of %llu", bytes,
+                (unsigned long long)c->cfg.scratch_budget);
+        throw Fail{DCX_ERR_BUDGET};
+    }
+    ensure_bytes(c, &c->scratch, &c->scratch_bytes, bytes);
+}
 
 void sync(dcx_ctx* c) { HIPX(c, hipStreamSynchronize(c->stream)); }
 
@@ -181,6 +188,7 @@ GridSync gsync(dcx_ctx* c, int grid) {
     g.nblocks = (uint32_t)grid;
     g.rank = 0;
     g.timeout_ticks = c->timeout_ticks;
+    g.epoch = 0;
     return g;
 }
 

@@ -311,16 +311,16 @@ dcx_status do_update(dcx_ctx* c, dcx_graph* g, const uint32_t* ins, uint32_t n_i
     } else {
         const auto t1 = Clock::now();
         const bool rows = g->rows_valid;
-        if (n_ins == 0 && m_del >= 0)
-            compute_kmax(c, g, m_del);  // levels above M_del cannot change
-        else if (n_ins > 0)
-            compute_kmax(c, g);
+        const int var =
+            c->cfg.variant == DCX_VARIANT_AUTO ? DCX_VARIANT_BATCHED : c->cfg.variant;
+        // deletions alone can only lower kmaxes: the old ones are an upper
+        // bound and the h-index fixpoint starts from them
+        if (n_ins + n_del > 0) compute_kmax(c, g, var, n_ins == 0);
         S.m_ins = batch_mvalue(c, g->kmaxes, dptr(i_ins), n_ins);
         S.kmax_ms = ms_since(t1);
         const int32_t R = std::max(m_del, S.m_ins);
         if (rows) {
             const auto t2 = Clock::now();
-            int var = c->cfg.variant == DCX_VARIANT_AUTO ? DCX_VARIANT_BATCHED : c->cfg.variant;
             if (R >= 0) {
                 const int32_t kr = std::min(R, g->kmax);
                 compute_rows(c, g, var, kr);

@@ -59,19 +59,28 @@ __device__ __forceinline__ void load_cache(int32_t (&vals)[kCache], const uint32
     }
 }
 
+// Values of the row being refined: rank space (core layout) or the output
+// array itself (full layout, every vertex a member).
+__device__ __forceinline__ int32_t* row_values(const PassSpec& P, const PassScratch& S) {
+    return P.core ? S.lv : P.lval;
+}
+
 // One vertex: returns its new value (<= lv).  Uniform across the wavefront.
 __device__ __forceinline__ int32_t refine(const PassSpec& P, const PassScratch& S, uint32_t v,
                                           int32_t lv) {
+    const int32_t* l = row_values(P, S);
     const uint32_t* on = P.K.nbr + P.K.off[v];
     const uint32_t* in = P.C.nbr + P.C.off[v];
-    const uint32_t no = S.rk[v], ni = S.rc[v];
-    int32_t ov[kCache], iv[kCache];
-    load_cache(ov, on, no, S.lv);
-    load_cache(iv, in, ni, S.lv);
+    const uint32_t no = P.core ? S.rk[v] : (uint32_t)P.K.deg[v];
     const int32_t k = P.k;
+    // the C side only matters for k > 0
+    const uint32_t ni = k == 0 ? 0u : (P.core ? S.rc[v] : (uint32_t)P.C.deg[v]);
+    int32_t ov[kCache], iv[kCache];
+    load_cache(ov, on, no, l);
+    load_cache(iv, in, ni, l);
     auto pred = [&](int32_t h) -> bool {
-        if ((int32_t)count_ge(ov, on, no, S.lv, h) < h) return false;
-        return k == 0 || (int32_t)count_ge(iv, in, ni, S.lv, h) >= k;
+        if ((int32_t)count_ge(ov, on, no, l, h) < h) return false;
+        return k == 0 || (int32_t)count_ge(iv, in, ni, l, h) >= k;
     };
     if (lv == 0 || pred(lv)) return lv;
     int32_t lo = 0, hi = lv - 1;  // pred(0) holds for members of C_k
@@ -98,16 +107,22 @@ __device__ __forceinline__ uint32_t lower_bound(const uint32_t* a, uint32_t n, u
     return lo;
 }
 
-// Upper bound for the row: the previous row (l_k <= l_{k-1}) or the member
-// out-degree; also the member prefix lengths.
+// Upper bound for the row: the previous row (l_k <= l_{k-1}) or an older
+// value of the same row that can only have dropped, capped by the member
+// K-degree; in the core layout also the member prefix lengths.
 __global__ void k_hindex_init(PassSpec P, PassScratch S, const int32_t* prev_lv) {
     const uint32_t nm = P.nmembers;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nm; i += gridDim.x * blockDim.x) {
-        const uint32_t rk = lower_bound(P.K.nbr + P.K.off[i], (uint32_t)P.K.deg[i], nm);
-        const uint32_t rc = lower_bound(P.C.nbr + P.C.off[i], (uint32_t)P.C.deg[i], nm);
-        S.rk[i] = rk;
-        S.rc[i] = rc;
-        S.lv[i] = prev_lv ? min(prev_lv[i], (int32_t)rk) : (int32_t)rk;
+        uint32_t rk;
+        if (P.core) {
+            rk = lower_bound(P.K.nbr + P.K.off[i], (uint32_t)P.K.deg[i], nm);
+            S.rk[i] = rk;
+            S.rc[i] = lower_bound(P.C.nbr + P.C.off[i], (uint32_t)P.C.deg[i], nm);
+        } else {
+            rk = (uint32_t)P.K.deg[i];
+        }
+        const int32_t u = prev_lv ? min(prev_lv[i], (int32_t)rk) : (int32_t)rk;
+        row_values(P, S)[i] = u;
     }
 }
 
@@ -116,6 +131,7 @@ __global__ void k_hindex_init(PassSpec P, PassScratch S, const int32_t* prev_lv)
 __global__ __launch_bounds__(kBlk) void k_hindex_row(PassSpec P, PassScratch S, uint32_t* flags,
                                                      GridSync gs) {
     gs.rank = blockIdx.x;
+    gs.epoch = 0;
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / W;
     const uint32_t nwaves = gridDim.x * blockDim.x / W;
     const bool leader = blockIdx.x == 0 && threadIdx.x == 0;
@@ -136,12 +152,13 @@ __global__ __launch_bounds__(kBlk) void k_hindex_row(PassSpec P, PassScratch S, 
         uint32_t* cur = flags + (sweep % 3);
         if (leader) flags[(sweep + 1) % 3] = 0;
         bool any = false;
+        int32_t* const l = row_values(P, S);
         for (uint32_t v = wave; v < P.nmembers; v += nwaves) {
-            const int32_t lv = ld_relaxed(S.lv + v);
+            const int32_t lv = ld_relaxed(l + v);
             const int32_t nv = refine(P, S, v, lv);
             if (nv < lv) {
                 if (lane_id() == 0)
-                    __hip_atomic_store(S.lv + v, nv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_store(l + v, nv, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 any = true;
             }
         }
@@ -152,13 +169,14 @@ __global__ __launch_bounds__(kBlk) void k_hindex_row(PassSpec P, PassScratch S, 
     }
 }
 
-// Writes the row back to original ids and its maximum.
+// Writes the row back to original ids (core layout) and its maximum.
 __global__ void k_hindex_out(PassSpec P, PassScratch S, int32_t* lmax_out) {
     int32_t m = 0;
+    const int32_t* l = row_values(P, S);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < P.nmembers;
          i += gridDim.x * blockDim.x) {
-        const int32_t x = S.lv[i];
-        P.out_row[P.order[i]] = x;
+        const int32_t x = l[i];
+        if (P.core) P.out_row[P.order[i]] = x;
         m = max(m, x);
     }
 #pragma unroll
@@ -184,8 +202,7 @@ hipError_t launch_hindex_row(const PassSpec& p, const PassScratch& s, const int3
     hipLaunchKernelGGL(k_hindex_init, dim3(ib), dim3(256), 0, st, p, s, prev_lv);
     e = hipMemsetAsync(gs.w, 0, kSyncResetWords * sizeof(uint32_t), st);
     if (e != hipSuccess) return e;
-    e = hipMemsetAsync(gs.w + kSyncGroups, 0,
-                       (sync_words(gs.nblocks) - kSyncGroups) * sizeof(uint32_t), st);
+    e = hipMemsetAsync(gs.w + kSyncFlags, 0, gs.nblocks * sizeof(uint32_t), st);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_hindex_row, dim3(gs.nblocks), dim3(kBlk), 0, st, p, s, flags, gs);
     hipLaunchKernelGGL(k_hindex_out, dim3(ib), dim3(256), 0, st, p, s, lmax_out);

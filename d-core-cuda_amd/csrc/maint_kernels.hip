@@ -9,9 +9,8 @@
 //     done beforehand and relocation of lists that would overflow);
 //   * delete: swap the edge with the last used slot (the reference has no
 //     working delete: src/graph_gpu.h:18-19, src/graph.cpp:73-108).
-// Plus the small reductions the maintenance driver needs: edge lookup, the
-// M value of a batch (findMValue, main.cu:344-364) and the fix-up after a
-// level-capped kmax peel.
+// Plus the small reductions the maintenance driver needs: edge lookup and
+// the M value of a batch (findMValue, main.cu:344-364).
 #include <hip/hip_runtime.h>
 
 #include <climits>
@@ -145,14 +144,6 @@ __global__ void k_mvalue(const int32_t* kmaxes, const uint32_t* pairs, uint32_t 
     if (lane_id() == 0 && best != INT_MIN) atomicMax(m, best);
 }
 
-// After a kmax peel capped at level cap: vertices still alive (-1) have a new
-// kmax above cap, which only vertices with an old kmax above cap can keep;
-// the rest land exactly on cap + 1.
-__global__ void k_capped_fixup(int32_t* kmaxes, const int32_t* old, uint32_t V, int32_t cap) {
-    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x)
-        if (kmaxes[v] < 0) kmaxes[v] = max(old[v], cap + 1);
-}
-
 unsigned blocks_for(uint64_t n, unsigned per_block, unsigned cap = 4096) {
     uint64_t b = (n + per_block - 1) / per_block;
     if (b == 0) b = 1;
@@ -211,14 +202,6 @@ hipError_t launch_mvalue(const int32_t* kmaxes, const uint32_t* pairs, uint32_t 
     if (!n) return hipSuccess;
     hipLaunchKernelGGL(k_mvalue, dim3(blocks_for(n, 256, 256)), dim3(256), 0, st, kmaxes, pairs, n,
                        m);
-    return hipGetLastError();
-}
-
-hipError_t launch_capped_fixup(int32_t* kmaxes, const int32_t* old, uint32_t V, int32_t cap,
-                               hipStream_t st) {
-    if (!V) return hipSuccess;
-    hipLaunchKernelGGL(k_capped_fixup, dim3(blocks_for(V, 256)), dim3(256), 0, st, kmaxes, old, V,
-                       cap);
     return hipGetLastError();
 }
 

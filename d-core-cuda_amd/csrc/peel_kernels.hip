@@ -166,7 +166,7 @@ __device__ __forceinline__ void poison(const PassScratch& S, uint32_t x) {
 // or the team's error word instead of looping.
 template <int BLK, bool MULTI>
 __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slots,
-                         TileLds<BLK>& T, int32_t* lmax_out, const GridSync* gs, uint32_t* err,
+                         TileLds<BLK>& T, int32_t* lmax_out, GridSync* gs, uint32_t* err,
                          uint32_t shrink_at) {
     const uint32_t tid = threadIdx.x;
     uint32_t tw = MULTI ? gs->rank : 0u;
@@ -337,7 +337,6 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
                 break;
             }
             L = max(L + 1, (int32_t)min(mnall, (uint32_t)INT_MAX));
-            if (L > P.level_cap) break;  // survivors keep lval = -1
         }
     }
     if (tw == 0 && tid == 0 && lmax_out) *lmax_out = nm ? L : 0;
@@ -352,6 +351,7 @@ __global__ __launch_bounds__(kBlk) void k_pass_grid(PassSpec P, PassScratch S, u
                                                     uint32_t shrink_at) {
     __shared__ TileLds<kBlk> T;
     gs.rank = blockIdx.x;
+    gs.epoch = 0;
     run_pass<kBlk, true>(P, S, slots, T, lmax_out, &gs, nullptr, shrink_at);
 }
 
@@ -375,6 +375,7 @@ __global__ __launch_bounds__(kBlk) void k_rows_teams(PassSpec base, const uint32
     gs.nblocks = td.size;
     gs.rank = blockIdx.x - td.first_block;
     gs.timeout_ticks = timeout_ticks;
+    gs.epoch = 0;
     for (uint32_t p = td.pass_begin; p < td.pass_end; ++p) {
         const int32_t k = team_ks[p];
         PassSpec P = base;
@@ -514,8 +515,7 @@ hipError_t launch_pass_grid(const PassSpec& p, const PassScratch& s, uint32_t* s
     // the sync words must start at zero for every launch (error word kept)
     hipError_t e = hipMemsetAsync(gs.w, 0, kSyncResetWords * sizeof(uint32_t), st);
     if (e != hipSuccess) return e;
-    e = hipMemsetAsync(gs.w + kSyncGroups, 0,
-                       (sync_words(gs.nblocks) - kSyncGroups) * sizeof(uint32_t), st);
+    e = hipMemsetAsync(gs.w + kSyncFlags, 0, gs.nblocks * sizeof(uint32_t), st);
     if (e != hipSuccess) return e;
     const uint32_t shrink_at = 16 * kBlk;
     hipLaunchKernelGGL(k_pass_grid, dim3(gs.nblocks), dim3(kBlk), 0, st, p, s, slots, lmax_out, gs,
@@ -527,8 +527,7 @@ hipError_t launch_rows_teams(const PassSpec& base, const uint32_t* vk, int32_t* 
                              int32_t* lmax_rows, const TeamDesc* teams, const uint32_t* block_team,
                              const int32_t* team_ks, uint32_t blocks, void* scratch,
                              uint32_t* sync_pool, uint32_t* slot_pool, uint64_t timeout_ticks,
-                             uint32_t* err, hipStream_t st) {
-    const uint32_t shrink_at = 4 * kBlk;
+                             uint32_t* err, uint32_t shrink_at, hipStream_t st) {
     hipLaunchKernelGGL(k_rows_teams, dim3(blocks), dim3(kBlk), 0, st, base, vk, rows, lmax_rows,
                        teams, block_team, team_ks, scratch, sync_pool, slot_pool, timeout_ticks,
                        err, shrink_at);

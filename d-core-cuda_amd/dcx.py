@@ -29,7 +29,8 @@ class DcxError(RuntimeError):
 
 class Config(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("variant", ctypes.c_int32),
-                ("rows_slots", ctypes.c_int32), ("reserved0", ctypes.c_int32),
+                ("rows_slots", ctypes.c_int32), ("team_max", ctypes.c_int32),
+                ("shrink_at", ctypes.c_int32), ("reserved0", ctypes.c_int32),
                 ("scratch_budget", ctypes.c_uint64), ("reserved", ctypes.c_uint64 * 4)]
 
 
@@ -208,12 +209,15 @@ def device_count():
 class Context:
     """A dcx_ctx on one device (library-owned stream and scratch)."""
 
-    def __init__(self, device=0, variant=VARIANT_AUTO, rows_slots=0, scratch_budget=0):
+    def __init__(self, device=0, variant=VARIANT_AUTO, rows_slots=0, scratch_budget=0,
+                 team_max=0, shrink_at=0):
         cfg = Config()
         check(lib().dcx_config_init(ctypes.byref(cfg)), "dcx_config_init")
         cfg.device = device
         cfg.variant = variant
         cfg.rows_slots = rows_slots
+        cfg.team_max = team_max
+        cfg.shrink_at = shrink_at
         cfg.scratch_budget = scratch_budget
         h = ctypes.c_void_p()
         check(lib().dcx_ctx_create(ctypes.byref(cfg), ctypes.byref(h)), "dcx_ctx_create")
@@ -246,6 +250,28 @@ class Context:
 
     def upload_csr(self, V, E, csr):
         return Graph(self, V, None, None, 0, csr=(E, csr))
+
+    def upload_batch(self, graphs, slack=0):
+        """graphs: list of (V, src, dst).  Returns (Graph of the disjoint
+        union, vbase[n+1])."""
+        n = len(graphs)
+        Vs = np.array([g[0] for g in graphs], np.uint32)
+        srcs = [np.ascontiguousarray(g[1], np.uint32) for g in graphs]
+        dsts = [np.ascontiguousarray(g[2], np.uint32) for g in graphs]
+        Es = np.array([s.size for s in srcs], np.uint64)
+        SP = ctypes.c_void_p * max(n, 1)
+        sp = SP(*[P(s) for s in srcs])
+        dp = SP(*[P(d) for d in dsts])
+        vbase = np.empty(n + 1, np.uint32)
+        h = ctypes.c_void_p()
+        check(lib().dcx_graph_upload_batch(self.h, u32(n), P(Vs), P(Es), sp, dp, u32(slack),
+                                           P(vbase), ctypes.byref(h)),
+              "dcx_graph_upload_batch", self.h)
+        g = Graph.__new__(Graph)
+        g.ctx = self
+        g.h = h
+        g.V = int(vbase[-1])
+        return g, vbase
 
 
 class Graph:
@@ -332,6 +358,16 @@ class Graph:
         check(lib().dcx_get_ragged(self.ctx.h, self.h, P(rp), P(vals)), "dcx_get_ragged",
               self.ctx.h)
         return rp, vals[:n.value]
+
+    def batch_summary(self, vbase):
+        """Per-part (kmax, lmax) arrays of a decomposed batch."""
+        vbase = np.ascontiguousarray(vbase, np.uint32)
+        n = vbase.size - 1
+        km = np.empty(n, np.int32)
+        lm = np.empty(n, np.int32)
+        check(lib().dcx_batch_summary(self.ctx.h, self.h, u32(n), P(vbase), P(km), P(lm)),
+              "dcx_batch_summary", self.ctx.h)
+        return km, lm
 
     def update(self, ins=None, dels=None):
         """Maintained batch (dcx_update): deletions first, then insertions.

@@ -124,9 +124,14 @@ typedef enum dcx_variant {
 typedef struct dcx_config {
     int32_t device;          /* HIP device ordinal */
     int32_t variant;         /* default dcx_variant for dcx_decompose */
-    int32_t rows_slots;      /* concurrent passes in batched mode (0 = auto) */
+    /* Batched-rows launch shape (0 = library default; dcx_autotune picks
+     * them per graph): */
+    int32_t rows_slots;      /* workgroups of the launch (default 2 per CU) */
+    int32_t team_max;        /* most workgroups sharing one k-pass (default 32) */
+    int32_t shrink_at;       /* a team finishes a pass on one workgroup once
+                                this few vertices are alive (default 4096) */
     int32_t reserved0;
-    uint64_t scratch_budget; /* bytes of device scratch for batched passes (0 = auto) */
+    uint64_t scratch_budget; /* max bytes of pass scratch (0 = no limit) */
     uint64_t reserved[4];
 } dcx_config;
 
@@ -198,6 +203,23 @@ dcx_status dcx_decompose(dcx_ctx* ctx, dcx_graph* g, int32_t variant, int32_t* k
 dcx_status dcx_get_kmaxes(dcx_ctx* ctx, const dcx_graph* g, int32_t* kmaxes);
 dcx_status dcx_get_rows(dcx_ctx* ctx, const dcx_graph* g, int32_t k0, int32_t nrows,
                         int32_t* rows);
+/* ------------------------------------------------------------------ */
+/* Batches of graphs.                                                 */
+/* ------------------------------------------------------------------ */
+
+/* Uploads n graphs (edge lists, ids local to each graph) as one device
+ * graph, their disjoint union: graph i occupies ids [vbase[i], vbase[i+1])
+ * (vbase has n+1 entries, filled here).  dcx_decompose on it decomposes
+ * every graph in the same launches; graph i's kmaxes and rows are the
+ * union's restricted to its range (rows above its own kmax are zero).
+ * No reference counterpart: the reference decomposes one graph per run. */
+dcx_status dcx_graph_upload_batch(dcx_ctx* ctx, uint32_t n, const uint32_t* V, const uint64_t* E,
+                                  const uint32_t* const* src, const uint32_t* const* dst,
+                                  uint32_t slack, uint32_t* vbase, dcx_graph** out);
+/* Per-graph kmax and lmax of a decomposed batch (-1 / 0 for an empty graph). */
+dcx_status dcx_batch_summary(dcx_ctx* ctx, const dcx_graph* g, uint32_t n, const uint32_t* vbase,
+                             int32_t* kmax, int32_t* lmax);
+
 /* ------------------------------------------------------------------ */
 /* Edge updates and maintenance.                                      */
 /* ------------------------------------------------------------------ */
