@@ -16,6 +16,7 @@ PassSpec kmax_spec(const dcx_graph* g) {
     p.nmembers = g->V;
     p.k = 0;
     p.lval = g->kmaxes;
+    p.stats = g->pstats;
     return p;
 }
 
@@ -43,12 +44,32 @@ PassSpec core_row_spec(const dcx_graph* g, int32_t k, int32_t* out_row) {
     p.nmembers = k <= g->kmax ? g->h_vk[k] : 0u;
     p.k = k;
     p.out_row = out_row;
+    p.stats = g->pstats + (size_t)kStatWords * (k + 1);
     return p;
 }
 
 // Runs one pass on the whole grid; returns its last level.  Events ev[a],
 // ev[b] (if a >= 0) bracket the kernel.
-int32_t run_grid_pass(dcx_ctx* c, const PassSpec& p, int a = -1, int b = -1) {
+constexpr uint32_t kTraceCap = 1u << 16;
+
+// Adds the phase trace to a pass when the context asks for it:
+// trace_phases 1 traces grid passes, 2 + k the team pass of row k in the
+// batched launch.
+PassSpec traced(dcx_ctx* c, PassSpec p, bool team = false) {
+    p.trace_k = -1;
+    if (c->cfg.trace_phases <= 0 || (c->cfg.trace_phases == 1) == team) return p;
+    if (team) p.trace_k = c->cfg.trace_phases - 2;
+    if (!c->trace) {
+        dev_alloc(c, &c->trace, (size_t)kTraceWords * (kTraceCap + 1));
+        HIPX(c, hipMemsetAsync(c->trace, 0, 4, c->stream));
+    }
+    p.trace = c->trace;
+    p.trace_cap = kTraceCap;
+    return p;
+}
+
+int32_t run_grid_pass(dcx_ctx* c, const PassSpec& p0, int a = -1, int b = -1) {
+    const PassSpec p = traced(c, p0);
     ensure_scratch(c, (size_t)kScratchArrays * std::max<uint32_t>(p.V, 1) * 4);
     PassScratch s = scratch_at(c->scratch, std::max<uint32_t>(p.V, 1));
     clear_kernel_errors(c);
@@ -87,6 +108,20 @@ void build_core(dcx_ctx* c, dcx_graph* g) {
     g->core_valid = true;
 }
 
+// Grows a device array to n elements keeping its first `keep` elements.
+template <class T>
+void grow_keep(dcx_ctx* c, T** p, size_t* cap, size_t n, size_t keep) {
+    if (*cap >= n) return;
+    T* q = nullptr;
+    dev_alloc(c, &q, n);
+    if (*p && keep)
+        HIPX(c, hipMemcpyAsync(q, *p, keep * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
+    sync(c);
+    dev_free(*p);
+    *p = q;
+    *cap = n;
+}
+
 // In-core numbers, then the kmax order, V_k, E_k and the core-ordered
 // graph.  The plain variant peels (the reference's kmax pass); the others
 // run the h-index fixpoint over in-neighbours, from the in-degree or --
@@ -101,7 +136,12 @@ void compute_kmax(dcx_ctx* c, dcx_graph* g, int variant = DCX_VARIANT_BATCHED,
         g->kmax_valid = true;
         return;
     }
-    if (variant == DCX_VARIANT_PEEL) {
+    grow_keep(c, &g->pstats, &g->pstats_cap, (size_t)kStatWords, 0);
+    // Level-synchronous peel by default; the h-index fixpoint when asked
+    // for, and for deletion-only updates, where it starts from the old
+    // values (an upper bound) and settles in a few sweeps.
+    const bool use_peel = variant == DCX_VARIANT_PEEL || (variant != DCX_VARIANT_HINDEX && !warm);
+    if (use_peel) {
         g->kmax = run_grid_pass(c, kmax_spec(g), 0, 1);
     } else {
         const int32_t* prev = nullptr;
@@ -125,6 +165,11 @@ void compute_kmax(dcx_ctx* c, dcx_graph* g, int variant = DCX_VARIANT_BATCHED,
     }
     HIPX(c, hipEventRecord(c->ev[7], c->stream));
     const size_t need = (size_t)g->kmax + 2;
+    grow_keep(c, &g->pstats, &g->pstats_cap, need * kStatWords, (size_t)kStatWords);
+    g->h_pass_variant.resize(need, 0);
+    g->h_pass_team.resize(need, 0);
+    g->h_pass_variant[0] = (int8_t)(use_peel ? DCX_VARIANT_PEEL : DCX_VARIANT_HINDEX);
+    g->h_pass_team[0] = (uint16_t)(use_peel ? c->grid_blocks : c->hindex_grid);
     if (g->vk_cap < need) {
         dev_free(g->vk);
         dev_free(g->hist);
@@ -156,20 +201,6 @@ void compute_kmax(dcx_ctx* c, dcx_graph* g, int variant = DCX_VARIANT_BATCHED,
     c->timing.kmax_ms = elapsed(c, 0, 1);
     c->timing.order_ms = elapsed(c, 7, 2);
     g->kmax_valid = true;
-}
-
-// Grows a device array to n elements keeping its first `keep` elements.
-template <class T>
-void grow_keep(dcx_ctx* c, T** p, size_t* cap, size_t n, size_t keep) {
-    if (*cap >= n) return;
-    T* q = nullptr;
-    dev_alloc(c, &q, n);
-    if (*p && keep)
-        HIPX(c, hipMemcpyAsync(q, *p, keep * sizeof(T), hipMemcpyDeviceToDevice, c->stream));
-    sync(c);
-    dev_free(*p);
-    *p = q;
-    *cap = n;
 }
 
 // Static schedule of the passes k = 0..kr onto teams of workgroups: a pass
@@ -297,15 +328,19 @@ void compute_rows(dcx_ctx* c, dcx_graph* g, int variant, int32_t kr = -1) {
             HIPX(c, launch_hindex_row(p, s, k ? prev.lv : nullptr, c->ctrs + kCtrFlags,
                                       g->lmax_rows + k, gsync(c, c->hindex_grid), c->stream));
             ++launches;
+            g->h_pass_variant[k + 1] = DCX_VARIANT_HINDEX;
+            g->h_pass_team[k + 1] = (uint16_t)c->hindex_grid;
         }
     } else if (variant == DCX_VARIANT_PEEL) {
         ensure_scratch(c, (size_t)kScratchArrays * V * 4);
         HIPX(c, hipEventRecord(c->ev[5], c->stream));
         for (int32_t k = 0; k <= kr; ++k) {
-            PassSpec p = core_row_spec(g, k, g->rows + (size_t)k * V);
+            PassSpec p = traced(c, core_row_spec(g, k, g->rows + (size_t)k * V));
             HIPX(c, launch_pass_grid(p, scratch_at(c->scratch, V), c->ctrs, g->lmax_rows + k,
                                      gsync(c, c->grid_blocks), c->stream));
             ++launches;
+            g->h_pass_variant[k + 1] = DCX_VARIANT_PEEL;
+            g->h_pass_team[k + 1] = (uint16_t)c->grid_blocks;
         }
     } else {
         // batched: one persistent launch, teams of workgroups per pass
@@ -314,6 +349,11 @@ void compute_rows(dcx_ctx* c, dcx_graph* g, int variant, int32_t kr = -1) {
         const uint32_t tmax = c->cfg.team_max > 0 ? (uint32_t)c->cfg.team_max : kTeamMax;
         const uint32_t shrink = c->cfg.shrink_at > 0 ? (uint32_t)c->cfg.shrink_at : kShrinkAt;
         Sched s = make_schedule(g, kr, G, tmax);
+        for (const TeamDesc& td : s.teams)
+            for (uint32_t i = td.pass_begin; i < td.pass_end; ++i) {
+                g->h_pass_variant[s.ks[i] + 1] = DCX_VARIANT_BATCHED;
+                g->h_pass_team[s.ks[i] + 1] = (uint16_t)td.size;
+            }
         const size_t nt = s.teams.size(), nb = s.block_team.size(), nk = s.ks.size();
         const size_t bytes = nt * sizeof(TeamDesc) + 4 * nb + 4 * nk + 64;
         ensure_bytes(c, &c->sched, &c->sched_bytes, bytes);
@@ -334,7 +374,7 @@ void compute_rows(dcx_ctx* c, dcx_graph* g, int variant, int32_t kr = -1) {
         }
         HIPX(c, hipMemsetAsync(c->pools, 0, 4 * pool_words, c->stream));
         ensure_scratch(c, 4 * s.scratch_words);
-        PassSpec basep = core_row_spec(g, 0, g->rows);
+        PassSpec basep = traced(c, core_row_spec(g, 0, g->rows), true);
         HIPX(c, hipEventRecord(c->ev[5], c->stream));
         HIPX(c, launch_rows_teams(basep, g->vk, g->rows, g->lmax_rows, d_teams, d_bt, d_ks,
                                   (uint32_t)nb, c->scratch, c->pools,
@@ -413,6 +453,7 @@ void free_graph_arrays(dcx_graph* g) {
     dev_free(g->out_capv);
     dev_free(g->in_capv);
     dev_free(g->kmax_old);
+    dev_free(g->pstats);
     dev_free(g->kmaxes);
     dev_free(g->order);
     dev_free(g->vk);
@@ -499,6 +540,7 @@ dcx_status dcx_ctx_create(const dcx_config* cfg, dcx_ctx** out) {
         clock_khz <= 0)
         clock_khz = 100000;  // 100 MHz, the CDNA constant clock
     c->timeout_ticks = (uint64_t)(kBarrierTimeoutS * 1000.0 * clock_khz);
+    c->wall_khz = clock_khz;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
         return DCX_ERR_HIP;
     for (auto& e : c->ev)
@@ -522,6 +564,7 @@ dcx_status dcx_ctx_destroy(dcx_ctx* c) {
     dev_free(c->tmp);
     dev_free(c->sched);
     dev_free(c->pools);
+    dev_free(c->trace);
     dev_free(c->ctrs);
     dev_free(c->d_scalars);
     dev_free(c->upd_dev);
@@ -550,6 +593,10 @@ dcx_status dcx_graph_upload_csr(dcx_ctx* ctx, uint32_t V, uint64_t E, const uint
         return DCX_ERR_INVALID_ARG;
     return guarded(ctx, [&]() -> dcx_status {
         *out = nullptr;
+        if (V > kMaxVertices) {
+            set_err(ctx, "V = %u exceeds the supported %u vertices", V, kMaxVertices);
+            return DCX_ERR_CAPACITY;
+        }
         // validate the layout on the host: monotone offsets, degrees within
         // capacity, ids in range, sum of degrees equal on both sides
         uint64_t so = 0, si = 0;
@@ -670,6 +717,67 @@ dcx_status dcx_graph_info(const dcx_graph* g, uint32_t* V, uint64_t* E, int32_t*
     if (kmax) *kmax = g->kmax_valid ? g->kmax : -1;
     if (lmax) *lmax = g->rows_valid ? g->lmax : -1;
     return DCX_OK;
+}
+
+dcx_status dcx_graph_pass_log(dcx_ctx* ctx, const dcx_graph* g, dcx_pass_log* out,
+                              uint32_t cap, uint32_t* n) {
+    dcx_status s = check_graph(ctx, g);
+    if (s != DCX_OK) return s;
+    if (!n || (cap && !out)) return DCX_ERR_INVALID_ARG;
+    return guarded(ctx, [&]() -> dcx_status {
+        *n = 0;
+        if (!g->kmax_valid || g->V == 0) return DCX_OK;
+        const uint32_t np = (uint32_t)g->kmax + 2;
+        *n = np;
+        if (!cap) return DCX_OK;
+        const uint32_t m = std::min(cap, np);
+        std::vector<uint32_t> st((size_t)m * kStatWords);
+        sync(ctx);
+        HIPX(ctx, hipMemcpy(st.data(), g->pstats, st.size() * 4, hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < m; ++i) {
+            dcx_pass_log& e = out[i];
+            std::memset(&e, 0, sizeof e);
+            e.k = (int32_t)i - 1;
+            const bool ran = i == 0 || (g->rows_valid && (int32_t)i - 1 < g->rows_k);
+            e.variant = ran ? g->h_pass_variant[i] : 0;
+            e.team = ran ? g->h_pass_team[i] : 0;
+            e.members = i == 0 ? g->V : g->h_vk[i - 1];
+            e.edges = i == 0 ? g->E : g->h_ek[i - 1];
+            if (!ran) continue;
+            const uint32_t* w = st.data() + (size_t)i * kStatWords;
+            e.phases = w[kStatPhases];
+            e.levels = w[kStatLevels];
+            const uint64_t ticks = (uint64_t)w[kStatTicksLo] | ((uint64_t)w[kStatTicksHi] << 32);
+            e.ms = (double)ticks / ctx->wall_khz;
+        }
+        return DCX_OK;
+    });
+}
+
+dcx_status dcx_last_trace(dcx_ctx* ctx, dcx_phase_trace* out, uint32_t cap, uint32_t* n) {
+    if (!ctx || !n || (cap && !out)) return DCX_ERR_INVALID_ARG;
+    return guarded(ctx, [&]() -> dcx_status {
+        *n = 0;
+        if (!ctx->trace) return DCX_OK;
+        sync(ctx);
+        uint32_t cnt = 0;
+        HIPX(ctx, hipMemcpy(&cnt, ctx->trace, 4, hipMemcpyDeviceToHost));
+        *n = cnt;
+        const uint32_t m = std::min(cap, cnt);
+        if (!m) return DCX_OK;
+        std::vector<uint32_t> w((size_t)m * kTraceWords);
+        HIPX(ctx, hipMemcpy(w.data(), ctx->trace + kTraceWords, w.size() * 4,
+                            hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < m; ++i) {
+            const uint32_t* e = w.data() + (size_t)i * kTraceWords;
+            out[i].level = (int32_t)(e[kTraceLevel] & 0x7fffffffu);
+            out[i].seed = (e[kTraceLevel] >> 31) ? 1 : 0;
+            out[i].removed = e[kTraceFrontier];
+            out[i].alive = e[kTraceAlive];
+            out[i].us = 1000.0 * e[kTraceTicks] / ctx->wall_khz;
+        }
+        return DCX_OK;
+    });
 }
 
 dcx_status dcx_graph_capacity(const dcx_graph* g, uint64_t* out_cap, uint64_t* in_cap) {

@@ -45,7 +45,7 @@ dcx_status dcx_graph_upload_batch(dcx_ctx* ctx, uint32_t n, const uint32_t* V, c
         te += E[i];
         if (E[i] && (!src[i] || !dst[i])) return DCX_ERR_INVALID_ARG;
     }
-    if (tv > 0xffffffffull) return DCX_ERR_CAPACITY;
+    if (tv > kMaxVertices) return DCX_ERR_CAPACITY;
     std::vector<uint32_t> us, ud;
     try {
         us.resize(te);

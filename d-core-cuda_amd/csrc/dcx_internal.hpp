@@ -45,6 +45,28 @@ struct PassSpec {
     int32_t k;
     int32_t* out_row;       // core: [V] output row in original ids (non-members hold 0)
     int32_t* lval;          // full: [V] output; core: ignored (scratch used)
+    uint32_t* stats;        // nullable; [kStatWords] written by the pass leader at the end
+    uint32_t* trace;        // nullable; phase trace: [0] = entries, then kTraceWords each
+    uint32_t trace_cap;     // entries the trace can hold
+    int32_t trace_k;        // batched launch: the row whose team pass is traced
+};
+
+// Phase trace entry (peel passes run on one team when tracing is on).
+enum : uint32_t {
+    kTraceLevel = 0,    // level L; bit 31 set for a seed phase
+    kTraceFrontier = 1, // vertices removed by the phase
+    kTraceAlive = 2,    // alive-list length after the phase
+    kTraceTicks = 3,    // wall-clock ticks since the previous phase ended
+    kTraceWords = 4
+};
+
+// Per-pass log written by the device (one entry per pass).
+enum : uint32_t {
+    kStatPhases = 0,  // synchronised phases (peel) or sweeps (h-index)
+    kStatLevels = 1,  // level iterations (peel); 0 for h-index
+    kStatTicksLo = 2, // wall-clock ticks from start to end of the pass
+    kStatTicksHi = 3,
+    kStatWords = 4
 };
 
 // Per-pass device scratch, sized for `cap` vertices.

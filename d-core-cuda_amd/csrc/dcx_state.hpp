@@ -23,6 +23,8 @@ struct dcx_ctx {
     int grid_blocks = 0;    // workgroups of a whole-grid pass / of the rows launch
     int hindex_grid = 0;
     uint64_t timeout_ticks = 0;  // wall-clock ticks one team barrier may take
+    double wall_khz = 100000.0;  // device wall clock rate
+    uint32_t* trace = nullptr;   // phase trace of the last grid pass (cfg.trace_phases)
     hipStream_t stream = nullptr;
     std::mutex mu;
     std::string err;
@@ -57,6 +59,8 @@ constexpr uint32_t kMaxGridBlocks = kCtrWords - kCtrSync - kSyncFlags;
 // residency of the whole grid is never in question.
 constexpr int kGridBlocksPerCu = 2;
 constexpr double kBarrierTimeoutS = 30.0;
+// Vertex ids must stay below 2^31 (the peel combines 2 * id + side keys).
+constexpr uint32_t kMaxVertices = 0x7fffffffu;
 }  // namespace
 
 struct dcx_graph {
@@ -90,6 +94,10 @@ struct dcx_graph {
     size_t core_e_cap = 0;
     bool core_valid = false;
     int32_t* kmax_old = nullptr; // [V] previous kmaxes during an update
+    uint32_t* pstats = nullptr;  // [(kmax+2) * kStatWords] device pass log: kmax pass, rows
+    size_t pstats_cap = 0;
+    std::vector<int8_t> h_pass_variant;  // [kmax+2] variant that last ran each pass
+    std::vector<uint16_t> h_pass_team;   // [kmax+2] workgroups that ran it
     int32_t* rows = nullptr;     // (kmax+1) x V
     size_t rows_cap = 0;         // rows allocated
     int32_t rows_k = 0;          // rows currently held (kmax+1 when valid)

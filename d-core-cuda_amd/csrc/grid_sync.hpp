@@ -4,10 +4,16 @@
 // flag word (no two workgroups write the same word, so arrivals do not
 // serialise on one atomic counter); the threads of rank 0 poll all flags in
 // parallel, and once every flag has reached e rank 0 stores e into the
-// release word that the other workgroups poll.  Agent-scope release and
-// acquire fences make every global write made before the barrier visible
-// to every workgroup after it (across XCDs).  Waiting is bounded by wall
-// clock: a workgroup that waits longer than `timeout_ticks` raises the
+// release word that the other workgroups poll.  Every wavefront issues its
+// own agent-scope release fence (an L2 write-back on CDNA) before the
+// workgroup barrier, so every global write made before the team barrier
+// is visible to every workgroup after it (across XCDs); thread 0's
+// agent-scope acquire fence followed by the workgroup barrier covers the
+// way out.  (A single fence by thread 0 after the workgroup barrier is
+// cheaper but lost writes on gfx950 under load, so each wavefront fences
+// its own.)  The fences dominate the cost, which grows with the team size,
+// so teams are kept as small as their work allows.  Waiting is bounded by
+// wall clock: a workgroup that waits longer than `timeout_ticks` raises the
 // abort word, and every workgroup that sees the abort word leaves its
 // barrier with `false`, so no wavefront can wait forever even if the team
 // were not co-resident.
@@ -21,9 +27,11 @@ namespace dcx {
 enum : uint32_t {
     kSyncRelease = 0,  // epoch released by rank 0
     kSyncAbort = 1,    // nonzero: leave every barrier with false
-    kSyncError = 2,    // error code reported by the kernel (0 = none)
-    kSyncFlags = 3,    // first per-rank arrival flag
-    kSyncResetWords = 2  // words 0-1 zeroed before each launch (flags too)
+    kSyncDone = 2,     // passes a team has finished (team_resync)
+    kSyncEpoch = 3,    // rank 0's barrier epoch at the end of its last pass
+    kSyncError = 4,    // error code reported by the kernel (0 = none)
+    kSyncFlags = 5,    // first per-rank arrival flag
+    kSyncResetWords = 4  // words 0-3 zeroed before each launch (flags too)
 };
 
 // Words needed for a team of n workgroups.
@@ -85,6 +93,7 @@ __device__ inline bool grid_barrier(GridSync& g) {
         if (threadIdx.x == 0 && s_ok) {
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            __builtin_amdgcn_s_waitcnt(0);
             __hip_atomic_store(g.w + kSyncRelease, e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     } else if (threadIdx.x == 0) {
@@ -105,10 +114,58 @@ __device__ inline bool grid_barrier(GridSync& g) {
     }
     if (threadIdx.x == 0) {
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        __builtin_amdgcn_s_waitcnt(0);  // invalidation done before anyone reads
         if (sync_aborted(g)) s_ok = 0;
     }
     __syncthreads();
     return s_ok != 0;
+}
+
+// Joins a team again after a pass from which members may have left early
+// (a pass narrows its team as its alive list shrinks, and the narrowed
+// team's barriers advance only the epochs of its members).  Rank 0 calls
+// it once its own narrowed team has finished the pass: it publishes its
+// epoch and the pass count `seq`; every other member waits for `seq` and
+// adopts that epoch, so the next barrier has the same number in every
+// member.  Every thread of every member must call it.
+__device__ inline bool team_resync(GridSync& g, uint32_t seq) {
+    __shared__ uint32_t s_ep, s_ok2;
+    if (threadIdx.x == 0) {
+        s_ok2 = 1;
+        s_ep = g.epoch;
+        if (g.rank == 0) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            __builtin_amdgcn_s_waitcnt(0);
+            __hip_atomic_store(g.w + kSyncEpoch, g.epoch, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            __builtin_amdgcn_s_waitcnt(0);
+            __hip_atomic_store(g.w + kSyncDone, seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            const uint64_t t0 = wall_clock64();
+            while (__hip_atomic_load(g.w + kSyncDone, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
+                   seq) {
+                if (sync_aborted(g)) {
+                    s_ok2 = 0;
+                    break;
+                }
+                if (wall_clock64() - t0 > g.timeout_ticks) {
+                    sync_fail(g, kErrTimeout);
+                    s_ok2 = 0;
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(1);
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            __builtin_amdgcn_s_waitcnt(0);
+            s_ep = __hip_atomic_load(g.w + kSyncEpoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+    __syncthreads();
+    g.epoch = s_ep;
+    const bool ok = s_ok2 != 0;
+    __syncthreads();  // s_ep, s_ok2 are read before anyone reuses them
+    return ok;
 }
 
 }  // namespace dcx

@@ -135,6 +135,7 @@ __global__ __launch_bounds__(kBlk) void k_hindex_row(PassSpec P, PassScratch S, 
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / W;
     const uint32_t nwaves = gridDim.x * blockDim.x / W;
     const bool leader = blockIdx.x == 0 && threadIdx.x == 0;
+    const uint64_t t_start = wall_clock64();
     if (leader) {
         flags[0] = 0;
         flags[1] = 0;
@@ -166,6 +167,13 @@ __global__ __launch_bounds__(kBlk) void k_hindex_row(PassSpec P, PassScratch S, 
         if (!grid_barrier(gs)) return;
         ++sweep;
         if (__hip_atomic_load(cur, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) break;
+    }
+    if (leader && P.stats) {
+        const uint64_t dt = wall_clock64() - t_start;
+        P.stats[kStatPhases] = (uint32_t)sweep;
+        P.stats[kStatLevels] = 0;
+        P.stats[kStatTicksLo] = (uint32_t)dt;
+        P.stats[kStatTicksHi] = (uint32_t)(dt >> 32);
     }
 }
 

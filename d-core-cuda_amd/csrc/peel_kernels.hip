@@ -44,6 +44,7 @@ namespace {
 
 constexpr int W = 64;
 constexpr int kBlk = 512;
+constexpr int kUnroll = 4;
 constexpr int32_t kPoison = INT_MIN / 2;
 
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (W - 1)); }
@@ -141,11 +142,14 @@ __device__ __forceinline__ uint32_t lower_bound(const uint32_t* a, uint32_t n, u
     return lo;
 }
 
+// Atomic stores: the words share a cache line with slots that other
+// workgroups update with atomics at the same time.
 __device__ __forceinline__ void slot_reset(uint32_t* s) {
-    s[0] = 0;         // frontier appends
-    s[1] = 0;         // alive-list appends
-    s[2] = UINT_MAX;  // smallest key degree seen above the level
-    s[3] = 0;
+    __hip_atomic_store(s + 0, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);  // frontier
+    __hip_atomic_store(s + 1, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);  // alive list
+    // smallest key degree seen above the level
+    __hip_atomic_store(s + 2, (uint32_t)UINT_MAX, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(s + 3, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __device__ __forceinline__ uint32_t slot_read(const uint32_t* p) {
@@ -159,9 +163,11 @@ __device__ __forceinline__ void poison(const PassScratch& S, uint32_t x) {
 
 // One full peel pass.  MULTI: a team of workgroups (gs != nullptr; `slots`
 // in global memory); otherwise the calling workgroup alone (`slots` may be
-// in LDS).  A team shrinks to rank 0 when the alive list falls to
-// `shrink_at` entries; the other members return early (the caller
-// re-synchronises the team before reusing the scratch).  Logic errors
+// in LDS).  A team narrows as its alive list shrinks: at the start of a
+// level it keeps ceil(alive / shrink_at) members (once that is at most
+// half of them, or one), so its barriers stay as cheap as the remaining
+// work allows; the others return early and the caller joins them again
+// (team_resync) before the scratch is reused.  Logic errors
 // (counters that do not add up) are reported into *err (single workgroup)
 // or the team's error word instead of looping.
 template <int BLK, bool MULTI>
@@ -186,6 +192,22 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
         }
     };
     int32_t* const lval = P.core ? S.lv : P.lval;
+    const uint64_t t_start = wall_clock64();
+    uint64_t t_prev = t_start;
+    // phase trace, written by the team leader after each barrier
+    auto trace = [&](uint32_t phase, bool seed, int32_t lvl, uint32_t nfr, uint32_t nal) {
+        if (!P.trace || tw != 0 || tid != 0) return;
+        const uint64_t now = wall_clock64();
+        if (phase < P.trace_cap) {
+            uint32_t* e = P.trace + kTraceWords * (1 + (size_t)phase);
+            e[kTraceLevel] = (uint32_t)lvl | (seed ? 0x80000000u : 0u);
+            e[kTraceFrontier] = nfr;
+            e[kTraceAlive] = nal;
+            e[kTraceTicks] = (uint32_t)min<uint64_t>(now - t_prev, 0xffffffffull);
+        }
+        P.trace[0] = min(phase + 1, P.trace_cap);
+        t_prev = now;
+    };
 
     if (tw == 0 && tid == 0) {
         slot_reset(slots);
@@ -227,11 +249,14 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
             fail(kErrInconsistent);
             break;
         }
-        if (MULTI && !solo && nA <= shrink_at) {
-            if (tw != 0) return;
-            solo = true;
-            tw = 0;
-            tn = 1;
+        if (MULTI && !solo && shrink_at > 0) {
+            const uint32_t want = max(1u, min(tn, (uint32_t)((nA + shrink_at - 1) / shrink_at)));
+            if (want == 1 || 2 * want <= tn) {
+                if (tw >= want) return;
+                tn = want;
+                gs->nblocks = want;
+                solo = want == 1;
+            }
         }
         // ---- seed level L from the alive list; survivors go to B
         uint32_t* cur = slots + 4 * (s % 3);
@@ -258,6 +283,7 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
         uint32_t nF = slot_read(&cur[0]);
         nA = slot_read(&cur[1]);
         uint32_t mnall = slot_read(&cur[2]);
+        trace(s, true, L, nF, nA);
         A = B;
         {
             uint32_t* t = B;
@@ -272,9 +298,13 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
             uint32_t* nxt = slots + 4 * (s % 3);
             if (tw == 0 && tid == 0) slot_reset(slots + 4 * ((s + 1) % 3));
             uint32_t mn2 = UINT_MAX;
-            for (uint32_t base = tw * BLK; base < nF; base += tn * BLK) {
+            // Frontier items per workgroup: a team splits even a small
+            // frontier over all its members (whole tiles would leave most
+            // of them idle while a few walk long edge lists).
+            const uint32_t ipw = solo ? (uint32_t)BLK : min((uint32_t)BLK, (nF + tn - 1) / tn);
+            for (uint32_t base = tw * ipw; base < nF; base += tn * ipw) {
                 const uint32_t i = base + tid;
-                const bool has = i < nF;
+                const bool has = tid < ipw && i < nF;
                 const uint32_t v = has ? F[i] : 0u;
                 __syncthreads();
                 uint32_t kl = 0, cl = 0;
@@ -285,43 +315,61 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
                     cl = P.core ? S.rc[v] : (uint32_t)P.C.deg[v];
                 }
                 T.klen[tid] = kl;
-                if (tid == 0) T.nitems = min(nF - base, (uint32_t)BLK);
+                if (tid == 0) T.nitems = min(nF - base, ipw);
                 T.scan[tid] = block_excl_scan<BLK>(kl + cl, T.wsum, &T.total);
                 const uint32_t total = T.total;
-                for (uint32_t eb = 0; eb < total; eb += BLK) {
-                    const uint32_t e = eb + tid;
-                    bool claim = false;
-                    uint32_t x = 0;
-                    if (e < total) {
-                        const uint32_t it = tile_find<BLK>(T, e);
-                        const uint32_t t = e - T.scan[it];
-                        const bool kside = t < T.klen[it];
-                        x = kside ? P.K.nbr[T.koff[it] + t]
-                                  : P.C.nbr[T.coff[it] + (t - T.klen[it])];
-                        if (kside) {
-                            // x loses a C-neighbour: constraint degree
-                            const int32_t old = atomicSub(&S.cd[x], 1);
-                            claim = k > 0 && old == k;
-                        } else {
-                            // x loses a K-neighbour: key degree
-                            const int32_t old = atomicSub(&S.kd[x], 1);
-                            if (old == L + 1)
-                                claim = true;
-                            else if (old - 1 > L)
-                                mn2 = min(mn2, (uint32_t)(old - 1));
-                        }
-                        if (claim) {
-                            claim = atomicCAS(&lval[x], -1, L) == -1;
-                            if (claim) poison(S, x);
+                // kUnroll independent edges per thread per step, so their
+                // loads and atomics overlap instead of forming one chain.
+                for (uint32_t eb = 0; eb < total; eb += kUnroll * BLK) {
+                    uint32_t x[kUnroll];
+                    bool ks[kUnroll], ok[kUnroll], claim[kUnroll];
+#pragma unroll
+                    for (int u = 0; u < kUnroll; ++u) {
+                        const uint32_t e = eb + u * BLK + tid;
+                        ok[u] = e < total;
+                        x[u] = 0;
+                        ks[u] = false;
+                        if (ok[u]) {
+                            const uint32_t it = tile_find<BLK>(T, e);
+                            const uint32_t t = e - T.scan[it];
+                            ks[u] = t < T.klen[it];
+                            x[u] = ks[u] ? P.K.nbr[T.koff[it] + t]
+                                         : P.C.nbr[T.coff[it] + (t - T.klen[it])];
                         }
                     }
-                    wave_append(claim, x, G, &nxt[0]);
+                    int32_t old[kUnroll];
+#pragma unroll
+                    for (int u = 0; u < kUnroll; ++u) {
+                        // K-side edge: x loses a C-neighbour (constraint
+                        // degree); C-side edge: x loses a K-neighbour (key)
+                        old[u] = ok[u] ? atomicSub(ks[u] ? &S.cd[x[u]] : &S.kd[x[u]], 1) : 0;
+                    }
+#pragma unroll
+                    for (int u = 0; u < kUnroll; ++u) {
+                        claim[u] = false;
+                        if (ok[u]) {
+                            if (ks[u]) {
+                                claim[u] = k > 0 && old[u] == k;
+                            } else if (old[u] == L + 1) {
+                                claim[u] = true;
+                            } else if (old[u] - 1 > L) {
+                                mn2 = min(mn2, (uint32_t)(old[u] - 1));
+                            }
+                        }
+                        if (claim[u]) {
+                            claim[u] = atomicCAS(&lval[x[u]], -1, L) == -1;
+                            if (claim[u]) poison(S, x[u]);
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < kUnroll; ++u) wave_append(claim[u], x[u], G, &nxt[0]);
                 }
             }
             wave_min_to(mn2, &nxt[2]);
             if (!barrier()) return;
             nF = slot_read(&nxt[0]);
             mnall = min(mnall, slot_read(&nxt[2]));
+            trace(s, false, L, nF, nA);
             {
                 uint32_t* t = F;
                 F = G;
@@ -339,11 +387,23 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
             L = max(L + 1, (int32_t)min(mnall, (uint32_t)INT_MAX));
         }
     }
-    if (tw == 0 && tid == 0 && lmax_out) *lmax_out = nm ? L : 0;
+    if (tw == 0 && tid == 0) {
+        if (lmax_out) *lmax_out = nm ? L : 0;
+        if (P.stats) {
+            const uint64_t dt = wall_clock64() - t_start;
+            P.stats[kStatPhases] = s;
+            P.stats[kStatLevels] = levels;
+            P.stats[kStatTicksLo] = (uint32_t)dt;
+            P.stats[kStatTicksHi] = (uint32_t)(dt >> 32);
+        }
+    }
     if (P.core) {
         // levels back to original ids (rows hold 0 for non-members already)
         for (uint32_t i = tw * BLK + tid; i < nm; i += tn * BLK) P.out_row[P.order[i]] = S.lv[i];
     }
+    // the whole (narrowed) team is done with the scratch before rank 0
+    // hands the team on (team_resync)
+    if (MULTI && !solo) barrier();
 }
 
 __global__ __launch_bounds__(kBlk) void k_pass_grid(PassSpec P, PassScratch S, uint32_t* slots,
@@ -382,13 +442,16 @@ __global__ __launch_bounds__(kBlk) void k_rows_teams(PassSpec base, const uint32
         P.k = k;
         P.nmembers = vk[k];
         P.out_row = rows + (size_t)k * base.V;
+        P.stats = base.stats ? base.stats + (size_t)kStatWords * k : nullptr;
+        if (k != base.trace_k) P.trace = nullptr;
         if (td.size == 1) {
             run_pass<kBlk, false>(P, S, lslots, T, lmax_rows + k, nullptr, err, 0);
             __syncthreads();
         } else {
+            gs.nblocks = td.size;  // the pass may have narrowed the team
             run_pass<kBlk, true>(P, S, slot_pool + td.slot_off, T, lmax_rows + k, &gs, err,
                                  shrink_at);
-            if (!grid_barrier(gs)) return;
+            if (!team_resync(gs, p - td.pass_begin + 1)) return;
         }
     }
 }

@@ -30,7 +30,7 @@ class DcxError(RuntimeError):
 class Config(ctypes.Structure):
     _fields_ = [("device", ctypes.c_int32), ("variant", ctypes.c_int32),
                 ("rows_slots", ctypes.c_int32), ("team_max", ctypes.c_int32),
-                ("shrink_at", ctypes.c_int32), ("reserved0", ctypes.c_int32),
+                ("shrink_at", ctypes.c_int32), ("trace_phases", ctypes.c_int32),
                 ("scratch_budget", ctypes.c_uint64), ("reserved", ctypes.c_uint64 * 4)]
 
 
@@ -39,6 +39,19 @@ class Timing(ctypes.Structure):
                 ("rows_ms", ctypes.c_double), ("rows_kernel_ms", ctypes.c_double),
                 ("total_ms", ctypes.c_double), ("rows_launches", ctypes.c_int32),
                 ("rows_slots", ctypes.c_int32), ("reserved", ctypes.c_int32 * 6)]
+
+
+class PassLog(ctypes.Structure):
+    _fields_ = [("k", ctypes.c_int32), ("variant", ctypes.c_int32),
+                ("members", ctypes.c_uint32), ("phases", ctypes.c_uint32),
+                ("levels", ctypes.c_uint32), ("team", ctypes.c_uint32),
+                ("edges", ctypes.c_uint64), ("ms", ctypes.c_double)]
+
+
+class PhaseTrace(ctypes.Structure):
+    _fields_ = [("level", ctypes.c_int32), ("seed", ctypes.c_int32),
+                ("removed", ctypes.c_uint32), ("alive", ctypes.c_uint32),
+                ("us", ctypes.c_double)]
 
 
 class UpdateStats(ctypes.Structure):
@@ -210,7 +223,7 @@ class Context:
     """A dcx_ctx on one device (library-owned stream and scratch)."""
 
     def __init__(self, device=0, variant=VARIANT_AUTO, rows_slots=0, scratch_budget=0,
-                 team_max=0, shrink_at=0):
+                 team_max=0, shrink_at=0, trace_phases=0):
         cfg = Config()
         check(lib().dcx_config_init(ctypes.byref(cfg)), "dcx_config_init")
         cfg.device = device
@@ -218,6 +231,7 @@ class Context:
         cfg.rows_slots = rows_slots
         cfg.team_max = team_max
         cfg.shrink_at = shrink_at
+        cfg.trace_phases = trace_phases
         cfg.scratch_budget = scratch_budget
         h = ctypes.c_void_p()
         check(lib().dcx_ctx_create(ctypes.byref(cfg), ctypes.byref(h)), "dcx_ctx_create")
@@ -244,6 +258,20 @@ class Context:
         t = Timing()
         check(lib().dcx_last_timing(self.h, ctypes.byref(t)), "dcx_last_timing", self.h)
         return t
+
+    def last_trace(self):
+        """Phase trace of the last grid peel pass (needs trace_phases=1):
+        structured numpy array (level, seed, removed, alive, us)."""
+        n = ctypes.c_uint32()
+        check(lib().dcx_last_trace(self.h, None, u32(0), ctypes.byref(n)), "dcx_last_trace", self.h)
+        arr = (PhaseTrace * max(n.value, 1))()
+        check(lib().dcx_last_trace(self.h, arr, u32(n.value), ctypes.byref(n)), "dcx_last_trace",
+              self.h)
+        out = np.zeros(n.value, [("level", np.int32), ("seed", np.int32), ("removed", np.uint32),
+                                 ("alive", np.uint32), ("us", np.float64)])
+        for i in range(n.value):
+            out[i] = (arr[i].level, arr[i].seed, arr[i].removed, arr[i].alive, arr[i].us)
+        return out
 
     def upload_edges(self, V, src, dst, slack=0):
         return Graph(self, V, src, dst, slack)
@@ -358,6 +386,17 @@ class Graph:
         check(lib().dcx_get_ragged(self.ctx.h, self.h, P(rp), P(vals)), "dcx_get_ragged",
               self.ctx.h)
         return rp, vals[:n.value]
+
+    def pass_log(self):
+        """Device log of the resident decomposition's passes (dcx_graph_pass_log):
+        a list of dicts, entry 0 the kmax pass, entry 1+k row k."""
+        n = ctypes.c_uint32()
+        check(lib().dcx_graph_pass_log(self.ctx.h, self.h, None, u32(0), ctypes.byref(n)),
+              "dcx_graph_pass_log", self.ctx.h)
+        arr = (PassLog * max(n.value, 1))()
+        check(lib().dcx_graph_pass_log(self.ctx.h, self.h, arr, u32(n.value), ctypes.byref(n)),
+              "dcx_graph_pass_log", self.ctx.h)
+        return [{f: getattr(arr[i], f) for f, _ in PassLog._fields_} for i in range(n.value)]
 
     def batch_summary(self, vbase):
         """Per-part (kmax, lmax) arrays of a decomposed batch."""

@@ -130,7 +130,9 @@ typedef struct dcx_config {
     int32_t team_max;        /* most workgroups sharing one k-pass (default 32) */
     int32_t shrink_at;       /* a team finishes a pass on one workgroup once
                                 this few vertices are alive (default 4096) */
-    int32_t reserved0;
+    int32_t trace_phases;    /* phase log for dcx_last_trace: 1 = grid passes
+                                (kmax pass, plain rows); 2 + k = the team pass
+                                of row k in the batched launch; 0 = off */
     uint64_t scratch_budget; /* max bytes of pass scratch (0 = no limit) */
     uint64_t reserved[4];
 } dcx_config;
@@ -174,6 +176,35 @@ dcx_status dcx_graph_free(dcx_graph* g);
 /* V, current E, and kmax/lmax of the resident decomposition (-1 if none). */
 dcx_status dcx_graph_info(const dcx_graph* g, uint32_t* V, uint64_t* E, int32_t* kmax,
                           int32_t* lmax);
+/* Log of the passes of the resident decomposition, as the device recorded
+ * them (no counterpart in the reference, which prints only the kmax/lmax
+ * times, main.cu:256).  Entry 0 is the kmax pass (k = -1), entry 1+k row k;
+ * rows not yet computed have variant 0.  These are the per-pass features
+ * and timings the variant scheduler learns from. */
+typedef struct dcx_pass_log {
+    int32_t k;         /* -1 for the kmax pass */
+    int32_t variant;   /* dcx_variant that ran the pass (0 = not run) */
+    uint32_t members;  /* vertices of the pass (V, or V_k of the (k,0)-core) */
+    uint32_t phases;   /* synchronised phases (peel) or sweeps (h-index) */
+    uint32_t levels;   /* level iterations (peel), 0 for h-index */
+    uint32_t team;     /* workgroups that ran it */
+    uint64_t edges;    /* edges of the pass (E, or E_k) */
+    double ms;         /* device wall-clock time of the pass */
+} dcx_pass_log;
+/* One synchronised phase of the last peel pass run on the whole grid (the
+ * kmax pass, or a row of the plain variant) when cfg.trace_phases is set:
+ * the per-iteration frontier statistics the scheduler learns from. */
+typedef struct dcx_phase_trace {
+    int32_t level;     /* level L of the phase */
+    int32_t seed;      /* 1: seeding phase (alive-list scan), 0: cascade */
+    uint32_t removed;  /* vertices the phase removed */
+    uint32_t alive;    /* alive-list length after it */
+    double us;         /* device time since the previous phase */
+} dcx_phase_trace;
+dcx_status dcx_last_trace(dcx_ctx* ctx, dcx_phase_trace* out, uint32_t cap, uint32_t* n);
+/* *n = kmax + 2 entries available; copies min(cap, *n) into out. */
+dcx_status dcx_graph_pass_log(dcx_ctx* ctx, const dcx_graph* g, dcx_pass_log* out,
+                              uint32_t cap, uint32_t* n);
 /* Capacity of each adjacency array (E + free slots). */
 dcx_status dcx_graph_capacity(const dcx_graph* g, uint64_t* out_cap, uint64_t* in_cap);
 /* Copies the device CSR back (arrays sized V+1, V, capacity). */

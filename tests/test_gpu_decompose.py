@@ -127,6 +127,71 @@ def test_klist_unfiltered_and_filtered(ctx, k):
     g.free()
 
 
+@pytest.mark.parametrize("shape", [(8, 2, 64), (64, 4, 512), (0, 64, 1), (0, 1, 0),
+                                   (512, 32, 100000)])
+def test_batched_launch_shapes(shape):
+    """Any launch shape (workgroups, team cap, shrink threshold) gives the
+    same bits: these are the knobs the autotuner searches."""
+    slots, team_max, shrink = shape
+    src, dst = dcx.gen_rmat(14, 16 << 14, 5)
+    V = 1 << 14
+    ref = oracle.decompose(oracle.csr_from_edges(V, src, dst))
+    say(f"rmat14 batched shape slots={slots} team_max={team_max} shrink_at={shrink}")
+    with dcx.Context(0, rows_slots=slots, team_max=team_max, shrink_at=shrink) as c:
+        K, km, lmax, rows, _, _ = gpu_decompose(c, V, src, dst, dcx.VARIANT_BATCHED)
+    assert K == ref[0] and lmax == ref[2]
+    assert np.array_equal(km, ref[1]) and np.array_equal(rows, ref[3])
+
+
+def test_team_barrier_visibility_repeated():
+    """Multi-workgroup teams hand frontiers, alive lists and levels to each
+    other through the team barrier; any write the barrier fails to publish
+    shows up as a wrong level or an inconsistent-counter error.  Several
+    launch shapes, each run repeatedly, must all give the oracle's bits."""
+    src, dst = dcx.gen_rmat(17, 16 << 17, 9)
+    V = 1 << 17
+    ref = oracle.decompose(oracle.csr_from_edges(V, src, dst))
+    for slots, team_max, shrink in [(0, 0, 0), (64, 16, 64), (256, 64, 1), (0, 8, 100000)]:
+        with dcx.Context(0, rows_slots=slots, team_max=team_max, shrink_at=shrink) as c:
+            g = c.upload_edges(V, src, dst)
+            for rep in range(3):
+                for var in (dcx.VARIANT_BATCHED, dcx.VARIANT_PEEL):
+                    say(f"rmat17 repeat {rep} variant={var} slots={slots} team_max={team_max}"
+                        f" shrink_at={shrink}")
+                    K, lmax = g.decompose(var)
+                    assert K == ref[0] and lmax == ref[2]
+                    assert np.array_equal(g.kmaxes(), ref[1])
+                    assert np.array_equal(g.rows(), ref[3])
+            g.free()
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+def test_pass_log(ctx, variant):
+    """The device pass log covers the kmax pass and every row."""
+    src, dst = dcx.gen_rmat(11, 16 << 11, 3)
+    V = 1 << 11
+    say(f"rmat11 pass log variant={variant}")
+    g = ctx.upload_edges(V, src, dst)
+    K, lmax = g.decompose(variant)
+    km = g.kmaxes()
+    log = g.pass_log()
+    g.free()
+    assert len(log) == K + 2
+    assert log[0]["k"] == -1 and log[0]["members"] == V and log[0]["edges"] == len(src)
+    assert log[0]["variant"] in (dcx.VARIANT_PEEL, dcx.VARIANT_HINDEX) and log[0]["phases"] > 0
+    c = oracle.csr_from_edges(V, src, dst)
+    u = np.repeat(np.arange(V), np.diff(c.out_off.astype(np.int64)))
+    m = np.minimum(km[u], km[c.out_nbr])
+    for k in range(K + 1):
+        e = log[k + 1]
+        assert e["k"] == k and e["variant"] == variant and e["team"] >= 1
+        assert e["members"] == int((km >= k).sum())
+        assert e["edges"] == int((m >= k).sum())
+        assert e["phases"] >= 1 and e["ms"] >= 0.0
+        if variant != dcx.VARIANT_HINDEX:
+            assert e["levels"] >= 1
+
+
 def test_rmat20_full(ctx):
     """The headline graph: R-MAT scale 20, 16*2^20 samples, seed 1."""
     s, d = dcx.gen_rmat(20, 16 << 20, seed=1)
