@@ -10,9 +10,16 @@ order, and every l-row k = 0..kmax.  Each dcx_decompose call returns after
 its stream has finished, so host wall time brackets exactly K steps.
 
 Prints one JSON line with the metric (GEdges/s = E / T_step), the roofline of
-the dominant kernel (the batched rows kernel; algorithmic bytes
+the dominant kernel (the batched rows kernel k_rows_teams; algorithmic bytes
 sum_k (8 E_k + 20 V_k) over the (k,0)-cores, divided by its HIP-event time on
-the library's stream), and the CPU oracle timed on the host cores.
+the library's stream), and the CPU oracle timed on the host cores.  The
+launch shape of the rows kernel is autotuned on the graph before timing
+(dcx_autotune, untimed; every shape gives the same bits).
+
+``--workload chunglu256`` instead times the batched workload: 256 seeded
+Chung-Lu digraphs of log-uniform size decomposed by one union launch
+sequence, against the per-graph, per-k launch loop (plain variant), with
+every graph's output compared bit for bit.
 """
 import argparse
 import json
@@ -66,11 +73,20 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="rmat", choices=["rmat", "chunglu256", "serve"])
+    ap.add_argument("--serve-scale", type=int, default=22)
+    ap.add_argument("--serve-batches", type=int, default=10000)
+    ap.add_argument("--serve-loads", default="0.5,0.9",
+                    help="serve: Poisson loads as fractions of the measured capacity")
     ap.add_argument("--scale", type=int, default=20)
-    ap.add_argument("--variant", default="batched", choices=["batched", "peel"])
+    ap.add_argument("--variant", default="batched", choices=["batched", "peel", "hindex"])
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-variant timing")
+    ap.add_argument("--no-hindex", action="store_true", help="skip the h-index timing")
+    ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--baseline-graphs", type=int, default=256,
+                    help="chunglu256: graphs timed in the per-graph loop")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -79,7 +95,14 @@ def main():
               "k-ranges across GPUs (see DESIGN.md), which is not built or measured here.",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "chunglu256":
+        return run_chunglu(args)
+    if args.workload == "serve":
+        return run_serve(args)
+    return run_rmat(args)
 
+
+def run_rmat(args):
     import dcx
 
     scale = args.scale
@@ -91,9 +114,18 @@ def main():
     gen_s = time.perf_counter() - t0
     print(f"# rmat{scale}: V={V} E={E} (gen {gen_s:.1f}s)", file=sys.stderr, flush=True)
 
-    variant = dcx.VARIANT_BATCHED if args.variant == "batched" else dcx.VARIANT_PEEL
+    variant = {"batched": dcx.VARIANT_BATCHED, "peel": dcx.VARIANT_PEEL,
+               "hindex": dcx.VARIANT_HINDEX}[args.variant]
     ctx = dcx.Context(0)
     g = ctx.upload_edges(V, src, dst, slack=0)
+    tune = None
+    if variant == dcx.VARIANT_BATCHED and not args.no_tune:
+        g.kmax()
+        r = g.autotune(reps=2)
+        tune = {"rows_slots": r.rows_slots, "team_max": r.team_max, "shrink_at": r.shrink_at,
+                "candidates": r.candidates, "best_rows_ms": round(r.best_ms, 3),
+                "default_rows_ms": round(r.default_ms, 3)}
+        print(f"# autotune: {tune}", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         g.decompose(variant)
     kms, rows_k, tot = [], [], []
@@ -123,7 +155,8 @@ def main():
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 5),
         "traffic": read_traffic(),
-        "kernel": "k_rows_batched" if variant == dcx.VARIANT_BATCHED else "k_pass_grid",
+        "kernel": {dcx.VARIANT_BATCHED: "k_rows_teams", dcx.VARIANT_PEEL: "k_pass_grid",
+                   dcx.VARIANT_HINDEX: "k_hindex_row"}[variant],
         "kernel_ms": round(rows_ms, 3),
         "alg_bytes": b_rows,
     }
@@ -135,6 +168,16 @@ def main():
         extra["plain_peel_ms"] = round(tp.total_ms, 2)
         extra["plain_rows_ms"] = round(tp.rows_ms, 2)
         extra["batched_speedup_vs_plain"] = round(tp.total_ms / float(np.mean(tot)), 2)
+    if not args.no_hindex and variant != dcx.VARIANT_HINDEX:
+        g.decompose(dcx.VARIANT_HINDEX)
+        th = ctx.timing()
+        extra["hindex_ms"] = round(th.total_ms, 2)
+        extra["hindex_rows_ms"] = round(th.rows_ms, 2)
+        extra["hindex_kmax_ms"] = round(th.kmax_ms, 2)
+    if tune:
+        extra["autotune"] = tune
+    extra["full_decomposition_roofline_frac"] = round(
+        (b_rows + b_kmax) / (float(np.mean(tot)) * 1e-3) / 1e9 / HBM_PEAK_GBS, 5)
 
     cpu = None
     if not args.no_cpu:
@@ -185,6 +228,247 @@ def main():
     line.update(extra)
     g.free()
     ctx.close()
+    print(json.dumps(line), flush=True)
+
+
+def run_chunglu(args):
+    import dcx
+    import workloads
+
+    t0 = time.perf_counter()
+    graphs = workloads.chunglu_batch()
+    E = int(sum(s.size for _, s, _ in graphs))
+    Vt = int(sum(V for V, _, _ in graphs))
+    print(f"# chunglu256: {len(graphs)} graphs V={Vt} E={E} (gen {time.perf_counter() - t0:.1f}s)",
+          file=sys.stderr, flush=True)
+    ctx = dcx.Context(0)
+    g, vbase = ctx.upload_batch(graphs)
+    g.kmax()
+    tune = None
+    if not args.no_tune:
+        r = g.autotune(reps=2)
+        tune = {"rows_slots": r.rows_slots, "team_max": r.team_max, "shrink_at": r.shrink_at,
+                "candidates": r.candidates, "best_rows_ms": round(r.best_ms, 3),
+                "default_rows_ms": round(r.default_ms, 3)}
+        print(f"# autotune: {tune}", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        g.decompose(dcx.VARIANT_BATCHED)
+    rows_k, kms = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        K, lmax = g.decompose(dcx.VARIANT_BATCHED)
+        rows_k.append(ctx.timing().rows_kernel_ms)
+        kms.append(ctx.timing().kmax_ms)
+    ms_per_step = (time.perf_counter() - t0) / args.steps * 1e3
+    timing = ctx.timing()
+    print(f"# batched: {ms_per_step:.2f} ms/step kmax={K} lmax={lmax}", file=sys.stderr,
+          flush=True)
+    kmaxes = g.kmaxes()
+    rp_u, vals_u = g.ragged()
+    kk, ll = g.batch_summary(vbase)
+    # algorithmic bytes of the union's passes (kmax pass + rows)
+    src_u = np.concatenate([s.astype(np.int64) + int(vbase[i]) for i, (_, s, _) in
+                            enumerate(graphs)])
+    dst_u = np.concatenate([d.astype(np.int64) + int(vbase[i]) for i, (_, _, d) in
+                            enumerate(graphs)])
+    vk, ek = core_sizes(kmaxes, src_u, dst_u, K)
+    del src_u, dst_u
+    b_rows = float((8 * ek + 20 * vk).sum())
+    rows_ms = float(np.mean(rows_k))
+    g.free()
+
+    # baseline: per-graph, per-k launches (plain variant), bit-exact compare
+    nb = min(args.baseline_graphs, len(graphs))
+    base_ms, base_e, equal = 0.0, 0, True
+    for i in range(nb):
+        V, s, d = graphs[i]
+        gi = ctx.upload_edges(V, s, d)
+        t1 = time.perf_counter()
+        Ki, Li = gi.decompose(dcx.VARIANT_PEEL)
+        base_ms += (time.perf_counter() - t1) * 1e3
+        base_e += s.size
+        rp_i, vals_i = gi.ragged()
+        a, b = int(vbase[i]), int(vbase[i + 1])
+        ok = (Ki == kk[i] and Li == ll[i]
+              and np.array_equal(gi.kmaxes(), kmaxes[a:b])
+              and np.array_equal(vals_i, vals_u[rp_u[a]:rp_u[b]]))
+        equal = equal and bool(ok)
+        gi.free()
+    print(f"# per-graph loop: {nb} graphs {base_ms:.1f} ms, equal={equal}", file=sys.stderr,
+          flush=True)
+    base_rate = base_e / (base_ms * 1e-3) / 1e9
+    value = E / (ms_per_step * 1e-3) / 1e9
+    achieved = b_rows / (rows_ms * 1e-3) / 1e9
+    line = {
+        "metric": "batched D-core decomposition throughput over 256 Chung-Lu digraphs "
+                  "(GEdges/s = sum E / T)",
+        "value": round(value, 6),
+        "unit": "GEdges/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": "chunglu256: 256 Chung-Lu digraphs, samples log-uniform "
+                               "1e4..2e6, avg degree 8, gamma 2.1, seeds 1000+i",
+                   "graphs": len(graphs), "V": Vt, "E": E, "kmax": K, "lmax": lmax,
+                   "rows_slots": timing.rows_slots},
+        "breakdown_ms": {"kmax_pass": round(float(np.mean(kms)), 3),
+                         "order": round(timing.order_ms, 3), "rows_kernel": round(rows_ms, 3)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                     "kernel": "k_rows_teams", "kernel_ms": round(rows_ms, 3),
+                     "alg_bytes": b_rows},
+        "per_graph_baseline": {"graphs": nb, "edges": base_e, "ms": round(base_ms, 2),
+                               "GEdges_s": round(base_rate, 6),
+                               "variant": "plain (one launch per k per graph)",
+                               "output_equal_to_batched": equal},
+        "speedup_vs_per_graph": round(value / base_rate, 2),
+        "autotune": tune,
+        "cpu_baseline": None,
+    }
+    ctx.close()
+    print(json.dumps(line), flush=True)
+
+
+def run_serve(args):
+    """Continuous-batching maintenance of an R-MAT base graph under a seeded
+    stream of update batches (sizes log-uniform in [1, 4096], 70% inserts of
+    R-MAT-sampled new edges, 30% deletes of existing edges).  Capacity: the
+    stream submitted as fast as possible (the server groups what is queued),
+    run in four segments with the maintained state checked against a full
+    GPU recomputation (dcx_state_hash) after each.  Then Poisson arrivals at
+    the given fractions of that capacity on a fresh copy of the base graph;
+    latency = completion - scheduled arrival."""
+    import dcx
+
+    scale, nb = args.serve_scale, args.serve_batches
+    V = 1 << scale
+    t0 = time.perf_counter()
+    src, dst = dcx.gen_rmat(scale, 16 << scale, seed=2)
+    io, ip, do, dp = dcx.gen_update_stream(V, src, dst, scale, nb, max_batch=4096,
+                                           p_insert=0.7, seed=3)
+    reqs = [(ip[io[b]:io[b + 1]], dp[do[b]:do[b + 1]]) for b in range(nb)]
+    edits = int(io[-1] + do[-1])
+    print(f"# serve: rmat{scale} E={src.size}, {nb} batches, {edits} edits "
+          f"(gen {time.perf_counter() - t0:.1f}s)", file=sys.stderr, flush=True)
+    ctx = dcx.Context(0)
+
+    def fresh():
+        g = ctx.upload_edges(V, src, dst, slack=4)
+        g.decompose()
+        return g
+
+    def collect(sv, res, want, label):
+        last = time.perf_counter()
+        while len(res) < want:
+            for r in sv.poll(wait_ms=500):
+                res[r["id"]] = r
+            if time.perf_counter() - last > 20:
+                last = time.perf_counter()
+                print(f"# {label}: {len(res)}/{want} done", file=sys.stderr, flush=True)
+
+    # ---- capacity, with verification between segments
+    g = fresh()
+    seg = max(1, nb // 4)
+    checks = []
+    t_busy = 0.0
+    done_total = 0
+    for s0 in range(0, nb, seg):
+        with g.serve() as sv:
+            tseg = time.perf_counter()
+            ids = [sv.submit(ins, dels) for ins, dels in reqs[s0:s0 + seg]]
+            res = {}
+            collect(sv, res, len(ids), f"capacity segment {s0 // seg}")
+            t_busy += time.perf_counter() - tseg
+            st = sv.stop()
+        done_total += len(ids)
+        bad = sum(1 for r in res.values() if r["status"] != 0)
+        h_maint = g.state_hash()
+        g.decompose()
+        h_full = g.state_hash()
+        checks.append({"after_batches": s0 + len(ids), "maintained_equals_recomputed":
+                       h_maint == h_full, "failed_requests": bad, "groups": st["groups"]})
+        print(f"# capacity segment {s0 // seg}: {len(ids)} requests in {st['groups']} groups, "
+              f"state ok={h_maint == h_full}", file=sys.stderr, flush=True)
+    capacity = done_total / t_busy
+    g.free()
+
+    # ---- Poisson loads
+    loads = []
+    for frac in [float(x) for x in args.serve_loads.split(",") if x]:
+        lam = frac * capacity
+        rng = np.random.default_rng(3)
+        arrivals = np.cumsum(rng.exponential(1.0 / lam, size=nb))
+        g = fresh()
+        with g.serve() as sv:
+            res, sub_lag = {}, {}
+            base = time.perf_counter()
+            last = base
+            for i, (ins, dels) in enumerate(reqs):
+                while True:
+                    now = time.perf_counter() - base
+                    if now >= arrivals[i]:
+                        break
+                    for r in sv.poll(wait_ms=0):
+                        res[r["id"]] = r
+                    time.sleep(min(0.0005, arrivals[i] - now))
+                rid = sv.submit(ins, dels)
+                sub_lag[rid] = (time.perf_counter() - base) - arrivals[i]
+                if time.perf_counter() - last > 20:
+                    last = time.perf_counter()
+                    print(f"# load {frac}: submitted {i + 1}/{nb}", file=sys.stderr, flush=True)
+            collect(sv, res, nb, f"load {frac}")
+            wall = time.perf_counter() - base
+            st = sv.stop()
+        h_maint = g.state_hash()
+        g.decompose()
+        ok = h_maint == g.state_hash()
+        g.free()
+        lat = np.array([r["done_ms"] - r["submit_ms"] + 1e3 * sub_lag[r["id"]]
+                        for r in res.values()])
+        svc = np.array([r["done_ms"] - r["start_ms"] for r in res.values()])
+        loads.append({"load": frac, "rate_req_s": round(lam, 2),
+                      "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                      "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                      "service_p50_ms": round(float(np.percentile(svc, 50)), 3),
+                      "service_p99_ms": round(float(np.percentile(svc, 99)), 3),
+                      "throughput_req_s": round(nb / wall, 2),
+                      "throughput_edits_s": round(edits / wall, 1),
+                      "groups": st["groups"], "max_group": st["max_group"],
+                      "final_state_equals_recomputed": bool(ok),
+                      "failed_requests": sum(1 for r in res.values() if r["status"] != 0)})
+        print(f"# load {frac}: {loads[-1]}", file=sys.stderr, flush=True)
+    ctx.close()
+    head = loads[-1] if loads else {}
+    line = {
+        "metric": "p99 edge-batch update latency under continuous batching (ms)",
+        "value": head.get("p99_ms"),
+        "unit": "ms",
+        "n_gpus": 1,
+        "steps": nb,
+        "warmup": 0,
+        "ms_per_step": round(1e3 / capacity, 3),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": f"serve: rmat{scale} seed 2 base, {nb} update batches "
+                               "(log-uniform 1..4096 edits, 70% insert / 30% delete, seed 3), "
+                               "Poisson arrivals",
+                   "V": V, "E": int(src.size), "edits": edits,
+                   "load_of_headline": head.get("load")},
+        "capacity_req_s": round(capacity, 2),
+        "capacity_edits_s": round(edits / t_busy, 1),
+        "loads": loads,
+        "checks": checks,
+        "cpu_baseline": None,
+    }
     print(json.dumps(line), flush=True)
 
 

@@ -49,6 +49,33 @@ __global__ void k_keys_to_nbr(const uint64_t* keys, uint64_t n, uint64_t mask, u
 
 __global__ void k_set_u32(uint32_t* p, uint32_t v) { *p = v; }
 
+// out[0] += sum of deg, out[1] += entries of [0, deg) that are not vertex
+// ids (one wavefront per vertex).
+__global__ void k_audit(uint32_t V, const uint32_t* off, const int32_t* deg, const uint32_t* nbr,
+                        unsigned long long* out) {
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / W;
+    const uint32_t nw = gridDim.x * blockDim.x / W;
+    uint64_t sum = 0, bad = 0;
+    for (uint32_t v = wave; v < V; v += nw) {
+        const int32_t d = deg[v];
+        if (lane_id() == 0) sum += d < 0 ? 0 : (uint64_t)d;
+        if (d < 0) {
+            if (lane_id() == 0) ++bad;
+            continue;
+        }
+        const uint32_t* l = nbr + off[v];
+        for (uint32_t j = lane_id(); j < (uint32_t)d; j += W) bad += l[j] >= V;
+    }
+    for (int dd = W / 2; dd > 0; dd >>= 1) {
+        sum += __shfl_xor(sum, dd, W);
+        bad += __shfl_xor(bad, dd, W);
+    }
+    if (lane_id() == 0) {
+        atomicAdd(out, (unsigned long long)sum);
+        atomicAdd(out + 1, (unsigned long long)bad);
+    }
+}
+
 unsigned grid_for(uint64_t n, unsigned per, unsigned cap = 8192) {
     uint64_t b = (n + per - 1) / per;
     if (!b) b = 1;
@@ -67,6 +94,13 @@ size_t core_graph_tmp_bytes(uint32_t V, uint64_t E) {
     // keys + alternate keys + scan words + radix table
     return 16ull * (E ? E : 1) + 4ull * (scan_tmp_words(V) + 1) +
            4ull * (radix_tmp_words(E) + 1) + 256;
+}
+
+hipError_t launch_audit(uint32_t V, Csr side, unsigned long long* out, hipStream_t st) {
+    if (V == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_audit, dim3(grid_for(V, 4)), dim3(256), 0, st, V, side.off, side.deg,
+                       side.nbr, out);
+    return hipGetLastError();
 }
 
 hipError_t build_core_graph(uint32_t V, uint64_t E, const uint32_t* order, Csr out, Csr in,

@@ -82,9 +82,32 @@ int32_t run_grid_pass(dcx_ctx* c, const PassSpec& p0, int a = -1, int b = -1) {
     return c->h_scalars[0];
 }
 
+// Checks that both adjacency sides hold exactly E valid entries before a
+// rebuild sizes its buffers from E (one read of the lists; a graph that
+// fails it is marked broken instead of being read out of bounds).
+void audit_adjacency(dcx_ctx* c, dcx_graph* g) {
+    unsigned long long* d = reinterpret_cast<unsigned long long*>(c->d_scalars + 24);
+    HIPX(c, hipMemsetAsync(d, 0, 32, c->stream));
+    HIPX(c, launch_audit(g->V, Csr{g->out_off, g->out_deg, g->out_nbr}, d, c->stream));
+    HIPX(c, launch_audit(g->V, Csr{g->in_off, g->in_deg, g->in_nbr}, d + 2, c->stream));
+    HIPX(c, hipMemcpyAsync(c->h_scalars + 24, d, 32, hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    uint64_t w[4];
+    std::memcpy(w, c->h_scalars + 24, 32);
+    if (w[0] != g->E || w[2] != g->E || w[1] || w[3]) {
+        g->broken = true;
+        g->kmax_valid = g->rows_valid = false;
+        set_err(c, "adjacency audit: degree sums out %llu in %llu, E %llu, bad ids %llu/%llu",
+                (unsigned long long)w[0], (unsigned long long)w[2], (unsigned long long)g->E,
+                (unsigned long long)w[1], (unsigned long long)w[3]);
+        throw Fail{DCX_ERR_INTERNAL};
+    }
+}
+
 // Relabels by kmax and sorts every list (see core_graph.hip).
 void build_core(dcx_ctx* c, dcx_graph* g) {
     const uint32_t V = g->V;
+    audit_adjacency(c, g);
     if (!g->rank) {
         dev_alloc(c, &g->rank, V);
         dev_alloc(c, &g->cout_off, (size_t)V + 1);
@@ -978,3 +1001,5 @@ dcx_status dcx_get_ragged(dcx_ctx* ctx, const dcx_graph* g, int64_t* row_ptr, in
 // use the context and graph internals above).
 #include "dcx_update_impl.hpp"
 #include "dcx_batch_impl.hpp"
+#include "dcx_tune_impl.hpp"
+#include "dcx_serve_impl.hpp"

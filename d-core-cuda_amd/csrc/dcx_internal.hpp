@@ -147,6 +147,9 @@ size_t radix_tmp_words(uint64_t n);
 hipError_t launch_radix_sort_u64(uint64_t* keys, uint64_t* alt, uint64_t n, int bits,
                                  uint32_t* table, uint64_t** result, hipStream_t st);
 size_t core_graph_tmp_bytes(uint32_t V, uint64_t E);
+// Adds sum(deg) to out[0] and the count of list entries in [0, deg) that
+// are not vertex ids (< V) to out[1].
+hipError_t launch_audit(uint32_t V, Csr side, unsigned long long* out, hipStream_t st);
 hipError_t build_core_graph(uint32_t V, uint64_t E, const uint32_t* order, Csr out, Csr in,
                             uint32_t* rank, uint32_t* cout_off, int32_t* cout_deg,
                             uint32_t* cout_nbr, uint32_t* cin_off, int32_t* cin_deg,

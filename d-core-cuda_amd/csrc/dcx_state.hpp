@@ -65,6 +65,8 @@ constexpr uint32_t kMaxVertices = 0x7fffffffu;
 
 struct dcx_graph {
     dcx_ctx* ctx = nullptr;
+    bool served = false;  // owned by a running dcx_server: API calls refused
+    bool broken = false;  // an edit failed half-way: the adjacency is not trusted
     uint32_t V = 0;
     uint64_t E = 0;
     // Adjacency arrays: lists start at off[v] with room for capv[v] entries;
@@ -236,6 +238,14 @@ dcx_status check_graph(dcx_ctx* ctx, const dcx_graph* g) {
     if (g->ctx != ctx) {
         set_err(ctx, "graph belongs to another context");
         return DCX_ERR_INVALID_ARG;
+    }
+    if (g->served) {
+        set_err(ctx, "graph is being served (dcx_serve_stop first)");
+        return DCX_ERR_STATE;
+    }
+    if (g->broken) {
+        set_err(ctx, "graph is inconsistent after a failed edit; upload it again");
+        return DCX_ERR_STATE;
     }
     return DCX_OK;
 }

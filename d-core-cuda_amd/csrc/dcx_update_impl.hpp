@@ -53,17 +53,19 @@ Groups group_pairs(const uint32_t* pairs, uint32_t n, bool by_src) {
     return g;
 }
 
+// Shape checks of a pair list (ids in range, no self-loop, no repeat); the
+// detail goes to c's error text when c is given.
 dcx_status validate_pairs(dcx_ctx* c, uint32_t V, const uint32_t* p, uint32_t n,
                           const char* what) {
     std::vector<uint64_t> keys(n);
     for (uint32_t i = 0; i < n; ++i) {
         const uint32_t u = p[2 * i], v = p[2 * i + 1];
         if (u >= V || v >= V) {
-            set_err(c, "%s edge %u: vertex id out of range (V=%u)", what, i, V);
+            if (c) set_err(c, "%s edge %u: vertex id out of range (V=%u)", what, i, V);
             return DCX_ERR_INVALID_ARG;
         }
         if (u == v) {
-            set_err(c, "%s edge %u: self-loop %u->%u", what, i, u, v);
+            if (c) set_err(c, "%s edge %u: self-loop %u->%u", what, i, u, v);
             return DCX_ERR_INVALID_ARG;
         }
         keys[i] = (uint64_t)u << 32 | v;
@@ -71,8 +73,9 @@ dcx_status validate_pairs(dcx_ctx* c, uint32_t V, const uint32_t* p, uint32_t n,
     std::sort(keys.begin(), keys.end());
     for (uint32_t i = 1; i < n; ++i)
         if (keys[i] == keys[i - 1]) {
-            set_err(c, "%s batch lists %u->%u twice", what, (uint32_t)(keys[i] >> 32),
-                    (uint32_t)keys[i]);
+            if (c)
+                set_err(c, "%s batch lists %u->%u twice", what, (uint32_t)(keys[i] >> 32),
+                        (uint32_t)keys[i]);
             return DCX_ERR_INVALID_ARG;
         }
     return DCX_OK;
@@ -181,80 +184,99 @@ double ms_since(Clock::time_point t) {
     return std::chrono::duration<double, std::milli>(Clock::now() - t).count();
 }
 
-dcx_status do_update(dcx_ctx* c, dcx_graph* g, const uint32_t* ins, uint32_t n_ins,
-                     const uint32_t* del, uint32_t n_del, bool maintain, dcx_update_stats* st) {
-    const auto t0 = Clock::now();
-    dcx_update_stats S{};
-    const uint32_t V = g->V;
-    dcx_status s = validate_pairs(c, V, del, n_del, "delete");
-    if (s != DCX_OK) return s;
-    s = validate_pairs(c, V, ins, n_ins, "insert");
-    if (s != DCX_OK) return s;
-    if ((uint64_t)n_ins > g->E + (uint64_t)n_del + 0xffffffffull) return DCX_ERR_CAPACITY;
-
-    const Groups Gdo = group_pairs(del, n_del, true), Gdi = group_pairs(del, n_del, false);
-    const Groups Gio = group_pairs(ins, n_ins, true), Gii = group_pairs(ins, n_ins, false);
-
-    // ---- stage everything in one upload
+// Device layout of one update batch: the pair lists, their groupings by
+// source and by target, and the result words the checks write.
+struct Staged {
+    Groups G[4];  // deletions by source / by target, insertions by source / by target
     std::vector<size_t> at;
-    size_t words = 0;
+    size_t in_words = 0, words = 0;
+    size_t i_del = 0, i_ins = 0, i_grp[4][4] = {}, i_fdel = 0, i_fins = 0, i_ovo = 0, i_ovi = 0,
+           i_miss = 0;
+    uint32_t n_ins = 0, n_del = 0;
+};
+
+Staged layout_batch(const uint32_t* ins, uint32_t n_ins, const uint32_t* del, uint32_t n_del) {
+    Staged L;
+    L.n_ins = n_ins;
+    L.n_del = n_del;
+    L.G[0] = group_pairs(del, n_del, true);
+    L.G[1] = group_pairs(del, n_del, false);
+    L.G[2] = group_pairs(ins, n_ins, true);
+    L.G[3] = group_pairs(ins, n_ins, false);
     auto reserve = [&](size_t n) {
-        at.push_back(words);
-        words += n;
-        return at.size() - 1;
+        L.at.push_back(L.words);
+        L.words += n;
+        return L.at.size() - 1;
     };
-    const size_t i_del = reserve(2ull * n_del), i_ins = reserve(2ull * n_ins);
-    const Groups* gs[4] = {&Gdo, &Gdi, &Gio, &Gii};
-    size_t i_grp[4][4];
+    L.i_del = reserve(2ull * n_del);
+    L.i_ins = reserve(2ull * n_ins);
     for (int q = 0; q < 4; ++q) {
-        i_grp[q][0] = reserve(gs[q]->key.size());
-        i_grp[q][1] = reserve(gs[q]->start.size());
-        i_grp[q][2] = reserve(gs[q]->cnt.size());
-        i_grp[q][3] = reserve(gs[q]->val.size());
+        L.i_grp[q][0] = reserve(L.G[q].key.size());
+        L.i_grp[q][1] = reserve(L.G[q].start.size());
+        L.i_grp[q][2] = reserve(L.G[q].cnt.size());
+        L.i_grp[q][3] = reserve(L.G[q].val.size());
     }
-    const size_t in_words = words;
-    const size_t i_fdel = reserve(n_del), i_fins = reserve(n_ins);
-    const size_t i_ovo = reserve(Gio.key.size()), i_ovi = reserve(Gii.key.size());
-    const size_t i_miss = reserve(1);
-    ensure_staging(c, words);
-    uint32_t* H = c->upd_host;
+    L.in_words = L.words;
+    L.i_fdel = reserve(n_del);
+    L.i_fins = reserve(n_ins);
+    L.i_ovo = reserve(L.G[2].key.size());
+    L.i_ovi = reserve(L.G[3].key.size());
+    L.i_miss = reserve(1);
+    return L;
+}
+
+// Writes the batch's inputs (words [0, in_words)) into host memory H.
+void fill_batch(const Staged& L, uint32_t* H, const uint32_t* ins, const uint32_t* del) {
     auto put = [&](size_t idx, const uint32_t* src, size_t n) {
-        if (n) std::memcpy(H + at[idx], src, 4 * n);
+        if (n) std::memcpy(H + L.at[idx], src, 4 * n);
     };
-    put(i_del, del, 2ull * n_del);
-    put(i_ins, ins, 2ull * n_ins);
+    put(L.i_del, del, 2ull * L.n_del);
+    put(L.i_ins, ins, 2ull * L.n_ins);
     for (int q = 0; q < 4; ++q) {
-        put(i_grp[q][0], gs[q]->key.data(), gs[q]->key.size());
-        put(i_grp[q][1], gs[q]->start.data(), gs[q]->start.size());
-        put(i_grp[q][2], gs[q]->cnt.data(), gs[q]->cnt.size());
-        put(i_grp[q][3], gs[q]->val.data(), gs[q]->val.size());
+        put(L.i_grp[q][0], L.G[q].key.data(), L.G[q].key.size());
+        put(L.i_grp[q][1], L.G[q].start.data(), L.G[q].start.size());
+        put(L.i_grp[q][2], L.G[q].cnt.data(), L.G[q].cnt.size());
+        put(L.i_grp[q][3], L.G[q].val.data(), L.G[q].val.size());
     }
-    uint32_t* D = c->upd_dev;
-    auto dptr = [&](size_t idx) { return D + at[idx]; };
-    if (in_words)
-        HIPX(c, hipMemcpyAsync(D, H, 4 * in_words, hipMemcpyHostToDevice, c->stream));
-    HIPX(c, hipMemsetAsync(D + in_words, 0, 4 * (words - in_words), c->stream));
+}
+
+// Applies one validated-by-shape batch to the device graph: checks that
+// deleted edges exist and inserted ones do not (unless deleted first in
+// the same batch), moves lists that would overflow, then deletes and
+// inserts.  H/D: host (pinned) and device staging of L.words words whose
+// inputs are already on the device when `uploaded` (else uploaded here on
+// the library stream).  On a state error (NOT_FOUND, duplicate insert) the
+// graph is unchanged.  *m_del = M over the deletions with the kmaxes
+// resident before the batch (-1 if none or no kmaxes).
+dcx_status edit_batch(dcx_ctx* c, dcx_graph* g, const Staged& L, uint32_t* H, uint32_t* D,
+                      bool uploaded, const uint32_t* ins, const uint32_t* del, int32_t* m_del,
+                      uint32_t* relocated) {
+    const uint32_t n_ins = L.n_ins, n_del = L.n_del;
+    auto dptr = [&](size_t idx) { return D + L.at[idx]; };
+    if (!uploaded && L.in_words)
+        HIPX(c, hipMemcpyAsync(D, H, 4 * L.in_words, hipMemcpyHostToDevice, c->stream));
+    HIPX(c, hipMemsetAsync(D + L.in_words, 0, 4 * (L.words - L.in_words), c->stream));
     DevGroups dg[4];
     for (int q = 0; q < 4; ++q)
-        dg[q] = DevGroups{dptr(i_grp[q][0]), dptr(i_grp[q][1]), dptr(i_grp[q][2]),
-                          dptr(i_grp[q][3]), (uint32_t)gs[q]->key.size()};
+        dg[q] = DevGroups{dptr(L.i_grp[q][0]), dptr(L.i_grp[q][1]), dptr(L.i_grp[q][2]),
+                          dptr(L.i_grp[q][3]), (uint32_t)L.G[q].key.size()};
 
     // ---- existence: deletions must exist, insertions must not (unless the
     // same edge is deleted first in this batch)
-    HIPX(c, launch_find_edges(g->out_off, g->out_deg, g->out_nbr, dptr(i_del), n_del,
-                              dptr(i_fdel), c->stream));
-    HIPX(c, launch_find_edges(g->out_off, g->out_deg, g->out_nbr, dptr(i_ins), n_ins,
-                              dptr(i_fins), c->stream));
+    HIPX(c, launch_find_edges(g->out_off, g->out_deg, g->out_nbr, dptr(L.i_del), n_del,
+                              dptr(L.i_fdel), c->stream));
+    HIPX(c, launch_find_edges(g->out_off, g->out_deg, g->out_nbr, dptr(L.i_ins), n_ins,
+                              dptr(L.i_fins), c->stream));
     HIPX(c, launch_check_room(g->out_deg, g->out_capv, dg[2].key, dg[2].cnt, dg[2].n,
-                              dptr(i_ovo), c->stream));
-    HIPX(c, launch_check_room(g->in_deg, g->in_capv, dg[3].key, dg[3].cnt, dg[3].n, dptr(i_ovi),
-                              c->stream));
-    if (words > in_words)
-        HIPX(c, hipMemcpyAsync(H + in_words, D + in_words, 4 * (words - in_words),
+                              dptr(L.i_ovo), c->stream));
+    HIPX(c, launch_check_room(g->in_deg, g->in_capv, dg[3].key, dg[3].cnt, dg[3].n,
+                              dptr(L.i_ovi), c->stream));
+    if (L.words > L.in_words)
+        HIPX(c, hipMemcpyAsync(H + L.in_words, D + L.in_words, 4 * (L.words - L.in_words),
                                hipMemcpyDeviceToHost, c->stream));
     sync(c);
     for (uint32_t i = 0; i < n_del; ++i)
-        if (!H[at[i_fdel] + i]) {
+        if (!H[L.at[L.i_fdel] + i]) {
             set_err(c, "delete: edge %u->%u is not in the graph", del[2 * i], del[2 * i + 1]);
             return DCX_ERR_NOT_FOUND;
         }
@@ -263,7 +285,7 @@ dcx_status do_update(dcx_ctx* c, dcx_graph* g, const uint32_t* ins, uint32_t n_i
         for (uint32_t i = 0; i < n_del; ++i)
             deleted.insert((uint64_t)del[2 * i] << 32 | del[2 * i + 1]);
         for (uint32_t i = 0; i < n_ins; ++i)
-            if (H[at[i_fins] + i] &&
+            if (H[L.at[L.i_fins] + i] &&
                 !deleted.count((uint64_t)ins[2 * i] << 32 | ins[2 * i + 1])) {
                 set_err(c, "insert: edge %u->%u is already in the graph", ins[2 * i],
                         ins[2 * i + 1]);
@@ -272,62 +294,98 @@ dcx_status do_update(dcx_ctx* c, dcx_graph* g, const uint32_t* ins, uint32_t n_i
     }
 
     // ---- M over deletions with the old kmaxes (before anything changes)
-    const bool have_k = maintain && g->kmax_valid;
-    int32_t m_del = have_k ? batch_mvalue(c, g->kmaxes, dptr(i_del), n_del) : -1;
-    S.kmax_before = g->kmax_valid ? g->kmax : -1;
+    *m_del = g->kmax_valid ? batch_mvalue(c, g->kmaxes, dptr(L.i_del), n_del) : -1;
 
     // ---- structural update
-    std::vector<uint32_t> ovo(H + at[i_ovo], H + at[i_ovo] + Gio.key.size());
-    std::vector<uint32_t> ovi(H + at[i_ovi], H + at[i_ovi] + Gii.key.size());
-    S.relocated = relocate_side(c, g, true, Gio, ovo.data());
-    S.relocated += relocate_side(c, g, false, Gii, ovi.data());
-    g->relocations += S.relocated;
+    std::vector<uint32_t> ovo(H + L.at[L.i_ovo], H + L.at[L.i_ovo] + L.G[2].key.size());
+    std::vector<uint32_t> ovi(H + L.at[L.i_ovi], H + L.at[L.i_ovi] + L.G[3].key.size());
+    uint32_t rel = relocate_side(c, g, true, L.G[2], ovo.data());
+    rel += relocate_side(c, g, false, L.G[3], ovi.data());
+    g->relocations += rel;
+    if (relocated) *relocated = rel;
     HIPX(c, launch_apply_deletes(g->out_off, g->out_deg, g->out_nbr, dg[0].key, dg[0].start,
-                                 dg[0].cnt, dg[0].val, dg[0].n, dptr(i_miss), c->stream));
+                                 dg[0].cnt, dg[0].val, dg[0].n, dptr(L.i_miss), c->stream));
     HIPX(c, launch_apply_deletes(g->in_off, g->in_deg, g->in_nbr, dg[1].key, dg[1].start,
-                                 dg[1].cnt, dg[1].val, dg[1].n, dptr(i_miss), c->stream));
+                                 dg[1].cnt, dg[1].val, dg[1].n, dptr(L.i_miss), c->stream));
     HIPX(c, launch_apply_inserts(g->out_off, g->out_deg, g->out_nbr, dg[2].key, dg[2].start,
                                  dg[2].cnt, dg[2].val, dg[2].n, c->stream));
     HIPX(c, launch_apply_inserts(g->in_off, g->in_deg, g->in_nbr, dg[3].key, dg[3].start,
                                  dg[3].cnt, dg[3].val, dg[3].n, c->stream));
-    HIPX(c, hipMemcpyAsync(H + at[i_miss], dptr(i_miss), 4, hipMemcpyDeviceToHost, c->stream));
+    HIPX(c, hipMemcpyAsync(H + L.at[L.i_miss], dptr(L.i_miss), 4, hipMemcpyDeviceToHost,
+                           c->stream));
     sync(c);
-    g->E = g->E + n_ins - n_del;
-    if (H[at[i_miss]]) {
-        set_err(c, "delete: %u edges vanished between check and apply", H[at[i_miss]]);
+    if (H[L.at[L.i_miss]]) {
+        g->broken = true;
+        g->kmax_valid = g->rows_valid = false;
+        set_err(c, "delete: %u edges vanished between check and apply", H[L.at[L.i_miss]]);
         throw Fail{DCX_ERR_INTERNAL};
     }
+    g->E = g->E + n_ins - n_del;
+    return DCX_OK;
+}
+
+// Brings kmaxes and the rows back after edits: `ins` (host, n_ins pairs)
+// are all edges inserted since the decomposition was last valid, m_del the
+// largest M of the deletions (old kmaxes), deletion_only whether nothing
+// was inserted.  Fills the maintenance fields of *S.
+void maintain(dcx_ctx* c, dcx_graph* g, const uint32_t* ins, uint32_t n_ins, int32_t m_del,
+              bool changed, bool deletion_only, dcx_update_stats* S) {
+    const auto t1 = Clock::now();
+    const bool rows = g->rows_valid;
+    const int var = c->cfg.variant == DCX_VARIANT_AUTO ? DCX_VARIANT_BATCHED : c->cfg.variant;
+    // deletions alone can only lower kmaxes: the old ones are an upper
+    // bound and the h-index fixpoint starts from them
+    if (changed) compute_kmax(c, g, var, deletion_only);
+    S->m_ins = -1;
+    if (n_ins) {
+        ensure_staging(c, 2ull * n_ins);
+        std::memcpy(c->upd_host, ins, 8ull * n_ins);
+        HIPX(c, hipMemcpyAsync(c->upd_dev, c->upd_host, 8ull * n_ins, hipMemcpyHostToDevice,
+                               c->stream));
+        S->m_ins = batch_mvalue(c, g->kmaxes, c->upd_dev, n_ins);
+    }
+    S->kmax_ms = ms_since(t1);
+    const int32_t R = std::max(m_del, S->m_ins);
+    S->rows_recomputed = 0;
+    if (rows) {
+        const auto t2 = Clock::now();
+        if (R >= 0) {
+            const int32_t kr = std::min(R, g->kmax);
+            compute_rows(c, g, var, kr);
+            S->rows_recomputed = kr + 1;
+        }
+        S->rows_ms = ms_since(t2);
+    }
+}
+
+dcx_status do_update(dcx_ctx* c, dcx_graph* g, const uint32_t* ins, uint32_t n_ins,
+                     const uint32_t* del, uint32_t n_del, bool maintain_values,
+                     dcx_update_stats* st) {
+    const auto t0 = Clock::now();
+    dcx_update_stats S{};
+    const uint32_t V = g->V;
+    dcx_status s = validate_pairs(c, V, del, n_del, "delete");
+    if (s != DCX_OK) return s;
+    s = validate_pairs(c, V, ins, n_ins, "insert");
+    if (s != DCX_OK) return s;
+    if ((uint64_t)n_ins > g->E + (uint64_t)n_del + 0xffffffffull) return DCX_ERR_CAPACITY;
+    const Staged L = layout_batch(ins, n_ins, del, n_del);
+    ensure_staging(c, L.words);
+    fill_batch(L, c->upd_host, ins, del);
+    S.kmax_before = g->kmax_valid ? g->kmax : -1;
+    int32_t m_del = -1;
+    s = edit_batch(c, g, L, c->upd_host, c->upd_dev, false, ins, del, &m_del, &S.relocated);
+    if (s != DCX_OK) return s;
     S.inserted = n_ins;
     S.deleted = n_del;
     S.insert_ms = ms_since(t0);
-
-    // ---- bring the decomposition back
     S.m_del = m_del;
     S.m_ins = -1;
-    S.rows_recomputed = 0;
-    if (!have_k) {
+    if (!(maintain_values && g->kmax_valid)) {
         g->kmax_valid = false;
         g->rows_valid = false;
     } else {
-        const auto t1 = Clock::now();
-        const bool rows = g->rows_valid;
-        const int var =
-            c->cfg.variant == DCX_VARIANT_AUTO ? DCX_VARIANT_BATCHED : c->cfg.variant;
-        // deletions alone can only lower kmaxes: the old ones are an upper
-        // bound and the h-index fixpoint starts from them
-        if (n_ins + n_del > 0) compute_kmax(c, g, var, n_ins == 0);
-        S.m_ins = batch_mvalue(c, g->kmaxes, dptr(i_ins), n_ins);
-        S.kmax_ms = ms_since(t1);
-        const int32_t R = std::max(m_del, S.m_ins);
-        if (rows) {
-            const auto t2 = Clock::now();
-            if (R >= 0) {
-                const int32_t kr = std::min(R, g->kmax);
-                compute_rows(c, g, var, kr);
-                S.rows_recomputed = kr + 1;
-            }
-            S.rows_ms = ms_since(t2);
-        }
+        maintain(c, g, ins, n_ins, m_del, n_ins + n_del > 0, n_ins == 0, &S);
     }
     S.kmax_after = g->kmax_valid ? g->kmax : -1;
     S.total_ms = ms_since(t0);

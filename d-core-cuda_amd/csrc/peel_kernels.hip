@@ -317,6 +317,8 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
                 T.klen[tid] = kl;
                 if (tid == 0) T.nitems = min(nF - base, ipw);
                 T.scan[tid] = block_excl_scan<BLK>(kl + cl, T.wsum, &T.total);
+                // every thread's prefix is in LDS before any thread searches it
+                __syncthreads();
                 const uint32_t total = T.total;
                 // kUnroll independent edges per thread per step, so their
                 // loads and atomics overlap instead of forming one chain.

@@ -54,6 +54,35 @@ class PhaseTrace(ctypes.Structure):
                 ("us", ctypes.c_double)]
 
 
+class TuneResult(ctypes.Structure):
+    _fields_ = [("rows_slots", ctypes.c_int32), ("team_max", ctypes.c_int32),
+                ("shrink_at", ctypes.c_int32), ("candidates", ctypes.c_int32),
+                ("best_ms", ctypes.c_double), ("default_ms", ctypes.c_double),
+                ("reserved", ctypes.c_double * 4)]
+
+
+class ServeConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget", ctypes.c_uint64), ("max_inflight", ctypes.c_uint32),
+                ("max_group", ctypes.c_uint32), ("max_group_edges", ctypes.c_uint32),
+                ("hash_states", ctypes.c_int32), ("reserved", ctypes.c_uint64 * 4)]
+
+
+class ServeResult(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_uint64), ("status", ctypes.c_int32), ("group", ctypes.c_uint32),
+                ("inserted", ctypes.c_uint32), ("deleted", ctypes.c_uint32),
+                ("kmax_after", ctypes.c_int32), ("rows_recomputed", ctypes.c_int32),
+                ("submit_ms", ctypes.c_double), ("admit_ms", ctypes.c_double),
+                ("start_ms", ctypes.c_double), ("done_ms", ctypes.c_double),
+                ("state_hash", ctypes.c_uint64)]
+
+
+class ServeStats(ctypes.Structure):
+    _fields_ = [("requests", ctypes.c_uint64), ("groups", ctypes.c_uint64),
+                ("max_group", ctypes.c_uint32), ("max_waiting", ctypes.c_uint32),
+                ("busy_ms", ctypes.c_double), ("elapsed_ms", ctypes.c_double),
+                ("reserved", ctypes.c_uint64 * 4)]
+
+
 class UpdateStats(ctypes.Structure):
     _fields_ = [("inserted", ctypes.c_uint32), ("deleted", ctypes.c_uint32),
                 ("relocated", ctypes.c_uint32), ("m_del", ctypes.c_int32),
@@ -122,6 +151,25 @@ def gen_chunglu(V, samples, seed, gamma=2.1):
     check(lib().dcx_gen_chunglu(u32(V), u64(samples), f64(gamma), u64(seed), P(src), P(dst),
                                 u64(samples), ctypes.byref(E)), "dcx_gen_chunglu")
     return src[:E.value].copy(), dst[:E.value].copy()
+
+
+def gen_update_stream(V, src, dst, scale, nbatches, max_batch=4096, p_insert=0.7, seed=3,
+                      a=0.57, b=0.19, c=0.19):
+    """Update batches against the R-MAT graph (V=2^scale, src, dst):
+    returns (ins_off, ins_pairs, del_off, del_pairs), batch b's inserts being
+    ins_pairs[ins_off[b]:ins_off[b+1]] as (n, 2) rows."""
+    src = np.ascontiguousarray(src, np.uint32)
+    dst = np.ascontiguousarray(dst, np.uint32)
+    io = np.zeros(nbatches + 1, np.uint64)
+    do = np.zeros(nbatches + 1, np.uint64)
+    args = (u32(V), u64(src.size), P(src), P(dst), u32(scale), f64(a), f64(b), f64(c),
+            u32(nbatches), u32(max_batch), f64(p_insert), u64(seed), P(io), P(do))
+    check(lib().dcx_gen_update_stream(*args, None, u64(0), None, u64(0)), "dcx_gen_update_stream")
+    ip = np.empty((int(io[-1]), 2), np.uint32)
+    dp = np.empty((int(do[-1]), 2), np.uint32)
+    check(lib().dcx_gen_update_stream(*args, P(ip), u64(ip.size), P(dp), u64(dp.size)),
+          "dcx_gen_update_stream")
+    return io.astype(np.int64), ip, do.astype(np.int64), dp
 
 
 def csr_build(V, src, dst, slack=0):
@@ -387,6 +435,14 @@ class Graph:
               self.ctx.h)
         return rp, vals[:n.value]
 
+    def autotune(self, reps=2):
+        """Tunes the batched rows launch on this graph (dcx_autotune); the
+        context keeps the winning shape.  Returns TuneResult."""
+        r = TuneResult()
+        check(lib().dcx_autotune(self.ctx.h, self.h, i32(reps), ctypes.byref(r)), "dcx_autotune",
+              self.ctx.h)
+        return r
+
     def pass_log(self):
         """Device log of the resident decomposition's passes (dcx_graph_pass_log):
         a list of dicts, entry 0 the kmax pass, entry 1+k row k."""
@@ -397,6 +453,17 @@ class Graph:
         check(lib().dcx_graph_pass_log(self.ctx.h, self.h, arr, u32(n.value), ctypes.byref(n)),
               "dcx_graph_pass_log", self.ctx.h)
         return [{f: getattr(arr[i], f) for f, _ in PassLog._fields_} for i in range(n.value)]
+
+    def state_hash(self):
+        """dcx_state_hash of the resident decomposition."""
+        h = ctypes.c_uint64()
+        check(lib().dcx_state_hash(self.ctx.h, self.h, ctypes.byref(h)), "dcx_state_hash",
+              self.ctx.h)
+        return h.value
+
+    def serve(self, **kw):
+        """Starts a maintenance server on this graph (dcx_serve_start)."""
+        return Server(self, **kw)
 
     def batch_summary(self, vbase):
         """Per-part (kmax, lmax) arrays of a decomposed batch."""
@@ -478,3 +545,80 @@ def compact_csr(g):
         idx = np.repeat(off[:-1], deg) + (np.arange(int(coff[-1])) - np.repeat(coff[:-1].astype(np.int64), deg))
         res += [coff, g[side + "_nbr"][idx].astype(np.uint32)]
     return res
+
+
+class Server:
+    """Continuous-batching maintenance server (dcx_serve_*).  While it runs,
+    the graph refuses other calls; stop() drains every submitted request."""
+
+    def __init__(self, graph, hbm_budget=0, max_inflight=0, max_group=0, max_group_edges=0,
+                 hash_states=False):
+        cfg = ServeConfig()
+        check(lib().dcx_serve_config_init(ctypes.byref(cfg)), "dcx_serve_config_init")
+        if hbm_budget:
+            cfg.hbm_budget = hbm_budget
+        if max_inflight:
+            cfg.max_inflight = max_inflight
+        if max_group:
+            cfg.max_group = max_group
+        if max_group_edges:
+            cfg.max_group_edges = max_group_edges
+        cfg.hash_states = 1 if hash_states else 0
+        self.graph = graph
+        self.h = ctypes.c_void_p()
+        check(lib().dcx_serve_start(graph.ctx.h, graph.h, ctypes.byref(cfg), ctypes.byref(self.h)),
+              "dcx_serve_start", graph.ctx.h)
+        lib().dcx_serve_last_error.restype = ctypes.c_char_p
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        if self.h:
+            try:
+                self.stop()
+            except DcxError:
+                if exc[0] is None:
+                    raise
+        return False
+
+    def __del__(self):
+        # never leave the worker thread running into interpreter shutdown
+        if getattr(self, "h", None):
+            try:
+                self.stop()
+            except Exception:
+                pass
+
+    def _check(self, status, where):
+        if status != 0:
+            detail = lib().dcx_serve_last_error(self.h).decode(errors="replace")
+            raise DcxError(status, where, detail)
+
+    def submit(self, ins=None, dels=None):
+        """Queues one request; returns its id."""
+        ins = _pairs(ins)
+        dels = _pairs(dels)
+        rid = ctypes.c_uint64()
+        self._check(lib().dcx_serve_submit(self.h, P(ins), u32(ins.size // 2), P(dels),
+                                           u32(dels.size // 2), ctypes.byref(rid)),
+                    "dcx_serve_submit")
+        return rid.value
+
+    def poll(self, cap=4096, wait_ms=0):
+        """Finished results (list of dicts), waiting up to wait_ms for one."""
+        arr = (ServeResult * cap)()
+        n = ctypes.c_uint32()
+        self._check(lib().dcx_serve_poll(self.h, arr, u32(cap), ctypes.byref(n), i32(wait_ms)),
+                    "dcx_serve_poll")
+        return [{f: getattr(arr[i], f) for f, _ in ServeResult._fields_} for i in range(n.value)]
+
+    def stop(self):
+        """Drains, stops and frees the server; returns its stats as a dict."""
+        st = ServeStats()
+        detail = lib().dcx_serve_last_error(self.h).decode(errors="replace")
+        h, self.h = self.h, None
+        status = lib().dcx_serve_stop(h, ctypes.byref(st))
+        if status != 0:
+            raise DcxError(status, "dcx_serve_stop", detail)
+        return {f: getattr(st, f) for f, _ in ServeStats._fields_ if f != "reserved"}

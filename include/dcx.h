@@ -64,6 +64,22 @@ dcx_status dcx_gen_rmat(uint32_t scale, uint64_t samples, double a, double b, do
 dcx_status dcx_gen_chunglu(uint32_t V, uint64_t samples, double gamma, uint64_t seed,
                            uint32_t* src, uint32_t* dst, uint64_t cap, uint64_t* E_out);
 
+/* Seeded stream of update batches against an R-MAT graph of V = 2^scale
+ * vertices with edges (src, dst): batch b has floor(exp(U ln max_batch))
+ * edits (clipped to [1, max_batch]), each an insert with probability
+ * p_insert (an R-MAT(a,b,c) sample not in the graph) or else a delete (a
+ * uniformly chosen current edge); the graph evolves batch by batch.  The
+ * edits of batch b are ins_pairs[2*ins_off[b] .. 2*ins_off[b+1]) and the
+ * same for deletions.  With ins_pairs and del_pairs NULL only the offsets
+ * are filled (sizing).  Replaces the one-edge samplers getRandomInsertEdge /
+ * getRandomDeleteEdge (src/graph.cpp:110-144) for batched streams. */
+dcx_status dcx_gen_update_stream(uint32_t V, uint64_t E, const uint32_t* src,
+                                 const uint32_t* dst, uint32_t scale, double a, double b,
+                                 double c, uint32_t nbatches, uint32_t max_batch,
+                                 double p_insert, uint64_t seed, uint64_t* ins_off,
+                                 uint64_t* del_off, uint32_t* ins_pairs, uint64_t ins_cap,
+                                 uint32_t* del_pairs, uint64_t del_cap);
+
 /* Edge list -> in/out CSR with `slack` free slots per vertex:
  * off[v+1]-off[v] = deg(v)+slack, neighbours in input order, free slots
  * 0xFFFFFFFF.  Arrays: off[V+1], deg[V], nbr[E + slack*V].
@@ -250,6 +266,84 @@ dcx_status dcx_graph_upload_batch(dcx_ctx* ctx, uint32_t n, const uint32_t* V, c
 /* Per-graph kmax and lmax of a decomposed batch (-1 / 0 for an empty graph). */
 dcx_status dcx_batch_summary(dcx_ctx* ctx, const dcx_graph* g, uint32_t n, const uint32_t* vbase,
                              int32_t* kmax, int32_t* lmax);
+
+/* ---- continuous-batching maintenance server ---------------------------
+ * Replaces the reference's one-batch-at-a-time maintenance loop
+ * (maintenanceFast, main.cu:1665-1751, driven from main.cu:1905-1924) with
+ * a server that owns one decomposed graph and applies a stream of update
+ * requests.  Admission: a request is staged in a fixed HBM pool
+ * (hbm_budget) and uploaded on a copy stream as soon as it fits and an
+ * in-flight slot is free, overlapping the maintenance running on the
+ * device.  The worker applies every admitted request's edits in arrival
+ * order (each checked against the graph as it then is, so each request's
+ * status is what it would get alone) and then maintains kmaxes and rows
+ * once for the group.  While served, the graph refuses other API calls
+ * (DCX_ERR_STATE); results not polled before dcx_serve_stop are dropped. */
+typedef struct dcx_server dcx_server;
+typedef struct dcx_serve_config {
+    uint64_t hbm_budget;       /* bytes of device staging for admitted requests (64 MiB) */
+    uint32_t max_inflight;     /* admitted requests at most (4096) */
+    uint32_t max_group;        /* requests maintained together at most (256; 1 = one by one) */
+    uint32_t max_group_edges;  /* edits per group at most, unless one request has more (65536) */
+    int32_t hash_states;       /* 1: each result carries dcx_state_hash after its group */
+    uint64_t reserved[4];
+} dcx_serve_config;
+typedef struct dcx_serve_result {
+    uint64_t id;               /* from dcx_serve_submit */
+    int32_t status;            /* DCX_OK, or the error the request alone would get */
+    uint32_t group;            /* maintenance group that applied it (0: rejected at submit) */
+    uint32_t inserted, deleted;
+    int32_t kmax_after;        /* kmax after its group */
+    int32_t rows_recomputed;   /* rows its group recomputed */
+    double submit_ms, admit_ms, start_ms, done_ms;  /* server clock (ms since start) */
+    uint64_t state_hash;       /* after its group, if hash_states */
+} dcx_serve_result;
+typedef struct dcx_serve_stats {
+    uint64_t requests, groups;
+    uint32_t max_group, max_waiting;
+    double busy_ms, elapsed_ms;
+    uint64_t reserved[4];
+} dcx_serve_stats;
+dcx_status dcx_serve_config_init(dcx_serve_config* cfg);
+/* Starts a server on a decomposed graph (NULL cfg = defaults). */
+dcx_status dcx_serve_start(dcx_ctx* ctx, dcx_graph* g, const dcx_serve_config* cfg,
+                           dcx_server** out);
+/* Queues one update request (deletions then insertions, the dcx_update
+ * contract); never waits for the device.  DCX_ERR_BUDGET: the request
+ * alone exceeds hbm_budget. */
+dcx_status dcx_serve_submit(dcx_server* sv, const uint32_t* ins, uint32_t n_ins,
+                            const uint32_t* del, uint32_t n_del, uint64_t* id);
+/* Moves up to cap finished results to out (waiting up to wait_ms for one). */
+dcx_status dcx_serve_poll(dcx_server* sv, dcx_serve_result* out, uint32_t cap, uint32_t* n,
+                          int32_t wait_ms);
+/* Detail of the server's first failure ("" if none).  After a failure the
+ * server applies nothing more: later requests complete with that status. */
+const char* dcx_serve_last_error(dcx_server* sv);
+/* Drains every submitted request, stops and frees the server. */
+dcx_status dcx_serve_stop(dcx_server* sv, dcx_serve_stats* stats);
+/* Hash of the resident decomposition: the 64-bit sum over v and k = -1..kmax
+ * of mix64(v * 0x9E3779B97F4A7C15 + (k + 1) * 0xC2B2AE3D27D4EB4F + value),
+ * value = kmaxes[v] for k = -1 and row k otherwise (mix64 = the SplitMix64
+ * finaliser).  Independent of evaluation order; no reference counterpart
+ * (the reference compares whole arrays on the host, src/utils.h:13-43). */
+dcx_status dcx_state_hash(dcx_ctx* ctx, dcx_graph* g, uint64_t* hash);
+
+/* Autotuner for the batched rows launch (no reference counterpart: the
+ * reference's launch shape is fixed, src/common.h:38-56).  Times the rows
+ * launch on `g` for candidate shapes (workgroups, team_max, shrink_at) by
+ * coordinate descent, best of `reps` runs each (0 = 2), and leaves the
+ * fastest configured in ctx with g's rows computed by it.  Every shape
+ * gives identical results; only the time differs. */
+typedef struct dcx_tune_result {
+    int32_t rows_slots;   /* chosen workgroups of the launch */
+    int32_t team_max;     /* chosen team cap */
+    int32_t shrink_at;    /* chosen narrowing threshold */
+    int32_t candidates;   /* shapes timed */
+    double best_ms;       /* rows kernel time of the chosen shape */
+    double default_ms;    /* rows kernel time of the default shape */
+    double reserved[4];
+} dcx_tune_result;
+dcx_status dcx_autotune(dcx_ctx* ctx, dcx_graph* g, int32_t reps, dcx_tune_result* out);
 
 /* ------------------------------------------------------------------ */
 /* Edge updates and maintenance.                                      */

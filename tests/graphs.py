@@ -218,3 +218,30 @@ def check_invariants(V, src, dst, kmaxes, rows):
 
 def product_pairs(it):
     return list(itertools.product(*it))
+
+
+# ---- state hash (restates dcx_state_hash, include/dcx.h) ----------------
+
+_G1 = np.uint64(0x9E3779B97F4A7C15)
+_G2 = np.uint64(0xC2B2AE3D27D4EB4F)
+
+
+def _mix64(z):
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return z ^ (z >> np.uint64(31))
+
+
+def state_hash(kmaxes, rows):
+    """Sum over v and k = -1..kmax of mix64(v*G1 + (k+1)*G2 + value) mod
+    2^64, value = kmaxes[v] for k = -1, rows[k][v] otherwise."""
+    V = len(kmaxes)
+    v = np.arange(V, dtype=np.uint64)
+    total = np.uint64(0)
+    with np.errstate(over="ignore"):
+        layers = [np.asarray(kmaxes)] + [np.asarray(r) for r in rows]
+        for kk, vals in enumerate(layers):
+            x = np.asarray(vals, dtype=np.int64).astype(np.uint32).astype(np.uint64)
+            t = _mix64(v * _G1 + np.uint64(kk) * _G2 + x)
+            total = total + np.sum(t, dtype=np.uint64)
+    return int(total)

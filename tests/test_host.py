@@ -249,3 +249,60 @@ def test_no_device_is_reported():
 ER16K_SEED1 = "317d929d8c944663"
 RMAT12_SEED3 = "b48e861089c2e588"
 CL5K_SEED1001 = "22a21e4350670d46"
+
+
+def test_chunglu_batch_sizes_pinned():
+    """The batched workload's sizes come from SplitMix64(1000+i) draws:
+    log-uniform in [1e4, 2e6], V = ceil(m/8)."""
+    import math
+    import workloads
+    s = workloads.chunglu_sizes()
+    assert len(s) == 256
+    assert s[:4] == [(4338, 34704), (7001, 56003), (175766, 1406123), (30654, 245228)]
+    assert all(1e4 <= m <= 2e6 and V == math.ceil(m / 8) for V, m in s)
+    assert sum(m for _, m in s) == 87328167
+    # the textbook SplitMix64 first output for state 0
+    assert workloads.splitmix64(0)[1] == 0xE220A8397B1DCDAF
+
+
+def test_update_stream_is_valid_and_seeded():
+    """Every batch of the serving stream is applicable to the graph the
+    previous batches produced: deletes name current edges, inserts new
+    ones, no self-loops or repeats, sizes in [1, max_batch]."""
+    scale = 10
+    V, src, dst = 1 << scale, *dcx.gen_rmat(scale, 16 << scale, 2)
+    io, ip, do, dp = dcx.gen_update_stream(V, src, dst, scale, 300, max_batch=64, seed=3)
+    io2, ip2, do2, dp2 = dcx.gen_update_stream(V, src, dst, scale, 300, max_batch=64, seed=3)
+    assert np.array_equal(ip, ip2) and np.array_equal(dp, dp2)
+    cur = set(zip(src.tolist(), dst.tolist()))
+    n_ins = n_del = 0
+    for b in range(300):
+        ins = [tuple(x) for x in ip[io[b]:io[b + 1]].tolist()]
+        dels = [tuple(x) for x in dp[do[b]:do[b + 1]].tolist()]
+        assert 1 <= len(ins) + len(dels) <= 64
+        assert len(set(ins)) == len(ins) and len(set(dels)) == len(dels)
+        assert all(d in cur for d in dels)
+        assert all(i not in cur and i[0] != i[1] and max(i) < V for i in ins)
+        cur.difference_update(dels)
+        cur.update(ins)
+        n_ins += len(ins)
+        n_del += len(dels)
+    frac = n_ins / (n_ins + n_del)
+    assert 0.64 < frac < 0.76, frac
+    # sizing call alone gives the same offsets
+    io3, _, do3, _ = dcx.gen_update_stream(V, src, dst, scale, 300, max_batch=64, seed=3)
+    assert np.array_equal(io3, io) and np.array_equal(do3, do)
+
+
+def test_state_hash_restatement_on_fixture():
+    """The hash restatement is order independent and sees every cell."""
+    import graphs
+    V, s, d = dcx.text_read(os.path.join(GOLDEN, "digraph"), prune=False)
+    K0, L0, km0, rows0 = dcx.decomp_read(os.path.join(GOLDEN, "digraph.decomp"), V)
+    h = graphs.state_hash(km0, rows0)
+    rows1 = rows0.copy()
+    rows1[3, 0] += 1
+    assert graphs.state_hash(km0, rows1) != h
+    km1 = km0.copy()
+    km1[[0, 1]] = km1[[1, 0]]
+    assert graphs.state_hash(km1, rows0) != h
