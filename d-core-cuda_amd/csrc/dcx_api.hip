@@ -228,12 +228,14 @@ void compute_kmax(dcx_ctx* c, dcx_graph* g, int variant = DCX_VARIANT_BATCHED,
 
 // Static schedule of the passes k = 0..kr onto teams of workgroups: a pass
 // whose work (2 E_k + V_k) exceeds the per-workgroup share gets its own
-// team sized to it (up to team_max, using at most 3/4 of the grid); the
-// remaining passes go to single-workgroup teams, longest first, each to the
-// least loaded team.
+// team sized to it (up to team_max and to one XCD's workgroups, using at
+// most 3/4 of the grid), placed on the XCD with the most free workgroups;
+// the remaining passes go to single-workgroup teams on the free
+// workgroups, longest first, each to the least loaded team.  Team scratch
+// regions start on 128-byte lines.
 struct Sched {
     std::vector<TeamDesc> teams;
-    std::vector<uint32_t> block_team;
+    std::vector<uint32_t> block_map;  // [G] team of each block (~0u idle), [G] rank
     std::vector<int32_t> ks;
     uint64_t scratch_words = 0;
     uint32_t sync_stride = 0;
@@ -254,28 +256,48 @@ Sched make_schedule(const dcx_graph* g, int32_t kr, uint32_t G, uint32_t team_ma
     std::vector<uint32_t> idx(np);
     for (uint32_t k = 0; k < np; ++k) idx[k] = k;
     std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return w[a] > w[b]; });
+    // free workgroups of each XCD, in increasing block order
+    std::vector<std::vector<uint32_t>> xfree(kXcds);
+    for (uint32_t b = G; b-- > 0;) xfree[b % kXcds].push_back(b);  // back() = lowest
+    const uint32_t per_xcd = G / kXcds;
     Sched s;
-    s.sync_stride = sync_words(team_max);
+    s.sync_stride = (sync_words(team_max) + kSlotStride - 1) / kSlotStride * kSlotStride;
+    s.block_map.assign(2ull * G, ~0u);
     const uint32_t big_budget = G * 3 / 4;
     uint32_t used = 0;
     std::vector<uint32_t> small;
+    std::vector<std::vector<uint32_t>> team_blocks;
     for (uint32_t k : idx) {
-        const uint32_t t = (uint32_t)std::min<double>(team_max, w[k] / share);
+        uint32_t t = (uint32_t)std::min<double>(std::min(team_max, per_xcd), w[k] / share);
+        uint32_t x = 0;
+        for (uint32_t y = 1; y < kXcds; ++y)
+            if (xfree[y].size() > xfree[x].size()) x = y;
+        t = std::min<uint32_t>(t, (uint32_t)xfree[x].size());
         if (t >= 2 && used + t <= big_budget) {
             TeamDesc td{};
-            td.first_block = used;
+            td.xcd = x;
             td.size = t;
             td.pass_begin = (uint32_t)s.ks.size();
             s.ks.push_back((int32_t)k);
             td.pass_end = (uint32_t)s.ks.size();
             td.scratch_cap = std::max<uint32_t>(g->h_vk[k], 1);
+            std::vector<uint32_t> blocks;
+            for (uint32_t r = 0; r < t; ++r) {
+                blocks.push_back(xfree[x].back());
+                xfree[x].pop_back();
+            }
+            team_blocks.push_back(std::move(blocks));
             s.teams.push_back(td);
             used += t;
         } else {
             small.push_back(k);
         }
     }
-    const uint32_t nsingle = std::min<uint32_t>(G - used, (uint32_t)small.size());
+    std::vector<uint32_t> rest;
+    for (uint32_t x = 0; x < kXcds; ++x)
+        for (uint32_t b : xfree[x]) rest.push_back(b);
+    std::sort(rest.begin(), rest.end());
+    const uint32_t nsingle = std::min<uint32_t>((uint32_t)rest.size(), (uint32_t)small.size());
     std::vector<std::vector<int32_t>> lists(nsingle);
     std::vector<double> load(nsingle, 0.0);
     for (uint32_t k : small) {  // already longest first
@@ -285,7 +307,7 @@ Sched make_schedule(const dcx_graph* g, int32_t kr, uint32_t G, uint32_t team_ma
     }
     for (uint32_t t = 0; t < nsingle; ++t) {
         TeamDesc td{};
-        td.first_block = used + t;
+        td.xcd = rest[t] % kXcds;
         td.size = 1;
         td.pass_begin = (uint32_t)s.ks.size();
         uint32_t capv = 1;
@@ -296,18 +318,61 @@ Sched make_schedule(const dcx_graph* g, int32_t kr, uint32_t G, uint32_t team_ma
         td.pass_end = (uint32_t)s.ks.size();
         td.scratch_cap = capv;
         s.teams.push_back(td);
+        team_blocks.push_back({rest[t]});
     }
     uint64_t off = 0;
     for (uint32_t t = 0; t < s.teams.size(); ++t) {
         TeamDesc& td = s.teams[t];
+        td.scratch_cap = (td.scratch_cap + kSlotStride - 1) / kSlotStride * kSlotStride;
         td.scratch_off = off;
         off += (uint64_t)kScratchArrays * td.scratch_cap;
         td.sync_off = t * s.sync_stride;
-        td.slot_off = t * 12;
-        for (uint32_t b = 0; b < td.size; ++b) s.block_team.push_back(t);
+        td.slot_off = t * kSlotWords;
+        for (uint32_t r = 0; r < team_blocks[t].size(); ++r) {
+            s.block_map[team_blocks[t][r]] = t;
+            s.block_map[G + team_blocks[t][r]] = r;
+        }
     }
     s.scratch_words = off;
     return s;
+}
+
+// Appends the device log of the failed row passes (k <= kr) to the error
+// text: which pass, on how many workgroups, and where its counters stood.
+void describe_failed_passes(dcx_ctx* c, const dcx_graph* g, int32_t kr) {
+    std::vector<uint32_t> st((size_t)(kr + 2) * kStatWords);
+    if (hipMemcpy(st.data(), g->pstats, st.size() * 4, hipMemcpyDeviceToHost) != hipSuccess)
+        return;
+    std::string d = c->err;
+    int shown = 0;
+    for (int32_t k = 0; k <= kr && shown < 4; ++k) {
+        const uint32_t* w = st.data() + (size_t)(k + 1) * kStatWords;
+        if (!w[kStatFailed]) continue;
+        char buf[256];
+        std::snprintf(buf, sizeof buf,
+                      "; row %d failed: team %u, members %u, removed %u, alive %u, level %u, "
+                      "levels %u, phases %u",
+                      k, (unsigned)g->h_pass_team[k + 1], g->h_vk[k], w[kStatRemoved],
+                      w[kStatAlive], w[kStatLevel], w[kStatLevels], w[kStatPhases]);
+        d += buf;
+        ++shown;
+    }
+    c->err = d;
+}
+
+// Appends the record the first failing pass member wrote.
+void describe_fail_record(dcx_ctx* c) {
+    uint32_t r[kFailWords] = {};
+    if (!c->fail_rec ||
+        hipMemcpy(r, c->fail_rec, sizeof r, hipMemcpyDeviceToHost) != hipSuccess || !r[0])
+        return;
+    char buf[400];
+    std::snprintf(buf, sizeof buf,
+                  "; first failure: k %u member %u/%u (rank %u) phase %u levels %u removed %u "
+                  "of %u alive %u min %u level %d epoch %u leader-phase %u release %u nblocks %u",
+                  r[10], r[1], r[2], r[11], r[3], r[4], r[5], r[6], r[7], r[8], (int32_t)r[9],
+                  r[12], r[13], r[14], r[15]);
+    c->err += buf;
 }
 
 // Computes rows k = 0..kr (kr < 0: all rows 0..kmax) with the given
@@ -333,9 +398,12 @@ void compute_rows(dcx_ctx* c, dcx_graph* g, int variant, int32_t kr = -1) {
     uint32_t nteam_err = 0, team_err_stride = 0;
     const uint32_t* team_err = nullptr;
     clear_kernel_errors(c);
+    if (!c->fail_rec) dev_alloc(c, &c->fail_rec, kFailWords);
+    HIPX(c, hipMemsetAsync(c->fail_rec, 0, kFailWords * 4, c->stream));
     HIPX(c, hipEventRecord(c->ev[3], c->stream));
     HIPX(c, hipMemsetAsync(g->rows, 0, nre * V * sizeof(int32_t), c->stream));
     int launches = 0, slots_used = 1;
+    std::vector<std::pair<uint32_t, int32_t>> last_teams;  // (size, first k) for diagnosis
     if (variant == DCX_VARIANT_HINDEX) {
         // rows in k order, each started from the row above (l_k <= l_{k-1})
         ensure_scratch(c, 2ull * kScratchArrays * V * 4);
@@ -372,24 +440,28 @@ void compute_rows(dcx_ctx* c, dcx_graph* g, int variant, int32_t kr = -1) {
         const uint32_t tmax = c->cfg.team_max > 0 ? (uint32_t)c->cfg.team_max : kTeamMax;
         const uint32_t shrink = c->cfg.shrink_at > 0 ? (uint32_t)c->cfg.shrink_at : kShrinkAt;
         Sched s = make_schedule(g, kr, G, tmax);
+        const uint32_t busy = (uint32_t)std::count_if(s.block_map.begin(), s.block_map.begin() + G,
+                                                      [](uint32_t t) { return t != ~0u; });
+        for (const TeamDesc& td : s.teams)
+            last_teams.emplace_back(td.size, td.pass_begin < td.pass_end ? s.ks[td.pass_begin] : -1);
         for (const TeamDesc& td : s.teams)
             for (uint32_t i = td.pass_begin; i < td.pass_end; ++i) {
                 g->h_pass_variant[s.ks[i] + 1] = DCX_VARIANT_BATCHED;
                 g->h_pass_team[s.ks[i] + 1] = (uint16_t)td.size;
             }
-        const size_t nt = s.teams.size(), nb = s.block_team.size(), nk = s.ks.size();
-        const size_t bytes = nt * sizeof(TeamDesc) + 4 * nb + 4 * nk + 64;
+        const size_t nt = s.teams.size(), nb = G, nk = s.ks.size();
+        const size_t bytes = nt * sizeof(TeamDesc) + 8 * nb + 4 * nk + 64;
         ensure_bytes(c, &c->sched, &c->sched_bytes, bytes);
         char* base = static_cast<char*>(c->sched);
         TeamDesc* d_teams = reinterpret_cast<TeamDesc*>(base);
         uint32_t* d_bt = reinterpret_cast<uint32_t*>(base + nt * sizeof(TeamDesc));
-        int32_t* d_ks = reinterpret_cast<int32_t*>(d_bt + nb);
+        int32_t* d_ks = reinterpret_cast<int32_t*>(d_bt + 2 * nb);
         std::vector<char> host(bytes);
         std::memcpy(host.data(), s.teams.data(), nt * sizeof(TeamDesc));
-        std::memcpy(host.data() + nt * sizeof(TeamDesc), s.block_team.data(), 4 * nb);
-        std::memcpy(host.data() + nt * sizeof(TeamDesc) + 4 * nb, s.ks.data(), 4 * nk);
+        std::memcpy(host.data() + nt * sizeof(TeamDesc), s.block_map.data(), 8 * nb);
+        std::memcpy(host.data() + nt * sizeof(TeamDesc) + 8 * nb, s.ks.data(), 4 * nk);
         HIPX(c, hipMemcpyAsync(c->sched, host.data(), bytes, hipMemcpyHostToDevice, c->stream));
-        const size_t pool_words = nt * (s.sync_stride + 12);
+        const size_t pool_words = nt * ((size_t)s.sync_stride + kSlotWords);
         if (c->pools_words < pool_words) {
             dev_free(c->pools);
             dev_alloc(c, &c->pools, pool_words);
@@ -398,23 +470,42 @@ void compute_rows(dcx_ctx* c, dcx_graph* g, int variant, int32_t kr = -1) {
         HIPX(c, hipMemsetAsync(c->pools, 0, 4 * pool_words, c->stream));
         ensure_scratch(c, 4 * s.scratch_words);
         PassSpec basep = traced(c, core_row_spec(g, 0, g->rows), true);
+        basep.fail_rec = c->fail_rec;
+
         HIPX(c, hipEventRecord(c->ev[5], c->stream));
         HIPX(c, launch_rows_teams(basep, g->vk, g->rows, g->lmax_rows, d_teams, d_bt, d_ks,
                                   (uint32_t)nb, c->scratch, c->pools,
                                   c->pools + nt * s.sync_stride, c->timeout_ticks,
                                   c->ctrs + kCtrErr, shrink, c->stream));
         launches = 1;
-        slots_used = (int)nb;
+        slots_used = (int)busy;
         team_err = c->pools + kSyncError;
         nteam_err = (uint32_t)nt;
         team_err_stride = s.sync_stride;
     }
     HIPX(c, hipEventRecord(c->ev[6], c->stream));
+    try {
+        check_kernel_errors(c, "l-rows", team_err, nteam_err, team_err_stride);
+    } catch (const Fail&) {
+        describe_failed_passes(c, g, (int32_t)nre - 1);
+        describe_fail_record(c);
+        if (!last_teams.empty()) {
+            std::string d = c->err + "; teams:";
+            for (size_t t = 0; t < last_teams.size() && t < 6; ++t) {
+                char buf[96];
+                std::snprintf(buf, sizeof buf, " [%zu: size %u, k %d]", t, last_teams[t].first,
+                              last_teams[t].second);
+                d += buf;
+            }
+            c->err = d;
+        }
+        throw;
+    }
     std::vector<int32_t> lm(nrows);
     HIPX(c, hipMemcpyAsync(lm.data(), g->lmax_rows, nrows * sizeof(int32_t),
                            hipMemcpyDeviceToHost, c->stream));
     HIPX(c, hipEventRecord(c->ev[4], c->stream));
-    check_kernel_errors(c, "l-rows", team_err, nteam_err, team_err_stride);
+    sync(c);
     c->timing.rows_ms = elapsed(c, 3, 4);
     c->timing.rows_kernel_ms = elapsed(c, 5, 6);
     c->timing.rows_launches = launches;
@@ -588,6 +679,7 @@ dcx_status dcx_ctx_destroy(dcx_ctx* c) {
     dev_free(c->sched);
     dev_free(c->pools);
     dev_free(c->trace);
+    dev_free(c->fail_rec);
     dev_free(c->ctrs);
     dev_free(c->d_scalars);
     dev_free(c->upd_dev);
@@ -761,7 +853,9 @@ dcx_status dcx_graph_pass_log(dcx_ctx* ctx, const dcx_graph* g, dcx_pass_log* ou
             dcx_pass_log& e = out[i];
             std::memset(&e, 0, sizeof e);
             e.k = (int32_t)i - 1;
-            const bool ran = i == 0 || (g->rows_valid && (int32_t)i - 1 < g->rows_k);
+            // a row's entry is reported once it was ever computed (after a
+            // failed launch too, for diagnosis)
+            const bool ran = i == 0 || g->h_pass_variant[i] != 0;
             e.variant = ran ? g->h_pass_variant[i] : 0;
             e.team = ran ? g->h_pass_team[i] : 0;
             e.members = i == 0 ? g->V : g->h_vk[i - 1];
@@ -772,6 +866,10 @@ dcx_status dcx_graph_pass_log(dcx_ctx* ctx, const dcx_graph* g, dcx_pass_log* ou
             e.levels = w[kStatLevels];
             const uint64_t ticks = (uint64_t)w[kStatTicksLo] | ((uint64_t)w[kStatTicksHi] << 32);
             e.ms = (double)ticks / ctx->wall_khz;
+            e.failed = w[kStatFailed];
+            e.removed = w[kStatRemoved];
+            e.alive = w[kStatAlive];
+            e.level = (int32_t)w[kStatLevel];
         }
         return DCX_OK;
     });

@@ -49,7 +49,14 @@ struct PassSpec {
     uint32_t* trace;        // nullable; phase trace: [0] = entries, then kTraceWords each
     uint32_t trace_cap;     // entries the trace can hold
     int32_t trace_k;        // batched launch: the row whose team pass is traced
+    uint32_t* fail_rec;     // nullable; [kFailWords] what the first failing member saw
 };
+
+enum : uint32_t { kFailWords = 16 };
+
+// Phase-counter slots of a peel pass: 3 slots, each in its own 128-byte
+// line (kSlotStride words), kSlotWords words per pass or team.
+enum : uint32_t { kSlotStride = 32, kSlotWords = 3 * kSlotStride };
 
 // Phase trace entry (peel passes run on one team when tracing is on).
 enum : uint32_t {
@@ -66,7 +73,11 @@ enum : uint32_t {
     kStatLevels = 1,  // level iterations (peel); 0 for h-index
     kStatTicksLo = 2, // wall-clock ticks from start to end of the pass
     kStatTicksHi = 3,
-    kStatWords = 4
+    kStatFailed = 4,  // nonzero: the pass stopped on inconsistent counters
+    kStatRemoved = 5, // vertices removed when it ended
+    kStatAlive = 6,   // alive-list length when it ended
+    kStatLevel = 7,   // level when it ended
+    kStatWords = 8
 };
 
 // Per-pass device scratch, sized for `cap` vertices.
@@ -100,15 +111,20 @@ __host__ __device__ inline PassScratch scratch_at(void* base, uint64_t cap) {
 
 // Team of workgroups running a list of passes (batched rows).
 struct TeamDesc {
-    uint32_t first_block;  // first workgroup of the team
+    uint32_t xcd;          // XCD all of the team's workgroups run on
     uint32_t size;         // workgroups in the team
     uint32_t pass_begin;   // passes team_ks[pass_begin .. pass_end)
     uint32_t pass_end;
     uint64_t scratch_off;  // uint32 words into the scratch pool
     uint64_t scratch_cap;  // vertices the team's scratch holds
     uint32_t sync_off;     // words into the sync pool (sync_words(size))
-    uint32_t slot_off;     // words into the slot pool (12)
+    uint32_t slot_off;     // words into the slot pool (kSlotWords)
 };
+
+// Workgroups are dispatched to the XCDs round-robin: workgroup b runs on
+// XCD b % kXcds.  A team is built from workgroups of one XCD, so the data
+// its members hand each other between phases stays within one L2.
+constexpr uint32_t kXcds = 8;
 
 // ---- peel kernels (peel_kernels.hip) ----------------------------------
 // One pass on the whole grid (bounded team barrier); `slots` 12 words,
@@ -116,8 +132,9 @@ struct TeamDesc {
 hipError_t launch_pass_grid(const PassSpec& p, const PassScratch& s, uint32_t* slots,
                             int32_t* lmax_out, const GridSync& gs, hipStream_t st);
 int pass_grid_blocks_per_cu();
-// Rows of one graph (core layout): the launch has `blocks` workgroups,
-// block_team[b] names the team of block b.  Pass k of team t is
+// Rows of one graph (core layout): the launch has `blocks` workgroups;
+// block_team[b] names the team of block b (~0u: the block has no work) and
+// block_team[blocks + b] its rank in the team.  Pass k of team t is
 // team_ks[..]; base describes the graph (k, nmembers, out_row set per pass).
 hipError_t launch_rows_teams(const PassSpec& base, const uint32_t* vk, int32_t* rows,
                              int32_t* lmax_rows, const TeamDesc* teams, const uint32_t* block_team,

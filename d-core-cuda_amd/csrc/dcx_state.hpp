@@ -25,6 +25,7 @@ struct dcx_ctx {
     uint64_t timeout_ticks = 0;  // wall-clock ticks one team barrier may take
     double wall_khz = 100000.0;  // device wall clock rate
     uint32_t* trace = nullptr;   // phase trace of the last grid pass (cfg.trace_phases)
+    uint32_t* fail_rec = nullptr;  // [kFailWords] first failing pass member's view
     hipStream_t stream = nullptr;
     std::mutex mu;
     std::string err;
@@ -53,7 +54,11 @@ struct dcx_ctx {
 };
 
 namespace {
-constexpr int kCtrFlags = 16, kCtrErr = 24, kCtrSync = 64, kCtrWords = 1024;
+// ctrs layout (32-bit words): [0, kSlotWords) phase slots of grid passes;
+// three h-index change flags kSlotStride apart from kCtrFlags; the error
+// word of single-workgroup passes; the grid barrier's words from kCtrSync.
+// Every group starts its own 128-byte line.
+constexpr int kCtrFlags = 96, kCtrErr = 192, kCtrSync = 256, kCtrWords = 1024;
 constexpr uint32_t kMaxGridBlocks = kCtrWords - kCtrSync - kSyncFlags;
 // Workgroups per CU for grid-wide passes: far below the occupancy limit, so
 // residency of the whole grid is never in question.
@@ -222,13 +227,20 @@ void check_kernel_errors(dcx_ctx* c, const char* what, const uint32_t* team_err 
                                c->stream));
     sync(c);
     int32_t e = std::max(c->h_scalars[32], c->h_scalars[33]);
-    for (uint32_t t = 0; t < nteam; ++t) e = std::max(e, (int32_t)te[(size_t)t * stride]);
+    int32_t first_team = -1;
+    for (uint32_t t = 0; t < nteam; ++t) {
+        const int32_t x = (int32_t)te[(size_t)t * stride];
+        if (x && first_team < 0) first_team = (int32_t)t;
+        e = std::max(e, x);
+    }
     if (e == (int32_t)kErrTimeout) {
         set_err(c, "%s: a team barrier timed out (workgroups not co-resident?)", what);
         throw Fail{DCX_ERR_TIMEOUT};
     }
     if (e != 0) {
-        set_err(c, "%s: kernel counters inconsistent (code %d)", what, e);
+        set_err(c, "%s: kernel counters inconsistent (code %d; grid word %d, workgroup word %d, "
+                   "first team %d)",
+                what, e, c->h_scalars[32], c->h_scalars[33], first_team);
         throw Fail{DCX_ERR_INTERNAL};
     }
 }

@@ -48,14 +48,28 @@ struct GridSync {
     uint32_t epoch;          // barriers passed so far (same in every member)
 };
 
+// Cross-workgroup words (barrier flags, phase counters) are read and
+// written only with read-modify-write atomics, resolved at one point for
+// the whole device.  Each one is issued WITH return and its old value
+// consumed before the caller goes on: a no-return atomic counts as done
+// when the L2 has accepted it, and on gfx950 one was observed to take
+// effect after the next barrier (a counter reset landing on top of the
+// next phase's increments).  Consuming the returned value (here: a store to
+// an LDS sink, which must wait for it) means the operation has been
+// performed before anything that follows in program order.
+__device__ __forceinline__ void consume(uint32_t x) {
+    __shared__ uint32_t s_sink;
+    *(volatile uint32_t*)&s_sink = x;
+}
+__device__ __forceinline__ uint32_t coherent_read(uint32_t* p) { return atomicAdd(p, 0u); }
+__device__ __forceinline__ void coherent_write(uint32_t* p, uint32_t v) { consume(atomicExch(p, v)); }
+
 __device__ inline void sync_fail(const GridSync& g, uint32_t code) {
-    atomicMax(g.w + kSyncError, code);
-    __hip_atomic_store(g.w + kSyncAbort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    consume(atomicMax(g.w + kSyncError, code));
+    coherent_write(g.w + kSyncAbort, 1u);
 }
 
-__device__ inline bool sync_aborted(const GridSync& g) {
-    return __hip_atomic_load(g.w + kSyncAbort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0;
-}
+__device__ inline bool sync_aborted(const GridSync& g) { return coherent_read(g.w + kSyncAbort) != 0; }
 
 // Every thread of every workgroup of the team must call this.  Returns
 // false when the launch is being aborted.
@@ -66,16 +80,13 @@ __device__ inline bool grid_barrier(GridSync& g) {
     // every wavefront publishes its own stores before the flag goes up
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
     __syncthreads();
-    if (threadIdx.x == 0)
-        __hip_atomic_store(g.w + kSyncFlags + g.rank, e, __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x == 0) coherent_write(g.w + kSyncFlags + g.rank, e);
     if (g.rank == 0) {
         // every thread of rank 0 watches a strided subset of the flags
         const uint64_t t0 = wall_clock64();
         bool ok = true;
         for (uint32_t i = threadIdx.x; i < g.nblocks && ok; i += blockDim.x) {
-            while (__hip_atomic_load(g.w + kSyncFlags + i, __ATOMIC_RELAXED,
-                                     __HIP_MEMORY_SCOPE_AGENT) < e) {
+            while (coherent_read(g.w + kSyncFlags + i) < e) {
                 if (sync_aborted(g)) {
                     ok = false;
                     break;
@@ -94,12 +105,11 @@ __device__ inline bool grid_barrier(GridSync& g) {
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
             __builtin_amdgcn_s_waitcnt(0);
-            __hip_atomic_store(g.w + kSyncRelease, e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            coherent_write(g.w + kSyncRelease, e);
         }
     } else if (threadIdx.x == 0) {
         const uint64_t t0 = wall_clock64();
-        while (__hip_atomic_load(g.w + kSyncRelease, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
-               e) {
+        while (coherent_read(g.w + kSyncRelease) < e) {
             if (sync_aborted(g)) {
                 s_ok = 0;
                 break;
@@ -136,15 +146,11 @@ __device__ inline bool team_resync(GridSync& g, uint32_t seq) {
         if (g.rank == 0) {
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
             __builtin_amdgcn_s_waitcnt(0);
-            __hip_atomic_store(g.w + kSyncEpoch, g.epoch, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-            __builtin_amdgcn_s_waitcnt(0);
-            __hip_atomic_store(g.w + kSyncDone, seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            coherent_write(g.w + kSyncEpoch, g.epoch);
+            coherent_write(g.w + kSyncDone, seq);
         } else {
             const uint64_t t0 = wall_clock64();
-            while (__hip_atomic_load(g.w + kSyncDone, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
-                   seq) {
+            while (coherent_read(g.w + kSyncDone) < seq) {
                 if (sync_aborted(g)) {
                     s_ok2 = 0;
                     break;
@@ -158,7 +164,7 @@ __device__ inline bool team_resync(GridSync& g, uint32_t seq) {
             }
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             __builtin_amdgcn_s_waitcnt(0);
-            s_ep = __hip_atomic_load(g.w + kSyncEpoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            s_ep = coherent_read(g.w + kSyncEpoch);
         }
     }
     __syncthreads();

@@ -136,10 +136,12 @@ __global__ __launch_bounds__(kBlk) void k_hindex_row(PassSpec P, PassScratch S, 
     const uint32_t nwaves = gridDim.x * blockDim.x / W;
     const bool leader = blockIdx.x == 0 && threadIdx.x == 0;
     const uint64_t t_start = wall_clock64();
+    __shared__ uint32_t s_changed;
+    // change flags kSlotStride words apart (own lines), RMW access only
     if (leader) {
-        flags[0] = 0;
-        flags[1] = 0;
-        flags[2] = 0;
+        coherent_write(flags, 0u);
+        coherent_write(flags + kSlotStride, 0u);
+        coherent_write(flags + 2 * kSlotStride, 0u);
     }
     if (!grid_barrier(gs)) return;
     int32_t sweep = 0;
@@ -150,8 +152,8 @@ __global__ __launch_bounds__(kBlk) void k_hindex_row(PassSpec P, PassScratch S, 
             if (threadIdx.x == 0) sync_fail(gs, kErrInconsistent);
             return;
         }
-        uint32_t* cur = flags + (sweep % 3);
-        if (leader) flags[(sweep + 1) % 3] = 0;
+        uint32_t* cur = flags + kSlotStride * (sweep % 3);
+        if (leader) coherent_write(flags + kSlotStride * ((sweep + 1) % 3), 0u);
         bool any = false;
         int32_t* const l = row_values(P, S);
         for (uint32_t v = wave; v < P.nmembers; v += nwaves) {
@@ -163,10 +165,16 @@ __global__ __launch_bounds__(kBlk) void k_hindex_row(PassSpec P, PassScratch S, 
                 any = true;
             }
         }
-        if (any && lane_id() == 0) atomicAdd(cur, 1u);
+        if (any && lane_id() == 0) consume(atomicAdd(cur, 1u));
         if (!grid_barrier(gs)) return;
         ++sweep;
-        if (__hip_atomic_load(cur, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) break;
+        // one coherent read per workgroup: every workgroup must take the
+        // same decision
+        if (threadIdx.x == 0) s_changed = coherent_read(cur);
+        __syncthreads();
+        const uint32_t changed = s_changed;
+        __syncthreads();
+        if (changed == 0) break;
     }
     if (leader && P.stats) {
         const uint64_t dt = wall_clock64() - t_start;
@@ -174,6 +182,10 @@ __global__ __launch_bounds__(kBlk) void k_hindex_row(PassSpec P, PassScratch S, 
         P.stats[kStatLevels] = 0;
         P.stats[kStatTicksLo] = (uint32_t)dt;
         P.stats[kStatTicksHi] = (uint32_t)(dt >> 32);
+        P.stats[kStatFailed] = 0;
+        P.stats[kStatRemoved] = P.nmembers;
+        P.stats[kStatAlive] = 0;
+        P.stats[kStatLevel] = 0;
     }
 }
 

@@ -62,17 +62,18 @@ __device__ __forceinline__ void wave_append(bool pred, uint32_t val, uint32_t* a
     if (pred) arr[base + __popcll(m & ((1ull << lane_id()) - 1ull))] = val;
 }
 
-// Wavefront minimum, then one atomicMin per wavefront.
+// Wavefront minimum, then one atomicMin per wavefront (performed before
+// returning: its result is consumed, see coherent_write).
 __device__ __forceinline__ void wave_min_to(uint32_t x, uint32_t* dst) {
 #pragma unroll
     for (int d = W / 2; d > 0; d >>= 1) x = min(x, (uint32_t)__shfl_xor(x, d, W));
-    if (lane_id() == 0 && x != UINT_MAX) atomicMin(dst, x);
+    if (lane_id() == 0 && x != UINT_MAX) consume(atomicMin(dst, x));
 }
 
 __device__ __forceinline__ void wave_max_to(int32_t x, int32_t* dst) {
 #pragma unroll
     for (int d = W / 2; d > 0; d >>= 1) x = max(x, __shfl_xor(x, d, W));
-    if (lane_id() == 0) atomicMax(dst, x);
+    if (lane_id() == 0) consume((uint32_t)atomicMax(dst, x));
 }
 
 template <int BLK>
@@ -142,18 +143,30 @@ __device__ __forceinline__ uint32_t lower_bound(const uint32_t* a, uint32_t n, u
     return lo;
 }
 
-// Atomic stores: the words share a cache line with slots that other
-// workgroups update with atomics at the same time.
+// Phase counters: three slots (triple-buffered by phase), each in its own
+// 128-byte line, touched only with RMW atomics (coherent_read/_write of
+// grid_sync.hpp).  Words: [0] frontier appends, [1] alive-list appends,
+// [2] smallest key degree seen above the level.
 __device__ __forceinline__ void slot_reset(uint32_t* s) {
-    __hip_atomic_store(s + 0, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);  // frontier
-    __hip_atomic_store(s + 1, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);  // alive list
-    // smallest key degree seen above the level
-    __hip_atomic_store(s + 2, (uint32_t)UINT_MAX, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(s + 3, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    coherent_write(s + 0, 0u);
+    coherent_write(s + 1, 0u);
+    coherent_write(s + 2, (uint32_t)UINT_MAX);
 }
 
-__device__ __forceinline__ uint32_t slot_read(const uint32_t* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+// One coherent read of a slot per workgroup, shared through LDS; every
+// thread must call it (contains workgroup barriers).
+__device__ __forceinline__ void slot_read3(uint32_t* s, uint32_t* lds, uint32_t& a, uint32_t& b,
+                                           uint32_t& c) {
+    if (threadIdx.x == 0) {
+        lds[0] = coherent_read(s + 0);
+        lds[1] = coherent_read(s + 1);
+        lds[2] = coherent_read(s + 2);
+    }
+    __syncthreads();
+    a = lds[0];
+    b = lds[1];
+    c = lds[2];
+    __syncthreads();
 }
 
 __device__ __forceinline__ void poison(const PassScratch& S, uint32_t x) {
@@ -178,17 +191,42 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
     uint32_t tw = MULTI ? gs->rank : 0u;
     uint32_t tn = MULTI ? gs->nblocks : 1u;
     bool solo = !MULTI || tn == 1;
+    // pass state (declared first: the lambdas below read it)
+    uint32_t removed = 0, s = 0, levels = 0, nA = 0, mnall_dbg = 0;
+    int32_t L = 0;
     auto barrier = [&]() -> bool {
-        if (MULTI && !solo) return grid_barrier(*gs);
+        if (MULTI && !solo) {
+            if (tw == 0 && tid == 0) coherent_write(slots + 3, s);
+            return grid_barrier(*gs);
+        }
         __syncthreads();
         return true;
     };
+    bool failed = false;
     auto fail = [&](uint32_t code) {
+        failed = true;
+        if (tid == 0 && P.fail_rec && atomicCAS(P.fail_rec, 0u, 1u) == 0u) {
+            P.fail_rec[1] = tw;
+            P.fail_rec[2] = tn;
+            P.fail_rec[3] = s;
+            P.fail_rec[4] = levels;
+            P.fail_rec[5] = removed;
+            P.fail_rec[6] = P.nmembers;
+            P.fail_rec[7] = nA;
+            P.fail_rec[8] = mnall_dbg;
+            P.fail_rec[9] = (uint32_t)L;
+            P.fail_rec[10] = (uint32_t)P.k;
+            P.fail_rec[11] = MULTI ? gs->rank : 0u;
+            P.fail_rec[12] = MULTI ? gs->epoch : 0u;
+            P.fail_rec[13] = coherent_read(slots + 3);  // leader's last phase
+            P.fail_rec[14] = MULTI ? coherent_read(gs->w + kSyncRelease) : 0u;
+            P.fail_rec[15] = MULTI ? gs->nblocks : 0u;
+        }
         if (tid == 0) {
             if (MULTI)
                 sync_fail(*gs, code);
             else
-                atomicMax(err, code);
+                consume(atomicMax(err, code));
         }
     };
     int32_t* const lval = P.core ? S.lv : P.lval;
@@ -209,10 +247,11 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
         t_prev = now;
     };
 
+    __shared__ uint32_t s_rd[3];
     if (tw == 0 && tid == 0) {
         slot_reset(slots);
-        slot_reset(slots + 4);
-        slot_reset(slots + 8);
+        slot_reset(slots + kSlotStride);
+        slot_reset(slots + 2 * kSlotStride);
     }
 
     // ---- set-up: live degrees (member prefixes in the core layout)
@@ -235,8 +274,7 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
 
     const uint32_t* A = nullptr;  // first level: the members in id order
     uint32_t *B = S.a0, *Bn = S.a1, *F = S.f0, *G = S.f1;
-    uint32_t nA = nm, removed = 0, s = 0, levels = 0;
-    int32_t L = 0;
+    nA = nm;
     const int32_t k = P.k;
     // Without membership restriction the constraint can fail before any
     // removal (KList semantics on the whole graph, main.cu:144-169).
@@ -252,6 +290,10 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
         if (MULTI && !solo && shrink_at > 0) {
             const uint32_t want = max(1u, min(tn, (uint32_t)((nA + shrink_at - 1) / shrink_at)));
             if (want == 1 || 2 * want <= tn) {
+                // The narrowed team resets phase slots without the members
+                // that leave; they must all have finished reading the last
+                // slot first, so the whole team meets once more here.
+                if (!barrier()) return;
                 if (tw >= want) return;
                 tn = want;
                 gs->nblocks = want;
@@ -259,8 +301,8 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
             }
         }
         // ---- seed level L from the alive list; survivors go to B
-        uint32_t* cur = slots + 4 * (s % 3);
-        if (tw == 0 && tid == 0) slot_reset(slots + 4 * ((s + 1) % 3));
+        uint32_t* cur = slots + kSlotStride * (s % 3);
+        if (tw == 0 && tid == 0) slot_reset(slots + kSlotStride * ((s + 1) % 3));
         uint32_t mn = UINT_MAX;
         for (uint32_t base = tw * BLK; base < nA; base += tn * BLK) {
             const uint32_t i = base + tid;
@@ -280,9 +322,9 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
         }
         wave_min_to(mn, &cur[2]);
         if (!barrier()) return;
-        uint32_t nF = slot_read(&cur[0]);
-        nA = slot_read(&cur[1]);
-        uint32_t mnall = slot_read(&cur[2]);
+        uint32_t nF, mnall;
+        slot_read3(cur, s_rd, nF, nA, mnall);
+
         trace(s, true, L, nF, nA);
         A = B;
         {
@@ -295,8 +337,8 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
 
         // ---- cascade inside level L until no vertex crosses a threshold
         while (nF > 0) {
-            uint32_t* nxt = slots + 4 * (s % 3);
-            if (tw == 0 && tid == 0) slot_reset(slots + 4 * ((s + 1) % 3));
+            uint32_t* nxt = slots + kSlotStride * (s % 3);
+            if (tw == 0 && tid == 0) slot_reset(slots + kSlotStride * ((s + 1) % 3));
             uint32_t mn2 = UINT_MAX;
             // Frontier items per workgroup: a team splits even a small
             // frontier over all its members (whole tiles would leave most
@@ -369,8 +411,10 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
             }
             wave_min_to(mn2, &nxt[2]);
             if (!barrier()) return;
-            nF = slot_read(&nxt[0]);
-            mnall = min(mnall, slot_read(&nxt[2]));
+            uint32_t unused, mn_c;
+            slot_read3(nxt, s_rd, nF, unused, mn_c);
+            mnall = min(mnall, mn_c);
+
             trace(s, false, L, nF, nA);
             {
                 uint32_t* t = F;
@@ -382,6 +426,7 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
         }
         if (removed < nm) {
             // alive vertices exist, so some degree was recorded
+            mnall_dbg = mnall;
             if (nA == 0 || mnall == UINT_MAX) {
                 fail(kErrInconsistent);
                 break;
@@ -397,6 +442,10 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
             P.stats[kStatLevels] = levels;
             P.stats[kStatTicksLo] = (uint32_t)dt;
             P.stats[kStatTicksHi] = (uint32_t)(dt >> 32);
+            P.stats[kStatFailed] = failed ? 1u : 0u;
+            P.stats[kStatRemoved] = removed;
+            P.stats[kStatAlive] = nA;
+            P.stats[kStatLevel] = (uint32_t)L;
         }
     }
     if (P.core) {
@@ -428,14 +477,16 @@ __global__ __launch_bounds__(kBlk) void k_rows_teams(PassSpec base, const uint32
                                                      uint64_t timeout_ticks, uint32_t* err,
                                                      uint32_t shrink_at) {
     __shared__ TileLds<kBlk> T;
-    __shared__ uint32_t lslots[12];
-    const TeamDesc td = teams[block_team[blockIdx.x]];
+    __shared__ uint32_t lslots[kSlotWords];
+    const uint32_t team = block_team[blockIdx.x];
+    if (team == ~0u) return;  // an idle block (its XCD had no team for it)
+    const TeamDesc td = teams[team];
     const PassScratch S =
         scratch_at(static_cast<uint32_t*>(scratch) + td.scratch_off, td.scratch_cap);
     GridSync gs;
     gs.w = sync_pool + td.sync_off;
     gs.nblocks = td.size;
-    gs.rank = blockIdx.x - td.first_block;
+    gs.rank = block_team[gridDim.x + blockIdx.x];
     gs.timeout_ticks = timeout_ticks;
     gs.epoch = 0;
     for (uint32_t p = td.pass_begin; p < td.pass_end; ++p) {

@@ -45,7 +45,9 @@ class PassLog(ctypes.Structure):
     _fields_ = [("k", ctypes.c_int32), ("variant", ctypes.c_int32),
                 ("members", ctypes.c_uint32), ("phases", ctypes.c_uint32),
                 ("levels", ctypes.c_uint32), ("team", ctypes.c_uint32),
-                ("edges", ctypes.c_uint64), ("ms", ctypes.c_double)]
+                ("edges", ctypes.c_uint64), ("ms", ctypes.c_double),
+                ("failed", ctypes.c_uint32), ("removed", ctypes.c_uint32),
+                ("alive", ctypes.c_uint32), ("level", ctypes.c_int32)]
 
 
 class PhaseTrace(ctypes.Structure):

@@ -206,6 +206,10 @@ typedef struct dcx_pass_log {
     uint32_t team;     /* workgroups that ran it */
     uint64_t edges;    /* edges of the pass (E, or E_k) */
     double ms;         /* device wall-clock time of the pass */
+    uint32_t failed;   /* 1: stopped on inconsistent counters */
+    uint32_t removed;  /* vertices removed when it ended */
+    uint32_t alive;    /* alive-list length when it ended */
+    int32_t level;     /* level when it ended */
 } dcx_pass_log;
 /* One synchronised phase of the last peel pass run on the whole grid (the
  * kmax pass, or a row of the plain variant) when cfg.trace_phases is set:

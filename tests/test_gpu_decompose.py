@@ -165,6 +165,27 @@ def test_team_barrier_visibility_repeated():
             g.free()
 
 
+def test_narrowing_teams_repeated_rmat20():
+    """Regression: a team that narrows must not reuse a phase-counter slot
+    that a leaving member is still reading (R-MAT 20 seed 2 hit it at the
+    level where the densest 1351-vertex core's alive list falls below one
+    workgroup's share).  Every repetition must give the same bits."""
+    src, dst = dcx.gen_rmat(20, 16 << 20, 2)
+    V = 1 << 20
+    with dcx.Context(0) as c:
+        g = c.upload_edges(V, src, dst, slack=4)
+        hashes = set()
+        for rep in range(10):
+            say(f"rmat20 seed 2 slack 4 batched repeat {rep}")
+            g.decompose(dcx.VARIANT_BATCHED)
+            hashes.add(g.state_hash())
+        say("rmat20 seed 2 slack 4 peel")
+        g.decompose(dcx.VARIANT_PEEL)
+        hashes.add(g.state_hash())
+        g.free()
+    assert len(hashes) == 1
+
+
 @pytest.mark.parametrize("variant", VARIANTS)
 def test_pass_log(ctx, variant):
     """The device pass log covers the kmax pass and every row."""
