@@ -2,6 +2,7 @@
 // decomposition drivers.  Every entry point returns a dcx_status; HIP
 // failures are caught at each call and reported, never ignored.
 #include "dcx_state.hpp"
+#include "rng.hpp"
 
 namespace {
 
@@ -244,12 +245,16 @@ constexpr uint32_t kTeamMax = 32;
 constexpr uint32_t kShrinkAt = 4096;
 constexpr int kRowsBlocksPerCu = 2;
 
-Sched make_schedule(const dcx_graph* g, int32_t kr, uint32_t G, uint32_t team_max) {
+// want/wts (optional): team width and weight of each pass chosen by the
+// cost model; without them the width follows the pass's share of the work.
+Sched make_schedule(const dcx_graph* g, int32_t kr, uint32_t G, uint32_t team_max,
+                    const std::vector<uint32_t>* want = nullptr,
+                    const std::vector<double>* wts = nullptr) {
     const uint32_t np = (uint32_t)kr + 1;
     std::vector<double> w(np);
     double W = 0;
     for (uint32_t k = 0; k < np; ++k) {
-        w[k] = 2.0 * (double)g->h_ek[k] + (double)g->h_vk[k] + 64.0;
+        w[k] = wts ? (*wts)[k] : 2.0 * (double)g->h_ek[k] + (double)g->h_vk[k] + 64.0;
         W += w[k];
     }
     const double share = W / G;
@@ -268,7 +273,8 @@ Sched make_schedule(const dcx_graph* g, int32_t kr, uint32_t G, uint32_t team_ma
     std::vector<uint32_t> small;
     std::vector<std::vector<uint32_t>> team_blocks;
     for (uint32_t k : idx) {
-        uint32_t t = (uint32_t)std::min<double>(std::min(team_max, per_xcd), w[k] / share);
+        uint32_t t = want ? std::min((*want)[k], std::min(team_max, per_xcd))
+                          : (uint32_t)std::min<double>(std::min(team_max, per_xcd), w[k] / share);
         uint32_t x = 0;
         for (uint32_t y = 1; y < kXcds; ++y)
             if (xfree[y].size() > xfree[x].size()) x = y;
@@ -335,6 +341,119 @@ Sched make_schedule(const dcx_graph* g, int32_t kr, uint32_t G, uint32_t team_ma
     }
     s.scratch_words = off;
     return s;
+}
+
+// Scheduler features of row pass k (kMlpIn floats): sizes of the (k,0)-core,
+// the vertices whose kmax is k, the row's level/phase counts from its last
+// run (0 if unknown) and the team width it would run on.  Scaled to about
+// [0, 1].
+void pass_features(const dcx_graph* g, int32_t k, uint32_t team, uint32_t levels,
+                   uint32_t phases, float* f) {
+    const double Vk = g->h_vk[k], Ek = (double)g->h_ek[k];
+    const double Vn = k + 1 <= g->kmax ? g->h_vk[k + 1] : 0.0;
+    f[0] = (float)(std::log2(1.0 + Vk) / 20.0);
+    f[1] = (float)(std::log2(1.0 + Ek) / 30.0);
+    f[2] = (float)(std::log2(1.0 + Ek / std::max(Vk, 1.0)) / 10.0);
+    f[3] = (float)((k + 0.5) / (g->kmax + 1.0));
+    f[4] = (float)(std::log2((double)std::max(team, 1u)) / 8.0);
+    f[5] = (float)(std::log2(1.0 + Vk - Vn) / 20.0);
+    f[6] = (float)(std::log2(1.0 + levels) / 10.0);
+    f[7] = (float)(std::log2(1.0 + phases) / 12.0);
+}
+
+void mlp_stage(dcx_ctx* c, dcx_mlp* m, size_t floats) {
+    if (m->buf_floats >= floats) return;
+    dev_free(m->buf);
+    dev_alloc(c, &m->buf, floats);
+    m->buf_floats = floats;
+}
+
+// Predictions (log2 ms) for n samples, computed on the device.
+std::vector<float> mlp_predict(dcx_ctx* c, dcx_mlp* m, const std::vector<float>& X, uint32_t n) {
+    mlp_stage(c, m, (size_t)n * (kMlpIn + 1));
+    std::vector<float> y(n);
+    if (!n) return y;
+    HIPX(c, hipMemcpyAsync(m->buf, X.data(), 4ull * n * kMlpIn, hipMemcpyHostToDevice, c->stream));
+    HIPX(c, launch_mlp_forward(m->p, m->buf, n, m->buf + (size_t)n * kMlpIn, c->stream));
+    HIPX(c, hipMemcpyAsync(y.data(), m->buf + (size_t)n * kMlpIn, 4ull * n, hipMemcpyDeviceToHost,
+                           c->stream));
+    sync(c);
+    return y;
+}
+
+// Team widths from the cost model: the predicted time d(k, t) of every
+// pass at every candidate width, then the smallest makespan T for which
+// each pass gets the narrowest width with d <= T, the wide teams fit in 3/4
+// of the grid and the single-workgroup passes (packed longest first) fit
+// in the rest.  Returns false if no width assignment was found.
+bool mlp_team_sizes(dcx_ctx* c, const dcx_graph* g, int32_t kr, uint32_t G, uint32_t team_max,
+                    std::vector<uint32_t>& want, std::vector<double>& wts) {
+    const uint32_t np = (uint32_t)kr + 1;
+    const uint32_t cap = std::min(team_max, G / kXcds);
+    std::vector<uint32_t> cands{1};
+    for (uint32_t t = 2; t <= cap; t *= 2) cands.push_back(t);
+    const uint32_t nc = (uint32_t)cands.size();
+    const bool have_log = g->h_pstats.size() >= ((size_t)np + 1) * kStatWords;
+    std::vector<float> X((size_t)np * nc * kMlpIn);
+    for (uint32_t k = 0; k < np; ++k) {
+        uint32_t lv = 0, ph = 0;
+        if (have_log) {
+            ph = g->h_pstats[(size_t)(k + 1) * kStatWords + kStatPhases];
+            lv = g->h_pstats[(size_t)(k + 1) * kStatWords + kStatLevels];
+        }
+        for (uint32_t j = 0; j < nc; ++j)
+            pass_features(g, (int32_t)k, cands[j], lv, ph, &X[((size_t)k * nc + j) * kMlpIn]);
+    }
+    const std::vector<float> y = mlp_predict(c, c->sched_mlp, X, np * nc);
+    std::vector<double> d((size_t)np * nc);
+    for (size_t i = 0; i < d.size(); ++i) d[i] = std::exp2((double)y[i]);
+    auto assign = [&](double T, std::vector<uint32_t>& sz) -> bool {
+        uint32_t big = 0;
+        std::vector<double> singles;
+        for (uint32_t k = 0; k < np; ++k) {
+            uint32_t j = 0;
+            while (j + 1 < nc && d[(size_t)k * nc + j] > T) ++j;
+            if (d[(size_t)k * nc + j] > T) return false;
+            sz[k] = cands[j];
+            if (cands[j] >= 2)
+                big += cands[j];
+            else
+                singles.push_back(d[(size_t)k * nc]);
+        }
+        if (big > G * 3 / 4) return false;
+        const uint32_t slots = G - big;
+        if (singles.empty()) return true;
+        if (!slots) return false;
+        std::sort(singles.begin(), singles.end(), std::greater<double>());
+        std::vector<double> load(std::min<size_t>(slots, singles.size()), 0.0);
+        for (double x : singles) {
+            auto it = std::min_element(load.begin(), load.end());
+            *it += x;
+            if (*it > T) return false;
+        }
+        return true;
+    };
+    double lo = 0.0, hi = 0.0;
+    for (uint32_t k = 0; k < np; ++k) hi += d[(size_t)k * nc];  // one workgroup does all
+    std::vector<uint32_t> sz(np, 1);
+    if (!assign(hi, sz)) return false;
+    for (int it = 0; it < 40; ++it) {
+        const double mid = 0.5 * (lo + hi);
+        std::vector<uint32_t> t2(np, 1);
+        if (assign(mid, t2))
+            hi = mid;
+        else
+            lo = mid;
+    }
+    assign(hi, sz);
+    want = sz;
+    wts.resize(np);
+    for (uint32_t k = 0; k < np; ++k) {
+        uint32_t j = 0;
+        while (cands[j] != sz[k]) ++j;
+        wts[k] = d[(size_t)k * nc + j] * sz[k];  // longest (in workgroup time) placed first
+    }
+    return true;
 }
 
 // Appends the device log of the failed row passes (k <= kr) to the error
@@ -439,7 +558,12 @@ void compute_rows(dcx_ctx* c, dcx_graph* g, int variant, int32_t kr = -1) {
                                                               : c->num_cus * kRowsBlocksPerCu);
         const uint32_t tmax = c->cfg.team_max > 0 ? (uint32_t)c->cfg.team_max : kTeamMax;
         const uint32_t shrink = c->cfg.shrink_at > 0 ? (uint32_t)c->cfg.shrink_at : kShrinkAt;
-        Sched s = make_schedule(g, kr, G, tmax);
+        std::vector<uint32_t> want;
+        std::vector<double> wts;
+        const bool by_model = c->sched_mlp && mlp_team_sizes(c, g, kr, G, tmax, want, wts);
+        Sched s = by_model ? make_schedule(g, kr, G, tmax, &want, &wts)
+                           : make_schedule(g, kr, G, tmax);
+        c->timing.rows_scheduler = by_model ? 1 : 0;
         const uint32_t busy = (uint32_t)std::count_if(s.block_map.begin(), s.block_map.begin() + G,
                                                       [](uint32_t t) { return t != ~0u; });
         for (const TeamDesc& td : s.teams)
@@ -512,6 +636,9 @@ void compute_rows(dcx_ctx* c, dcx_graph* g, int variant, int32_t kr = -1) {
     c->timing.rows_slots = slots_used;
     g->lmax = 0;
     for (int32_t x : lm) g->lmax = std::max(g->lmax, x);
+    g->h_pstats.resize(((size_t)K + 2) * kStatWords);
+    HIPX(c, hipMemcpy(g->h_pstats.data(), g->pstats, g->h_pstats.size() * 4,
+                      hipMemcpyDeviceToHost));
     g->rows_k = K + 1;
     g->rows_valid = true;
 }
@@ -1101,3 +1228,4 @@ dcx_status dcx_get_ragged(dcx_ctx* ctx, const dcx_graph* g, int64_t* row_ptr, in
 #include "dcx_batch_impl.hpp"
 #include "dcx_tune_impl.hpp"
 #include "dcx_serve_impl.hpp"
+#include "dcx_mlp_impl.hpp"

@@ -126,6 +126,20 @@ struct TeamDesc {
 // its members hand each other between phases stays within one L2.
 constexpr uint32_t kXcds = 8;
 
+// ---- cost-model MLP (mlp_kernels.hip) ----------------------------------
+constexpr int kMlpIn = 8, kMlpHidden = 32;
+constexpr int kMlpParams = kMlpHidden * kMlpIn + kMlpHidden + kMlpHidden * kMlpHidden +
+                           kMlpHidden + kMlpHidden + 1;
+// y[s] = f(X[s]) for n samples of kMlpIn floats.
+hipError_t launch_mlp_forward(const float* p, const float* X, uint32_t n, float* y,
+                              hipStream_t st);
+// grad = sum over samples of d/dp 0.5 (f(X[s]) - T[s])^2; loss = that sum.
+hipError_t launch_mlp_backward(const float* p, const float* X, const float* T, uint32_t n,
+                               float* grad, float* loss, hipStream_t st);
+// One Adam step (beta 0.9 / 0.999) on grad / n, step number t >= 1.
+hipError_t launch_mlp_adam(float* p, const float* grad, float* m, float* v, uint32_t n, float lr,
+                           int t, hipStream_t st);
+
 // ---- peel kernels (peel_kernels.hip) ----------------------------------
 // One pass on the whole grid (bounded team barrier); `slots` 12 words,
 // gs.w zeroed by the launcher.

@@ -26,6 +26,7 @@ struct dcx_ctx {
     double wall_khz = 100000.0;  // device wall clock rate
     uint32_t* trace = nullptr;   // phase trace of the last grid pass (cfg.trace_phases)
     uint32_t* fail_rec = nullptr;  // [kFailWords] first failing pass member's view
+    dcx_mlp* sched_mlp = nullptr;  // cost model of the rows scheduler (nullptr: heuristic)
     hipStream_t stream = nullptr;
     std::mutex mu;
     std::string err;
@@ -68,6 +69,20 @@ constexpr double kBarrierTimeoutS = 30.0;
 constexpr uint32_t kMaxVertices = 0x7fffffffu;
 }  // namespace
 
+// Cost model of the variant scheduler (mlp_kernels.hip): parameters,
+// Adam moments, gradient and a staging buffer on the device.
+struct dcx_mlp {
+    dcx_ctx* ctx = nullptr;
+    float* p = nullptr;     // [kMlpParams]
+    float* m = nullptr;     // Adam first moment
+    float* v = nullptr;     // Adam second moment
+    float* grad = nullptr;  // [kMlpParams]
+    float* loss = nullptr;  // [1]
+    int step = 0;           // Adam steps taken
+    float* buf = nullptr;   // staging for samples and predictions
+    size_t buf_floats = 0;
+};
+
 struct dcx_graph {
     dcx_ctx* ctx = nullptr;
     bool served = false;  // owned by a running dcx_server: API calls refused
@@ -103,6 +118,7 @@ struct dcx_graph {
     int32_t* kmax_old = nullptr; // [V] previous kmaxes during an update
     uint32_t* pstats = nullptr;  // [(kmax+2) * kStatWords] device pass log: kmax pass, rows
     size_t pstats_cap = 0;
+    std::vector<uint32_t> h_pstats;      // host copy of pstats after the last rows launch
     std::vector<int8_t> h_pass_variant;  // [kmax+2] variant that last ran each pass
     std::vector<uint16_t> h_pass_team;   // [kmax+2] workgroups that ran it
     int32_t* rows = nullptr;     // (kmax+1) x V

@@ -38,7 +38,8 @@ class Timing(ctypes.Structure):
     _fields_ = [("kmax_ms", ctypes.c_double), ("order_ms", ctypes.c_double),
                 ("rows_ms", ctypes.c_double), ("rows_kernel_ms", ctypes.c_double),
                 ("total_ms", ctypes.c_double), ("rows_launches", ctypes.c_int32),
-                ("rows_slots", ctypes.c_int32), ("reserved", ctypes.c_int32 * 6)]
+                ("rows_slots", ctypes.c_int32), ("rows_scheduler", ctypes.c_int32),
+                ("reserved", ctypes.c_int32 * 5)]
 
 
 class PassLog(ctypes.Structure):
@@ -309,6 +310,12 @@ class Context:
         check(lib().dcx_last_timing(self.h, ctypes.byref(t)), "dcx_last_timing", self.h)
         return t
 
+    def set_scheduler(self, mlp):
+        """Batched rows take team widths from the cost model (None: the
+        work-share rule)."""
+        check(lib().dcx_set_scheduler(self.h, mlp.h if mlp is not None else None),
+              "dcx_set_scheduler", self.h)
+
     def last_trace(self):
         """Phase trace of the last grid peel pass (needs trace_phases=1):
         structured numpy array (level, seed, removed, alive, us)."""
@@ -456,6 +463,19 @@ class Graph:
               "dcx_graph_pass_log", self.ctx.h)
         return [{f: getattr(arr[i], f) for f, _ in PassLog._fields_} for i in range(n.value)]
 
+    def pass_features(self, with_log=True):
+        """(X, T) training samples from the last batched rows launch."""
+        n = ctypes.c_uint32()
+        check(lib().dcx_graph_pass_features(self.ctx.h, self.h, i32(1 if with_log else 0),
+                                            u32(0), None, None, ctypes.byref(n)),
+              "dcx_graph_pass_features", self.ctx.h)
+        X = np.empty((max(n.value, 1), MLP.N_IN), np.float32)
+        T = np.empty(max(n.value, 1), np.float32)
+        check(lib().dcx_graph_pass_features(self.ctx.h, self.h, i32(1 if with_log else 0),
+                                            u32(n.value), P(X), P(T), ctypes.byref(n)),
+              "dcx_graph_pass_features", self.ctx.h)
+        return X[:n.value], T[:n.value]
+
     def state_hash(self):
         """dcx_state_hash of the resident decomposition."""
         h = ctypes.c_uint64()
@@ -547,6 +567,62 @@ def compact_csr(g):
         idx = np.repeat(off[:-1], deg) + (np.arange(int(coff[-1])) - np.repeat(coff[:-1].astype(np.int64), deg))
         res += [coff, g[side + "_nbr"][idx].astype(np.uint32)]
     return res
+
+
+class MLP:
+    """Cost model of the batched-rows scheduler (dcx_mlp_*): 8 inputs, two
+    hidden layers of 32 ReLU units, one output, fp32, on the device."""
+
+    N_IN = 8
+
+    def __init__(self, ctx, seed=1):
+        self.ctx = ctx
+        self.h = ctypes.c_void_p()
+        check(lib().dcx_mlp_create(ctx.h, u64(seed), ctypes.byref(self.h)), "dcx_mlp_create",
+              ctx.h)
+        self.n_params = int(lib().dcx_mlp_param_count())
+
+    def free(self):
+        if self.h:
+            check(lib().dcx_mlp_free(self.h), "dcx_mlp_free")
+            self.h = None
+
+    @property
+    def params(self):
+        out = np.empty(self.n_params, np.float32)
+        check(lib().dcx_mlp_get_params(self.h, P(out)), "dcx_mlp_get_params", self.ctx.h)
+        return out
+
+    @params.setter
+    def params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.n_params
+        check(lib().dcx_mlp_set_params(self.h, P(p)), "dcx_mlp_set_params", self.ctx.h)
+
+    def predict(self, X):
+        X = np.ascontiguousarray(X, np.float32).reshape(-1, self.N_IN)
+        y = np.empty(X.shape[0], np.float32)
+        check(lib().dcx_mlp_predict(self.h, P(X), u32(X.shape[0]), P(y)), "dcx_mlp_predict",
+              self.ctx.h)
+        return y
+
+    def gradient(self, X, T):
+        X = np.ascontiguousarray(X, np.float32).reshape(-1, self.N_IN)
+        T = np.ascontiguousarray(T, np.float32)
+        g = np.empty(self.n_params, np.float32)
+        loss = ctypes.c_float()
+        check(lib().dcx_mlp_gradient(self.h, P(X), P(T), u32(X.shape[0]), P(g),
+                                     ctypes.byref(loss)), "dcx_mlp_gradient", self.ctx.h)
+        return g, loss.value
+
+    def train(self, X, T, epochs=500, lr=3e-3):
+        X = np.ascontiguousarray(X, np.float32).reshape(-1, self.N_IN)
+        T = np.ascontiguousarray(T, np.float32)
+        loss = ctypes.c_float()
+        check(lib().dcx_mlp_train(self.h, P(X), P(T), u32(X.shape[0]), u32(epochs),
+                                  ctypes.c_float(lr), ctypes.byref(loss)), "dcx_mlp_train",
+              self.ctx.h)
+        return loss.value
 
 
 class Server:

@@ -163,8 +163,9 @@ typedef struct dcx_timing {
     double rows_kernel_ms; /* the rows kernel launches alone */
     double total_ms;       /* kmax_ms + order_ms + rows_ms */
     int32_t rows_launches; /* kernel launches spent on the rows */
-    int32_t rows_slots;    /* concurrent k-passes (batched variant) */
-    int32_t reserved[6];
+    int32_t rows_slots;    /* workgroups busy in the batched launch */
+    int32_t rows_scheduler; /* batched team widths: 0 work-share rule, 1 cost model */
+    int32_t reserved[5];
 } dcx_timing;
 
 dcx_status dcx_config_init(dcx_config* cfg);
@@ -331,6 +332,40 @@ dcx_status dcx_serve_stop(dcx_server* sv, dcx_serve_stats* stats);
  * finaliser).  Independent of evaluation order; no reference counterpart
  * (the reference compares whole arrays on the host, src/utils.h:13-43). */
 dcx_status dcx_state_hash(dcx_ctx* ctx, dcx_graph* g, uint64_t* hash);
+
+/* ---- learned variant scheduler ---------------------------------------
+ * A small fp32 MLP (8 inputs, two hidden layers of 32 ReLU units, one
+ * output) predicts log2(ms) of a row pass from its features (sizes of the
+ * (k,0)-core, the vertices whose kmax is k, the level/phase counts of the
+ * row's last run, and the team width it would run on); forward, backward
+ * and Adam run in hand-written HIP kernels.  Attached to a context, it sets
+ * each pass's team width in the batched launch (the narrowest width whose
+ * predicted time fits the smallest feasible makespan); without it the width
+ * follows the pass's share of the work.  Every width gives the same bits.
+ * No reference counterpart (the reference's schedule is fixed,
+ * main.cu:197-262). */
+typedef struct dcx_mlp dcx_mlp;
+uint32_t dcx_mlp_param_count(void);
+/* Parameter layout: W1[32][8], b1[32], W2[32][32], b2[32], W3[32], b3. */
+dcx_status dcx_mlp_create(dcx_ctx* ctx, uint64_t seed, dcx_mlp** out);
+dcx_status dcx_mlp_free(dcx_mlp* m);
+dcx_status dcx_mlp_get_params(dcx_mlp* m, float* out);
+dcx_status dcx_mlp_set_params(dcx_mlp* m, const float* in);
+/* y[s] = f(X[s*8 .. s*8+8)) on the device. */
+dcx_status dcx_mlp_predict(dcx_mlp* m, const float* X, uint32_t n, float* y);
+/* grad = sum_s d/dp 0.5 (f(X_s) - T_s)^2, *loss = sum_s 0.5 (f(X_s) - T_s)^2. */
+dcx_status dcx_mlp_gradient(dcx_mlp* m, const float* X, const float* T, uint32_t n, float* grad,
+                            float* loss);
+/* `epochs` full-batch Adam steps (beta 0.9 / 0.999) on the mean of the above. */
+dcx_status dcx_mlp_train(dcx_mlp* m, const float* X, const float* T, uint32_t n, uint32_t epochs,
+                         float lr, float* final_loss);
+/* Attach (or with NULL detach) the model as the batched rows scheduler. */
+dcx_status dcx_set_scheduler(dcx_ctx* ctx, dcx_mlp* m);
+/* Training samples from the resident decomposition's pass log (rows run in
+ * the batched launch): features (with_log = 0: level/phase counts set to
+ * unknown) and targets log2(ms + 0.01); *n = samples available. */
+dcx_status dcx_graph_pass_features(dcx_ctx* ctx, const dcx_graph* g, int32_t with_log,
+                                   uint32_t cap, float* X, float* T, uint32_t* n);
 
 /* Autotuner for the batched rows launch (no reference counterpart: the
  * reference's launch shape is fixed, src/common.h:38-56).  Times the rows

@@ -306,3 +306,24 @@ def test_state_hash_restatement_on_fixture():
     km1 = km0.copy()
     km1[[0, 1]] = km1[[1, 0]]
     assert graphs.state_hash(km1, rows0) != h
+
+
+def test_mlp_reference_gradient_matches_finite_differences():
+    """The CPU restatement of the cost model is pinned by central finite
+    differences (so it can judge the device kernels)."""
+    import mlp_ref
+    rng = np.random.default_rng(3)
+    p = rng.normal(0, 0.3, mlp_ref.N_PARAMS)
+    X = rng.uniform(0, 1, (20, 8))
+    T = rng.normal(0, 1, 20)
+    g, loss = mlp_ref.gradient(p, X, T)
+    y, _ = mlp_ref.forward(p, X)
+    assert abs(loss - 0.5 * ((y - T) ** 2).sum()) < 1e-12
+    idx = rng.choice(p.size, 60, replace=False)
+    h = 1e-6
+    for i in idx:
+        q = p.copy(); q[i] += h
+        lp = 0.5 * ((mlp_ref.forward(q, X)[0] - T) ** 2).sum()
+        q[i] -= 2 * h
+        lm = 0.5 * ((mlp_ref.forward(q, X)[0] - T) ** 2).sum()
+        assert abs((lp - lm) / (2 * h) - g[i]) < 1e-5 * max(1.0, abs(g[i])), i
