@@ -47,7 +47,7 @@ def core_sizes(kmaxes, src, dst, K):
 def read_traffic():
     """HBM bytes per rows-kernel launch from the committed rocprofv3 PMC
     summary (FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated), or None."""
-    p = os.path.join(ROOT, "profiles", "rows_batched_rmat20_pmc.txt")
+    p = os.path.join(ROOT, "profiles", "k_rows_teams_rmat20_pmc.txt")
     try:
         for line in open(p):
             if line.startswith("traffic_bytes_per_launch:"):
@@ -83,6 +83,9 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-variant timing")
     ap.add_argument("--no-hindex", action="store_true", help="skip the h-index timing")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
+    ap.add_argument("--shape", default="",
+                    help="rows launch shape SLOTS,TEAM_MAX,SHRINK_AT instead of the autotuner "
+                         "(e.g. the shape an earlier run picked, for counter runs)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--baseline-graphs", type=int, default=256,
@@ -116,10 +119,11 @@ def run_rmat(args):
 
     variant = {"batched": dcx.VARIANT_BATCHED, "peel": dcx.VARIANT_PEEL,
                "hindex": dcx.VARIANT_HINDEX}[args.variant]
-    ctx = dcx.Context(0)
+    shape = [int(x) for x in args.shape.split(",")] if args.shape else [0, 0, 0]
+    ctx = dcx.Context(0, rows_slots=shape[0], team_max=shape[1], shrink_at=shape[2])
     g = ctx.upload_edges(V, src, dst, slack=0)
     tune = None
-    if variant == dcx.VARIANT_BATCHED and not args.no_tune:
+    if variant == dcx.VARIANT_BATCHED and not args.no_tune and not args.shape:
         g.kmax()
         r = g.autotune(reps=2)
         tune = {"rows_slots": r.rows_slots, "team_max": r.team_max, "shrink_at": r.shrink_at,
@@ -142,6 +146,10 @@ def run_rmat(args):
     print(f"# steps done: {ms_per_step:.2f} ms/step kmax={K} lmax={lmax}", file=sys.stderr,
           flush=True)
 
+    # Fingerprint of the result (kmaxes and all rows, hashed on the device:
+    # one streaming read of (K + 2) * V int32, also the known-size read that
+    # calibrates the FETCH_SIZE counter).
+    state_hash = g.state_hash()
     kmaxes = g.kmaxes()
     vk, ek = core_sizes(kmaxes, src.astype(np.int64), dst.astype(np.int64), K)
     b_rows = float((8 * ek + 20 * vk).sum())
@@ -213,7 +221,8 @@ def run_rmat(args):
             "workload": f"rmat{scale} full decomposition (kmax pass + all l-rows)",
             "V": V, "E": E, "samples": samples, "abcd": [0.57, 0.19, 0.19, 0.05], "seed": 1,
             "kmax": K, "lmax": lmax, "variant": args.variant,
-            "rows_slots": timing.rows_slots,
+            "rows_slots": timing.rows_slots, "shape": args.shape or "autotuned",
+            "state_hash": f"{state_hash:016x}",
         },
         "breakdown_ms": {
             "kmax_pass": round(float(np.mean(kms)), 3),
