@@ -58,6 +58,9 @@ constexpr uint32_t kTraceCap = 1u << 16;
 // batched launch.
 PassSpec traced(dcx_ctx* c, PassSpec p, bool team = false) {
     p.trace_k = -1;
+    p.pull_ratio = c->cfg.pull_ratio < 0 ? 0u
+                   : c->cfg.pull_ratio == 0 ? kDefaultPullRatio
+                                            : (uint32_t)c->cfg.pull_ratio;
     if (c->cfg.trace_phases <= 0 || (c->cfg.trace_phases == 1) == team) return p;
     if (team) p.trace_k = c->cfg.trace_phases - 2;
     if (!c->trace) {
@@ -1018,8 +1021,8 @@ dcx_status dcx_last_trace(dcx_ctx* ctx, dcx_phase_trace* out, uint32_t cap, uint
                             hipMemcpyDeviceToHost));
         for (uint32_t i = 0; i < m; ++i) {
             const uint32_t* e = w.data() + (size_t)i * kTraceWords;
-            out[i].level = (int32_t)(e[kTraceLevel] & 0x7fffffffu);
-            out[i].seed = (e[kTraceLevel] >> 31) ? 1 : 0;
+            out[i].level = (int32_t)(e[kTraceLevel] & kTraceLevelMask);
+            out[i].seed = (e[kTraceLevel] & kTraceSeedBit) ? 1 : (e[kTraceLevel] & kTracePullBit) ? 2 : 0;
             out[i].removed = e[kTraceFrontier];
             out[i].alive = e[kTraceAlive];
             out[i].us = 1000.0 * e[kTraceTicks] / ctx->wall_khz;

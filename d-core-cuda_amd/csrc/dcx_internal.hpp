@@ -50,22 +50,35 @@ struct PassSpec {
     uint32_t trace_cap;     // entries the trace can hold
     int32_t trace_k;        // batched launch: the row whose team pass is traced
     uint32_t* fail_rec;     // nullable; [kFailWords] what the first failing member saw
+    // A cascade phase pulls (every alive vertex counts its neighbours in the
+    // frontier) instead of pushing decrements from the frontier when the
+    // alive vertices' lists hold at most pull_ratio times the frontier's
+    // edges; 0 = always push.
+    uint32_t pull_ratio;
 };
 
 enum : uint32_t { kFailWords = 16 };
 
 // Phase-counter slots of a peel pass: 3 slots, each in its own 128-byte
 // line (kSlotStride words), kSlotWords words per pass or team.
+// Words of a slot: [0] frontier appends, [1] alive-list appends, [2] smallest
+// key degree above the level, [3] the leader's phase, [4..5] edges of the
+// frontier's lists (uint64), [6..7] edges of the alive lists (uint64, first
+// level only).
 enum : uint32_t { kSlotStride = 32, kSlotWords = 3 * kSlotStride };
 
 // Phase trace entry (peel passes run on one team when tracing is on).
 enum : uint32_t {
-    kTraceLevel = 0,    // level L; bit 31 set for a seed phase
+    kTraceLevel = 0,    // level L (low 30 bits), kTraceSeedBit, kTracePullBit
     kTraceFrontier = 1, // vertices removed by the phase
     kTraceAlive = 2,    // alive-list length after the phase
     kTraceTicks = 3,    // wall-clock ticks since the previous phase ended
     kTraceWords = 4
 };
+
+constexpr uint32_t kTraceSeedBit = 0x80000000u;  // a seeding phase
+constexpr uint32_t kTracePullBit = 0x40000000u;  // a cascade phase run as a pull
+constexpr uint32_t kTraceLevelMask = 0x3fffffffu;
 
 // Per-pass log written by the device (one entry per pass).
 enum : uint32_t {
@@ -91,8 +104,9 @@ struct PassScratch {
     uint32_t* f1;  // frontier pong
     uint32_t* a0;  // alive list ping
     uint32_t* a1;  // alive list pong
+    uint32_t* st;  // phase in which a vertex was removed (~0u while alive)
 };
-constexpr int kScratchArrays = 9;
+constexpr int kScratchArrays = 10;
 
 __host__ __device__ inline PassScratch scratch_at(void* base, uint64_t cap) {
     uint32_t* p = reinterpret_cast<uint32_t*>(base);
@@ -106,6 +120,7 @@ __host__ __device__ inline PassScratch scratch_at(void* base, uint64_t cap) {
     s.f1 = p + 6 * cap;
     s.a0 = p + 7 * cap;
     s.a1 = p + 8 * cap;
+    s.st = p + 9 * cap;
     return s;
 }
 
@@ -125,6 +140,9 @@ struct TeamDesc {
 // XCD b % kXcds.  A team is built from workgroups of one XCD, so the data
 // its members hand each other between phases stays within one L2.
 constexpr uint32_t kXcds = 8;
+
+// Default pull_ratio of the peel passes (PassSpec::pull_ratio).
+constexpr uint32_t kDefaultPullRatio = 4;
 
 // ---- cost-model MLP (mlp_kernels.hip) ----------------------------------
 constexpr int kMlpIn = 8, kMlpHidden = 32;

@@ -62,6 +62,34 @@ __device__ __forceinline__ void wave_append(bool pred, uint32_t val, uint32_t* a
     if (pred) arr[base + __popcll(m & ((1ull << lane_id()) - 1ull))] = val;
 }
 
+// wave_append that also adds the appended vertices' list lengths `len` to
+// the 64-bit total *etot (one more atomic per wavefront that appends).
+__device__ __forceinline__ void wave_append_len(bool pred, uint32_t val, uint32_t len,
+                                                uint32_t* arr, uint32_t* ctr,
+                                                unsigned long long* etot) {
+    const unsigned long long m = __ballot(pred);
+    if (m == 0ull) return;
+    const int leader = __ffsll(m) - 1;
+    uint32_t l = pred ? len : 0u;
+#pragma unroll
+    for (int d = W / 2; d > 0; d >>= 1) l += (uint32_t)__shfl_xor(l, d, W);
+    uint32_t base = 0;
+    if (lane_id() == leader) {
+        base = atomicAdd(ctr, (uint32_t)__popcll(m));
+        consume((uint32_t)atomicAdd(etot, (unsigned long long)l));
+    }
+    base = __shfl(base, leader, W);
+    if (pred) arr[base + __popcll(m & ((1ull << lane_id()) - 1ull))] = val;
+}
+
+// Wavefront sum of x into the 64-bit total *dst (performed before returning).
+__device__ __forceinline__ void wave_sum_to(uint32_t x, unsigned long long* dst) {
+    unsigned long long y = x;
+#pragma unroll
+    for (int d = W / 2; d > 0; d >>= 1) y += __shfl_xor(y, d, W);
+    if (lane_id() == 0 && y) consume((uint32_t)atomicAdd(dst, y));
+}
+
 // Wavefront minimum, then one atomicMin per wavefront (performed before
 // returning: its result is consumed, see coherent_write).
 __device__ __forceinline__ void wave_min_to(uint32_t x, uint32_t* dst) {
@@ -146,32 +174,69 @@ __device__ __forceinline__ uint32_t lower_bound(const uint32_t* a, uint32_t n, u
 // Phase counters: three slots (triple-buffered by phase), each in its own
 // 128-byte line, touched only with RMW atomics (coherent_read/_write of
 // grid_sync.hpp).  Words: [0] frontier appends, [1] alive-list appends,
-// [2] smallest key degree seen above the level.
+// [2] smallest key degree seen above the level, [3] the leader's phase,
+// [4..5] list edges of the new frontier, [6..7] list edges of the alive
+// vertices (first level only).
 __device__ __forceinline__ void slot_reset(uint32_t* s) {
     coherent_write(s + 0, 0u);
     coherent_write(s + 1, 0u);
     coherent_write(s + 2, (uint32_t)UINT_MAX);
+    coherent_write(s + 4, 0u);
+    coherent_write(s + 5, 0u);
+    coherent_write(s + 6, 0u);
+    coherent_write(s + 7, 0u);
 }
+
+__device__ __forceinline__ unsigned long long* slot_u64(uint32_t* s, int w) {
+    return reinterpret_cast<unsigned long long*>(s + w);
+}
+
+struct SlotView {
+    uint32_t nF, nA, mn;
+    uint64_t fedges, aedges;
+};
 
 // One coherent read of a slot per workgroup, shared through LDS; every
 // thread must call it (contains workgroup barriers).
-__device__ __forceinline__ void slot_read3(uint32_t* s, uint32_t* lds, uint32_t& a, uint32_t& b,
-                                           uint32_t& c) {
+__device__ __forceinline__ SlotView slot_read(uint32_t* s, uint32_t* lds) {
     if (threadIdx.x == 0) {
         lds[0] = coherent_read(s + 0);
         lds[1] = coherent_read(s + 1);
         lds[2] = coherent_read(s + 2);
+        lds[3] = coherent_read(s + 4);
+        lds[4] = coherent_read(s + 5);
+        lds[5] = coherent_read(s + 6);
+        lds[6] = coherent_read(s + 7);
     }
     __syncthreads();
-    a = lds[0];
-    b = lds[1];
-    c = lds[2];
+    SlotView v;
+    v.nF = lds[0];
+    v.nA = lds[1];
+    v.mn = lds[2];
+    v.fedges = (uint64_t)lds[3] | ((uint64_t)lds[4] << 32);
+    v.aedges = (uint64_t)lds[5] | ((uint64_t)lds[6] << 32);
     __syncthreads();
+    return v;
 }
 
-__device__ __forceinline__ void poison(const PassScratch& S, uint32_t x) {
-    __hip_atomic_store(S.kd + x, kPoison, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(S.cd + x, kPoison, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ void store_agent(int32_t* p, int32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Marks x removed in phase `ph`: both counters poisoned, removal phase
+// recorded for the pull phases' frontier test.
+__device__ __forceinline__ void poison(const PassScratch& S, uint32_t x, uint32_t ph) {
+    store_agent(S.kd + x, kPoison);
+    store_agent(S.cd + x, kPoison);
+    __hip_atomic_store(S.st + x, ph, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Lengths of v's K and C lists inside the pass (member prefixes in the core
+// layout, whole lists otherwise).
+__device__ __forceinline__ void list_lens(const PassSpec& P, const PassScratch& S, uint32_t v,
+                                          uint32_t& kl, uint32_t& cl) {
+    kl = P.core ? S.rk[v] : (uint32_t)P.K.deg[v];
+    cl = P.core ? S.rc[v] : (uint32_t)P.C.deg[v];
 }
 
 // One full peel pass.  MULTI: a team of workgroups (gs != nullptr; `slots`
@@ -233,12 +298,14 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
     const uint64_t t_start = wall_clock64();
     uint64_t t_prev = t_start;
     // phase trace, written by the team leader after each barrier
-    auto trace = [&](uint32_t phase, bool seed, int32_t lvl, uint32_t nfr, uint32_t nal) {
+    auto trace = [&](uint32_t phase, bool seed, bool pull, int32_t lvl, uint32_t nfr,
+                     uint32_t nal) {
         if (!P.trace || tw != 0 || tid != 0) return;
         const uint64_t now = wall_clock64();
         if (phase < P.trace_cap) {
             uint32_t* e = P.trace + kTraceWords * (1 + (size_t)phase);
-            e[kTraceLevel] = (uint32_t)lvl | (seed ? 0x80000000u : 0u);
+            e[kTraceLevel] = ((uint32_t)lvl & kTraceLevelMask) | (seed ? kTraceSeedBit : 0u) |
+                             (pull ? kTracePullBit : 0u);
             e[kTraceFrontier] = nfr;
             e[kTraceAlive] = nal;
             e[kTraceTicks] = (uint32_t)min<uint64_t>(now - t_prev, 0xffffffffull);
@@ -247,7 +314,7 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
         t_prev = now;
     };
 
-    __shared__ uint32_t s_rd[3];
+    __shared__ uint32_t s_rd[8];
     if (tw == 0 && tid == 0) {
         slot_reset(slots);
         slot_reset(slots + kSlotStride);
@@ -269,6 +336,7 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
             S.cd[i] = P.C.deg[i];
         }
         lval[i] = -1;
+        S.st[i] = ~0u;
     }
     if (!barrier()) return;
 
@@ -279,6 +347,10 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
     // Without membership restriction the constraint can fail before any
     // removal (KList semantics on the whole graph, main.cu:144-169).
     const bool check_c = !P.core && k > 0;
+    // List edges of the alive vertices and of the current frontier: every
+    // member of the team derives the same values from the slots, so all of
+    // them choose the same kind of cascade phase.
+    uint64_t pa = 0, fe = 0;
     while (removed < nm) {
         // Each removal needs at most two level iterations (a level number
         // taken from a stale minimum seeds nothing, the next one seeds the
@@ -288,7 +360,12 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
             break;
         }
         if (MULTI && !solo && shrink_at > 0) {
-            const uint32_t want = max(1u, min(tn, (uint32_t)((nA + shrink_at - 1) / shrink_at)));
+            // members worth keeping: one per shrink_at alive vertices or per
+            // 16 * shrink_at edges of their lists, whichever asks for more
+            // (a dense core has few vertices but long lists)
+            const uint64_t by_v = ((uint64_t)nA + shrink_at - 1) / shrink_at;
+            const uint64_t by_e = (pa + 16ull * shrink_at - 1) / (16ull * shrink_at);
+            const uint32_t want = (uint32_t)max<uint64_t>(1, min<uint64_t>(tn, max(by_v, by_e)));
             if (want == 1 || 2 * want <= tn) {
                 // The narrowed team resets phase slots without the members
                 // that leave; they must all have finished reading the last
@@ -304,6 +381,7 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
         uint32_t* cur = slots + kSlotStride * (s % 3);
         if (tw == 0 && tid == 0) slot_reset(slots + kSlotStride * ((s + 1) % 3));
         uint32_t mn = UINT_MAX;
+        const bool first = A == nullptr;
         for (uint32_t base = tw * BLK; base < nA; base += tn * BLK) {
             const uint32_t i = base + tid;
             const bool has = i < nA;
@@ -312,20 +390,31 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
             const int32_t d = alive ? S.kd[v] : 0;
             const bool seed = alive && (d <= L || (check_c && S.cd[v] < k));
             const bool keep = alive && !seed;
+            uint32_t len = 0;
+            if (seed || (first && keep)) {
+                uint32_t kl, cl;
+                list_lens(P, S, v, kl, cl);
+                len = kl + cl;
+            }
             if (seed) {
                 lval[v] = L;
-                poison(S, v);
+                poison(S, v, s);
             }
             if (keep) mn = min(mn, (uint32_t)d);
-            wave_append(seed, v, F, &cur[0]);
+            wave_append_len(seed, v, len, F, &cur[0], slot_u64(cur, 4));
             wave_append(keep, v, B, &cur[1]);
+            if (first) wave_sum_to(keep ? len : 0u, slot_u64(cur, 6));
         }
         wave_min_to(mn, &cur[2]);
         if (!barrier()) return;
-        uint32_t nF, mnall;
-        slot_read3(cur, s_rd, nF, nA, mnall);
+        const SlotView sv = slot_read(cur, s_rd);
+        uint32_t nF = sv.nF;
+        nA = sv.nA;
+        uint32_t mnall = sv.mn;
+        fe = sv.fedges;
+        pa = first ? sv.aedges : pa - fe;
 
-        trace(s, true, L, nF, nA);
+        trace(s, true, false, L, nF, nA);
         A = B;
         {
             uint32_t* t = B;
@@ -340,82 +429,158 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
             uint32_t* nxt = slots + kSlotStride * (s % 3);
             if (tw == 0 && tid == 0) slot_reset(slots + kSlotStride * ((s + 1) % 3));
             uint32_t mn2 = UINT_MAX;
-            // Frontier items per workgroup: a team splits even a small
-            // frontier over all its members (whole tiles would leave most
-            // of them idle while a few walk long edge lists).
-            const uint32_t ipw = solo ? (uint32_t)BLK : min((uint32_t)BLK, (nF + tn - 1) / tn);
-            for (uint32_t base = tw * ipw; base < nF; base += tn * ipw) {
-                const uint32_t i = base + tid;
-                const bool has = tid < ipw && i < nF;
-                const uint32_t v = has ? F[i] : 0u;
-                __syncthreads();
-                uint32_t kl = 0, cl = 0;
-                if (has) {
-                    T.koff[tid] = P.K.off[v];
-                    T.coff[tid] = P.C.off[v];
-                    kl = P.core ? S.rk[v] : (uint32_t)P.K.deg[v];
-                    cl = P.core ? S.rc[v] : (uint32_t)P.C.deg[v];
-                }
-                T.klen[tid] = kl;
-                if (tid == 0) T.nitems = min(nF - base, ipw);
-                T.scan[tid] = block_excl_scan<BLK>(kl + cl, T.wsum, &T.total);
-                // every thread's prefix is in LDS before any thread searches it
-                __syncthreads();
-                const uint32_t total = T.total;
-                // kUnroll independent edges per thread per step, so their
-                // loads and atomics overlap instead of forming one chain.
-                for (uint32_t eb = 0; eb < total; eb += kUnroll * BLK) {
-                    uint32_t x[kUnroll];
-                    bool ks[kUnroll], ok[kUnroll], claim[kUnroll];
+            const bool pull = P.pull_ratio != 0 && pa <= (uint64_t)P.pull_ratio * fe;
+            if (pull) {
+                // Pull: every alive vertex of the level's list counts its
+                // neighbours removed in the previous phase and updates its
+                // own counters -- no atomics on shared counters, which is
+                // what a dense frontier (many edges into few vertices)
+                // needs.  The counts equal the decrements a push would
+                // apply, so the claims and the minimum are the same.  One
+                // wavefront per alive vertex: coalesced list reads, the
+                // count is a popcount of ballots.
+                const uint32_t sprev = s - 1;
+                const uint32_t nw = BLK / W;
+                const uint32_t lane = (uint32_t)lane_id();
+                for (uint32_t i = tw * nw + tid / W; i < nA; i += tn * nw) {
+                    const uint32_t x = A[i];
+                    if (lval[x] != -1) continue;  // removed earlier in this level
+                    uint32_t kl, cl;
+                    list_lens(P, S, x, kl, cl);
+                    const uint32_t* kn = P.K.nbr + P.K.off[x];
+                    const uint32_t* cn = P.C.nbr + P.C.off[x];
+                    uint32_t ck = 0, cc = 0;  // the same in every lane
+                    for (uint32_t jb = 0; jb < kl + cl; jb += kUnroll * W) {
+                        bool hit[kUnroll], isk[kUnroll];
 #pragma unroll
-                    for (int u = 0; u < kUnroll; ++u) {
-                        const uint32_t e = eb + u * BLK + tid;
-                        ok[u] = e < total;
-                        x[u] = 0;
-                        ks[u] = false;
-                        if (ok[u]) {
-                            const uint32_t it = tile_find<BLK>(T, e);
-                            const uint32_t t = e - T.scan[it];
-                            ks[u] = t < T.klen[it];
-                            x[u] = ks[u] ? P.K.nbr[T.koff[it] + t]
-                                         : P.C.nbr[T.coff[it] + (t - T.klen[it])];
+                        for (int u = 0; u < kUnroll; ++u) {
+                            const uint32_t e = jb + u * W + lane;
+                            isk[u] = e < kl;
+                            const bool ok = e < kl + cl;
+                            const uint32_t y = !ok ? 0u : isk[u] ? kn[e] : cn[e - kl];
+                            hit[u] = ok && S.st[y] == sprev;
+                        }
+#pragma unroll
+                        for (int u = 0; u < kUnroll; ++u) {
+                            ck += (uint32_t)__popcll(__ballot(hit[u] && isk[u]));
+                            cc += (uint32_t)__popcll(__ballot(hit[u] && !isk[u]));
                         }
                     }
-                    int32_t old[kUnroll];
-#pragma unroll
-                    for (int u = 0; u < kUnroll; ++u) {
-                        // K-side edge: x loses a C-neighbour (constraint
-                        // degree); C-side edge: x loses a K-neighbour (key)
-                        old[u] = ok[u] ? atomicSub(ks[u] ? &S.cd[x[u]] : &S.kd[x[u]], 1) : 0;
+                    bool claim = false;
+                    if ((ck | cc) && lane == 0) {
+                        // x's own counters: nothing else writes them in a
+                        // pull phase
+                        const int32_t nk = S.kd[x] - (int32_t)ck;
+                        const int32_t nc = S.cd[x] - (int32_t)cc;
+                        claim = nk <= L || (k > 0 && nc < k);
+                        if (claim) {
+                            lval[x] = L;
+                            poison(S, x, s);
+                        } else {
+                            if (ck) {
+                                store_agent(S.kd + x, nk);
+                                mn2 = min(mn2, (uint32_t)nk);
+                            }
+                            if (cc) store_agent(S.cd + x, nc);
+                        }
                     }
+                    wave_append_len(claim, x, kl + cl, G, &nxt[0], slot_u64(nxt, 4));
+                }
+            } else {
+                // Push: frontier items per workgroup -- a team splits even
+                // a small frontier over all its members (whole tiles would
+                // leave most of them idle while a few walk long edge lists).
+                const uint32_t ipw =
+                    solo ? (uint32_t)BLK : min((uint32_t)BLK, (nF + tn - 1) / tn);
+                for (uint32_t base = tw * ipw; base < nF; base += tn * ipw) {
+                    const uint32_t i = base + tid;
+                    const bool has = tid < ipw && i < nF;
+                    const uint32_t v = has ? F[i] : 0u;
+                    __syncthreads();
+                    uint32_t kl = 0, cl = 0;
+                    if (has) {
+                        T.koff[tid] = P.K.off[v];
+                        T.coff[tid] = P.C.off[v];
+                        list_lens(P, S, v, kl, cl);
+                    }
+                    T.klen[tid] = kl;
+                    if (tid == 0) T.nitems = min(nF - base, ipw);
+                    T.scan[tid] = block_excl_scan<BLK>(kl + cl, T.wsum, &T.total);
+                    // every thread's prefix is in LDS before any thread searches it
+                    __syncthreads();
+                    const uint32_t total = T.total;
+                    // kUnroll independent edges per thread per step, so their
+                    // loads and atomics overlap instead of forming one chain.
+                    for (uint32_t eb = 0; eb < total; eb += kUnroll * BLK) {
+                        uint32_t x[kUnroll];
+                        bool ks[kUnroll], ok[kUnroll], claim[kUnroll];
 #pragma unroll
-                    for (int u = 0; u < kUnroll; ++u) {
-                        claim[u] = false;
-                        if (ok[u]) {
-                            if (ks[u]) {
-                                claim[u] = k > 0 && old[u] == k;
-                            } else if (old[u] == L + 1) {
-                                claim[u] = true;
-                            } else if (old[u] - 1 > L) {
-                                mn2 = min(mn2, (uint32_t)(old[u] - 1));
+                        for (int u = 0; u < kUnroll; ++u) {
+                            const uint32_t e = eb + u * BLK + tid;
+                            ok[u] = e < total;
+                            x[u] = 0;
+                            ks[u] = false;
+                            if (ok[u]) {
+                                const uint32_t it = tile_find<BLK>(T, e);
+                                const uint32_t t = e - T.scan[it];
+                                ks[u] = t < T.klen[it];
+                                x[u] = ks[u] ? P.K.nbr[T.koff[it] + t]
+                                             : P.C.nbr[T.coff[it] + (t - T.klen[it])];
                             }
                         }
-                        if (claim[u]) {
-                            claim[u] = atomicCAS(&lval[x[u]], -1, L) == -1;
-                            if (claim[u]) poison(S, x[u]);
+                        // Half of all edge visits reach a neighbour that
+                        // left earlier (every edge is visited from both
+                        // ends); a plain load of its level skips those
+                        // atomics.  A stale -1 only costs the atomic, which
+                        // then finds a poisoned counter.
+#pragma unroll
+                        for (int u = 0; u < kUnroll; ++u) ok[u] = ok[u] && lval[x[u]] == -1;
+                        int32_t old[kUnroll];
+#pragma unroll
+                        for (int u = 0; u < kUnroll; ++u) {
+                            // K-side edge: x loses a C-neighbour (constraint
+                            // degree); C-side edge: x loses a K-neighbour (key)
+                            old[u] = ok[u] ? atomicSub(ks[u] ? &S.cd[x[u]] : &S.kd[x[u]], 1) : 0;
+                        }
+#pragma unroll
+                        for (int u = 0; u < kUnroll; ++u) {
+                            claim[u] = false;
+                            if (ok[u]) {
+                                if (ks[u]) {
+                                    claim[u] = k > 0 && old[u] == k;
+                                } else if (old[u] == L + 1) {
+                                    claim[u] = true;
+                                } else if (old[u] - 1 > L) {
+                                    mn2 = min(mn2, (uint32_t)(old[u] - 1));
+                                }
+                            }
+                            if (claim[u]) {
+                                claim[u] = atomicCAS(&lval[x[u]], -1, L) == -1;
+                                if (claim[u]) poison(S, x[u], s);
+                            }
+                        }
+#pragma unroll
+                        for (int u = 0; u < kUnroll; ++u) {
+                            uint32_t len = 0;
+                            if (claim[u]) {
+                                uint32_t a, b;
+                                list_lens(P, S, x[u], a, b);
+                                len = a + b;
+                            }
+                            wave_append_len(claim[u], x[u], len, G, &nxt[0], slot_u64(nxt, 4));
                         }
                     }
-#pragma unroll
-                    for (int u = 0; u < kUnroll; ++u) wave_append(claim[u], x[u], G, &nxt[0]);
                 }
             }
             wave_min_to(mn2, &nxt[2]);
             if (!barrier()) return;
-            uint32_t unused, mn_c;
-            slot_read3(nxt, s_rd, nF, unused, mn_c);
-            mnall = min(mnall, mn_c);
+            const SlotView sv2 = slot_read(nxt, s_rd);
+            nF = sv2.nF;
+            mnall = min(mnall, sv2.mn);
+            fe = sv2.fedges;
+            pa -= fe;
 
-            trace(s, false, L, nF, nA);
+            trace(s, false, pull, L, nF, nA);
             {
                 uint32_t* t = F;
                 F = G;
@@ -477,7 +642,7 @@ __global__ __launch_bounds__(kBlk) void k_rows_teams(PassSpec base, const uint32
                                                      uint64_t timeout_ticks, uint32_t* err,
                                                      uint32_t shrink_at) {
     __shared__ TileLds<kBlk> T;
-    __shared__ uint32_t lslots[kSlotWords];
+    __shared__ __align__(16) uint32_t lslots[kSlotWords];
     const uint32_t team = block_team[blockIdx.x];
     if (team == ~0u) return;  // an idle block (its XCD had no team for it)
     const TeamDesc td = teams[team];

@@ -150,7 +150,14 @@ typedef struct dcx_config {
                                 (kmax pass, plain rows); 2 + k = the team pass
                                 of row k in the batched launch; 0 = off */
     uint64_t scratch_budget; /* max bytes of pass scratch (0 = no limit) */
-    uint64_t reserved[4];
+    /* A cascade phase of a peel pass pulls (each alive vertex counts its
+     * neighbours in the frontier) instead of pushing decrements from the
+     * frontier when the alive vertices' lists hold at most pull_ratio times
+     * the frontier's edges: 0 = library default (4), -1 = always push.
+     * Results are the same either way. */
+    int32_t pull_ratio;
+    int32_t reserved32;
+    uint64_t reserved[3];
 } dcx_config;
 
 /* Timings of the last decomposition on a context, from HIP events on the
@@ -217,7 +224,8 @@ typedef struct dcx_pass_log {
  * the per-iteration frontier statistics the scheduler learns from. */
 typedef struct dcx_phase_trace {
     int32_t level;     /* level L of the phase */
-    int32_t seed;      /* 1: seeding phase (alive-list scan), 0: cascade */
+    int32_t seed;      /* 1: seeding phase (alive-list scan), 0: cascade
+                          (push), 2: cascade run as a pull */
     uint32_t removed;  /* vertices the phase removed */
     uint32_t alive;    /* alive-list length after it */
     double us;         /* device time since the previous phase */

@@ -143,6 +143,41 @@ def test_batched_launch_shapes(shape):
     assert np.array_equal(km, ref[1]) and np.array_equal(rows, ref[3])
 
 
+@pytest.mark.parametrize("ratio", [-1, 1, 4, 1 << 24])
+def test_push_and_pull_phases(ratio):
+    """Cascade phases pushed from the frontier (ratio -1), pulled by the
+    alive vertices whenever allowed (1 << 24), or chosen per phase: the same
+    bits for the kmax pass, the rows of both peel variants and KList on the
+    whole graph."""
+    cases = [("rmat14", 1 << 14) + tuple(dcx.gen_rmat(14, 16 << 14, 5)),
+             ("er", 3000) + tuple(dcx.gen_er(3000, 30000, 8)),
+             ("complete12",) + graphs.complete(12)]
+    for name, V, src, dst in cases:
+        ref = oracle.decompose(oracle.csr_from_edges(V, src, dst))
+        with dcx.Context(0, pull_ratio=ratio) as c:
+            for var in (dcx.VARIANT_BATCHED, dcx.VARIANT_PEEL):
+                say(f"{name} pull_ratio={ratio} variant={var}")
+                K, km, lmax, rows, _, _ = gpu_decompose(c, V, src, dst, var)
+                assert K == ref[0] and lmax == ref[2], (name, var)
+                assert np.array_equal(km, ref[1]) and np.array_equal(rows, ref[3]), (name, var)
+            g = c.upload_edges(V, src, dst)
+            for k in (1, 3):
+                say(f"{name} pull_ratio={ratio} klist k={k}")
+                lk, lv = g.klist(k, dcx.VARIANT_PEEL)
+                rlk, rlv = oracle.klist(oracle.csr_from_edges(V, src, dst), k)
+                assert lk == rlk and np.array_equal(lv, rlv), (name, k)
+            g.free()
+    if ratio == 1 << 24:
+        # the pull path did run: the phase trace marks pulled phases
+        src, dst = dcx.gen_rmat(12, 16 << 12, 3)
+        with dcx.Context(0, pull_ratio=ratio, trace_phases=1) as c:
+            g = c.upload_edges(1 << 12, src, dst)
+            g.kmax()
+            tr = c.last_trace()
+            g.free()
+        assert (tr["seed"] == 2).any() and not (tr["seed"] == 0).any()
+
+
 def test_team_barrier_visibility_repeated():
     """Multi-workgroup teams hand frontiers, alive lists and levels to each
     other through the team barrier; any write the barrier fails to publish
