@@ -260,7 +260,9 @@ Sched make_schedule(const dcx_graph* g, int32_t kr, uint32_t G, uint32_t team_ma
         w[k] = wts ? (*wts)[k] : 2.0 * (double)g->h_ek[k] + (double)g->h_vk[k] + 64.0;
         W += w[k];
     }
-    const double share = W / G;
+    // widths follow each pass's share of the work within the 3/4 of the
+    // grid that wide teams may take
+    const double share = W / std::max<uint32_t>(G * 3 / 4, 1);
     std::vector<uint32_t> idx(np);
     for (uint32_t k = 0; k < np; ++k) idx[k] = k;
     std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return w[a] > w[b]; });
@@ -556,59 +558,107 @@ void compute_rows(dcx_ctx* c, dcx_graph* g, int variant, int32_t kr = -1) {
             g->h_pass_team[k + 1] = (uint16_t)c->grid_blocks;
         }
     } else {
-        // batched: one persistent launch, teams of workgroups per pass
+        // batched: rows whose core fits the LDS run LDS-resident, one
+        // workgroup each, taken longest first from a queue; the larger rows
+        // run on teams of workgroups in one persistent launch
+        c->timing.rows_scheduler = 0;
+        int32_t ksplit = 0;  // rows [0, ksplit) on teams, [ksplit, kr] in LDS
+        while (ksplit <= kr && g->h_vk[ksplit] > c->lds_cap) ++ksplit;
         const uint32_t G = (uint32_t)(c->cfg.rows_slots > 0 ? c->cfg.rows_slots
                                                               : c->num_cus * kRowsBlocksPerCu);
         const uint32_t tmax = c->cfg.team_max > 0 ? (uint32_t)c->cfg.team_max : kTeamMax;
         const uint32_t shrink = c->cfg.shrink_at > 0 ? (uint32_t)c->cfg.shrink_at : kShrinkAt;
-        std::vector<uint32_t> want;
-        std::vector<double> wts;
-        const bool by_model = c->sched_mlp && mlp_team_sizes(c, g, kr, G, tmax, want, wts);
-        Sched s = by_model ? make_schedule(g, kr, G, tmax, &want, &wts)
-                           : make_schedule(g, kr, G, tmax);
-        c->timing.rows_scheduler = by_model ? 1 : 0;
-        const uint32_t busy = (uint32_t)std::count_if(s.block_map.begin(), s.block_map.begin() + G,
-                                                      [](uint32_t t) { return t != ~0u; });
-        for (const TeamDesc& td : s.teams)
-            last_teams.emplace_back(td.size, td.pass_begin < td.pass_end ? s.ks[td.pass_begin] : -1);
-        for (const TeamDesc& td : s.teams)
-            for (uint32_t i = td.pass_begin; i < td.pass_end; ++i) {
-                g->h_pass_variant[s.ks[i] + 1] = DCX_VARIANT_BATCHED;
-                g->h_pass_team[s.ks[i] + 1] = (uint16_t)td.size;
-            }
-        const size_t nt = s.teams.size(), nb = G, nk = s.ks.size();
-        const size_t bytes = nt * sizeof(TeamDesc) + 8 * nb + 4 * nk + 64;
+        Sched s;
+        uint32_t busy = 0;
+        if (ksplit > 0) {
+            std::vector<uint32_t> want;
+            std::vector<double> wts;
+            const bool by_model =
+                c->sched_mlp && mlp_team_sizes(c, g, ksplit - 1, G, tmax, want, wts);
+            s = by_model ? make_schedule(g, ksplit - 1, G, tmax, &want, &wts)
+                         : make_schedule(g, ksplit - 1, G, tmax);
+            c->timing.rows_scheduler = by_model ? 1 : 0;
+            busy = (uint32_t)std::count_if(s.block_map.begin(), s.block_map.begin() + G,
+                                           [](uint32_t t) { return t != ~0u; });
+            for (const TeamDesc& td : s.teams)
+                last_teams.emplace_back(td.size,
+                                        td.pass_begin < td.pass_end ? s.ks[td.pass_begin] : -1);
+            for (const TeamDesc& td : s.teams)
+                for (uint32_t i = td.pass_begin; i < td.pass_end; ++i) {
+                    g->h_pass_variant[s.ks[i] + 1] = DCX_VARIANT_BATCHED;
+                    g->h_pass_team[s.ks[i] + 1] = (uint16_t)td.size;
+                }
+        }
+        std::vector<int32_t> lks;
+        for (int32_t k = ksplit; k <= kr; ++k) {
+            lks.push_back(k);
+            g->h_pass_variant[k + 1] = DCX_VARIANT_BATCHED;
+            g->h_pass_team[k + 1] = 0;  // LDS-resident
+        }
+        std::stable_sort(lks.begin(), lks.end(),
+                         [&](int32_t a, int32_t b) { return g->h_ek[a] > g->h_ek[b]; });
+        const uint32_t lcap =
+            lks.empty() ? 0u : (std::max<uint32_t>(g->h_vk[ksplit], 1) + 31) / 32 * 32;
+        const uint32_t lblocks = (uint32_t)std::min<size_t>(lks.size(), (size_t)c->num_cus);
+        // device copies: team table, block map, team passes, LDS passes
+        const size_t nt = s.teams.size(), nb = ksplit > 0 ? G : 0, nk = s.ks.size();
+        const size_t bytes = nt * sizeof(TeamDesc) + 8 * nb + 4 * nk + 4 * lks.size() + 64;
         ensure_bytes(c, &c->sched, &c->sched_bytes, bytes);
         char* base = static_cast<char*>(c->sched);
         TeamDesc* d_teams = reinterpret_cast<TeamDesc*>(base);
         uint32_t* d_bt = reinterpret_cast<uint32_t*>(base + nt * sizeof(TeamDesc));
         int32_t* d_ks = reinterpret_cast<int32_t*>(d_bt + 2 * nb);
+        int32_t* d_lks = d_ks + nk;
         std::vector<char> host(bytes);
         std::memcpy(host.data(), s.teams.data(), nt * sizeof(TeamDesc));
         std::memcpy(host.data() + nt * sizeof(TeamDesc), s.block_map.data(), 8 * nb);
         std::memcpy(host.data() + nt * sizeof(TeamDesc) + 8 * nb, s.ks.data(), 4 * nk);
+        std::memcpy(host.data() + nt * sizeof(TeamDesc) + 8 * nb + 4 * nk, lks.data(),
+                    4 * lks.size());
         HIPX(c, hipMemcpyAsync(c->sched, host.data(), bytes, hipMemcpyHostToDevice, c->stream));
         const size_t pool_words = nt * ((size_t)s.sync_stride + kSlotWords);
-        if (c->pools_words < pool_words) {
+        if (nt && c->pools_words < pool_words) {
             dev_free(c->pools);
             dev_alloc(c, &c->pools, pool_words);
             c->pools_words = pool_words;
         }
-        HIPX(c, hipMemsetAsync(c->pools, 0, 4 * pool_words, c->stream));
-        ensure_scratch(c, 4 * s.scratch_words);
+        if (nt) HIPX(c, hipMemsetAsync(c->pools, 0, 4 * pool_words, c->stream));
+        // the two launches run side by side, each on its own scratch
+        const uint64_t lds_off = (s.scratch_words + 31) / 32 * 32;
+        ensure_scratch(c, 4 * (lds_off + (uint64_t)lblocks * kScratchArrays * lcap));
+        if (!lks.empty()) HIPX(c, hipMemsetAsync(c->ctrs + kCtrQueue, 0, 4, c->stream));
         PassSpec basep = traced(c, core_row_spec(g, 0, g->rows), true);
         basep.fail_rec = c->fail_rec;
 
         HIPX(c, hipEventRecord(c->ev[5], c->stream));
-        HIPX(c, launch_rows_teams(basep, g->vk, g->rows, g->lmax_rows, d_teams, d_bt, d_ks,
-                                  (uint32_t)nb, c->scratch, c->pools,
-                                  c->pools + nt * s.sync_stride, c->timeout_ticks,
-                                  c->ctrs + kCtrErr, shrink, c->stream));
-        launches = 1;
-        slots_used = (int)busy;
-        team_err = c->pools + kSyncError;
-        nteam_err = (uint32_t)nt;
-        team_err_stride = s.sync_stride;
+        if (!lks.empty()) {
+            // LDS rows on the side stream: each of its workgroups needs a
+            // CU's LDS to itself and waits for nothing, so it starts where
+            // team workgroups have finished (or have not started yet) and
+            // never holds up a team for longer than one row
+            HIPX(c, hipEventRecord(c->fork, c->stream));
+            HIPX(c, hipStreamWaitEvent(c->side, c->fork, 0));
+        }
+        if (nt) {
+            HIPX(c, launch_rows_teams(basep, g->vk, g->rows, g->lmax_rows, d_teams, d_bt, d_ks,
+                                      (uint32_t)nb, c->scratch, c->pools,
+                                      c->pools + nt * s.sync_stride, c->timeout_ticks,
+                                      c->ctrs + kCtrErr, shrink, c->stream));
+            ++launches;
+            team_err = c->pools + kSyncError;
+            nteam_err = (uint32_t)nt;
+            team_err_stride = s.sync_stride;
+        }
+        if (!lks.empty()) {
+            HIPX(c, launch_rows_lds(basep, g->vk, g->rows, g->lmax_rows, d_lks,
+                                    (uint32_t)lks.size(), c->ctrs + kCtrQueue, lblocks,
+                                    static_cast<uint32_t*>(c->scratch) + lds_off, lcap,
+                                    c->ctrs + kCtrErr, c->side));
+            ++launches;
+            HIPX(c, hipEventRecord(c->join, c->side));
+            HIPX(c, hipStreamWaitEvent(c->stream, c->join, 0));
+        }
+        slots_used = (int)std::max(busy, lblocks);
     }
     HIPX(c, hipEventRecord(c->ev[6], c->stream));
     try {
@@ -783,12 +833,30 @@ dcx_status dcx_ctx_create(const dcx_config* cfg, dcx_ctx** out) {
             hipSuccess ||
         clock_khz <= 0)
         clock_khz = 100000;  // 100 MHz, the CDNA constant clock
+    // LDS-resident rows: 4 bytes of counters and 1 removed-bit per member
+    // in what the workgroup's LDS leaves beside the kernel's static part
+    int max_lds = 0;
+    if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, c->device) !=
+            hipSuccess)
+        max_lds = 0;
+    const int64_t room = (int64_t)max_lds - (int64_t)rows_lds_static_bytes() - 256;
+    uint32_t cap = room > 0 ? (uint32_t)std::min<int64_t>(kLdsMaxMembers, room * 32 / 132) : 0u;
+    cap = cap / 32 * 32;
+    if (cfg->lds_members < 0)
+        cap = 0;
+    else if (cfg->lds_members > 0)
+        cap = std::min<uint32_t>(cap, (uint32_t)cfg->lds_members);
+    c->lds_cap = cap;
     c->timeout_ticks = (uint64_t)(kBarrierTimeoutS * 1000.0 * clock_khz);
     c->wall_khz = clock_khz;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
         return DCX_ERR_HIP;
     for (auto& e : c->ev)
         if (hipEventCreate(&e) != hipSuccess) return DCX_ERR_HIP;
+    if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->join, hipEventDisableTiming) != hipSuccess)
+        return DCX_ERR_HIP;
     if (hipMalloc(&c->ctrs, kCtrWords * sizeof(uint32_t)) != hipSuccess)
         return DCX_ERR_OUT_OF_MEMORY;
     if (hipMemset(c->ctrs, 0, kCtrWords * sizeof(uint32_t)) != hipSuccess) return DCX_ERR_HIP;
@@ -818,6 +886,9 @@ dcx_status dcx_ctx_destroy(dcx_ctx* c) {
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->side) (void)hipStreamDestroy(c->side);
+    if (c->fork) (void)hipEventDestroy(c->fork);
+    if (c->join) (void)hipEventDestroy(c->join);
     delete c;
     return DCX_OK;
 }

@@ -173,6 +173,17 @@ hipError_t launch_rows_teams(const PassSpec& base, const uint32_t* vk, int32_t* 
                              const int32_t* team_ks, uint32_t blocks, void* scratch,
                              uint32_t* sync_pool, uint32_t* slot_pool, uint64_t timeout_ticks,
                              uint32_t* err, uint32_t shrink_at, hipStream_t st);
+// Rows whose core has at most `cap` (<= kLdsMaxMembers) members, one
+// LDS-resident workgroup each, taken longest first from ks[0..nks) through
+// the counter *queue (zero at launch).  Scratch: blocks * kScratchArrays *
+// cap words.
+constexpr uint32_t kLdsMaxMembers = 32767;
+hipError_t launch_rows_lds(const PassSpec& base, const uint32_t* vk, int32_t* rows,
+                           int32_t* lmax_rows, const int32_t* ks, uint32_t nks, uint32_t* queue,
+                           uint32_t blocks, void* scratch, uint32_t cap, uint32_t* err,
+                           hipStream_t st);
+// Static LDS of the LDS-resident rows kernel (the dynamic part comes on top).
+uint32_t rows_lds_static_bytes();
 hipError_t launch_order_by_kmax(const int32_t* kmaxes, uint32_t V, int32_t kmax,
                                 uint32_t* hist /*[kmax+2]*/, uint32_t* vk /*[kmax+2]*/,
                                 uint32_t* order, hipStream_t st);

@@ -162,7 +162,10 @@ dcx_status dcx_graph_pass_features(dcx_ctx* ctx, const dcx_graph* g, int32_t wit
             return DCX_OK;
         uint32_t m = 0;
         for (int32_t k = 0; k <= g->kmax; ++k) {
-            if (g->h_pass_variant[k + 1] != DCX_VARIANT_BATCHED) continue;
+            // team passes only: the LDS-resident rows (team 0) are not
+            // scheduled by width
+            if (g->h_pass_variant[k + 1] != DCX_VARIANT_BATCHED || g->h_pass_team[k + 1] == 0)
+                continue;
             const uint32_t* w = g->h_pstats.data() + (size_t)(k + 1) * kStatWords;
             const uint64_t ticks = (uint64_t)w[kStatTicksLo] | ((uint64_t)w[kStatTicksHi] << 32);
             const double ms = (double)ticks / ctx->wall_khz;

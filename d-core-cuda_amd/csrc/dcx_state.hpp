@@ -27,7 +27,12 @@ struct dcx_ctx {
     uint32_t* trace = nullptr;   // phase trace of the last grid pass (cfg.trace_phases)
     uint32_t* fail_rec = nullptr;  // [kFailWords] first failing pass member's view
     dcx_mlp* sched_mlp = nullptr;  // cost model of the rows scheduler (nullptr: heuristic)
+    uint32_t lds_cap = 0;  // most members of a row run LDS-resident (0: none)
     hipStream_t stream = nullptr;
+    // second stream (default priority) for the LDS-resident rows, which run
+    // beside the team launch; fork/join order it against `stream`
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     std::mutex mu;
     std::string err;
     dcx_timing timing{};
@@ -57,9 +62,10 @@ struct dcx_ctx {
 namespace {
 // ctrs layout (32-bit words): [0, kSlotWords) phase slots of grid passes;
 // three h-index change flags kSlotStride apart from kCtrFlags; the error
-// word of single-workgroup passes; the grid barrier's words from kCtrSync.
-// Every group starts its own 128-byte line.
-constexpr int kCtrFlags = 96, kCtrErr = 192, kCtrSync = 256, kCtrWords = 1024;
+// word of single-workgroup passes; the row queue of the LDS-resident rows
+// kernel; the grid barrier's words from kCtrSync.  Every group starts its
+// own 128-byte line.
+constexpr int kCtrFlags = 96, kCtrErr = 192, kCtrQueue = 224, kCtrSync = 256, kCtrWords = 1024;
 constexpr uint32_t kMaxGridBlocks = kCtrWords - kCtrSync - kSyncFlags;
 // Workgroups per CU for grid-wide passes: far below the occupancy limit, so
 // residency of the whole grid is never in question.

@@ -45,6 +45,10 @@ namespace {
 constexpr int W = 64;
 constexpr int kBlk = 512;
 constexpr int kUnroll = 4;
+constexpr int kSeedUnroll = 4;
+// edges in flight per thread in the LDS-resident pass (its atomics are
+// cheap, so the list reads' latency is what remains to hide)
+constexpr int kLdsUnroll = 8;
 constexpr int32_t kPoison = INT_MIN / 2;
 
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (W - 1)); }
@@ -382,28 +386,42 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
         if (tw == 0 && tid == 0) slot_reset(slots + kSlotStride * ((s + 1) % 3));
         uint32_t mn = UINT_MAX;
         const bool first = A == nullptr;
-        for (uint32_t base = tw * BLK; base < nA; base += tn * BLK) {
-            const uint32_t i = base + tid;
-            const bool has = i < nA;
-            const uint32_t v = has ? (A ? A[i] : i) : 0u;
-            const bool alive = has && lval[v] == -1;
-            const int32_t d = alive ? S.kd[v] : 0;
-            const bool seed = alive && (d <= L || (check_c && S.cd[v] < k));
-            const bool keep = alive && !seed;
-            uint32_t len = 0;
-            if (seed || (first && keep)) {
-                uint32_t kl, cl;
-                list_lens(P, S, v, kl, cl);
-                len = kl + cl;
+        // kSeedUnroll independent list entries per thread per step
+        for (uint32_t base = tw * BLK * kSeedUnroll; base < nA; base += tn * BLK * kSeedUnroll) {
+            uint32_t v[kSeedUnroll];
+            bool has[kSeedUnroll];
+#pragma unroll
+            for (int u = 0; u < kSeedUnroll; ++u) {
+                const uint32_t i = base + u * BLK + tid;
+                has[u] = i < nA;
+                v[u] = has[u] ? (A ? A[i] : i) : 0u;
             }
-            if (seed) {
-                lval[v] = L;
-                poison(S, v, s);
+            int32_t d[kSeedUnroll];
+            bool alive[kSeedUnroll];
+#pragma unroll
+            for (int u = 0; u < kSeedUnroll; ++u) {
+                alive[u] = has[u] && lval[v[u]] == -1;
+                d[u] = alive[u] ? S.kd[v[u]] : 0;
             }
-            if (keep) mn = min(mn, (uint32_t)d);
-            wave_append_len(seed, v, len, F, &cur[0], slot_u64(cur, 4));
-            wave_append(keep, v, B, &cur[1]);
-            if (first) wave_sum_to(keep ? len : 0u, slot_u64(cur, 6));
+#pragma unroll
+            for (int u = 0; u < kSeedUnroll; ++u) {
+                const bool seed = alive[u] && (d[u] <= L || (check_c && S.cd[v[u]] < k));
+                const bool keep = alive[u] && !seed;
+                uint32_t len = 0;
+                if (seed || (first && keep)) {
+                    uint32_t kl, cl;
+                    list_lens(P, S, v[u], kl, cl);
+                    len = kl + cl;
+                }
+                if (seed) {
+                    lval[v[u]] = L;
+                    poison(S, v[u], s);
+                }
+                if (keep) mn = min(mn, (uint32_t)d[u]);
+                wave_append_len(seed, v[u], len, F, &cur[0], slot_u64(cur, 4));
+                wave_append(keep, v[u], B, &cur[1]);
+                if (first) wave_sum_to(keep ? len : 0u, slot_u64(cur, 6));
+            }
         }
         wave_min_to(mn, &cur[2]);
         if (!barrier()) return;
@@ -622,6 +640,262 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
     if (MULTI && !solo) barrier();
 }
 
+// ---- LDS-resident pass for small cores -------------------------------
+//
+// A row whose (k,0)-core has at most kLdsMaxMembers members runs on one
+// workgroup with both live-degree counters of every member packed into one
+// LDS word (key degree in the low half, constraint degree in the high
+// half) and a removed-bit per member, so its per-edge atomics never leave
+// the CU.  Counts inside a core of n members are at most n - 1 < 2^15, and
+// a counter is decremented once per removed neighbour (decrements to
+// removed members are skipped or, when racing with the removal, still
+// bounded by the live count), so neither half ever borrows or overflows.
+// Levels, frontiers and alive lists stay in the pass scratch (global).
+
+__device__ __forceinline__ bool lds_removed(const uint32_t* rmv, uint32_t x) {
+    return (rmv[x >> 5] >> (x & 31)) & 1u;
+}
+
+template <int BLK>
+__device__ void run_pass_lds(const PassSpec& P, const PassScratch& S, uint32_t* slots,
+                             TileLds<BLK>& T, uint32_t* cnt, uint32_t* rmv, int32_t* lmax_out,
+                             uint32_t* err) {
+    const uint32_t tid = threadIdx.x;
+    const uint32_t nm = P.nmembers;
+    const uint32_t nwords = (nm + 31) / 32;
+    const uint64_t t_start = wall_clock64();
+    for (uint32_t i = tid; i < nm; i += BLK) {
+        const uint32_t rk = lower_bound(P.K.nbr + P.K.off[i], (uint32_t)P.K.deg[i], nm);
+        const uint32_t rc = lower_bound(P.C.nbr + P.C.off[i], (uint32_t)P.C.deg[i], nm);
+        S.rk[i] = rk;
+        S.rc[i] = rc;
+        cnt[i] = rk | (rc << 16);
+        S.lv[i] = -1;
+    }
+    for (uint32_t i = tid; i < nwords; i += BLK) rmv[i] = 0;
+    if (tid < 2) {  // both phase slots: no appends yet, no minimum
+        slots[kSlotStride * tid + 0] = 0;
+        slots[kSlotStride * tid + 1] = 0;
+        slots[kSlotStride * tid + 2] = UINT_MAX;
+    }
+    __syncthreads();
+
+    const uint32_t* A = nullptr;
+    uint32_t *B = S.a0, *Bn = S.a1, *F = S.f0, *G = S.f1;
+    uint32_t nA = nm, removed = 0, s = 0, levels = 0;
+    int32_t L = 0;
+    const int32_t k = P.k;
+    bool failed = false;
+    uint64_t t_prev = t_start;
+    auto trace = [&](bool seed, uint32_t nfr) {
+        if (!P.trace || tid != 0) return;
+        const uint64_t now = wall_clock64();
+        if (s < P.trace_cap) {
+            uint32_t* e = P.trace + kTraceWords * (1 + (size_t)s);
+            e[kTraceLevel] = ((uint32_t)L & kTraceLevelMask) | (seed ? kTraceSeedBit : 0u);
+            e[kTraceFrontier] = nfr;
+            e[kTraceAlive] = nA;
+            e[kTraceTicks] = (uint32_t)min<uint64_t>(now - t_prev, 0xffffffffull);
+        }
+        P.trace[0] = min(s + 1, P.trace_cap);
+        t_prev = now;
+    };
+    while (removed < nm) {
+        if (++levels > 2 * nm + 2) {
+            failed = true;
+            break;
+        }
+        uint32_t* cur = slots + kSlotStride * (s & 1);
+        uint32_t* other = slots + kSlotStride * ((s + 1) & 1);
+        if (tid == 0) {
+            other[0] = 0;
+            other[1] = 0;
+            other[2] = UINT_MAX;
+        }
+        uint32_t mn = UINT_MAX;
+        for (uint32_t base = 0; base < nA; base += BLK * kSeedUnroll) {
+            uint32_t v[kSeedUnroll];
+            bool has[kSeedUnroll];
+#pragma unroll
+            for (int u = 0; u < kSeedUnroll; ++u) {
+                const uint32_t i = base + u * BLK + tid;
+                has[u] = i < nA;
+                v[u] = has[u] ? (A ? A[i] : i) : 0u;
+            }
+#pragma unroll
+            for (int u = 0; u < kSeedUnroll; ++u) {
+                const bool alive = has[u] && !lds_removed(rmv, v[u]);
+                const int32_t d = alive ? (int32_t)(cnt[v[u]] & 0xffffu) : 0;
+                const bool seed = alive && d <= L;
+                const bool keep = alive && !seed;
+                if (seed) {
+                    atomicOr(&rmv[v[u] >> 5], 1u << (v[u] & 31));
+                    S.lv[v[u]] = L;
+                }
+                if (keep) mn = min(mn, (uint32_t)d);
+                wave_append(seed, v[u], F, &cur[0]);
+                wave_append(keep, v[u], B, &cur[1]);
+            }
+        }
+        wave_min_to(mn, &cur[2]);
+        __syncthreads();
+        uint32_t nF = cur[0];
+        nA = cur[1];
+        uint32_t mnall = cur[2];
+        __syncthreads();
+        trace(true, nF);
+        A = B;
+        {
+            uint32_t* t = B;
+            B = Bn;
+            Bn = t;
+        }
+        ++s;
+        removed += nF;
+
+        while (nF > 0) {
+            uint32_t* nxt = slots + kSlotStride * (s & 1);
+            uint32_t* nx2 = slots + kSlotStride * ((s + 1) & 1);
+            if (tid == 0) {
+                nx2[0] = 0;
+                nx2[1] = 0;
+                nx2[2] = UINT_MAX;
+            }
+            uint32_t mn2 = UINT_MAX;
+            for (uint32_t base = 0; base < nF; base += BLK) {
+                const uint32_t i = base + tid;
+                const bool has = i < nF;
+                const uint32_t v = has ? F[i] : 0u;
+                __syncthreads();
+                uint32_t kl = 0, cl = 0;
+                if (has) {
+                    T.koff[tid] = P.K.off[v];
+                    T.coff[tid] = P.C.off[v];
+                    kl = S.rk[v];
+                    cl = S.rc[v];
+                }
+                T.klen[tid] = kl;
+                if (tid == 0) T.nitems = min(nF - base, (uint32_t)BLK);
+                T.scan[tid] = block_excl_scan<BLK>(kl + cl, T.wsum, &T.total);
+                __syncthreads();
+                const uint32_t total = T.total;
+                for (uint32_t eb = 0; eb < total; eb += kLdsUnroll * BLK) {
+                    uint32_t x[kLdsUnroll];
+                    bool ks[kLdsUnroll], ok[kLdsUnroll], claim[kLdsUnroll];
+#pragma unroll
+                    for (int u = 0; u < kLdsUnroll; ++u) {
+                        const uint32_t e = eb + u * BLK + tid;
+                        ok[u] = e < total;
+                        x[u] = 0;
+                        ks[u] = false;
+                        if (ok[u]) {
+                            const uint32_t it = tile_find<BLK>(T, e);
+                            const uint32_t t = e - T.scan[it];
+                            ks[u] = t < T.klen[it];
+                            x[u] = ks[u] ? P.K.nbr[T.koff[it] + t]
+                                         : P.C.nbr[T.coff[it] + (t - T.klen[it])];
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < kLdsUnroll; ++u) {
+                        claim[u] = false;
+                        if (ok[u] && !lds_removed(rmv, x[u])) {
+                            if (ks[u]) {
+                                // x loses a C-neighbour: constraint half
+                                const uint32_t old = atomicSub(&cnt[x[u]], 0x10000u);
+                                claim[u] = k > 0 && (int32_t)(old >> 16) == k;
+                            } else {
+                                const int32_t old = (int32_t)(atomicSub(&cnt[x[u]], 1u) & 0xffffu);
+                                if (old == L + 1)
+                                    claim[u] = true;
+                                else if (old - 1 > L)
+                                    mn2 = min(mn2, (uint32_t)(old - 1));
+                            }
+                            if (claim[u]) {
+                                const uint32_t bit = 1u << (x[u] & 31);
+                                claim[u] = !(atomicOr(&rmv[x[u] >> 5], bit) & bit);
+                                if (claim[u]) S.lv[x[u]] = L;
+                            }
+                        }
+                    }
+#pragma unroll
+                    for (int u = 0; u < kLdsUnroll; ++u) wave_append(claim[u], x[u], G, &nxt[0]);
+                }
+            }
+            wave_min_to(mn2, &nxt[2]);
+            __syncthreads();
+            nF = nxt[0];
+            mnall = min(mnall, nxt[2]);
+            __syncthreads();
+            trace(false, nF);
+            {
+                uint32_t* t = F;
+                F = G;
+                G = t;
+            }
+            ++s;
+            removed += nF;
+        }
+        if (removed < nm) {
+            if (nA == 0 || mnall == UINT_MAX) {
+                failed = true;
+                break;
+            }
+            L = max(L + 1, (int32_t)min(mnall, (uint32_t)INT_MAX));
+        }
+    }
+    if (tid == 0) {
+        if (lmax_out) *lmax_out = nm ? L : 0;
+        if (failed) consume(atomicMax(err, (uint32_t)kErrInconsistent));
+        if (P.stats) {
+            const uint64_t dt = wall_clock64() - t_start;
+            P.stats[kStatPhases] = s;
+            P.stats[kStatLevels] = levels;
+            P.stats[kStatTicksLo] = (uint32_t)dt;
+            P.stats[kStatTicksHi] = (uint32_t)(dt >> 32);
+            P.stats[kStatFailed] = failed ? 1u : 0u;
+            P.stats[kStatRemoved] = removed;
+            P.stats[kStatAlive] = nA;
+            P.stats[kStatLevel] = (uint32_t)L;
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = tid; i < nm; i += BLK) P.out_row[P.order[i]] = S.lv[i];
+}
+
+// Small rows: each workgroup takes the next row from a queue (longest
+// first) and runs it LDS-resident.  Dynamic LDS: cap counter words, then
+// cap / 32 removed-bit words.
+constexpr int kLdsBlk = 1024;
+__global__ __launch_bounds__(kLdsBlk) void k_rows_lds(PassSpec base, const uint32_t* vk,
+                                                      int32_t* rows, int32_t* lmax_rows,
+                                                      const int32_t* ks, uint32_t nks,
+                                                      uint32_t* queue, void* scratch,
+                                                      uint32_t cap, uint32_t* err) {
+    extern __shared__ uint32_t dyn[];
+    __shared__ TileLds<kLdsBlk> T;
+    __shared__ uint32_t lslots[2 * kSlotStride];
+    __shared__ uint32_t s_next;
+    const PassScratch S = scratch_at(
+        static_cast<uint32_t*>(scratch) + (size_t)blockIdx.x * kScratchArrays * cap, cap);
+    for (;;) {
+        if (threadIdx.x == 0) s_next = atomicAdd(queue, 1u);
+        __syncthreads();
+        const uint32_t idx = s_next;
+        __syncthreads();
+        if (idx >= nks) break;
+        const int32_t k = ks[idx];
+        PassSpec P = base;
+        P.k = k;
+        P.nmembers = vk[k];
+        P.out_row = rows + (size_t)k * base.V;
+        P.stats = base.stats ? base.stats + (size_t)kStatWords * k : nullptr;
+        if (k != base.trace_k) P.trace = nullptr;
+        run_pass_lds<kLdsBlk>(P, S, lslots, T, dyn, dyn + cap, lmax_rows + k, err);
+        __syncthreads();
+    }
+}
+
 __global__ __launch_bounds__(kBlk) void k_pass_grid(PassSpec P, PassScratch S, uint32_t* slots,
                                                     int32_t* lmax_out, GridSync gs,
                                                     uint32_t shrink_at) {
@@ -812,6 +1086,24 @@ hipError_t launch_rows_teams(const PassSpec& base, const uint32_t* vk, int32_t* 
     hipLaunchKernelGGL(k_rows_teams, dim3(blocks), dim3(kBlk), 0, st, base, vk, rows, lmax_rows,
                        teams, block_team, team_ks, scratch, sync_pool, slot_pool, timeout_ticks,
                        err, shrink_at);
+    return hipGetLastError();
+}
+
+uint32_t rows_lds_static_bytes() {
+    hipFuncAttributes a{};
+    if (hipFuncGetAttributes(&a, reinterpret_cast<const void*>(k_rows_lds)) != hipSuccess)
+        return 0;
+    return (uint32_t)a.sharedSizeBytes;
+}
+
+hipError_t launch_rows_lds(const PassSpec& base, const uint32_t* vk, int32_t* rows,
+                           int32_t* lmax_rows, const int32_t* ks, uint32_t nks, uint32_t* queue,
+                           uint32_t blocks, void* scratch, uint32_t cap, uint32_t* err,
+                           hipStream_t st) {
+    if (cap > kLdsMaxMembers) return hipErrorInvalidValue;
+    const size_t dyn = 4ull * (cap + (cap + 31) / 32);
+    hipLaunchKernelGGL(k_rows_lds, dim3(blocks), dim3(kLdsBlk), dyn, st, base, vk, rows,
+                       lmax_rows, ks, nks, queue, scratch, cap, err);
     return hipGetLastError();
 }
 

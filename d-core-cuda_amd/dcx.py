@@ -32,7 +32,7 @@ class Config(ctypes.Structure):
                 ("rows_slots", ctypes.c_int32), ("team_max", ctypes.c_int32),
                 ("shrink_at", ctypes.c_int32), ("trace_phases", ctypes.c_int32),
                 ("scratch_budget", ctypes.c_uint64), ("pull_ratio", ctypes.c_int32),
-                ("reserved32", ctypes.c_int32), ("reserved", ctypes.c_uint64 * 3)]
+                ("lds_members", ctypes.c_int32), ("reserved", ctypes.c_uint64 * 3)]
 
 
 class Timing(ctypes.Structure):
@@ -275,7 +275,7 @@ class Context:
     """A dcx_ctx on one device (library-owned stream and scratch)."""
 
     def __init__(self, device=0, variant=VARIANT_AUTO, rows_slots=0, scratch_budget=0,
-                 team_max=0, shrink_at=0, trace_phases=0, pull_ratio=0):
+                 team_max=0, shrink_at=0, trace_phases=0, pull_ratio=0, lds_members=0):
         cfg = Config()
         check(lib().dcx_config_init(ctypes.byref(cfg)), "dcx_config_init")
         cfg.device = device
@@ -286,6 +286,7 @@ class Context:
         cfg.trace_phases = trace_phases
         cfg.scratch_budget = scratch_budget
         cfg.pull_ratio = pull_ratio
+        cfg.lds_members = lds_members
         h = ctypes.c_void_p()
         check(lib().dcx_ctx_create(ctypes.byref(cfg), ctypes.byref(h)), "dcx_ctx_create")
         self.h = h

@@ -156,7 +156,10 @@ typedef struct dcx_config {
      * the frontier's edges: 0 = library default (4), -1 = always push.
      * Results are the same either way. */
     int32_t pull_ratio;
-    int32_t reserved32;
+    /* Batched rows whose (k,0)-core has at most lds_members members run on
+     * one workgroup with their degree counters in LDS: 0 = as many as the
+     * device's LDS holds (at most 32767), -1 = none. */
+    int32_t lds_members;
     uint64_t reserved[3];
 } dcx_config;
 

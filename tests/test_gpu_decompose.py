@@ -178,6 +178,32 @@ def test_push_and_pull_phases(ratio):
         assert (tr["seed"] == 2).any() and not (tr["seed"] == 0).any()
 
 
+@pytest.mark.parametrize("lds_members", [-1, 64, 2000, 0])
+def test_lds_resident_rows(lds_members):
+    """Rows run LDS-resident (counters packed in LDS, one workgroup per row)
+    up to lds_members core members and on teams above: every split gives
+    the oracle's bits, and the pass log marks which rows ran where."""
+    for scale, seed in ((14, 5), (16, 2)):
+        src, dst = dcx.gen_rmat(scale, 16 << scale, seed)
+        V = 1 << scale
+        ref = oracle.decompose(oracle.csr_from_edges(V, src, dst))
+        say(f"rmat{scale} lds_members={lds_members}")
+        with dcx.Context(0, lds_members=lds_members) as c:
+            g = c.upload_edges(V, src, dst)
+            K, lmax = g.decompose(dcx.VARIANT_BATCHED)
+            assert K == ref[0] and lmax == ref[2]
+            assert np.array_equal(g.kmaxes(), ref[1]) and np.array_equal(g.rows(), ref[3])
+            log = g.pass_log()
+            g.free()
+        vk = np.bincount(ref[1], minlength=K + 1)[::-1].cumsum()[::-1]
+        for k in range(K + 1):
+            in_lds = log[k + 1]["team"] == 0
+            if lds_members == -1:
+                assert not in_lds
+            elif lds_members > 0:
+                assert in_lds == (vk[k] <= lds_members), (k, vk[k])
+
+
 def test_team_barrier_visibility_repeated():
     """Multi-workgroup teams hand frontiers, alive lists and levels to each
     other through the team barrier; any write the barrier fails to publish
@@ -240,7 +266,9 @@ def test_pass_log(ctx, variant):
     m = np.minimum(km[u], km[c.out_nbr])
     for k in range(K + 1):
         e = log[k + 1]
-        assert e["k"] == k and e["variant"] == variant and e["team"] >= 1
+        # team 0: a batched row that ran LDS-resident on one workgroup
+        assert e["k"] == k and e["variant"] == variant
+        assert e["team"] >= (0 if variant == dcx.VARIANT_BATCHED else 1)
         assert e["members"] == int((km >= k).sum())
         assert e["edges"] == int((m >= k).sum())
         assert e["phases"] >= 1 and e["ms"] >= 0.0
