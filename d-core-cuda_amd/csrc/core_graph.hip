@@ -70,9 +70,18 @@ __global__ void k_audit(uint32_t V, const uint32_t* off, const int32_t* deg, con
         sum += __shfl_xor(sum, dd, W);
         bad += __shfl_xor(bad, dd, W);
     }
+    // one pair of atomics per block (same-address atomics serialise)
+    __shared__ unsigned long long s_sum, s_bad;
+    if (threadIdx.x == 0) s_sum = s_bad = 0;
+    __syncthreads();
     if (lane_id() == 0) {
-        atomicAdd(out, (unsigned long long)sum);
-        atomicAdd(out + 1, (unsigned long long)bad);
+        atomicAdd(&s_sum, (unsigned long long)sum);
+        atomicAdd(&s_bad, (unsigned long long)bad);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        atomicAdd(out, s_sum);
+        atomicAdd(out + 1, s_bad);
     }
 }
 
@@ -98,8 +107,8 @@ size_t core_graph_tmp_bytes(uint32_t V, uint64_t E) {
 
 hipError_t launch_audit(uint32_t V, Csr side, unsigned long long* out, hipStream_t st) {
     if (V == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_audit, dim3(grid_for(V, 4)), dim3(256), 0, st, V, side.off, side.deg,
-                       side.nbr, out);
+    hipLaunchKernelGGL(k_audit, dim3(grid_for(V, 16, 2048)), dim3(1024), 0, st, V, side.off,
+                       side.deg, side.nbr, out);
     return hipGetLastError();
 }
 

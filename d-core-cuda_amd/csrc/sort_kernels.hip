@@ -196,7 +196,11 @@ hipError_t launch_excl_scan_u32(const uint32_t* in, uint32_t* out, uint64_t n, u
     return hipGetLastError();
 }
 
-size_t radix_tmp_words(uint64_t n) { return (size_t)kDigits * tiles_for(n) + 1; }
+// digit table (kDigits * tiles) + 1, then the tile sums of its scan
+size_t radix_tmp_words(uint64_t n) {
+    const uint64_t m = (uint64_t)kDigits * tiles_for(n);
+    return (size_t)m + 1 + scan_tmp_words(m) + 1;
+}
 
 hipError_t launch_radix_sort_u64(uint64_t* keys, uint64_t* alt, uint64_t n, int bits,
                                  uint32_t* table, uint64_t** result, hipStream_t st) {
@@ -205,10 +209,13 @@ hipError_t launch_radix_sort_u64(uint64_t* keys, uint64_t* alt, uint64_t n, int 
     const uint32_t nt = tiles_for(n);
     uint64_t* a = keys;
     uint64_t* b = alt;
+    const uint64_t m = (uint64_t)kDigits * nt;
+    uint32_t* sums = table + m + 1;
     for (int shift = 0; shift < bits; shift += 4) {
         hipLaunchKernelGGL(k_radix_hist, dim3(nt), dim3(kSortBlk), 0, st, a, n, shift, nt, table);
-        hipLaunchKernelGGL(k_radix_scan, dim3(1), dim3(kSortBlk), 0, st, table,
-                           (uint64_t)kDigits * nt);
+        // the digit table scanned by tiles (one block per tile, in place)
+        hipError_t e = launch_excl_scan_u32(table, table, m, sums, st);
+        if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_radix_scatter, dim3(nt), dim3(kSortBlk), 0, st, a, b, n, shift, nt,
                            table);
         uint64_t* t = a;
