@@ -73,7 +73,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="rmat", choices=["rmat", "chunglu256", "serve"])
+    ap.add_argument("--workload", default="rmat",
+                    choices=["rmat", "chunglu256", "serve", "learned"])
+    ap.add_argument("--learned-scale", type=int, default=24,
+                    help="learned: log2 of the vertex count of the two graphs")
+    ap.add_argument("--no-plain-learned", action="store_true",
+                    help="learned: skip the plain one-k-per-launch baseline")
     ap.add_argument("--serve-scale", type=int, default=22)
     ap.add_argument("--serve-batches", type=int, default=10000)
     ap.add_argument("--serve-loads", default="0.5,0.9",
@@ -102,6 +107,8 @@ def main():
         return run_chunglu(args)
     if args.workload == "serve":
         return run_serve(args)
+    if args.workload == "learned":
+        return run_learned(args)
     return run_rmat(args)
 
 
@@ -476,6 +483,138 @@ def run_serve(args):
         "capacity_edits_s": round(edits / t_busy, 1),
         "loads": loads,
         "checks": checks,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
+
+def learned_training_set(dcx, log):
+    """Per-pass features and measured times the decomposition logs, from
+    R-MAT 20 and Chung-Lu 2^20 decomposed at several team caps (so that the
+    samples cover the widths the scheduler chooses between)."""
+    Xs, Ts = [], []
+    small = [("rmat20", 1 << 20) + tuple(dcx.gen_rmat(20, 16 << 20, 11)),
+             ("chunglu20", 1 << 20) + tuple(dcx.gen_chunglu(1 << 20, 1 << 24, 12))]
+    for name, V, s, d in small:
+        for tm in (2, 4, 8, 16, 32, 64):
+            with dcx.Context(0, team_max=tm) as c:
+                g = c.upload_edges(V, s, d)
+                for _ in range(2):
+                    g.decompose(dcx.VARIANT_BATCHED)
+                for wl in (True, False):
+                    X, T = g.pass_features(with_log=wl)
+                    Xs.append(X)
+                    Ts.append(T)
+                g.free()
+        log(f"training data: {name} done, {sum(len(t) for t in Ts)} samples")
+    return Xs, Ts
+
+
+def run_learned(args):
+    """R-MAT scale 24 and a Chung-Lu digraph of the same size, each decomposed
+    with the work-share team widths, with team widths from the GPU-trained
+    cost model, and with the plain one-k-per-launch peel; bit-exactness
+    between the three is checked by the device state hash."""
+    import dcx
+
+    def log(msg):
+        print(f"# {msg}", file=sys.stderr, flush=True)
+
+    sc = args.learned_scale
+    V = 1 << sc
+    t0 = time.perf_counter()
+    Xs, Ts = learned_training_set(dcx, log)
+    ctx = dcx.Context(0)
+    model = dcx.MLP(ctx, seed=7)
+    results = []
+    for name, gen in ((f"rmat{sc}", lambda: dcx.gen_rmat(sc, 16 << sc, 4)),
+                      (f"chunglu{sc}", lambda: dcx.gen_chunglu(V, 16 << sc, 5))):
+        t1 = time.perf_counter()
+        s, d = gen()
+        E = int(s.size)
+        g = ctx.upload_edges(V, s, d)
+        del s, d
+        log(f"{name}: V={V} E={E} generated and uploaded in {time.perf_counter() - t1:.1f}s")
+        # work-share widths (the fixed heuristic); its logs join the training set
+        ctx.set_scheduler(None)
+        heur = []
+        for _ in range(3):
+            K, lmax = g.decompose(dcx.VARIANT_BATCHED)
+            t = ctx.timing()
+            heur.append((t.total_ms, t.rows_kernel_ms, t.kmax_ms, t.order_ms))
+        h_hash = g.state_hash()
+        log(f"{name}: kmax={K} lmax={lmax} heuristic {[round(x[0], 1) for x in heur]} ms")
+        for wl in (True, False):
+            X, T = g.pass_features(with_log=wl)
+            Xs.append(X)
+            Ts.append(T)
+        X = np.concatenate(Xs)
+        T = np.concatenate(Ts)
+        t1 = time.perf_counter()
+        loss = model.train(X, T, epochs=2000, lr=3e-3)
+        corr = float(np.corrcoef(model.predict(X), T)[0, 1])
+        log(f"{name}: model trained on {len(T)} samples in {time.perf_counter() - t1:.1f}s, "
+            f"loss {loss:.4f}, fit corr {corr:.3f}")
+        ctx.set_scheduler(model)
+        learned = []
+        for _ in range(3):
+            g.decompose(dcx.VARIANT_BATCHED)
+            t = ctx.timing()
+            learned.append((t.total_ms, t.rows_kernel_ms, t.kmax_ms, t.order_ms,
+                            t.rows_scheduler))
+        m_hash = g.state_hash()
+        ctx.set_scheduler(None)
+        log(f"{name}: learned {[round(x[0], 1) for x in learned]} ms, "
+            f"hash equal {m_hash == h_hash}")
+        plain = None
+        if not args.no_plain_learned:
+            g.decompose(dcx.VARIANT_PEEL)
+            tp = ctx.timing()
+            plain = {"total_ms": round(tp.total_ms, 2), "rows_ms": round(tp.rows_ms, 2),
+                     "launches": tp.rows_launches,
+                     "output_equal": g.state_hash() == h_hash}
+            log(f"{name}: plain {tp.total_ms:.1f} ms, equal {plain['output_equal']}")
+        hb = min(heur)
+        lb = min(learned)
+        results.append({
+            "graph": name, "V": V, "E": E, "kmax": K, "lmax": lmax,
+            "heuristic_ms": round(hb[0], 2), "heuristic_rows_kernel_ms": round(hb[1], 2),
+            "learned_ms": round(lb[0], 2), "learned_rows_kernel_ms": round(lb[1], 2),
+            "learned_scheduler_used": bool(lb[4]),
+            "kmax_pass_ms": round(lb[2], 2), "order_ms": round(lb[3], 2),
+            "learned_speedup_vs_heuristic": round(hb[0] / lb[0], 3),
+            "plain": plain,
+            "learned_speedup_vs_plain": round(plain["total_ms"] / lb[0], 2) if plain else None,
+            "state_hash": f"{h_hash:016x}",
+            "bit_exact_learned_vs_heuristic": m_hash == h_hash,
+            "model_fit_corr": round(corr, 3),
+        })
+        g.free()
+    model.free()
+    ctx.close()
+    E_tot = sum(r["E"] for r in results)
+    T_tot = sum(r["learned_ms"] for r in results)
+    line = {
+        "metric": "learned-scheduled D-core decomposition throughput over R-MAT and Chung-Lu "
+                  f"2^{sc} (GEdges/s = sum E / sum T)",
+        "value": round(E_tot / (T_tot * 1e-3) / 1e9, 6),
+        "unit": "GEdges/s",
+        "n_gpus": 1,
+        "steps": 3,
+        "warmup": 0,
+        "ms_per_step": round(T_tot, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": f"learned: R-MAT {sc} (16 * 2^{sc} samples, seed 4) and Chung-Lu "
+                               f"V = 2^{sc}, 16 * 2^{sc} samples, gamma 2.1, seed 5; MLP 8-32-32-1 "
+                               "trained on the GPU from logged passes",
+                   "graphs": [r["graph"] for r in results]},
+        "graphs": results,
+        "wall_s": round(time.perf_counter() - t0, 1),
         "cpu_baseline": None,
     }
     print(json.dumps(line), flush=True)
