@@ -4,15 +4,14 @@
 // flag word (no two workgroups write the same word, so arrivals do not
 // serialise on one atomic counter); the threads of rank 0 poll all flags in
 // parallel, and once every flag has reached e rank 0 stores e into the
-// release word that the other workgroups poll.  Every wavefront issues its
-// own agent-scope release fence (an L2 write-back on CDNA) before the
-// workgroup barrier, so every global write made before the team barrier
-// is visible to every workgroup after it (across XCDs); thread 0's
-// agent-scope acquire fence followed by the workgroup barrier covers the
-// way out.  (A single fence by thread 0 after the workgroup barrier is
-// cheaper but lost writes on gfx950 under load, so each wavefront fences
-// its own.)  The fences dominate the cost, which grows with the team size,
-// so teams are kept as small as their work allows.  Waiting is bounded by
+// release word that the other workgroups poll.  Every wavefront waits
+// until its stores are performed, then thread 0 issues the workgroup's one
+// agent-scope release fence (an L2 write-back on CDNA), so every global
+// write made before the team barrier is visible to every workgroup after
+// it (across XCDs); thread 0's agent-scope acquire fence followed by the
+// workgroup barrier covers the way out.  The fences dominate the cost, so
+// there is one pair per workgroup and barrier, and teams are kept as small
+// as their work allows.  Waiting is bounded by
 // wall clock: a workgroup that waits longer than `timeout_ticks` raises the
 // abort word, and every workgroup that sees the abort word leaves its
 // barrier with `false`, so no wavefront can wait forever even if the team
@@ -77,47 +76,58 @@ __device__ inline bool grid_barrier(GridSync& g) {
     __shared__ uint32_t s_ok;
     const uint32_t e = ++g.epoch;
     if (threadIdx.x == 0) s_ok = 1;
-    // every wavefront publishes its own stores before the flag goes up
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    // Every wavefront waits for its own stores to be performed (they are in
+    // the L2 once counted), then thread 0 alone writes the L2 back: one
+    // write-back per workgroup and barrier.  (A write-back per wavefront
+    // costs the same for each and, with hundreds of workgroups syncing,
+    // kept every XCD's L2 busy flushing.)
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    if (threadIdx.x == 0) coherent_write(g.w + kSyncFlags + g.rank, e);
+    if (threadIdx.x == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __builtin_amdgcn_s_waitcnt(0);
+        coherent_write(g.w + kSyncFlags + g.rank, e);
+    }
     if (g.rank == 0) {
         // every thread of rank 0 watches a strided subset of the flags
         const uint64_t t0 = wall_clock64();
         bool ok = true;
         for (uint32_t i = threadIdx.x; i < g.nblocks && ok; i += blockDim.x) {
-            while (coherent_read(g.w + kSyncFlags + i) < e) {
-                if (sync_aborted(g)) {
-                    ok = false;
-                    break;
-                }
-                if (wall_clock64() - t0 > g.timeout_ticks) {
-                    sync_fail(g, kErrTimeout);
-                    ok = false;
-                    break;
+            for (uint32_t spin = 1; coherent_read(g.w + kSyncFlags + i) < e; ++spin) {
+                // the abort word and the clock are checked every few polls:
+                // each check is another round trip on the waiting path
+                if ((spin & 15) == 0) {
+                    if (sync_aborted(g)) {
+                        ok = false;
+                        break;
+                    }
+                    if (wall_clock64() - t0 > g.timeout_ticks) {
+                        sync_fail(g, kErrTimeout);
+                        ok = false;
+                        break;
+                    }
                 }
                 __builtin_amdgcn_s_sleep(1);
             }
         }
         if (!ok) s_ok = 0;
         __syncthreads();
-        if (threadIdx.x == 0 && s_ok) {
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-            __builtin_amdgcn_s_waitcnt(0);
-            coherent_write(g.w + kSyncRelease, e);
-        }
+        // rank 0 released its own writes before raising its flag; the
+        // release word carries no data of its own
+        if (threadIdx.x == 0 && s_ok) coherent_write(g.w + kSyncRelease, e);
     } else if (threadIdx.x == 0) {
         const uint64_t t0 = wall_clock64();
-        while (coherent_read(g.w + kSyncRelease) < e) {
-            if (sync_aborted(g)) {
-                s_ok = 0;
-                break;
-            }
-            if (wall_clock64() - t0 > g.timeout_ticks) {
-                sync_fail(g, kErrTimeout);
-                s_ok = 0;
-                break;
+        for (uint32_t spin = 1; coherent_read(g.w + kSyncRelease) < e; ++spin) {
+            if ((spin & 15) == 0) {
+                if (sync_aborted(g)) {
+                    s_ok = 0;
+                    break;
+                }
+                if (wall_clock64() - t0 > g.timeout_ticks) {
+                    sync_fail(g, kErrTimeout);
+                    s_ok = 0;
+                    break;
+                }
             }
             __builtin_amdgcn_s_sleep(1);
         }
@@ -125,8 +135,10 @@ __device__ inline bool grid_barrier(GridSync& g) {
     if (threadIdx.x == 0) {
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         __builtin_amdgcn_s_waitcnt(0);  // invalidation done before anyone reads
-        if (sync_aborted(g)) s_ok = 0;
     }
+    // A member that failed raised the abort word before arriving here; the
+    // others learn it at their next wait (every wait polls the abort word),
+    // so the barrier itself adds no round trip to check it.
     __syncthreads();
     return s_ok != 0;
 }

@@ -46,6 +46,8 @@ constexpr int W = 64;
 constexpr int kBlk = 512;
 constexpr int kUnroll = 4;
 constexpr int kSeedUnroll = 4;
+// core-layout ranks whose decrements a workgroup combines in LDS (per side)
+constexpr uint32_t kCombRanks = 4096;
 // edges in flight per thread in the LDS-resident pass (its atomics are
 // cheap, so the list reads' latency is what remains to hide)
 constexpr int kLdsUnroll = 8;
@@ -181,14 +183,10 @@ __device__ __forceinline__ uint32_t lower_bound(const uint32_t* a, uint32_t n, u
 // [2] smallest key degree seen above the level, [3] the leader's phase,
 // [4..5] list edges of the new frontier, [6..7] list edges of the alive
 // vertices (first level only).
-__device__ __forceinline__ void slot_reset(uint32_t* s) {
-    coherent_write(s + 0, 0u);
-    coherent_write(s + 1, 0u);
-    coherent_write(s + 2, (uint32_t)UINT_MAX);
-    coherent_write(s + 4, 0u);
-    coherent_write(s + 5, 0u);
-    coherent_write(s + 6, 0u);
-    coherent_write(s + 7, 0u);
+// Called by threads 0..7 of one wavefront: each resets one word, so the
+// resets are one round trip instead of a chain of seven.
+__device__ __forceinline__ void slot_reset(uint32_t* s, uint32_t lane) {
+    if (lane < 8 && lane != 3) coherent_write(s + lane, lane == 2 ? (uint32_t)UINT_MAX : 0u);
 }
 
 __device__ __forceinline__ unsigned long long* slot_u64(uint32_t* s, int w) {
@@ -203,14 +201,10 @@ struct SlotView {
 // One coherent read of a slot per workgroup, shared through LDS; every
 // thread must call it (contains workgroup barriers).
 __device__ __forceinline__ SlotView slot_read(uint32_t* s, uint32_t* lds) {
-    if (threadIdx.x == 0) {
-        lds[0] = coherent_read(s + 0);
-        lds[1] = coherent_read(s + 1);
-        lds[2] = coherent_read(s + 2);
-        lds[3] = coherent_read(s + 4);
-        lds[4] = coherent_read(s + 5);
-        lds[5] = coherent_read(s + 6);
-        lds[6] = coherent_read(s + 7);
+    // seven lanes read one word each (one round trip)
+    if (threadIdx.x < 7) {
+        const uint32_t w = threadIdx.x < 3 ? threadIdx.x : threadIdx.x + 1;
+        lds[threadIdx.x] = coherent_read(s + w);
     }
     __syncthreads();
     SlotView v;
@@ -255,7 +249,7 @@ __device__ __forceinline__ void list_lens(const PassSpec& P, const PassScratch& 
 template <int BLK, bool MULTI>
 __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slots,
                          TileLds<BLK>& T, int32_t* lmax_out, GridSync* gs, uint32_t* err,
-                         uint32_t shrink_at) {
+                         uint32_t shrink_at, uint32_t* comb) {
     const uint32_t tid = threadIdx.x;
     uint32_t tw = MULTI ? gs->rank : 0u;
     uint32_t tn = MULTI ? gs->nblocks : 1u;
@@ -319,10 +313,10 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
     };
 
     __shared__ uint32_t s_rd[8];
-    if (tw == 0 && tid == 0) {
-        slot_reset(slots);
-        slot_reset(slots + kSlotStride);
-        slot_reset(slots + 2 * kSlotStride);
+    if (tw == 0) {
+        slot_reset(slots, tid);
+        slot_reset(slots + kSlotStride, tid);
+        slot_reset(slots + 2 * kSlotStride, tid);
     }
 
     // ---- set-up: live degrees (member prefixes in the core layout)
@@ -383,7 +377,7 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
         }
         // ---- seed level L from the alive list; survivors go to B
         uint32_t* cur = slots + kSlotStride * (s % 3);
-        if (tw == 0 && tid == 0) slot_reset(slots + kSlotStride * ((s + 1) % 3));
+        if (tw == 0) slot_reset(slots + kSlotStride * ((s + 1) % 3), tid);
         uint32_t mn = UINT_MAX;
         const bool first = A == nullptr;
         // kSeedUnroll independent list entries per thread per step
@@ -445,7 +439,7 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
         // ---- cascade inside level L until no vertex crosses a threshold
         while (nF > 0) {
             uint32_t* nxt = slots + kSlotStride * (s % 3);
-            if (tw == 0 && tid == 0) slot_reset(slots + kSlotStride * ((s + 1) % 3));
+            if (tw == 0) slot_reset(slots + kSlotStride * ((s + 1) % 3), tid);
             uint32_t mn2 = UINT_MAX;
             const bool pull = P.pull_ratio != 0 && pa <= (uint64_t)P.pull_ratio * fe;
             if (pull) {
@@ -553,6 +547,18 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
                         // then finds a poisoned counter.
 #pragma unroll
                         for (int u = 0; u < kUnroll; ++u) ok[u] = ok[u] && lval[x[u]] == -1;
+                        if (comb) {
+                            // the first kCombRanks ranks are the densest
+                            // core's vertices, which most frontier edges hit:
+                            // their decrements are summed in LDS and applied
+                            // once per workgroup and phase (flush below)
+#pragma unroll
+                            for (int u = 0; u < kUnroll; ++u)
+                                if (ok[u] && x[u] < kCombRanks) {
+                                    atomicAdd(&comb[(ks[u] ? kCombRanks : 0u) + x[u]], 1u);
+                                    ok[u] = false;
+                                }
+                        }
                         int32_t old[kUnroll];
 #pragma unroll
                         for (int u = 0; u < kUnroll; ++u) {
@@ -588,6 +594,41 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
                             wave_append_len(claim[u], x[u], len, G, &nxt[0], slot_u64(nxt, 4));
                         }
                     }
+                }
+            }
+            if (comb && !pull) {
+                // apply the combined decrements: one atomic per touched
+                // counter; the atomic whose subtraction takes the counter
+                // across its threshold claims the vertex
+                __syncthreads();
+                for (uint32_t j = tid; j < 2 * kCombRanks; j += BLK) {
+                    const uint32_t c = comb[j];
+                    const bool cside = j >= kCombRanks;
+                    const uint32_t x = cside ? j - kCombRanks : j;
+                    bool claim = false;
+                    if (c) {
+                        comb[j] = 0;
+                        const int32_t old = atomicSub(cside ? &S.cd[x] : &S.kd[x], (int32_t)c);
+                        const int32_t nw = old - (int32_t)c;
+                        if (cside) {
+                            claim = k > 0 && old >= k && nw < k;
+                        } else if (old >= L + 1 && nw <= L) {
+                            claim = true;
+                        } else if (nw > L) {
+                            mn2 = min(mn2, (uint32_t)nw);
+                        }
+                        if (claim) {
+                            claim = atomicCAS(&lval[x], -1, L) == -1;
+                            if (claim) poison(S, x, s);
+                        }
+                    }
+                    uint32_t len = 0;
+                    if (claim) {
+                        uint32_t a, b;
+                        list_lens(P, S, x, a, b);
+                        len = a + b;
+                    }
+                    wave_append_len(claim, x, len, G, &nxt[0], slot_u64(nxt, 4));
                 }
             }
             wave_min_to(mn2, &nxt[2]);
@@ -902,7 +943,7 @@ __global__ __launch_bounds__(kBlk) void k_pass_grid(PassSpec P, PassScratch S, u
     __shared__ TileLds<kBlk> T;
     gs.rank = blockIdx.x;
     gs.epoch = 0;
-    run_pass<kBlk, true>(P, S, slots, T, lmax_out, &gs, nullptr, shrink_at);
+    run_pass<kBlk, true>(P, S, slots, T, lmax_out, &gs, nullptr, shrink_at, nullptr);
 }
 
 // Persistent rows kernel: block b belongs to team block_team[b]; a team
@@ -917,8 +958,11 @@ __global__ __launch_bounds__(kBlk) void k_rows_teams(PassSpec base, const uint32
                                                      uint32_t shrink_at) {
     __shared__ TileLds<kBlk> T;
     __shared__ __align__(16) uint32_t lslots[kSlotWords];
+    __shared__ uint32_t comb[2 * kCombRanks];
     const uint32_t team = block_team[blockIdx.x];
     if (team == ~0u) return;  // an idle block (its XCD had no team for it)
+    for (uint32_t j = threadIdx.x; j < 2 * kCombRanks; j += kBlk) comb[j] = 0;
+    __syncthreads();
     const TeamDesc td = teams[team];
     const PassScratch S =
         scratch_at(static_cast<uint32_t*>(scratch) + td.scratch_off, td.scratch_cap);
@@ -937,12 +981,12 @@ __global__ __launch_bounds__(kBlk) void k_rows_teams(PassSpec base, const uint32
         P.stats = base.stats ? base.stats + (size_t)kStatWords * k : nullptr;
         if (k != base.trace_k) P.trace = nullptr;
         if (td.size == 1) {
-            run_pass<kBlk, false>(P, S, lslots, T, lmax_rows + k, nullptr, err, 0);
+            run_pass<kBlk, false>(P, S, lslots, T, lmax_rows + k, nullptr, err, 0, comb);
             __syncthreads();
         } else {
             gs.nblocks = td.size;  // the pass may have narrowed the team
             run_pass<kBlk, true>(P, S, slot_pool + td.slot_off, T, lmax_rows + k, &gs, err,
-                                 shrink_at);
+                                 shrink_at, comb);
             if (!team_resync(gs, p - td.pass_begin + 1)) return;
         }
     }
