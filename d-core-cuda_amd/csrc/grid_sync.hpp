@@ -71,8 +71,12 @@ __device__ inline void sync_fail(const GridSync& g, uint32_t code) {
 __device__ inline bool sync_aborted(const GridSync& g) { return coherent_read(g.w + kSyncAbort) != 0; }
 
 // Every thread of every workgroup of the team must call this.  Returns
-// false when the launch is being aborted.
-__device__ inline bool grid_barrier(GridSync& g) {
+// false when the launch is being aborted.  fenced = false skips the L2
+// write-back and invalidation: only for callers whose data crossing the
+// barrier is written and read with agent-coherent accesses (atomics,
+// agent-scope atomic loads and stores), which the stores' completion
+// (waited for below) already makes visible.
+__device__ inline bool grid_barrier(GridSync& g, bool fenced = true) {
     __shared__ uint32_t s_ok;
     const uint32_t e = ++g.epoch;
     if (threadIdx.x == 0) s_ok = 1;
@@ -84,7 +88,7 @@ __device__ inline bool grid_barrier(GridSync& g) {
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
     if (threadIdx.x == 0) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        if (fenced) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         __builtin_amdgcn_s_waitcnt(0);
         coherent_write(g.w + kSyncFlags + g.rank, e);
     }
@@ -132,7 +136,7 @@ __device__ inline bool grid_barrier(GridSync& g) {
             __builtin_amdgcn_s_sleep(1);
         }
     }
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0 && fenced) {
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         __builtin_amdgcn_s_waitcnt(0);  // invalidation done before anyone reads
     }

@@ -55,8 +55,29 @@ constexpr int32_t kPoison = INT_MIN / 2;
 
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (W - 1)); }
 
+// Loads and stores of pass state that other workgroups of a team read or
+// wrote in earlier phases.  C (a multi-workgroup team): agent-coherent
+// (atomic relaxed at agent scope), so the phase barriers need no L2
+// write-back or invalidation; otherwise plain (one workgroup shares its
+// L1).
+template <bool C, typename T>
+__device__ __forceinline__ T ld(const T* p) {
+    if constexpr (C)
+        return __hip_atomic_load(const_cast<T*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else
+        return *p;
+}
+template <bool C, typename T>
+__device__ __forceinline__ void st(T* p, T v) {
+    if constexpr (C)
+        __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else
+        *p = v;
+}
+
 // Append `val` to arr[] for every lane with `pred`, one atomic per wavefront.
 // Every lane of the wavefront must reach this call.
+template <bool C = false>
 __device__ __forceinline__ void wave_append(bool pred, uint32_t val, uint32_t* arr,
                                             uint32_t* ctr) {
     const unsigned long long m = __ballot(pred);
@@ -65,11 +86,12 @@ __device__ __forceinline__ void wave_append(bool pred, uint32_t val, uint32_t* a
     uint32_t base = 0;
     if (lane_id() == leader) base = atomicAdd(ctr, (uint32_t)__popcll(m));
     base = __shfl(base, leader, W);
-    if (pred) arr[base + __popcll(m & ((1ull << lane_id()) - 1ull))] = val;
+    if (pred) st<C>(&arr[base + __popcll(m & ((1ull << lane_id()) - 1ull))], val);
 }
 
 // wave_append that also adds the appended vertices' list lengths `len` to
 // the 64-bit total *etot (one more atomic per wavefront that appends).
+template <bool C = false>
 __device__ __forceinline__ void wave_append_len(bool pred, uint32_t val, uint32_t len,
                                                 uint32_t* arr, uint32_t* ctr,
                                                 unsigned long long* etot) {
@@ -85,7 +107,7 @@ __device__ __forceinline__ void wave_append_len(bool pred, uint32_t val, uint32_
         consume((uint32_t)atomicAdd(etot, (unsigned long long)l));
     }
     base = __shfl(base, leader, W);
-    if (pred) arr[base + __popcll(m & ((1ull << lane_id()) - 1ull))] = val;
+    if (pred) st<C>(&arr[base + __popcll(m & ((1ull << lane_id()) - 1ull))], val);
 }
 
 // Wavefront sum of x into the 64-bit total *dst (performed before returning).
@@ -257,10 +279,14 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
     // pass state (declared first: the lambdas below read it)
     uint32_t removed = 0, s = 0, levels = 0, nA = 0, mnall_dbg = 0;
     int32_t L = 0;
-    auto barrier = [&]() -> bool {
+    // Phase barriers are unfenced: everything the team hands over between
+    // phases goes through agent-coherent accesses (ld/st<MULTI>, atomics).
+    // The set-up barrier and the last one are fenced (plain stores of the
+    // set-up and of the output rows).
+    auto barrier = [&](bool fenced = false) -> bool {
         if (MULTI && !solo) {
             if (tw == 0 && tid == 0) coherent_write(slots + 3, s);
-            return grid_barrier(*gs);
+            return grid_barrier(*gs, fenced);
         }
         __syncthreads();
         return true;
@@ -336,7 +362,7 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
         lval[i] = -1;
         S.st[i] = ~0u;
     }
-    if (!barrier()) return;
+    if (!barrier(true)) return;
 
     const uint32_t* A = nullptr;  // first level: the members in id order
     uint32_t *B = S.a0, *Bn = S.a1, *F = S.f0, *G = S.f1;
@@ -388,18 +414,19 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
             for (int u = 0; u < kSeedUnroll; ++u) {
                 const uint32_t i = base + u * BLK + tid;
                 has[u] = i < nA;
-                v[u] = has[u] ? (A ? A[i] : i) : 0u;
+                v[u] = has[u] ? (A ? ld<MULTI>(&A[i]) : i) : 0u;
             }
             int32_t d[kSeedUnroll];
             bool alive[kSeedUnroll];
 #pragma unroll
             for (int u = 0; u < kSeedUnroll; ++u) {
-                alive[u] = has[u] && lval[v[u]] == -1;
-                d[u] = alive[u] ? S.kd[v[u]] : 0;
+                alive[u] = has[u] && ld<MULTI>(&lval[v[u]]) == -1;
+                d[u] = alive[u] ? ld<MULTI>(&S.kd[v[u]]) : 0;
             }
 #pragma unroll
             for (int u = 0; u < kSeedUnroll; ++u) {
-                const bool seed = alive[u] && (d[u] <= L || (check_c && S.cd[v[u]] < k));
+                const bool seed =
+                    alive[u] && (d[u] <= L || (check_c && ld<MULTI>(&S.cd[v[u]]) < k));
                 const bool keep = alive[u] && !seed;
                 uint32_t len = 0;
                 if (seed || (first && keep)) {
@@ -408,12 +435,12 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
                     len = kl + cl;
                 }
                 if (seed) {
-                    lval[v[u]] = L;
+                    st<MULTI>(&lval[v[u]], L);
                     poison(S, v[u], s);
                 }
                 if (keep) mn = min(mn, (uint32_t)d[u]);
-                wave_append_len(seed, v[u], len, F, &cur[0], slot_u64(cur, 4));
-                wave_append(keep, v[u], B, &cur[1]);
+                wave_append_len<MULTI>(seed, v[u], len, F, &cur[0], slot_u64(cur, 4));
+                wave_append<MULTI>(keep, v[u], B, &cur[1]);
                 if (first) wave_sum_to(keep ? len : 0u, slot_u64(cur, 6));
             }
         }
@@ -455,8 +482,8 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
                 const uint32_t nw = BLK / W;
                 const uint32_t lane = (uint32_t)lane_id();
                 for (uint32_t i = tw * nw + tid / W; i < nA; i += tn * nw) {
-                    const uint32_t x = A[i];
-                    if (lval[x] != -1) continue;  // removed earlier in this level
+                    const uint32_t x = ld<MULTI>(&A[i]);
+                    if (ld<MULTI>(&lval[x]) != -1) continue;  // removed earlier in this level
                     uint32_t kl, cl;
                     list_lens(P, S, x, kl, cl);
                     const uint32_t* kn = P.K.nbr + P.K.off[x];
@@ -470,7 +497,7 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
                             isk[u] = e < kl;
                             const bool ok = e < kl + cl;
                             const uint32_t y = !ok ? 0u : isk[u] ? kn[e] : cn[e - kl];
-                            hit[u] = ok && S.st[y] == sprev;
+                            hit[u] = ok && ld<MULTI>(&S.st[y]) == sprev;
                         }
 #pragma unroll
                         for (int u = 0; u < kUnroll; ++u) {
@@ -482,11 +509,11 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
                     if ((ck | cc) && lane == 0) {
                         // x's own counters: nothing else writes them in a
                         // pull phase
-                        const int32_t nk = S.kd[x] - (int32_t)ck;
-                        const int32_t nc = S.cd[x] - (int32_t)cc;
+                        const int32_t nk = ld<MULTI>(&S.kd[x]) - (int32_t)ck;
+                        const int32_t nc = ld<MULTI>(&S.cd[x]) - (int32_t)cc;
                         claim = nk <= L || (k > 0 && nc < k);
                         if (claim) {
-                            lval[x] = L;
+                            st<MULTI>(&lval[x], L);
                             poison(S, x, s);
                         } else {
                             if (ck) {
@@ -496,7 +523,7 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
                             if (cc) store_agent(S.cd + x, nc);
                         }
                     }
-                    wave_append_len(claim, x, kl + cl, G, &nxt[0], slot_u64(nxt, 4));
+                    wave_append_len<MULTI>(claim, x, kl + cl, G, &nxt[0], slot_u64(nxt, 4));
                 }
             } else {
                 // Push: frontier items per workgroup -- a team splits even
@@ -507,7 +534,7 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
                 for (uint32_t base = tw * ipw; base < nF; base += tn * ipw) {
                     const uint32_t i = base + tid;
                     const bool has = tid < ipw && i < nF;
-                    const uint32_t v = has ? F[i] : 0u;
+                    const uint32_t v = has ? ld<MULTI>(&F[i]) : 0u;
                     __syncthreads();
                     uint32_t kl = 0, cl = 0;
                     if (has) {
@@ -591,7 +618,8 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
                                 list_lens(P, S, x[u], a, b);
                                 len = a + b;
                             }
-                            wave_append_len(claim[u], x[u], len, G, &nxt[0], slot_u64(nxt, 4));
+                            wave_append_len<MULTI>(claim[u], x[u], len, G, &nxt[0],
+                                                   slot_u64(nxt, 4));
                         }
                     }
                 }
@@ -628,7 +656,7 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
                         list_lens(P, S, x, a, b);
                         len = a + b;
                     }
-                    wave_append_len(claim, x, len, G, &nxt[0], slot_u64(nxt, 4));
+                    wave_append_len<MULTI>(claim, x, len, G, &nxt[0], slot_u64(nxt, 4));
                 }
             }
             wave_min_to(mn2, &nxt[2]);
@@ -674,11 +702,12 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
     }
     if (P.core) {
         // levels back to original ids (rows hold 0 for non-members already)
-        for (uint32_t i = tw * BLK + tid; i < nm; i += tn * BLK) P.out_row[P.order[i]] = S.lv[i];
+        for (uint32_t i = tw * BLK + tid; i < nm; i += tn * BLK)
+            P.out_row[P.order[i]] = ld<MULTI>(&S.lv[i]);
     }
     // the whole (narrowed) team is done with the scratch before rank 0
     // hands the team on (team_resync)
-    if (MULTI && !solo) barrier();
+    if (MULTI && !solo) barrier(true);
 }
 
 // ---- LDS-resident pass for small cores -------------------------------
