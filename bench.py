@@ -89,7 +89,8 @@ def main():
     ap.add_argument("--no-hindex", action="store_true", help="skip the h-index timing")
     ap.add_argument("--no-tune", action="store_true", help="skip the launch-shape autotuner")
     ap.add_argument("--shape", default="",
-                    help="rows launch shape SLOTS,TEAM_MAX,SHRINK_AT instead of the autotuner "
+                    help="rows launch shape SLOTS,TEAM_MAX,SHRINK_AT[,LDS_MEMBERS] instead of the "
+                         "autotuner "
                          "(e.g. the shape an earlier run picked, for counter runs)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -127,13 +128,16 @@ def run_rmat(args):
     variant = {"batched": dcx.VARIANT_BATCHED, "peel": dcx.VARIANT_PEEL,
                "hindex": dcx.VARIANT_HINDEX}[args.variant]
     shape = [int(x) for x in args.shape.split(",")] if args.shape else [0, 0, 0]
-    ctx = dcx.Context(0, rows_slots=shape[0], team_max=shape[1], shrink_at=shape[2])
+    shape += [0] * (4 - len(shape))
+    ctx = dcx.Context(0, rows_slots=shape[0], team_max=shape[1], shrink_at=shape[2],
+                      lds_members=shape[3])
     g = ctx.upload_edges(V, src, dst, slack=0)
     tune = None
     if variant == dcx.VARIANT_BATCHED and not args.no_tune and not args.shape:
         g.kmax()
         r = g.autotune(reps=2)
         tune = {"rows_slots": r.rows_slots, "team_max": r.team_max, "shrink_at": r.shrink_at,
+                "lds_members": r.lds_members,
                 "candidates": r.candidates, "best_rows_ms": round(r.best_ms, 3),
                 "default_rows_ms": round(r.default_ms, 3)}
         print(f"# autotune: {tune}", file=sys.stderr, flush=True)
@@ -264,6 +268,7 @@ def run_chunglu(args):
     if not args.no_tune:
         r = g.autotune(reps=2)
         tune = {"rows_slots": r.rows_slots, "team_max": r.team_max, "shrink_at": r.shrink_at,
+                "lds_members": r.lds_members,
                 "candidates": r.candidates, "best_rows_ms": round(r.best_ms, 3),
                 "default_rows_ms": round(r.default_ms, 3)}
         print(f"# autotune: {tune}", file=sys.stderr, flush=True)

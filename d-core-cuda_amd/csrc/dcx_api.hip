@@ -842,6 +842,7 @@ dcx_status dcx_ctx_create(const dcx_config* cfg, dcx_ctx** out) {
     const int64_t room = (int64_t)max_lds - (int64_t)rows_lds_static_bytes() - 256;
     uint32_t cap = room > 0 ? (uint32_t)std::min<int64_t>(kLdsMaxMembers, room * 32 / 132) : 0u;
     cap = cap / 32 * 32;
+    c->lds_cap_max = cap;
     if (cfg->lds_members < 0)
         cap = 0;
     else if (cfg->lds_members > 0)

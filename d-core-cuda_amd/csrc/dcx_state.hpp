@@ -27,7 +27,8 @@ struct dcx_ctx {
     uint32_t* trace = nullptr;   // phase trace of the last grid pass (cfg.trace_phases)
     uint32_t* fail_rec = nullptr;  // [kFailWords] first failing pass member's view
     dcx_mlp* sched_mlp = nullptr;  // cost model of the rows scheduler (nullptr: heuristic)
-    uint32_t lds_cap = 0;  // most members of a row run LDS-resident (0: none)
+    uint32_t lds_cap = 0;      // most members of a row run LDS-resident (0: none)
+    uint32_t lds_cap_max = 0;  // what the device's LDS allows (lds_cap <= this)
     hipStream_t stream = nullptr;
     // second stream (default priority) for the LDS-resident rows, which run
     // beside the team launch; fork/join order it against `stream`

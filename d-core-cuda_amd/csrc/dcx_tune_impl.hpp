@@ -1,8 +1,9 @@
 // Autotuner for the batched rows launch: included at the end of dcx_api.hip.
 //
-// The batched launch has three shape knobs -- workgroups in the launch,
-// the most workgroups one k-pass may get (team_max), and the alive-list
-// length per team member below which a team narrows (shrink_at).  Every
+// The batched launch has four shape knobs -- workgroups in the launch,
+// the most workgroups one k-pass may get (team_max), the alive-list length
+// per team member below which a team narrows (shrink_at), and the largest
+// core that runs LDS-resident beside the teams (lds cap).  Every
 // shape gives the same bits (the passes are exact whatever their team), so
 // the tuner only measures: coordinate descent over the three knobs, each
 // candidate timed with HIP events around the rows kernel on the graph
@@ -15,12 +16,14 @@ namespace {
 
 struct TuneShape {
     int32_t slots, team_max, shrink_at;
+    uint32_t lds;
 };
 
 double time_rows_shape(dcx_ctx* c, dcx_graph* g, const TuneShape& t, int reps) {
     c->cfg.rows_slots = t.slots;
     c->cfg.team_max = t.team_max;
     c->cfg.shrink_at = t.shrink_at;
+    c->lds_cap = t.lds;
     double best = 1e300;
     for (int r = 0; r < reps; ++r) {
         compute_rows(c, g, DCX_VARIANT_BATCHED);
@@ -43,7 +46,8 @@ dcx_status dcx_autotune(dcx_ctx* ctx, dcx_graph* g, int32_t reps, dcx_tune_resul
         const int32_t slot_opts[] = {cus, 2 * cus};
         const int32_t team_opts[] = {4, 8, 16, 32, 64, 128, 256};
         const int32_t shrink_opts[] = {1024, 2048, 4096, 8192, 16384};
-        TuneShape best{2 * cus, (int32_t)kTeamMax, (int32_t)kShrinkAt};
+        const uint32_t lds_opts[] = {ctx->lds_cap_max, 16384, 8192, 4096, 0};
+        TuneShape best{2 * cus, (int32_t)kTeamMax, (int32_t)kShrinkAt, ctx->lds_cap};
         int tried = 0;
         const double def_ms = time_rows_shape(ctx, g, best, reps);
         double best_ms = def_ms;
@@ -57,13 +61,17 @@ dcx_status dcx_autotune(dcx_ctx* ctx, dcx_graph* g, int32_t reps, dcx_tune_resul
             }
         };
         for (int32_t tm : team_opts)
-            if (tm != best.team_max) consider(TuneShape{best.slots, tm, best.shrink_at});
+            if (tm != best.team_max) consider(TuneShape{best.slots, tm, best.shrink_at, best.lds});
         const int32_t tm_best = best.team_max;
         for (int32_t sh : shrink_opts)
-            if (sh != best.shrink_at) consider(TuneShape{best.slots, tm_best, sh});
+            if (sh != best.shrink_at) consider(TuneShape{best.slots, tm_best, sh, best.lds});
         const int32_t sh_best = best.shrink_at;
         for (int32_t sl : slot_opts)
-            if (sl != best.slots) consider(TuneShape{sl, tm_best, sh_best});
+            if (sl != best.slots) consider(TuneShape{sl, tm_best, sh_best, best.lds});
+        const int32_t sl_best = best.slots;
+        for (uint32_t lc : lds_opts)
+            if (lc <= ctx->lds_cap_max && lc != best.lds)
+                consider(TuneShape{sl_best, tm_best, sh_best, lc});
         // leave the winner configured and its rows resident
         time_rows_shape(ctx, g, best, 1);
         if (out) {
@@ -74,6 +82,7 @@ dcx_status dcx_autotune(dcx_ctx* ctx, dcx_graph* g, int32_t reps, dcx_tune_resul
             out->candidates = tried;
             out->best_ms = best_ms;
             out->default_ms = def_ms;
+            out->lds_members = best.lds ? (int32_t)best.lds : -1;
         }
         return DCX_OK;
     });

@@ -726,6 +726,30 @@ __device__ __forceinline__ bool lds_removed(const uint32_t* rmv, uint32_t x) {
     return (rmv[x >> 5] >> (x & 31)) & 1u;
 }
 
+// One frontier edge of the LDS-resident pass: x loses a neighbour, on the
+// constraint side (high half) when ks, else on the key side.  Returns true
+// when this call claims x (at most one call per vertex does).
+__device__ __forceinline__ bool lds_edge(uint32_t* cnt, uint32_t* rmv, uint32_t x, bool ks,
+                                         int32_t k, int32_t L, uint32_t& mn2) {
+    if (lds_removed(rmv, x)) return false;
+    bool claim = false;
+    if (ks) {
+        const uint32_t old = atomicSub(&cnt[x], 0x10000u);
+        claim = k > 0 && (int32_t)(old >> 16) == k;
+    } else {
+        const int32_t old = (int32_t)(atomicSub(&cnt[x], 1u) & 0xffffu);
+        if (old == L + 1)
+            claim = true;
+        else if (old - 1 > L)
+            mn2 = min(mn2, (uint32_t)(old - 1));
+    }
+    if (claim) {
+        const uint32_t bit = 1u << (x & 31);
+        claim = !(atomicOr(&rmv[x >> 5], bit) & bit);
+    }
+    return claim;
+}
+
 template <int BLK>
 __device__ void run_pass_lds(const PassSpec& P, const PassScratch& S, uint32_t* slots,
                              TileLds<BLK>& T, uint32_t* cnt, uint32_t* rmv, int32_t* lmax_out,
@@ -747,6 +771,7 @@ __device__ void run_pass_lds(const PassSpec& P, const PassScratch& S, uint32_t* 
         slots[kSlotStride * tid + 0] = 0;
         slots[kSlotStride * tid + 1] = 0;
         slots[kSlotStride * tid + 2] = UINT_MAX;
+        *slot_u64(slots + kSlotStride * tid, 4) = 0;
     }
     __syncthreads();
 
@@ -781,6 +806,7 @@ __device__ void run_pass_lds(const PassSpec& P, const PassScratch& S, uint32_t* 
             other[0] = 0;
             other[1] = 0;
             other[2] = UINT_MAX;
+            *slot_u64(other, 4) = 0;
         }
         uint32_t mn = UINT_MAX;
         for (uint32_t base = 0; base < nA; base += BLK * kSeedUnroll) {
@@ -798,12 +824,14 @@ __device__ void run_pass_lds(const PassSpec& P, const PassScratch& S, uint32_t* 
                 const int32_t d = alive ? (int32_t)(cnt[v[u]] & 0xffffu) : 0;
                 const bool seed = alive && d <= L;
                 const bool keep = alive && !seed;
+                uint32_t len = 0;
                 if (seed) {
                     atomicOr(&rmv[v[u] >> 5], 1u << (v[u] & 31));
                     S.lv[v[u]] = L;
+                    len = S.rk[v[u]] + S.rc[v[u]];
                 }
                 if (keep) mn = min(mn, (uint32_t)d);
-                wave_append(seed, v[u], F, &cur[0]);
+                wave_append_len(seed, v[u], len, F, &cur[0], slot_u64(cur, 4));
                 wave_append(keep, v[u], B, &cur[1]);
             }
         }
@@ -812,6 +840,7 @@ __device__ void run_pass_lds(const PassSpec& P, const PassScratch& S, uint32_t* 
         uint32_t nF = cur[0];
         nA = cur[1];
         uint32_t mnall = cur[2];
+        uint64_t fe = *slot_u64(cur, 4);
         __syncthreads();
         trace(true, nF);
         A = B;
@@ -830,72 +859,99 @@ __device__ void run_pass_lds(const PassSpec& P, const PassScratch& S, uint32_t* 
                 nx2[0] = 0;
                 nx2[1] = 0;
                 nx2[2] = UINT_MAX;
+                *slot_u64(nx2, 4) = 0;
             }
             uint32_t mn2 = UINT_MAX;
-            for (uint32_t base = 0; base < nF; base += BLK) {
-                const uint32_t i = base + tid;
-                const bool has = i < nF;
-                const uint32_t v = has ? F[i] : 0u;
-                __syncthreads();
-                uint32_t kl = 0, cl = 0;
-                if (has) {
-                    T.koff[tid] = P.K.off[v];
-                    T.coff[tid] = P.C.off[v];
-                    kl = S.rk[v];
-                    cl = S.rc[v];
-                }
-                T.klen[tid] = kl;
-                if (tid == 0) T.nitems = min(nF - base, (uint32_t)BLK);
-                T.scan[tid] = block_excl_scan<BLK>(kl + cl, T.wsum, &T.total);
-                __syncthreads();
-                const uint32_t total = T.total;
-                for (uint32_t eb = 0; eb < total; eb += kLdsUnroll * BLK) {
-                    uint32_t x[kLdsUnroll];
-                    bool ks[kLdsUnroll], ok[kLdsUnroll], claim[kLdsUnroll];
+            if (fe >= 64ull * nF) {
+                // long lists (a dense core): one wavefront per frontier
+                // vertex, coalesced list reads, no tile search
+                const uint32_t lane = (uint32_t)lane_id();
+                for (uint32_t i = tid / W; i < nF; i += BLK / W) {
+                    const uint32_t v = F[i];
+                    const uint32_t kl = S.rk[v], tot = kl + S.rc[v];
+                    const uint32_t* kn = P.K.nbr + P.K.off[v];
+                    const uint32_t* cn = P.C.nbr + P.C.off[v];
+                    for (uint32_t jb = 0; jb < tot; jb += kLdsUnroll * W) {
+                        uint32_t x[kLdsUnroll];
+                        bool ks[kLdsUnroll], claim[kLdsUnroll];
 #pragma unroll
-                    for (int u = 0; u < kLdsUnroll; ++u) {
-                        const uint32_t e = eb + u * BLK + tid;
-                        ok[u] = e < total;
-                        x[u] = 0;
-                        ks[u] = false;
-                        if (ok[u]) {
-                            const uint32_t it = tile_find<BLK>(T, e);
-                            const uint32_t t = e - T.scan[it];
-                            ks[u] = t < T.klen[it];
-                            x[u] = ks[u] ? P.K.nbr[T.koff[it] + t]
-                                         : P.C.nbr[T.coff[it] + (t - T.klen[it])];
+                        for (int u = 0; u < kLdsUnroll; ++u) {
+                            const uint32_t e = jb + u * W + lane;
+                            ks[u] = e < kl;
+                            x[u] = e < tot ? (ks[u] ? kn[e] : cn[e - kl]) : 0u;
+                            claim[u] = false;
                         }
-                    }
 #pragma unroll
-                    for (int u = 0; u < kLdsUnroll; ++u) {
-                        claim[u] = false;
-                        if (ok[u] && !lds_removed(rmv, x[u])) {
-                            if (ks[u]) {
-                                // x loses a C-neighbour: constraint half
-                                const uint32_t old = atomicSub(&cnt[x[u]], 0x10000u);
-                                claim[u] = k > 0 && (int32_t)(old >> 16) == k;
-                            } else {
-                                const int32_t old = (int32_t)(atomicSub(&cnt[x[u]], 1u) & 0xffffu);
-                                if (old == L + 1)
-                                    claim[u] = true;
-                                else if (old - 1 > L)
-                                    mn2 = min(mn2, (uint32_t)(old - 1));
-                            }
+                        for (int u = 0; u < kLdsUnroll; ++u)
+                            if (jb + u * W + lane < tot)
+                                claim[u] = lds_edge(cnt, rmv, x[u], ks[u], k, L, mn2);
+#pragma unroll
+                        for (int u = 0; u < kLdsUnroll; ++u) {
+                            uint32_t len = 0;
                             if (claim[u]) {
-                                const uint32_t bit = 1u << (x[u] & 31);
-                                claim[u] = !(atomicOr(&rmv[x[u] >> 5], bit) & bit);
-                                if (claim[u]) S.lv[x[u]] = L;
+                                S.lv[x[u]] = L;
+                                len = S.rk[x[u]] + S.rc[x[u]];
                             }
+                            wave_append_len(claim[u], x[u], len, G, &nxt[0], slot_u64(nxt, 4));
                         }
                     }
+                }
+            } else {
+                for (uint32_t base = 0; base < nF; base += BLK) {
+                    const uint32_t i = base + tid;
+                    const bool has = i < nF;
+                    const uint32_t v = has ? F[i] : 0u;
+                    __syncthreads();
+                    uint32_t kl = 0, cl = 0;
+                    if (has) {
+                        T.koff[tid] = P.K.off[v];
+                        T.coff[tid] = P.C.off[v];
+                        kl = S.rk[v];
+                        cl = S.rc[v];
+                    }
+                    T.klen[tid] = kl;
+                    if (tid == 0) T.nitems = min(nF - base, (uint32_t)BLK);
+                    T.scan[tid] = block_excl_scan<BLK>(kl + cl, T.wsum, &T.total);
+                    __syncthreads();
+                    const uint32_t total = T.total;
+                    for (uint32_t eb = 0; eb < total; eb += kLdsUnroll * BLK) {
+                        uint32_t x[kLdsUnroll];
+                        bool ks[kLdsUnroll], claim[kLdsUnroll];
 #pragma unroll
-                    for (int u = 0; u < kLdsUnroll; ++u) wave_append(claim[u], x[u], G, &nxt[0]);
+                        for (int u = 0; u < kLdsUnroll; ++u) {
+                            const uint32_t e = eb + u * BLK + tid;
+                            x[u] = 0;
+                            ks[u] = false;
+                            claim[u] = false;
+                            if (e < total) {
+                                const uint32_t it = tile_find<BLK>(T, e);
+                                const uint32_t t = e - T.scan[it];
+                                ks[u] = t < T.klen[it];
+                                x[u] = ks[u] ? P.K.nbr[T.koff[it] + t]
+                                             : P.C.nbr[T.coff[it] + (t - T.klen[it])];
+                            }
+                        }
+#pragma unroll
+                        for (int u = 0; u < kLdsUnroll; ++u)
+                            if (eb + u * BLK + tid < total)
+                                claim[u] = lds_edge(cnt, rmv, x[u], ks[u], k, L, mn2);
+#pragma unroll
+                        for (int u = 0; u < kLdsUnroll; ++u) {
+                            uint32_t len = 0;
+                            if (claim[u]) {
+                                S.lv[x[u]] = L;
+                                len = S.rk[x[u]] + S.rc[x[u]];
+                            }
+                            wave_append_len(claim[u], x[u], len, G, &nxt[0], slot_u64(nxt, 4));
+                        }
+                    }
                 }
             }
             wave_min_to(mn2, &nxt[2]);
             __syncthreads();
             nF = nxt[0];
             mnall = min(mnall, nxt[2]);
+            fe = *slot_u64(nxt, 4);
             __syncthreads();
             trace(false, nF);
             {
@@ -944,7 +1000,7 @@ __global__ __launch_bounds__(kLdsBlk) void k_rows_lds(PassSpec base, const uint3
                                                       uint32_t cap, uint32_t* err) {
     extern __shared__ uint32_t dyn[];
     __shared__ TileLds<kLdsBlk> T;
-    __shared__ uint32_t lslots[2 * kSlotStride];
+    __shared__ __align__(16) uint32_t lslots[2 * kSlotStride];
     __shared__ uint32_t s_next;
     const PassScratch S = scratch_at(
         static_cast<uint32_t*>(scratch) + (size_t)blockIdx.x * kScratchArrays * cap, cap);

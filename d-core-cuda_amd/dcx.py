@@ -62,7 +62,8 @@ class TuneResult(ctypes.Structure):
     _fields_ = [("rows_slots", ctypes.c_int32), ("team_max", ctypes.c_int32),
                 ("shrink_at", ctypes.c_int32), ("candidates", ctypes.c_int32),
                 ("best_ms", ctypes.c_double), ("default_ms", ctypes.c_double),
-                ("reserved", ctypes.c_double * 4)]
+                ("lds_members", ctypes.c_int32), ("reserved32", ctypes.c_int32),
+                ("reserved", ctypes.c_double * 3)]
 
 
 class ServeConfig(ctypes.Structure):

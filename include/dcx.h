@@ -391,7 +391,10 @@ typedef struct dcx_tune_result {
     int32_t candidates;   /* shapes timed */
     double best_ms;       /* rows kernel time of the chosen shape */
     double default_ms;    /* rows kernel time of the default shape */
-    double reserved[4];
+    int32_t lds_members;  /* chosen LDS-resident row cap (-1: no LDS rows), as
+                             dcx_config.lds_members takes it */
+    int32_t reserved32;
+    double reserved[3];
 } dcx_tune_result;
 dcx_status dcx_autotune(dcx_ctx* ctx, dcx_graph* g, int32_t reps, dcx_tune_result* out);
 
