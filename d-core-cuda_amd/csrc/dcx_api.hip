@@ -527,6 +527,8 @@ void compute_rows(dcx_ctx* c, dcx_graph* g, int variant, int32_t kr = -1) {
     HIPX(c, hipEventRecord(c->ev[3], c->stream));
     HIPX(c, hipMemsetAsync(g->rows, 0, nre * V * sizeof(int32_t), c->stream));
     int launches = 0, slots_used = 1;
+    int32_t split = kr + 1;  // batched: first LDS-resident row
+    bool teams_ran = false, lds_ran = false;
     std::vector<std::pair<uint32_t, int32_t>> last_teams;  // (size, first k) for diagnosis
     if (variant == DCX_VARIANT_HINDEX) {
         // rows in k order, each started from the row above (l_k <= l_{k-1})
@@ -648,17 +650,23 @@ void compute_rows(dcx_ctx* c, dcx_graph* g, int variant, int32_t kr = -1) {
             team_err = c->pools + kSyncError;
             nteam_err = (uint32_t)nt;
             team_err_stride = s.sync_stride;
+            HIPX(c, hipEventRecord(c->sev[2], c->stream));
         }
         if (!lks.empty()) {
+            HIPX(c, hipEventRecord(c->sev[0], c->side));
             HIPX(c, launch_rows_lds(basep, g->vk, g->rows, g->lmax_rows, d_lks,
                                     (uint32_t)lks.size(), c->ctrs + kCtrQueue, lblocks,
                                     static_cast<uint32_t*>(c->scratch) + lds_off, lcap,
                                     c->ctrs + kCtrErr, c->side));
             ++launches;
+            HIPX(c, hipEventRecord(c->sev[1], c->side));
             HIPX(c, hipEventRecord(c->join, c->side));
             HIPX(c, hipStreamWaitEvent(c->stream, c->join, 0));
         }
         slots_used = (int)std::max(busy, lblocks);
+        split = ksplit;
+        teams_ran = nt > 0;
+        lds_ran = !lks.empty();
     }
     HIPX(c, hipEventRecord(c->ev[6], c->stream));
     try {
@@ -687,6 +695,19 @@ void compute_rows(dcx_ctx* c, dcx_graph* g, int variant, int32_t kr = -1) {
     c->timing.rows_kernel_ms = elapsed(c, 5, 6);
     c->timing.rows_launches = launches;
     c->timing.rows_slots = slots_used;
+    c->timing.rows_split = split;
+    c->timing.rows_teams_ms = 0.0f;
+    c->timing.rows_lds_ms = 0.0f;
+    if (teams_ran) {
+        float ms = 0;
+        HIPX(c, hipEventElapsedTime(&ms, c->ev[5], c->sev[2]));
+        c->timing.rows_teams_ms = ms;
+    }
+    if (lds_ran) {
+        float ms = 0;
+        HIPX(c, hipEventElapsedTime(&ms, c->sev[0], c->sev[1]));
+        c->timing.rows_lds_ms = ms;
+    }
     g->lmax = 0;
     for (int32_t x : lm) g->lmax = std::max(g->lmax, x);
     g->h_pstats.resize(((size_t)K + 2) * kStatWords);
@@ -856,7 +877,9 @@ dcx_status dcx_ctx_create(const dcx_config* cfg, dcx_ctx** out) {
         if (hipEventCreate(&e) != hipSuccess) return DCX_ERR_HIP;
     if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->join, hipEventDisableTiming) != hipSuccess)
+        hipEventCreateWithFlags(&c->join, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreate(&c->sev[0]) != hipSuccess || hipEventCreate(&c->sev[1]) != hipSuccess ||
+        hipEventCreate(&c->sev[2]) != hipSuccess)
         return DCX_ERR_HIP;
     if (hipMalloc(&c->ctrs, kCtrWords * sizeof(uint32_t)) != hipSuccess)
         return DCX_ERR_OUT_OF_MEMORY;
@@ -890,6 +913,8 @@ dcx_status dcx_ctx_destroy(dcx_ctx* c) {
     if (c->side) (void)hipStreamDestroy(c->side);
     if (c->fork) (void)hipEventDestroy(c->fork);
     if (c->join) (void)hipEventDestroy(c->join);
+    for (auto& e : c->sev)
+        if (e) (void)hipEventDestroy(e);
     delete c;
     return DCX_OK;
 }

@@ -34,6 +34,9 @@ struct dcx_ctx {
     // beside the team launch; fork/join order it against `stream`
     hipStream_t side = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
+    // [0], [1]: brackets of the LDS-resident launch on `side`; [2]: end of
+    // the team launch on `stream` (which ev[5] opens)
+    hipEvent_t sev[3] = {};
     std::mutex mu;
     std::string err;
     dcx_timing timing{};

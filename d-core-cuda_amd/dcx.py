@@ -40,7 +40,8 @@ class Timing(ctypes.Structure):
                 ("rows_ms", ctypes.c_double), ("rows_kernel_ms", ctypes.c_double),
                 ("total_ms", ctypes.c_double), ("rows_launches", ctypes.c_int32),
                 ("rows_slots", ctypes.c_int32), ("rows_scheduler", ctypes.c_int32),
-                ("reserved", ctypes.c_int32 * 5)]
+                ("rows_split", ctypes.c_int32), ("rows_teams_ms", ctypes.c_float),
+                ("rows_lds_ms", ctypes.c_float), ("reserved", ctypes.c_int32 * 2)]
 
 
 class PassLog(ctypes.Structure):

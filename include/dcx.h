@@ -175,7 +175,10 @@ typedef struct dcx_timing {
     int32_t rows_launches; /* kernel launches spent on the rows */
     int32_t rows_slots;    /* workgroups busy in the batched launch */
     int32_t rows_scheduler; /* batched team widths: 0 work-share rule, 1 cost model */
-    int32_t reserved[5];
+    int32_t rows_split;    /* batched: rows k >= rows_split ran LDS-resident */
+    float rows_teams_ms;   /* batched: the team launch alone */
+    float rows_lds_ms;     /* batched: the LDS-resident launch alone (beside it) */
+    int32_t reserved[2];
 } dcx_timing;
 
 dcx_status dcx_config_init(dcx_config* cfg);
