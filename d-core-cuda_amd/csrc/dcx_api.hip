@@ -106,12 +106,13 @@ void audit_adjacency(dcx_ctx* c, dcx_graph* g) {
                 (unsigned long long)w[1], (unsigned long long)w[3]);
         throw Fail{DCX_ERR_INTERNAL};
     }
+    g->audited = true;
 }
 
 // Relabels by kmax and sorts every list (see core_graph.hip).
 void build_core(dcx_ctx* c, dcx_graph* g) {
     const uint32_t V = g->V;
-    audit_adjacency(c, g);
+    if (!g->audited) audit_adjacency(c, g);  // once per edit, not per rebuild
     if (!g->rank) {
         dev_alloc(c, &g->rank, V);
         dev_alloc(c, &g->cout_off, (size_t)V + 1);
@@ -566,8 +567,11 @@ void compute_rows(dcx_ctx* c, dcx_graph* g, int variant, int32_t kr = -1) {
         c->timing.rows_scheduler = 0;
         int32_t ksplit = 0;  // rows [0, ksplit) on teams, [ksplit, kr] in LDS
         while (ksplit <= kr && g->h_vk[ksplit] > c->lds_cap) ++ksplit;
-        const uint32_t G = (uint32_t)(c->cfg.rows_slots > 0 ? c->cfg.rows_slots
-                                                              : c->num_cus * kRowsBlocksPerCu);
+        // every workgroup of the launch must be resident at once (a team's
+        // members wait for each other): never more than the CUs hold
+        const uint32_t G = (uint32_t)std::min<int64_t>(
+            c->cfg.rows_slots > 0 ? c->cfg.rows_slots : c->num_cus * kRowsBlocksPerCu,
+            (int64_t)c->num_cus * c->rows_per_cu);
         const uint32_t tmax = c->cfg.team_max > 0 ? (uint32_t)c->cfg.team_max : kTeamMax;
         const uint32_t shrink = c->cfg.shrink_at > 0 ? (uint32_t)c->cfg.shrink_at : kShrinkAt;
         Sched s;
@@ -847,6 +851,7 @@ dcx_status dcx_ctx_create(const dcx_config* cfg, dcx_ctx** out) {
     c->num_cus = prop.multiProcessorCount;
     c->grid_blocks = (int)std::min<uint32_t>(
         kMaxGridBlocks, c->num_cus * std::min(kGridBlocksPerCu, pass_grid_blocks_per_cu()));
+    c->rows_per_cu = std::max(1, rows_teams_blocks_per_cu());
     c->hindex_grid = (int)std::min<uint32_t>(
         kMaxGridBlocks, c->num_cus * std::min(kGridBlocksPerCu, hindex_blocks_per_cu()));
     int clock_khz = 0;

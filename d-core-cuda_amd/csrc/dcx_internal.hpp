@@ -64,7 +64,8 @@ enum : uint32_t { kFailWords = 16 };
 // Words of a slot: [0] frontier appends, [1] alive-list appends, [2] smallest
 // key degree above the level, [3] the leader's phase, [4..5] edges of the
 // frontier's lists (uint64), [6..7] edges of the alive lists (uint64, first
-// level only).
+// level only), [8] frontier vertices with long lists (kept apart), [9] alive
+// vertices with long lists (first level only).
 enum : uint32_t { kSlotStride = 32, kSlotWords = 3 * kSlotStride };
 
 // Phase trace entry (peel passes run on one team when tracing is on).
@@ -105,8 +106,11 @@ struct PassScratch {
     uint32_t* a0;  // alive list ping
     uint32_t* a1;  // alive list pong
     uint32_t* st;  // phase in which a vertex was removed (~0u while alive)
+    uint32_t* fl;  // frontier vertices with long lists, two halves (ping-pong)
+    uint32_t* lp;  // alive vertices with long lists, then their partial
+                   // pull counts (key side, constraint side): quarters
 };
-constexpr int kScratchArrays = 10;
+constexpr int kScratchArrays = 12;
 
 __host__ __device__ inline PassScratch scratch_at(void* base, uint64_t cap) {
     uint32_t* p = reinterpret_cast<uint32_t*>(base);
@@ -121,6 +125,8 @@ __host__ __device__ inline PassScratch scratch_at(void* base, uint64_t cap) {
     s.a0 = p + 7 * cap;
     s.a1 = p + 8 * cap;
     s.st = p + 9 * cap;
+    s.fl = p + 10 * cap;
+    s.lp = p + 11 * cap;
     return s;
 }
 
@@ -164,6 +170,8 @@ hipError_t launch_mlp_adam(float* p, const float* grad, float* m, float* v, uint
 hipError_t launch_pass_grid(const PassSpec& p, const PassScratch& s, uint32_t* slots,
                             int32_t* lmax_out, const GridSync& gs, hipStream_t st);
 int pass_grid_blocks_per_cu();
+// Workgroups of the team rows kernel that fit one CU at once.
+int rows_teams_blocks_per_cu();
 // Rows of one graph (core layout): the launch has `blocks` workgroups;
 // block_team[b] names the team of block b (~0u: the block has no work) and
 // block_team[blocks + b] its rank in the team.  Pass k of team t is

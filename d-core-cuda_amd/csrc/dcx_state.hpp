@@ -22,6 +22,7 @@ struct dcx_ctx {
     int num_cus = 0;
     int grid_blocks = 0;    // workgroups of a whole-grid pass / of the rows launch
     int hindex_grid = 0;
+    int rows_per_cu = 1;    // team rows workgroups that fit one CU (occupancy)
     uint64_t timeout_ticks = 0;  // wall-clock ticks one team barrier may take
     double wall_khz = 100000.0;  // device wall clock rate
     uint32_t* trace = nullptr;   // phase trace of the last grid pass (cfg.trace_phases)
@@ -97,6 +98,7 @@ struct dcx_graph {
     dcx_ctx* ctx = nullptr;
     bool served = false;  // owned by a running dcx_server: API calls refused
     bool broken = false;  // an edit failed half-way: the adjacency is not trusted
+    bool audited = false; // the adjacency passed the audit and was not edited since
     uint32_t V = 0;
     uint64_t E = 0;
     // Adjacency arrays: lists start at off[v] with room for capv[v] entries;

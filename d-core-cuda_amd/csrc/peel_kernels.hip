@@ -46,8 +46,11 @@ constexpr int W = 64;
 constexpr int kBlk = 512;
 constexpr int kUnroll = 4;
 constexpr int kSeedUnroll = 4;
-// core-layout ranks whose decrements a workgroup combines in LDS (per side)
+// core-layout ranks whose decrements a workgroup combines in LDS (per side);
+// in the original-id layout, slots of the hashed table (keys, then counts)
 constexpr uint32_t kCombRanks = 4096;
+constexpr uint32_t kCombLog2 = 12;
+constexpr uint32_t kCombEmpty = 0xffffffffu;
 // edges in flight per thread in the LDS-resident pass (its atomics are
 // cheap, so the list reads' latency is what remains to hide)
 constexpr int kLdsUnroll = 8;
@@ -116,6 +119,50 @@ __device__ __forceinline__ void wave_sum_to(uint32_t x, unsigned long long* dst)
 #pragma unroll
     for (int d = W / 2; d > 0; d >>= 1) y += __shfl_xor(y, d, W);
     if (lane_id() == 0 && y) consume((uint32_t)atomicAdd(dst, y));
+}
+
+// wave_append into arr[0, cap): appends beyond cap are counted but dropped.
+template <bool C>
+__device__ __forceinline__ void wave_append_capped(bool pred, uint32_t val, uint32_t* arr,
+                                                   uint32_t* ctr, uint32_t cap) {
+    const unsigned long long m = __ballot(pred);
+    if (m == 0ull) return;
+    const int leader = __ffsll(m) - 1;
+    uint32_t base = 0;
+    if (lane_id() == leader) base = atomicAdd(ctr, (uint32_t)__popcll(m));
+    base = __shfl(base, leader, W);
+    const uint32_t idx = base + __popcll(m & ((1ull << lane_id()) - 1ull));
+    if (pred && idx < cap) st<C>(&arr[idx], val);
+}
+
+// Frontier vertices whose lists hold at least kLongList edges are kept in a
+// list of their own (fl, fl_cap entries; slot word 8 counts them), so that
+// every member of a team can take a slice of each such list instead of one
+// member walking it alone.  The rest, and long ones beyond fl_cap, go to G.
+// Both count their list edges into slot words 4..5.
+constexpr uint32_t kLongList = 4096;
+template <bool C>
+__device__ __forceinline__ void append_frontier(bool pred, uint32_t v, uint32_t len, uint32_t* G,
+                                                uint32_t* fl, uint32_t fl_cap, uint32_t* slot) {
+    const bool lng = pred && len >= kLongList;
+    bool placed = false;
+    const unsigned long long m = __ballot(lng);
+    if (m) {
+        const int leader = __ffsll(m) - 1;
+        uint32_t base = 0;
+        if (lane_id() == leader) base = atomicAdd(&slot[8], (uint32_t)__popcll(m));
+        base = __shfl(base, leader, W);
+        if (lng) {
+            const uint32_t idx = base + __popcll(m & ((1ull << lane_id()) - 1ull));
+            if (idx < fl_cap) {
+                st<C>(&fl[idx], v);
+                placed = true;
+            }
+        }
+        wave_sum_to(placed ? len : 0u, reinterpret_cast<unsigned long long*>(slot + 4));
+    }
+    wave_append_len<C>(pred && !placed, v, len, G, &slot[0],
+                       reinterpret_cast<unsigned long long*>(slot + 4));
 }
 
 // Wavefront minimum, then one atomicMin per wavefront (performed before
@@ -204,11 +251,12 @@ __device__ __forceinline__ uint32_t lower_bound(const uint32_t* a, uint32_t n, u
 // grid_sync.hpp).  Words: [0] frontier appends, [1] alive-list appends,
 // [2] smallest key degree seen above the level, [3] the leader's phase,
 // [4..5] list edges of the new frontier, [6..7] list edges of the alive
-// vertices (first level only).
-// Called by threads 0..7 of one wavefront: each resets one word, so the
-// resets are one round trip instead of a chain of seven.
+// vertices (first level only), [8] frontier vertices with long lists,
+// [9] alive vertices with long lists (first level only).
+// Called by threads 0..9 of one wavefront: each resets one word, so the
+// resets are one round trip instead of a chain.
 __device__ __forceinline__ void slot_reset(uint32_t* s, uint32_t lane) {
-    if (lane < 8 && lane != 3) coherent_write(s + lane, lane == 2 ? (uint32_t)UINT_MAX : 0u);
+    if (lane < 10 && lane != 3) coherent_write(s + lane, lane == 2 ? (uint32_t)UINT_MAX : 0u);
 }
 
 __device__ __forceinline__ unsigned long long* slot_u64(uint32_t* s, int w) {
@@ -216,15 +264,15 @@ __device__ __forceinline__ unsigned long long* slot_u64(uint32_t* s, int w) {
 }
 
 struct SlotView {
-    uint32_t nF, nA, mn;
+    uint32_t nF, nA, mn, nL, nLA;
     uint64_t fedges, aedges;
 };
 
 // One coherent read of a slot per workgroup, shared through LDS; every
 // thread must call it (contains workgroup barriers).
 __device__ __forceinline__ SlotView slot_read(uint32_t* s, uint32_t* lds) {
-    // seven lanes read one word each (one round trip)
-    if (threadIdx.x < 7) {
+    // nine lanes read one word each (one round trip)
+    if (threadIdx.x < 9) {
         const uint32_t w = threadIdx.x < 3 ? threadIdx.x : threadIdx.x + 1;
         lds[threadIdx.x] = coherent_read(s + w);
     }
@@ -235,6 +283,8 @@ __device__ __forceinline__ SlotView slot_read(uint32_t* s, uint32_t* lds) {
     v.mn = lds[2];
     v.fedges = (uint64_t)lds[3] | ((uint64_t)lds[4] << 32);
     v.aedges = (uint64_t)lds[5] | ((uint64_t)lds[6] << 32);
+    v.nL = lds[7];
+    v.nLA = lds[8];
     __syncthreads();
     return v;
 }
@@ -268,7 +318,11 @@ __device__ __forceinline__ void list_lens(const PassSpec& P, const PassScratch& 
 // (team_resync) before the scratch is reused.  Logic errors
 // (counters that do not add up) are reported into *err (single workgroup)
 // or the team's error word instead of looping.
-template <int BLK, bool MULTI>
+// Decrement combiners of a push phase (LDS, per workgroup): none; by rank
+// (the first kCombRanks core ranks, core layout only); hashed (any ids).
+enum : int { kCombNone = 0, kCombByRank = 1, kCombHashed = 2 };
+
+template <int BLK, bool MULTI, int COMB>
 __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slots,
                          TileLds<BLK>& T, int32_t* lmax_out, GridSync* gs, uint32_t* err,
                          uint32_t shrink_at, uint32_t* comb) {
@@ -338,7 +392,8 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
         t_prev = now;
     };
 
-    __shared__ uint32_t s_rd[8];
+    __shared__ uint32_t s_rd[12];
+    __shared__ uint32_t s_part[2];
     if (tw == 0) {
         slot_reset(slots, tid);
         slot_reset(slots + kSlotStride, tid);
@@ -362,10 +417,19 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
         lval[i] = -1;
         S.st[i] = ~0u;
     }
+    // long alive list (first quarter) and its partial counts (next two)
+    const uint32_t la_cap = nm / 4;
+    uint32_t* const LA = S.lp;
+    uint32_t* const PK = S.lp + la_cap;
+    uint32_t* const PC = S.lp + 2 * la_cap;
+    for (uint32_t i = tw * BLK + tid; i < 2 * la_cap; i += tn * BLK) PK[i] = 0;
     if (!barrier(true)) return;
 
     const uint32_t* A = nullptr;  // first level: the members in id order
     uint32_t *B = S.a0, *Bn = S.a1, *F = S.f0, *G = S.f1;
+    // long-list frontier vertices: FL (current) and GL (next), nL of them
+    const uint32_t fl_cap = nm / 2;
+    uint32_t *FL = S.fl, *GL = S.fl + fl_cap, nL = 0;
     nA = nm;
     const int32_t k = P.k;
     // Without membership restriction the constraint can fail before any
@@ -375,6 +439,7 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
     // member of the team derives the same values from the slots, so all of
     // them choose the same kind of cascade phase.
     uint64_t pa = 0, fe = 0;
+    uint32_t nLA = 0;  // long alive list length (0 when it overflowed)
     while (removed < nm) {
         // Each removal needs at most two level iterations (a level number
         // taken from a stale minimum seeds nothing, the next one seeds the
@@ -439,7 +504,9 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
                     poison(S, v[u], s);
                 }
                 if (keep) mn = min(mn, (uint32_t)d[u]);
-                wave_append_len<MULTI>(seed, v[u], len, F, &cur[0], slot_u64(cur, 4));
+                // kept vertices with long lists: their pulls are sliced
+                if (first) wave_append_capped<MULTI>(keep && len >= kLongList, v[u], LA, &cur[9], la_cap);
+                append_frontier<MULTI>(seed, v[u], len, F, FL, fl_cap, cur);
                 wave_append<MULTI>(keep, v[u], B, &cur[1]);
                 if (first) wave_sum_to(keep ? len : 0u, slot_u64(cur, 6));
             }
@@ -448,12 +515,14 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
         if (!barrier()) return;
         const SlotView sv = slot_read(cur, s_rd);
         uint32_t nF = sv.nF;
+        if (first) nLA = sv.nLA <= la_cap ? sv.nLA : 0u;  // overflow: no slicing
+        nL = min(sv.nL, fl_cap);
         nA = sv.nA;
         uint32_t mnall = sv.mn;
         fe = sv.fedges;
         pa = first ? sv.aedges : pa - fe;
 
-        trace(s, true, false, L, nF, nA);
+        trace(s, true, false, L, nF + nL, nA);
         A = B;
         {
             uint32_t* t = B;
@@ -461,14 +530,93 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
             Bn = t;
         }
         ++s;
-        removed += nF;
+        removed += nF + nL;
 
         // ---- cascade inside level L until no vertex crosses a threshold
-        while (nF > 0) {
+        while (nF + nL > 0) {
             uint32_t* nxt = slots + kSlotStride * (s % 3);
             if (tw == 0) slot_reset(slots + kSlotStride * ((s + 1) % 3), tid);
             uint32_t mn2 = UINT_MAX;
-            const bool pull = P.pull_ratio != 0 && pa <= (uint64_t)P.pull_ratio * fe;
+            // kUnroll edges held by this thread: x[u] loses a neighbour on
+            // the constraint side (ks[u]) or the key side; ok[u] = a real edge
+            auto visit = [&](uint32_t(&x)[kUnroll], bool(&ks)[kUnroll], bool(&ok)[kUnroll]) {
+                bool claim[kUnroll];
+                // Half of all edge visits reach a neighbour that
+                // left earlier (every edge is visited from both
+                // ends); a plain load of its level skips those
+                // atomics.  A stale -1 only costs the atomic, which
+                // then finds a poisoned counter.
+#pragma unroll
+                for (int u = 0; u < kUnroll; ++u) ok[u] = ok[u] && lval[x[u]] == -1;
+                if (COMB == kCombByRank) {
+                    // the first kCombRanks ranks are the densest
+                    // core's vertices, which most frontier edges hit:
+                    // their decrements are summed in LDS and applied
+                    // once per workgroup and phase (flush below)
+#pragma unroll
+                    for (int u = 0; u < kUnroll; ++u)
+                        if (ok[u] && x[u] < kCombRanks) {
+                            atomicAdd(&comb[(ks[u] ? kCombRanks : 0u) + x[u]], 1u);
+                            ok[u] = false;
+                        }
+                } else if (COMB == kCombHashed) {
+                    // any layout: a hashed table of
+                    // (vertex, side) keys, first come first served; a hub
+                    // hit by thousands of frontier edges takes a slot early
+#pragma unroll
+                    for (int u = 0; u < kUnroll; ++u)
+                        if (ok[u]) {
+                            const uint32_t key = (x[u] << 1) | (ks[u] ? 1u : 0u);
+                            const uint32_t h = (key * 2654435761u) >> (32 - kCombLog2);
+                            uint32_t cur = comb[h];
+                            if (cur == kCombEmpty) {
+                                cur = atomicCAS(&comb[h], kCombEmpty, key);
+                                if (cur == kCombEmpty) cur = key;
+                            }
+                            if (cur == key) {
+                                atomicAdd(&comb[kCombRanks + h], 1u);
+                                ok[u] = false;
+                            }
+                        }
+                }
+                int32_t old[kUnroll];
+#pragma unroll
+                for (int u = 0; u < kUnroll; ++u) {
+                    // K-side edge: x loses a C-neighbour (constraint
+                    // degree); C-side edge: x loses a K-neighbour (key)
+                    old[u] = ok[u] ? atomicSub(ks[u] ? &S.cd[x[u]] : &S.kd[x[u]], 1) : 0;
+                }
+#pragma unroll
+                for (int u = 0; u < kUnroll; ++u) {
+                    claim[u] = false;
+                    if (ok[u]) {
+                        if (ks[u]) {
+                            claim[u] = k > 0 && old[u] == k;
+                        } else if (old[u] == L + 1) {
+                            claim[u] = true;
+                        } else if (old[u] - 1 > L) {
+                            mn2 = min(mn2, (uint32_t)(old[u] - 1));
+                        }
+                    }
+                    if (claim[u]) {
+                        claim[u] = atomicCAS(&lval[x[u]], -1, L) == -1;
+                        if (claim[u]) poison(S, x[u], s);
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < kUnroll; ++u) {
+                    uint32_t len = 0;
+                    if (claim[u]) {
+                        uint32_t a, b;
+                        list_lens(P, S, x[u], a, b);
+                        len = a + b;
+                    }
+                    append_frontier<MULTI>(claim[u], x[u], len, G, GL, fl_cap, nxt);
+                }
+            };
+            // a pull reads every alive vertex's lists and every entry of
+            // the level's alive list (most of which may have left by now)
+            const bool pull = P.pull_ratio != 0 && pa + nA <= (uint64_t)P.pull_ratio * fe;
             if (pull) {
                 // Pull: every alive vertex of the level's list counts its
                 // neighbours removed in the previous phase and updates its
@@ -486,6 +634,7 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
                     if (ld<MULTI>(&lval[x]) != -1) continue;  // removed earlier in this level
                     uint32_t kl, cl;
                     list_lens(P, S, x, kl, cl);
+                    if (nLA && kl + cl >= kLongList) continue;  // sliced below
                     const uint32_t* kn = P.K.nbr + P.K.off[x];
                     const uint32_t* cn = P.C.nbr + P.C.off[x];
                     uint32_t ck = 0, cc = 0;  // the same in every lane
@@ -523,7 +672,82 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
                             if (cc) store_agent(S.cd + x, nc);
                         }
                     }
-                    wave_append_len<MULTI>(claim, x, kl + cl, G, &nxt[0], slot_u64(nxt, 4));
+                    append_frontier<MULTI>(claim, x, kl + cl, G, GL, fl_cap, nxt);
+                }
+                if (nLA) {
+                    // long alive lists: every member counts the hits in its
+                    // slice of each and adds them to the list's partial
+                    // counts; after a team barrier one wavefront per list
+                    // applies them
+                    for (uint32_t li = 0; li < nLA; ++li) {
+                        const uint32_t x = ld<MULTI>(&LA[li]);
+                        if (ld<MULTI>(&lval[x]) != -1) continue;  // the same for the workgroup
+                        uint32_t kl, cl;
+                        list_lens(P, S, x, kl, cl);
+                        const uint32_t tot = kl + cl;
+                        const uint32_t lo = (uint32_t)((uint64_t)tot * tw / tn);
+                        const uint32_t hi = (uint32_t)((uint64_t)tot * (tw + 1) / tn);
+                        const uint32_t* kn = P.K.nbr + P.K.off[x];
+                        const uint32_t* cn = P.C.nbr + P.C.off[x];
+                        uint32_t ck = 0, cc = 0;
+                        for (uint32_t e = lo + tid; e < hi; e += BLK) {
+                            const bool isk = e < kl;
+                            const uint32_t y = isk ? kn[e] : cn[e - kl];
+                            if (ld<MULTI>(&S.st[y]) == sprev) {
+                                if (isk)
+                                    ++ck;
+                                else
+                                    ++cc;
+                            }
+                        }
+#pragma unroll
+                        for (int d = W / 2; d > 0; d >>= 1) {
+                            ck += (uint32_t)__shfl_xor(ck, d, W);
+                            cc += (uint32_t)__shfl_xor(cc, d, W);
+                        }
+                        if (tid == 0) s_part[0] = s_part[1] = 0;
+                        __syncthreads();
+                        if (lane == 0 && (ck | cc)) {
+                            atomicAdd(&s_part[0], ck);
+                            atomicAdd(&s_part[1], cc);
+                        }
+                        __syncthreads();
+                        if (tid == 0) {
+                            if (s_part[0]) consume(atomicAdd(&PK[li], s_part[0]));
+                            if (s_part[1]) consume(atomicAdd(&PC[li], s_part[1]));
+                        }
+                        __syncthreads();
+                    }
+                    if (!barrier()) return;
+                    for (uint32_t li = tw * nw + tid / W; li < nLA; li += tn * nw) {
+                        const uint32_t x = ld<MULTI>(&LA[li]);
+                        bool claim = false;
+                        uint32_t len = 0;
+                        if (ld<MULTI>(&lval[x]) == -1 && lane == 0) {
+                            // read and clear the partial counts
+                            const uint32_t ck = atomicAnd(&PK[li], 0u);
+                            const uint32_t cc = atomicAnd(&PC[li], 0u);
+                            if (ck | cc) {
+                                const int32_t nk = ld<MULTI>(&S.kd[x]) - (int32_t)ck;
+                                const int32_t nc = ld<MULTI>(&S.cd[x]) - (int32_t)cc;
+                                claim = nk <= L || (k > 0 && nc < k);
+                                if (claim) {
+                                    st<MULTI>(&lval[x], L);
+                                    poison(S, x, s);
+                                    uint32_t kl, cl;
+                                    list_lens(P, S, x, kl, cl);
+                                    len = kl + cl;
+                                } else {
+                                    if (ck) {
+                                        store_agent(S.kd + x, nk);
+                                        mn2 = min(mn2, (uint32_t)nk);
+                                    }
+                                    if (cc) store_agent(S.cd + x, nc);
+                                }
+                            }
+                        }
+                        append_frontier<MULTI>(claim, x, len, G, GL, fl_cap, nxt);
+                    }
                 }
             } else {
                 // Push: frontier items per workgroup -- a team splits even
@@ -552,7 +776,7 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
                     // loads and atomics overlap instead of forming one chain.
                     for (uint32_t eb = 0; eb < total; eb += kUnroll * BLK) {
                         uint32_t x[kUnroll];
-                        bool ks[kUnroll], ok[kUnroll], claim[kUnroll];
+                        bool ks[kUnroll], ok[kUnroll];
 #pragma unroll
                         for (int u = 0; u < kUnroll; ++u) {
                             const uint32_t e = eb + u * BLK + tid;
@@ -567,75 +791,60 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
                                              : P.C.nbr[T.coff[it] + (t - T.klen[it])];
                             }
                         }
-                        // Half of all edge visits reach a neighbour that
-                        // left earlier (every edge is visited from both
-                        // ends); a plain load of its level skips those
-                        // atomics.  A stale -1 only costs the atomic, which
-                        // then finds a poisoned counter.
-#pragma unroll
-                        for (int u = 0; u < kUnroll; ++u) ok[u] = ok[u] && lval[x[u]] == -1;
-                        if (comb) {
-                            // the first kCombRanks ranks are the densest
-                            // core's vertices, which most frontier edges hit:
-                            // their decrements are summed in LDS and applied
-                            // once per workgroup and phase (flush below)
-#pragma unroll
-                            for (int u = 0; u < kUnroll; ++u)
-                                if (ok[u] && x[u] < kCombRanks) {
-                                    atomicAdd(&comb[(ks[u] ? kCombRanks : 0u) + x[u]], 1u);
-                                    ok[u] = false;
-                                }
-                        }
-                        int32_t old[kUnroll];
+                        visit(x, ks, ok);
+                    }
+                }
+                // long lists: every member walks its slice of each of them
+                for (uint32_t li = 0; li < nL; ++li) {
+                    const uint32_t v = ld<MULTI>(&FL[li]);
+                    uint32_t kl, cl;
+                    list_lens(P, S, v, kl, cl);
+                    const uint32_t tot = kl + cl;
+                    const uint32_t lo = (uint32_t)((uint64_t)tot * tw / tn);
+                    const uint32_t hi = (uint32_t)((uint64_t)tot * (tw + 1) / tn);
+                    const uint32_t* kn = P.K.nbr + P.K.off[v];
+                    const uint32_t* cn = P.C.nbr + P.C.off[v];
+                    for (uint32_t eb = lo; eb < hi; eb += kUnroll * BLK) {
+                        uint32_t x[kUnroll];
+                        bool ks[kUnroll], ok[kUnroll];
 #pragma unroll
                         for (int u = 0; u < kUnroll; ++u) {
-                            // K-side edge: x loses a C-neighbour (constraint
-                            // degree); C-side edge: x loses a K-neighbour (key)
-                            old[u] = ok[u] ? atomicSub(ks[u] ? &S.cd[x[u]] : &S.kd[x[u]], 1) : 0;
+                            const uint32_t e = eb + u * BLK + tid;
+                            ok[u] = e < hi;
+                            ks[u] = e < kl;
+                            x[u] = ok[u] ? (ks[u] ? kn[e] : cn[e - kl]) : 0u;
                         }
-#pragma unroll
-                        for (int u = 0; u < kUnroll; ++u) {
-                            claim[u] = false;
-                            if (ok[u]) {
-                                if (ks[u]) {
-                                    claim[u] = k > 0 && old[u] == k;
-                                } else if (old[u] == L + 1) {
-                                    claim[u] = true;
-                                } else if (old[u] - 1 > L) {
-                                    mn2 = min(mn2, (uint32_t)(old[u] - 1));
-                                }
-                            }
-                            if (claim[u]) {
-                                claim[u] = atomicCAS(&lval[x[u]], -1, L) == -1;
-                                if (claim[u]) poison(S, x[u], s);
-                            }
-                        }
-#pragma unroll
-                        for (int u = 0; u < kUnroll; ++u) {
-                            uint32_t len = 0;
-                            if (claim[u]) {
-                                uint32_t a, b;
-                                list_lens(P, S, x[u], a, b);
-                                len = a + b;
-                            }
-                            wave_append_len<MULTI>(claim[u], x[u], len, G, &nxt[0],
-                                                   slot_u64(nxt, 4));
-                        }
+                        visit(x, ks, ok);
                     }
                 }
             }
-            if (comb && !pull) {
+            if (COMB != kCombNone && !pull) {
                 // apply the combined decrements: one atomic per touched
                 // counter; the atomic whose subtraction takes the counter
                 // across its threshold claims the vertex
                 __syncthreads();
                 for (uint32_t j = tid; j < 2 * kCombRanks; j += BLK) {
-                    const uint32_t c = comb[j];
-                    const bool cside = j >= kCombRanks;
-                    const uint32_t x = cside ? j - kCombRanks : j;
+                    uint32_t c, x;
+                    bool cside;
+                    if (COMB == kCombByRank) {
+                        c = comb[j];
+                        cside = j >= kCombRanks;
+                        x = cside ? j - kCombRanks : j;
+                        if (c) comb[j] = 0;
+                    } else {
+                        // the first half of the table holds keys, the
+                        // second their counts
+                        const uint32_t key = j < kCombRanks ? comb[j] : kCombEmpty;
+                        c = key != kCombEmpty ? comb[kCombRanks + j] : 0u;
+                        cside = key & 1u;
+                        x = key >> 1;
+                        if (key != kCombEmpty) {
+                            comb[j] = kCombEmpty;
+                            comb[kCombRanks + j] = 0;
+                        }
+                    }
                     bool claim = false;
                     if (c) {
-                        comb[j] = 0;
                         const int32_t old = atomicSub(cside ? &S.cd[x] : &S.kd[x], (int32_t)c);
                         const int32_t nw = old - (int32_t)c;
                         if (cside) {
@@ -656,25 +865,29 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
                         list_lens(P, S, x, a, b);
                         len = a + b;
                     }
-                    wave_append_len<MULTI>(claim, x, len, G, &nxt[0], slot_u64(nxt, 4));
+                    append_frontier<MULTI>(claim, x, len, G, GL, fl_cap, nxt);
                 }
             }
             wave_min_to(mn2, &nxt[2]);
             if (!barrier()) return;
             const SlotView sv2 = slot_read(nxt, s_rd);
             nF = sv2.nF;
+            nL = min(sv2.nL, fl_cap);
             mnall = min(mnall, sv2.mn);
             fe = sv2.fedges;
             pa -= fe;
 
-            trace(s, false, pull, L, nF, nA);
+            trace(s, false, pull, L, nF + nL, nA);
             {
                 uint32_t* t = F;
                 F = G;
                 G = t;
+                t = FL;
+                FL = GL;
+                GL = t;
             }
             ++s;
-            removed += nF;
+            removed += nF + nL;
         }
         if (removed < nm) {
             // alive vertices exist, so some degree was recorded
@@ -1022,18 +1235,20 @@ __global__ __launch_bounds__(kLdsBlk) void k_rows_lds(PassSpec base, const uint3
     }
 }
 
-__global__ __launch_bounds__(kBlk) void k_pass_grid(PassSpec P, PassScratch S, uint32_t* slots,
+__global__ __launch_bounds__(kBlk, 4) void k_pass_grid(PassSpec P, PassScratch S, uint32_t* slots,
                                                     int32_t* lmax_out, GridSync gs,
                                                     uint32_t shrink_at) {
     __shared__ TileLds<kBlk> T;
     gs.rank = blockIdx.x;
     gs.epoch = 0;
-    run_pass<kBlk, true>(P, S, slots, T, lmax_out, &gs, nullptr, shrink_at, nullptr);
+    run_pass<kBlk, true, kCombNone>(P, S, slots, T, lmax_out, &gs, nullptr, shrink_at, nullptr);
 }
 
 // Persistent rows kernel: block b belongs to team block_team[b]; a team
 // runs its passes one after another, re-synchronising between them.
-__global__ __launch_bounds__(kBlk) void k_rows_teams(PassSpec base, const uint32_t* vk,
+// Two workgroups of kBlk threads must be resident per CU (the launch packs
+// two per CU and a team's members wait for each other): 4 waves per SIMD.
+__global__ __launch_bounds__(kBlk, 4) void k_rows_teams(PassSpec base, const uint32_t* vk,
                                                      int32_t* rows, int32_t* lmax_rows,
                                                      const TeamDesc* teams,
                                                      const uint32_t* block_team,
@@ -1042,7 +1257,6 @@ __global__ __launch_bounds__(kBlk) void k_rows_teams(PassSpec base, const uint32
                                                      uint64_t timeout_ticks, uint32_t* err,
                                                      uint32_t shrink_at) {
     __shared__ TileLds<kBlk> T;
-    __shared__ __align__(16) uint32_t lslots[kSlotWords];
     __shared__ uint32_t comb[2 * kCombRanks];
     const uint32_t team = block_team[blockIdx.x];
     if (team == ~0u) return;  // an idle block (its XCD had no team for it)
@@ -1065,15 +1279,13 @@ __global__ __launch_bounds__(kBlk) void k_rows_teams(PassSpec base, const uint32
         P.out_row = rows + (size_t)k * base.V;
         P.stats = base.stats ? base.stats + (size_t)kStatWords * k : nullptr;
         if (k != base.trace_k) P.trace = nullptr;
-        if (td.size == 1) {
-            run_pass<kBlk, false>(P, S, lslots, T, lmax_rows + k, nullptr, err, 0, comb);
-            __syncthreads();
-        } else {
-            gs.nblocks = td.size;  // the pass may have narrowed the team
-            run_pass<kBlk, true>(P, S, slot_pool + td.slot_off, T, lmax_rows + k, &gs, err,
-                                 shrink_at, comb);
-            if (!team_resync(gs, p - td.pass_begin + 1)) return;
-        }
+        // one code path for every team size (a team of one runs the same
+        // pass with workgroup barriers), which keeps the kernel within the
+        // registers of two resident workgroups per CU
+        gs.nblocks = td.size;  // the pass may have narrowed the team
+        run_pass<kBlk, true, kCombByRank>(P, S, slot_pool + td.slot_off, T, lmax_rows + k, &gs,
+                                          err, shrink_at, comb);
+        if (!team_resync(gs, p - td.pass_begin + 1)) return;
     }
 }
 
@@ -1157,12 +1369,24 @@ __global__ void k_core_edges(const int32_t* kmaxes, Csr out, uint32_t V, int32_t
     for (uint32_t v = wave; v < V; v += nw) {
         const int32_t kv = kmaxes[v];
         const uint32_t o = out.off[v], n = (uint32_t)out.deg[v];
-        for (uint32_t j = lane_id(); j < n; j += W) {
-            const int32_t m = min(kv, kmaxes[out.nbr[o + j]]);
-            if (lds)
-                atomicAdd(&h[m], 1ull);
-            else
-                atomicAdd(&hist[m], 1ull);
+        for (uint32_t jb = 0; jb < n; jb += W) {
+            const uint32_t j = jb + lane_id();
+            const int32_t m = j < n ? min(kv, kmaxes[out.nbr[o + j]]) : -1;
+            // most of a list's edges end at a vertex of kmax >= kv, i.e. in
+            // the same bin: one add per wavefront for those
+            const unsigned long long same = __ballot(m == kv);
+            if (same && lane_id() == __ffsll(same) - 1) {
+                if (lds)
+                    atomicAdd(&h[kv], (unsigned long long)__popcll(same));
+                else
+                    atomicAdd(&hist[kv], (unsigned long long)__popcll(same));
+            }
+            if (m >= 0 && m != kv) {
+                if (lds)
+                    atomicAdd(&h[m], 1ull);
+                else
+                    atomicAdd(&hist[m], 1ull);
+            }
         }
     }
     __syncthreads();
@@ -1186,6 +1410,13 @@ __global__ void k_max_i32(const int32_t* p, size_t n, int32_t* out) {
 }
 
 }  // namespace
+
+int rows_teams_blocks_per_cu() {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_rows_teams, kBlk, 0) != hipSuccess)
+        return 1;
+    return n > 0 ? n : 1;
+}
 
 int pass_grid_blocks_per_cu() {
     int n = 0;
