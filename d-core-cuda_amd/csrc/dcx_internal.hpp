@@ -147,8 +147,12 @@ struct TeamDesc {
 // its members hand each other between phases stays within one L2.
 constexpr uint32_t kXcds = 8;
 
-// Default pull_ratio of the peel passes (PassSpec::pull_ratio).
-constexpr uint32_t kDefaultPullRatio = 4;
+// Default pull_ratio of the peel passes (PassSpec::pull_ratio): push only.
+// With removed neighbours skipped, dense-core decrements combined in LDS and
+// long lists sliced across the team, pushes beat pulls on every graph
+// measured (R-MAT 20: 48.3 ms push only, 49.9 ms at ratio 4; Chung-Lu 2^22:
+// 127 ms against 151 ms); pulls stay available as an exact alternative.
+constexpr uint32_t kDefaultPullRatio = 0;
 
 // ---- cost-model MLP (mlp_kernels.hip) ----------------------------------
 constexpr int kMlpIn = 8, kMlpHidden = 32;

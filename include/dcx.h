@@ -152,9 +152,9 @@ typedef struct dcx_config {
     uint64_t scratch_budget; /* max bytes of pass scratch (0 = no limit) */
     /* A cascade phase of a peel pass pulls (each alive vertex counts its
      * neighbours in the frontier) instead of pushing decrements from the
-     * frontier when the alive vertices' lists hold at most pull_ratio times
-     * the frontier's edges: 0 = library default (4), -1 = always push.
-     * Results are the same either way. */
+     * frontier when the alive vertices' lists and the alive list hold at
+     * most pull_ratio times the frontier's edges: 0 = library default
+     * (always push), -1 = always push.  Results are the same either way. */
     int32_t pull_ratio;
     /* Batched rows whose (k,0)-core has at most lds_members members run on
      * one workgroup with their degree counters in LDS: 0 = as many as the
