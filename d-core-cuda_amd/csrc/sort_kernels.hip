@@ -1,13 +1,15 @@
 // Device primitives for building the core-ordered graph: exclusive scan of
 // uint32 arrays and a stable LSD radix sort of uint64 keys.
 //
-// Radix sort: 4-bit digits.  Per pass, every block owns a tile of
+// Radix sort: 8-bit digits.  Per pass, every block owns a tile of
 // kTile = kSortBlk * kPerThread consecutive keys: (1) it histograms the
-// digit of its tile, (2) one block scans the digit-major table of tile
-// histograms, (3) every block re-reads its tile and writes each key to
-// table[digit][tile] + (rank of the key among equal digits before it in
-// the tile) -- ranks come from a per-thread count table scanned in LDS,
-// which keeps the sort stable.
+// digit of its tile, (2) the digit-major table of tile histograms is
+// scanned, (3) every block re-reads its tile in kPerThread rounds of
+// kSortBlk consecutive keys and writes each key to table[digit][tile] +
+// (keys of that digit in earlier rounds) + (in earlier wavefronts of the
+// round) + (in earlier lanes of its wavefront: a popcount of the lanes
+// whose digit matches, found with one ballot per digit bit).  Rounds,
+// wavefronts and lanes follow the input order, so the sort is stable.
 #include <hip/hip_runtime.h>
 
 #include "dcx_internal.hpp"
@@ -18,38 +20,13 @@ namespace {
 constexpr int kSortBlk = 256;
 constexpr int kPerThread = 8;
 constexpr int kTile = kSortBlk * kPerThread;
-constexpr int kDigits = 16;
+constexpr int kDigitBits = 8;
+constexpr int kDigits = 1 << kDigitBits;
+constexpr int kSortWaves = kSortBlk / 64;
+static_assert(kDigits == kSortBlk, "one thread per digit in the scatter's prefix step");
 constexpr int W = 64;
 
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (W - 1)); }
-
-// In-LDS exclusive scan of n <= 4096 uint32 values by one 256-thread block.
-__device__ void lds_excl_scan(uint32_t* a, int n) {
-    __shared__ uint32_t part[kSortBlk];
-    const int t = threadIdx.x;
-    const int per = (n + kSortBlk - 1) / kSortBlk;
-    const int lo = t * per, hi = min(n, lo + per);
-    uint32_t s = 0;
-    for (int i = lo; i < hi; ++i) s += a[i];
-    part[t] = s;
-    __syncthreads();
-    if (t == 0) {
-        uint32_t acc = 0;
-        for (int i = 0; i < kSortBlk; ++i) {
-            const uint32_t x = part[i];
-            part[i] = acc;
-            acc += x;
-        }
-    }
-    __syncthreads();
-    uint32_t acc = part[t];
-    for (int i = lo; i < hi; ++i) {
-        const uint32_t x = a[i];
-        a[i] = acc;
-        acc += x;
-    }
-    __syncthreads();
-}
 
 __global__ __launch_bounds__(kSortBlk) void k_radix_hist(const uint64_t* keys, uint64_t n,
                                                           int shift, uint32_t ntiles,
@@ -97,33 +74,46 @@ __global__ __launch_bounds__(kSortBlk) void k_radix_scan(uint32_t* table, uint64
 __global__ __launch_bounds__(kSortBlk) void k_radix_scatter(const uint64_t* in, uint64_t* out,
                                                              uint64_t n, int shift, uint32_t ntiles,
                                                              const uint32_t* table) {
-    // cnt[d * kSortBlk + t]: keys of digit d held by thread t (thread t owns
-    // the tile's items t*kPerThread .. +kPerThread, so thread order is key
-    // order and the scan below gives stable ranks)
-    __shared__ uint32_t cnt[kDigits * kSortBlk];
-    const int t = threadIdx.x;
-    for (int d = 0; d < kDigits; ++d) cnt[d * kSortBlk + t] = 0;
-    const uint64_t base = (uint64_t)blockIdx.x * kTile + (uint64_t)t * kPerThread;
-    uint64_t k[kPerThread];
+    __shared__ uint32_t run[kDigits];              // keys of each digit placed so far
+    __shared__ uint32_t wc[kSortWaves][kDigits];   // this round: per wavefront, then prefix
+    __shared__ uint32_t gbase[kDigits];            // the tile's start in each digit's bucket
+    const int t = threadIdx.x, w = t / W, ln = lane_id();
+    run[t] = 0;
+    gbase[t] = table[(uint64_t)t * ntiles + blockIdx.x];
+    const uint64_t tbase = (uint64_t)blockIdx.x * kTile;
     for (int j = 0; j < kPerThread; ++j) {
-        const uint64_t i = base + j;
-        k[j] = i < n ? in[i] : 0;
-        if (i < n) cnt[((k[j] >> shift) & (kDigits - 1)) * kSortBlk + t]++;
-    }
-    __syncthreads();
-    lds_excl_scan(cnt, kDigits * kSortBlk);
-    // cnt[d][t] = keys of the tile with a smaller digit + keys of digit d in
-    // threads before t; the digit's own start is cnt[d][0]
-    __shared__ uint32_t dstart[kDigits];
-    if (t < kDigits) dstart[t] = cnt[t * kSortBlk];
-    __syncthreads();
-    for (int j = 0; j < kPerThread; ++j) {
-        const uint64_t i = base + j;
-        if (i < n) {
-            const int d = (int)((k[j] >> shift) & (kDigits - 1));
-            const uint32_t r = cnt[d * kSortBlk + t]++ - dstart[d];
-            out[(uint64_t)table[(uint64_t)d * ntiles + blockIdx.x] + r] = k[j];
+        const uint64_t i = tbase + (uint64_t)j * kSortBlk + t;
+        const bool has = i < n;
+        const uint64_t key = has ? in[i] : 0ull;
+        const uint32_t d = (uint32_t)((key >> shift) & (kDigits - 1));
+        // lanes of this wavefront holding a key of the same digit
+        unsigned long long peers = __ballot(has);
+#pragma unroll
+        for (int b = 0; b < kDigitBits; ++b) {
+            const bool bit = (d >> b) & 1u;
+            const unsigned long long bb = __ballot(bit);
+            peers &= bit ? bb : ~bb;
         }
+        const uint32_t rank = (uint32_t)__popcll(peers & ((1ull << ln) - 1ull));
+#pragma unroll
+        for (int v = 0; v < kSortWaves; ++v) wc[v][t] = 0;
+        __syncthreads();
+        if (has && rank + 1 == (uint32_t)__popcll(peers)) wc[w][d] = (uint32_t)__popcll(peers);
+        __syncthreads();
+        {
+            // thread t: digit t, prefix over the wavefronts
+            uint32_t acc = run[t];
+#pragma unroll
+            for (int v = 0; v < kSortWaves; ++v) {
+                const uint32_t c = wc[v][t];
+                wc[v][t] = acc;
+                acc += c;
+            }
+            run[t] = acc;
+        }
+        __syncthreads();
+        if (has) out[(uint64_t)gbase[d] + wc[w][d] + rank] = key;
+        __syncthreads();  // wc is cleared at the start of the next round
     }
 }
 
@@ -211,7 +201,7 @@ hipError_t launch_radix_sort_u64(uint64_t* keys, uint64_t* alt, uint64_t n, int 
     uint64_t* b = alt;
     const uint64_t m = (uint64_t)kDigits * nt;
     uint32_t* sums = table + m + 1;
-    for (int shift = 0; shift < bits; shift += 4) {
+    for (int shift = 0; shift < bits; shift += kDigitBits) {
         hipLaunchKernelGGL(k_radix_hist, dim3(nt), dim3(kSortBlk), 0, st, a, n, shift, nt, table);
         // the digit table scanned by tiles (one block per tile, in place)
         hipError_t e = launch_excl_scan_u32(table, table, m, sums, st);
