@@ -139,11 +139,13 @@ __device__ __forceinline__ void wave_append_capped(bool pred, uint32_t val, uint
 // list of their own (fl, fl_cap entries; slot word 8 counts them), so that
 // every member of a team can take a slice of each such list instead of one
 // member walking it alone.  The rest, and long ones beyond fl_cap, go to G.
-// Both count their list edges into slot words 4..5.
+// Both count their list edges into *etot (the workgroup's LDS total, added
+// to slot words 4..5 once per phase).
 constexpr uint32_t kLongList = 4096;
 template <bool C>
 __device__ __forceinline__ void append_frontier(bool pred, uint32_t v, uint32_t len, uint32_t* G,
-                                                uint32_t* fl, uint32_t fl_cap, uint32_t* slot) {
+                                                uint32_t* fl, uint32_t fl_cap, uint32_t* slot,
+                                                unsigned long long* etot) {
     const bool lng = pred && len >= kLongList;
     bool placed = false;
     const unsigned long long m = __ballot(lng);
@@ -159,10 +161,9 @@ __device__ __forceinline__ void append_frontier(bool pred, uint32_t v, uint32_t 
                 placed = true;
             }
         }
-        wave_sum_to(placed ? len : 0u, reinterpret_cast<unsigned long long*>(slot + 4));
+        wave_sum_to(placed ? len : 0u, etot);
     }
-    wave_append_len<C>(pred && !placed, v, len, G, &slot[0],
-                       reinterpret_cast<unsigned long long*>(slot + 4));
+    wave_append_len<C>(pred && !placed, v, len, G, &slot[0], etot);
 }
 
 // Wavefront minimum, then one atomicMin per wavefront (performed before
@@ -394,6 +395,19 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
 
     __shared__ uint32_t s_rd[12];
     __shared__ uint32_t s_part[2];
+    // list-edge totals of the phase, summed in LDS and added to the slot
+    // once per workgroup (per-wavefront global atomics on two words
+    // serialised big phases)
+    __shared__ unsigned long long s_etot, s_atot;
+    if (tid == 0) s_etot = s_atot = 0;
+    auto flush_totals = [&](uint32_t* slot) {
+        __syncthreads();
+        if (tid == 0) {
+            if (s_etot) consume((uint32_t)atomicAdd(slot_u64(slot, 4), s_etot));
+            if (s_atot) consume((uint32_t)atomicAdd(slot_u64(slot, 6), s_atot));
+            s_etot = s_atot = 0;
+        }
+    };
     if (tw == 0) {
         slot_reset(slots, tid);
         slot_reset(slots + kSlotStride, tid);
@@ -506,12 +520,13 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
                 if (keep) mn = min(mn, (uint32_t)d[u]);
                 // kept vertices with long lists: their pulls are sliced
                 if (first) wave_append_capped<MULTI>(keep && len >= kLongList, v[u], LA, &cur[9], la_cap);
-                append_frontier<MULTI>(seed, v[u], len, F, FL, fl_cap, cur);
+                append_frontier<MULTI>(seed, v[u], len, F, FL, fl_cap, cur, &s_etot);
                 wave_append<MULTI>(keep, v[u], B, &cur[1]);
-                if (first) wave_sum_to(keep ? len : 0u, slot_u64(cur, 6));
+                if (first) wave_sum_to(keep ? len : 0u, &s_atot);
             }
         }
         wave_min_to(mn, &cur[2]);
+        flush_totals(cur);
         if (!barrier()) return;
         const SlotView sv = slot_read(cur, s_rd);
         uint32_t nF = sv.nF;
@@ -611,7 +626,7 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
                         list_lens(P, S, x[u], a, b);
                         len = a + b;
                     }
-                    append_frontier<MULTI>(claim[u], x[u], len, G, GL, fl_cap, nxt);
+                    append_frontier<MULTI>(claim[u], x[u], len, G, GL, fl_cap, nxt, &s_etot);
                 }
             };
             // a pull reads every alive vertex's lists and every entry of
@@ -672,7 +687,7 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
                             if (cc) store_agent(S.cd + x, nc);
                         }
                     }
-                    append_frontier<MULTI>(claim, x, kl + cl, G, GL, fl_cap, nxt);
+                    append_frontier<MULTI>(claim, x, kl + cl, G, GL, fl_cap, nxt, &s_etot);
                 }
                 if (nLA) {
                     // long alive lists: every member counts the hits in its
@@ -746,7 +761,7 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
                                 }
                             }
                         }
-                        append_frontier<MULTI>(claim, x, len, G, GL, fl_cap, nxt);
+                        append_frontier<MULTI>(claim, x, len, G, GL, fl_cap, nxt, &s_etot);
                     }
                 }
             } else {
@@ -865,10 +880,11 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
                         list_lens(P, S, x, a, b);
                         len = a + b;
                     }
-                    append_frontier<MULTI>(claim, x, len, G, GL, fl_cap, nxt);
+                    append_frontier<MULTI>(claim, x, len, G, GL, fl_cap, nxt, &s_etot);
                 }
             }
             wave_min_to(mn2, &nxt[2]);
+            flush_totals(nxt);
             if (!barrier()) return;
             const SlotView sv2 = slot_read(nxt, s_rd);
             nF = sv2.nF;
