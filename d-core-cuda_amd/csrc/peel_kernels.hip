@@ -121,6 +121,55 @@ __device__ __forceinline__ void wave_sum_to(uint32_t x, unsigned long long* dst)
     if (lane_id() == 0 && y) consume((uint32_t)atomicAdd(dst, y));
 }
 
+// Workgroup-aggregated appends of up to U values per thread to two lists
+// (pa -> arra, pb -> arrb): one global atomic per list per call instead of
+// one per wavefront.  Every thread of the workgroup must call it (it
+// contains barriers); scr: LDS, 2 * (BLK / W) + 2 words.
+template <int BLK, bool C, int U>
+__device__ __forceinline__ void block_append2(const bool (&pa)[U], uint32_t* arra, uint32_t* ctra,
+                                              const bool (&pb)[U], uint32_t* arrb, uint32_t* ctrb,
+                                              const uint32_t (&val)[U], uint32_t* scr) {
+    constexpr int NW = BLK / W;
+    const int w = (int)(threadIdx.x / W), ln = lane_id();
+    const unsigned long long lt = (1ull << ln) - 1ull;
+    unsigned long long ma[U], mb[U];
+    uint32_t ca = 0, cb = 0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        ma[u] = __ballot(pa[u]);
+        mb[u] = __ballot(pb[u]);
+        ca += (uint32_t)__popcll(ma[u]);
+        cb += (uint32_t)__popcll(mb[u]);
+    }
+    if (ln == 0) {
+        scr[w] = ca;
+        scr[NW + w] = cb;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t ta = 0, tb = 0;
+        for (int i = 0; i < NW; ++i) {
+            const uint32_t x = scr[i], y = scr[NW + i];
+            scr[i] = ta;
+            scr[NW + i] = tb;
+            ta += x;
+            tb += y;
+        }
+        scr[2 * NW] = ta ? atomicAdd(ctra, ta) : 0u;
+        scr[2 * NW + 1] = tb ? atomicAdd(ctrb, tb) : 0u;
+    }
+    __syncthreads();
+    uint32_t oa = scr[2 * NW] + scr[w], ob = scr[2 * NW + 1] + scr[NW + w];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+        if (pa[u]) st<C>(&arra[oa + __popcll(ma[u] & lt)], val[u]);
+        oa += (uint32_t)__popcll(ma[u]);
+        if (pb[u]) st<C>(&arrb[ob + __popcll(mb[u] & lt)], val[u]);
+        ob += (uint32_t)__popcll(mb[u]);
+    }
+    __syncthreads();  // scr is reused by the next call
+}
+
 // wave_append into arr[0, cap): appends beyond cap are counted but dropped.
 template <bool C>
 __device__ __forceinline__ void wave_append_capped(bool pred, uint32_t val, uint32_t* arr,
@@ -142,6 +191,11 @@ __device__ __forceinline__ void wave_append_capped(bool pred, uint32_t val, uint
 // Both count their list edges into *etot (the workgroup's LDS total, added
 // to slot words 4..5 once per phase).
 constexpr uint32_t kLongList = 4096;
+// Appends go per workgroup (one atomic per list and workgroup) instead of
+// per wavefront when kAggMembers or more workgroups append to the same
+// counters: from kAggEdges frontier edges on in a push, always in a seed.
+constexpr uint64_t kAggEdges = 1u << 16;
+constexpr uint32_t kAggMembers = 64;
 template <bool C>
 __device__ __forceinline__ void append_frontier(bool pred, uint32_t v, uint32_t len, uint32_t* G,
                                                 uint32_t* fl, uint32_t fl_cap, uint32_t* slot,
@@ -395,6 +449,7 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
 
     __shared__ uint32_t s_rd[12];
     __shared__ uint32_t s_part[2];
+    __shared__ uint32_t s_scr[2 * (BLK / W) + 2];
     // list-edge totals of the phase, summed in LDS and added to the slot
     // once per workgroup (per-wavefront global atomics on two words
     // serialised big phases)
@@ -485,6 +540,7 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
         if (tw == 0) slot_reset(slots + kSlotStride * ((s + 1) % 3), tid);
         uint32_t mn = UINT_MAX;
         const bool first = A == nullptr;
+        const bool sagg = tn >= kAggMembers;  // the same in every member
         // kSeedUnroll independent list entries per thread per step
         for (uint32_t base = tw * BLK * kSeedUnroll; base < nA; base += tn * BLK * kSeedUnroll) {
             uint32_t v[kSeedUnroll];
@@ -502,6 +558,7 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
                 alive[u] = has[u] && ld<MULTI>(&lval[v[u]]) == -1;
                 d[u] = alive[u] ? ld<MULTI>(&S.kd[v[u]]) : 0;
             }
+            bool to_f[kSeedUnroll], to_b[kSeedUnroll];
 #pragma unroll
             for (int u = 0; u < kSeedUnroll; ++u) {
                 const bool seed =
@@ -520,10 +577,21 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
                 if (keep) mn = min(mn, (uint32_t)d[u]);
                 // kept vertices with long lists: their pulls are sliced
                 if (first) wave_append_capped<MULTI>(keep && len >= kLongList, v[u], LA, &cur[9], la_cap);
-                append_frontier<MULTI>(seed, v[u], len, F, FL, fl_cap, cur, &s_etot);
-                wave_append<MULTI>(keep, v[u], B, &cur[1]);
+                // seeds with long lists go to their own list, the rest and
+                // the survivors are appended per workgroup below
+                append_frontier<MULTI>(seed && (!sagg || len >= kLongList), v[u], len, F, FL,
+                                       fl_cap, cur, &s_etot);
+                to_f[u] = sagg && seed && len < kLongList;
+                to_b[u] = sagg && keep;
+                if (!sagg)
+                    wave_append<MULTI>(keep, v[u], B, &cur[1]);
+                else
+                    wave_sum_to(to_f[u] ? len : 0u, &s_etot);
                 if (first) wave_sum_to(keep ? len : 0u, &s_atot);
             }
+            if (sagg)
+                block_append2<BLK, MULTI, kSeedUnroll>(to_f, F, &cur[0], to_b, B, &cur[1], v,
+                                                       s_scr);
         }
         wave_min_to(mn, &cur[2]);
         flush_totals(cur);
@@ -552,6 +620,8 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
             uint32_t* nxt = slots + kSlotStride * (s % 3);
             if (tw == 0) slot_reset(slots + kSlotStride * ((s + 1) % 3), tid);
             uint32_t mn2 = UINT_MAX;
+            // the same in every member
+            const bool agg = fe >= kAggEdges && tn >= kAggMembers;
             // kUnroll edges held by this thread: x[u] loses a neighbour on
             // the constraint side (ks[u]) or the key side; ok[u] = a real edge
             auto visit = [&](uint32_t(&x)[kUnroll], bool(&ks)[kUnroll], bool(&ok)[kUnroll]) {
@@ -618,6 +688,7 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
                         if (claim[u]) poison(S, x[u], s);
                     }
                 }
+                bool to_g[kUnroll], none[kUnroll];
 #pragma unroll
                 for (int u = 0; u < kUnroll; ++u) {
                     uint32_t len = 0;
@@ -626,8 +697,18 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
                         list_lens(P, S, x[u], a, b);
                         len = a + b;
                     }
-                    append_frontier<MULTI>(claim[u], x[u], len, G, GL, fl_cap, nxt, &s_etot);
+                    // a big frontier claims many vertices per step: those
+                    // are appended per workgroup; a small one per wavefront
+                    // (no extra barriers on the latency-bound path)
+                    append_frontier<MULTI>(claim[u] && (!agg || len >= kLongList), x[u], len, G,
+                                           GL, fl_cap, nxt, &s_etot);
+                    to_g[u] = agg && claim[u] && len < kLongList;
+                    none[u] = false;
+                    if (agg) wave_sum_to(to_g[u] ? len : 0u, &s_etot);
                 }
+                if (agg)
+                    block_append2<BLK, MULTI, kUnroll>(to_g, G, &nxt[0], none, G, &nxt[0], x,
+                                                       s_scr);
             };
             // a pull reads every alive vertex's lists and every entry of
             // the level's alive list (most of which may have left by now)
