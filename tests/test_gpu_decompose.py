@@ -143,6 +143,22 @@ def test_batched_launch_shapes(shape):
     assert np.array_equal(km, ref[1]) and np.array_equal(rows, ref[3])
 
 
+def hub_graph():
+    """A sparse random digraph plus three hubs linked both ways to 6,000
+    vertices each: lists far above the 4,096 edges from which frontier and
+    alive vertices are walked in slices by the whole team."""
+    V = 20000
+    rng = np.random.default_rng(17)
+    s, d = dcx.gen_er(V, 60000, 21)
+    src, dst = [s.astype(np.int64)], [d.astype(np.int64)]
+    for h in (0, 1, 2):
+        nb = rng.choice(np.arange(3, V), size=6000, replace=False)
+        src += [np.full(nb.size, h), nb]
+        dst += [nb, np.full(nb.size, h)]
+    _, src, dst = graphs.dedup(V, zip(np.concatenate(src).tolist(), np.concatenate(dst).tolist()))
+    return V, src, dst
+
+
 @pytest.mark.parametrize("ratio", [-1, 1, 4, 1 << 24])
 def test_push_and_pull_phases(ratio):
     """Cascade phases pushed from the frontier (ratio -1), pulled by the
@@ -151,10 +167,12 @@ def test_push_and_pull_phases(ratio):
     whole graph."""
     cases = [("rmat14", 1 << 14) + tuple(dcx.gen_rmat(14, 16 << 14, 5)),
              ("er", 3000) + tuple(dcx.gen_er(3000, 30000, 8)),
-             ("complete12",) + graphs.complete(12)]
+             ("complete12",) + graphs.complete(12),
+             ("hubs",) + hub_graph()]
     for name, V, src, dst in cases:
         ref = oracle.decompose(oracle.csr_from_edges(V, src, dst))
-        with dcx.Context(0, pull_ratio=ratio) as c:
+        # lds_members=-1: every row on the team path these phases belong to
+        with dcx.Context(0, pull_ratio=ratio, lds_members=-1) as c:
             for var in (dcx.VARIANT_BATCHED, dcx.VARIANT_PEEL):
                 say(f"{name} pull_ratio={ratio} variant={var}")
                 K, km, lmax, rows, _, _ = gpu_decompose(c, V, src, dst, var)
