@@ -262,25 +262,35 @@ Sched make_schedule(const dcx_graph* g, int32_t kr, uint32_t G, uint32_t team_ma
         W += w[k];
     }
     // widths follow each pass's share of the work within the 3/4 of the
-    // grid that wide teams may take
-    const double share = W / std::max<uint32_t>(G * 3 / 4, 1);
+    // grid that wide teams may take; when no pass is left for single
+    // workgroups, the teams take the whole grid instead
+    const uint32_t tcap = std::min(team_max, G / kXcds);
+    uint32_t big_budget = std::max<uint32_t>(G * 3 / 4, 1);
+    double share = W / big_budget;
+    if (!want) {
+        bool any_small = false;
+        for (uint32_t k = 0; k < np && !any_small; ++k)
+            any_small = std::min<double>(tcap, w[k] / share) < 2.0;
+        if (!any_small) {
+            big_budget = std::max<uint32_t>(G, 1);
+            share = W / big_budget;
+        }
+    }
     std::vector<uint32_t> idx(np);
     for (uint32_t k = 0; k < np; ++k) idx[k] = k;
     std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return w[a] > w[b]; });
     // free workgroups of each XCD, in increasing block order
     std::vector<std::vector<uint32_t>> xfree(kXcds);
     for (uint32_t b = G; b-- > 0;) xfree[b % kXcds].push_back(b);  // back() = lowest
-    const uint32_t per_xcd = G / kXcds;
     Sched s;
     s.sync_stride = (sync_words(team_max) + kSlotStride - 1) / kSlotStride * kSlotStride;
     s.block_map.assign(2ull * G, ~0u);
-    const uint32_t big_budget = G * 3 / 4;
     uint32_t used = 0;
     std::vector<uint32_t> small;
     std::vector<std::vector<uint32_t>> team_blocks;
     for (uint32_t k : idx) {
-        uint32_t t = want ? std::min((*want)[k], std::min(team_max, per_xcd))
-                          : (uint32_t)std::min<double>(std::min(team_max, per_xcd), w[k] / share);
+        uint32_t t = want ? std::min((*want)[k], tcap)
+                          : (uint32_t)std::min<double>(tcap, w[k] / share);
         uint32_t x = 0;
         for (uint32_t y = 1; y < kXcds; ++y)
             if (xfree[y].size() > xfree[x].size()) x = y;
