@@ -155,6 +155,44 @@ void grow_keep(dcx_ctx* c, T** p, size_t* cap, size_t n, size_t keep) {
 // run the h-index fixpoint over in-neighbours, from the in-degree or --
 // `warm`, valid only when no kmax can have grown (a deletion-only update) --
 // from the resident kmaxes.
+// The kmax pass (in-degree peel) and row 0 (out-degree peel, which needs no
+// kmaxes) in one launch, each on half of the grid: both are chains of
+// ~600 barrier-separated phases, so running them side by side costs little
+// more than one.  Row 0 goes to g->row0 (original ids) with its log words
+// and lmax in g->row0_stats; returns kmax.
+int32_t run_kmax_with_row0(dcx_ctx* c, dcx_graph* g) {
+    const uint32_t V = g->V;
+    const int G1 = c->grid_blocks / 2, G0 = c->grid_blocks - G1;
+    ensure_scratch(c, 2ull * kScratchArrays * V * 4);
+    if (!c->ctrs2) {
+        dev_alloc(c, &c->ctrs2, kCtrWords);
+        HIPX(c, hipMemsetAsync(c->ctrs2, 0, kCtrWords * 4, c->stream));
+    }
+    if (!g->row0) dev_alloc(c, &g->row0, V);
+    if (!g->row0_stats) dev_alloc(c, &g->row0_stats, kStatWords + 1);
+    const PassSpec p0 = traced(c, kmax_spec(g));
+    PassSpec p1 = full_row_spec(g, 0, g->row0);
+    p1.pull_ratio = p0.pull_ratio;
+    p1.stats = g->row0_stats;
+    GridSync g0 = gsync(c, G0), g1 = gsync(c, G1);
+    g1.w = c->ctrs2 + kCtrSync;
+    clear_kernel_errors(c);
+    HIPX(c, hipMemsetAsync(g1.w + kSyncError, 0, 4, c->stream));
+    HIPX(c, hipEventRecord(c->ev[0], c->stream));
+    HIPX(c, launch_pass_pair(p0, scratch_at(c->scratch, V), c->ctrs, c->d_scalars, g0, p1,
+                             scratch_at(static_cast<uint32_t*>(c->scratch) +
+                                            (size_t)kScratchArrays * V,
+                                        V),
+                             c->ctrs2, reinterpret_cast<int32_t*>(g->row0_stats + kStatWords),
+                             g1, c->stream));
+    HIPX(c, hipEventRecord(c->ev[1], c->stream));
+    HIPX(c, hipMemcpyAsync(c->h_scalars, c->d_scalars, sizeof(int32_t), hipMemcpyDeviceToHost,
+                           c->stream));
+    check_kernel_errors(c, "kmax pass beside row 0", g1.w + kSyncError, 1, 1);
+    g->row0_ready = true;
+    return c->h_scalars[0];
+}
+
 void compute_kmax(dcx_ctx* c, dcx_graph* g, int variant = DCX_VARIANT_BATCHED,
                   bool warm = false) {
     if (!g->kmaxes) dev_alloc(c, &g->kmaxes, g->V);
@@ -169,8 +207,13 @@ void compute_kmax(dcx_ctx* c, dcx_graph* g, int variant = DCX_VARIANT_BATCHED,
     // for, and for deletion-only updates, where it starts from the old
     // values (an upper bound) and settles in a few sweeps.
     const bool use_peel = variant == DCX_VARIANT_PEEL || (variant != DCX_VARIANT_HINDEX && !warm);
+    g->row0_ready = false;
     if (use_peel) {
-        g->kmax = run_grid_pass(c, kmax_spec(g), 0, 1);
+        // the batched rows start from row 0 when it is a team row (it bounds
+        // every other row): compute it beside the kmax pass
+        const bool pair = variant == DCX_VARIANT_BATCHED && g->V > c->lds_cap &&
+                          c->grid_blocks >= 2 * (int)kXcds;
+        g->kmax = pair ? run_kmax_with_row0(c, g) : run_grid_pass(c, kmax_spec(g), 0, 1);
     } else {
         const int32_t* prev = nullptr;
         if (warm && g->kmax_valid) {
@@ -251,13 +294,13 @@ constexpr int kRowsBlocksPerCu = 2;
 
 // want/wts (optional): team width and weight of each pass chosen by the
 // cost model; without them the width follows the pass's share of the work.
-Sched make_schedule(const dcx_graph* g, int32_t kr, uint32_t G, uint32_t team_max,
-                    const std::vector<uint32_t>* want = nullptr,
+Sched make_schedule(const dcx_graph* g, int32_t k_first, int32_t kr, uint32_t G,
+                    uint32_t team_max, const std::vector<uint32_t>* want = nullptr,
                     const std::vector<double>* wts = nullptr) {
-    const uint32_t np = (uint32_t)kr + 1;
-    std::vector<double> w(np);
+    const uint32_t np = (uint32_t)kr + 1, k0 = (uint32_t)std::max(k_first, 0);
+    std::vector<double> w(np, 0.0);
     double W = 0;
-    for (uint32_t k = 0; k < np; ++k) {
+    for (uint32_t k = k0; k < np; ++k) {
         w[k] = wts ? (*wts)[k] : 2.0 * (double)g->h_ek[k] + (double)g->h_vk[k] + 64.0;
         W += w[k];
     }
@@ -267,17 +310,16 @@ Sched make_schedule(const dcx_graph* g, int32_t kr, uint32_t G, uint32_t team_ma
     const uint32_t tcap = std::min(team_max, G / kXcds);
     uint32_t big_budget = std::max<uint32_t>(G * 3 / 4, 1);
     double share = W / big_budget;
-    if (!want) {
-        bool any_small = false;
-        for (uint32_t k = 0; k < np && !any_small; ++k)
-            any_small = std::min<double>(tcap, w[k] / share) < 2.0;
-        if (!any_small) {
-            big_budget = std::max<uint32_t>(G, 1);
-            share = W / big_budget;
-        }
+    bool any_small = false;
+    for (uint32_t k = k0; k < np && !any_small; ++k)
+        any_small = want ? std::min((*want)[k], tcap) < 2
+                         : std::min<double>(tcap, w[k] / share) < 2.0;
+    if (!any_small) {
+        big_budget = std::max<uint32_t>(G, 1);
+        share = W / big_budget;
     }
-    std::vector<uint32_t> idx(np);
-    for (uint32_t k = 0; k < np; ++k) idx[k] = k;
+    std::vector<uint32_t> idx;
+    for (uint32_t k = k0; k < np; ++k) idx.push_back(k);
     std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return w[a] > w[b]; });
     // free workgroups of each XCD, in increasing block order
     std::vector<std::vector<uint32_t>> xfree(kXcds);
@@ -400,11 +442,13 @@ std::vector<float> mlp_predict(dcx_ctx* c, dcx_mlp* m, const std::vector<float>&
 // Team widths from the cost model: the predicted time d(k, t) of every
 // pass at every candidate width, then the smallest makespan T for which
 // each pass gets the narrowest width with d <= T, the wide teams fit in 3/4
-// of the grid and the single-workgroup passes (packed longest first) fit
-// in the rest.  Returns false if no width assignment was found.
-bool mlp_team_sizes(dcx_ctx* c, const dcx_graph* g, int32_t kr, uint32_t G, uint32_t team_max,
-                    std::vector<uint32_t>& want, std::vector<double>& wts) {
-    const uint32_t np = (uint32_t)kr + 1;
+// of the grid (all of it when no pass is left for a single workgroup) and
+// the single-workgroup passes (packed longest first) fit in the rest.
+// Passes below k_first are not scheduled.  Returns false if no width
+// assignment was found.
+bool mlp_team_sizes(dcx_ctx* c, const dcx_graph* g, int32_t k_first, int32_t kr, uint32_t G,
+                    uint32_t team_max, std::vector<uint32_t>& want, std::vector<double>& wts) {
+    const uint32_t np = (uint32_t)kr + 1, k0 = (uint32_t)std::max(k_first, 0);
     const uint32_t cap = std::min(team_max, G / kXcds);
     std::vector<uint32_t> cands{1};
     for (uint32_t t = 2; t <= cap; t *= 2) cands.push_back(t);
@@ -426,7 +470,7 @@ bool mlp_team_sizes(dcx_ctx* c, const dcx_graph* g, int32_t kr, uint32_t G, uint
     auto assign = [&](double T, std::vector<uint32_t>& sz) -> bool {
         uint32_t big = 0;
         std::vector<double> singles;
-        for (uint32_t k = 0; k < np; ++k) {
+        for (uint32_t k = k0; k < np; ++k) {
             uint32_t j = 0;
             while (j + 1 < nc && d[(size_t)k * nc + j] > T) ++j;
             if (d[(size_t)k * nc + j] > T) return false;
@@ -436,7 +480,7 @@ bool mlp_team_sizes(dcx_ctx* c, const dcx_graph* g, int32_t kr, uint32_t G, uint
             else
                 singles.push_back(d[(size_t)k * nc]);
         }
-        if (big > G * 3 / 4) return false;
+        if (big > (singles.empty() ? G : G * 3 / 4)) return false;
         const uint32_t slots = G - big;
         if (singles.empty()) return true;
         if (!slots) return false;
@@ -450,7 +494,7 @@ bool mlp_team_sizes(dcx_ctx* c, const dcx_graph* g, int32_t kr, uint32_t G, uint
         return true;
     };
     double lo = 0.0, hi = 0.0;
-    for (uint32_t k = 0; k < np; ++k) hi += d[(size_t)k * nc];  // one workgroup does all
+    for (uint32_t k = k0; k < np; ++k) hi += d[(size_t)k * nc];  // one workgroup does all
     std::vector<uint32_t> sz(np, 1);
     if (!assign(hi, sz)) return false;
     for (int it = 0; it < 40; ++it) {
@@ -463,8 +507,8 @@ bool mlp_team_sizes(dcx_ctx* c, const dcx_graph* g, int32_t kr, uint32_t G, uint
     }
     assign(hi, sz);
     want = sz;
-    wts.resize(np);
-    for (uint32_t k = 0; k < np; ++k) {
+    wts.assign(np, 0.0);
+    for (uint32_t k = k0; k < np; ++k) {
         uint32_t j = 0;
         while (cands[j] != sz[k]) ++j;
         wts[k] = d[(size_t)k * nc + j] * sz[k];  // longest (in workgroup time) placed first
@@ -539,7 +583,7 @@ void compute_rows(dcx_ctx* c, dcx_graph* g, int variant, int32_t kr = -1) {
     HIPX(c, hipMemsetAsync(g->rows, 0, nre * V * sizeof(int32_t), c->stream));
     int launches = 0, slots_used = 1;
     int32_t split = kr + 1;  // batched: first LDS-resident row
-    bool teams_ran = false, lds_ran = false;
+    bool teams_ran = false, lds_ran = false, first_ran = false;
     std::vector<std::pair<uint32_t, int32_t>> last_teams;  // (size, first k) for diagnosis
     if (variant == DCX_VARIANT_HINDEX) {
         // rows in k order, each started from the row above (l_k <= l_{k-1})
@@ -584,15 +628,25 @@ void compute_rows(dcx_ctx* c, dcx_graph* g, int variant, int32_t kr = -1) {
             (int64_t)c->num_cus * c->rows_per_cu);
         const uint32_t tmax = c->cfg.team_max > 0 ? (uint32_t)c->cfg.team_max : kTeamMax;
         const uint32_t shrink = c->cfg.shrink_at > 0 ? (uint32_t)c->cfg.shrink_at : kShrinkAt;
+        // Row 0 first, on the whole grid, when other rows run on teams: a
+        // (k,l)-core lies inside the (0,l)-core, so row 0 bounds every other
+        // row, and the team and LDS passes seed with that bound
+        // (PassSpec::ub); most vertices then leave in the seeding phase of
+        // their level instead of at the end of a cascade.
+        // (When the kmax pass computed row 0 beside it, only its copy and
+        // the bound by rank are left to do here.)
+        const bool have_row0 = g->row0_ready;
+        const int32_t k_first = ksplit >= 2 || have_row0 ? 1 : 0;
+        g->rows_first_grid = k_first;
         Sched s;
         uint32_t busy = 0;
-        if (ksplit > 0) {
+        if (ksplit > k_first) {
             std::vector<uint32_t> want;
             std::vector<double> wts;
-            const bool by_model =
-                c->sched_mlp && mlp_team_sizes(c, g, ksplit - 1, G, tmax, want, wts);
-            s = by_model ? make_schedule(g, ksplit - 1, G, tmax, &want, &wts)
-                         : make_schedule(g, ksplit - 1, G, tmax);
+            const bool by_model = c->sched_mlp && mlp_team_sizes(c, g, k_first, ksplit - 1, G,
+                                                                 tmax, want, wts);
+            s = by_model ? make_schedule(g, k_first, ksplit - 1, G, tmax, &want, &wts)
+                         : make_schedule(g, k_first, ksplit - 1, G, tmax);
             c->timing.rows_scheduler = by_model ? 1 : 0;
             busy = (uint32_t)std::count_if(s.block_map.begin(), s.block_map.begin() + G,
                                            [](uint32_t t) { return t != ~0u; });
@@ -617,7 +671,7 @@ void compute_rows(dcx_ctx* c, dcx_graph* g, int variant, int32_t kr = -1) {
             lks.empty() ? 0u : (std::max<uint32_t>(g->h_vk[ksplit], 1) + 31) / 32 * 32;
         const uint32_t lblocks = (uint32_t)std::min<size_t>(lks.size(), (size_t)c->num_cus);
         // device copies: team table, block map, team passes, LDS passes
-        const size_t nt = s.teams.size(), nb = ksplit > 0 ? G : 0, nk = s.ks.size();
+        const size_t nt = s.teams.size(), nb = ksplit > k_first ? G : 0, nk = s.ks.size();
         const size_t bytes = nt * sizeof(TeamDesc) + 8 * nb + 4 * nk + 4 * lks.size() + 64;
         ensure_bytes(c, &c->sched, &c->sched_bytes, bytes);
         char* base = static_cast<char*>(c->sched);
@@ -639,14 +693,41 @@ void compute_rows(dcx_ctx* c, dcx_graph* g, int variant, int32_t kr = -1) {
             c->pools_words = pool_words;
         }
         if (nt) HIPX(c, hipMemsetAsync(c->pools, 0, 4 * pool_words, c->stream));
-        // the two launches run side by side, each on its own scratch
+        // the two launches run side by side, each on its own scratch (row
+        // 0's grid pass runs before them on the same memory)
         const uint64_t lds_off = (s.scratch_words + 31) / 32 * 32;
-        ensure_scratch(c, 4 * (lds_off + (uint64_t)lblocks * kScratchArrays * lcap));
+        ensure_scratch(c, 4 * std::max<uint64_t>(lds_off + (uint64_t)lblocks * kScratchArrays * lcap,
+                                                 k_first ? (uint64_t)kScratchArrays * V : 0));
         if (!lks.empty()) HIPX(c, hipMemsetAsync(c->ctrs + kCtrQueue, 0, 4, c->stream));
         PassSpec basep = traced(c, core_row_spec(g, 0, g->rows), true);
         basep.fail_rec = c->fail_rec;
+        if (k_first && !g->row_bound) dev_alloc(c, &g->row_bound, V);
 
         HIPX(c, hipEventRecord(c->ev[5], c->stream));
+        if (k_first && have_row0) {
+            HIPX(c, hipMemcpyAsync(g->rows, g->row0, 4ull * V, hipMemcpyDeviceToDevice,
+                                   c->stream));
+            HIPX(c, hipMemcpyAsync(g->pstats + kStatWords, g->row0_stats, 4ull * kStatWords,
+                                   hipMemcpyDeviceToDevice, c->stream));
+            HIPX(c, hipMemcpyAsync(g->lmax_rows, g->row0_stats + kStatWords, 4,
+                                   hipMemcpyDeviceToDevice, c->stream));
+            HIPX(c, launch_gather_rank(g->row0, g->order, V, g->row_bound, c->stream));
+            basep.ub = g->row_bound;
+            g->h_pass_variant[1] = DCX_VARIANT_BATCHED;
+            g->h_pass_team[1] = (uint16_t)c->grid_blocks;
+        } else if (k_first) {
+            const PassScratch s0 = scratch_at(c->scratch, V);
+            HIPX(c, launch_pass_grid(traced(c, core_row_spec(g, 0, g->rows)), s0, c->ctrs,
+                                     g->lmax_rows, gsync(c, c->grid_blocks), c->stream));
+            ++launches;
+            // the pass leaves its levels by rank in the scratch
+            HIPX(c, hipMemcpyAsync(g->row_bound, s0.lv, 4ull * V, hipMemcpyDeviceToDevice,
+                                   c->stream));
+            basep.ub = g->row_bound;
+            g->h_pass_variant[1] = DCX_VARIANT_BATCHED;
+            g->h_pass_team[1] = (uint16_t)c->grid_blocks;
+        }
+        HIPX(c, hipEventRecord(c->sev[3], c->stream));
         if (!lks.empty()) {
             // LDS rows on the side stream: each of its workgroups needs a
             // CU's LDS to itself and waits for nothing, so it starts where
@@ -679,6 +760,7 @@ void compute_rows(dcx_ctx* c, dcx_graph* g, int variant, int32_t kr = -1) {
         }
         slots_used = (int)std::max(busy, lblocks);
         split = ksplit;
+        first_ran = k_first > 0;
         teams_ran = nt > 0;
         lds_ran = !lks.empty();
     }
@@ -712,10 +794,16 @@ void compute_rows(dcx_ctx* c, dcx_graph* g, int variant, int32_t kr = -1) {
     c->timing.rows_split = split;
     c->timing.rows_teams_ms = 0.0f;
     c->timing.rows_lds_ms = 0.0f;
+    c->timing.rows_first_ms = 0.0f;
     if (teams_ran) {
         float ms = 0;
-        HIPX(c, hipEventElapsedTime(&ms, c->ev[5], c->sev[2]));
+        HIPX(c, hipEventElapsedTime(&ms, c->sev[3], c->sev[2]));
         c->timing.rows_teams_ms = ms;
+    }
+    if (first_ran) {
+        float ms = 0;
+        HIPX(c, hipEventElapsedTime(&ms, c->ev[5], c->sev[3]));
+        c->timing.rows_first_ms = ms;
     }
     if (lds_ran) {
         float ms = 0;
@@ -782,6 +870,9 @@ void free_graph_arrays(dcx_graph* g) {
     dev_free(g->out_capv);
     dev_free(g->in_capv);
     dev_free(g->kmax_old);
+    dev_free(g->row_bound);
+    dev_free(g->row0);
+    dev_free(g->row0_stats);
     dev_free(g->pstats);
     dev_free(g->kmaxes);
     dev_free(g->order);
@@ -892,10 +983,10 @@ dcx_status dcx_ctx_create(const dcx_config* cfg, dcx_ctx** out) {
         if (hipEventCreate(&e) != hipSuccess) return DCX_ERR_HIP;
     if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->join, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreate(&c->sev[0]) != hipSuccess || hipEventCreate(&c->sev[1]) != hipSuccess ||
-        hipEventCreate(&c->sev[2]) != hipSuccess)
+        hipEventCreateWithFlags(&c->join, hipEventDisableTiming) != hipSuccess)
         return DCX_ERR_HIP;
+    for (auto& e : c->sev)
+        if (hipEventCreate(&e) != hipSuccess) return DCX_ERR_HIP;
     if (hipMalloc(&c->ctrs, kCtrWords * sizeof(uint32_t)) != hipSuccess)
         return DCX_ERR_OUT_OF_MEMORY;
     if (hipMemset(c->ctrs, 0, kCtrWords * sizeof(uint32_t)) != hipSuccess) return DCX_ERR_HIP;
@@ -918,6 +1009,7 @@ dcx_status dcx_ctx_destroy(dcx_ctx* c) {
     dev_free(c->trace);
     dev_free(c->fail_rec);
     dev_free(c->ctrs);
+    dev_free(c->ctrs2);
     dev_free(c->d_scalars);
     dev_free(c->upd_dev);
     if (c->h_scalars) (void)hipHostFree(c->h_scalars);

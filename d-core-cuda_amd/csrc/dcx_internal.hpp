@@ -55,6 +55,11 @@ struct PassSpec {
     // alive vertices' lists hold at most pull_ratio times the frontier's
     // edges; 0 = always push.
     uint32_t pull_ratio;
+    // Nullable (core passes only): per rank, an upper bound of the row's
+    // value.  A member still alive at level L with ub <= L cannot be in the
+    // (k, L+1)-core, so it is seeded at L directly instead of waiting for
+    // a cascade to reach it; the values are unchanged.
+    const int32_t* ub;
 };
 
 enum : uint32_t { kFailWords = 16 };
@@ -171,6 +176,15 @@ hipError_t launch_mlp_adam(float* p, const float* grad, float* m, float* v, uint
 // ---- peel kernels (peel_kernels.hip) ----------------------------------
 // One pass on the whole grid (bounded team barrier); `slots` 12 words,
 // gs.w zeroed by the launcher.
+// Two grid passes side by side in one launch (gs0.nblocks + gs1.nblocks
+// workgroups, all resident).
+hipError_t launch_pass_pair(const PassSpec& p0, const PassScratch& s0, uint32_t* slots0,
+                            int32_t* lmax0, const GridSync& gs0, const PassSpec& p1,
+                            const PassScratch& s1, uint32_t* slots1, int32_t* lmax1,
+                            const GridSync& gs1, hipStream_t st);
+// out[r] = in[order[r]] for r < V.
+hipError_t launch_gather_rank(const int32_t* in, const uint32_t* order, uint32_t V, int32_t* out,
+                              hipStream_t st);
 hipError_t launch_pass_grid(const PassSpec& p, const PassScratch& s, uint32_t* slots,
                             int32_t* lmax_out, const GridSync& gs, hipStream_t st);
 int pass_grid_blocks_per_cu();

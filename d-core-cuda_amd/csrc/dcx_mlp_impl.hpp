@@ -164,7 +164,8 @@ dcx_status dcx_graph_pass_features(dcx_ctx* ctx, const dcx_graph* g, int32_t wit
         for (int32_t k = 0; k <= g->kmax; ++k) {
             // team passes only: the LDS-resident rows (team 0) are not
             // scheduled by width
-            if (g->h_pass_variant[k + 1] != DCX_VARIANT_BATCHED || g->h_pass_team[k + 1] == 0)
+            if (g->h_pass_variant[k + 1] != DCX_VARIANT_BATCHED || g->h_pass_team[k + 1] == 0 ||
+                k < g->rows_first_grid)
                 continue;
             const uint32_t* w = g->h_pstats.data() + (size_t)(k + 1) * kStatWords;
             const uint64_t ticks = (uint64_t)w[kStatTicksLo] | ((uint64_t)w[kStatTicksHi] << 32);

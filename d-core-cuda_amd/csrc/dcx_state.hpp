@@ -35,9 +35,10 @@ struct dcx_ctx {
     // beside the team launch; fork/join order it against `stream`
     hipStream_t side = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
-    // [0], [1]: brackets of the LDS-resident launch on `side`; [2]: end of
-    // the team launch on `stream` (which ev[5] opens)
-    hipEvent_t sev[3] = {};
+    // [0], [1]: brackets of the LDS-resident launch on `side`; [3], [2]:
+    // start and end of the team launch on `stream` (ev[5] opens the rows
+    // kernels, row 0's grid pass first when the other rows use its bound)
+    hipEvent_t sev[4] = {};
     std::mutex mu;
     std::string err;
     dcx_timing timing{};
@@ -49,6 +50,7 @@ struct dcx_ctx {
     // small device words: [0,12) grid-pass phase slots, [16,20) h-index sweep
     // flags, [24] batched error word, [64, 1024) grid barrier words
     uint32_t* ctrs = nullptr;
+    uint32_t* ctrs2 = nullptr;  // slots and barrier words of the second pass of a pair
     int32_t* d_scalars = nullptr;  // 64 words of kernel results
     int32_t* h_scalars = nullptr;  // pinned mirror (words 32.. hold error words)
     hipEvent_t ev[8] = {};
@@ -128,6 +130,13 @@ struct dcx_graph {
     size_t core_e_cap = 0;
     bool core_valid = false;
     int32_t* kmax_old = nullptr; // [V] previous kmaxes during an update
+    int32_t* row_bound = nullptr; // [V] row 0 by rank: bound of the batched rows' values
+    int32_t rows_first_grid = 0;  // batched: leading rows not run by the rows launch (row 0)
+    // Row 0 computed beside the kmax pass (it needs no kmaxes): values by
+    // original id, its pass-log words and lmax; valid until the edges change.
+    int32_t* row0 = nullptr;
+    uint32_t* row0_stats = nullptr;
+    bool row0_ready = false;
     uint32_t* pstats = nullptr;  // [(kmax+2) * kStatWords] device pass log: kmax pass, rows
     size_t pstats_cap = 0;
     std::vector<uint32_t> h_pstats;      // host copy of pstats after the last rows launch

@@ -509,6 +509,7 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
     // them choose the same kind of cascade phase.
     uint64_t pa = 0, fe = 0;
     uint32_t nLA = 0;  // long alive list length (0 when it overflowed)
+    const int32_t* const ub = P.core ? P.ub : nullptr;
     while (removed < nm) {
         // Each removal needs at most two level iterations (a level number
         // taken from a stale minimum seeds nothing, the next one seeds the
@@ -557,6 +558,8 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
             for (int u = 0; u < kSeedUnroll; ++u) {
                 alive[u] = has[u] && ld<MULTI>(&lval[v[u]]) == -1;
                 d[u] = alive[u] ? ld<MULTI>(&S.kd[v[u]]) : 0;
+                // the bound acts like a key degree that never drops
+                if (ub && alive[u]) d[u] = min(d[u], ub[v[u]]);
             }
             bool to_f[kSeedUnroll], to_b[kSeedUnroll];
 #pragma unroll
@@ -1090,6 +1093,7 @@ __device__ void run_pass_lds(const PassSpec& P, const PassScratch& S, uint32_t* 
     uint32_t nA = nm, removed = 0, s = 0, levels = 0;
     int32_t L = 0;
     const int32_t k = P.k;
+    const int32_t* const ub = P.ub;
     bool failed = false;
     uint64_t t_prev = t_start;
     auto trace = [&](bool seed, uint32_t nfr) {
@@ -1131,7 +1135,9 @@ __device__ void run_pass_lds(const PassSpec& P, const PassScratch& S, uint32_t* 
 #pragma unroll
             for (int u = 0; u < kSeedUnroll; ++u) {
                 const bool alive = has[u] && !lds_removed(rmv, v[u]);
-                const int32_t d = alive ? (int32_t)(cnt[v[u]] & 0xffffu) : 0;
+                int32_t d = alive ? (int32_t)(cnt[v[u]] & 0xffffu) : 0;
+                // the bound acts like a key degree that never drops
+                if (ub && alive) d = min(d, ub[v[u]]);
                 const bool seed = alive && d <= L;
                 const bool keep = alive && !seed;
                 uint32_t len = 0;
@@ -1341,6 +1347,34 @@ __global__ __launch_bounds__(kBlk, 4) void k_pass_grid(PassSpec P, PassScratch S
     run_pass<kBlk, true, kCombNone>(P, S, slots, T, lmax_out, &gs, nullptr, shrink_at, nullptr);
 }
 
+// Two independent grid passes in one launch: blocks [0, g0.nblocks) run
+// pass 0, the rest pass 1, each half with its own barrier words, slots and
+// scratch (the kmax pass beside row 0, which needs no kmaxes).
+struct PassPair {
+    PassSpec P[2];
+    PassScratch S[2];
+    uint32_t* slots[2];
+    int32_t* lmax_out[2];
+    GridSync gs[2];
+};
+
+__global__ __launch_bounds__(kBlk, 4) void k_pass_pair(PassPair pp, uint32_t shrink_at) {
+    __shared__ TileLds<kBlk> T;
+    const int h = blockIdx.x >= pp.gs[0].nblocks ? 1 : 0;
+    GridSync gs = pp.gs[h];
+    gs.rank = h ? blockIdx.x - pp.gs[0].nblocks : blockIdx.x;
+    gs.epoch = 0;
+    run_pass<kBlk, true, kCombNone>(pp.P[h], pp.S[h], pp.slots[h], T, pp.lmax_out[h], &gs,
+                                    nullptr, shrink_at, nullptr);
+}
+
+// out[r] = in[order[r]]: a row by rank.
+__global__ void k_gather_rank(const int32_t* in, const uint32_t* order, uint32_t V,
+                              int32_t* out) {
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < V; r += gridDim.x * blockDim.x)
+        out[r] = in[order[r]];
+}
+
 // Persistent rows kernel: block b belongs to team block_team[b]; a team
 // runs its passes one after another, re-synchronising between them.
 // Two workgroups of kBlk threads must be resident per CU (the launch packs
@@ -1532,6 +1566,43 @@ hipError_t launch_pass_grid(const PassSpec& p, const PassScratch& s, uint32_t* s
     const uint32_t shrink_at = 16 * kBlk;
     hipLaunchKernelGGL(k_pass_grid, dim3(gs.nblocks), dim3(kBlk), 0, st, p, s, slots, lmax_out, gs,
                        shrink_at);
+    return hipGetLastError();
+}
+
+hipError_t launch_pass_pair(const PassSpec& p0, const PassScratch& s0, uint32_t* slots0,
+                            int32_t* lmax0, const GridSync& gs0, const PassSpec& p1,
+                            const PassScratch& s1, uint32_t* slots1, int32_t* lmax1,
+                            const GridSync& gs1, hipStream_t st) {
+    const GridSync* g[2] = {&gs0, &gs1};
+    for (const GridSync* x : g) {
+        hipError_t e = hipMemsetAsync(x->w, 0, kSyncResetWords * sizeof(uint32_t), st);
+        if (e != hipSuccess) return e;
+        e = hipMemsetAsync(x->w + kSyncFlags, 0, x->nblocks * sizeof(uint32_t), st);
+        if (e != hipSuccess) return e;
+    }
+    PassPair pp{};
+    pp.P[0] = p0;
+    pp.P[1] = p1;
+    pp.S[0] = s0;
+    pp.S[1] = s1;
+    pp.slots[0] = slots0;
+    pp.slots[1] = slots1;
+    pp.lmax_out[0] = lmax0;
+    pp.lmax_out[1] = lmax1;
+    pp.gs[0] = gs0;
+    pp.gs[1] = gs1;
+    const uint32_t shrink_at = 16 * kBlk;
+    hipLaunchKernelGGL(k_pass_pair, dim3(gs0.nblocks + gs1.nblocks), dim3(kBlk), 0, st, pp,
+                       shrink_at);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_rank(const int32_t* in, const uint32_t* order, uint32_t V, int32_t* out,
+                              hipStream_t st) {
+    uint32_t blocks = (V + 255u) / 256u;
+    if (blocks > 4096u) blocks = 4096u;
+    if (blocks == 0) blocks = 1;
+    hipLaunchKernelGGL(k_gather_rank, dim3(blocks), dim3(256), 0, st, in, order, V, out);
     return hipGetLastError();
 }
 

@@ -41,7 +41,8 @@ class Timing(ctypes.Structure):
                 ("total_ms", ctypes.c_double), ("rows_launches", ctypes.c_int32),
                 ("rows_slots", ctypes.c_int32), ("rows_scheduler", ctypes.c_int32),
                 ("rows_split", ctypes.c_int32), ("rows_teams_ms", ctypes.c_float),
-                ("rows_lds_ms", ctypes.c_float), ("reserved", ctypes.c_int32 * 2)]
+                ("rows_lds_ms", ctypes.c_float), ("rows_first_ms", ctypes.c_float),
+                ("reserved", ctypes.c_int32 * 1)]
 
 
 class PassLog(ctypes.Structure):

@@ -178,7 +178,8 @@ typedef struct dcx_timing {
     int32_t rows_split;    /* batched: rows k >= rows_split ran LDS-resident */
     float rows_teams_ms;   /* batched: the team launch alone */
     float rows_lds_ms;     /* batched: the LDS-resident launch alone (beside it) */
-    int32_t reserved[2];
+    float rows_first_ms;   /* batched: row 0's grid pass, run first to bound the others */
+    int32_t reserved[1];
 } dcx_timing;
 
 dcx_status dcx_config_init(dcx_config* cfg);
