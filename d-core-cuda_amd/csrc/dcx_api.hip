@@ -136,6 +136,17 @@ void build_core(dcx_ctx* c, dcx_graph* g) {
     g->core_valid = true;
 }
 
+// Builds the core-ordered graph for the resident kmaxes unless it is
+// current; the time goes to timing.order_ms.
+void ensure_core(dcx_ctx* c, dcx_graph* g) {
+    if (g->core_valid) return;
+    HIPX(c, hipEventRecord(c->ev[7], c->stream));
+    build_core(c, g);
+    HIPX(c, hipEventRecord(c->ev[2], c->stream));
+    sync(c);
+    c->timing.order_ms += elapsed(c, 7, 2);
+}
+
 // Grows a device array to n elements keeping its first `keep` elements.
 template <class T>
 void grow_keep(dcx_ctx* c, T** p, size_t* cap, size_t n, size_t keep) {
@@ -254,7 +265,8 @@ void compute_kmax(dcx_ctx* c, dcx_graph* g, int variant = DCX_VARIANT_BATCHED,
     HIPX(c, launch_order_by_kmax(g->kmaxes, g->V, g->kmax, g->hist, g->vk, g->order, c->stream));
     HIPX(c, launch_core_edges(g->kmaxes, Csr{g->out_off, g->out_deg, g->out_nbr}, g->V, g->kmax,
                               g->ekh, c->stream));
-    build_core(c, g);
+    // the core-ordered graph is built when a pass first needs it
+    // (ensure_core): whole-graph row passes do not
     g->h_vk.assign(need, 0);
     std::vector<uint64_t> eh(need, 0);
     HIPX(c, hipMemcpyAsync(g->h_vk.data(), g->vk, need * sizeof(uint32_t), hipMemcpyDeviceToHost,
@@ -289,6 +301,7 @@ struct Sched {
     uint32_t sync_stride = 0;
 };
 constexpr uint32_t kTeamMax = 32;
+constexpr int32_t kFewRows = 4;  // batched: at most this many rows above row 0 run whole-graph
 constexpr uint32_t kShrinkAt = 4096;
 constexpr int kRowsBlocksPerCu = 2;
 
@@ -574,6 +587,14 @@ void compute_rows(dcx_ctx* c, dcx_graph* g, int variant, int32_t kr = -1) {
     grow_keep(c, &g->rows, &g->rows_cap, nrows * V, keep_rows * V);
     grow_keep(c, &g->lmax_rows, &g->lmax_rows_cap, nrows, keep_rows);
     const size_t nre = (size_t)kr + 1;
+    // Few rows on a big graph (a Chung-Lu 2^24 graph has kmax 2): with row
+    // 0 already computed beside the kmax pass, rows 1..kr run as
+    // whole-graph passes two per launch, bounded by row 0, and the
+    // core-ordered graph is never built (its sort would cost more than the
+    // rows themselves).
+    const bool few = variant == DCX_VARIANT_BATCHED && g->row0_ready && kr >= 1 &&
+                     kr <= kFewRows;
+    if (!few) ensure_core(c, g);
     uint32_t nteam_err = 0, team_err_stride = 0;
     const uint32_t* team_err = nullptr;
     clear_kernel_errors(c);
@@ -585,7 +606,52 @@ void compute_rows(dcx_ctx* c, dcx_graph* g, int variant, int32_t kr = -1) {
     int32_t split = kr + 1;  // batched: first LDS-resident row
     bool teams_ran = false, lds_ran = false, first_ran = false;
     std::vector<std::pair<uint32_t, int32_t>> last_teams;  // (size, first k) for diagnosis
-    if (variant == DCX_VARIANT_HINDEX) {
+    if (few) {
+        g->rows_first_grid = kr + 1;  // no row is a team pass
+        ensure_scratch(c, 2ull * kScratchArrays * V * 4);
+        HIPX(c, hipMemsetAsync(c->ctrs2 + kCtrSync + kSyncError, 0, 4, c->stream));
+        HIPX(c, hipEventRecord(c->ev[5], c->stream));
+        HIPX(c, hipMemcpyAsync(g->rows, g->row0, 4ull * V, hipMemcpyDeviceToDevice, c->stream));
+        HIPX(c, hipMemcpyAsync(g->pstats + kStatWords, g->row0_stats, 4ull * kStatWords,
+                               hipMemcpyDeviceToDevice, c->stream));
+        HIPX(c, hipMemcpyAsync(g->lmax_rows, g->row0_stats + kStatWords, 4,
+                               hipMemcpyDeviceToDevice, c->stream));
+        g->h_pass_variant[1] = DCX_VARIANT_BATCHED;
+        g->h_pass_team[1] = (uint16_t)c->grid_blocks;
+        HIPX(c, hipEventRecord(c->sev[3], c->stream));
+        const PassScratch s0 = scratch_at(c->scratch, V);
+        const PassScratch s1 =
+            scratch_at(static_cast<uint32_t*>(c->scratch) + (size_t)kScratchArrays * V, V);
+        auto spec = [&](int32_t k) {
+            PassSpec p = traced(c, full_row_spec(g, k, g->rows + (size_t)k * V));
+            p.trace = nullptr;
+            p.stats = g->pstats + (size_t)kStatWords * (k + 1);
+            p.ub = g->row0;
+            return p;
+        };
+        for (int32_t k = 1; k <= kr; k += 2) {
+            if (k + 1 <= kr) {
+                const int G1 = c->grid_blocks / 2, G0 = c->grid_blocks - G1;
+                GridSync g0 = gsync(c, G0), g1 = gsync(c, G1);
+                g1.w = c->ctrs2 + kCtrSync;
+                HIPX(c, launch_pass_pair(spec(k), s0, c->ctrs, g->lmax_rows + k, g0, spec(k + 1),
+                                         s1, c->ctrs2, g->lmax_rows + k + 1, g1, c->stream));
+                g->h_pass_team[k + 1] = (uint16_t)G0;
+                g->h_pass_team[k + 2] = (uint16_t)G1;
+                g->h_pass_variant[k + 2] = DCX_VARIANT_BATCHED;
+            } else {
+                HIPX(c, launch_pass_grid(spec(k), s0, c->ctrs, g->lmax_rows + k,
+                                         gsync(c, c->grid_blocks), c->stream));
+                g->h_pass_team[k + 1] = (uint16_t)c->grid_blocks;
+            }
+            g->h_pass_variant[k + 1] = DCX_VARIANT_BATCHED;
+            ++launches;
+        }
+        team_err = c->ctrs2 + kCtrSync + kSyncError;
+        nteam_err = 1;
+        team_err_stride = 1;
+        first_ran = true;
+    } else if (variant == DCX_VARIANT_HINDEX) {
         // rows in k order, each started from the row above (l_k <= l_{k-1})
         ensure_scratch(c, 2ull * kScratchArrays * V * 4);
         HIPX(c, hipEventRecord(c->ev[5], c->stream));
@@ -1299,6 +1365,7 @@ dcx_status dcx_klist(dcx_ctx* ctx, dcx_graph* g, int32_t k, int32_t variant, int
             // otherwise it runs on the whole graph exactly like KList.  The
             // h-index fixpoint needs the core, so it computes kmaxes first.
             if (variant == DCX_VARIANT_HINDEX && !g->kmax_valid) compute_kmax(ctx, g);
+            if (g->kmax_valid) ensure_core(ctx, g);
             if (g->kmax_valid && g->core_valid) {
                 HIPX(ctx, hipMemsetAsync(row, 0, 4ull * V, ctx->stream));
                 PassSpec p = core_row_spec(g, k, row);

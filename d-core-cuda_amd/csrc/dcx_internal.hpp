@@ -55,10 +55,11 @@ struct PassSpec {
     // alive vertices' lists hold at most pull_ratio times the frontier's
     // edges; 0 = always push.
     uint32_t pull_ratio;
-    // Nullable (core passes only): per rank, an upper bound of the row's
-    // value.  A member still alive at level L with ub <= L cannot be in the
-    // (k, L+1)-core, so it is seeded at L directly instead of waiting for
-    // a cascade to reach it; the values are unchanged.
+    // Nullable: per vertex (rank in the core layout, id in the full one) an
+    // upper bound of the row's value.  A vertex still alive at level L with
+    // ub <= L cannot be in the (k, L+1)-core, so it is seeded at L directly
+    // instead of waiting for a cascade to reach it; the values are
+    // unchanged.
     const int32_t* ub;
 };
 

@@ -509,7 +509,7 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
     // them choose the same kind of cascade phase.
     uint64_t pa = 0, fe = 0;
     uint32_t nLA = 0;  // long alive list length (0 when it overflowed)
-    const int32_t* const ub = P.core ? P.ub : nullptr;
+    const int32_t* const ub = P.ub;
     while (removed < nm) {
         // Each removal needs at most two level iterations (a level number
         // taken from a stale minimum seeds nothing, the next one seeds the

@@ -308,3 +308,56 @@ def test_rmat20_full(ctx):
         assert np.array_equal(g.kmaxes(), ref[1])
         assert np.array_equal(g.rows(), ref[3])
         g.free()
+
+
+@pytest.mark.parametrize("samples,seed,want_k", [(6 << 17, 9, 1), (8 << 16, 3, 2),
+                                                 (16 << 16, 3, 3), (24 << 16, 4, 4)])
+def test_few_rows_whole_graph_path(samples, seed, want_k):
+    """Graphs with kmax <= 4 and more vertices than an LDS-resident row holds:
+    row 0 runs beside the kmax pass and rows 1..kmax run as whole-graph
+    passes, two per launch, seeded with row 0 as their bound; the
+    core-ordered graph is not built.  Every row must equal the oracle's, the
+    pass log must cover them, and maintained states must equal a fresh
+    recomputation."""
+    V = 1 << 17 if want_k == 1 else 1 << 16
+    src, dst = dcx.gen_chunglu(V, samples, seed)
+    ref = oracle.decompose(oracle.csr_from_edges(V, src, dst))
+    assert ref[0] == want_k
+    say(f"chunglu V={V} samples={samples} seed={seed} kmax={want_k} batched")
+    with dcx.Context(0) as c:
+        g = c.upload_edges(V, src, dst, slack=4)
+        K, lmax = g.decompose(dcx.VARIANT_BATCHED)
+        assert K == ref[0] and lmax == ref[2]
+        assert np.array_equal(g.kmaxes(), ref[1]) and np.array_equal(g.rows(), ref[3])
+        log = g.pass_log()
+        assert len(log) == K + 2
+        for k in range(K + 1):
+            assert log[k + 1]["team"] > 1 and log[k + 1]["phases"] >= 1
+        # a single row through the same context still matches (the core
+        # graph is built on demand for it)
+        say(f"chunglu V={V} klist k=1 after the whole-graph path")
+        lk, lv = g.klist(1, dcx.VARIANT_PEEL)
+        assert np.array_equal(lv, ref[3][1])
+        rng = np.random.default_rng(seed)
+        for step in range(3):
+            say(f"chunglu V={V} update step {step}")
+            e = np.stack([src, dst], axis=1)
+            dels = e[rng.choice(len(e), 50, replace=False)]
+            have = set(zip(src.tolist(), dst.tolist()))
+            ins = []
+            while len(ins) < 80:
+                a, b = (int(x) for x in rng.integers(0, V // 64, 2))
+                if a != b and (a, b) not in have:
+                    have.add((a, b))
+                    ins.append((a, b))
+            ins = np.array(ins, np.uint32)
+            g.update(ins=ins, dels=dels)
+            dset = set(map(tuple, dels.tolist()))
+            keep = np.array([(a, b) not in dset for a, b in e.tolist()])
+            src = np.concatenate([src[keep], ins[:, 0]]).astype(np.uint32)
+            dst = np.concatenate([dst[keep], ins[:, 1]]).astype(np.uint32)
+            ref = oracle.decompose(oracle.csr_from_edges(V, src, dst))
+            assert g.info()[2:] == (ref[0], ref[2])
+            assert np.array_equal(g.kmaxes(), ref[1])
+            assert np.array_equal(g.rows(), ref[3])
+        g.free()
