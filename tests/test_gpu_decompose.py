@@ -361,3 +361,32 @@ def test_few_rows_whole_graph_path(samples, seed, want_k):
             assert np.array_equal(g.kmaxes(), ref[1])
             assert np.array_equal(g.rows(), ref[3])
         g.free()
+
+
+def test_kmax_and_row0_pair():
+    """The kmax pass and row 0 run side by side in one launch, each on half
+    of the grid with its own barrier words and slots, whenever the graph is
+    too big for LDS-resident rows -- forced here for small graphs with
+    lds_members = -1 -- and row 0 then bounds every other row.  kmaxes, row
+    0 and every other row must equal the oracle's, on graphs with hubs,
+    stars, isolated vertices, cycles and complete digraphs, repeatedly."""
+    cases = [("hubs", *hub_graph()), ("out_star", *graphs.out_star(5000)),
+             ("in_star", *graphs.in_star(5000)), ("cycle", *graphs.cycle(3000)),
+             ("complete40", *graphs.complete(40)), ("bi_kab", *graphs.bi_kab(30, 1200))]
+    s, d = dcx.gen_rmat(15, 16 << 15, 6)
+    cases.append(("rmat15", 1 << 15, s.astype(np.int64), d.astype(np.int64)))
+    s, d = dcx.gen_er(1 << 14, 5 << 14, 8)
+    cases.append(("er_isolated", 1 << 15, s.astype(np.int64), d.astype(np.int64)))
+    with dcx.Context(0, lds_members=-1) as c:
+        for name, V, src, dst in cases:
+            ref = oracle.decompose(oracle.csr_from_edges(V, src, dst))
+            g = c.upload_edges(V, src, dst)
+            for rep in range(3):
+                say(f"{name} kmax/row-0 pair repeat {rep}")
+                K, lmax = g.decompose(dcx.VARIANT_BATCHED)
+                assert K == ref[0] and lmax == ref[2], name
+                assert np.array_equal(g.kmaxes(), ref[1]), name
+                assert np.array_equal(g.rows(), ref[3]), name
+            log = g.pass_log()
+            assert log[0]["phases"] >= 2 and log[1]["phases"] >= 2
+            g.free()
