@@ -619,6 +619,10 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
         removed += nF + nL;
 
         // ---- cascade inside level L until no vertex crosses a threshold
+        // (none when the seeds took every alive vertex: their decrements
+        // would only reach removed ones -- at the last level of a dense
+        // core that is millions of list entries read for nothing)
+        if (nA == 0) nF = nL = 0;
         while (nF + nL > 0) {
             uint32_t* nxt = slots + kSlotStride * (s % 3);
             if (tw == 0) slot_reset(slots + kSlotStride * ((s + 1) % 3), tid);
@@ -1168,6 +1172,7 @@ __device__ void run_pass_lds(const PassSpec& P, const PassScratch& S, uint32_t* 
         ++s;
         removed += nF;
 
+        if (nA == 0) nF = 0;  // the seeds took every alive vertex
         while (nF > 0) {
             uint32_t* nxt = slots + kSlotStride * (s & 1);
             uint32_t* nx2 = slots + kSlotStride * ((s + 1) & 1);
