@@ -46,7 +46,7 @@ dcx_status dcx_autotune(dcx_ctx* ctx, dcx_graph* g, int32_t reps, dcx_tune_resul
         const int32_t slot_opts[] = {cus, 2 * cus};
         const int32_t team_opts[] = {4, 8, 16, 32, 64, 128, 256};
         const int32_t shrink_opts[] = {1024, 2048, 4096, 8192, 16384};
-        const uint32_t lds_opts[] = {ctx->lds_cap_max, 16384, 8192, 4096, 0};
+        const uint32_t lds_opts[] = {ctx->lds_cap_max, 28672, 24576, 20480, 16384, 8192, 0};
         TuneShape best{2 * cus, (int32_t)kTeamMax, (int32_t)kShrinkAt, ctx->lds_cap};
         int tried = 0;
         const double def_ms = time_rows_shape(ctx, g, best, reps);
