@@ -54,6 +54,8 @@ constexpr uint32_t kCombEmpty = 0xffffffffu;
 // edges in flight per thread in the LDS-resident pass (its atomics are
 // cheap, so the list reads' latency is what remains to hide)
 constexpr int kLdsUnroll = 8;
+// packed bound of an LDS-resident member meaning "none below 2^16 - 1"
+constexpr uint32_t kLdsUbNone = 0xffffu;
 constexpr int32_t kPoison = INT_MIN / 2;
 
 __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (W - 1)); }
@@ -1127,21 +1129,33 @@ __device__ void run_pass_lds(const PassSpec& P, const PassScratch& S, uint32_t* 
             *slot_u64(other, 4) = 0;
         }
         uint32_t mn = UINT_MAX;
+        // Alive-list entries carry the member's bound in their high half
+        // (members are < 2^15; kLdsUbNone: no bound below 2^16 - 1), so
+        // later levels read it with the entry instead of from the bound
+        // array.
         for (uint32_t base = 0; base < nA; base += BLK * kSeedUnroll) {
-            uint32_t v[kSeedUnroll];
+            uint32_t v[kSeedUnroll], b[kSeedUnroll];
             bool has[kSeedUnroll];
 #pragma unroll
             for (int u = 0; u < kSeedUnroll; ++u) {
                 const uint32_t i = base + u * BLK + tid;
                 has[u] = i < nA;
-                v[u] = has[u] ? (A ? A[i] : i) : 0u;
+                uint32_t e = 0;
+                if (has[u]) {
+                    if (A)
+                        e = A[i];
+                    else
+                        e = i | (ub ? (uint32_t)min(ub[i], (int32_t)kLdsUbNone) : kLdsUbNone) << 16;
+                }
+                v[u] = e & 0xffffu;
+                b[u] = e >> 16;
             }
 #pragma unroll
             for (int u = 0; u < kSeedUnroll; ++u) {
                 const bool alive = has[u] && !lds_removed(rmv, v[u]);
                 int32_t d = alive ? (int32_t)(cnt[v[u]] & 0xffffu) : 0;
                 // the bound acts like a key degree that never drops
-                if (ub && alive) d = min(d, ub[v[u]]);
+                if (b[u] != kLdsUbNone) d = min(d, (int32_t)b[u]);
                 const bool seed = alive && d <= L;
                 const bool keep = alive && !seed;
                 uint32_t len = 0;
@@ -1152,7 +1166,7 @@ __device__ void run_pass_lds(const PassSpec& P, const PassScratch& S, uint32_t* 
                 }
                 if (keep) mn = min(mn, (uint32_t)d);
                 wave_append_len(seed, v[u], len, F, &cur[0], slot_u64(cur, 4));
-                wave_append(keep, v[u], B, &cur[1]);
+                wave_append(keep, v[u] | (b[u] << 16), B, &cur[1]);
             }
         }
         wave_min_to(mn, &cur[2]);
