@@ -575,6 +575,7 @@ void compute_rows(dcx_ctx* c, dcx_graph* g, int variant, int32_t kr = -1) {
     const bool had_rows = g->rows_valid;
     const int32_t old_rows = had_rows ? g->rows_k : 0;
     g->rows_valid = false;
+    g->rows_first_grid = 0;  // set below by the batched paths
     if (K < 0) {
         g->lmax = 0;
         g->rows_k = 0;
