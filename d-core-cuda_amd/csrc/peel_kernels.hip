@@ -860,10 +860,9 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
                 // leave most of them idle while a few walk long edge lists).
                 const uint32_t ipw =
                     solo ? (uint32_t)BLK : min((uint32_t)BLK, (nF + tn - 1) / tn);
-                for (uint32_t base = tw * ipw; base < nF; base += tn * ipw) {
-                    const uint32_t i = base + tid;
-                    const bool has = tid < ipw && i < nF;
-                    const uint32_t v = has ? ld<MULTI>(&F[i]) : 0u;
+                // one tile: up to ipw frontier vertices (`has`, v per
+                // thread, `items` of them), their lists edge-parallel
+                auto push_tile = [&](bool has, uint32_t v, uint32_t items) {
                     __syncthreads();
                     uint32_t kl = 0, cl = 0;
                     if (has) {
@@ -872,7 +871,7 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
                         list_lens(P, S, v, kl, cl);
                     }
                     T.klen[tid] = kl;
-                    if (tid == 0) T.nitems = min(nF - base, ipw);
+                    if (tid == 0) T.nitems = items;
                     T.scan[tid] = block_excl_scan<BLK>(kl + cl, T.wsum, &T.total);
                     // every thread's prefix is in LDS before any thread searches it
                     __syncthreads();
@@ -898,6 +897,11 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
                         }
                         visit(x, ks, ok);
                     }
+                };
+                for (uint32_t base = tw * ipw; base < nF; base += tn * ipw) {
+                    const uint32_t i = base + tid;
+                    const bool has = tid < ipw && i < nF;
+                    push_tile(has, has ? ld<MULTI>(&F[i]) : 0u, min(nF - base, ipw));
                 }
                 // long lists: every member walks its slice of each of them
                 for (uint32_t li = 0; li < nL; ++li) {
