@@ -6,6 +6,8 @@
 
 namespace {
 
+constexpr int32_t kFewRows = 4;  // batched: at most this many rows above row 0 run whole-graph
+
 // kmax pass: the peel with in and out swapped and k = 0 (swapInOut,
 // src/common.h:114-120), whole graph, output straight into kmaxes.
 PassSpec kmax_spec(const dcx_graph* g) {
@@ -263,10 +265,19 @@ void compute_kmax(dcx_ctx* c, dcx_graph* g, int variant = DCX_VARIANT_BATCHED,
     }
     if (!g->order) dev_alloc(c, &g->order, g->V);
     HIPX(c, launch_order_by_kmax(g->kmaxes, g->V, g->kmax, g->hist, g->vk, g->order, c->stream));
+    // The E_k histogram runs on the side stream beside the core-ordered
+    // graph's build (both need only the kmaxes and the order).  The core is
+    // built here unless the rows will be whole-graph passes (few rows after
+    // row 0); otherwise when a pass first needs it (ensure_core).
+    HIPX(c, hipEventRecord(c->fork, c->stream));
+    HIPX(c, hipStreamWaitEvent(c->side, c->fork, 0));
     HIPX(c, launch_core_edges(g->kmaxes, Csr{g->out_off, g->out_deg, g->out_nbr}, g->V, g->kmax,
-                              g->ekh, c->stream));
-    // the core-ordered graph is built when a pass first needs it
-    // (ensure_core): whole-graph row passes do not
+                              g->ekh, c->side));
+    HIPX(c, hipEventRecord(c->join, c->side));
+    const bool few_rows = variant == DCX_VARIANT_BATCHED && g->row0_ready && g->kmax >= 1 &&
+                          g->kmax <= kFewRows;
+    if (!few_rows) build_core(c, g);
+    HIPX(c, hipStreamWaitEvent(c->stream, c->join, 0));
     g->h_vk.assign(need, 0);
     std::vector<uint64_t> eh(need, 0);
     HIPX(c, hipMemcpyAsync(g->h_vk.data(), g->vk, need * sizeof(uint32_t), hipMemcpyDeviceToHost,
@@ -301,7 +312,6 @@ struct Sched {
     uint32_t sync_stride = 0;
 };
 constexpr uint32_t kTeamMax = 32;
-constexpr int32_t kFewRows = 4;  // batched: at most this many rows above row 0 run whole-graph
 constexpr uint32_t kShrinkAt = 4096;
 constexpr int kRowsBlocksPerCu = 2;
 
