@@ -100,8 +100,9 @@ def main():
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus > 1 or world > 1:
-        print("bench.py: this build runs on one GPU; the decomposition would shard by "
-              "k-ranges across GPUs (see DESIGN.md), which is not built or measured here.",
+        print("bench.py: runs on one GPU only. Across GPUs the decomposition would shard "
+              "by k-ranges (each GPU a replica of the core-ordered graph and its own kmax "
+              "pass), which is not built or measured.",
               file=sys.stderr)
         sys.exit(2)
     if args.workload == "chunglu256":
