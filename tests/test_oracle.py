@@ -1,4 +1,4 @@
-"""Pins the CPU oracle before it is trusted as the judge (CPU only).
+"""Pins the CPU oracle before it is trusted as the reference of every GPU test (CPU only).
 
 Checks, each independent of the oracle's bucket-queue peel:
   * the reference's golden vector dataset/digraph(.decomp);
