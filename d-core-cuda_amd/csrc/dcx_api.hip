@@ -7,6 +7,7 @@
 namespace {
 
 constexpr int32_t kFewRows = 4;  // batched: at most this many rows above row 0 run whole-graph
+constexpr int kRow0Eighths = 2;   // share of the grid row 0's pass takes beside the kmax pass
 
 // kmax pass: the peel with in and out swapped and k = 0 (swapInOut,
 // src/common.h:114-120), whole graph, output straight into kmaxes.
@@ -175,7 +176,7 @@ void grow_keep(dcx_ctx* c, T** p, size_t* cap, size_t n, size_t keep) {
 // and lmax in g->row0_stats; returns kmax.
 int32_t run_kmax_with_row0(dcx_ctx* c, dcx_graph* g) {
     const uint32_t V = g->V;
-    const int G1 = c->grid_blocks / 2, G0 = c->grid_blocks - G1;
+    const int G1 = c->grid_blocks * kRow0Eighths / 8, G0 = c->grid_blocks - G1;
     ensure_scratch(c, 2ull * kScratchArrays * V * 4);
     if (!c->ctrs2) {
         dev_alloc(c, &c->ctrs2, kCtrWords);
@@ -191,18 +192,25 @@ int32_t run_kmax_with_row0(dcx_ctx* c, dcx_graph* g) {
     g1.w = c->ctrs2 + kCtrSync;
     clear_kernel_errors(c);
     HIPX(c, hipMemsetAsync(g1.w + kSyncError, 0, 4, c->stream));
-    HIPX(c, hipEventRecord(c->ev[0], c->stream));
-    HIPX(c, launch_pass_pair(p0, scratch_at(c->scratch, V), c->ctrs, c->d_scalars, g0, p1,
+    // row 0 on a stream of its own: it goes on beside the order kernels
+    // once the kmax pass is done (joined in compute_kmax)
+    HIPX(c, hipEventRecord(c->fork, c->stream));
+    HIPX(c, hipStreamWaitEvent(c->side2, c->fork, 0));
+    HIPX(c, launch_pass_grid(p1,
                              scratch_at(static_cast<uint32_t*>(c->scratch) +
                                             (size_t)kScratchArrays * V,
                                         V),
                              c->ctrs2, reinterpret_cast<int32_t*>(g->row0_stats + kStatWords),
-                             g1, c->stream));
+                             g1, c->side2));
+    HIPX(c, hipEventRecord(c->row0_done, c->side2));
+    g->row0_pending = true;
+    HIPX(c, hipEventRecord(c->ev[0], c->stream));
+    HIPX(c, launch_pass_grid(p0, scratch_at(c->scratch, V), c->ctrs, c->d_scalars, g0,
+                             c->stream));
     HIPX(c, hipEventRecord(c->ev[1], c->stream));
     HIPX(c, hipMemcpyAsync(c->h_scalars, c->d_scalars, sizeof(int32_t), hipMemcpyDeviceToHost,
                            c->stream));
-    check_kernel_errors(c, "kmax pass beside row 0", g1.w + kSyncError, 1, 1);
-    g->row0_ready = true;
+    check_kernel_errors(c, "kmax pass");
     return c->h_scalars[0];
 }
 
@@ -221,6 +229,7 @@ void compute_kmax(dcx_ctx* c, dcx_graph* g, int variant = DCX_VARIANT_BATCHED,
     // values (an upper bound) and settles in a few sweeps.
     const bool use_peel = variant == DCX_VARIANT_PEEL || (variant != DCX_VARIANT_HINDEX && !warm);
     g->row0_ready = false;
+    g->row0_pending = false;
     if (use_peel) {
         // the batched rows start from row 0 when it is a team row (it bounds
         // every other row): compute it beside the kmax pass
@@ -274,10 +283,11 @@ void compute_kmax(dcx_ctx* c, dcx_graph* g, int variant = DCX_VARIANT_BATCHED,
     HIPX(c, launch_core_edges(g->kmaxes, Csr{g->out_off, g->out_deg, g->out_nbr}, g->V, g->kmax,
                               g->ekh, c->side));
     HIPX(c, hipEventRecord(c->join, c->side));
-    const bool few_rows = variant == DCX_VARIANT_BATCHED && g->row0_ready && g->kmax >= 1 &&
+    const bool few_rows = variant == DCX_VARIANT_BATCHED && g->row0_pending && g->kmax >= 1 &&
                           g->kmax <= kFewRows;
     if (!few_rows) build_core(c, g);
     HIPX(c, hipStreamWaitEvent(c->stream, c->join, 0));
+    if (g->row0_pending) HIPX(c, hipStreamWaitEvent(c->stream, c->row0_done, 0));
     g->h_vk.assign(need, 0);
     std::vector<uint64_t> eh(need, 0);
     HIPX(c, hipMemcpyAsync(g->h_vk.data(), g->vk, need * sizeof(uint32_t), hipMemcpyDeviceToHost,
@@ -294,6 +304,21 @@ void compute_kmax(dcx_ctx* c, dcx_graph* g, int variant = DCX_VARIANT_BATCHED,
     }
     c->timing.kmax_ms = elapsed(c, 0, 1);
     c->timing.order_ms = elapsed(c, 7, 2);
+    if (g->row0_pending) {
+        g->row0_pending = false;
+        // row 0's pass has ended (the sync above waited for it)
+        HIPX(c, hipMemcpy(c->h_scalars + 34, c->ctrs2 + kCtrSync + kSyncError, 4,
+                          hipMemcpyDeviceToHost));
+        if (c->h_scalars[34] != 0) {
+            set_err(c, "row 0 pass: %s (code %d)",
+                    c->h_scalars[34] == (int32_t)kErrTimeout ? "a barrier timed out"
+                                                             : "kernel counters inconsistent",
+                    c->h_scalars[34]);
+            throw Fail{c->h_scalars[34] == (int32_t)kErrTimeout ? DCX_ERR_TIMEOUT
+                                                                 : DCX_ERR_INTERNAL};
+        }
+        g->row0_ready = true;
+    }
     g->kmax_valid = true;
 }
 
@@ -1058,6 +1083,9 @@ dcx_status dcx_ctx_create(const dcx_config* cfg, dcx_ctx** out) {
         return DCX_ERR_HIP;
     for (auto& e : c->ev)
         if (hipEventCreate(&e) != hipSuccess) return DCX_ERR_HIP;
+    if (hipStreamCreateWithFlags(&c->side2, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->row0_done, hipEventDisableTiming) != hipSuccess)
+        return DCX_ERR_HIP;
     if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->join, hipEventDisableTiming) != hipSuccess)
@@ -1095,6 +1123,11 @@ dcx_status dcx_ctx_destroy(dcx_ctx* c) {
         if (e) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     if (c->side) (void)hipStreamDestroy(c->side);
+    if (c->side2) {
+        (void)hipStreamSynchronize(c->side2);
+        (void)hipStreamDestroy(c->side2);
+    }
+    if (c->row0_done) (void)hipEventDestroy(c->row0_done);
     if (c->fork) (void)hipEventDestroy(c->fork);
     if (c->join) (void)hipEventDestroy(c->join);
     for (auto& e : c->sev)

@@ -35,6 +35,10 @@ struct dcx_ctx {
     // beside the team launch; fork/join order it against `stream`
     hipStream_t side = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
+    // third stream (default priority): row 0's pass beside the kmax pass
+    // and the order kernels; row0_done marks its end
+    hipStream_t side2 = nullptr;
+    hipEvent_t row0_done = nullptr;
     // [0], [1]: brackets of the LDS-resident launch on `side`; [3], [2]:
     // start and end of the team launch on `stream` (ev[5] opens the rows
     // kernels, row 0's grid pass first when the other rows use its bound)
@@ -137,6 +141,7 @@ struct dcx_graph {
     int32_t* row0 = nullptr;
     uint32_t* row0_stats = nullptr;
     bool row0_ready = false;
+    bool row0_pending = false;  // row 0's pass queued on side2, not yet joined
     uint32_t* pstats = nullptr;  // [(kmax+2) * kStatWords] device pass log: kmax pass, rows
     size_t pstats_cap = 0;
     std::vector<uint32_t> h_pstats;      // host copy of pstats after the last rows launch
@@ -188,6 +193,9 @@ dcx_status guarded(dcx_ctx* c, F&& f) {
         }
         return f();
     } catch (const Fail& x) {
+        // a pass queued on the second side stream may still run on the
+        // scratch arrays: let it end before the next call reuses them
+        if (c->side2) (void)hipStreamSynchronize(c->side2);
         return x.st;
     } catch (const std::bad_alloc&) {
         set_err(c, "host allocation failed");
