@@ -294,6 +294,54 @@ def test_pass_log(ctx, variant):
             assert e["levels"] >= 1
 
 
+def test_core_edge_histogram(ctx):
+    """E_k (edges whose ends both have kmax >= k) from the device histogram
+    equals a count over the edge list, on graphs whose out-lists are long
+    (hubs, stars), empty (isolated vertices) or spread over many vertices
+    per wavefront, and after edits have left gaps and relocated lists."""
+    cases = [("hubs", *hub_graph()), ("out_star", *graphs.out_star(5000)),
+             ("in_star", *graphs.in_star(5000)), ("complete40", *graphs.complete(40))]
+    s, d = dcx.gen_er(1 << 14, 5 << 14, 8)
+    cases.append(("er_isolated", (1 << 15) + 37, s.astype(np.int64), d.astype(np.int64)))
+    s, d = dcx.gen_rmat(15, 16 << 15, 6)
+    cases.append(("rmat15", 1 << 15, s.astype(np.int64), d.astype(np.int64)))
+
+    def check(name, g, V, src, dst):
+        K, _ = g.decompose(dcx.VARIANT_BATCHED)
+        km = g.kmaxes().astype(np.int64)
+        m = np.minimum(km[np.asarray(src, np.int64)], km[np.asarray(dst, np.int64)])
+        want = np.bincount(m, minlength=K + 1)[::-1].cumsum()[::-1]
+        log = g.pass_log()
+        got = np.array([log[k + 1]["edges"] for k in range(K + 1)])
+        assert np.array_equal(got, want), name
+
+    rng = np.random.default_rng(3)
+    for name, V, src, dst in cases:
+        say(f"{name} E_k histogram")
+        g = ctx.upload_edges(V, src, dst)
+        check(name, g, V, src, dst)
+        if name == "rmat15":
+            for step in range(2):
+                say(f"{name} E_k histogram after edit batch {step}")
+                e = np.stack([np.asarray(src), np.asarray(dst)], axis=1)
+                dels = e[rng.choice(len(e), 2000, replace=False)]
+                have = set(zip(np.asarray(src).tolist(), np.asarray(dst).tolist()))
+                ins = []
+                while len(ins) < 3000:
+                    a, b = (int(x) for x in rng.integers(0, 512, 2))
+                    if a != b and (a, b) not in have:
+                        have.add((a, b))
+                        ins.append((a, b))
+                ins = np.array(ins, np.uint32)
+                g.update(ins=ins, dels=dels.astype(np.uint32))
+                dset = set(map(tuple, dels.tolist()))
+                keep = np.array([(a, b) not in dset for a, b in e.tolist()])
+                src = np.concatenate([np.asarray(src)[keep], ins[:, 0]])
+                dst = np.concatenate([np.asarray(dst)[keep], ins[:, 1]])
+                check(f"{name} edit {step}", g, V, src, dst)
+        g.free()
+
+
 def test_rmat20_full(ctx):
     """The headline graph: R-MAT scale 20, 16*2^20 samples, seed 1."""
     s, d = dcx.gen_rmat(20, 16 << 20, seed=1)
