@@ -7,7 +7,6 @@
 namespace {
 
 constexpr int32_t kFewRows = 4;  // batched: at most this many rows above row 0 run whole-graph
-constexpr int kRow0Eighths = 2;   // share of the grid row 0's pass takes beside the kmax pass
 
 // kmax pass: the peel with in and out swapped and k = 0 (swapInOut,
 // src/common.h:114-120), whole graph, output straight into kmaxes.
@@ -176,7 +175,7 @@ void grow_keep(dcx_ctx* c, T** p, size_t* cap, size_t n, size_t keep) {
 // and lmax in g->row0_stats; returns kmax.
 int32_t run_kmax_with_row0(dcx_ctx* c, dcx_graph* g) {
     const uint32_t V = g->V;
-    const int G1 = c->grid_blocks * kRow0Eighths / 8, G0 = c->grid_blocks - G1;
+    const int G1 = std::max(1, c->grid_blocks * g->row0_eighths / 8), G0 = c->grid_blocks - G1;
     ensure_scratch(c, 2ull * kScratchArrays * V * 4);
     if (!c->ctrs2) {
         dev_alloc(c, &c->ctrs2, kCtrWords);
@@ -287,7 +286,10 @@ void compute_kmax(dcx_ctx* c, dcx_graph* g, int variant = DCX_VARIANT_BATCHED,
                           g->kmax <= kFewRows;
     if (!few_rows) build_core(c, g);
     HIPX(c, hipStreamWaitEvent(c->stream, c->join, 0));
-    if (g->row0_pending) HIPX(c, hipStreamWaitEvent(c->stream, c->row0_done, 0));
+    if (g->row0_pending) {
+        HIPX(c, hipEventRecord(c->order_done, c->stream));
+        HIPX(c, hipStreamWaitEvent(c->stream, c->row0_done, 0));
+    }
     g->h_vk.assign(need, 0);
     std::vector<uint64_t> eh(need, 0);
     HIPX(c, hipMemcpyAsync(g->h_vk.data(), g->vk, need * sizeof(uint32_t), hipMemcpyDeviceToHost,
@@ -318,6 +320,14 @@ void compute_kmax(dcx_ctx* c, dcx_graph* g, int variant = DCX_VARIANT_BATCHED,
                                                                  : DCX_ERR_INTERNAL};
         }
         g->row0_ready = true;
+        // next run's share: row 0 should end while the order kernels run
+        float r_end = 0, o_end = 0;
+        HIPX(c, hipEventElapsedTime(&r_end, c->ev[0], c->row0_done));
+        HIPX(c, hipEventElapsedTime(&o_end, c->ev[0], c->order_done));
+        if (r_end > o_end && g->row0_eighths < 4)
+            ++g->row0_eighths;  // the library stream waited for row 0
+        else if (r_end < c->timing.kmax_ms && g->row0_eighths > 1)
+            --g->row0_eighths;  // row 0 ended before the kmax pass
     }
     g->kmax_valid = true;
 }
@@ -1084,7 +1094,7 @@ dcx_status dcx_ctx_create(const dcx_config* cfg, dcx_ctx** out) {
     for (auto& e : c->ev)
         if (hipEventCreate(&e) != hipSuccess) return DCX_ERR_HIP;
     if (hipStreamCreateWithFlags(&c->side2, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&c->row0_done, hipEventDisableTiming) != hipSuccess)
+        hipEventCreate(&c->row0_done) != hipSuccess || hipEventCreate(&c->order_done) != hipSuccess)
         return DCX_ERR_HIP;
     if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess ||
@@ -1128,6 +1138,7 @@ dcx_status dcx_ctx_destroy(dcx_ctx* c) {
         (void)hipStreamDestroy(c->side2);
     }
     if (c->row0_done) (void)hipEventDestroy(c->row0_done);
+    if (c->order_done) (void)hipEventDestroy(c->order_done);
     if (c->fork) (void)hipEventDestroy(c->fork);
     if (c->join) (void)hipEventDestroy(c->join);
     for (auto& e : c->sev)

@@ -36,9 +36,10 @@ struct dcx_ctx {
     hipStream_t side = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
     // third stream (default priority): row 0's pass beside the kmax pass
-    // and the order kernels; row0_done marks its end
+    // and the order kernels; row0_done marks its end (timed), order_done
+    // the end of the order kernels on `stream`
     hipStream_t side2 = nullptr;
-    hipEvent_t row0_done = nullptr;
+    hipEvent_t row0_done = nullptr, order_done = nullptr;
     // [0], [1]: brackets of the LDS-resident launch on `side`; [3], [2]:
     // start and end of the team launch on `stream` (ev[5] opens the rows
     // kernels, row 0's grid pass first when the other rows use its bound)
@@ -142,6 +143,11 @@ struct dcx_graph {
     uint32_t* row0_stats = nullptr;
     bool row0_ready = false;
     bool row0_pending = false;  // row 0's pass queued on side2, not yet joined
+    // Eighths of the grid row 0's pass takes beside the kmax pass, adapted
+    // after every such run: both passes are chains of dependent phases on
+    // power-law graphs (few workgroups suffice) but throughput-bound on
+    // graphs with few levels (half each).
+    int32_t row0_eighths = 2;
     uint32_t* pstats = nullptr;  // [(kmax+2) * kStatWords] device pass log: kmax pass, rows
     size_t pstats_cap = 0;
     std::vector<uint32_t> h_pstats;      // host copy of pstats after the last rows launch
