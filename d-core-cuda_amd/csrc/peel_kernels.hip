@@ -1511,6 +1511,7 @@ __global__ void k_kmax_scatter(const int32_t* kmaxes, uint32_t V, int32_t kmax, 
 }
 
 // hist[m] += edges whose endpoints' smaller kmax is m (one wave per vertex).
+constexpr int kCoreEdgesUnroll = 4;
 __global__ void k_core_edges(const int32_t* kmaxes, Csr out, uint32_t V, int32_t kmax,
                              unsigned long long* hist) {
     __shared__ unsigned long long h[kHistLds];
@@ -1523,23 +1524,36 @@ __global__ void k_core_edges(const int32_t* kmaxes, Csr out, uint32_t V, int32_t
     for (uint32_t v = wave; v < V; v += nw) {
         const int32_t kv = kmaxes[v];
         const uint32_t o = out.off[v], n = (uint32_t)out.deg[v];
-        for (uint32_t jb = 0; jb < n; jb += W) {
-            const uint32_t j = jb + lane_id();
-            const int32_t m = j < n ? min(kv, kmaxes[out.nbr[o + j]]) : -1;
-            // most of a list's edges end at a vertex of kmax >= kv, i.e. in
-            // the same bin: one add per wavefront for those
-            const unsigned long long same = __ballot(m == kv);
-            if (same && lane_id() == __ffsll(same) - 1) {
-                if (lds)
-                    atomicAdd(&h[kv], (unsigned long long)__popcll(same));
-                else
-                    atomicAdd(&hist[kv], (unsigned long long)__popcll(same));
+        // kCoreEdgesUnroll slices of a long list per round: their loads are
+        // independent, so a hub's list is a chain of n / 256 round trips
+        for (uint32_t jb = 0; jb < n; jb += kCoreEdgesUnroll * W) {
+            uint32_t u[kCoreEdgesUnroll];
+#pragma unroll
+            for (int r = 0; r < kCoreEdgesUnroll; ++r) {
+                const uint32_t j = jb + (uint32_t)(r * W + lane_id());
+                u[r] = j < n ? out.nbr[o + j] : 0u;
             }
-            if (m >= 0 && m != kv) {
-                if (lds)
-                    atomicAdd(&h[m], 1ull);
-                else
-                    atomicAdd(&hist[m], 1ull);
+            int32_t m[kCoreEdgesUnroll];
+#pragma unroll
+            for (int r = 0; r < kCoreEdgesUnroll; ++r)
+                m[r] = jb + (uint32_t)(r * W + lane_id()) < n ? min(kv, kmaxes[u[r]]) : -1;
+#pragma unroll
+            for (int r = 0; r < kCoreEdgesUnroll; ++r) {
+                // most of a list's edges end at a vertex of kmax >= kv, i.e.
+                // in the same bin: one add per wavefront for those
+                const unsigned long long same = __ballot(m[r] == kv);
+                if (same && lane_id() == __ffsll(same) - 1) {
+                    if (lds)
+                        atomicAdd(&h[kv], (unsigned long long)__popcll(same));
+                    else
+                        atomicAdd(&hist[kv], (unsigned long long)__popcll(same));
+                }
+                if (m[r] >= 0 && m[r] != kv) {
+                    if (lds)
+                        atomicAdd(&h[m[r]], 1ull);
+                    else
+                        atomicAdd(&hist[m[r]], 1ull);
+                }
             }
         }
     }
