@@ -214,7 +214,7 @@ int32_t run_kmax_with_row0(dcx_ctx* c, dcx_graph* g) {
 }
 
 void compute_kmax(dcx_ctx* c, dcx_graph* g, int variant = DCX_VARIANT_BATCHED,
-                  bool warm = false) {
+                  bool warm = false, bool clear_rows = false) {
     if (!g->kmaxes) dev_alloc(c, &g->kmaxes, g->V);
     g->core_valid = false;
     if (g->V == 0) {
@@ -229,12 +229,23 @@ void compute_kmax(dcx_ctx* c, dcx_graph* g, int variant = DCX_VARIANT_BATCHED,
     const bool use_peel = variant == DCX_VARIANT_PEEL || (variant != DCX_VARIANT_HINDEX && !warm);
     g->row0_ready = false;
     g->row0_pending = false;
+    g->rows_cleared = 0;
     if (use_peel) {
         // the batched rows start from row 0 when it is a team row (it bounds
         // every other row): compute it beside the kmax pass
         const bool pair = variant == DCX_VARIANT_BATCHED && g->V > c->lds_cap &&
                           c->grid_blocks >= 2 * (int)kXcds;
         g->kmax = pair ? run_kmax_with_row0(c, g) : run_grid_pass(c, kmax_spec(g), 0, 1);
+        if (pair && clear_rows && g->kmax >= 0) {
+            // every row follows: clear them on the second stream after row
+            // 0, beside the order kernels
+            const size_t cells = ((size_t)g->kmax + 1) * g->V;
+            g->rows_valid = false;
+            grow_keep(c, &g->rows, &g->rows_cap, cells, 0);
+            HIPX(c, hipMemsetAsync(g->rows, 0, cells * sizeof(int32_t), c->side2));
+            HIPX(c, hipEventRecord(c->clear_done, c->side2));
+            g->rows_cleared = cells;
+        }
     } else {
         const int32_t* prev = nullptr;
         if (warm && g->kmax_valid) {
@@ -290,6 +301,7 @@ void compute_kmax(dcx_ctx* c, dcx_graph* g, int variant = DCX_VARIANT_BATCHED,
         HIPX(c, hipEventRecord(c->order_done, c->stream));
         HIPX(c, hipStreamWaitEvent(c->stream, c->row0_done, 0));
     }
+    if (g->rows_cleared) HIPX(c, hipStreamWaitEvent(c->stream, c->clear_done, 0));
     g->h_vk.assign(need, 0);
     std::vector<uint64_t> eh(need, 0);
     HIPX(c, hipMemcpyAsync(g->h_vk.data(), g->vk, need * sizeof(uint32_t), hipMemcpyDeviceToHost,
@@ -647,7 +659,9 @@ void compute_rows(dcx_ctx* c, dcx_graph* g, int variant, int32_t kr = -1) {
     if (!c->fail_rec) dev_alloc(c, &c->fail_rec, kFailWords);
     HIPX(c, hipMemsetAsync(c->fail_rec, 0, kFailWords * 4, c->stream));
     HIPX(c, hipEventRecord(c->ev[3], c->stream));
-    HIPX(c, hipMemsetAsync(g->rows, 0, nre * V * sizeof(int32_t), c->stream));
+    if (g->rows_cleared != nre * V)
+        HIPX(c, hipMemsetAsync(g->rows, 0, nre * V * sizeof(int32_t), c->stream));
+    g->rows_cleared = 0;
     int launches = 0, slots_used = 1;
     int32_t split = kr + 1;  // batched: first LDS-resident row
     bool teams_ran = false, lds_ran = false, first_ran = false;
@@ -1094,7 +1108,8 @@ dcx_status dcx_ctx_create(const dcx_config* cfg, dcx_ctx** out) {
     for (auto& e : c->ev)
         if (hipEventCreate(&e) != hipSuccess) return DCX_ERR_HIP;
     if (hipStreamCreateWithFlags(&c->side2, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&c->row0_done) != hipSuccess || hipEventCreate(&c->order_done) != hipSuccess)
+        hipEventCreate(&c->row0_done) != hipSuccess || hipEventCreate(&c->order_done) != hipSuccess ||
+        hipEventCreateWithFlags(&c->clear_done, hipEventDisableTiming) != hipSuccess)
         return DCX_ERR_HIP;
     if (hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess ||
@@ -1139,6 +1154,7 @@ dcx_status dcx_ctx_destroy(dcx_ctx* c) {
     }
     if (c->row0_done) (void)hipEventDestroy(c->row0_done);
     if (c->order_done) (void)hipEventDestroy(c->order_done);
+    if (c->clear_done) (void)hipEventDestroy(c->clear_done);
     if (c->fork) (void)hipEventDestroy(c->fork);
     if (c->join) (void)hipEventDestroy(c->join);
     for (auto& e : c->sev)
@@ -1467,7 +1483,7 @@ dcx_status dcx_decompose(dcx_ctx* ctx, dcx_graph* g, int32_t variant, int32_t* k
     return guarded(ctx, [&]() -> dcx_status {
         ctx->timing = dcx_timing{};
         g->rows_valid = false;
-        compute_kmax(ctx, g, variant);
+        compute_kmax(ctx, g, variant, false, true);
         compute_rows(ctx, g, variant);
         ctx->timing.total_ms = ctx->timing.kmax_ms + ctx->timing.order_ms + ctx->timing.rows_ms;
         if (kmax) *kmax = g->kmax;

@@ -40,6 +40,7 @@ struct dcx_ctx {
     // the end of the order kernels on `stream`
     hipStream_t side2 = nullptr;
     hipEvent_t row0_done = nullptr, order_done = nullptr;
+    hipEvent_t clear_done = nullptr;  // the rows' clear queued on side2 after row 0
     // [0], [1]: brackets of the LDS-resident launch on `side`; [3], [2]:
     // start and end of the team launch on `stream` (ev[5] opens the rows
     // kernels, row 0's grid pass first when the other rows use its bound)
@@ -148,6 +149,7 @@ struct dcx_graph {
     // power-law graphs (few workgroups suffice) but throughput-bound on
     // graphs with few levels (half each).
     int32_t row0_eighths = 2;
+    size_t rows_cleared = 0;  // cells of `rows` zeroed on side2 for the next rows launch
     uint32_t* pstats = nullptr;  // [(kmax+2) * kStatWords] device pass log: kmax pass, rows
     size_t pstats_cap = 0;
     std::vector<uint32_t> h_pstats;      // host copy of pstats after the last rows launch
