@@ -1574,3 +1574,4 @@ dcx_status dcx_get_ragged(dcx_ctx* ctx, const dcx_graph* g, int64_t* row_ptr, in
 #include "dcx_tune_impl.hpp"
 #include "dcx_serve_impl.hpp"
 #include "dcx_mlp_impl.hpp"
+#include "dcx_ingest_impl.hpp"

@@ -339,6 +339,20 @@ class Context:
     def upload_edges(self, V, src, dst, slack=0):
         return Graph(self, V, src, dst, slack)
 
+    def ingest_edges(self, V, src, dst, slack=0):
+        """The graph of upload_edges, built on the GPU (dcx_graph_ingest_edges)."""
+        src = np.ascontiguousarray(src, np.uint32)
+        dst = np.ascontiguousarray(dst, np.uint32)
+        h = ctypes.c_void_p()
+        check(lib().dcx_graph_ingest_edges(self.h, u32(V), u64(src.size), P(src), P(dst),
+                                           ctypes.c_int32(0), u32(slack), ctypes.byref(h)),
+              "dcx_graph_ingest_edges", self.h)
+        g = Graph.__new__(Graph)
+        g.ctx = self
+        g.h = h
+        g.V = V
+        return g
+
     def upload_csr(self, V, E, csr):
         return Graph(self, V, None, None, 0, csr=(E, csr))
 

@@ -203,6 +203,17 @@ dcx_status dcx_graph_upload_csr(dcx_ctx* ctx, uint32_t V, uint64_t E, const uint
 /* Builds the CSR (dcx_csr_build) and uploads it. */
 dcx_status dcx_graph_upload_edges(dcx_ctx* ctx, uint32_t V, uint64_t E, const uint32_t* src,
                                   const uint32_t* dst, uint32_t slack, dcx_graph** out);
+/* The same graph as dcx_graph_upload_edges (same layout, neighbours in
+ * input order, duplicates kept), built on the GPU: degree count, offset
+ * scan and a radix sort per side.  src/dst are host arrays, or device
+ * arrays when on_device is nonzero (then no host copy is made).  A pair with
+ * an id >= V or a self-loop fails with DCX_ERR_INVALID_ARG and names the
+ * first such pair.  Replaces the host CSR build of Graph::readFile
+ * (src/graph.cpp:146-250) followed by moveGraphToDevice
+ * (src/cuda_memory.h:5-41). */
+dcx_status dcx_graph_ingest_edges(dcx_ctx* ctx, uint32_t V, uint64_t E, const uint32_t* src,
+                                  const uint32_t* dst, int32_t on_device, uint32_t slack,
+                                  dcx_graph** out);
 dcx_status dcx_graph_free(dcx_graph* g);
 /* V, current E, and kmax/lmax of the resident decomposition (-1 if none). */
 dcx_status dcx_graph_info(const dcx_graph* g, uint32_t* V, uint64_t* E, int32_t* kmax,
