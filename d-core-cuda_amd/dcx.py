@@ -353,6 +353,26 @@ class Context:
         g.V = V
         return g
 
+    def to_device(self, a):
+        """A device copy of a uint32 array, allocated by the HIP runtime the
+        library uses (libamdhip64.so.7; another runtime in the same process,
+        such as torch's, cannot share the device with it)."""
+        return DeviceArray(np.ascontiguousarray(a, np.uint32))
+
+    def ingest_device(self, V, E, src_ptr, dst_ptr, slack=0):
+        """dcx_graph_ingest_edges from device arrays (addresses of E uint32
+        each, written and synchronized by the caller)."""
+        h = ctypes.c_void_p()
+        check(lib().dcx_graph_ingest_edges(self.h, u32(V), u64(E), ctypes.c_void_p(src_ptr),
+                                           ctypes.c_void_p(dst_ptr), ctypes.c_int32(1),
+                                           u32(slack), ctypes.byref(h)),
+              "dcx_graph_ingest_edges", self.h)
+        g = Graph.__new__(Graph)
+        g.ctx = self
+        g.h = h
+        g.V = V
+        return g
+
     def upload_csr(self, V, E, csr):
         return Graph(self, V, None, None, 0, csr=(E, csr))
 
@@ -377,6 +397,28 @@ class Context:
         g.h = h
         g.V = int(vbase[-1])
         return g, vbase
+
+
+class DeviceArray:
+    """Device memory holding a copy of a host uint32 array (hipMalloc and a
+    synchronous copy through the library's own HIP runtime)."""
+
+    def __init__(self, a):
+        lib()
+        hip = ctypes.CDLL("libamdhip64.so.7")
+        self._hip = hip
+        self.ptr = ctypes.c_void_p()
+        self.size = a.size
+        if hip.hipMalloc(ctypes.byref(self.ptr), ctypes.c_size_t(max(a.nbytes, 4))) != 0:
+            raise DcxError(3, "hipMalloc")
+        if a.nbytes and hip.hipMemcpy(self.ptr, P(a), ctypes.c_size_t(a.nbytes), 1) != 0:
+            self.free()
+            raise DcxError(3, "hipMemcpy")
+
+    def free(self):
+        if self.ptr:
+            self._hip.hipFree(self.ptr)
+            self.ptr = ctypes.c_void_p()
 
 
 class Graph:

@@ -115,3 +115,20 @@ def test_ingest_rejects_bad_pairs(ctx, bad):
     g = ctx.ingest_edges(5, np.array([0, 1], np.uint32), np.array([1, 0], np.uint32))
     assert g.decompose(dcx.VARIANT_BATCHED) == (1, 1)
     g.free()
+
+
+def test_ingest_from_device_arrays(ctx):
+    """on_device = 1: the pairs are read where they lie in HBM."""
+    s, d = dcx.gen_rmat(14, 16 << 14, 11)
+    V = 1 << 14
+    say("ingest rmat14 from device arrays")
+    ds, dd = ctx.to_device(s), ctx.to_device(d)
+    a = ctx.upload_edges(V, s, d, 2)
+    b = ctx.ingest_device(V, s.size, ds.ptr.value, dd.ptr.value, 2)
+    ca, cb = a.download_csr(), b.download_csr()
+    for key in ca:
+        assert np.array_equal(ca[key], cb[key]), key
+    a.free()
+    b.free()
+    ds.free()
+    dd.free()
