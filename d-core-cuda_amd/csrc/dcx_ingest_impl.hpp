@@ -7,9 +7,10 @@
 //   1. One pass over the pairs counts both degrees and records the first
 //      pair with an id out of range or a self-loop.
 //   2. Offsets are the exclusive scan of deg + slack.
-//   3. Per side, the keys (owner << B) | input index are radix-sorted.  The
+//   3. Per side, the keys (owner << B) | input index, built in input order,
+//      are radix-sorted on the owner bits only: the sort is stable, so the
 //      owner's edges then sit at sorted positions start(owner) + r in input
-//      order, where start(owner) = off[owner] - owner * slack, so edge r of
+//      order, where start(owner) = off[owner] - owner * slack, and edge r of
 //      the list goes to slot j + owner * slack for sorted position j.
 #pragma once
 
@@ -174,7 +175,7 @@ dcx_status dcx_graph_ingest_edges(dcx_ctx* ctx, uint32_t V, uint64_t E, const ui
                                    outs ? d_src : d_dst, E, B, keys);
                 HIPX(ctx, hipGetLastError());
                 uint64_t* sorted = nullptr;
-                HIPX(ctx, launch_radix_sort_u64(keys, alt, E, key_bits, table, &sorted, st));
+                HIPX(ctx, launch_radix_sort_u64(keys, alt, E, key_bits, table, &sorted, st, B));
                 hipLaunchKernelGGL(k_ingest_place, dim3(ingest_blocks(E)), dim3(256), 0, st,
                                    sorted, E, B, outs ? d_dst : d_src, slack,
                                    outs ? g->out_nbr : g->in_nbr);

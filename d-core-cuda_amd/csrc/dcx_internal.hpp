@@ -231,8 +231,11 @@ size_t scan_tmp_words(uint64_t n);
 hipError_t launch_excl_scan_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* tmp,
                                 hipStream_t st);
 size_t radix_tmp_words(uint64_t n);
+// Sorts by key bits [lo_bit, bits); being stable, it keeps the input order
+// of keys equal in those bits.
 hipError_t launch_radix_sort_u64(uint64_t* keys, uint64_t* alt, uint64_t n, int bits,
-                                 uint32_t* table, uint64_t** result, hipStream_t st);
+                                 uint32_t* table, uint64_t** result, hipStream_t st,
+                                 int lo_bit = 0);
 size_t core_graph_tmp_bytes(uint32_t V, uint64_t E);
 // Adds sum(deg) to out[0] and the count of list entries in [0, deg) that
 // are not vertex ids (< V) to out[1].

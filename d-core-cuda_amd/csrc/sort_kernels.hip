@@ -193,7 +193,7 @@ size_t radix_tmp_words(uint64_t n) {
 }
 
 hipError_t launch_radix_sort_u64(uint64_t* keys, uint64_t* alt, uint64_t n, int bits,
-                                 uint32_t* table, uint64_t** result, hipStream_t st) {
+                                 uint32_t* table, uint64_t** result, hipStream_t st, int lo_bit) {
     *result = keys;
     if (n < 2) return hipSuccess;
     const uint32_t nt = tiles_for(n);
@@ -201,7 +201,7 @@ hipError_t launch_radix_sort_u64(uint64_t* keys, uint64_t* alt, uint64_t n, int 
     uint64_t* b = alt;
     const uint64_t m = (uint64_t)kDigits * nt;
     uint32_t* sums = table + m + 1;
-    for (int shift = 0; shift < bits; shift += kDigitBits) {
+    for (int shift = lo_bit; shift < bits; shift += kDigitBits) {
         hipLaunchKernelGGL(k_radix_hist, dim3(nt), dim3(kSortBlk), 0, st, a, n, shift, nt, table);
         // the digit table scanned by tiles (one block per tile, in place)
         hipError_t e = launch_excl_scan_u32(table, table, m, sums, st);
