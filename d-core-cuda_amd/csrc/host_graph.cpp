@@ -186,6 +186,8 @@ dcx_status dcx_csr_build(uint32_t V, uint64_t E, const uint32_t* src, const uint
         od[src[e]]++;
         id[dst[e]]++;
     }
+    for (uint32_t v = 0; v < V; ++v)
+        if (od[v] > 0x7fffffffu || id[v] > 0x7fffffffu) return DCX_ERR_CAPACITY;  // int32 degree
     uint64_t so = 0, si = 0;
     for (uint32_t v = 0; v < V; ++v) {
         out_off[v] = (uint32_t)so;

@@ -7,9 +7,17 @@
 #pragma once
 #include <cstdint>
 
+// The same functions serve the host generators and the device sampler of
+// dcx_gen_rmat_device (compiled by hipcc), so both draw identical samples.
+#if defined(__HIPCC__)
+#define DCX_HD __host__ __device__
+#else
+#define DCX_HD
+#endif
+
 namespace dcx {
 
-inline uint64_t mix64(uint64_t z) {
+DCX_HD inline uint64_t mix64(uint64_t z) {
     z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
     z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
     return z ^ (z >> 31);
@@ -20,16 +28,16 @@ constexpr uint64_t kGolden = 0x9e3779b97f4a7c15ULL;
 // Sequential splitmix64 stream.
 struct SplitMix {
     uint64_t s;
-    explicit SplitMix(uint64_t seed) : s(seed) {}
-    uint64_t next() { return mix64(s += kGolden); }
+    DCX_HD explicit SplitMix(uint64_t seed) : s(seed) {}
+    DCX_HD uint64_t next() { return mix64(s += kGolden); }
     // uniform in [0, 1) with 53 random bits
-    double uniform() { return (double)(next() >> 11) * 0x1.0p-53; }
+    DCX_HD double uniform() { return (double)(next() >> 11) * 0x1.0p-53; }
     // uniform integer in [0, n) by multiply-high (bias < 2^-32 * n)
-    uint32_t below(uint32_t n) { return (uint32_t)(((next() >> 32) * (uint64_t)n) >> 32); }
+    DCX_HD uint32_t below(uint32_t n) { return (uint32_t)(((next() >> 32) * (uint64_t)n) >> 32); }
 };
 
 // Stream for sample `i` of a generator seeded with `seed`.
-inline SplitMix stream_for(uint64_t seed, uint64_t i) {
+DCX_HD inline SplitMix stream_for(uint64_t seed, uint64_t i) {
     return SplitMix(mix64(seed * kGolden + 0x632be59bd9b4e019ULL) ^ mix64(i + 1));
 }
 

@@ -13,6 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libdcx.so")
 
 VARIANT_AUTO, VARIANT_PEEL, VARIANT_BATCHED, VARIANT_HINDEX = 0, 1, 2, 3
+INGEST_ON_DEVICE, INGEST_DROP_SELF_LOOPS, INGEST_DEDUP = 1, 2, 4
 STATUS = {
     0: "DCX_OK", 1: "DCX_ERR_INVALID_ARG", 2: "DCX_ERR_NO_DEVICE", 3: "DCX_ERR_HIP",
     4: "DCX_ERR_OUT_OF_MEMORY", 5: "DCX_ERR_CAPACITY", 6: "DCX_ERR_IO",
@@ -359,19 +360,53 @@ class Context:
         such as torch's, cannot share the device with it)."""
         return DeviceArray(np.ascontiguousarray(a, np.uint32))
 
-    def ingest_device(self, V, E, src_ptr, dst_ptr, slack=0):
-        """dcx_graph_ingest_edges from device arrays (addresses of E uint32
-        each, written and synchronized by the caller)."""
+    def ingest_device(self, V, src, dst, slack=0, E=None):
+        """dcx_graph_ingest_edges from two DeviceArray objects (the first E
+        entries of each, default all of them)."""
+        return self.ingest(V, src, dst, slack=slack, E=E)
+
+    def ingest(self, V, src, dst, slack=0, drop_self_loops=False, dedup=False, E=None):
+        """dcx_graph_ingest: src/dst are numpy arrays (copied to the device)
+        or DeviceArray objects; E (device arrays only) takes a prefix."""
+        flags = (INGEST_DROP_SELF_LOOPS if drop_self_loops else 0) | (INGEST_DEDUP if dedup else 0)
+        if isinstance(src, DeviceArray) != isinstance(dst, DeviceArray):
+            raise TypeError("src and dst must both be host arrays or both DeviceArray")
+        if isinstance(src, DeviceArray):
+            n = min(src.size, dst.size)
+            E = n if E is None else int(E)
+            if E < 0 or E > n:
+                raise ValueError(f"E = {E} exceeds the device arrays ({src.size}, {dst.size})")
+            flags |= INGEST_ON_DEVICE
+            ps, pd = src.ptr, dst.ptr
+        else:
+            if E is not None:
+                raise ValueError("E applies to device arrays only")
+            src = np.ascontiguousarray(src, np.uint32)
+            dst = np.ascontiguousarray(dst, np.uint32)
+            if src.size != dst.size:
+                raise ValueError("src and dst differ in length")
+            E = src.size
+            ps, pd = P(src), P(dst)
         h = ctypes.c_void_p()
-        check(lib().dcx_graph_ingest_edges(self.h, u32(V), u64(E), ctypes.c_void_p(src_ptr),
-                                           ctypes.c_void_p(dst_ptr), ctypes.c_int32(1),
-                                           u32(slack), ctypes.byref(h)),
-              "dcx_graph_ingest_edges", self.h)
+        check(lib().dcx_graph_ingest(self.h, u32(V), u64(E), ps, pd, u32(flags), u32(slack),
+                                     ctypes.byref(h)),
+              "dcx_graph_ingest", self.h)
         g = Graph.__new__(Graph)
         g.ctx = self
         g.h = h
         g.V = V
         return g
+
+    def gen_rmat_device(self, scale, samples, seed, a=0.57, b=0.19, c=0.19):
+        """Raw R-MAT samples (those of gen_rmat before its dedup) drawn on
+        the device: two DeviceArray of `samples` uint32."""
+        src = DeviceArray.empty(samples)
+        dst = DeviceArray.empty(samples)
+        check(lib().dcx_gen_rmat_device(self.h, u32(scale), u64(samples), ctypes.c_double(a),
+                                        ctypes.c_double(b), ctypes.c_double(c), u64(seed),
+                                        src.ptr, dst.ptr),
+              "dcx_gen_rmat_device", self.h)
+        return src, dst
 
     def upload_csr(self, V, E, csr):
         return Graph(self, V, None, None, 0, csr=(E, csr))
@@ -403,17 +438,31 @@ class DeviceArray:
     """Device memory holding a copy of a host uint32 array (hipMalloc and a
     synchronous copy through the library's own HIP runtime)."""
 
-    def __init__(self, a):
+    def __init__(self, a=None, size=None):
         lib()
         hip = ctypes.CDLL("libamdhip64.so.7")
         self._hip = hip
         self.ptr = ctypes.c_void_p()
-        self.size = a.size
-        if hip.hipMalloc(ctypes.byref(self.ptr), ctypes.c_size_t(max(a.nbytes, 4))) != 0:
+        self.size = a.size if a is not None else int(size)
+        nbytes = 4 * self.size
+        if hip.hipMalloc(ctypes.byref(self.ptr), ctypes.c_size_t(max(nbytes, 4))) != 0:
             raise DcxError(3, "hipMalloc")
-        if a.nbytes and hip.hipMemcpy(self.ptr, P(a), ctypes.c_size_t(a.nbytes), 1) != 0:
+        if a is not None and nbytes and hip.hipMemcpy(self.ptr, P(a), ctypes.c_size_t(nbytes),
+                                                      1) != 0:
             self.free()
             raise DcxError(3, "hipMemcpy")
+
+    @classmethod
+    def empty(cls, n):
+        """Uninitialised device memory for n uint32."""
+        return cls(None, size=n)
+
+    def numpy(self):
+        out = np.empty(self.size, np.uint32)
+        if self.size and self._hip.hipMemcpy(P(out), self.ptr, ctypes.c_size_t(4 * self.size),
+                                             2) != 0:
+            raise DcxError(3, "hipMemcpy")
+        return out
 
     def free(self):
         if self.ptr:

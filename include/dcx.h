@@ -214,6 +214,32 @@ dcx_status dcx_graph_upload_edges(dcx_ctx* ctx, uint32_t V, uint64_t E, const ui
 dcx_status dcx_graph_ingest_edges(dcx_ctx* ctx, uint32_t V, uint64_t E, const uint32_t* src,
                                   const uint32_t* dst, int32_t on_device, uint32_t slack,
                                   dcx_graph** out);
+/* Ingest options (bit flags of dcx_graph_ingest). */
+enum {
+    DCX_INGEST_ON_DEVICE = 1,       /* src/dst are device arrays */
+    DCX_INGEST_DROP_SELF_LOOPS = 2, /* skip pairs u->u, as Graph::readFile does
+                                       (src/graph.cpp:167), instead of failing */
+    DCX_INGEST_DEDUP = 4            /* keep each pair once; every list is then
+                                       sorted by neighbour id */
+};
+/* dcx_graph_ingest_edges with options.  Without DCX_INGEST_DEDUP the graph is
+ * the host build of the input (neighbours in input order, duplicates kept);
+ * with it, the host build of the sorted, deduplicated pair list (what the
+ * seeded generators return).  Ids >= V fail with DCX_ERR_INVALID_ARG, and so
+ * do self-loops unless dropped.  The graph's E counts the kept pairs.
+ * Replaces Graph::readFile's CSR build (src/graph.cpp:146-250) on raw pairs,
+ * including the dedup a generator does before writing its file. */
+dcx_status dcx_graph_ingest(dcx_ctx* ctx, uint32_t V, uint64_t E, const uint32_t* src,
+                            const uint32_t* dst, uint32_t flags, uint32_t slack,
+                            dcx_graph** out);
+/* The raw samples of dcx_gen_rmat (before its sort, self-loop drop and
+ * dedup) drawn on the device into caller-allocated device arrays of
+ * `samples` entries each; with dcx_graph_ingest(DCX_INGEST_ON_DEVICE |
+ * DCX_INGEST_DROP_SELF_LOOPS | DCX_INGEST_DEDUP) they give the graph of
+ * dcx_gen_rmat followed by dcx_graph_upload_edges without a host round trip.
+ * No reference counterpart (the reference only loads files). */
+dcx_status dcx_gen_rmat_device(dcx_ctx* ctx, uint32_t scale, uint64_t samples, double a,
+                               double b, double c, uint64_t seed, uint32_t* src, uint32_t* dst);
 dcx_status dcx_graph_free(dcx_graph* g);
 /* V, current E, and kmax/lmax of the resident decomposition (-1 if none). */
 dcx_status dcx_graph_info(const dcx_graph* g, uint32_t* V, uint64_t* E, int32_t* kmax,

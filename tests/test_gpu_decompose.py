@@ -343,12 +343,13 @@ def test_core_edge_histogram(ctx):
 
 
 def test_rmat20_full(ctx):
-    """The headline graph: R-MAT scale 20, 16*2^20 samples, seed 1."""
+    """The headline graph: R-MAT scale 20, 16*2^20 samples, seed 1, in the
+    batched, h-index and plain one-k-per-launch variants."""
     s, d = dcx.gen_rmat(20, 16 << 20, seed=1)
     V = 1 << 20
     say(f"rmat20 oracle E={len(s)}")
     ref = oracle.decompose(oracle.csr_from_edges(V, s, d))
-    for var in [dcx.VARIANT_BATCHED, dcx.VARIANT_HINDEX]:
+    for var in [dcx.VARIANT_BATCHED, dcx.VARIANT_HINDEX, dcx.VARIANT_PEEL]:
         say(f"rmat20 gpu variant={var}")
         g = ctx.upload_edges(V, s, d)
         K, lmax = g.decompose(var)

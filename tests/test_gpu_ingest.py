@@ -124,11 +124,130 @@ def test_ingest_from_device_arrays(ctx):
     say("ingest rmat14 from device arrays")
     ds, dd = ctx.to_device(s), ctx.to_device(d)
     a = ctx.upload_edges(V, s, d, 2)
-    b = ctx.ingest_device(V, s.size, ds.ptr.value, dd.ptr.value, 2)
-    ca, cb = a.download_csr(), b.download_csr()
-    for key in ca:
-        assert np.array_equal(ca[key], cb[key]), key
+    b = ctx.ingest_device(V, ds, dd, 2)
+    same_csr(a, b, "rmat14 device")
     a.free()
     b.free()
+    # a prefix of the device arrays
+    say("ingest rmat14 prefix from device arrays")
+    a = ctx.upload_edges(V, s[:1000], d[:1000], 0)
+    b = ctx.ingest_device(V, ds, dd, 0, E=1000)
+    same_csr(a, b, "rmat14 device prefix")
+    a.free()
+    b.free()
+    with pytest.raises(ValueError):
+        ctx.ingest_device(V, ds, dd, 0, E=s.size + 1)
     ds.free()
     dd.free()
+
+
+def same_csr(a, b, name):
+    ca, cb = a.download_csr(), b.download_csr()
+    for key in ca:
+        assert np.array_equal(ca[key], cb[key]), (name, key)
+    assert a.info() == b.info(), name
+
+
+def test_ingest_rmat18_slot_for_slot(ctx):
+    """R-MAT 18 (more than 2^22 pairs: three digit passes of the owner sort,
+    the owner bits starting off a byte boundary), shuffled so that every
+    hub's edges lie far apart in the input."""
+    s, d = dcx.gen_rmat(18, 16 << 18, 21)
+    V = 1 << 18
+    p = np.random.default_rng(3).permutation(s.size)
+    s, d = s[p], d[p]
+    for slack in (0, 1):
+        say(f"ingest rmat18 shuffled E={s.size} slack={slack}")
+        a = ctx.upload_edges(V, s, d, slack)
+        b = ctx.ingest(V, s, d, slack)
+        same_csr(a, b, f"rmat18 slack={slack}")
+        a.free()
+        b.free()
+
+
+@pytest.mark.parametrize("dedup", [False, True])
+def test_ingest_drops_self_loops(ctx, dedup):
+    """With self-loops dropped (Graph::readFile skips them,
+    src/graph.cpp:167) the graph is the host build of the list without
+    them; with dedup as well, of the sorted list of distinct pairs."""
+    rng = np.random.default_rng(5)
+    V = 3000
+    s = rng.integers(0, V, 60000).astype(np.uint32)
+    d = rng.integers(0, V, 60000).astype(np.uint32)
+    d[::7] = s[::7]  # many self-loops
+    s = np.concatenate([s, s[:5000]])  # and repeated pairs
+    d = np.concatenate([d, d[:5000]])
+    keep = s != d
+    if dedup:
+        key = np.unique((s[keep].astype(np.uint64) << 32) | d[keep])
+        hs, hd = (key >> 32).astype(np.uint32), (key & 0xffffffff).astype(np.uint32)
+    else:
+        hs, hd = s[keep], d[keep]
+    for slack in (0, 2):
+        say(f"ingest drop self-loops dedup={dedup} slack={slack}")
+        a = ctx.upload_edges(V, hs, hd, slack)
+        b = ctx.ingest(V, s, d, slack, drop_self_loops=True, dedup=dedup)
+        same_csr(a, b, f"drop dedup={dedup}")
+        a.free()
+        b.free()
+    # the text loader drops self-loops the same way
+    if not dedup:
+        say("ingest drop self-loops against dcx_text_read")
+        import tempfile, os
+        with tempfile.TemporaryDirectory() as tdir:
+            path = os.path.join(tdir, "loops.txt")
+            dcx.text_write(path, V, s, d)
+            V2, ts, td = dcx.text_read(path, prune=False)
+        a = ctx.upload_edges(V2, ts, td, 1)
+        b = ctx.ingest(V, s, d, 1, drop_self_loops=True)
+        same_csr(a, b, "text_read")
+        a.free()
+        b.free()
+
+
+@pytest.mark.parametrize("scale,seed", [(12, 3), (16, 8)])
+def test_device_rmat_samples_and_dedup(ctx, scale, seed):
+    """Raw R-MAT samples drawn on the device equal the host generator's
+    draws (their sorted distinct non-loop pairs are dcx_gen_rmat's output),
+    and ingesting them with self-loop drop and dedup gives the graph of
+    dcx_gen_rmat uploaded from the host, slot for slot."""
+    V = 1 << scale
+    samples = 16 << scale
+    say(f"device rmat {scale} samples={samples}")
+    ds, dd = ctx.gen_rmat_device(scale, samples, seed)
+    rs, rd = ds.numpy(), dd.numpy()
+    s, d = dcx.gen_rmat(scale, samples, seed)
+    keep = rs != rd
+    key = np.unique((rs[keep].astype(np.uint64) << 32) | rd[keep])
+    assert np.array_equal((key >> 32).astype(np.uint32), s)
+    assert np.array_equal((key & 0xffffffff).astype(np.uint32), d)
+    for slack in (0, 1):
+        say(f"device rmat {scale} ingest dedup slack={slack}")
+        a = ctx.upload_edges(V, s, d, slack)
+        b = ctx.ingest(V, ds, dd, slack, drop_self_loops=True, dedup=True)
+        same_csr(a, b, f"device rmat {scale}")
+        K, lmax = b.decompose(dcx.VARIANT_BATCHED)
+        assert (K, lmax) == a.decompose(dcx.VARIANT_BATCHED)
+        assert b.state_hash() == a.state_hash()
+        a.free()
+        b.free()
+    ds.free()
+    dd.free()
+
+
+def test_ingest_device_rejects_bad_pairs(ctx):
+    """A bad pair in device arrays fails with its index and leaves the
+    context usable."""
+    s = np.array([0, 1, 2, 3, 4], np.uint32)
+    d = np.array([1, 2, 9, 4, 0], np.uint32)
+    ds, dd = ctx.to_device(s), ctx.to_device(d)
+    say("ingest device arrays with an id out of range")
+    with pytest.raises(dcx.DcxError) as e:
+        ctx.ingest_device(5, ds, dd)
+    assert e.value.status == 1 and "pair 2" in str(e.value)
+    ds.free()
+    dd.free()
+    g = ctx.ingest(5, np.array([0, 1], np.uint32), np.array([1, 0], np.uint32), 0,
+                   drop_self_loops=True, dedup=True)
+    assert g.decompose(dcx.VARIANT_BATCHED) == (1, 1)
+    g.free()

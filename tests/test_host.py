@@ -327,3 +327,43 @@ def test_mlp_reference_gradient_matches_finite_differences():
         q[i] -= 2 * h
         lm = 0.5 * ((mlp_ref.forward(q, X)[0] - T) ** 2).sum()
         assert abs((lp - lm) / (2 * h) - g[i]) < 1e-5 * max(1.0, abs(g[i])), i
+
+
+def test_kmax_edge_batches_hand_derived():
+    """getKmaxEdgeBatches (main.cu:1441-1525) on a case worked out by hand.
+
+    Root of an edge: the endpoint with the larger kmax, the target on ties
+    (main.cu:1449-1451); its kmax is the edge's bucket.
+      kmaxes: 0:3 1:3 2:1 3:5 4:1 5:3
+      e0 0->1 root 1 (tie: target) K3   e1 1->0 root 0 K3
+      e2 2->3 root 3 K5                 e3 3->2 root 3 (larger: source) K5
+      e4 4->1 root 1 K3                 e5 5->1 root 1 (tie) K3
+      e6 2->4 root 4 (tie) K1           e7 4->2 root 2 (tie) K1
+    KEDGE (mode 0): per bucket, ascending, repeated passes taking edges
+    whose root is not yet in the pass's batch:
+      K1: {e6,e7};  K3: {e0,e1}, {e4}, {e5};  K5: {e2}, {e3}
+    KMAX (mode 1): per round, the first remaining edge of each bucket whose
+    kmax differs by more than one from the round's first edge:
+      {e6,e0,e2}, {e7,e1,e3}, {e4}, {e5}
+    A source-on-ties root would put e5 in the first K3 batch; a
+    smaller-kmax root would move e2 to bucket 1."""
+    import dcx
+    km = np.array([3, 3, 1, 5, 1, 3], np.int32)
+    pairs = np.array([[0, 1], [1, 0], [2, 3], [3, 2], [4, 1], [5, 1], [2, 4], [4, 2]],
+                     np.uint32)
+    b, nb = dcx.kmax_edge_batches(6, km, pairs, mode=0)
+    assert nb == 6
+    assert b.tolist() == [1, 1, 4, 5, 2, 3, 0, 0]
+    b, nb = dcx.kmax_edge_batches(6, km, pairs, mode=1)
+    assert nb == 4
+    assert b.tolist() == [0, 1, 0, 1, 2, 3, 0, 1]
+    # KMAX compares each bucket with the round's FIRST edge only: buckets
+    # 1, 3, 4 all join one round (4 - 1 > 1 even though 4 - 3 = 1)
+    km2 = np.array([1, 1, 3, 3, 4, 4], np.int32)
+    p2 = np.array([[0, 1], [2, 3], [4, 5]], np.uint32)
+    b, nb = dcx.kmax_edge_batches(6, km2, p2, mode=1)
+    assert nb == 1 and b.tolist() == [0, 0, 0]
+    # and a bucket one away from the first waits for the next round
+    km3 = np.array([1, 1, 2, 2, 5, 5], np.int32)
+    b, nb = dcx.kmax_edge_batches(6, km3, p2, mode=1)
+    assert nb == 2 and b.tolist() == [0, 1, 0]
