@@ -4,10 +4,11 @@
 
 #include <algorithm>
 #include <fstream>
+#include <map>
 #include <numeric>
 #include <set>
 
-namespace dcore {
+namespace dcx_host {
 
 void check(dcx_status s, const char* where, const dcx_ctx* ctx) {
     if (s == DCX_OK) return;
@@ -152,15 +153,26 @@ void Graph::insertEdges(const std::vector<std::pair<vertex, vertex>>& e) {
 
 void Graph::insertEdgesInPlace(const std::vector<std::pair<vertex, vertex>>& e) {
     // Uses the free slots like src/graph.cpp:63-70, but rebuilds with the
-    // same gap when a list is full instead of overwriting its neighbour.
+    // same gap when a list lacks room for all of the batch's edges into it
+    // instead of overwriting its neighbour.
+    std::map<vertex, unsigned> need_out, need_in;
     for (auto& [u, v] : e) {
-        const bool room_out = out_neighbors_offset[u] + out_degrees[u] < out_neighbors_offset[u + 1];
-        const bool room_in = in_neighbors_offset[v] + in_degrees[v] < in_neighbors_offset[v + 1];
-        if (!room_out || !room_in) {
+        if (u >= V || v >= V) throw Error(DCX_ERR_INVALID_ARG, "insertEdgesInPlace: id >= V");
+        ++need_out[u];
+        ++need_in[v];
+    }
+    auto room = [](const std::vector<offset>& off, const std::vector<degree>& deg, vertex x,
+                   unsigned n) { return off[x] + deg[x] + n <= off[x + 1]; };
+    for (auto& [u, n] : need_out)
+        if (!room(out_neighbors_offset, out_degrees, u, n)) {
             insertEdges(e);
             return;
         }
-    }
+    for (auto& [v, n] : need_in)
+        if (!room(in_neighbors_offset, in_degrees, v, n)) {
+            insertEdges(e);
+            return;
+        }
     for (auto& [u, v] : e) {
         out_neighbors[out_neighbors_offset[u] + out_degrees[u]++] = v;
         in_neighbors[in_neighbors_offset[v] + in_degrees[v]++] = u;
@@ -432,4 +444,4 @@ void writeDecompFileCPU(const GraphInterface& g, const std::string& filename) {
           "dcx_decompcpu_write");
 }
 
-}  // namespace dcore
+}  // namespace dcx_host

@@ -4,21 +4,24 @@
 // reference so that a caller of llysdal/d-core-cuda can switch over:
 //
 //   reference                                     here
-//   GraphInterface (src/common.h:64-79)           dcore::GraphInterface
-//   Graph (src/graph.h:11-37)                     dcore::Graph (host CSR + files)
-//   GraphGPU (src/graph_gpu.h:10-22)              dcore::GraphGPU (device-resident)
-//   dcore(Graph&) (main.cu:197-262)               dcore::dcore(Graph&)
-//   KList / KListInplace (main.cu:144-195)        dcore::KList / KListInplace
-//   checkKmax / checkDCore (main.cu:264-293)      dcore::checkKmax / checkDCore
-//   maintenance / maintenanceFast (main.cu:1536-1751)  dcore::maintenance / maintenanceFast
-//   getKmaxEdgeBatches (main.cu:1441-1525)        dcore::getKmaxEdgeBatches
-//   generateGraph (src/graph_gen.h:8)             dcore::generateGraph (seeded)
-//   read/writeDecompFile(CPU) (src/utils.h:76-124) dcore::readDecompFile / writeDecompFile(CPU)
+//   GraphInterface (src/common.h:64-79)           dcx_host::GraphInterface
+//   Graph (src/graph.h:11-37)                     dcx_host::Graph (host CSR + files)
+//   GraphGPU (src/graph_gpu.h:10-22)              dcx_host::GraphGPU (device-resident)
+//   dcore(Graph&) (main.cu:197-262)               dcx_host::dcore(Device&, Graph&)
+//   KList / KListInplace (main.cu:144-195)        dcx_host::KList / KListInplace
+//   checkKmax / checkDCore (main.cu:264-293)      dcx_host::checkKmax / checkDCore
+//   maintenance / maintenanceFast (main.cu:1536-1751)  dcx_host::maintenance / maintenanceFast
+//   getKmaxEdgeBatches (main.cu:1441-1525)        dcx_host::getKmaxEdgeBatches
+//   generateGraph (src/graph_gen.h:8)             dcx_host::generateGraph (seeded)
+//   read/writeDecompFile(CPU) (src/utils.h:76-124) dcx_host::readDecompFile / writeDecompFile(CPU)
 //
-// Differences that callers see, all deliberate: device state lives behind
+// This namespace is the library's own C++ API: device state lives behind
 // GraphGPU (no device_*_pointers structs to pass around); every failure
-// throws dcore::Error carrying the dcx_status instead of going unchecked;
-// edge deletion works; generateGraph takes a seed.
+// throws dcx_host::Error carrying the dcx_status instead of going unchecked;
+// edge deletion works; generateGraph takes a seed.  dcore_ref.hpp puts the
+// reference's exact global-namespace signatures (dcore(Graph&), KList(V,
+// g_p, a_p, k), maintenance(g, g_p, m_p, a_p, ...), the allocate* family)
+// on top of it, for callers written against the reference.
 #pragma once
 
 #include <chrono>
@@ -31,7 +34,7 @@
 
 #include "dcx.h"
 
-namespace dcore {
+namespace dcx_host {
 
 typedef int degree;       // src/common.h:59
 typedef unsigned vertex;  // src/common.h:61
@@ -183,4 +186,4 @@ bool readDecompFile(GraphInterface& g, const std::string& filename);  // <filena
 void writeDecompFile(const GraphInterface& g, const std::string& filename);
 void writeDecompFileCPU(const GraphInterface& g, const std::string& filename);  // .decompCPU
 
-}  // namespace dcore
+}  // namespace dcx_host

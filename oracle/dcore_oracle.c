@@ -40,6 +40,11 @@
  *   dco_klist_hindex / dco_kmax_hindex
  *     - equality with the peel on every test graph (two different
  *       algorithms, fixpoint argument in the A.3 design, main.cu:982-1242).
+ *   dco_state_hash (rows peeled on their (k,0)-cores, streamed into a hash)
+ *     - equality, part by part, with the hash formula applied to the dense
+ *       output of dco_kmax / dco_lrows on the fixture, random graphs and
+ *       closed forms (tests/test_oracle.py::test_streaming_hash_*); a
+ *       changed cell moves the hash.
  */
 #include "dcore_oracle.h"
 
@@ -72,22 +77,46 @@
  * constraint lists C(v) decide membership with threshold k (in-lists for an
  * l-pass).  Removing v lowers the key degree of every w in C(v) (v was in
  * K(w)) and the constraint degree of every u in K(v). */
-static int32_t peel(uint32_t V, const uint64_t* koff, const uint32_t* knbr,
-                    const uint64_t* coff, const uint32_t* cnbr, int32_t k,
-                    int32_t* l) {
+/* Entries of a sorted list below n (its prefix of members). */
+static uint64_t count_below(const uint32_t* a, uint64_t len, uint32_t n) {
+    uint64_t lo = 0, hi = len;
+    while (lo < hi) {
+        uint64_t mid = (lo + hi) >> 1;
+        if (a[mid] < n)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+/* prefix == 0: the graph is vertices [0, V) with whole lists.
+ * prefix != 0: only vertices [0, V) of a larger graph whose lists are
+ * sorted take part (the (k,0)-core in kmax-descending rank order); each
+ * list is read up to its first entry >= V. */
+static int32_t peel_n(uint32_t V, const uint64_t* koff, const uint32_t* knbr,
+                      const uint64_t* coff, const uint32_t* cnbr, int32_t k,
+                      int32_t* l, int prefix) {
     if (V == 0) return -1;
     int32_t* kd = (int32_t*)malloc(sizeof(int32_t) * V);
     int32_t* cd = (int32_t*)malloc(sizeof(int32_t) * V);
     uint8_t* state = (uint8_t*)calloc(V, 1); /* 0 alive, 1 removed */
     uint32_t* stack = (uint32_t*)malloc(sizeof(uint32_t) * V);
     int32_t maxkd = 0;
+    uint64_t csum = 0;
     for (uint32_t v = 0; v < V; ++v) {
-        kd[v] = (int32_t)(koff[v + 1] - koff[v]);
-        cd[v] = (int32_t)(coff[v + 1] - coff[v]);
+        if (prefix) {
+            kd[v] = (int32_t)count_below(knbr + koff[v], koff[v + 1] - koff[v], V);
+            cd[v] = (int32_t)count_below(cnbr + coff[v], coff[v + 1] - coff[v], V);
+        } else {
+            kd[v] = (int32_t)(koff[v + 1] - koff[v]);
+            cd[v] = (int32_t)(coff[v + 1] - coff[v]);
+        }
+        csum += (uint64_t)cd[v];
         if (kd[v] > maxkd) maxkd = kd[v];
     }
     /* lazy buckets: at most V initial entries plus one per key decrement */
-    uint64_t cap = (uint64_t)V + coff[V];
+    uint64_t cap = (uint64_t)V + csum;
     uint32_t* pv = (uint32_t*)malloc(sizeof(uint32_t) * cap);
     uint64_t* pnext = (uint64_t*)malloc(sizeof(uint64_t) * cap);
     uint64_t* head = (uint64_t*)malloc(sizeof(uint64_t) * ((size_t)maxkd + 1));
@@ -130,6 +159,7 @@ static int32_t peel(uint32_t V, const uint64_t* koff, const uint32_t* knbr,
             for (uint64_t o = coff[v]; o < coff[v + 1]; ++o) {
                 uint32_t w = cnbr[o];
 #endif
+                if (w >= V) break; /* prefix mode: past the members */
                 if (state[w]) continue;
                 if (--kd[w] <= L) {
                     state[w] = 1;
@@ -143,6 +173,7 @@ static int32_t peel(uint32_t V, const uint64_t* koff, const uint32_t* knbr,
             }
             for (uint64_t o = koff[v]; o < koff[v + 1]; ++o) {
                 uint32_t u = knbr[o];
+                if (u >= V) break; /* prefix mode: past the members */
                 if (state[u]) continue;
                 if (CFAIL(--cd[u], k)) {
                     state[u] = 1;
@@ -161,6 +192,12 @@ static int32_t peel(uint32_t V, const uint64_t* koff, const uint32_t* knbr,
     free(pnext);
     free(head);
     return last;
+}
+
+static int32_t peel(uint32_t V, const uint64_t* koff, const uint32_t* knbr,
+                    const uint64_t* coff, const uint32_t* cnbr, int32_t k,
+                    int32_t* l) {
+    return peel_n(V, koff, knbr, coff, cnbr, k, l, 0);
 }
 
 int32_t dco_klist(uint32_t V, const uint64_t* out_off, const uint32_t* out_nbr,
@@ -307,4 +344,122 @@ int32_t dco_kmax_hindex(uint32_t V, const uint64_t* in_off,
     free(nxt);
     free(hist);
     return kmax;
+}
+
+/* ---- streaming state hash ---------------------------------------------
+ * The hash that dcx_state_hash defines (include/dcx.h): the 64-bit sum over
+ * every vertex v and k = -1..kmax of mix64(v*G1 + (k+1)*G2 + value), value
+ * = kmaxes[v] for k = -1 and lmaxes[k][v] otherwise (0 outside the
+ * (k,0)-core), mix64 the SplitMix64 finaliser.  Each row is computed, added
+ * and dropped, so a scale-24 decomposition needs O(V) per thread instead of
+ * (kmax+1) * V cells.  Row k is peeled on its (k,0)-core alone: vertices
+ * relabelled by kmax descending (ties by id) make the core the rank prefix
+ * [0, V_k), and lists sorted by rank make each member's member-neighbours
+ * a list prefix (peel_n in prefix mode).  The values are those of the
+ * whole-graph peel: a vertex outside the (k,0)-core fails the constraint at
+ * level 0 there and cannot affect a member's degrees afterwards. */
+static uint64_t hmix(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+#define HG1 0x9E3779B97F4A7C15ULL
+#define HG2 0xC2B2AE3D27D4EB4FULL
+
+static int cmp_u32(const void* a, const void* b) {
+    uint32_t x = *(const uint32_t*)a, y = *(const uint32_t*)b;
+    return x < y ? -1 : x > y;
+}
+
+/* Lists of one side in rank space, each sorted. */
+static void rank_lists(uint32_t V, const uint64_t* off, const uint32_t* nbr,
+                       const uint32_t* order, const uint32_t* rank,
+                       uint64_t* roff, uint32_t* rnbr) {
+    roff[0] = 0;
+    for (uint32_t r = 0; r < V; ++r) {
+        uint32_t v = order[r];
+        roff[r + 1] = roff[r] + (off[v + 1] - off[v]);
+    }
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 4096)
+#endif
+    for (int64_t r = 0; r < (int64_t)V; ++r) {
+        uint32_t v = order[r];
+        uint64_t o = roff[r];
+        for (uint64_t i = off[v]; i < off[v + 1]; ++i) rnbr[o++] = rank[nbr[i]];
+        qsort(rnbr + roff[r], roff[r + 1] - roff[r], sizeof(uint32_t), cmp_u32);
+    }
+}
+
+uint64_t dco_state_hash(uint32_t V, const uint64_t* out_off, const uint32_t* out_nbr,
+                        const uint64_t* in_off, const uint32_t* in_nbr, int nthreads,
+                        int32_t* kmax_out, int32_t* lmax_out, uint64_t* row_hash) {
+    if (kmax_out) *kmax_out = -1;
+    if (lmax_out) *lmax_out = -1;
+    if (V == 0) return 0;
+    int32_t* km = (int32_t*)malloc(sizeof(int32_t) * V);
+    int32_t kmax = peel(V, in_off, in_nbr, out_off, out_nbr, 0, km);
+    /* order by kmax descending, ties by id */
+    uint32_t* cnt = (uint32_t*)calloc((size_t)kmax + 2, sizeof(uint32_t));
+    for (uint32_t v = 0; v < V; ++v) cnt[kmax - km[v]]++;
+    uint32_t* start = (uint32_t*)malloc(sizeof(uint32_t) * ((size_t)kmax + 2));
+    uint32_t acc = 0;
+    for (int32_t j = 0; j <= kmax; ++j) {
+        start[j] = acc;
+        acc += cnt[j];
+    }
+    uint32_t* order = (uint32_t*)malloc(sizeof(uint32_t) * V);
+    uint32_t* rank = (uint32_t*)malloc(sizeof(uint32_t) * V);
+    for (uint32_t v = 0; v < V; ++v) {
+        uint32_t r = start[kmax - km[v]]++;
+        order[r] = v;
+        rank[v] = r;
+    }
+    /* V_k = members of the (k,0)-core = ranks below vk[k] */
+    uint32_t* vk = (uint32_t*)calloc((size_t)kmax + 2, sizeof(uint32_t));
+    for (uint32_t v = 0; v < V; ++v) vk[km[v]]++;
+    for (int32_t k = kmax - 1; k >= 0; --k) vk[k] += vk[k + 1];
+    uint64_t* coff_o = (uint64_t*)malloc(sizeof(uint64_t) * ((size_t)V + 1));
+    uint64_t* coff_i = (uint64_t*)malloc(sizeof(uint64_t) * ((size_t)V + 1));
+    uint32_t* cnbr_o = (uint32_t*)malloc(sizeof(uint32_t) * (out_off[V] ? out_off[V] : 1));
+    uint32_t* cnbr_i = (uint32_t*)malloc(sizeof(uint32_t) * (in_off[V] ? in_off[V] : 1));
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+    omp_set_num_threads(nthreads);
+#endif
+    rank_lists(V, out_off, out_nbr, order, rank, coff_o, cnbr_o);
+    rank_lists(V, in_off, in_nbr, order, rank, coff_i, cnbr_i);
+    uint64_t total = 0;
+    for (uint32_t v = 0; v < V; ++v) total += hmix(v * HG1 + (uint64_t)(uint32_t)km[v]);
+    if (row_hash) row_hash[0] = total;
+    int32_t lmax = 0;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : total) reduction(max : lmax) num_threads(nthreads)
+#endif
+    for (int32_t k = 0; k <= kmax; ++k) {
+        uint32_t n = vk[k];
+        int32_t* l = (int32_t*)malloc(sizeof(int32_t) * (n ? n : 1));
+        int32_t lk = peel_n(n, coff_o, cnbr_o, coff_i, cnbr_i, k, l, 1);
+        uint64_t h = 0;
+        const uint64_t kk = (uint64_t)(k + 1) * HG2;
+        for (uint32_t r = 0; r < V; ++r)
+            h += hmix(order[r] * HG1 + kk + (r < n ? (uint64_t)(uint32_t)l[r] : 0));
+        free(l);
+        if (row_hash) row_hash[k + 1] = h;
+        total += h;
+        if (lk > lmax) lmax = lk;
+    }
+    if (kmax_out) *kmax_out = kmax;
+    if (lmax_out) *lmax_out = lmax;
+    free(km);
+    free(cnt);
+    free(start);
+    free(order);
+    free(rank);
+    free(vk);
+    free(coff_o);
+    free(coff_i);
+    free(cnbr_o);
+    free(cnbr_i);
+    return total;
 }

@@ -61,6 +61,18 @@ int32_t dco_kmax_hindex(uint32_t V,
                         const uint64_t* in_off, const uint32_t* in_nbr,
                         int32_t* kmaxes, int32_t* iters);
 
+/* Hash of the whole decomposition (the formula of dcx_state_hash in the
+ * product's include/dcx.h) computed one row at a time on its (k,0)-core and
+ * never stored: memory O(V + E) plus O(V) per thread, so scale-24 graphs fit.
+ * *kmax_out, *lmax_out receive kmax and lmax; row_hash (may be NULL, else
+ * kmax + 2 entries, sized from a previous dco_kmax) receives
+ * the kmaxes' part at [0] and row k's at [k + 1]. */
+uint64_t dco_state_hash(uint32_t V,
+                        const uint64_t* out_off, const uint32_t* out_nbr,
+                        const uint64_t* in_off, const uint32_t* in_nbr,
+                        int nthreads, int32_t* kmax_out, int32_t* lmax_out,
+                        uint64_t* row_hash);
+
 #ifdef __cplusplus
 }
 #endif

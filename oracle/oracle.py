@@ -48,6 +48,9 @@ def lib():
         L.dco_kmax_hindex.argtypes = [ctypes.c_uint32, _u64p, _u32p, _i32p,
                                       ctypes.POINTER(ctypes.c_int32)]
         L.dco_kmax_hindex.restype = ctypes.c_int32
+        L.dco_state_hash.argtypes = csr + [ctypes.c_int, ctypes.POINTER(ctypes.c_int32),
+                                           ctypes.POINTER(ctypes.c_int32), ctypes.c_void_p]
+        L.dco_state_hash.restype = ctypes.c_uint64
         _lib = L
     return _lib
 
@@ -121,6 +124,23 @@ def decompose(g, nthreads=0):
     rows = np.empty((max(K + 1, 0), g.V), dtype=np.int32)
     lmax = int(lib().dco_lrows(*g.args(), K, rows, int(nthreads))) if K >= 0 else 0
     return K, km, lmax, rows
+
+
+def state_hash(g, nthreads=0, rows=False):
+    """dcx_state_hash of the decomposition of g, computed row by row on each
+    (k,0)-core without storing the rows.  Returns (hash, kmax, lmax), plus
+    the per-part hashes (kmaxes, then row 0..kmax) when rows=True."""
+    K = ctypes.c_int32(0)
+    L = ctypes.c_int32(0)
+    part = None
+    if rows:
+        kk, _ = kmax(g)
+        part = np.zeros(max(kk + 2, 1), np.uint64)
+    h = lib().dco_state_hash(*g.args(), int(nthreads), ctypes.byref(K), ctypes.byref(L),
+                             None if part is None else part.ctypes.data_as(ctypes.c_void_p))
+    if rows:
+        return int(h), K.value, L.value, part
+    return int(h), K.value, L.value
 
 
 def klist_hindex(g, kmaxes, k):

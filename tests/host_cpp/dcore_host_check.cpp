@@ -20,7 +20,7 @@
 
 #include "dcore.hpp"
 
-using namespace dcore;
+using namespace dcx_host;
 
 static int failures = 0, checks = 0;
 #define CHECK(cond)                                                        \
@@ -49,7 +49,19 @@ static bool consistent(const Graph& g) {
         so += g.out_degrees[v];
         si += g.in_degrees[v];
     }
-    return so == g.E && si == g.E;
+    if (so != g.E || si != g.E) return false;
+    // every list within its slots and holding exactly the graph's edges
+    std::multiset<std::pair<vertex, vertex>> want(g.edges.begin(), g.edges.end()), outs, ins;
+    for (unsigned v = 0; v < g.V; ++v) {
+        if (g.out_neighbors_offset[v] + g.out_degrees[v] > g.out_neighbors_offset[v + 1] ||
+            g.in_neighbors_offset[v] + g.in_degrees[v] > g.in_neighbors_offset[v + 1])
+            return false;
+        for (degree j = 0; j < g.out_degrees[v]; ++j)
+            outs.emplace(v, g.out_neighbors[g.out_neighbors_offset[v] + j]);
+        for (degree j = 0; j < g.in_degrees[v]; ++j)
+            ins.emplace(g.in_neighbors[g.in_neighbors_offset[v] + j], v);
+    }
+    return outs == want && ins == want;
 }
 
 static std::vector<std::pair<vertex, vertex>> random_new_edges(Graph& g, unsigned n) {
@@ -132,6 +144,20 @@ static void files(const std::string& dir) {
     CHECK(gg.E == 2005 && consistent(gg));
     gg.deleteEdgesInPlace(add);
     CHECK(gg.E == 2000 && consistent(gg));
+    // several edges into one list in one in-place batch: the gap of 1 holds
+    // only one, so the graph is rebuilt instead of overrunning the next list
+    {
+        std::vector<std::pair<vertex, vertex>> fan;
+        for (vertex x = 0; x < gg.V && fan.size() < 3; ++x) {
+            bool present = false;
+            for (auto& e : gg.edges) present |= e == std::make_pair((vertex)5, x);
+            if (x != 5 && !present) fan.emplace_back(5, x);
+        }
+        gg.insertEdgesInPlace(fan);
+        CHECK(gg.E == 2003 && consistent(gg));
+        gg.deleteEdgesInPlace(fan);
+        CHECK(gg.E == 2000 && consistent(gg));
+    }
 }
 
 static void device(const std::string& dir) {
@@ -140,7 +166,7 @@ static void device(const std::string& dir) {
     Graph g(dir + "/digraph");
     Graph golden(dir + "/digraph");
     CHECK(readDecompFile(golden, dir + "/digraph"));
-    dcore::dcore(dev, g);
+    dcx_host::dcore(dev, g);
     CHECK(g.kmax == golden.kmax && g.lmax == golden.lmax);
     CHECK(g.kmaxes == golden.kmaxes && g.lmaxes == golden.lmaxes);
 
@@ -158,8 +184,8 @@ static void device(const std::string& dir) {
     // a bigger graph: every variant gives the same result
     generateGraph(dir + "/big", 4000, 40000, 3);
     Graph b1(dir + "/big", 1, false), b2(dir + "/big", 1, false);
-    dcore::dcore(dev, b1, DCX_VARIANT_BATCHED);
-    dcore::dcore(dev, b2, DCX_VARIANT_PEEL);
+    dcx_host::dcore(dev, b1, DCX_VARIANT_BATCHED);
+    dcx_host::dcore(dev, b2, DCX_VARIANT_PEEL);
     CHECK(b1.kmax == b2.kmax && b1.kmaxes == b2.kmaxes && b1.lmaxes == b2.lmaxes);
 
     // maintenance (the SINGLE_INSERT / SPEED_TEST flow): inserts, deletes,

@@ -150,6 +150,48 @@ def test_lrows_thread_count_invariant():
     assert la == lb and np.array_equal(a, b)
 
 
+def _stream_cases():
+    out = [oracle.load_text_graphcpu(os.path.join(GOLDEN, "digraph"))]
+    out += [graphs.random_gnm(700, 9000, 80 + sd) for sd in range(3)]
+    out += [graphs.random_digraph(12, 0.35, sd) for sd in range(5)]
+    out += [graphs.complete(7), graphs.cycle(11), graphs.out_star(40), graphs.in_star(40),
+            graphs.bi_kab(4, 9), graphs.circulant(30, 5), graphs.dag(40, 0.2, 3)]
+    # isolated vertices, hubs, and a graph with no edges
+    V, s, d = graphs.random_gnm(300, 2500, 7)
+    out.append((V + 50, s, d))
+    out.append((6, np.array([], np.int64), np.array([], np.int64)))
+    return out
+
+
+def test_streaming_hash_equals_dense():
+    """dco_state_hash (each row peeled on its (k,0)-core in kmax-rank order,
+    added to the hash and dropped) equals the dcx_state_hash formula applied
+    to the dense decomposition of the pinned whole-graph peel, part by part
+    (kmaxes and every row), on the reference's fixture, random graphs and
+    closed forms; kmax and lmax agree too."""
+    for V, s, d in _stream_cases():
+        g = oracle.csr_from_edges(V, s, d)
+        K, km, lmax, rows = oracle.decompose(g)
+        h, K2, L2, parts = oracle.state_hash(g, rows=True)
+        assert (K2, L2) == (K, lmax)
+        assert h == graphs.state_hash(km, rows)
+        assert int(parts[0]) == graphs.state_hash(km, [])
+        # the sum of the parts is the hash (mod 2^64)
+        assert int(np.sum(parts, dtype=np.uint64)) == h
+
+
+def test_streaming_hash_catches_a_changed_cell():
+    """The hash moves when one cell of one row changes (the check a wrong
+    row value would fail)."""
+    V, s, d = graphs.random_gnm(500, 7000, 3)
+    g = oracle.csr_from_edges(V, s, d)
+    K, km, lmax, rows = oracle.decompose(g)
+    h, _, _ = oracle.state_hash(g)
+    bad = rows.copy()
+    bad[K // 2][int(np.argmax(rows[K // 2]))] -= 1
+    assert graphs.state_hash(km, bad) != h
+
+
 PLANTS = {
     1: "constraint threshold off by one (<= k instead of < k)",
     2: "key decrement applied to out-neighbours instead of in-neighbours",
