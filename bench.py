@@ -73,8 +73,16 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="rmat",
-                    choices=["rmat", "chunglu256", "serve", "learned", "ingest"])
+    ap.add_argument("--workload", default="headline",
+                    choices=["headline", "rmat", "chunglu256", "serve", "learned", "ingest"])
+    ap.add_argument("--schedule", default="learned", choices=["learned", "workshare"],
+                    help="headline: team widths from the cost model or the work-share rule")
+    ap.add_argument("--quick-train", action="store_true",
+                    help="headline: train the cost model on the graph's own passes only")
+    ap.add_argument("--plain", action="store_true",
+                    help="headline: also time the plain one-k-per-launch peel")
+    ap.add_argument("--no-rmat20", action="store_true",
+                    help="headline: skip the R-MAT 20 seed 1 timing")
     ap.add_argument("--ingest-scale", type=int, default=22,
                     help="ingest: R-MAT scale of the edge list")
     ap.add_argument("--learned-scale", type=int, default=24,
@@ -85,7 +93,8 @@ def main():
     ap.add_argument("--serve-batches", type=int, default=10000)
     ap.add_argument("--serve-loads", default="0.5,0.9",
                     help="serve: Poisson loads as fractions of the measured capacity")
-    ap.add_argument("--scale", type=int, default=20)
+    ap.add_argument("--scale", type=int, default=None,
+                    help="R-MAT scale (headline: 24, rmat: 20)")
     ap.add_argument("--variant", default="batched", choices=["batched", "peel", "hindex"])
     ap.add_argument("--no-plain", action="store_true", help="skip the plain-variant timing")
     ap.add_argument("--no-hindex", action="store_true", help="skip the h-index timing")
@@ -115,7 +124,11 @@ def main():
         return run_learned(args)
     if args.workload == "ingest":
         return run_ingest(args)
-    return run_rmat(args)
+    if args.workload == "rmat":
+        args.scale = args.scale or 20
+        return run_rmat(args)
+    args.scale = args.scale or 24
+    return run_headline(args)
 
 
 def run_ingest(args):
@@ -210,6 +223,244 @@ def run_ingest(args):
                                    "Graph::readFile) plus its upload, same edge list"},
     }
     print(json.dumps(line), flush=True)
+
+
+def read_pmc(name):
+    """Calibrated HBM bytes per launch from a committed rocprofv3 PMC summary
+    under profiles/ (a 'traffic_bytes_per_launch:' line), or None."""
+    try:
+        for line in open(os.path.join(ROOT, "profiles", name)):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
+def cpu_info():
+    model = platform.processor() or platform.machine()
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return model, os.cpu_count() or 1, avail
+
+
+def oracle_sample(g, V, threads, log):
+    """The CPU oracle on a bounded sample of the same decomposition, on the
+    graph downloaded from the device: the kmax pass, the relabelling, and
+    every stride-th row on `threads` threads, then three rows on one thread;
+    each scaled to the whole decomposition by the rows' algorithmic bytes."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    t0 = time.perf_counter()
+    c = g.download_csr()
+    csr = oracle.CSR(V, c["out_off"].astype(np.uint64), c["out_nbr"],
+                     c["in_off"].astype(np.uint64), c["in_nbr"])
+    del c
+    log(f"cpu baseline: graph downloaded in {time.perf_counter() - t0:.1f}s")
+    K, t, w_s, w_all = oracle.time_sample(csr, threads, 16)
+    est = t["kmax_s"] + t["order_s"] + t["rows_s"] * w_all / w_s
+    log(f"cpu baseline ({threads} threads): {t}, estimate {est:.1f}s")
+    K1, t1, w1, _ = oracle.time_sample(csr, 1, max(K // 3, 1))
+    est1 = t["kmax_s"] + t["order_s"] * threads + t1["rows_s"] * w_all / w1
+    log(f"cpu baseline (1 thread): rows {t1['rows_s']:.1f}s, estimate {est1:.1f}s")
+    return {"threads": threads, "parts_s": t, "rows_sampled_frac": round(w_s / w_all, 4),
+            "estimate_s": est, "one_thread": {"rows_s": t1["rows_s"],
+                                              "rows_sampled_frac": round(w1 / w_all, 4),
+                                              "estimate_s": est1}}
+
+
+def run_headline(args):
+    """R-MAT scale 24 (2^28 samples, seed 4), learned schedule: the
+    learned-scheduling workload's R-MAT graph, the largest single-GPU
+    decomposition the targets are stated on."""
+    import dcx
+
+    def log(msg):
+        print(f"# {msg}", file=sys.stderr, flush=True)
+
+    scale, seed = args.scale, (4 if args.scale == 24 else 1)
+    V, samples = 1 << scale, 16 << scale
+    ctx = dcx.Context(0)
+    tg0 = time.perf_counter()
+    s, d = ctx.gen_rmat_device(scale, samples, seed)
+    tg1 = time.perf_counter()
+    g = ctx.ingest(V, s, d, 0, drop_self_loops=True, dedup=True)
+    tg2 = time.perf_counter()
+    s.free()
+    d.free()
+    E = int(g.info()[1])
+    log(f"rmat{scale}: V={V} E={E}; device sampling {1e3 * (tg1 - tg0):.0f} ms, "
+        f"dedup ingest {1e3 * (tg2 - tg1):.0f} ms")
+
+    # the fixed work-share schedule, then the cost model trained on the GPU
+    # from logged passes (smaller graphs at several team caps, and this one)
+    ctx.set_scheduler(None)
+    g.decompose(dcx.VARIANT_BATCHED)
+    heur = []
+    for _ in range(2):
+        g.decompose(dcx.VARIANT_BATCHED)
+        heur.append(ctx.timing().total_ms)
+    h_hash = g.state_hash()
+    model = None
+    if args.schedule == "learned":
+        Xs, Ts = learned_training_set(dcx, log) if not args.quick_train else ([], [])
+        for wl in (True, False):
+            X, T = g.pass_features(with_log=wl)
+            Xs.append(X)
+            Ts.append(T)
+        X, T = np.concatenate(Xs), np.concatenate(Ts)
+        model = dcx.MLP(ctx, seed=7)
+        loss = model.train(X, T, epochs=2000, lr=3e-3)
+        log(f"cost model trained on {len(T)} samples, loss {loss:.4f}")
+        ctx.set_scheduler(model)
+    for _ in range(args.warmup):
+        g.decompose(dcx.VARIANT_BATCHED)
+    tims = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        K, lmax = g.decompose(dcx.VARIANT_BATCHED)
+        tims.append(ctx.timing())
+    wall = time.perf_counter() - t0
+    ms_per_step = wall / args.steps * 1e3
+    state_hash = g.state_hash()
+    used_model = bool(tims[-1].rows_scheduler)
+    log(f"steps done: {ms_per_step:.1f} ms/step kmax={K} lmax={lmax} (work-share "
+        f"{min(heur):.1f} ms)")
+
+    mean = lambda f: float(np.mean([getattr(t, f) for t in tims]))  # noqa: E731
+    log_rows = g.pass_log()
+    vk = np.array([p["members"] for p in log_rows[1:]], np.float64)
+    ek = np.array([p["edges"] for p in log_rows[1:]], np.float64)
+    per_row = 8 * ek + 20 * vk
+    b_kmax = 8.0 * E + 20.0 * V
+    split = int(tims[-1].rows_split)
+    first = 1 if mean("rows_first_ms") > 0 else 0
+    teams_ms, lds_ms = mean("rows_teams_ms"), mean("rows_lds_ms")
+    kernels = {
+        "k_rows_teams": (float(per_row[first:split].sum()), teams_ms, f"{first}..{split - 1}"),
+        "k_rows_lds": (float(per_row[split:].sum()), lds_ms, f"{split}..{K}"),
+        "k_pass_grid (kmax pass)": (b_kmax, mean("kmax_ms"), "kmax pass"),
+    }
+    dom = max(kernels, key=lambda k: kernels[k][1])
+    b_dom, ms_dom, rows_dom = kernels[dom]
+    achieved = b_dom / (ms_dom * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 5),
+                "traffic": read_pmc(f"k_rows_teams_rmat{scale}_pmc.txt")
+                if dom == "k_rows_teams" else None,
+                "kernel": dom.split(" ")[0], "kernel_ms": round(ms_dom, 3), "alg_bytes": b_dom,
+                "rows": rows_dom}
+    b_all = float(per_row.sum()) + b_kmax
+    extra = {
+        "kernels": {k: {"alg_bytes": v[0], "ms": round(v[1], 3), "rows": v[2],
+                        "GBs": round(v[0] / (v[1] * 1e-3) / 1e9, 2) if v[1] > 0 else None}
+                    for k, v in kernels.items()},
+        "full_decomposition": {"alg_bytes": b_all, "ms": round(ms_per_step, 3),
+                               "roofline_frac": round(b_all / (ms_per_step * 1e-3) / 1e9 /
+                                                      HBM_PEAK_GBS, 5),
+                               "edge_passes_per_s": round(float(ek.sum() + E) /
+                                                          (ms_per_step * 1e-3), 1)},
+        "breakdown_ms": {"kmax_pass": round(mean("kmax_ms"), 3),
+                         "order": round(mean("order_ms"), 3),
+                         "rows_kernel": round(mean("rows_kernel_ms"), 3),
+                         "rows_total": round(mean("rows_ms"), 3)},
+        "schedule": {"kind": args.schedule, "model_used": used_model,
+                     "work_share_ms": round(min(heur), 2),
+                     "learned_speedup_vs_work_share": round(min(heur) / ms_per_step, 3),
+                     "bit_exact_vs_work_share": state_hash == h_hash},
+        "ingest_ms": {"device_sampling": round(1e3 * (tg1 - tg0), 1),
+                      "dedup_ingest": round(1e3 * (tg2 - tg1), 1)},
+    }
+    gold = os.path.join(ROOT, "tests", "golden", "hashes.json")
+    if os.path.exists(gold):
+        want = json.load(open(gold)).get(f"rmat{scale}_s{seed}")
+        if want:
+            extra["oracle_hash_equal"] = want["hash"] == f"{state_hash:016x}"
+    if model is not None:
+        ctx.set_scheduler(None)
+        model.free()
+    if args.plain:
+        g.decompose(dcx.VARIANT_PEEL)
+        tp = ctx.timing()
+        extra["plain_peel_ms"] = round(tp.total_ms, 1)
+        extra["speedup_vs_plain"] = round(tp.total_ms / ms_per_step, 2)
+        extra["plain_bit_exact"] = g.state_hash() == state_hash
+        log(f"plain one-k-per-launch peel: {tp.total_ms:.0f} ms")
+    cpu = None
+    if not args.no_cpu:
+        model_name, nproc, avail = cpu_info()
+        threads = args.cpu_threads or min(avail, 16)
+        smp = oracle_sample(g, V, threads, log)
+        cpu = {"value": round(E / smp["estimate_s"] / 1e9, 6), "unit": "GEdges/s",
+               "cores": threads, "kind": "port",
+               "sample": (f"C oracle on the same rmat{scale} graph (downloaded from the device): "
+                          f"kmax pass {smp['parts_s']['kmax_s']:.1f}s, kmax-rank relabelling "
+                          f"{smp['parts_s']['order_s']:.1f}s, every 16th row "
+                          f"({100 * smp['rows_sampled_frac']:.1f}% of the rows' bytes) "
+                          f"{smp['parts_s']['rows_s']:.1f}s on {threads} threads, scaled by "
+                          f"bytes to all rows: {smp['estimate_s']:.1f}s per decomposition"),
+               "cpu_model": model_name, "nproc": nproc, "cpus_available": avail,
+               "estimate_s": round(smp["estimate_s"], 2),
+               "one_thread_estimate_s": round(smp["one_thread"]["estimate_s"], 1),
+               "one_thread_value": round(E / smp["one_thread"]["estimate_s"] / 1e9, 6)}
+    if not args.no_rmat20:
+        g.free()
+        extra["rmat20"] = rmat20_quick(ctx, dcx, log)
+    else:
+        g.free()
+    ctx.close()
+    line = {
+        "metric": "full D-core decomposition throughput (GEdges/s = E_dedup / T_full)",
+        "value": round(E / (ms_per_step * 1e-3) / 1e9, 6),
+        "unit": "GEdges/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": f"rmat{scale} full decomposition (kmax pass + all l-rows), "
+                               f"{args.schedule} schedule",
+                   "V": V, "E": E, "samples": samples, "abcd": [0.57, 0.19, 0.19, 0.05],
+                   "seed": seed, "kmax": K, "lmax": lmax, "state_hash": f"{state_hash:016x}"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    line.update(extra)
+    print(json.dumps(line), flush=True)
+
+
+def rmat20_quick(ctx, dcx, log):
+    """R-MAT 20 seed 1 (the plain-GPU workload) in the same run: default
+    launch shape, mean of 10 decompositions."""
+    s, d = ctx.gen_rmat_device(20, 16 << 20, 1)
+    g = ctx.ingest(1 << 20, s, d, 0, drop_self_loops=True, dedup=True)
+    s.free()
+    d.free()
+    for _ in range(3):
+        g.decompose(dcx.VARIANT_BATCHED)
+    t0 = time.perf_counter()
+    for _ in range(10):
+        K, L = g.decompose(dcx.VARIANT_BATCHED)
+    ms = (time.perf_counter() - t0) / 10 * 1e3
+    out = {"ms_per_step": round(ms, 3), "E": int(g.info()[1]), "kmax": K, "lmax": L,
+           "state_hash": f"{g.state_hash():016x}"}
+    g.free()
+    log(f"rmat20: {ms:.2f} ms per decomposition")
+    return out
 
 
 def run_rmat(args):

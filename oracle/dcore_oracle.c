@@ -463,3 +463,104 @@ uint64_t dco_state_hash(uint32_t V, const uint64_t* out_off, const uint32_t* out
     free(cnbr_i);
     return total;
 }
+
+/* ---- timed sample (the CPU baseline of a benchmark) --------------------
+ * Runs, on the host threads, the parts of one oracle decomposition: the
+ * kmax pass, the kmax-rank relabelling of dco_state_hash, and the rows
+ * k = 0, stride, 2*stride, ... (each peeled on its (k,0)-core, rows in
+ * parallel over nthreads); the rest of the rows are not run.  Returns kmax;
+ * t[0..2] = wall seconds of the three parts, w[0] = sum of (8 E_k + 20 V_k)
+ * over the sampled rows and w[1] over all rows (the algorithmic bytes of
+ * the rows, the unit by which a caller scales the sample up to the whole
+ * decomposition). */
+#ifdef _OPENMP
+static double now_s(void) { return omp_get_wtime(); }
+#else
+#include <time.h>
+static double now_s(void) {
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+#endif
+
+int32_t dco_time_sample(uint32_t V, const uint64_t* out_off, const uint32_t* out_nbr,
+                        const uint64_t* in_off, const uint32_t* in_nbr, int nthreads,
+                        int32_t stride, double* t, double* w) {
+    t[0] = t[1] = t[2] = 0;
+    w[0] = w[1] = 0;
+    if (V == 0) return -1;
+    if (stride < 1) stride = 1;
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+    omp_set_num_threads(nthreads);
+#endif
+    double t0 = now_s();
+    int32_t* km = (int32_t*)malloc(sizeof(int32_t) * V);
+    int32_t kmax = peel(V, in_off, in_nbr, out_off, out_nbr, 0, km);
+    double t1 = now_s();
+    uint32_t* cnt = (uint32_t*)calloc((size_t)kmax + 2, sizeof(uint32_t));
+    for (uint32_t v = 0; v < V; ++v) cnt[kmax - km[v]]++;
+    uint32_t* start = (uint32_t*)malloc(sizeof(uint32_t) * ((size_t)kmax + 2));
+    uint32_t acc = 0;
+    for (int32_t j = 0; j <= kmax; ++j) {
+        start[j] = acc;
+        acc += cnt[j];
+    }
+    uint32_t* order = (uint32_t*)malloc(sizeof(uint32_t) * V);
+    uint32_t* rank = (uint32_t*)malloc(sizeof(uint32_t) * V);
+    for (uint32_t v = 0; v < V; ++v) {
+        uint32_t r = start[kmax - km[v]]++;
+        order[r] = v;
+        rank[v] = r;
+    }
+    uint32_t* vk = (uint32_t*)calloc((size_t)kmax + 2, sizeof(uint32_t));
+    for (uint32_t v = 0; v < V; ++v) vk[km[v]]++;
+    for (int32_t k = kmax - 1; k >= 0; --k) vk[k] += vk[k + 1];
+    uint64_t* coff_o = (uint64_t*)malloc(sizeof(uint64_t) * ((size_t)V + 1));
+    uint64_t* coff_i = (uint64_t*)malloc(sizeof(uint64_t) * ((size_t)V + 1));
+    uint32_t* cnbr_o = (uint32_t*)malloc(sizeof(uint32_t) * (out_off[V] ? out_off[V] : 1));
+    uint32_t* cnbr_i = (uint32_t*)malloc(sizeof(uint32_t) * (in_off[V] ? in_off[V] : 1));
+    rank_lists(V, out_off, out_nbr, order, rank, coff_o, cnbr_o);
+    rank_lists(V, in_off, in_nbr, order, rank, coff_i, cnbr_i);
+    double t2 = now_s();
+    /* E_k: edges whose endpoints both have kmax >= k */
+    uint64_t* ek = (uint64_t*)calloc((size_t)kmax + 2, sizeof(uint64_t));
+    for (uint32_t u = 0; u < V; ++u)
+        for (uint64_t o = out_off[u]; o < out_off[u + 1]; ++o) {
+            int32_t a = km[u], b = km[out_nbr[o]];
+            ek[a < b ? a : b]++;
+        }
+    for (int32_t k = kmax - 1; k >= 0; --k) ek[k] += ek[k + 1];
+    for (int32_t k = 0; k <= kmax; ++k) {
+        double wk = 8.0 * (double)ek[k] + 20.0 * (double)vk[k];
+        w[1] += wk;
+        if (k % stride == 0) w[0] += wk;
+    }
+    double t3 = now_s();
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+#endif
+    for (int32_t k = 0; k <= kmax; k += stride) {
+        uint32_t n = vk[k];
+        int32_t* l = (int32_t*)malloc(sizeof(int32_t) * (n ? n : 1));
+        peel_n(n, coff_o, cnbr_o, coff_i, cnbr_i, k, l, 1);
+        free(l);
+    }
+    double t4 = now_s();
+    t[0] = t1 - t0;
+    t[1] = t2 - t1;
+    t[2] = t4 - t3;
+    free(km);
+    free(cnt);
+    free(start);
+    free(order);
+    free(rank);
+    free(vk);
+    free(ek);
+    free(coff_o);
+    free(coff_i);
+    free(cnbr_o);
+    free(cnbr_i);
+    return kmax;
+}

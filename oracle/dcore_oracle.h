@@ -73,6 +73,16 @@ uint64_t dco_state_hash(uint32_t V,
                         int nthreads, int32_t* kmax_out, int32_t* lmax_out,
                         uint64_t* row_hash);
 
+/* Timed sample of one decomposition for a CPU baseline: the kmax pass, the
+ * kmax-rank relabelling and the rows k = 0, stride, 2*stride, ... on their
+ * cores.  t[3] receives the wall seconds of the three parts, w[2] the
+ * algorithmic bytes (8 E_k + 20 V_k summed) of the sampled rows and of all
+ * rows.  Returns kmax. */
+int32_t dco_time_sample(uint32_t V,
+                        const uint64_t* out_off, const uint32_t* out_nbr,
+                        const uint64_t* in_off, const uint32_t* in_nbr,
+                        int nthreads, int32_t stride, double* t, double* w);
+
 #ifdef __cplusplus
 }
 #endif

@@ -51,6 +51,9 @@ def lib():
         L.dco_state_hash.argtypes = csr + [ctypes.c_int, ctypes.POINTER(ctypes.c_int32),
                                            ctypes.POINTER(ctypes.c_int32), ctypes.c_void_p]
         L.dco_state_hash.restype = ctypes.c_uint64
+        L.dco_time_sample.argtypes = csr + [ctypes.c_int, ctypes.c_int32, ctypes.c_void_p,
+                                            ctypes.c_void_p]
+        L.dco_time_sample.restype = ctypes.c_int32
         _lib = L
     return _lib
 
@@ -141,6 +144,18 @@ def state_hash(g, nthreads=0, rows=False):
     if rows:
         return int(h), K.value, L.value, part
     return int(h), K.value, L.value
+
+
+def time_sample(g, nthreads=0, stride=1):
+    """Times the oracle on a sample of one decomposition (see dco_time_sample):
+    returns (kmax, {kmax_s, order_s, rows_s}, sampled row bytes, all row bytes)."""
+    t = np.zeros(3, np.float64)
+    w = np.zeros(2, np.float64)
+    K = lib().dco_time_sample(*g.args(), int(nthreads), int(stride),
+                              t.ctypes.data_as(ctypes.c_void_p),
+                              w.ctypes.data_as(ctypes.c_void_p))
+    return int(K), {"kmax_s": float(t[0]), "order_s": float(t[1]), "rows_s": float(t[2])}, \
+        float(w[0]), float(w[1])
 
 
 def klist_hindex(g, kmaxes, k):

@@ -1,0 +1,182 @@
+"""GPU parity at the full sizes of the benchmark workloads, against oracle
+state hashes committed in tests/golden/hashes.json (written by
+tests/golden/make_hashes.py with the CPU oracle's streaming mode, which
+evaluates the dcx_state_hash formula row by row without holding the dense
+rows).  The reference's criterion for maintenance is "maintained ==
+recomputed" (main.cu:1905-1924, 2194-2211); here both the decomposition and
+every maintained state are compared with the oracle's recomputation.
+
+* R-MAT 20 seed 1 (plain GPU workload): batched, plain one-k-per-launch and
+  h-index variants (oracle arrays are compared in full in
+  test_gpu_decompose.py; here the hash of each variant).
+* R-MAT 24 seed 4 and Chung-Lu 2^24 seed 5 (learned-scheduling workload):
+  work-share and learned team widths; the graph is sampled on the device and
+  deduplicated there (the same graph as the host generator's).
+* 256 Chung-Lu graphs (batched workload): their disjoint union in one
+  launch sequence, and each graph's own kmax and lmax.
+* R-MAT 22 seed 2 under 10,000 update batches (serving workload): the
+  continuous-batching server's state after requests 0, 9, 99, 999, 4999 and
+  9999, grouped; the first 100 requests also one per group.
+"""
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import dcx
+import workloads
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+GOLD = json.load(open(os.path.join(GOLDEN, "hashes.json")))
+
+
+def say(msg):
+    print(f"[case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def gold(name):
+    return GOLD[name]
+
+
+@pytest.mark.timeout(600)
+def test_rmat20_variants_equal_oracle_hash():
+    G = gold("rmat20_s1")
+    with dcx.Context(0) as c:
+        say("rmat20 seed 1: device sampling and dedup ingest")
+        s, d = c.gen_rmat_device(20, 16 << 20, 1)
+        g = c.ingest(1 << 20, s, d, 0, drop_self_loops=True, dedup=True)
+        s.free()
+        d.free()
+        assert g.info()[1] == G["E"]
+        for var in (dcx.VARIANT_BATCHED, dcx.VARIANT_PEEL, dcx.VARIANT_HINDEX):
+            say(f"rmat20 variant {var}")
+            K, L = g.decompose(var)
+            assert (K, L) == (G["kmax"], G["lmax"])
+            assert f"{g.state_hash():016x}" == G["hash"], var
+        g.free()
+
+
+def _learned_pair(c, g, G, name):
+    """Work-share widths, then widths from a cost model trained on the GPU
+    from this graph's own pass log: both must give the oracle's hash."""
+    c.set_scheduler(None)
+    say(f"{name}: work-share schedule")
+    K, L = g.decompose(dcx.VARIANT_BATCHED)
+    assert (K, L) == (G["kmax"], G["lmax"])
+    assert f"{g.state_hash():016x}" == G["hash"]
+    X1, T1 = g.pass_features(with_log=True)
+    X0, T0 = g.pass_features(with_log=False)
+    X, T = np.concatenate([X1, X0]), np.concatenate([T1, T0])
+    m = dcx.MLP(c, seed=3)
+    m.train(X, T, epochs=800, lr=3e-3)
+    c.set_scheduler(m)
+    say(f"{name}: learned schedule")
+    K, L = g.decompose(dcx.VARIANT_BATCHED)
+    used = c.timing().rows_scheduler
+    h = g.state_hash()
+    c.set_scheduler(None)
+    m.free()
+    assert (K, L) == (G["kmax"], G["lmax"])
+    assert f"{h:016x}" == G["hash"]
+    return used
+
+
+@pytest.mark.timeout(900)
+def test_rmat24_work_share_and_learned_equal_oracle_hash():
+    G = gold("rmat24_s4")
+    with dcx.Context(0) as c:
+        say("rmat24 seed 4: device sampling and dedup ingest")
+        s, d = c.gen_rmat_device(24, 1 << 28, 4)
+        g = c.ingest(1 << 24, s, d, 0, drop_self_loops=True, dedup=True)
+        s.free()
+        d.free()
+        assert g.info()[1] == G["E"]
+        _learned_pair(c, g, G, "rmat24")
+        g.free()
+
+
+@pytest.mark.timeout(900)
+def test_chunglu24_work_share_and_learned_equal_oracle_hash():
+    G = gold("chunglu24_s5")
+    V = 1 << 24
+    say("chunglu24 seed 5: host generation")
+    s, d = dcx.gen_chunglu(V, 1 << 28, 5)
+    assert s.size == G["E"]
+    with dcx.Context(0) as c:
+        g = c.ingest(V, s, d, 0)
+        del s, d
+        _learned_pair(c, g, G, "chunglu24")
+        g.free()
+
+
+@pytest.mark.timeout(900)
+def test_chunglu256_union_equals_oracle_hash():
+    G = gold("chunglu256")
+    say("chunglu256: host generation")
+    gl = workloads.chunglu_batch()
+    with dcx.Context(0) as c:
+        say("chunglu256: one union, one decomposition")
+        g, vbase = c.upload_batch(gl)
+        K, L = g.decompose(dcx.VARIANT_BATCHED)
+        assert (K, L) == (G["kmax"], G["lmax"])
+        assert f"{g.state_hash():016x}" == G["hash"]
+        km, lm = g.batch_summary(vbase)
+        assert [[int(a), int(b)] for a, b in zip(km, lm)] == G["per_graph_kmax_lmax"]
+        g.free()
+
+
+def _serve_segments(c, g, reqs, checkpoints, **kw):
+    """Submits reqs in segments ending at each checkpoint, waiting for every
+    request of a segment before the next starts, so that a checkpoint
+    request closes its group; returns {checkpoint: state hash} and the
+    results."""
+    out, res = {}, {}
+    with g.serve(hash_states=True, **kw) as sv:
+        start = 0
+        for cp in checkpoints:
+            ids = [sv.submit(ins, dels) for ins, dels in reqs[start:cp + 1]]
+            while not all(i in res for i in ids):
+                for r in sv.poll(wait_ms=200):
+                    res[r["id"]] = r
+            r = res[ids[-1]]
+            assert r["status"] == 0, (cp, r)
+            out[cp] = f"{r['state_hash']:016x}"
+            start = cp + 1
+        stats = sv.stop()
+    return out, res, stats
+
+
+@pytest.mark.timeout(1200)
+def test_serve_rmat22_stream_equals_oracle_hashes():
+    G = gold("serve_rmat22")
+    say("serve rmat22: base graph and 10,000-batch stream")
+    V, s, d = workloads.rmat(22, G["base_seed"])
+    nb = G["nbatches"]
+    io, ip, do, dp = dcx.gen_update_stream(V, s, d, 22, nb, max_batch=G["max_batch"],
+                                           p_insert=G["p_insert"], seed=G["stream_seed"])
+    reqs = [(ip[io[b]:io[b + 1]], dp[do[b]:do[b + 1]]) for b in range(nb)]
+    cps = sorted(int(k) for k in G["checkpoints"])
+    with dcx.Context(0) as c:
+        g = c.upload_edges(V, s, d, slack=2)
+        g.decompose()
+        say("serve rmat22: first 100 requests one per group")
+        alone, _, _ = _serve_segments(c, g, reqs[:100], [x for x in cps if x < 100],
+                                      max_group=1)
+        for cp, h in alone.items():
+            assert h == G["checkpoints"][str(cp)]["hash"], ("alone", cp)
+        g.free()
+        g = c.upload_edges(V, s, d, slack=2)
+        g.decompose()
+        say("serve rmat22: all 10,000 requests grouped")
+        mixed, res, stats = _serve_segments(c, g, reqs, cps)
+        assert stats["groups"] < nb  # requests really were grouped
+        for cp, h in mixed.items():
+            assert h == G["checkpoints"][str(cp)]["hash"], ("grouped", cp)
+        info = g.info()
+        assert info[1] == G["checkpoints"][str(cps[-1])]["E"]
+        g.free()
