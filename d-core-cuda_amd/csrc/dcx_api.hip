@@ -140,7 +140,10 @@ void build_core(dcx_ctx* c, dcx_graph* g) {
 
 // Builds the core-ordered graph for the resident kmaxes unless it is
 // current; the time goes to timing.order_ms.
+void ensure_order(dcx_ctx* c, dcx_graph* g);
+
 void ensure_core(dcx_ctx* c, dcx_graph* g) {
+    ensure_order(c, g);
     if (g->core_valid) return;
     HIPX(c, hipEventRecord(c->ev[7], c->stream));
     build_core(c, g);
@@ -217,6 +220,7 @@ void compute_kmax(dcx_ctx* c, dcx_graph* g, int variant = DCX_VARIANT_BATCHED,
                   bool warm = false, bool clear_rows = false) {
     if (!g->kmaxes) dev_alloc(c, &g->kmaxes, g->V);
     g->core_valid = false;
+    g->order_valid = false;
     if (g->V == 0) {
         g->kmax = -1;
         g->kmax_valid = true;
@@ -342,6 +346,50 @@ void compute_kmax(dcx_ctx* c, dcx_graph* g, int variant = DCX_VARIANT_BATCHED,
             --g->row0_eighths;  // row 0 ended before the kmax pass
     }
     g->kmax_valid = true;
+    g->order_valid = true;
+}
+
+// The kmax order, V_k and E_k of the resident kmaxes when local maintenance
+// changed the kmaxes since they were computed (the core-ordered graph is
+// rebuilt from them when a pass needs it).
+void ensure_order(dcx_ctx* c, dcx_graph* g) {
+    if (g->order_valid || !g->kmax_valid) return;
+    g->core_valid = false;
+    if (g->V == 0 || g->kmax < 0) {
+        g->order_valid = true;
+        return;
+    }
+    const size_t need = (size_t)g->kmax + 2;
+    grow_keep(c, &g->pstats, &g->pstats_cap, need * kStatWords, 0);
+    g->h_pass_variant.assign(need, 0);
+    g->h_pass_team.assign(need, 0);
+    if (g->vk_cap < need) {
+        dev_free(g->vk);
+        dev_free(g->hist);
+        dev_free(g->ekh);
+        dev_alloc(c, &g->vk, need);
+        dev_alloc(c, &g->hist, need);
+        dev_alloc(c, &g->ekh, need);
+        g->vk_cap = need;
+    }
+    if (!g->order) dev_alloc(c, &g->order, g->V);
+    HIPX(c, launch_order_by_kmax(g->kmaxes, g->V, g->kmax, g->hist, g->vk, g->order, c->stream));
+    HIPX(c, launch_core_edges(g->kmaxes, Csr{g->out_off, g->out_deg, g->out_nbr}, g->V, g->kmax,
+                              g->ekh, c->stream));
+    g->h_vk.assign(need, 0);
+    std::vector<uint64_t> eh(need, 0);
+    HIPX(c, hipMemcpyAsync(g->h_vk.data(), g->vk, need * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                           c->stream));
+    HIPX(c, hipMemcpyAsync(eh.data(), g->ekh, need * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                           c->stream));
+    sync(c);
+    g->h_ek.assign((size_t)g->kmax + 1, 0);
+    uint64_t acc = 0;
+    for (int32_t k = g->kmax; k >= 0; --k) {
+        acc += eh[k];
+        g->h_ek[k] = acc;
+    }
+    g->order_valid = true;
 }
 
 // Static schedule of the passes k = 0..kr onto teams of workgroups: a pass
@@ -627,6 +675,7 @@ void describe_fail_record(dcx_ctx* c) {
 // Computes rows k = 0..kr (kr < 0: all rows 0..kmax) with the given
 // variant; rows above kr keep their values (they must be valid already).
 void compute_rows(dcx_ctx* c, dcx_graph* g, int variant, int32_t kr = -1) {
+    ensure_order(c, g);
     const uint32_t V = g->V;
     const int32_t K = g->kmax;
     const bool had_rows = g->rows_valid;
@@ -1014,6 +1063,9 @@ void free_graph_arrays(dcx_graph* g) {
     dev_free(g->cin_nbr);
     dev_free(g->rows);
     dev_free(g->lmax_rows);
+    dev_free(g->ins_cnt_in);
+    dev_free(g->ins_cnt_out);
+    dev_free(g->changed);
 }
 
 }  // namespace
@@ -1142,6 +1194,11 @@ dcx_status dcx_ctx_destroy(dcx_ctx* c) {
     dev_free(c->ctrs2);
     dev_free(c->d_scalars);
     dev_free(c->upd_dev);
+    dev_free(c->local_scratch);
+    dev_free(c->local_tasks);
+    if (c->local_tasks_host) (void)hipHostFree(c->local_tasks_host);
+    dev_free(c->local_words);
+    dev_free(c->local_dm);
     if (c->h_scalars) (void)hipHostFree(c->h_scalars);
     if (c->upd_host) (void)hipHostFree(c->upd_host);
     for (auto& e : c->ev)
@@ -1313,6 +1370,9 @@ dcx_status dcx_graph_pass_log(dcx_ctx* ctx, const dcx_graph* g, dcx_pass_log* ou
     return guarded(ctx, [&]() -> dcx_status {
         *n = 0;
         if (!g->kmax_valid || g->V == 0) return DCX_OK;
+        // sizes of the current cores (local maintenance may have moved them;
+        // the passes' own words describe the last decomposition that ran)
+        ensure_order(ctx, const_cast<dcx_graph*>(g));
         const uint32_t np = (uint32_t)g->kmax + 2;
         *n = np;
         if (!cap) return DCX_OK;

@@ -266,4 +266,52 @@ hipError_t launch_mvalue(const int32_t* kmaxes, const uint32_t* pairs, uint32_t 
 hipError_t launch_fill_i32(int32_t* p, int32_t v, size_t n, hipStream_t st);
 hipError_t launch_max_i32(const int32_t* p, size_t n, int32_t* out, hipStream_t st);
 
+// ---- localized maintenance (local_kernels.hip) --------------------------
+enum : int32_t { kLocalKmaxDel = 1, kLocalRowDel = 2, kLocalCoreIns = 3, kLocalRowIns = 4 };
+struct LocalTaskDesc {
+    int32_t type;  // kLocal*
+    int32_t k;     // row (row tasks); 0 kmaxes / 1 row 0 (core insertion)
+};
+// Inputs of one launch of maintenance tasks (one workgroup per task at a
+// time, tasks taken from *task_ctr).  Graph lists in the slack layout.
+struct LocalArgs {
+    uint32_t V;
+    Csr out, in;
+    int32_t* kmaxes;
+    int32_t* rows;  // row k at rows + k * V
+    const LocalTaskDesc* tasks;
+    uint32_t ntasks;
+    uint32_t* task_ctr;  // zero at launch
+    uint32_t* scratch;   // gridDim.x slots of local_slot_words(qcap) words
+    uint32_t qcap;       // worklist capacity per slot
+    uint32_t* fail;      // bit (type) set when a task of that type overflowed
+    // deletions (applied to the graph already)
+    const uint32_t* del_pairs;
+    uint32_t n_del;
+    const int32_t* del_m;         // per deleted edge: min of its endpoints' old kmaxes
+    const uint32_t* del_targets;  // distinct targets of deleted edges
+    uint32_t n_del_targets;
+    uint32_t* changed;     // (vertex, old kmax) pairs lowered by the deletions
+    uint32_t* n_changed;   // their count (zero before the kmax deletion task)
+    uint32_t changed_cap;
+    // insertions (applied to the graph already)
+    const uint32_t* ins_pairs;
+    uint32_t n_ins;
+    const uint32_t* gin_key;   // insertion groups by target: key, count
+    const uint32_t* gin_cnt;
+    uint32_t n_gin;
+    const uint32_t* gout_key;  // by source
+    const uint32_t* gout_cnt;
+    uint32_t n_gout;
+    const int32_t* ins_cnt_in;   // [V] the batch's entries at the end of each in-list
+    const int32_t* ins_cnt_out;  // [V] same for out-lists
+};
+__host__ __device__ inline size_t local_slot_words(uint32_t qcap) { return 5ull * qcap; }
+hipError_t launch_local_tasks(const LocalArgs& a, uint32_t blocks, hipStream_t st);
+hipError_t launch_edge_min_kmax(const int32_t* kmaxes, const uint32_t* pairs, uint32_t n,
+                                int32_t* out, hipStream_t st);
+hipError_t launch_set_counts(int32_t* cnt, const uint32_t* key, const uint32_t* c, uint32_t n,
+                             bool set, hipStream_t st);
+hipError_t launch_max_masked(const int32_t* x, uint32_t n, int32_t* out, hipStream_t st);
+
 }  // namespace dcx

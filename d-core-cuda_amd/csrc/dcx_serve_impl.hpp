@@ -222,6 +222,7 @@ struct dcx_server {
                 int32_t m_del = -1;
                 std::vector<uint32_t> ins_all;
                 bool changed = false, any_ins = false;
+                EditLog log;  // the group's net effect, for local maintenance
                 for (size_t i = 0; i < grp.size(); ++i) {
                     ServeReq* r = grp[i];
                     HIPX(c, hipStreamWaitEvent(c->stream, r->up, 0));
@@ -231,14 +232,22 @@ struct dcx_server {
                                         r->ins.data(), r->del.data(), &md, &rel);
                     if (sts[i] != DCX_OK) continue;
                     m_del = std::max(m_del, md);
+                    log.add(r->del.data(), (uint32_t)(r->del.size() / 2), false);
+                    log.add(r->ins.data(), (uint32_t)(r->ins.size() / 2), true);
                     ins_all.insert(ins_all.end(), r->ins.begin(), r->ins.end());
                     changed = changed || !r->ins.empty() || !r->del.empty();
                     any_ins = any_ins || !r->ins.empty();
                     U.relocated += rel;
                 }
-                if (g->kmax_valid)
+                if (g->kmax_valid && changed && local_enabled(c, g)) {
+                    const NetEdits N = log.net();
+                    U.net_deleted = (int32_t)(N.del.size() / 2);
+                    U.net_inserted = (int32_t)(N.ins.size() / 2);
+                    U.path = local_maintain(c, g, N, &U);
+                } else if (g->kmax_valid) {
                     maintain(c, g, ins_all.data(), (uint32_t)(ins_all.size() / 2), m_del, changed,
                              !any_ins, &U);
+                }
                 U.kmax_after = g->kmax_valid ? g->kmax : -1;
                 if (cfg.hash_states) hash = state_hash(c, g);
                 return DCX_OK;

@@ -65,6 +65,18 @@ struct dcx_ctx {
     size_t sched_bytes = 0;
     uint32_t* pools = nullptr;  // team sync words + team slots
     size_t pools_words = 0;
+    // localized maintenance: per-slot worklists, task list, small words
+    // ([0] task counter, [1] recorded kmax changes, [2] failure bits,
+    // [3] largest kmax, [4] largest row-0 value, [5] edits not found), and
+    // the per-edge old kmax minimum of the deletions
+    uint32_t* local_scratch = nullptr;
+    size_t local_scratch_bytes = 0;
+    LocalTaskDesc* local_tasks = nullptr;
+    LocalTaskDesc* local_tasks_host = nullptr;  // pinned
+    size_t local_tasks_cap = 0;
+    uint32_t* local_words = nullptr;
+    int32_t* local_dm = nullptr;
+    size_t local_dm_cap = 0;
     // staging for update batches (device + pinned host), grown on demand
     uint32_t* upd_dev = nullptr;
     size_t upd_dev_words = 0;
@@ -162,6 +174,14 @@ struct dcx_graph {
     size_t lmax_rows_cap = 0;
     int32_t lmax = -1;
     bool rows_valid = false;
+    // the kmax order, V_k and E_k follow the resident kmaxes (false after
+    // local maintenance changed kmaxes; recomputed when a pass needs them)
+    bool order_valid = false;
+    // localized maintenance: the batch's entries at the end of each list
+    // ([V], zero outside a maintenance), and the vertices whose kmax fell
+    int32_t* ins_cnt_in = nullptr;
+    int32_t* ins_cnt_out = nullptr;
+    uint32_t* changed = nullptr;  // [2 V]
 };
 
 namespace {

@@ -22,6 +22,7 @@
 #pragma once
 
 #include <chrono>
+#include <unordered_map>
 #include <unordered_set>
 
 namespace {
@@ -360,6 +361,226 @@ void maintain(dcx_ctx* c, dcx_graph* g, const uint32_t* ins, uint32_t n_ins, int
     }
 }
 
+// ---- localized maintenance ---------------------------------------------
+
+// Net effect of validated edits applied in order (per request: deletions,
+// then insertions): edges present before and absent after (del), and
+// absent before and present after (ins).  An edge removed and put back, or
+// put in and removed again, is in neither.
+struct NetEdits {
+    std::vector<uint32_t> del, ins;
+};
+
+struct EditLog {
+    std::unordered_map<uint64_t, uint8_t> first_last;  // bit 0: first op inserts, bit 1: last op inserts
+    void add(const uint32_t* p, uint32_t n, bool insert) {
+        for (uint32_t i = 0; i < n; ++i) {
+            const uint64_t key = (uint64_t)p[2 * i] << 32 | p[2 * i + 1];
+            auto it = first_last.find(key);
+            if (it == first_last.end())
+                first_last.emplace(key, insert ? 3 : 0);
+            else
+                it->second = (uint8_t)((it->second & 1) | (insert ? 2 : 0));
+        }
+    }
+    NetEdits net() const {
+        std::vector<uint64_t> d, ins;
+        for (const auto& [key, fl] : first_last) {
+            if (fl == 0) d.push_back(key);        // deleted, not re-inserted
+            else if (fl == 3) ins.push_back(key); // inserted, not deleted again
+        }
+        std::sort(d.begin(), d.end());
+        std::sort(ins.begin(), ins.end());
+        NetEdits out;
+        for (uint64_t k : d) {
+            out.del.push_back((uint32_t)(k >> 32));
+            out.del.push_back((uint32_t)k);
+        }
+        for (uint64_t k : ins) {
+            out.ins.push_back((uint32_t)(k >> 32));
+            out.ins.push_back((uint32_t)k);
+        }
+        return out;
+    }
+};
+
+bool local_enabled(const dcx_ctx* c, const dcx_graph* g) {
+    return c->cfg.maintenance == 0 && g->kmax_valid && g->rows_valid && g->kmax >= 0 &&
+           g->rows_k == g->kmax + 1 && g->V > 0;
+}
+
+// Maintains kmaxes and rows after the edits N (already applied to the
+// graph) by local tasks (local_kernels.hip).  Returns 1, or 2 when a task's
+// worklist overflowed and everything was recomputed instead.
+int local_maintain(dcx_ctx* c, dcx_graph* g, const NetEdits& N, dcx_update_stats* S) {
+    const auto t0 = Clock::now();
+    const uint32_t V = g->V;
+    const uint32_t nD = (uint32_t)(N.del.size() / 2), nI = (uint32_t)(N.ins.size() / 2);
+    const int32_t K = g->kmax;
+    hipStream_t st = c->stream;
+    const Staged L = layout_batch(N.ins.data(), nI, N.del.data(), nD);
+    ensure_staging(c, L.words);
+    fill_batch(L, c->upd_host, N.ins.data(), N.del.data());
+    uint32_t* Dv = c->upd_dev;
+    if (L.in_words)
+        HIPX(c, hipMemcpyAsync(Dv, c->upd_host, 4 * L.in_words, hipMemcpyHostToDevice, st));
+    auto dptr = [&](size_t idx) { return Dv + L.at[idx]; };
+    DevGroups dg[4];
+    for (int q = 0; q < 4; ++q)
+        dg[q] = DevGroups{dptr(L.i_grp[q][0]), dptr(L.i_grp[q][1]), dptr(L.i_grp[q][2]),
+                          dptr(L.i_grp[q][3]), (uint32_t)L.G[q].key.size()};
+    // worklists: one slot per resident workgroup
+    const uint32_t qcap = (uint32_t)std::min<uint64_t>(1u << 18, 2ull * V + 64);
+    const uint32_t slots = (uint32_t)std::max(1, 2 * c->num_cus);
+    ensure_bytes(c, reinterpret_cast<void**>(&c->local_scratch), &c->local_scratch_bytes,
+                 (size_t)slots * local_slot_words(qcap) * 4);
+    if (!c->local_words) dev_alloc(c, &c->local_words, 16);
+    if (!g->ins_cnt_in) {
+        dev_alloc(c, &g->ins_cnt_in, V);
+        dev_alloc(c, &g->ins_cnt_out, V);
+        HIPX(c, hipMemsetAsync(g->ins_cnt_in, 0, 4ull * V, st));
+        HIPX(c, hipMemsetAsync(g->ins_cnt_out, 0, 4ull * V, st));
+    }
+    if (!g->changed) dev_alloc(c, &g->changed, 2ull * V);
+    if (c->local_dm_cap < std::max<size_t>(nD, 1)) {
+        dev_free(c->local_dm);
+        c->local_dm_cap = 0;
+        dev_alloc(c, &c->local_dm, std::max<size_t>(nD, 1024));
+        c->local_dm_cap = std::max<size_t>(nD, 1024);
+    }
+    uint32_t* W = c->local_words;
+    HIPX(c, hipMemsetAsync(W, 0, 16 * 4, st));
+    HIPX(c, launch_edge_min_kmax(g->kmaxes, dptr(L.i_del), nD, c->local_dm, st));
+    // the insertions come out again: deletions are maintained on G - D
+    if (nI) {
+        HIPX(c, launch_apply_deletes(g->out_off, g->out_deg, g->out_nbr, dg[2].key, dg[2].start,
+                                     dg[2].cnt, dg[2].val, dg[2].n, W + 5, st));
+        HIPX(c, launch_apply_deletes(g->in_off, g->in_deg, g->in_nbr, dg[3].key, dg[3].start,
+                                     dg[3].cnt, dg[3].val, dg[3].n, W + 5, st));
+    }
+    LocalArgs A{};
+    A.V = V;
+    A.out = Csr{g->out_off, g->out_deg, g->out_nbr};
+    A.in = Csr{g->in_off, g->in_deg, g->in_nbr};
+    A.kmaxes = g->kmaxes;
+    A.rows = g->rows;
+    A.task_ctr = W;
+    A.n_changed = W + 1;
+    A.fail = W + 2;
+    A.scratch = c->local_scratch;
+    A.qcap = qcap;
+    A.del_pairs = dptr(L.i_del);
+    A.n_del = nD;
+    A.del_m = c->local_dm;
+    A.del_targets = dg[1].key;
+    A.n_del_targets = dg[1].n;
+    A.changed = g->changed;
+    A.changed_cap = V;
+    A.ins_pairs = dptr(L.i_ins);
+    A.n_ins = nI;
+    A.gin_key = dg[3].key;
+    A.gin_cnt = dg[3].cnt;
+    A.n_gin = dg[3].n;
+    A.gout_key = dg[2].key;
+    A.gout_cnt = dg[2].cnt;
+    A.n_gout = dg[2].n;
+    A.ins_cnt_in = g->ins_cnt_in;
+    A.ins_cnt_out = g->ins_cnt_out;
+    // task lists go through a pinned buffer, each launch's at its own offset
+    // (no copy is overwritten before it ran)
+    std::vector<LocalTaskDesc> tasks;
+    size_t toff = 0;
+    auto run = [&]() {
+        if (tasks.empty()) return;
+        const size_t need = toff + tasks.size();
+        if (c->local_tasks_cap < need) {
+            sync(c);
+            dev_free(c->local_tasks);
+            if (c->local_tasks_host) (void)hipHostFree(c->local_tasks_host);
+            c->local_tasks_host = nullptr;
+            c->local_tasks_cap = 0;
+            const size_t cap = std::max<size_t>(2 * need, 4096);
+            dev_alloc(c, &c->local_tasks, cap);
+            HIPX(c, hipHostMalloc(reinterpret_cast<void**>(&c->local_tasks_host),
+                                  cap * sizeof(LocalTaskDesc), hipHostMallocDefault));
+            c->local_tasks_cap = cap;
+        }
+        std::memcpy(c->local_tasks_host + toff, tasks.data(), tasks.size() * sizeof(LocalTaskDesc));
+        HIPX(c, hipMemcpyAsync(c->local_tasks + toff, c->local_tasks_host + toff,
+                               tasks.size() * sizeof(LocalTaskDesc), hipMemcpyHostToDevice, st));
+        HIPX(c, hipMemsetAsync(W, 0, 4, st));  // task counter
+        A.tasks = c->local_tasks + toff;
+        A.ntasks = (uint32_t)tasks.size();
+        A.rows = g->rows;
+        HIPX(c, launch_local_tasks(A, std::min<uint32_t>(slots, A.ntasks), st));
+        toff = need;
+        tasks.clear();
+    };
+    if (nD) {
+        tasks.push_back({kLocalKmaxDel, 0});
+        run();
+        for (int32_t k = 0; k <= K; ++k) tasks.push_back({kLocalRowDel, k});
+        run();
+    }
+    if (nI) {
+        HIPX(c, launch_apply_inserts(g->out_off, g->out_deg, g->out_nbr, dg[2].key, dg[2].start,
+                                     dg[2].cnt, dg[2].val, dg[2].n, st));
+        HIPX(c, launch_apply_inserts(g->in_off, g->in_deg, g->in_nbr, dg[3].key, dg[3].start,
+                                     dg[3].cnt, dg[3].val, dg[3].n, st));
+        HIPX(c, launch_set_counts(g->ins_cnt_in, dg[3].key, dg[3].cnt, dg[3].n, true, st));
+        HIPX(c, launch_set_counts(g->ins_cnt_out, dg[2].key, dg[2].cnt, dg[2].n, true, st));
+        tasks.push_back({kLocalCoreIns, 0});
+        tasks.push_back({kLocalCoreIns, 1});
+        run();
+        HIPX(c, launch_set_counts(g->ins_cnt_in, dg[3].key, dg[3].cnt, dg[3].n, false, st));
+        HIPX(c, launch_set_counts(g->ins_cnt_out, dg[2].key, dg[2].cnt, dg[2].n, false, st));
+    }
+    HIPX(c, launch_max_masked(g->kmaxes, V, reinterpret_cast<int32_t*>(W + 3), st));
+    uint32_t* hw = reinterpret_cast<uint32_t*>(c->h_scalars + 40);
+    HIPX(c, hipMemcpyAsync(hw, W, 8 * 4, hipMemcpyDeviceToHost, st));
+    sync(c);
+    S->kmax_ms = ms_since(t0);
+    auto fallback = [&]() {
+        const int var = c->cfg.variant == DCX_VARIANT_AUTO ? DCX_VARIANT_BATCHED : c->cfg.variant;
+        if (hw[5]) {
+            g->broken = true;
+            g->kmax_valid = g->rows_valid = false;
+            set_err(c, "local maintenance: %u edges missing while taking insertions out", hw[5]);
+            throw Fail{DCX_ERR_INTERNAL};
+        }
+        compute_kmax(c, g, var);
+        compute_rows(c, g, var);
+        S->rows_recomputed = g->kmax + 1;
+        return 2;
+    };
+    if (hw[2] || hw[5]) return fallback();
+    const int32_t K2 = (int32_t)hw[3];
+    if (nI) {
+        if (K2 > K) {
+            const size_t nrows = (size_t)K2 + 1;
+            grow_keep(c, &g->rows, &g->rows_cap, nrows * V, (size_t)(K + 1) * V);
+            grow_keep(c, &g->lmax_rows, &g->lmax_rows_cap, nrows, (size_t)K + 1);
+            HIPX(c, hipMemsetAsync(g->rows + (size_t)(K + 1) * V, 0,
+                                   (size_t)(K2 - K) * V * sizeof(int32_t), st));
+        }
+        for (int32_t k = 1; k <= K2; ++k) tasks.push_back({kLocalRowIns, k});
+        run();
+    }
+    HIPX(c, launch_max_masked(g->rows, V, reinterpret_cast<int32_t*>(W + 4), st));
+    HIPX(c, hipMemcpyAsync(hw, W, 8 * 4, hipMemcpyDeviceToHost, st));
+    sync(c);
+    if (hw[2]) return fallback();
+    g->kmax = K2;
+    g->lmax = (int32_t)hw[4];
+    g->rows_k = K2 + 1;
+    g->order_valid = false;
+    g->core_valid = false;
+    g->row0_ready = false;
+    S->rows_recomputed = 0;
+    S->rows_ms = ms_since(t0) - S->kmax_ms;
+    return 1;
+}
+
 dcx_status do_update(dcx_ctx* c, dcx_graph* g, const uint32_t* ins, uint32_t n_ins,
                      const uint32_t* del, uint32_t n_del, bool maintain_values,
                      dcx_update_stats* st) {
@@ -386,6 +607,14 @@ dcx_status do_update(dcx_ctx* c, dcx_graph* g, const uint32_t* ins, uint32_t n_i
     if (!(maintain_values && g->kmax_valid)) {
         g->kmax_valid = false;
         g->rows_valid = false;
+    } else if (local_enabled(c, g)) {
+        EditLog log;
+        log.add(del, n_del, false);
+        log.add(ins, n_ins, true);
+        const NetEdits N = log.net();
+        S.net_deleted = (int32_t)(N.del.size() / 2);
+        S.net_inserted = (int32_t)(N.ins.size() / 2);
+        S.path = local_maintain(c, g, N, &S);
     } else {
         maintain(c, g, ins, n_ins, m_del, n_ins + n_del > 0, n_ins == 0, &S);
     }

@@ -33,7 +33,8 @@ class Config(ctypes.Structure):
                 ("rows_slots", ctypes.c_int32), ("team_max", ctypes.c_int32),
                 ("shrink_at", ctypes.c_int32), ("trace_phases", ctypes.c_int32),
                 ("scratch_budget", ctypes.c_uint64), ("pull_ratio", ctypes.c_int32),
-                ("lds_members", ctypes.c_int32), ("reserved", ctypes.c_uint64 * 3)]
+                ("lds_members", ctypes.c_int32), ("maintenance", ctypes.c_int32),
+                ("reserved32", ctypes.c_int32), ("reserved", ctypes.c_uint64 * 2)]
 
 
 class Timing(ctypes.Structure):
@@ -98,7 +99,8 @@ class UpdateStats(ctypes.Structure):
                 ("kmax_before", ctypes.c_int32), ("kmax_after", ctypes.c_int32),
                 ("insert_ms", ctypes.c_double), ("kmax_ms", ctypes.c_double),
                 ("rows_ms", ctypes.c_double), ("total_ms", ctypes.c_double),
-                ("reserved", ctypes.c_int32 * 8)]
+                ("path", ctypes.c_int32), ("net_deleted", ctypes.c_int32),
+                ("net_inserted", ctypes.c_int32), ("reserved", ctypes.c_int32 * 5)]
 
 
 _lib = None
@@ -279,7 +281,8 @@ class Context:
     """A dcx_ctx on one device (library-owned stream and scratch)."""
 
     def __init__(self, device=0, variant=VARIANT_AUTO, rows_slots=0, scratch_budget=0,
-                 team_max=0, shrink_at=0, trace_phases=0, pull_ratio=0, lds_members=0):
+                 team_max=0, shrink_at=0, trace_phases=0, pull_ratio=0, lds_members=0,
+                 maintenance=0):
         cfg = Config()
         check(lib().dcx_config_init(ctypes.byref(cfg)), "dcx_config_init")
         cfg.device = device
@@ -291,6 +294,7 @@ class Context:
         cfg.scratch_budget = scratch_budget
         cfg.pull_ratio = pull_ratio
         cfg.lds_members = lds_members
+        cfg.maintenance = maintenance
         h = ctypes.c_void_p()
         check(lib().dcx_ctx_create(ctypes.byref(cfg), ctypes.byref(h)), "dcx_ctx_create")
         self.h = h

@@ -160,7 +160,15 @@ typedef struct dcx_config {
      * one workgroup with their degree counters in LDS: 0 = as many as the
      * device's LDS holds (at most 32767), -1 = none. */
     int32_t lds_members;
-    uint64_t reserved[3];
+    /* Maintenance after an edge batch (dcx_update, the server):
+     * 0 = local (default): only the vertices the batch can change are
+     *     visited -- h-index refinement from the old values after deletions,
+     *     candidate regions with upper bounds after insertions -- with a
+     *     recomputation when a task's worklist overflows;
+     * 1 = recompute kmaxes and the rows k <= R.  Results are identical. */
+    int32_t maintenance;
+    int32_t reserved32;
+    uint64_t reserved[2];
 } dcx_config;
 
 /* Timings of the last decomposition on a context, from HIP events on the
@@ -453,7 +461,11 @@ typedef struct dcx_update_stats {
     int32_t rows_recomputed; /* l-rows k = 0..rows_recomputed-1 were recomputed */
     int32_t kmax_before, kmax_after;
     double insert_ms, kmax_ms, rows_ms, total_ms;
-    int32_t reserved[8];
+    int32_t path;            /* 0: recomputation, 1: local maintenance, 2: local maintenance
+                                that fell back to recomputation (worklist overflow) */
+    int32_t net_deleted, net_inserted; /* the batch's net effect (an edge deleted and
+                                inserted again in one batch changes nothing) */
+    int32_t reserved[5];
 } dcx_update_stats;
 
 /* Applies a batch -- deletions first, then insertions; pairs are (u,v)
