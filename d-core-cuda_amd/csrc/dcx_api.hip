@@ -1,6 +1,8 @@
 // C-ABI entry points of the dcx library: contexts, device graphs and the
 // decomposition drivers.  Every entry point returns a dcx_status; HIP
 // failures are caught at each call and reported, never ignored.
+#include <chrono>
+
 #include "dcx_state.hpp"
 #include "rng.hpp"
 
@@ -1120,6 +1122,8 @@ dcx_status dcx_ctx_create(const dcx_config* cfg, dcx_ctx** out) {
     if (cfg->rows_slots < 0 || cfg->rows_slots > (int32_t)kMaxGridBlocks || cfg->team_max < 0 ||
         cfg->team_max > 256 || cfg->shrink_at < 0)
         return DCX_ERR_INVALID_ARG;
+    if (cfg->maintenance < 0 || cfg->maintenance > 2 || cfg->local_budget < 0)
+        return DCX_ERR_INVALID_ARG;
     std::unique_ptr<dcx_ctx> c(new (std::nothrow) dcx_ctx);
     if (!c) return DCX_ERR_OUT_OF_MEMORY;
     c->cfg = *cfg;
@@ -1543,8 +1547,12 @@ dcx_status dcx_decompose(dcx_ctx* ctx, dcx_graph* g, int32_t variant, int32_t* k
     return guarded(ctx, [&]() -> dcx_status {
         ctx->timing = dcx_timing{};
         g->rows_valid = false;
+        const auto t0 = std::chrono::steady_clock::now();
         compute_kmax(ctx, g, variant, false, true);
         compute_rows(ctx, g, variant);
+        sync(ctx);
+        g->full_ms = std::chrono::duration<double, std::milli>(
+                         std::chrono::steady_clock::now() - t0).count();
         ctx->timing.total_ms = ctx->timing.kmax_ms + ctx->timing.order_ms + ctx->timing.rows_ms;
         if (kmax) *kmax = g->kmax;
         if (lmax) *lmax = g->lmax;

@@ -266,6 +266,12 @@ hipError_t launch_mvalue(const int32_t* kmaxes, const uint32_t* pairs, uint32_t 
 hipError_t launch_fill_i32(int32_t* p, int32_t v, size_t n, hipStream_t st);
 hipError_t launch_max_i32(const int32_t* p, size_t n, int32_t* out, hipStream_t st);
 
+// ---- edge batches by root kmax (batch_form.hip) --------------------------
+size_t edge_batch_tmp_bytes(uint32_t n, uint32_t nbuckets);
+hipError_t launch_edge_batches(const int32_t* kmaxes, uint32_t V, const uint32_t* pairs, uint32_t n,
+                               int mode, uint32_t nbuckets, uint32_t* batch, uint32_t* nbatch,
+                               void* tmp, hipStream_t st);
+
 // ---- localized maintenance (local_kernels.hip) --------------------------
 enum : int32_t { kLocalKmaxDel = 1, kLocalRowDel = 2, kLocalCoreIns = 3, kLocalRowIns = 4 };
 struct LocalTaskDesc {
@@ -305,6 +311,18 @@ struct LocalArgs {
     uint32_t n_gout;
     const int32_t* ins_cnt_in;   // [V] the batch's entries at the end of each in-list
     const int32_t* ins_cnt_out;  // [V] same for out-lists
+    // counters by task type t (1..4): [4 t] vertices evaluated, [4 t + 1]
+    // candidates claimed, [4 t + 2] list entries read by the evaluations,
+    // [4 t + 3] frontier expansions
+    unsigned long long* counters;
+    // work budgets: once the list entries read by evaluations and
+    // expansions (all task types) exceed `budget`, or `ticks` of the
+    // constant wall clock have passed since the first task of the batch
+    // checked it (start kept in *t_start, zero before), every task stops
+    // (failure bit 0x80) and the batch is recomputed instead; 0 = none
+    unsigned long long budget;
+    unsigned long long ticks;
+    unsigned long long* t_start;
 };
 __host__ __device__ inline size_t local_slot_words(uint32_t qcap) { return 5ull * qcap; }
 hipError_t launch_local_tasks(const LocalArgs& a, uint32_t blocks, hipStream_t st);

@@ -244,6 +244,7 @@ struct dcx_server {
                     U.net_deleted = (int32_t)(N.del.size() / 2);
                     U.net_inserted = (int32_t)(N.ins.size() / 2);
                     U.path = local_maintain(c, g, N, &U);
+                    local_result(c, g, U.path);
                 } else if (g->kmax_valid) {
                     maintain(c, g, ins_all.data(), (uint32_t)(ins_all.size() / 2), m_del, changed,
                              !any_ins, &U);

@@ -77,6 +77,7 @@ struct dcx_ctx {
     uint32_t* local_words = nullptr;
     int32_t* local_dm = nullptr;
     size_t local_dm_cap = 0;
+    unsigned long long local_counters[20] = {};  // of the last local maintenance
     // staging for update batches (device + pinned host), grown on demand
     uint32_t* upd_dev = nullptr;
     size_t upd_dev_words = 0;
@@ -177,6 +178,12 @@ struct dcx_graph {
     // the kmax order, V_k and E_k follow the resident kmaxes (false after
     // local maintenance changed kmaxes; recomputed when a pass needs them)
     bool order_valid = false;
+    // host time of the last full decomposition or recomputation of this
+    // graph (ms; 0 = none yet): the time budget of local maintenance
+    double full_ms = 0.0;
+    // budgeted local maintenance: batches still to send to recomputation,
+    // and the current backoff (0 after a batch kept local)
+    uint32_t local_skip = 0, local_backoff = 0;
     // localized maintenance: the batch's entries at the end of each list
     // ([V], zero outside a maintenance), and the vertices whose kmax fell
     int32_t* ins_cnt_in = nullptr;

@@ -3,19 +3,28 @@
 // Mirrors the reference's maintenance drivers (maintenance /
 // maintenanceFast, main.cu:1536-1751): apply a batch to the device graph,
 // then bring kmaxes and the l-rows back to the values a full recomputation
-// would give.  The reference raises kmax by h-index refinement and then
-// recomputes rows k = 0..M+1 (main.cu:1731-1741); here:
-//   * deletions are applied first, then insertions, in place on the device
-//     (capacity is checked before anything changes; lists that would
-//     overflow are moved to the tail region);
-//   * kmaxes: a deletion-only batch re-peels levels 0..M_del only
-//     (M_del = max over deleted (u,v) of min(old kmax u, old kmax v), the
-//     getMValue quantity, main.cu:344-364); a batch with insertions
-//     re-peels fully;
-//   * rows: D_{k,l} can change only if a deleted edge had both endpoints in
-//     the old core or an inserted edge has both in the new one, so rows
-//     k <= R = max(M_del(old kmaxes), M_ins(new kmaxes)) are recomputed
-//     (all of them in one batched launch) and rows above R are kept.
+// would give.  The reference raises kmaxes by h-index refinement and then
+// recomputes rows k = 0..M+1 (main.cu:1731-1741).  Here:
+//   * the batch is validated, then applied in place on the device,
+//     deletions first (capacity is checked before anything changes; lists
+//     that would overflow are moved to the tail region);
+//   * by default (dcx_config.maintenance = 0) maintenance is local
+//     (local_maintain, tasks in local_kernels.hip): the batch's net edits
+//     D and I are maintained as "D on G - D, then I": the insertions are
+//     taken out of the lists again, kmaxes and rows are lowered by h-index
+//     refinement from their old values (seeds: endpoints of D, vertices
+//     leaving a core), the insertions are put back at the ends of the
+//     lists, kmaxes and row 0 are raised in +1 rounds over candidate
+//     regions, and rows k >= 1 over candidate regions bounded by the new
+//     row 0.  A task whose worklist overflows makes the batch fall back to
+//     recomputation;
+//   * recomputation (maintenance = 1, and the fallback): kmaxes re-peeled
+//     (a warm h-index fixpoint from the old kmaxes when the batch only
+//     deletes), then rows k <= R recomputed in one batched launch, where
+//     D_{k,l} can change only if a deleted edge had both endpoints in the
+//     old core or an inserted edge has both in the new one: R =
+//     max(M_del(old kmaxes), M_ins(new kmaxes)), the getMValue quantity
+//     (main.cu:344-364).
 // The bound uses the new kmaxes for insertions because a batch of inserts
 // can raise kmaxes by more than one (e.g. a clique inserted among isolated
 // vertices), which a bound from the old kmaxes would miss.
@@ -404,9 +413,32 @@ struct EditLog {
     }
 };
 
-bool local_enabled(const dcx_ctx* c, const dcx_graph* g) {
-    return c->cfg.maintenance == 0 && g->kmax_valid && g->rows_valid && g->kmax >= 0 &&
-           g->rows_k == g->kmax + 1 && g->V > 0;
+// Whether this batch goes to local maintenance.  In the budgeted mode
+// (maintenance = 0) a batch given up sends the next local_backoff batches
+// straight to recomputation, the backoff doubling (up to 64) with each
+// further batch given up and cleared by one kept: a stream whose batches
+// all exceed the budget then costs about 1 + 1.25/64 recomputations per
+// batch, one whose batches are cheap stays local.
+bool local_enabled(const dcx_ctx* c, dcx_graph* g) {
+    if (c->cfg.maintenance == 1 || !g->kmax_valid || !g->rows_valid || g->kmax < 0 ||
+        g->rows_k != g->kmax + 1 || g->V == 0)
+        return false;
+    if (c->cfg.maintenance == 0 && g->local_skip > 0) {
+        --g->local_skip;
+        return false;
+    }
+    return true;
+}
+
+// Updates the backoff after local maintenance returned `path`.
+void local_result(const dcx_ctx* c, dcx_graph* g, int path) {
+    if (c->cfg.maintenance != 0) return;
+    if (path == 1) {
+        g->local_backoff = 0;
+    } else {
+        g->local_backoff = g->local_backoff ? std::min<uint32_t>(2 * g->local_backoff, 64) : 1;
+        g->local_skip = g->local_backoff;
+    }
 }
 
 // Maintains kmaxes and rows after the edits N (already applied to the
@@ -434,7 +466,7 @@ int local_maintain(dcx_ctx* c, dcx_graph* g, const NetEdits& N, dcx_update_stats
     const uint32_t slots = (uint32_t)std::max(1, 2 * c->num_cus);
     ensure_bytes(c, reinterpret_cast<void**>(&c->local_scratch), &c->local_scratch_bytes,
                  (size_t)slots * local_slot_words(qcap) * 4);
-    if (!c->local_words) dev_alloc(c, &c->local_words, 16);
+    if (!c->local_words) dev_alloc(c, &c->local_words, 16 + 2 * 20);
     if (!g->ins_cnt_in) {
         dev_alloc(c, &g->ins_cnt_in, V);
         dev_alloc(c, &g->ins_cnt_out, V);
@@ -449,7 +481,7 @@ int local_maintain(dcx_ctx* c, dcx_graph* g, const NetEdits& N, dcx_update_stats
         c->local_dm_cap = std::max<size_t>(nD, 1024);
     }
     uint32_t* W = c->local_words;
-    HIPX(c, hipMemsetAsync(W, 0, 16 * 4, st));
+    HIPX(c, hipMemsetAsync(W, 0, (16 + 2 * 20) * 4, st));
     HIPX(c, launch_edge_min_kmax(g->kmaxes, dptr(L.i_del), nD, c->local_dm, st));
     // the insertions come out again: deletions are maintained on G - D
     if (nI) {
@@ -486,6 +518,20 @@ int local_maintain(dcx_ctx* c, dcx_graph* g, const NetEdits& N, dcx_update_stats
     A.n_gout = dg[2].n;
     A.ins_cnt_in = g->ins_cnt_in;
     A.ins_cnt_out = g->ins_cnt_out;
+    A.counters = reinterpret_cast<unsigned long long*>(W + 16);
+    // budget: local_budget list entries per edge when set; else a quarter
+    // of the last full recomputation's time (at least 0.2 ms), so that a
+    // batch given up costs at most 1.25 recomputations; else (no time
+    // known yet) 12 entries per edge, ~1/8 of a recomputation's list reads
+    A.t_start = reinterpret_cast<unsigned long long*>(W + 8);
+    if (c->cfg.maintenance == 2) {
+    } else if (c->cfg.local_budget > 0) {
+        A.budget = (unsigned long long)c->cfg.local_budget * g->E;
+    } else if (g->full_ms > 0) {
+        A.ticks = (unsigned long long)(std::max(0.25 * g->full_ms, 0.2) * c->wall_khz);
+    } else {
+        A.budget = std::max<unsigned long long>(12ull * g->E, 1ull << 20);
+    }
     // task lists go through a pinned buffer, each launch's at its own offset
     // (no copy is overwritten before it ran)
     std::vector<LocalTaskDesc> tasks;
@@ -548,8 +594,11 @@ int local_maintain(dcx_ctx* c, dcx_graph* g, const NetEdits& N, dcx_update_stats
             set_err(c, "local maintenance: %u edges missing while taking insertions out", hw[5]);
             throw Fail{DCX_ERR_INTERNAL};
         }
+        const auto tf = Clock::now();
         compute_kmax(c, g, var);
         compute_rows(c, g, var);
+        sync(c);
+        g->full_ms = ms_since(tf);
         S->rows_recomputed = g->kmax + 1;
         return 2;
     };
@@ -563,6 +612,8 @@ int local_maintain(dcx_ctx* c, dcx_graph* g, const NetEdits& N, dcx_update_stats
             HIPX(c, hipMemsetAsync(g->rows + (size_t)(K + 1) * V, 0,
                                    (size_t)(K2 - K) * V * sizeof(int32_t), st));
         }
+        // rows 1..K2, one task each, bounded by the new row 0; the largest
+        // rows (lowest k) start first
         for (int32_t k = 1; k <= K2; ++k) tasks.push_back({kLocalRowIns, k});
         run();
     }
@@ -570,6 +621,7 @@ int local_maintain(dcx_ctx* c, dcx_graph* g, const NetEdits& N, dcx_update_stats
     HIPX(c, hipMemcpyAsync(hw, W, 8 * 4, hipMemcpyDeviceToHost, st));
     sync(c);
     if (hw[2]) return fallback();
+    HIPX(c, hipMemcpy(c->local_counters, W + 16, sizeof c->local_counters, hipMemcpyDeviceToHost));
     g->kmax = K2;
     g->lmax = (int32_t)hw[4];
     g->rows_k = K2 + 1;
@@ -615,6 +667,7 @@ dcx_status do_update(dcx_ctx* c, dcx_graph* g, const uint32_t* ins, uint32_t n_i
         S.net_deleted = (int32_t)(N.del.size() / 2);
         S.net_inserted = (int32_t)(N.ins.size() / 2);
         S.path = local_maintain(c, g, N, &S);
+        local_result(c, g, S.path);
     } else {
         maintain(c, g, ins, n_ins, m_del, n_ins + n_del > 0, n_ins == 0, &S);
     }
@@ -634,6 +687,60 @@ dcx_status dcx_update(dcx_ctx* ctx, dcx_graph* g, const uint32_t* ins, uint32_t 
     if (s != DCX_OK) return s;
     if ((n_ins && !ins) || (n_del && !del)) return DCX_ERR_INVALID_ARG;
     return guarded(ctx, [&]() { return do_update(ctx, g, ins, n_ins, del, n_del, true, stats); });
+}
+
+dcx_status dcx_last_maintenance_counters(const dcx_ctx* ctx, uint64_t* out, uint32_t n) {
+    if (!ctx || !out || n < 20) return DCX_ERR_INVALID_ARG;
+    for (int i = 0; i < 20; ++i) out[i] = ctx->local_counters[i];
+    return DCX_OK;
+}
+
+dcx_status dcx_kmax_edge_batches_device(dcx_ctx* ctx, dcx_graph* g, const uint32_t* pairs,
+                                        uint32_t n, int32_t mode, uint32_t* batch_of_edge,
+                                        uint32_t* n_batches) {
+    dcx_status s = check_graph(ctx, g);
+    if (s != DCX_OK) return s;
+    if (!n_batches || (n && (!pairs || !batch_of_edge)) || (mode != 0 && mode != 1))
+        return DCX_ERR_INVALID_ARG;
+    return guarded(ctx, [&]() -> dcx_status {
+        if (!g->kmax_valid) {
+            set_err(ctx, "dcx_kmax_edge_batches_device: no resident kmaxes (decompose first)");
+            return DCX_ERR_STATE;
+        }
+        for (uint32_t i = 0; i < 2 * n; ++i)
+            if (pairs[i] >= g->V) {
+                set_err(ctx, "edge %u: vertex id out of range (V=%u)", i / 2, g->V);
+                return DCX_ERR_INVALID_ARG;
+            }
+        const uint32_t nbuckets = (uint32_t)std::max(g->kmax, 0) + 1;
+        std::vector<void*> tmp;
+        struct Free {
+            std::vector<void*>& t;
+            ~Free() {
+                for (void* p : t) (void)hipFree(p);
+            }
+        } free_tmp{tmp};
+        uint32_t *d_pairs = nullptr, *d_batch = nullptr, *d_nb = nullptr;
+        void* work = nullptr;
+        dev_alloc(ctx, &d_pairs, 2ull * std::max<uint32_t>(n, 1));
+        tmp.push_back(d_pairs);
+        dev_alloc(ctx, &d_batch, std::max<uint32_t>(n, 1));
+        tmp.push_back(d_batch);
+        dev_alloc(ctx, &d_nb, 1);
+        tmp.push_back(d_nb);
+        HIPX(ctx, hipMalloc(&work, edge_batch_tmp_bytes(n, nbuckets)));
+        tmp.push_back(work);
+        if (n)
+            HIPX(ctx, hipMemcpyAsync(d_pairs, pairs, 8ull * n, hipMemcpyHostToDevice, ctx->stream));
+        HIPX(ctx, launch_edge_batches(g->kmaxes, g->V, d_pairs, n, mode, nbuckets, d_batch, d_nb,
+                                      work, ctx->stream));
+        if (n)
+            HIPX(ctx, hipMemcpyAsync(batch_of_edge, d_batch, 4ull * n, hipMemcpyDeviceToHost,
+                                     ctx->stream));
+        HIPX(ctx, hipMemcpyAsync(n_batches, d_nb, 4, hipMemcpyDeviceToHost, ctx->stream));
+        sync(ctx);
+        return DCX_OK;
+    });
 }
 
 dcx_status dcx_graph_insert_edges(dcx_ctx* ctx, dcx_graph* g, const uint32_t* pairs,

@@ -6,13 +6,14 @@
 //
 // Every task runs on one workgroup (no grid-wide barrier, no co-residency
 // assumption) and works on one array of values -- the kmaxes, or row k --
-// in place.  A value cell keeps three flag bits above the value while a
+// in place.  A value cell keeps four flag bits above the value while a
 // task runs (all cleared before it ends):
 //   kQueued  the vertex is on the task's next worklist;
+//   kBusy    a wavefront is evaluating the vertex;
 //   kRegion  the vertex may rise (insertion tasks);
 //   kSeen    the vertex was tested as a candidate, or (kmax deletion) its
 //            change was recorded.
-// Values stay below 2^29 (the host falls back to recomputation otherwise).
+// Values stay below 2^28 (the host falls back to recomputation otherwise).
 //
 // Refinement (all tasks): the h-index operator of the pass,
 //     x(v) <- largest h <= x(v) with >= h K-neighbours of value >= h
@@ -43,14 +44,15 @@
 //                  that value (ED, main.cu:404-421) exceeds it; they get the
 //                  upper bound value + 1 (kmaxFindUpperBound, main.cu:489-545)
 //                  and are refined (kmaxRefineHIndex, main.cu:589-663).
-//   row insertion  row k in 1..M_ins, after the kmaxes and row 0 are final:
+//   row insertion  rows k = 1, 2, ... in order, after the kmaxes and row 0
+//                  are final (each row bounded by the new row k - 1):
 //                  a vertex whose value rises from a to a' lies, for every b
 //                  in (a, a'], in a weakly connected set of vertices that all
 //                  rise through b and that holds an endpoint of an inserted
 //                  edge.  So candidates are searched from those endpoints,
 //                  each carrying the interval (lo, hi] of levels its path can
-//                  rise through (lo >= its old value, hi <= new row 0, which
-//                  bounds every row); a candidate starts at its hi.
+//                  rise through (lo >= its old value, hi <= the new row
+//                  k - 1); a candidate starts at its hi.
 #include <hip/hip_runtime.h>
 
 #include "dcx_internal.hpp"
@@ -61,8 +63,9 @@ namespace {
 constexpr int W = 64;
 constexpr int kBlk = 512;
 constexpr int kWaves = kBlk / W;
-constexpr uint32_t kQueued = 0x80000000u, kRegion = 0x40000000u, kSeen = 0x20000000u;
-constexpr uint32_t kValMask = 0x1fffffffu;
+constexpr uint32_t kQueued = 0x80000000u, kRegion = 0x40000000u, kSeen = 0x20000000u,
+                   kBusy = 0x10000000u;
+constexpr uint32_t kValMask = 0x0fffffffu;
 
 __device__ __forceinline__ int lane() { return (int)(threadIdx.x & (W - 1)); }
 __device__ __forceinline__ int wave() { return (int)(threadIdx.x / W); }
@@ -167,6 +170,32 @@ __device__ __forceinline__ void wave_append(bool take, uint32_t x, uint32_t* lis
     }
 }
 
+// True (and the task marked as stopped) once the work counters of every
+// task type together exceed the budget.  Read by thread 0 only; callers
+// broadcast through sh.over after a barrier.
+__device__ __forceinline__ void check_budget(const LocalArgs& A, Shared& sh) {
+    if (threadIdx.x != 0) return;
+    bool over = false;
+    if (A.budget && A.counters) {
+        unsigned long long used = 0;
+        for (int t = 1; t <= 4; ++t)
+            used += __hip_atomic_load(A.counters + 4 * t + 2, __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT) +
+                    __hip_atomic_load(A.counters + 4 * t + 3, __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT);
+        over = used > A.budget;
+    }
+    if (A.ticks && A.t_start) {
+        const unsigned long long now = wall_clock64();
+        const unsigned long long prev = atomicCAS(A.t_start, 0ull, now);
+        over = over || (prev != 0 && now - prev > A.ticks);
+    }
+    if (over) {
+        sh.over = 1;
+        atomicOr(A.fail, 0x80u);
+    }
+}
+
 // Graph views and the task inputs.
 struct TaskCtx {
     const LocalArgs* A;
@@ -176,6 +205,7 @@ struct TaskCtx {
     const int32_t* ins_cnt;  // core insertion rounds: inserted entries per K list
     int32_t round;     // active prefix: deg - cnt + min(cnt, round + 1); -1: whole lists
     bool record;       // kmax deletion: record (vertex, old value) at its first change
+    int32_t type;      // kLocal* (for the counters)
 };
 
 __device__ __forceinline__ uint32_t klen_of(const TaskCtx& T, uint32_t v, uint32_t deg) {
@@ -206,17 +236,17 @@ __device__ __forceinline__ bool member(const TaskCtx& T, uint32_t v) {
     return T.k <= 0 || val(T.A->kmaxes, v) >= T.k;
 }
 
-// Queues the neighbours of v (just lowered to nv) that may have counted it:
-// members with a value above nv, and -- when only candidates may move -- in
-// the region.  The test does not also require value <= v's old value
-// (which alone decides whether v counted): a neighbour being evaluated at
-// the same time still shows its previous, higher value while it computes a
-// lower one from v's old value, and such a neighbour must be queued again.
-// Values only fall, so one whose value is at most nv now stays unaffected.
-// Lists: for rows, in-lists (their K lists hold v) and out-lists (their C
-// lists hold v, k > 0); for kmaxes out-lists; for row 0 (k = 0) in-lists.
-__device__ void push_neighbours(const TaskCtx& T, uint32_t v, int32_t nv, bool region_only,
-                                Slot& S, Shared& sh) {
+// Queues the neighbours of v (just lowered from old to nv) that may have
+// counted it: members (in the region, when only candidates may move) with
+// nv < value <= old -- a neighbour of a higher value did not count v at its
+// value -- and also, whatever their value above nv, those being evaluated
+// right now (kBusy): such a neighbour may still show its previous, higher
+// value while it computes a lower one from v's old value.  Values only
+// fall, so a neighbour at most nv stays unaffected.  Lists: for rows,
+// in-lists (their K lists hold v) and out-lists (their C lists hold v,
+// k > 0); for kmaxes out-lists; for row 0 (k = 0) in-lists.
+__device__ void push_neighbours(const TaskCtx& T, uint32_t v, int32_t nv, int32_t old,
+                                bool region_only, Slot& S, Shared& sh) {
     const LocalArgs& A = *T.A;
     for (int side = 0; side < 2; ++side) {
         const Csr* R = nullptr;
@@ -238,8 +268,8 @@ __device__ void push_neighbours(const TaskCtx& T, uint32_t v, int32_t nv, bool r
                 y = lst[i];
                 const uint32_t c = cell(T.x, y);
                 const int32_t vy = (int32_t)(c & kValMask);
-                if (vy > nv && !(c & kQueued) && (!region_only || (c & kRegion)) &&
-                    member(T, y)) {
+                if (vy > nv && (vy <= old || (c & kBusy)) && !(c & kQueued) &&
+                    (!region_only || (c & kRegion)) && member(T, y)) {
                     const uint32_t prev = atomicOr(cellp(T.x, y), kQueued);
                     take = !(prev & kQueued);
                 }
@@ -253,6 +283,7 @@ __device__ void push_neighbours(const TaskCtx& T, uint32_t v, int32_t nv, bool r
 // carry kQueued.  Returns false on worklist overflow.
 __device__ bool refine(const TaskCtx& T, bool region_only, Slot& S, Shared& sh) {
     for (;;) {
+        check_budget(*T.A, sh);
         __syncthreads();
         const uint32_t n = sh.ncur;
         if (n == 0 || sh.over) break;
@@ -261,11 +292,18 @@ __device__ bool refine(const TaskCtx& T, bool region_only, Slot& S, Shared& sh) 
         for (uint32_t i = wave(); i < n; i += kWaves) {
             const uint32_t v = S.cur[i];
             uint32_t c = 0;
-            if (lane() == 0) c = atomicAnd(cellp(T.x, v), ~kQueued);
+            if (lane() == 0) {
+                c = atomicAnd(cellp(T.x, v), ~kQueued);
+                atomicOr(cellp(T.x, v), kBusy);
+            }
             c = __shfl(c, 0, W);
             const int32_t old = (int32_t)(c & kValMask);
             const Lists L = lists_of(T, v);
             const int32_t nv = wave_eval(L, T.x, old);
+            if (lane() == 0 && T.A->counters) {
+                atomicAdd(T.A->counters + 4 * T.type, 1ull);
+                atomicAdd(T.A->counters + 4 * T.type + 2, (unsigned long long)(L.klen + L.clen));
+            }
             if (nv < old) {
                 if (lane() == 0) {
                     if (T.record && !(atomicOr(cellp(T.x, v), kSeen) & kSeen)) {
@@ -278,8 +316,11 @@ __device__ bool refine(const TaskCtx& T, bool region_only, Slot& S, Shared& sh) 
                         }
                     }
                     set_value(T.x, v, (uint32_t)nv);
+                    atomicAnd(cellp(T.x, v), ~kBusy);
                 }
-                push_neighbours(T, v, nv, region_only, S, sh);
+                push_neighbours(T, v, nv, old, region_only, S, sh);
+            } else if (lane() == 0) {
+                atomicAnd(cellp(T.x, v), ~kBusy);
             }
         }
         __syncthreads();
@@ -302,7 +343,7 @@ __device__ __forceinline__ bool try_queue(int32_t* x, uint32_t v) {
 // ---- deletions -----------------------------------------------------------
 
 __device__ void task_kmax_del(const LocalArgs& A, Slot& S, Shared& sh) {
-    TaskCtx T{&A, A.kmaxes, -1, true, nullptr, -1, true};
+    TaskCtx T{&A, A.kmaxes, -1, true, nullptr, -1, true, kLocalKmaxDel};
     if (threadIdx.x == 0) sh.ncur = 0;
     __syncthreads();
     for (uint32_t i = threadIdx.x; i < A.n_del_targets; i += kBlk) {
@@ -326,7 +367,7 @@ __device__ void task_kmax_del(const LocalArgs& A, Slot& S, Shared& sh) {
 
 __device__ void task_row_del(const LocalArgs& A, int32_t k, Slot& S, Shared& sh) {
     int32_t* x = A.rows + (size_t)k * A.V;
-    TaskCtx T{&A, x, k, false, nullptr, -1, false};
+    TaskCtx T{&A, x, k, false, nullptr, -1, false, kLocalRowDel};
     if (threadIdx.x == 0) sh.ncur = sh.nnxt = 0;
     __syncthreads();
     // vertices that left the (k,0)-core: value 0, their neighbours queued
@@ -339,7 +380,7 @@ __device__ void task_row_del(const LocalArgs& A, int32_t k, Slot& S, Shared& sh)
         if (lane() == 0) c = atomicAnd(cellp(x, v), ~kValMask);  // value 0
         c = __shfl(c, 0, W);
         const int32_t old = (int32_t)(c & kValMask);
-        if (old > 0) push_neighbours(T, v, 0, false, S, sh);  // into nxt
+        if (old > 0) push_neighbours(T, v, 0, old, false, S, sh);  // into nxt
     }
     __syncthreads();
     // pushes went to nxt: move them to cur
@@ -384,7 +425,8 @@ __device__ void task_core_ins(const LocalArgs& A, int side, Slot& S, Shared& sh)
     const uint32_t ng = kside ? A.n_gin : A.n_gout;
     const Csr& K = kside ? A.in : A.out;
     const Csr& R = kside ? A.out : A.in;
-    TaskCtx T{&A, x, kside ? -1 : 0, kside, kside ? A.ins_cnt_in : A.ins_cnt_out, 0, false};
+    TaskCtx T{&A, x, kside ? -1 : 0, kside, kside ? A.ins_cnt_in : A.ins_cnt_out, 0, false,
+              kLocalCoreIns};
     if (threadIdx.x == 0) sh.tmp = 0;
     __syncthreads();
     for (uint32_t g = threadIdx.x; g < ng; g += kBlk) atomicMax(&sh.tmp, gcnt[g]);
@@ -416,6 +458,8 @@ __device__ void task_core_ins(const LocalArgs& A, int side, Slot& S, Shared& sh)
         // candidates (ED: K-neighbours of at least the value, active
         // entries only, must exceed it), then expand the accepted ones
         for (;;) {
+            check_budget(A, sh);
+            __syncthreads();
             const uint32_t nc = min(sh.ncand, S.qcap);
             if (nc == 0 || sh.over) break;
             for (uint32_t i = wave(); i < nc; i += kWaves) {
@@ -432,6 +476,11 @@ __device__ void task_core_ins(const LocalArgs& A, int side, Slot& S, Shared& sh)
                         atomicOr(&sh.over, 1u);
                     }
                     if (ok) {
+                        if (A.counters) {
+                            atomicAdd(A.counters + 4 * kLocalCoreIns + 1, 1ull);
+                            atomicAdd(A.counters + 4 * kLocalCoreIns + 3,
+                                      (unsigned long long)R.deg[y]);
+                        }
                         atomicOr(cellp(x, y), kRegion);
                         const uint32_t f = atomicAdd(&sh.nnxt, 1u);
                         if (f < S.qcap)
@@ -496,7 +545,7 @@ __device__ void task_core_ins(const LocalArgs& A, int side, Slot& S, Shared& sh)
 // its frontier lo.
 __device__ __forceinline__ bool offer(int32_t* x, const int32_t* U, const int32_t* km, int32_t k,
                                       uint32_t y, int32_t lo, int32_t h, Slot& S, Shared& sh,
-                                      int32_t* lo_out) {
+                                      int32_t* lo_out, unsigned long long* T_counters) {
     if (val(km, y) < k) return false;
     const int32_t h2 = min(h, val(U, y));
     uint32_t c = cell(x, y);
@@ -513,6 +562,7 @@ __device__ __forceinline__ bool offer(int32_t* x, const int32_t* U, const int32_
         const uint32_t got = atomicCAS(cellp(x, y), c, kRegion | (uint32_t)h2);
         if (got == c) {
             const uint32_t at = atomicAdd(&sh.nreg, 1u);
+            if (T_counters) atomicAdd(T_counters + 4 * kLocalRowIns + 1, 1ull);
             if (at < S.qcap) {
                 S.reg[2 * at] = y;
                 S.reg[2 * at + 1] = (uint32_t)a;
@@ -527,10 +577,13 @@ __device__ __forceinline__ bool offer(int32_t* x, const int32_t* U, const int32_
 }
 
 // Frontier entries are (vertex, lo) pairs: cur/nxt hold qcap / 2 of them.
+// U bounds every new value of the row: the new row 0 (D_{k,l} is a subset
+// of D_{0,l}, so l_k <= l_0), which is final before any row task starts, so
+// that the rows k >= 1 run as independent tasks.
 __device__ void task_row_ins(const LocalArgs& A, int32_t k, Slot& S, Shared& sh) {
     int32_t* x = A.rows + (size_t)k * A.V;
-    const int32_t* U = A.rows;  // row 0, final
-    TaskCtx T{&A, x, k, false, nullptr, -1, false};
+    const int32_t* U = A.rows;
+    TaskCtx T{&A, x, k, false, nullptr, -1, false, kLocalRowIns};
     const uint32_t fcap = S.qcap / 2;
     if (threadIdx.x == 0) sh.ncur = sh.nnxt = sh.nreg = 0;
     __syncthreads();
@@ -540,7 +593,7 @@ __device__ void task_row_ins(const LocalArgs& A, int32_t k, Slot& S, Shared& sh)
         if (val(A.kmaxes, u) < k || val(A.kmaxes, v) < k) continue;
         const uint32_t y = (i & 1) ? v : u;
         int32_t lo = -1;
-        if (offer(x, U, A.kmaxes, k, y, -1, (int32_t)kValMask, S, sh, &lo)) {
+        if (offer(x, U, A.kmaxes, k, y, -1, (int32_t)kValMask, S, sh, &lo, A.counters)) {
             const uint32_t at = atomicAdd(&sh.ncur, 1u);
             if (at < fcap) {
                 S.cur[2 * at] = y;
@@ -551,6 +604,7 @@ __device__ void task_row_ins(const LocalArgs& A, int32_t k, Slot& S, Shared& sh)
         }
     }
     for (;;) {
+        check_budget(A, sh);
         __syncthreads();
         const uint32_t nf = min(sh.ncur, fcap);
         if (nf == 0 || sh.over) break;
@@ -560,6 +614,9 @@ __device__ void task_row_ins(const LocalArgs& A, int32_t k, Slot& S, Shared& sh)
             const uint32_t z = S.cur[2 * i];
             const int32_t lo = (int32_t)S.cur[2 * i + 1];
             const int32_t hz = val(x, z);
+            if (lane() == 0 && A.counters)
+                atomicAdd(A.counters + 4 * kLocalRowIns + 3,
+                          (unsigned long long)(A.out.deg[z] + A.in.deg[z]));
             for (int side = 0; side < 2; ++side) {
                 const Csr& R = side == 0 ? A.out : A.in;
                 const uint32_t* lst = R.nbr + R.off[z];
@@ -571,7 +628,7 @@ __device__ void task_row_ins(const LocalArgs& A, int32_t k, Slot& S, Shared& sh)
                     int32_t ylo = -1;
                     if (j < n) {
                         y = lst[j];
-                        take = offer(x, U, A.kmaxes, k, y, lo, hz, S, sh, &ylo);
+                        take = offer(x, U, A.kmaxes, k, y, lo, hz, S, sh, &ylo, A.counters);
                     }
                     const unsigned long long m = __ballot(take);
                     if (m) {

@@ -34,7 +34,7 @@ class Config(ctypes.Structure):
                 ("shrink_at", ctypes.c_int32), ("trace_phases", ctypes.c_int32),
                 ("scratch_budget", ctypes.c_uint64), ("pull_ratio", ctypes.c_int32),
                 ("lds_members", ctypes.c_int32), ("maintenance", ctypes.c_int32),
-                ("reserved32", ctypes.c_int32), ("reserved", ctypes.c_uint64 * 2)]
+                ("local_budget", ctypes.c_int32), ("reserved", ctypes.c_uint64 * 2)]
 
 
 class Timing(ctypes.Structure):
@@ -282,7 +282,7 @@ class Context:
 
     def __init__(self, device=0, variant=VARIANT_AUTO, rows_slots=0, scratch_budget=0,
                  team_max=0, shrink_at=0, trace_phases=0, pull_ratio=0, lds_members=0,
-                 maintenance=0):
+                 maintenance=0, local_budget=0):
         cfg = Config()
         check(lib().dcx_config_init(ctypes.byref(cfg)), "dcx_config_init")
         cfg.device = device
@@ -295,6 +295,7 @@ class Context:
         cfg.pull_ratio = pull_ratio
         cfg.lds_members = lds_members
         cfg.maintenance = maintenance
+        cfg.local_budget = local_budget
         h = ctypes.c_void_p()
         check(lib().dcx_ctx_create(ctypes.byref(cfg), ctypes.byref(h)), "dcx_ctx_create")
         self.h = h
@@ -611,6 +612,18 @@ class Graph:
         check(lib().dcx_batch_summary(self.ctx.h, self.h, u32(n), P(vbase), P(km), P(lm)),
               "dcx_batch_summary", self.ctx.h)
         return km, lm
+
+    def kmax_edge_batches(self, pairs, mode=0):
+        """getKmaxEdgeBatches on the device from the resident kmaxes
+        (dcx_kmax_edge_batches_device): (batch of each edge, batch count)."""
+        pairs = _pairs(pairs)
+        n = pairs.size // 2
+        out = np.empty(max(n, 1), np.uint32)
+        nb = ctypes.c_uint32()
+        check(lib().dcx_kmax_edge_batches_device(self.ctx.h, self.h, P(pairs), u32(n), i32(mode),
+                                                 P(out), ctypes.byref(nb)),
+              "dcx_kmax_edge_batches_device", self.ctx.h)
+        return out[:n], nb.value
 
     def update(self, ins=None, dels=None):
         """Maintained batch (dcx_update): deletions first, then insertions.

@@ -161,13 +161,20 @@ typedef struct dcx_config {
      * device's LDS holds (at most 32767), -1 = none. */
     int32_t lds_members;
     /* Maintenance after an edge batch (dcx_update, the server):
-     * 0 = local (default): only the vertices the batch can change are
-     *     visited -- h-index refinement from the old values after deletions,
-     *     candidate regions with upper bounds after insertions -- with a
-     *     recomputation when a task's worklist overflows;
-     * 1 = recompute kmaxes and the rows k <= R.  Results are identical. */
+     * 0 = local with a work budget (default): only the vertices the batch
+     *     can change are visited -- h-index refinement from the old values
+     *     after deletions, candidate regions with upper bounds after
+     *     insertions -- and a batch that exceeds the budget (local_budget)
+     *     or overflows a worklist is given up and recomputed instead;
+     * 1 = recompute kmaxes and the rows k <= R;
+     * 2 = local without the budget (still recomputed on overflow).
+     * Results are identical. */
     int32_t maintenance;
-    int32_t reserved32;
+    /* Budget of local maintenance: > 0 = list entries per edge that its
+     * tasks may read (a deterministic choice of path); 0 = a quarter of the
+     * graph's last full decomposition time (at least 0.2 ms), or 12 entries
+     * per edge while no time is known. */
+    int32_t local_budget;
     uint64_t reserved[2];
 } dcx_config;
 
@@ -461,8 +468,10 @@ typedef struct dcx_update_stats {
     int32_t rows_recomputed; /* l-rows k = 0..rows_recomputed-1 were recomputed */
     int32_t kmax_before, kmax_after;
     double insert_ms, kmax_ms, rows_ms, total_ms;
-    int32_t path;            /* 0: recomputation, 1: local maintenance, 2: local maintenance
-                                that fell back to recomputation (worklist overflow) */
+    int32_t path;            /* 0: recomputation (maintenance = 1, or the budgeted mode
+                                backing off after a batch given up), 1: local
+                                maintenance, 2: local maintenance given up for
+                                recomputation (budget exceeded or worklist overflow) */
     int32_t net_deleted, net_inserted; /* the batch's net effect (an edge deleted and
                                 inserted again in one batch changes nothing) */
     int32_t reserved[5];
@@ -478,6 +487,14 @@ typedef struct dcx_update_stats {
  * (DCX_ERR_INVALID_ARG), a deleted edge absent (DCX_ERR_NOT_FOUND). */
 dcx_status dcx_update(dcx_ctx* ctx, dcx_graph* g, const uint32_t* ins, uint32_t n_ins,
                       const uint32_t* del, uint32_t n_del, dcx_update_stats* stats);
+
+/* Work counters of the last local maintenance on a context (the figures the
+ * reference prints under PRINT_MAINTENANCE_STATS, main.cu:1552-1564, for
+ * this build's tasks): out[4 t + i] for task type t = 1 kmax deletion,
+ * 2 row deletion, 3 kmax/row-0 insertion rounds, 4 row insertion; i = 0
+ * vertices evaluated, 1 candidates taken in, 2 list entries the
+ * evaluations covered, 3 list entries of expanded candidates.  n >= 20. */
+dcx_status dcx_last_maintenance_counters(const dcx_ctx* ctx, uint64_t* out, uint32_t n);
 
 /* Structural edits only (the resident decomposition is dropped):
  * GraphGPU::insertEdges (src/graph_gpu.cu:71-74, graphInsertEdges :49-69) /
@@ -495,6 +512,17 @@ dcx_status dcx_graph_delete_edges(dcx_ctx* ctx, dcx_graph* g, const uint32_t* pa
 dcx_status dcx_kmax_edge_batches(uint32_t V, const int32_t* kmaxes, const uint32_t* pairs,
                                  uint32_t n, int32_t mode, uint32_t* batch_of_edge,
                                  uint32_t* n_batches);
+
+/* The same batches computed on the device from the graph's resident kmaxes
+ * (no kmax download): KEDGE by a stable sort of the edges by root and their
+ * rank among the root's edges; KMAX by one workgroup playing the rounds.
+ * pairs and batch_of_edge are host arrays as in dcx_kmax_edge_batches,
+ * which this equals edge for edge.  Replaces getKmaxEdgeBatches
+ * (main.cu:1441-1525) and its getKmaxFromDeviceMemory round trip
+ * (main.cu:1713). */
+dcx_status dcx_kmax_edge_batches_device(dcx_ctx* ctx, dcx_graph* g, const uint32_t* pairs,
+                                        uint32_t n, int32_t mode, uint32_t* batch_of_edge,
+                                        uint32_t* n_batches);
 
 /* Number of ragged values (sum over v of kmaxes[v]+1). */
 dcx_status dcx_ragged_size(dcx_ctx* ctx, const dcx_graph* g, uint64_t* n);

@@ -4,9 +4,11 @@ recomputation of the current edge set -- the reference's criterion,
 maintained == recomputed (main.cu:1905-1924, 2194-2211).  The batches mix
 deletions and insertions, hit hubs, insert cliques among isolated vertices
 (kmaxes that rise by more than one), delete whole cores, and grow and
-shrink kmax; every batch must take the local path (no fallback) unless
-stated otherwise.  The recompute path (dcx_config.maintenance = 1) gives the
-same states.
+shrink kmax.  These streams run without the work budget
+(dcx_config.maintenance = 2) and every batch must take the local path (no
+fallback) unless stated otherwise.  The recompute path (maintenance = 1) and
+the budgeted default (maintenance = 0, with a budget small enough that some
+batches are given to recomputation part way) give the same states.
 """
 import sys
 
@@ -54,7 +56,7 @@ def mixed(n_ins, n_del, hub_bias=False):
 
 @pytest.fixture(scope="module")
 def ctx():
-    c = dcx.Context(0)
+    c = dcx.Context(0, maintenance=2)
     yield c
     c.close()
 
@@ -128,24 +130,32 @@ def test_local_insert_and_delete_same_edge(ctx):
 
 
 def test_local_equals_recompute_path():
-    """maintenance = 0 (local) and 1 (recompute) give the same state hash
-    after every batch of an R-MAT 13 update stream."""
+    """maintenance = 2 (local, no budget), 1 (recompute) and 0 with a budget
+    of one list entry per edge (local, abandoned for recomputation once a
+    batch's tasks have read E list entries) give the same state hash after
+    every batch of an R-MAT 13 update stream; the budgeted run keeps some
+    batches local and gives some up (and sends some straight to
+    recomputation while it backs off)."""
     V = 1 << 13
     s, d = dcx.gen_rmat(13, 16 << 13, 2)
     io, ip, do, dp = dcx.gen_update_stream(V, s, d, 13, 30, max_batch=256, seed=4)
-    hashes = {}
-    for mode in (0, 1):
-        with dcx.Context(0, maintenance=mode) as c:
+    hashes, paths = {}, {}
+    for mode in (2, 1, 0):
+        with dcx.Context(0, maintenance=mode, local_budget=1 if mode == 0 else 0) as c:
             g = c.upload_edges(V, s, d, slack=2)
             g.decompose()
-            out = []
+            out, ps = [], []
             for b in range(30):
                 say(f"rmat13 stream mode {mode} batch {b}")
                 st = g.update(ins=ip[io[b]:io[b + 1]], dels=dp[do[b]:do[b + 1]])
-                assert st.path == (1 if mode == 0 else 0)
+                ps.append(st.path)
                 out.append(g.state_hash())
-            hashes[mode] = out
+            hashes[mode], paths[mode] = out, ps
             g.free()
+    assert all(p == 1 for p in paths[2]), paths[2]
+    assert all(p == 0 for p in paths[1]), paths[1]
+    assert {1, 2} <= set(paths[0]) <= {0, 1, 2}, paths[0]
+    assert hashes[2] == hashes[1]
     assert hashes[0] == hashes[1]
     es_s, es_d = s, d
     # and the last state equals the oracle's
@@ -155,4 +165,31 @@ def test_local_equals_recompute_path():
         cur.update(map(tuple, ip[io[b]:io[b + 1]].tolist()))
     e = np.array(sorted(cur), np.uint32)
     ref = oracle.decompose(oracle.csr_from_edges(V, e[:, 0], e[:, 1]))
-    assert hashes[0][-1] == graphs.state_hash(ref[1], ref[3])
+    assert hashes[2][-1] == graphs.state_hash(ref[1], ref[3])
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_device_edge_batches_equal_host(ctx, mode):
+    """getKmaxEdgeBatches on the device (from the resident kmaxes) equals the
+    host restatement -- itself pinned by hand-derived batches -- edge for
+    edge: R-MAT 12 with random pairs, a list dominated by one hub root,
+    pairs among the densest (lowest-id) vertices, and an empty list."""
+    s, d = dcx.gen_rmat(12, 16 << 12, 9)
+    V = 1 << 12
+    g = ctx.upload_edges(V, s, d)
+    g.decompose()
+    km = g.kmaxes()
+    rng = np.random.default_rng(mode)
+    cases = {
+        "random": rng.integers(0, V, (3000, 2)).astype(np.uint32),
+        "hub": np.stack([np.full(500, 0), rng.integers(1, V, 500)], 1).astype(np.uint32),
+        "low ids": rng.integers(0, 64, (2000, 2)).astype(np.uint32),
+        "empty": np.zeros((0, 2), np.uint32),
+    }
+    for name, p in cases.items():
+        say(f"device edge batches mode {mode}: {name} ({len(p)} pairs)")
+        want, nb_want = dcx.kmax_edge_batches(V, km, p, mode=mode)
+        got, nb_got = g.kmax_edge_batches(p, mode=mode)
+        assert nb_got == nb_want, name
+        assert np.array_equal(got, want), name
+    g.free()
