@@ -1173,6 +1173,9 @@ dcx_status dcx_ctx_create(const dcx_config* cfg, dcx_ctx** out) {
         return DCX_ERR_HIP;
     for (auto& e : c->sev)
         if (hipEventCreate(&e) != hipSuccess) return DCX_ERR_HIP;
+    // the static scratch of dcx_klist_noalloc: its address on this device
+    if (noalloc_scratch_address(&c->noalloc_scratch) != hipSuccess || !c->noalloc_scratch)
+        return DCX_ERR_HIP;
     if (hipMalloc(&c->ctrs, kCtrWords * sizeof(uint32_t)) != hipSuccess)
         return DCX_ERR_OUT_OF_MEMORY;
     if (hipMemset(c->ctrs, 0, kCtrWords * sizeof(uint32_t)) != hipSuccess) return DCX_ERR_HIP;
@@ -1529,6 +1532,45 @@ dcx_status dcx_klist(dcx_ctx* ctx, dcx_graph* g, int32_t k, int32_t variant, int
         }
         dev_free(row);
         if (lmax_k) *lmax_k = lk;
+        return DCX_OK;
+    });
+}
+
+dcx_status dcx_klist_noalloc(dcx_ctx* ctx, dcx_graph* g, int32_t k, int32_t* lvals,
+                             int32_t* lmax_k) {
+    dcx_status s = check_graph(ctx, g);
+    if (s != DCX_OK) return s;
+    if (k < 0 || !lvals || !lmax_k) return DCX_ERR_INVALID_ARG;
+    return guarded(ctx, [&]() -> dcx_status {
+        const uint32_t V = g->V;
+        if (V > kNoAllocMaxVertices) {
+            set_err(ctx, "dcx_klist_noalloc: V = %u exceeds its %u vertices of static scratch", V,
+                    kNoAllocMaxVertices);
+            return DCX_ERR_CAPACITY;
+        }
+        if (ctx->cfg.trace_phases) {
+            set_err(ctx, "dcx_klist_noalloc: the phase trace allocates; turn trace_phases off");
+            return DCX_ERR_STATE;
+        }
+        if (V == 0) {
+            *lmax_k = -1;
+            return DCX_OK;
+        }
+        // the whole-graph pass of dcx_klist (KList semantics), with its
+        // scratch in the static array and its output in the caller's array;
+        // the pass initialises every scratch word it reads
+        PassSpec p = full_row_spec(g, k, lvals);
+        p.trace_k = -1;
+        p.pull_ratio = ctx->cfg.pull_ratio < 0    ? 0u
+                       : ctx->cfg.pull_ratio == 0 ? kDefaultPullRatio
+                                                  : (uint32_t)ctx->cfg.pull_ratio;
+        clear_kernel_errors(ctx);
+        HIPX(ctx, launch_pass_grid(p, scratch_at(ctx->noalloc_scratch, V), ctx->ctrs,
+                                   ctx->d_scalars, gsync(ctx, ctx->grid_blocks), ctx->stream));
+        HIPX(ctx, hipMemcpyAsync(ctx->h_scalars, ctx->d_scalars, sizeof(int32_t),
+                                 hipMemcpyDeviceToHost, ctx->stream));
+        check_kernel_errors(ctx, "allocation-free row pass");
+        *lmax_k = ctx->h_scalars[0];
         return DCX_OK;
     });
 }

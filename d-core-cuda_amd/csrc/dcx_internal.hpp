@@ -272,6 +272,13 @@ hipError_t launch_edge_batches(const int32_t* kmaxes, uint32_t V, const uint32_t
                                int mode, uint32_t nbuckets, uint32_t* batch, uint32_t* nbatch,
                                void* tmp, hipStream_t st);
 
+// ---- allocation-free single row (noalloc.hip) ----------------------------
+// Most vertices dcx_klist_noalloc accepts: its pass scratch, kScratchArrays
+// words per vertex, is a static device array of that size (12 MiB).
+constexpr uint32_t kNoAllocMaxVertices = 1u << 18;
+// Address of the static scratch array on the current device.
+hipError_t noalloc_scratch_address(void** ptr);
+
 // ---- localized maintenance (local_kernels.hip) --------------------------
 enum : int32_t { kLocalKmaxDel = 1, kLocalRowDel = 2, kLocalCoreIns = 3, kLocalRowIns = 4 };
 struct LocalTaskDesc {

@@ -48,6 +48,9 @@ struct dcx_ctx {
     std::mutex mu;
     std::string err;
     dcx_timing timing{};
+    // this device's copy of the static scratch array of dcx_klist_noalloc
+    // (from hipGetSymbolAddress at context creation)
+    void* noalloc_scratch = nullptr;
     // grown-on-demand device scratch
     void* scratch = nullptr;
     size_t scratch_bytes = 0;

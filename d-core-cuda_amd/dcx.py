@@ -529,6 +529,17 @@ class Graph:
               "dcx_klist", self.ctx.h)
         return lk.value, l
 
+    def klist_noalloc(self, k, out):
+        """dcx_klist_noalloc: row k on the whole graph into `out`, a
+        DeviceArray of V uint32 (read back as int32 by the caller); returns
+        lmax_k."""
+        if out.size < self.V:
+            raise ValueError("output array smaller than V")
+        lk = ctypes.c_int32()
+        check(lib().dcx_klist_noalloc(self.ctx.h, self.h, i32(k), out.ptr, ctypes.byref(lk)),
+              "dcx_klist_noalloc", self.ctx.h)
+        return lk.value
+
     def decompose(self, variant=VARIANT_AUTO):
         K = ctypes.c_int32()
         Lm = ctypes.c_int32()

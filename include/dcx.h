@@ -311,6 +311,19 @@ dcx_status dcx_kmax(dcx_ctx* ctx, dcx_graph* g, int32_t* kmaxes, int32_t* kmax);
 dcx_status dcx_klist(dcx_ctx* ctx, dcx_graph* g, int32_t k, int32_t variant, int32_t* lvals,
                      int32_t* lmax_k);
 
+/* dcx_klist on the whole graph (KList semantics, main.cu:144-169) without
+ * any device allocation: no counterpart in the reference; for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  lvals is a caller-allocated
+ * DEVICE array of V int32 receiving the row; *lmax_k the last level.  The
+ * pass scratch is a static device array sized for at most 262,144 vertices
+ * (12 MiB per device, held until the process ends); a larger graph gets
+ * DCX_ERR_CAPACITY, and the phase trace (cfg.trace_phases) DCX_ERR_STATE.
+ * All calls on one GPU share that array and the context's counter words:
+ * calls that overlap on one GPU (another thread, another context) race. */
+dcx_status dcx_klist_noalloc(dcx_ctx* ctx, dcx_graph* g, int32_t k, int32_t* lvals,
+                             int32_t* lmax_k);
+
 /* Full decomposition (dcore, main.cu:197-262; checkDCore main.cu:276-293):
  * kmaxes and rows k = 0..kmax stay resident on the device graph. */
 dcx_status dcx_decompose(dcx_ctx* ctx, dcx_graph* g, int32_t variant, int32_t* kmax,
