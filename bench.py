@@ -77,8 +77,12 @@ def main():
                     choices=["headline", "rmat", "chunglu256", "serve", "learned", "ingest"])
     ap.add_argument("--schedule", default="learned", choices=["learned", "workshare"],
                     help="headline: team widths from the cost model or the work-share rule")
+    ap.add_argument("--full-train", action="store_true",
+                    help="headline: also train the cost model on passes logged from smaller "
+                         "graphs at several team caps (default: the graph's own passes only)")
     ap.add_argument("--quick-train", action="store_true",
-                    help="headline: train the cost model on the graph's own passes only")
+                    help="headline: the default (the graph's own passes only); kept for "
+                         "older command lines")
     ap.add_argument("--plain", action="store_true",
                     help="headline: also time the plain one-k-per-launch peel")
     ap.add_argument("--no-rmat20", action="store_true",
@@ -314,7 +318,8 @@ def run_headline(args):
         f"{1e3 * (tg2 - tg1):.0f} ms")
 
     # the fixed work-share schedule, then the cost model trained on the GPU
-    # from logged passes (smaller graphs at several team caps, and this one)
+    # from the passes those decompositions logged (with --full-train also
+    # from smaller graphs at several team caps)
     ctx.set_scheduler(None)
     g.decompose(dcx.VARIANT_BATCHED)
     heur = []
@@ -324,7 +329,7 @@ def run_headline(args):
     h_hash = g.state_hash()
     model = None
     if args.schedule == "learned":
-        Xs, Ts = learned_training_set(dcx, log) if not args.quick_train else ([], [])
+        Xs, Ts = learned_training_set(dcx, log) if args.full_train else ([], [])
         for wl in (True, False):
             X, T = g.pass_features(with_log=wl)
             Xs.append(X)
@@ -386,6 +391,8 @@ def run_headline(args):
                          "rows_kernel": round(mean("rows_kernel_ms"), 3),
                          "rows_total": round(mean("rows_ms"), 3)},
         "schedule": {"kind": args.schedule, "model_used": used_model,
+                     "training": "this graph's logged passes" + (
+                         " and smaller graphs' at several team caps" if args.full_train else ""),
                      "work_share_ms": round(min(heur), 2),
                      "learned_speedup_vs_work_share": round(min(heur) / ms_per_step, 3),
                      "bit_exact_vs_work_share": state_hash == h_hash},

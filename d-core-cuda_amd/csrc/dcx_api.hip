@@ -8,6 +8,7 @@
 
 namespace {
 
+constexpr int32_t kChainRows = 32;  // batched: one team row in kChainRows is peeled
 constexpr int32_t kFewRows = 4;  // batched: at most this many rows above row 0 run whole-graph
 
 // kmax pass: the peel with in and out swapped and k = 0 (swapInOut,
@@ -414,13 +415,16 @@ constexpr int kRowsBlocksPerCu = 2;
 
 // want/wts (optional): team width and weight of each pass chosen by the
 // cost model; without them the width follows the pass's share of the work.
+// take (optional): only the passes k with take[k] set are scheduled.
 Sched make_schedule(const dcx_graph* g, int32_t k_first, int32_t kr, uint32_t G,
                     uint32_t team_max, const std::vector<uint32_t>* want = nullptr,
-                    const std::vector<double>* wts = nullptr) {
+                    const std::vector<double>* wts = nullptr,
+                    const std::vector<char>* take = nullptr) {
     const uint32_t np = (uint32_t)kr + 1, k0 = (uint32_t)std::max(k_first, 0);
     std::vector<double> w(np, 0.0);
     double W = 0;
     for (uint32_t k = k0; k < np; ++k) {
+        if (take && !(*take)[k]) continue;
         w[k] = wts ? (*wts)[k] : 2.0 * (double)g->h_ek[k] + (double)g->h_vk[k] + 64.0;
         W += w[k];
     }
@@ -432,14 +436,16 @@ Sched make_schedule(const dcx_graph* g, int32_t k_first, int32_t kr, uint32_t G,
     double share = W / big_budget;
     bool any_small = false;
     for (uint32_t k = k0; k < np && !any_small; ++k)
-        any_small = want ? std::min((*want)[k], tcap) < 2
-                         : std::min<double>(tcap, w[k] / share) < 2.0;
+        if (!take || (*take)[k])
+            any_small = want ? std::min((*want)[k], tcap) < 2
+                             : std::min<double>(tcap, w[k] / share) < 2.0;
     if (!any_small) {
         big_budget = std::max<uint32_t>(G, 1);
         share = W / big_budget;
     }
     std::vector<uint32_t> idx;
-    for (uint32_t k = k0; k < np; ++k) idx.push_back(k);
+    for (uint32_t k = k0; k < np; ++k)
+        if (!take || (*take)[k]) idx.push_back(k);
     std::stable_sort(idx.begin(), idx.end(), [&](uint32_t a, uint32_t b) { return w[a] > w[b]; });
     // free workgroups of each XCD, in increasing block order
     std::vector<std::vector<uint32_t>> xfree(kXcds);
@@ -715,7 +721,7 @@ void compute_rows(dcx_ctx* c, dcx_graph* g, int variant, int32_t kr = -1) {
     g->rows_cleared = 0;
     int launches = 0, slots_used = 1;
     int32_t split = kr + 1;  // batched: first LDS-resident row
-    bool teams_ran = false, lds_ran = false, first_ran = false;
+    bool teams_ran = false, lds_ran = false, first_ran = false, chains_ran = false;
     std::vector<std::pair<uint32_t, int32_t>> last_teams;  // (size, first k) for diagnosis
     if (few) {
         g->rows_first_grid = kr + 1;  // no row is a team pass
@@ -815,15 +821,31 @@ void compute_rows(dcx_ctx* c, dcx_graph* g, int variant, int32_t kr = -1) {
         const bool have_row0 = g->row0_ready;
         const int32_t k_first = ksplit >= 2 || have_row0 ? 1 : 0;
         g->rows_first_grid = k_first;
+        // Chains: of the team rows, only every spacing-th (counted from row
+        // 0, which is ready before the team launch) is peeled; the rows
+        // between are derived from the row above after the team launch.
+        const int32_t spacing =
+            c->cfg.chain_rows == 0 ? kChainRows : std::max<int32_t>(c->cfg.chain_rows, 1);
+        std::vector<char> peel(kr + 1, 1);
+        std::vector<int32_t> chain_last(kr + 1, -1);
+        uint32_t npeel = 0;
+        if (spacing > 1 && k_first == 1) {
+            for (int32_t a = 0; a < ksplit; a += spacing) {
+                const int32_t b = std::min(a + spacing - 1, ksplit - 1);
+                chain_last[a] = b > a ? b : -1;
+                for (int32_t k = a + 1; k <= b; ++k) peel[k] = 0;
+            }
+        }
+        for (int32_t k = k_first; k < ksplit; ++k) npeel += peel[k] ? 1u : 0u;
         Sched s;
         uint32_t busy = 0;
-        if (ksplit > k_first) {
+        if (npeel) {
             std::vector<uint32_t> want;
             std::vector<double> wts;
             const bool by_model = c->sched_mlp && mlp_team_sizes(c, g, k_first, ksplit - 1, G,
                                                                  tmax, want, wts);
-            s = by_model ? make_schedule(g, k_first, ksplit - 1, G, tmax, &want, &wts)
-                         : make_schedule(g, k_first, ksplit - 1, G, tmax);
+            s = by_model ? make_schedule(g, k_first, ksplit - 1, G, tmax, &want, &wts, &peel)
+                         : make_schedule(g, k_first, ksplit - 1, G, tmax, nullptr, nullptr, &peel);
             c->timing.rows_scheduler = by_model ? 1 : 0;
             busy = (uint32_t)std::count_if(s.block_map.begin(), s.block_map.begin() + G,
                                            [](uint32_t t) { return t != ~0u; });
@@ -848,7 +870,7 @@ void compute_rows(dcx_ctx* c, dcx_graph* g, int variant, int32_t kr = -1) {
             lks.empty() ? 0u : (std::max<uint32_t>(g->h_vk[ksplit], 1) + 31) / 32 * 32;
         const uint32_t lblocks = (uint32_t)std::min<size_t>(lks.size(), (size_t)c->num_cus);
         // device copies: team table, block map, team passes, LDS passes
-        const size_t nt = s.teams.size(), nb = ksplit > k_first ? G : 0, nk = s.ks.size();
+        const size_t nt = s.teams.size(), nb = npeel ? G : 0, nk = s.ks.size();
         const size_t bytes = nt * sizeof(TeamDesc) + 8 * nb + 4 * nk + 4 * lks.size() + 64;
         ensure_bytes(c, &c->sched, &c->sched_bytes, bytes);
         char* base = static_cast<char*>(c->sched);
@@ -924,6 +946,65 @@ void compute_rows(dcx_ctx* c, dcx_graph* g, int variant, int32_t kr = -1) {
             team_err_stride = s.sync_stride;
             HIPX(c, hipEventRecord(c->sev[2], c->stream));
         }
+        if (std::any_of(chain_last.begin(), chain_last.end(), [](int32_t b) { return b >= 0; })) {
+            // one team per chain (short chains share a workgroup), sized by
+            // the work of its first derived row; launched behind the peeled
+            // rows on the same stream, beside the LDS-resident rows
+            const uint32_t Gc = (uint32_t)std::min<int64_t>(
+                G, (int64_t)c->num_cus * std::max(1, rows_chains_blocks_per_cu()));
+            std::vector<char> take(kr + 1, 0);
+            std::vector<double> cw(kr + 1, 0.0);
+            for (int32_t a = 0; a <= kr; ++a)
+                if (chain_last[a] >= 0) {
+                    take[a] = 1;
+                    cw[a] = 2.0 * (double)g->h_ek[a + 1] + (double)g->h_vk[a + 1] + 64.0;
+                }
+            const Sched cs = make_schedule(g, 0, ksplit - 1, Gc, tmax, nullptr, &cw, &take);
+            std::vector<int32_t> cks;
+            for (int32_t a : cs.ks) {
+                cks.push_back(a);
+                cks.push_back(chain_last[a]);
+            }
+            const size_t cnt = cs.teams.size();
+            const size_t cbytes = cnt * sizeof(TeamDesc) + 8ull * Gc + 4 * cks.size() + 64;
+            ensure_bytes(c, &c->chain_sched, &c->chain_sched_bytes, cbytes);
+            std::vector<char> hb(cbytes);
+            std::memcpy(hb.data(), cs.teams.data(), cnt * sizeof(TeamDesc));
+            std::memcpy(hb.data() + cnt * sizeof(TeamDesc), cs.block_map.data(), 8ull * Gc);
+            std::memcpy(hb.data() + cnt * sizeof(TeamDesc) + 8ull * Gc, cks.data(),
+                        4 * cks.size());
+            HIPX(c, hipMemcpyAsync(c->chain_sched, hb.data(), cbytes, hipMemcpyHostToDevice,
+                                   c->stream));
+            char* cb = static_cast<char*>(c->chain_sched);
+            const size_t cpool = cnt * ((size_t)cs.sync_stride + kSlotWords);
+            if (c->chain_pools_words < cpool) {
+                dev_free(c->chain_pools);
+                c->chain_pools_words = 0;
+                dev_alloc(c, &c->chain_pools, cpool);
+                c->chain_pools_words = cpool;
+            }
+            HIPX(c, hipMemsetAsync(c->chain_pools, 0, 4 * cpool, c->stream));
+            ensure_bytes(c, &c->chain_scratch, &c->chain_scratch_bytes,
+                         4 * std::max<uint64_t>(cs.scratch_words, 1));
+            HIPX(c, hipEventRecord(c->cev[0], c->stream));
+            HIPX(c, launch_rows_chains(
+                        V, basep.K, basep.C, g->order, g->vk, g->rows, g->lmax_rows, g->pstats,
+                        reinterpret_cast<const TeamDesc*>(cb),
+                        reinterpret_cast<const uint32_t*>(cb + cnt * sizeof(TeamDesc)),
+                        reinterpret_cast<const int32_t*>(cb + cnt * sizeof(TeamDesc) + 8ull * Gc),
+                        Gc, c->chain_scratch, c->chain_pools,
+                        c->chain_pools + cnt * cs.sync_stride, c->timeout_ticks,
+                        c->ctrs + kCtrErr, c->stream));
+            HIPX(c, hipEventRecord(c->cev[1], c->stream));
+            ++launches;
+            chains_ran = true;
+            for (const TeamDesc& td : cs.teams)
+                for (uint32_t i = td.pass_begin; i < td.pass_end; ++i)
+                    for (int32_t k = cs.ks[i] + 1; k <= chain_last[cs.ks[i]]; ++k) {
+                        g->h_pass_variant[k + 1] = DCX_VARIANT_HINDEX;
+                        g->h_pass_team[k + 1] = (uint16_t)td.size;
+                    }
+        }
         if (!lks.empty()) {
             HIPX(c, hipEventRecord(c->sev[0], c->side));
             HIPX(c, launch_rows_lds(basep, g->vk, g->rows, g->lmax_rows, d_lks,
@@ -972,6 +1053,12 @@ void compute_rows(dcx_ctx* c, dcx_graph* g, int variant, int32_t kr = -1) {
     c->timing.rows_teams_ms = 0.0f;
     c->timing.rows_lds_ms = 0.0f;
     c->timing.rows_first_ms = 0.0f;
+    c->timing.rows_chains_ms = 0.0f;
+    if (chains_ran) {
+        float ms = 0;
+        HIPX(c, hipEventElapsedTime(&ms, c->cev[0], c->cev[1]));
+        c->timing.rows_chains_ms = ms;
+    }
     if (teams_ran) {
         float ms = 0;
         HIPX(c, hipEventElapsedTime(&ms, c->sev[3], c->sev[2]));
@@ -1122,7 +1209,8 @@ dcx_status dcx_ctx_create(const dcx_config* cfg, dcx_ctx** out) {
     if (cfg->rows_slots < 0 || cfg->rows_slots > (int32_t)kMaxGridBlocks || cfg->team_max < 0 ||
         cfg->team_max > 256 || cfg->shrink_at < 0)
         return DCX_ERR_INVALID_ARG;
-    if (cfg->maintenance < 0 || cfg->maintenance > 2 || cfg->local_budget < 0)
+    if (cfg->maintenance < 0 || cfg->maintenance > 2 || cfg->local_budget < 0 ||
+        cfg->chain_rows < -1)
         return DCX_ERR_INVALID_ARG;
     std::unique_ptr<dcx_ctx> c(new (std::nothrow) dcx_ctx);
     if (!c) return DCX_ERR_OUT_OF_MEMORY;
@@ -1172,6 +1260,8 @@ dcx_status dcx_ctx_create(const dcx_config* cfg, dcx_ctx** out) {
         hipEventCreateWithFlags(&c->join, hipEventDisableTiming) != hipSuccess)
         return DCX_ERR_HIP;
     for (auto& e : c->sev)
+        if (hipEventCreate(&e) != hipSuccess) return DCX_ERR_HIP;
+    for (auto& e : c->cev)
         if (hipEventCreate(&e) != hipSuccess) return DCX_ERR_HIP;
     // the static scratch of dcx_klist_noalloc: its address on this device
     if (noalloc_scratch_address(&c->noalloc_scratch) != hipSuccess || !c->noalloc_scratch)
@@ -1223,6 +1313,11 @@ dcx_status dcx_ctx_destroy(dcx_ctx* c) {
     if (c->join) (void)hipEventDestroy(c->join);
     for (auto& e : c->sev)
         if (e) (void)hipEventDestroy(e);
+    for (auto& e : c->cev)
+        if (e) (void)hipEventDestroy(e);
+    dev_free(c->chain_sched);
+    dev_free(c->chain_pools);
+    dev_free(c->chain_scratch);
     delete c;
     return DCX_OK;
 }

@@ -279,6 +279,19 @@ constexpr uint32_t kNoAllocMaxVertices = 1u << 18;
 // Address of the static scratch array on the current device.
 hipError_t noalloc_scratch_address(void** ptr);
 
+// ---- rows derived from the row above (chain_kernels.hip) -----------------
+// Chains of the batched rows: team t derives, for each pair (a, b) of
+// chain_ks[2*pass_begin .. 2*pass_end), rows a+1..b from row a (already in
+// rows).  Scratch per team: 12 arrays of scratch_cap words.  Pool words as
+// for launch_rows_teams (sync words zeroed, slots anything).
+hipError_t launch_rows_chains(uint32_t V, const Csr& K, const Csr& C, const uint32_t* order,
+                              const uint32_t* vk, int32_t* rows, int32_t* lmax_rows,
+                              uint32_t* stats, const TeamDesc* teams, const uint32_t* block_team,
+                              const int32_t* chain_ks, uint32_t blocks, void* scratch,
+                              uint32_t* sync_pool, uint32_t* slot_pool, uint64_t timeout_ticks,
+                              uint32_t* err, hipStream_t st);
+int rows_chains_blocks_per_cu();
+
 // ---- localized maintenance (local_kernels.hip) --------------------------
 enum : int32_t { kLocalKmaxDel = 1, kLocalRowDel = 2, kLocalCoreIns = 3, kLocalRowIns = 4 };
 struct LocalTaskDesc {

@@ -45,6 +45,7 @@ struct dcx_ctx {
     // start and end of the team launch on `stream` (ev[5] opens the rows
     // kernels, row 0's grid pass first when the other rows use its bound)
     hipEvent_t sev[4] = {};
+    hipEvent_t cev[2] = {};  // around the chains launch
     std::mutex mu;
     std::string err;
     dcx_timing timing{};
@@ -68,6 +69,13 @@ struct dcx_ctx {
     size_t sched_bytes = 0;
     uint32_t* pools = nullptr;  // team sync words + team slots
     size_t pools_words = 0;
+    // chains launch: schedule, sync words + slots, scratch
+    void* chain_sched = nullptr;
+    size_t chain_sched_bytes = 0;
+    uint32_t* chain_pools = nullptr;
+    size_t chain_pools_words = 0;
+    void* chain_scratch = nullptr;
+    size_t chain_scratch_bytes = 0;
     // localized maintenance: per-slot worklists, task list, small words
     // ([0] task counter, [1] recorded kmax changes, [2] failure bits,
     // [3] largest kmax, [4] largest row-0 value, [5] edits not found), and

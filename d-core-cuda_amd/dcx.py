@@ -5,6 +5,7 @@ compute of its own: every result comes from the HIP library.  Loading fails
 loudly when libdcx.so is missing -- there is no CPU fallback.
 """
 import ctypes
+import weakref
 import os
 
 import numpy as np
@@ -34,7 +35,8 @@ class Config(ctypes.Structure):
                 ("shrink_at", ctypes.c_int32), ("trace_phases", ctypes.c_int32),
                 ("scratch_budget", ctypes.c_uint64), ("pull_ratio", ctypes.c_int32),
                 ("lds_members", ctypes.c_int32), ("maintenance", ctypes.c_int32),
-                ("local_budget", ctypes.c_int32), ("reserved", ctypes.c_uint64 * 2)]
+                ("local_budget", ctypes.c_int32), ("chain_rows", ctypes.c_int32),
+                ("reserved32", ctypes.c_int32), ("reserved", ctypes.c_uint64 * 1)]
 
 
 class Timing(ctypes.Structure):
@@ -44,7 +46,7 @@ class Timing(ctypes.Structure):
                 ("rows_slots", ctypes.c_int32), ("rows_scheduler", ctypes.c_int32),
                 ("rows_split", ctypes.c_int32), ("rows_teams_ms", ctypes.c_float),
                 ("rows_lds_ms", ctypes.c_float), ("rows_first_ms", ctypes.c_float),
-                ("reserved", ctypes.c_int32 * 1)]
+                ("rows_chains_ms", ctypes.c_float)]
 
 
 class PassLog(ctypes.Structure):
@@ -282,7 +284,7 @@ class Context:
 
     def __init__(self, device=0, variant=VARIANT_AUTO, rows_slots=0, scratch_budget=0,
                  team_max=0, shrink_at=0, trace_phases=0, pull_ratio=0, lds_members=0,
-                 maintenance=0, local_budget=0):
+                 maintenance=0, local_budget=0, chain_rows=0):
         cfg = Config()
         check(lib().dcx_config_init(ctypes.byref(cfg)), "dcx_config_init")
         cfg.device = device
@@ -296,12 +298,21 @@ class Context:
         cfg.lds_members = lds_members
         cfg.maintenance = maintenance
         cfg.local_budget = local_budget
+        cfg.chain_rows = chain_rows
         h = ctypes.c_void_p()
         check(lib().dcx_ctx_create(ctypes.byref(cfg), ctypes.byref(h)), "dcx_ctx_create")
         self.h = h
+        # graphs and models of this context: freed before it (the C-ABI
+        # requires it), also when an exception leaves them behind
+        self._children = weakref.WeakSet()
+
+    def _adopt(self, obj):
+        self._children.add(obj)
 
     def close(self):
         if self.h:
+            for ch in list(self._children):
+                ch.free()
             lib().dcx_ctx_destroy(self.h)
             self.h = None
 
@@ -356,6 +367,7 @@ class Context:
         g = Graph.__new__(Graph)
         g.ctx = self
         g.h = h
+        self._adopt(g)
         g.V = V
         return g
 
@@ -399,6 +411,7 @@ class Context:
         g = Graph.__new__(Graph)
         g.ctx = self
         g.h = h
+        self._adopt(g)
         g.V = V
         return g
 
@@ -435,6 +448,7 @@ class Context:
         g = Graph.__new__(Graph)
         g.ctx = self
         g.h = h
+        self._adopt(g)
         g.V = int(vbase[-1])
         return g, vbase
 
@@ -495,10 +509,12 @@ class Graph:
                   "dcx_graph_upload_csr", ctx.h)
         self.h = h
         self.V = V
+        ctx._adopt(self)
 
     def free(self):
         if self.h:
-            lib().dcx_graph_free(self.h)
+            if self.ctx.h:
+                lib().dcx_graph_free(self.h)
             self.h = None
 
     def __del__(self):
@@ -720,10 +736,12 @@ class MLP:
         check(lib().dcx_mlp_create(ctx.h, u64(seed), ctypes.byref(self.h)), "dcx_mlp_create",
               ctx.h)
         self.n_params = int(lib().dcx_mlp_param_count())
+        ctx._adopt(self)
 
     def free(self):
         if self.h:
-            check(lib().dcx_mlp_free(self.h), "dcx_mlp_free")
+            if self.ctx.h:
+                check(lib().dcx_mlp_free(self.h), "dcx_mlp_free")
             self.h = None
 
     @property

@@ -175,7 +175,14 @@ typedef struct dcx_config {
      * graph's last full decomposition time (at least 0.2 ms), or 12 entries
      * per edge while no time is known. */
     int32_t local_budget;
-    uint64_t reserved[2];
+    /* Batched rows: every chain_rows-th team row (from row 0) is peeled,
+     * and the rows between are derived from the row above by h-index
+     * refinement with per-vertex supports (only the cells that change are
+     * visited); 0 = the default spacing, 1 or -1 = every row peeled.
+     * Results are identical. */
+    int32_t chain_rows;
+    int32_t reserved32;
+    uint64_t reserved[1];
 } dcx_config;
 
 /* Timings of the last decomposition on a context, from HIP events on the
@@ -194,7 +201,7 @@ typedef struct dcx_timing {
     float rows_teams_ms;   /* batched: the team launch alone */
     float rows_lds_ms;     /* batched: the LDS-resident launch alone (beside it) */
     float rows_first_ms;   /* batched: row 0's grid pass, run first to bound the others */
-    int32_t reserved[1];
+    float rows_chains_ms;  /* batched: the launch deriving rows from the row above */
 } dcx_timing;
 
 dcx_status dcx_config_init(dcx_config* cfg);

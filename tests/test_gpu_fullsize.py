@@ -63,9 +63,9 @@ def test_rmat20_variants_equal_oracle_hash():
 
 def _learned_pair(c, g, G, name):
     """Work-share widths, then widths from a cost model trained on the GPU
-    from this graph's own pass log (from an R-MAT 16 graph's when this one
-    has no team row to log, as Chung-Lu 2^24 with kmax = 2): both must give
-    the oracle's hash."""
+    from this graph's own pass log (from an R-MAT 16 graph's, every team row
+    peeled, when this one has no team row to log, as Chung-Lu 2^24 with
+    kmax = 2): both must give the oracle's hash."""
     c.set_scheduler(None)
     say(f"{name}: work-share schedule")
     K, L = g.decompose(dcx.VARIANT_BATCHED)
@@ -76,10 +76,11 @@ def _learned_pair(c, g, G, name):
     X, T = np.concatenate([X1, X0]), np.concatenate([T1, T0])
     if len(X) == 0:
         s, d = dcx.gen_rmat(16, 16 << 16, 7)
-        h = c.upload_edges(1 << 16, s, d)
-        h.decompose(dcx.VARIANT_BATCHED)
-        X, T = h.pass_features(with_log=True)
-        h.free()
+        with dcx.Context(0, chain_rows=1) as c2:  # every team row peeled
+            h = c2.upload_edges(1 << 16, s, d)
+            h.decompose(dcx.VARIANT_BATCHED)
+            X, T = h.pass_features(with_log=True)
+            h.free()
         assert len(X) > 0
     m = dcx.MLP(c, seed=3)
     m.train(X, T, epochs=800, lr=3e-3)

@@ -69,7 +69,9 @@ def test_learned_scheduler_bit_exact_and_ablation():
     heur = []
     for tm in (2, 4, 8, 16, 32):
         say(f"rmat16 heuristic team_max={tm}")
-        with dcx.Context(0, team_max=tm) as c:
+        # chains off: the model schedules peeled team rows, and every team
+        # row of this graph would otherwise be derived from row 0
+        with dcx.Context(0, team_max=tm, chain_rows=1) as c:
             g = c.upload_edges(V, src, dst)
             for _ in range(2):
                 g.decompose(dcx.VARIANT_BATCHED)
@@ -83,7 +85,7 @@ def test_learned_scheduler_bit_exact_and_ablation():
     X = np.concatenate(Xs)
     T = np.concatenate(Ts)
     say(f"train on {len(T)} samples; schedule with the model")
-    with dcx.Context(0) as c:
+    with dcx.Context(0, chain_rows=1) as c:
         m = dcx.MLP(c, seed=1)
         m.train(X, T, epochs=2000, lr=3e-3)
         pred = m.predict(X)
