@@ -76,7 +76,7 @@ def main():
     ap.add_argument("--workload", default="headline",
                     choices=["headline", "rmat", "chunglu256", "serve", "learned", "ingest"])
     ap.add_argument("--schedule", default="learned", choices=["learned", "workshare"],
-                    help="headline: team widths from the cost model or the work-share rule")
+                    help="headline: rows planned by the cost model, or the fixed rule")
     ap.add_argument("--full-train", action="store_true",
                     help="headline: also train the cost model on passes logged from smaller "
                          "graphs at several team caps (default: the graph's own passes only)")
@@ -317,9 +317,12 @@ def run_headline(args):
         f"{1e3 * (tg1 - tg0):.0f} ms, {'upload' if args.host_graph else 'dedup ingest'} "
         f"{1e3 * (tg2 - tg1):.0f} ms")
 
-    # the fixed work-share schedule, then the cost model trained on the GPU
-    # from the passes those decompositions logged (with --full-train also
-    # from smaller graphs at several team caps)
+    # the fixed rule (every 32nd team row peeled, the rest derived from the
+    # row above; widths by work share), then the cost model trained on the
+    # GPU from what the decompositions logged: the fixed rule's runs, two
+    # exploration runs (every 8th row peeled, teams of up to 32 and of up
+    # to 4 workgroups), and two refinement rounds in which the model plans
+    # a run and is retrained with that run's log as well
     ctx.set_scheduler(None)
     g.decompose(dcx.VARIANT_BATCHED)
     heur = []
@@ -328,17 +331,34 @@ def run_headline(args):
         heur.append(ctx.timing().total_ms)
     h_hash = g.state_hash()
     model = None
+    n_samples = 0
+    refine_ms = []
     if args.schedule == "learned":
         Xs, Ts = learned_training_set(dcx, log) if args.full_train else ([], [])
-        for wl in (True, False):
-            X, T = g.pass_features(with_log=wl)
-            Xs.append(X)
-            Ts.append(T)
-        X, T = np.concatenate(Xs), np.concatenate(Ts)
+
+        def collect():
+            for wl in (True, False):
+                X, T = g.pass_features(with_log=wl)
+                Xs.append(X)
+                Ts.append(T)
+
+        collect()
+        for spacing, tmax in ((8, 0), (8, 4)):
+            ctx.set_rows_rule(spacing, tmax)
+            g.decompose(dcx.VARIANT_BATCHED)
+            collect()
+        ctx.set_rows_rule(0, 0)
         model = dcx.MLP(ctx, seed=7)
-        loss = model.train(X, T, epochs=2000, lr=3e-3)
-        log(f"cost model trained on {len(T)} samples, loss {loss:.4f}")
-        ctx.set_scheduler(model)
+        for rnd in range(3):
+            X, T = np.concatenate(Xs), np.concatenate(Ts)
+            loss = model.train(X, T, epochs=3000 if rnd == 0 else 1500, lr=3e-3)
+            n_samples = len(T)
+            log(f"cost model round {rnd}: {len(T)} samples, loss {loss:.4f}")
+            ctx.set_scheduler(model)
+            if rnd < 2:
+                g.decompose(dcx.VARIANT_BATCHED)
+                refine_ms.append(round(ctx.timing().total_ms, 2))
+                collect()
     for _ in range(args.warmup):
         g.decompose(dcx.VARIANT_BATCHED)
     tims = []
@@ -350,21 +370,43 @@ def run_headline(args):
     ms_per_step = wall / args.steps * 1e3
     state_hash = g.state_hash()
     used_model = bool(tims[-1].rows_scheduler)
-    log(f"steps done: {ms_per_step:.1f} ms/step kmax={K} lmax={lmax} (work-share "
+    log(f"steps done: {ms_per_step:.1f} ms/step kmax={K} lmax={lmax} (fixed rule "
         f"{min(heur):.1f} ms)")
 
     mean = lambda f: float(np.mean([getattr(t, f) for t in tims]))  # noqa: E731
+    # algorithmic bytes per row: a peeled row reads its core's lists and
+    # vertex words (8 E_k + 20 V_k); a derived row the list entries its
+    # walks read (8 bytes each: the entry and its neighbour's value, counted
+    # on the device) and 20 V_k of vertex words
     log_rows = g.pass_log()
-    vk = np.array([p["members"] for p in log_rows[1:]], np.float64)
-    ek = np.array([p["edges"] for p in log_rows[1:]], np.float64)
-    per_row = 8 * ek + 20 * vk
-    b_kmax = 8.0 * E + 20.0 * V
     split = int(tims[-1].rows_split)
     first = 1 if mean("rows_first_ms") > 0 else 0
+    team_b = lds_b = peel_equiv = 0.0
+    n_peeled = n_derived = 0
+    read_entries = core_entries = 0
+    for p in log_rows[1:]:
+        k, vk_, ek_ = p["k"], float(p["members"]), float(p["edges"])
+        full = 8 * ek_ + 20 * vk_
+        peel_equiv += full
+        if p["variant"] == dcx.VARIANT_DERIVED:
+            b = 8.0 * p["alive"] + 20 * vk_
+            n_derived += 1
+            read_entries += p["alive"]
+        else:
+            b = full
+            n_peeled += 1 if first <= k < split else 0
+            read_entries += 2 * p["edges"] if first <= k < split else 0
+        if first <= k < split:
+            team_b += b
+            core_entries += 2 * p["edges"]
+        elif k >= split:
+            lds_b += b
+    b_kmax = 8.0 * E + 20.0 * V
     teams_ms, lds_ms = mean("rows_teams_ms"), mean("rows_lds_ms")
     kernels = {
-        "k_rows_teams": (float(per_row[first:split].sum()), teams_ms, f"{first}..{split - 1}"),
-        "k_rows_lds": (float(per_row[split:].sum()), lds_ms, f"{split}..{K}"),
+        "k_rows_teams": (team_b, teams_ms, f"{first}..{split - 1} ({n_peeled} peeled, "
+                                           f"{n_derived} derived from the row above)"),
+        "k_rows_lds": (lds_b, lds_ms, f"{split}..{K}"),
         "k_pass_grid (kmax pass)": (b_kmax, mean("kmax_ms"), "kmax pass"),
     }
     dom = max(kernels, key=lambda k: kernels[k][1])
@@ -376,26 +418,33 @@ def run_headline(args):
                 if dom == "k_rows_teams" else None,
                 "kernel": dom.split(" ")[0], "kernel_ms": round(ms_dom, 3), "alg_bytes": b_dom,
                 "rows": rows_dom}
-    b_all = float(per_row.sum()) + b_kmax
+    b_all = team_b + lds_b + b_kmax + 8.0 * E + 20.0 * V  # + row 0 beside the kmax pass
     extra = {
         "kernels": {k: {"alg_bytes": v[0], "ms": round(v[1], 3), "rows": v[2],
                         "GBs": round(v[0] / (v[1] * 1e-3) / 1e9, 2) if v[1] > 0 else None}
                     for k, v in kernels.items()},
-        "full_decomposition": {"alg_bytes": b_all, "ms": round(ms_per_step, 3),
-                               "roofline_frac": round(b_all / (ms_per_step * 1e-3) / 1e9 /
-                                                      HBM_PEAK_GBS, 5),
-                               "edge_passes_per_s": round(float(ek.sum() + E) /
-                                                          (ms_per_step * 1e-3), 1)},
+        "full_decomposition": {
+            "alg_bytes": b_all, "ms": round(ms_per_step, 3),
+            "roofline_frac": round(b_all / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 5),
+            "peel_equivalent_bytes": peel_equiv + b_kmax,
+            "peel_equivalent_frac": round((peel_equiv + b_kmax) / (ms_per_step * 1e-3) / 1e9 /
+                                          HBM_PEAK_GBS, 5),
+            "team_rows_list_entries_read": int(read_entries),
+            "team_rows_core_list_entries": int(core_entries)},
         "breakdown_ms": {"kmax_pass": round(mean("kmax_ms"), 3),
                          "order": round(mean("order_ms"), 3),
                          "rows_kernel": round(mean("rows_kernel_ms"), 3),
                          "rows_total": round(mean("rows_ms"), 3)},
         "schedule": {"kind": args.schedule, "model_used": used_model,
-                     "training": "this graph's logged passes" + (
-                         " and smaller graphs' at several team caps" if args.full_train else ""),
-                     "work_share_ms": round(min(heur), 2),
-                     "learned_speedup_vs_work_share": round(min(heur) / ms_per_step, 3),
-                     "bit_exact_vs_work_share": state_hash == h_hash},
+                     "training": f"{n_samples} samples: this graph's logged team rows under the "
+                                 "fixed rule, two exploration runs (every 8th row peeled, team "
+                                 "caps 32 and 4) and two refinement rounds planned by the model" + (
+                                     ", and smaller graphs' at several team caps"
+                                     if args.full_train else ""),
+                     "refinement_rounds_ms": refine_ms,
+                     "fixed_rule_ms": round(min(heur), 2),
+                     "learned_speedup_vs_fixed_rule": round(min(heur) / ms_per_step, 3),
+                     "bit_exact_vs_fixed_rule": state_hash == h_hash},
         "ingest_ms": ({"host_sampling": round(1e3 * (tg1 - tg0), 1),
                        "upload": round(1e3 * (tg2 - tg1), 1)} if args.host_graph else
                       {"device_sampling": round(1e3 * (tg1 - tg0), 1),

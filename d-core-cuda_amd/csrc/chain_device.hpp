@@ -1,0 +1,663 @@
+// Rows derived from the row above ("chains") for the batched decomposition.
+//
+// Row k's values l_k are the greatest fixpoint of the local condition
+//   x(v) <= max{ l : |{u in out_k(v) : x(u) >= l}| >= l  and
+//                    |{w in in_k(v)  : x(w) >= l}| >= k }
+// over the members of the (k,0)-core (ranks [0, V_k) of the core layout;
+// out_k / in_k: the member prefixes of the sorted lists).  Since
+// D_{k,l} lies inside D_{k-1,l}, row k-1 restricted to V_k is an upper
+// bound of row k, and lowering the vertices that violate the condition
+// until none does reaches row k exactly (the h-index refinement of the
+// reference's kListRefineHIndex family, main.cu:1245-1438, here run from the
+// row above instead of from the degrees).  Between consecutive rows only a
+// small share of the cells changes, so a chain keeps, per member v, its
+// supports at its own level
+//   os(v) = |{u in out_k(v) : x(u) >= x(v)}|,  cs(v) = |{w in in_k(v) : x(w) >= x(v)}|
+// and visits only vertices whose support falls below the condition
+// (os < x or cs < k) and the lists of vertices that change or leave the
+// core -- instead of peeling all E_k edges of every row.
+//
+// One chain: row a (the anchor, already in `rows`) gives rows a+1..b.  Per
+// row k:
+//   1. the vertices leaving the core (ranks [V_k, V_{k-1})) are taken out
+//      of their neighbours' prefixes and supports;
+//   2. a scan of [0, V_k) queues every violator (cs < k catches the
+//      raised in-degree threshold);
+//   3. rounds: each queued vertex is re-evaluated from scratch (its value
+//      and both supports at that value; values only fall); then every
+//      vertex that fell from old to nv decrements the supports of the
+//      neighbours y with nv < x(y) <= old, and a decrement that takes a
+//      support below its threshold queues y for the next round;
+//   4. the row is written to `rows` (original ids).
+// Evaluations read the current values (a neighbour's new value may already
+// be visible); such a read can only make a support lower than exact, which
+// queues its vertex once more, and re-evaluation is exact.  Values never
+// drop below l_k (a vertex's l_k is feasible while every neighbour is at
+// or above its own l_k), so the fixpoint reached is l_k.
+//
+// A chain runs inside the persistent rows launch (k_rows_teams), on the
+// team that peeled its anchor or, for row 0's chain, on a team of its own:
+// workgroups of one XCD with the bounded team barrier of grid_sync.hpp,
+// every shared word accessed agent-coherently.  Lists of at least
+// kLongList entries are walked by a whole workgroup, the rest by one
+// wavefront.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <climits>
+
+#include "dcx_internal.hpp"
+#include "grid_sync.hpp"
+
+namespace dcx {
+namespace chain {
+
+constexpr int W = 64;
+constexpr int kCBlk = 512;
+constexpr int kCWaves = kCBlk / W;
+constexpr uint32_t kLongList = 2048;
+
+__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & (W - 1); }
+__device__ __forceinline__ uint32_t wave_id() { return threadIdx.x / W; }
+
+template <typename T>
+__device__ __forceinline__ T ldc(const T* p) {
+    return __hip_atomic_load(const_cast<T*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ void stc(T* p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
+#pragma unroll
+    for (int d = W / 2; d > 0; d >>= 1) x += (uint32_t)__shfl_xor(x, d, W);
+    return x;
+}
+
+__device__ __forceinline__ uint32_t first_not_below(const uint32_t* a, uint32_t n, uint32_t bound) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (a[mid] < bound)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// Per-chain state (team scratch, cap words each).
+struct ChainState {
+    int32_t* x;      // current value per rank
+    int32_t* os;     // out-support at own level
+    int32_t* cs;     // in-support at own level
+    uint32_t* pk;    // member prefix length of the K (out) list
+    uint32_t* pc;    // member prefix length of the C (in) list
+    uint32_t* mark;  // last round tag the vertex was queued for
+    uint32_t* qs0;   // short-list queues (ping-pong by round parity)
+    uint32_t* qs1;
+    uint32_t* ql0;   // long-list queues
+    uint32_t* ql1;
+    int32_t* oqs;    // old value of each short-queue entry
+    int32_t* oql;    // old value of each long-queue entry
+};
+
+__host__ __device__ __forceinline__ ChainState chain_state(uint32_t* p, uint64_t cap) {
+    ChainState s;
+    s.x = reinterpret_cast<int32_t*>(p);
+    s.os = reinterpret_cast<int32_t*>(p + cap);
+    s.cs = reinterpret_cast<int32_t*>(p + 2 * cap);
+    s.pk = p + 3 * cap;
+    s.pc = p + 4 * cap;
+    s.mark = p + 5 * cap;
+    s.qs0 = p + 6 * cap;
+    s.qs1 = p + 7 * cap;
+    s.ql0 = p + 8 * cap;
+    s.ql1 = p + 9 * cap;
+    s.oqs = reinterpret_cast<int32_t*>(p + 10 * cap);
+    s.oql = reinterpret_cast<int32_t*>(p + 11 * cap);
+    return s;
+}
+static_assert(kScratchArrays >= 12, "a chain needs 12 scratch arrays");
+__device__ __forceinline__ uint32_t* qshort(const ChainState& S, uint32_t par) {
+    return par ? S.qs1 : S.qs0;
+}
+__device__ __forceinline__ uint32_t* qlong(const ChainState& S, uint32_t par) {
+    return par ? S.ql1 : S.ql0;
+}
+
+// Slot words of a chain team (kSlotWords per team): [0..1] short-queue
+// lengths, [2..3] long-queue lengths (ping-pong by round parity), [4] row
+// maximum, [5] rounds of the current row, [8..9] list entries the row's
+// walks read (uint64).
+enum : uint32_t { kCqShort = 0, kCqLong = 2, kCqMax = 4, kCqRounds = 5, kCqVisited = 8 };
+__device__ __forceinline__ unsigned long long* visited_word(uint32_t* slots) {
+    return reinterpret_cast<unsigned long long*>(slots + kCqVisited);
+}
+
+struct ChainArgs {
+    uint32_t V;  // vertices (row stride of `rows`)
+    Csr K, C;    // core layout
+    const uint32_t* order;
+    const uint32_t* vk;
+    int32_t* rows;
+    int32_t* lmax_rows;
+    uint32_t* stats;  // nullable: [kStatWords] of row k at stats + kStatWords * k
+    uint32_t* err;
+};
+
+// Queues y for round tag `tag` unless it already is.
+__device__ __forceinline__ void enqueue(const ChainState& S, uint32_t* slots, uint32_t par,
+                                        uint32_t y, uint32_t tag) {
+    if (atomicMax(&S.mark[y], tag) >= tag) return;
+    const uint32_t len = ldc(&S.pk[y]) + ldc(&S.pc[y]);
+    if (len >= kLongList) {
+        const uint32_t at = atomicAdd(&slots[kCqLong + par], 1u);
+        stc(&qlong(S, par)[at], y);
+    } else {
+        const uint32_t at = atomicAdd(&slots[kCqShort + par], 1u);
+        stc(&qshort(S, par)[at], y);
+    }
+}
+
+// Counts of v's member neighbours with value >= l (K side, C side), summed
+// over the calling group: a wavefront (lane 0..63 of its lists) or the
+// whole workgroup (BLOCK; red: LDS of kCWaves * 2 words).
+template <bool BLOCK>
+__device__ __forceinline__ void count_ge(const ChainArgs& A, const ChainState& S, uint32_t v,
+                                         int32_t l, uint32_t* red, uint32_t& ck, uint32_t& cc,
+                                         uint64_t& vis) {
+    const uint32_t nk = ldc(&S.pk[v]), nc = ldc(&S.pc[v]);
+    if (!BLOCK || threadIdx.x == 0) vis += nk + nc;  // flushed by lane 0 of each wave
+    const uint32_t* kn = A.K.nbr + A.K.off[v];
+    const uint32_t* cn = A.C.nbr + A.C.off[v];
+    const uint32_t step = BLOCK ? kCBlk : W;
+    const uint32_t me = BLOCK ? threadIdx.x : lane_id();
+    uint32_t a = 0, b = 0;
+    for (uint32_t j = me; j < nk; j += step) a += ldc(&S.x[kn[j]]) >= l ? 1u : 0u;
+    for (uint32_t j = me; j < nc; j += step) b += ldc(&S.x[cn[j]]) >= l ? 1u : 0u;
+    a = wave_sum(a);
+    b = wave_sum(b);
+    if (BLOCK) {
+        __syncthreads();
+        if (lane_id() == 0) {
+            red[2 * wave_id()] = a;
+            red[2 * wave_id() + 1] = b;
+        }
+        __syncthreads();
+        a = b = 0;
+        for (int w = 0; w < kCWaves; ++w) {
+            a += red[2 * w];
+            b += red[2 * w + 1];
+        }
+    }
+    ck = a;
+    cc = b;
+}
+
+// Thresholds tested per pass over a vertex's lists when it is re-evaluated.
+constexpr int kProbe = 4;
+
+// For thresholds t[0..kProbe) (decreasing; an entry <= `below` is unused):
+// counts of v's member neighbours with value >= t[j], K side ck[j] and C
+// side cc[j], summed over the calling group (BLOCK: red holds
+// 2 * kProbe * kCWaves words).
+template <bool BLOCK>
+__device__ __forceinline__ void count_multi(const ChainArgs& A, const ChainState& S, uint32_t v,
+                                            const int32_t (&t)[kProbe], uint32_t* red,
+                                            uint32_t (&ck)[kProbe], uint32_t (&cc)[kProbe],
+                                            uint64_t& vis) {
+    const uint32_t nk = ldc(&S.pk[v]), nc = ldc(&S.pc[v]);
+    if (!BLOCK || threadIdx.x == 0) vis += nk + nc;  // flushed by lane 0 of each wave
+    const uint32_t* kn = A.K.nbr + A.K.off[v];
+    const uint32_t* cn = A.C.nbr + A.C.off[v];
+    const uint32_t step = BLOCK ? kCBlk : W;
+    const uint32_t me = BLOCK ? threadIdx.x : lane_id();
+    // per-lane counts of both sides packed in one word (K low, C high half:
+    // a lane sees at most 2^16 - 1 entries of one list, lists being shorter
+    // than 2^16 times the group's width), unpacked before the group sums
+    uint32_t acc[kProbe];
+#pragma unroll
+    for (int j = 0; j < kProbe; ++j) acc[j] = 0;
+    for (uint32_t i = me; i < nk; i += step) {
+        const int32_t xv = ldc(&S.x[kn[i]]);
+#pragma unroll
+        for (int j = 0; j < kProbe; ++j) acc[j] += xv >= t[j] ? 1u : 0u;
+    }
+    for (uint32_t i = me; i < nc; i += step) {
+        const int32_t xv = ldc(&S.x[cn[i]]);
+#pragma unroll
+        for (int j = 0; j < kProbe; ++j) acc[j] += xv >= t[j] ? 0x10000u : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < kProbe; ++j) {
+        ck[j] = wave_sum(acc[j] & 0xffffu);
+        cc[j] = wave_sum(acc[j] >> 16);
+    }
+    if (BLOCK) {
+        __syncthreads();
+        if (lane_id() == 0) {
+#pragma unroll
+            for (int j = 0; j < kProbe; ++j) {
+                red[(2 * j) * kCWaves + wave_id()] = ck[j];
+                red[(2 * j + 1) * kCWaves + wave_id()] = cc[j];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kProbe; ++j) {
+            uint32_t a = 0, b = 0;
+            for (int w = 0; w < kCWaves; ++w) {
+                a += red[(2 * j) * kCWaves + w];
+                b += red[(2 * j + 1) * kCWaves + w];
+            }
+            ck[j] = a;
+            cc[j] = b;
+        }
+    }
+}
+
+// Re-evaluates v (value old): its new value nv (at most old) and both
+// supports at it.  A value l is feasible when at least l K-neighbours and
+// at least k C-neighbours have values >= l; feasible values are closed
+// downwards and 0 always is (its supports are the prefix lengths), so nv is
+// the largest feasible l <= old.  Every pass over the lists tests kProbe
+// thresholds: the first old, old-1, old-2, old-4 (values most
+// often fall by one or two), the next ones spread over the interval left,
+// until the largest feasible value is a tested one.  The supports kept are
+// the counts of the pass that found nv feasible: a fresh count could read
+// neighbours that fell since and leave a support below its threshold that
+// no decrement crosses.  Returns nv; the caller's group has the same view.
+template <bool BLOCK>
+__device__ __forceinline__ int32_t evaluate(const ChainArgs& A, const ChainState& S, uint32_t v,
+                                            int32_t old, int32_t k, uint32_t* red, uint32_t& os,
+                                            uint32_t& cs, uint64_t& vis) {
+    int32_t lo = 0, hi = old;  // lo feasible with supports (lk, lc); values above hi are not
+    uint32_t lk = ldc(&S.pk[v]), lc = ldc(&S.pc[v]);
+    bool first = true;
+    while (lo < hi) {
+        int32_t t[kProbe];
+        if (first) {
+            t[0] = old;
+#pragma unroll
+            for (int j = 1; j < kProbe; ++j) t[j] = max(old - (1 << (j - 1)), lo);
+        } else {
+            const int32_t step = (hi - lo + kProbe - 1) / kProbe;
+#pragma unroll
+            for (int j = 0; j < kProbe; ++j) t[j] = max(hi - j * step, lo);
+        }
+        first = false;
+        uint32_t ck[kProbe], cc[kProbe];
+        count_multi<BLOCK>(A, S, v, t, red, ck, cc, vis);
+        // the largest tested threshold above lo that is feasible
+        int jf = -1;
+        int32_t below = hi + 1;  // the smallest tested value above lo, if none is feasible
+#pragma unroll
+        for (int j = 0; j < kProbe; ++j) {
+            if (jf >= 0 || t[j] <= lo) continue;
+            if ((int32_t)ck[j] >= t[j] && (int32_t)cc[j] >= k) {
+                jf = j;
+            } else {
+                below = t[j];
+            }
+        }
+        if (jf >= 0) {
+            int32_t tf = 0;
+            uint32_t fk = 0, fc = 0;
+#pragma unroll
+            for (int j = 0; j < kProbe; ++j)
+                if (j == jf) {
+                    tf = t[j];
+                    fk = ck[j];
+                    fc = cc[j];
+                }
+            lo = tf;
+            lk = fk;
+            lc = fc;
+        }
+        hi = below - 1;
+        if (hi < lo) hi = lo;
+    }
+    os = lk;
+    cs = lc;
+    return lo;
+}
+
+// v fell from old to nv: neighbours y with nv < x(y) <= old lose v from a
+// support; a support that crosses below its threshold queues y.
+template <bool BLOCK>
+__device__ __forceinline__ void notify(const ChainArgs& A, const ChainState& S, uint32_t* slots,
+                                       uint32_t par, uint32_t v, int32_t old, int32_t nv,
+                                       int32_t k, uint32_t tag, uint64_t& vis) {
+    const uint32_t nk = ldc(&S.pk[v]), nc = ldc(&S.pc[v]);
+    if (!BLOCK || threadIdx.x == 0) vis += nk + nc;  // flushed by lane 0 of each wave
+    const uint32_t* kn = A.K.nbr + A.K.off[v];
+    const uint32_t* cn = A.C.nbr + A.C.off[v];
+    const uint32_t step = BLOCK ? kCBlk : W;
+    const uint32_t me = BLOCK ? threadIdx.x : lane_id();
+    // u in C-list of v: v is in u's K (out) list -> os(u)
+    for (uint32_t j = me; j < nc; j += step) {
+        const uint32_t u = cn[j];
+        const int32_t xu = ldc(&S.x[u]);
+        if (xu > nv && xu <= old) {
+            const int32_t s = atomicSub(&S.os[u], 1);
+            if (s == xu) enqueue(S, slots, par, u, tag);
+        }
+    }
+    // w in K-list of v: v is in w's C (in) list -> cs(w)
+    for (uint32_t j = me; j < nk; j += step) {
+        const uint32_t w = kn[j];
+        const int32_t xw = ldc(&S.x[w]);
+        if (xw > nv && xw <= old) {
+            const int32_t s = atomicSub(&S.cs[w], 1);
+            if (s == k) enqueue(S, slots, par, w, tag);
+        }
+    }
+}
+
+// r (value xr) leaves the core at row k (new member bound V): its member
+// neighbours drop it from their prefixes and supports.
+template <bool BLOCK>
+__device__ __forceinline__ void leave(const ChainArgs& A, const ChainState& S, uint32_t r,
+                                      uint32_t V, uint64_t& vis) {
+    const int32_t xr = ldc(&S.x[r]);
+    const uint32_t nk = ldc(&S.pk[r]), nc = ldc(&S.pc[r]);
+    if (!BLOCK || threadIdx.x == 0) vis += nk + nc;  // flushed by lane 0 of each wave
+    const uint32_t* kn = A.K.nbr + A.K.off[r];
+    const uint32_t* cn = A.C.nbr + A.C.off[r];
+    const uint32_t step = BLOCK ? kCBlk : W;
+    const uint32_t me = BLOCK ? threadIdx.x : lane_id();
+    for (uint32_t j = me; j < nc; j += step) {
+        const uint32_t u = cn[j];
+        if (u >= V) continue;
+        consume(atomicSub(&S.pk[u], 1u));
+        if (xr >= ldc(&S.x[u])) consume((uint32_t)atomicSub(&S.os[u], 1));
+    }
+    for (uint32_t j = me; j < nk; j += step) {
+        const uint32_t w = kn[j];
+        if (w >= V) continue;
+        consume(atomicSub(&S.pc[w], 1u));
+        if (xr >= ldc(&S.x[w])) consume((uint32_t)atomicSub(&S.cs[w], 1));
+    }
+}
+
+
+// ---- set-up of row 0's chain on the whole grid, before the rows launch
+// (row 0 is ready by then; a team would do the same work on a sixteenth of
+// the chip).  S is the chain's team scratch, slots its team slots (zeroed).
+
+// x, marks and member prefixes for row 1 (one thread per rank).
+__global__ __launch_bounds__(256) void k_chain_preset_values(ChainArgs A, ChainState S) {
+    const uint32_t V0 = A.vk[0], V1 = A.vk[1];
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < V0;
+         r += gridDim.x * blockDim.x) {
+        S.x[r] = A.rows[A.order[r]];
+        S.mark[r] = 0u;
+        if (r < V1) {
+            S.pk[r] = first_not_below(A.K.nbr + A.K.off[r], (uint32_t)A.K.deg[r], V1);
+            S.pc[r] = first_not_below(A.C.nbr + A.C.off[r], (uint32_t)A.C.deg[r], V1);
+        }
+    }
+}
+
+// Supports at own level for ranks [0, V_1): one wavefront per vertex;
+// vertices with long lists are queued for k_chain_preset_long.
+__global__ __launch_bounds__(kCBlk) void k_chain_preset_supports(ChainArgs A, ChainState S,
+                                                                 uint32_t* slots) {
+    const uint32_t V1 = A.vk[1];
+    const uint32_t nw = gridDim.x * kCWaves;
+    uint64_t vis = 0;  // (not logged: row 1's log counts its own walks only)
+    for (uint32_t r = blockIdx.x * kCWaves + wave_id(); r < V1; r += nw) {
+        if (S.pk[r] + S.pc[r] >= kLongList) {
+            if (lane_id() == 0) S.ql0[atomicAdd(&slots[kCqLong], 1u)] = r;
+            continue;
+        }
+        uint32_t ck, cc;
+        count_ge<false>(A, S, r, S.x[r], nullptr, ck, cc, vis);
+        if (lane_id() == 0) {
+            S.os[r] = (int32_t)ck;
+            S.cs[r] = (int32_t)cc;
+        }
+    }
+}
+
+// Supports of the queued long-list vertices: one workgroup each.
+__global__ __launch_bounds__(kCBlk) void k_chain_preset_long(ChainArgs A, ChainState S,
+                                                             uint32_t* slots) {
+    __shared__ uint32_t red[2 * kCWaves];
+    const uint32_t nl = slots[kCqLong];
+    uint64_t vis = 0;  // (not logged)
+    for (uint32_t i = blockIdx.x; i < nl; i += gridDim.x) {
+        const uint32_t r = S.ql0[i];
+        uint32_t ck, cc;
+        count_ge<true>(A, S, r, S.x[r], red, ck, cc, vis);
+        if (threadIdx.x == 0) {
+            S.os[r] = (int32_t)ck;
+            S.cs[r] = (int32_t)cc;
+        }
+    }
+}
+
+// Derives rows a+1..b from row a on the calling team (gs: its barrier
+// words, rank and size).  With `preset`, the chain's values, prefixes and
+// supports for row a+1 are already in S (row 0's chain, set up by the
+// k_chain_preset_* kernels).  `tag` carries the round tags from chain to
+// chain of the team (they only grow).  Returns false when the team stopped
+// on a timeout or on inconsistent supports (reported through A.err).  The
+// pass-log entry of each derived row gets its rounds as phases, levels = 1
+// for the first row of a chain whose time includes the supports' set-up
+// (not preset) and 0 for the others.
+__device__ inline bool run_chain(const ChainArgs& A, const ChainState& S, uint32_t* slots,
+                                 GridSync& gs, int32_t a, int32_t b, bool preset, uint32_t& tag) {
+    __shared__ uint32_t red[2 * kProbe * kCWaves];
+    __shared__ uint32_t s_n[2];
+    const uint32_t tw = gs.rank, tn = gs.nblocks;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t gw = tw * kCWaves + wave_id(), nw = tn * kCWaves;
+    bool ok = true;
+    auto barrier = [&]() {
+        if (!ok) return;
+        if (tn > 1)
+            ok = grid_barrier(gs, false);
+        else
+            __syncthreads();
+    };
+    const uint64_t t_chain = wall_clock64();
+    // list entries read by this thread's group (counted by every lane,
+    // added once per wavefront or workgroup at the end of each row; the
+    // supports' set-up counts towards the first row)
+    uint64_t vis = 0;
+    const uint32_t Va = A.vk[a];
+    uint32_t V = A.vk[a + 1];
+    if (preset) {
+        if (tw == 0 && tid < 6) coherent_write(slots + tid, 0u);
+        barrier();
+    } else {
+        // ---- start from row a: values, prefixes and supports for row a+1
+        for (uint32_t r = tw * kCBlk + tid; r < Va; r += tn * kCBlk) {
+            stc(&S.x[r], A.rows[(size_t)a * A.V + A.order[r]]);
+            stc(&S.mark[r], 0u);
+            if (r < V) {
+                stc(&S.pk[r], first_not_below(A.K.nbr + A.K.off[r], (uint32_t)A.K.deg[r], V));
+                stc(&S.pc[r], first_not_below(A.C.nbr + A.C.off[r], (uint32_t)A.C.deg[r], V));
+            }
+        }
+        if (tw == 0 && tid < 6) coherent_write(slots + tid, 0u);
+        if (tw == 0 && tid == 0) consume((uint32_t)atomicExch(visited_word(slots), 0ull));
+        barrier();
+        // supports: one wavefront per vertex; long lists are collected and
+        // counted by whole workgroups
+        for (uint32_t r = gw; r < V && ok; r += nw) {
+            const uint32_t len = ldc(&S.pk[r]) + ldc(&S.pc[r]);
+            if (len >= kLongList) {
+                if (lane_id() == 0) stc(&qlong(S, 0)[atomicAdd(&slots[kCqLong], 1u)], r);
+                continue;
+            }
+            uint32_t ck, cc;
+            count_ge<false>(A, S, r, ldc(&S.x[r]), red, ck, cc, vis);
+            if (lane_id() == 0) {
+                stc(&S.os[r], (int32_t)ck);
+                stc(&S.cs[r], (int32_t)cc);
+            }
+        }
+        barrier();
+        if (!ok) {
+            if (tid == 0) consume(atomicMax(A.err, (uint32_t)kErrTimeout));
+            return false;
+        }
+        {
+            if (tid == 0) s_n[0] = coherent_read(&slots[kCqLong]);
+            __syncthreads();
+            const uint32_t nl = s_n[0];
+            __syncthreads();
+            for (uint32_t i = tw; i < nl; i += tn) {
+                const uint32_t r = ldc(&qlong(S, 0)[i]);
+                uint32_t ck, cc;
+                count_ge<true>(A, S, r, ldc(&S.x[r]), red, ck, cc, vis);
+                if (tid == 0) {
+                    stc(&S.os[r], (int32_t)ck);
+                    stc(&S.cs[r], (int32_t)cc);
+                }
+            }
+        }
+        // (its counter is cleared by the first round, after the barriers
+        // that every member passes only once it has read it)
+        barrier();
+    }
+    // ---- rows a+1 .. b
+    for (int32_t k = a + 1; k <= b && ok; ++k) {
+        const uint64_t t_row = k == a + 1 ? t_chain : wall_clock64();
+        V = A.vk[k];
+        if (k > a + 1) {
+            const uint32_t Vp = A.vk[k - 1];
+            // 1. leavers: long lists by workgroups, short by wavefronts
+            for (uint32_t r = V + tw; r < Vp; r += tn) {
+                if (ldc(&S.pk[r]) + ldc(&S.pc[r]) >= kLongList) leave<true>(A, S, r, V, vis);
+            }
+            for (uint32_t r = V + gw; r < Vp; r += nw) {
+                if (ldc(&S.pk[r]) + ldc(&S.pc[r]) < kLongList) leave<false>(A, S, r, V, vis);
+            }
+            barrier();
+        }
+        // 2. violators of row k
+        ++tag;
+        uint32_t par = tag & 1u;
+        if (tw == 0 && tid == 0) {
+            coherent_write(&slots[kCqShort + par], 0u);
+            coherent_write(&slots[kCqLong + par], 0u);
+            coherent_write(&slots[kCqMax], 0u);
+        }
+        barrier();
+        for (uint32_t r = tw * kCBlk + tid; r < V; r += tn * kCBlk) {
+            const int32_t xr = ldc(&S.x[r]);
+            if (ldc(&S.os[r]) < xr || ldc(&S.cs[r]) < k) enqueue(S, slots, par, r, tag);
+        }
+        barrier();
+        uint32_t rounds = 0;
+        for (;;) {
+            if (!ok) break;
+            if (tid == 0) {
+                s_n[0] = coherent_read(&slots[kCqShort + par]);
+                s_n[1] = coherent_read(&slots[kCqLong + par]);
+            }
+            __syncthreads();
+            const uint32_t ns = s_n[0], nl = s_n[1];
+            __syncthreads();
+            if (ns + nl == 0) break;
+            // every round lowers a value (spurious queueing aside, which
+            // one exact re-count ends): more rounds than twice the
+            // members means the supports are wrong
+            if (++rounds > 2 * V + 64) {
+                if (tid == 0) consume(atomicMax(A.err, (uint32_t)kErrInconsistent));
+                ok = false;
+                break;
+            }
+            const uint32_t ntag = tag + 1, npar = ntag & 1u;
+            if (tw == 0 && tid == 0) {
+                coherent_write(&slots[kCqShort + npar], 0u);
+                coherent_write(&slots[kCqLong + npar], 0u);
+            }
+            // A. evaluate
+            for (uint32_t i = tw; i < nl; i += tn) {
+                const uint32_t v = ldc(&qlong(S, par)[i]);
+                const int32_t old = ldc(&S.x[v]);
+                uint32_t os, cs;
+                const int32_t nv = evaluate<true>(A, S, v, old, k, red, os, cs, vis);
+                if (tid == 0) {
+                    stc(&S.os[v], (int32_t)os);
+                    stc(&S.cs[v], (int32_t)cs);
+                    stc(&S.oql[i], old);
+                    stc(&S.x[v], nv);
+                }
+                __syncthreads();
+            }
+            for (uint32_t i = gw; i < ns; i += nw) {
+                const uint32_t v = ldc(&qshort(S, par)[i]);
+                const int32_t old = ldc(&S.x[v]);
+                uint32_t os, cs;
+                const int32_t nv = evaluate<false>(A, S, v, old, k, red, os, cs, vis);
+                if (lane_id() == 0) {
+                    stc(&S.os[v], (int32_t)os);
+                    stc(&S.cs[v], (int32_t)cs);
+                    stc(&S.oqs[i], old);
+                    stc(&S.x[v], nv);
+                }
+            }
+            barrier();
+            // B. notify
+            for (uint32_t i = tw; i < nl; i += tn) {
+                const uint32_t v = ldc(&qlong(S, par)[i]);
+                const int32_t old = ldc(&S.oql[i]), nv = ldc(&S.x[v]);
+                if (nv < old) notify<true>(A, S, slots, npar, v, old, nv, k, ntag, vis);
+            }
+            for (uint32_t i = gw; i < ns; i += nw) {
+                const uint32_t v = ldc(&qshort(S, par)[i]);
+                const int32_t old = ldc(&S.oqs[i]), nv = ldc(&S.x[v]);
+                if (nv < old) notify<false>(A, S, slots, npar, v, old, nv, k, ntag, vis);
+            }
+            barrier();
+            tag = ntag;
+            par = npar;
+        }
+        if (!ok) break;
+        // 4. the row, and its maximum
+        int32_t mx = 0;
+        for (uint32_t r = tw * kCBlk + tid; r < V; r += tn * kCBlk) {
+            const int32_t xr = ldc(&S.x[r]);
+            A.rows[(size_t)k * A.V + A.order[r]] = xr;
+            mx = max(mx, xr);
+        }
+#pragma unroll
+        for (int d = W / 2; d > 0; d >>= 1) mx = max(mx, __shfl_xor(mx, d, W));
+        if (lane_id() == 0 && mx > 0) consume((uint32_t)atomicMax(&slots[kCqMax], (uint32_t)mx));
+        // wavefront walks counted on lane 0, workgroup walks on thread 0
+        if (lane_id() == 0 && vis) consume((uint32_t)atomicAdd(visited_word(slots), vis));
+        vis = 0;
+        barrier();
+        if (tw == 0 && tid == 0) {
+            const int32_t lm = V ? (int32_t)coherent_read(&slots[kCqMax]) : 0;
+            const uint64_t seen = atomicExch(visited_word(slots), 0ull);
+            A.lmax_rows[k] = lm;
+            if (A.stats) {
+                uint32_t* st = A.stats + (size_t)kStatWords * k;
+                const uint64_t dt = wall_clock64() - t_row;
+                st[kStatPhases] = rounds;
+                st[kStatLevels] = k == a + 1 && !preset ? 1u : 0u;
+                st[kStatTicksLo] = (uint32_t)dt;
+                st[kStatTicksHi] = (uint32_t)(dt >> 32);
+                st[kStatFailed] = 0;
+                st[kStatRemoved] = V;
+                st[kStatAlive] = (uint32_t)min<uint64_t>(seen, 0xffffffffull);
+                st[kStatLevel] = (uint32_t)lm;
+            }
+        }
+    }
+    // the team's next pass reuses the scratch
+    barrier();
+    if (!ok && tid == 0) consume(atomicMax(A.err, (uint32_t)kErrTimeout));
+    return ok;
+}
+
+}  // namespace chain
+}  // namespace dcx

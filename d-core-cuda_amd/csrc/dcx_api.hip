@@ -527,22 +527,99 @@ Sched make_schedule(const dcx_graph* g, int32_t k_first, int32_t kr, uint32_t G,
     return s;
 }
 
-// Scheduler features of row pass k (kMlpIn floats): sizes of the (k,0)-core,
-// the vertices whose kmax is k, the row's level/phase counts from its last
-// run (0 if unknown) and the team width it would run on.  Scaled to about
-// [0, 1].
-void pass_features(const dcx_graph* g, int32_t k, uint32_t team, uint32_t levels,
-                   uint32_t phases, float* f) {
+enum : int { kPassPeel = 0, kPassFirst = 1, kPassDerived = 2 };
+
+// Kind of row k's last run in the batched launch (-1: none, or another
+// variant), from the pass log.
+int last_pass_kind(const dcx_graph* g, int32_t k) {
+    if ((size_t)k + 1 >= g->h_pass_variant.size() || g->h_pass_team[k + 1] == 0) return -1;
+    if (g->h_pass_variant[k + 1] == DCX_VARIANT_BATCHED) return k >= g->rows_first_grid ? 0 : -1;
+    if (g->h_pass_variant[k + 1] != DCX_VARIANT_DERIVED) return -1;
+    if (g->h_pstats.size() < ((size_t)k + 2) * kStatWords) return -1;
+    return g->h_pstats[(size_t)(k + 1) * kStatWords + kStatLevels] ? kPassFirst : kPassDerived;
+}
+
+// Folds the team rows of the last batched launch (rows 0..kr) into the
+// per-row work statistics.
+void record_row_stats(dcx_graph* g, int32_t kr) {
+    const size_t n = (size_t)g->kmax + 1;
+    g->h_peel_phases.resize(n, 0);
+    g->h_der_rounds.resize(n, 0);
+    g->h_der_reads.resize(n, 0);
+    for (int32_t k = 0; k <= kr && (size_t)k < n; ++k) {
+        const int kind = last_pass_kind(g, k);
+        if (kind < 0) continue;
+        const uint32_t* w = g->h_pstats.data() + (size_t)(k + 1) * kStatWords;
+        if (kind == kPassPeel) {
+            g->h_peel_phases[k] = w[kStatPhases] + 1;
+        } else {
+            // a first row's walks include the supports' set-up: one pass
+            // over both member prefixes of every member, 2 E_k entries
+            uint64_t reads = w[kStatAlive];
+            if (kind == kPassFirst) reads -= std::min<uint64_t>(reads, 2 * g->h_ek[k]);
+            g->h_der_rounds[k] = w[kStatPhases] + 1;
+            g->h_der_reads[k] = reads + 1;
+        }
+    }
+}
+
+// Statistic of row k (value + 1 as stored), or of the nearest row that has
+// one (up to 64 rows away); 0 if none.
+template <typename T>
+T row_stat(const std::vector<T>& v, int32_t k) {
+    const int32_t n = (int32_t)v.size();
+    for (int32_t d = 0; d <= 64; ++d) {
+        if (k - d >= 0 && k - d < n && v[k - d]) return v[k - d];
+        if (k + d < n && v[k + d]) return v[k + d];
+    }
+    return 0;
+}
+
+// Scheduler features of row k run as `kind` (kPassPeel: peeled; kPassFirst:
+// derived from the row above as the first row of a chain, its time
+// including the supports' set-up; kPassDerived: derived, later in a chain)
+// on a team of `team` workgroups (kMlpIn floats, about [0, 1]): sizes of the
+// (k,0)-core and the vertices that left it since row k-1, the kind, the
+// width, and the work row k took when it last ran that way (phases of a
+// peel; rounds and list entries read of a derivation, plus the set-up's
+// 2 E_k entries for a first row), from this graph's batched launches (or
+// the nearest row's; with_stats = false: those two as unknown).
+void pass_features(const dcx_graph* g, int32_t k, uint32_t team, int kind, bool with_stats,
+                   float* f) {
     const double Vk = g->h_vk[k], Ek = (double)g->h_ek[k];
-    const double Vn = k + 1 <= g->kmax ? g->h_vk[k + 1] : 0.0;
-    f[0] = (float)(std::log2(1.0 + Vk) / 20.0);
-    f[1] = (float)(std::log2(1.0 + Ek) / 30.0);
-    f[2] = (float)(std::log2(1.0 + Ek / std::max(Vk, 1.0)) / 10.0);
-    f[3] = (float)((k + 0.5) / (g->kmax + 1.0));
+    const double Vp = k > 0 ? g->h_vk[k - 1] : Vk;
+    double steps = 0.0, reads = 0.0;
+    if (with_stats) {
+        if (kind == kPassPeel) {
+            const uint32_t ph = row_stat(g->h_peel_phases, k);
+            steps = ph ? ph - 1.0 : 0.0;
+            reads = 2.0 * Ek;
+        } else {
+            const uint32_t r = row_stat(g->h_der_rounds, k);
+            const uint64_t e = row_stat(g->h_der_reads, k);
+            steps = r ? r - 1.0 : 0.0;
+            reads = e ? (double)(e - 1) : 0.0;
+            if (kind == kPassFirst) reads += 2.0 * Ek;
+        }
+    }
+    f[0] = (float)(std::log2(1.0 + Vk) / 24.0);
+    f[1] = (float)(std::log2(1.0 + Ek) / 32.0);
+    f[2] = (float)(std::log2(1.0 + Vp - Vk) / 24.0);
+    f[3] = (float)kind * 0.5f;
     f[4] = (float)(std::log2((double)std::max(team, 1u)) / 8.0);
-    f[5] = (float)(std::log2(1.0 + Vk - Vn) / 20.0);
-    f[6] = (float)(std::log2(1.0 + levels) / 10.0);
-    f[7] = (float)(std::log2(1.0 + phases) / 12.0);
+    f[5] = (float)(std::log2(1.0 + steps) / 12.0);
+    f[6] = (float)(std::log2(1.0 + reads) / 36.0);
+    f[7] = (float)(std::log2(1.0 + Ek / std::max(Vk, 1.0)) / 12.0);
+}
+
+// True once a batched launch has logged team rows of g (the cost model's
+// plan needs their work statistics).
+bool have_row_stats(const dcx_graph* g) {
+    for (uint32_t x : g->h_peel_phases)
+        if (x) return true;
+    for (uint32_t x : g->h_der_rounds)
+        if (x) return true;
+    return false;
 }
 
 void mlp_stage(dcx_ctx* c, dcx_mlp* m, size_t floats) {
@@ -565,50 +642,101 @@ std::vector<float> mlp_predict(dcx_ctx* c, dcx_mlp* m, const std::vector<float>&
     return y;
 }
 
-// Team widths from the cost model: the predicted time d(k, t) of every
-// pass at every candidate width, then the smallest makespan T for which
-// each pass gets the narrowest width with d <= T, the wide teams fit in 3/4
-// of the grid (all of it when no pass is left for a single workgroup) and
-// the single-workgroup passes (packed longest first) fit in the rest.
-// Passes below k_first are not scheduled.  Returns false if no width
-// assignment was found.
-bool mlp_team_sizes(dcx_ctx* c, const dcx_graph* g, int32_t k_first, int32_t kr, uint32_t G,
-                    uint32_t team_max, std::vector<uint32_t>& want, std::vector<double>& wts) {
-    const uint32_t np = (uint32_t)kr + 1, k0 = (uint32_t)std::max(k_first, 0);
+// Plan of the team rows 1..kr (row 0 is ready before the launch) from the
+// cost model.  A unit is an anchor a (row 0, or a row peeled by the unit's
+// team) with the chain a+1..b derived from it on the same team; its
+// predicted time at width t is peel(a, t) (0 for row 0) + first(a+1, t) +
+// sum over k = a+2..b of derived(k, t).  For a makespan T the rows are cut
+// into units from the top, each extended while some width keeps it within
+// T, and each unit takes the narrowest width that does; T is the smallest
+// value (binary search) for which the wide units fit 3/4 of the grid (all
+// of it when no unit is left for a single workgroup) and the single-
+// workgroup units, packed longest first, fit the rest.  With `fixed` set,
+// peel/chain_last are given and only the widths are chosen.  Outputs:
+// peel[k] (row k is an anchor), chain_last[a] (last row of a's chain, -1:
+// none), and per anchor the width want[a] and the weight wts[a] (predicted
+// workgroup time, which places the units longest first).  Returns false
+// if no plan fits.
+bool mlp_rows_plan(dcx_ctx* c, const dcx_graph* g, int32_t kr, uint32_t G, uint32_t team_max,
+                   bool fixed, std::vector<char>& peel, std::vector<int32_t>& chain_last,
+                   std::vector<uint32_t>& want, std::vector<double>& wts) {
+    const uint32_t np = (uint32_t)kr + 1;
     const uint32_t cap = std::min(team_max, G / kXcds);
     std::vector<uint32_t> cands{1};
     for (uint32_t t = 2; t <= cap; t *= 2) cands.push_back(t);
     const uint32_t nc = (uint32_t)cands.size();
-    const bool have_log = g->h_pstats.size() >= ((size_t)np + 1) * kStatWords;
-    std::vector<float> X((size_t)np * nc * kMlpIn);
-    for (uint32_t k = 0; k < np; ++k) {
-        uint32_t lv = 0, ph = 0;
-        if (have_log) {
-            ph = g->h_pstats[(size_t)(k + 1) * kStatWords + kStatPhases];
-            lv = g->h_pstats[(size_t)(k + 1) * kStatWords + kStatLevels];
+    if (!have_row_stats(g)) return false;  // nothing logged yet: the fixed rule
+    // predictions: [kind][k][width]
+    std::vector<float> X((size_t)3 * np * nc * kMlpIn);
+    for (int kind = 0; kind < 3; ++kind)
+        for (uint32_t k = 0; k < np; ++k)
+            for (uint32_t j = 0; j < nc; ++j)
+                pass_features(g, (int32_t)k, cands[j], kind, true,
+                              &X[(((size_t)kind * np + k) * nc + j) * kMlpIn]);
+    const std::vector<float> y = mlp_predict(c, c->sched_mlp, X, 3 * np * nc);
+    auto d = [&](int kind, uint32_t k, uint32_t j) {
+        return std::exp2((double)y[((size_t)kind * np + k) * nc + j]);
+    };
+    // unit cost at every width, for the unit a..b
+    auto unit_cost = [&](uint32_t a, uint32_t b, uint32_t j) {
+        double t = a > 0 ? d(kPassPeel, a, j) : 0.0;
+        // (row 0's chain is set up before the launch: its first row is an
+        // ordinary derived row)
+        for (uint32_t k = a + 1; k <= b; ++k)
+            t += d(k == a + 1 && a > 0 ? kPassFirst : kPassDerived, k, j);
+        return t;
+    };
+    struct Unit {
+        uint32_t a, b, j;
+        double cost;
+    };
+    auto plan = [&](double T, std::vector<Unit>& units) -> bool {
+        units.clear();
+        if (fixed) {
+            for (uint32_t a = 0; a < np; ++a) {
+                if (!(a == 0 ? chain_last[0] >= 0 : peel[a] != 0)) continue;
+                const uint32_t b = chain_last[a] >= 0 ? (uint32_t)chain_last[a] : a;
+                uint32_t j = 0;
+                while (j + 1 < nc && unit_cost(a, b, j) > T) ++j;
+                if (unit_cost(a, b, j) > T) return false;
+                units.push_back({a, b, j, unit_cost(a, b, j)});
+            }
+        } else {
+            uint32_t a = 0;
+            std::vector<double> run(nc, 0.0);  // cost of a..k at each width
+            for (uint32_t k = 1; k <= np; ++k) {
+                bool extend = false;
+                if (k < np) {
+                    const int kind = k == a + 1 && a > 0 ? kPassFirst : kPassDerived;
+                    for (uint32_t j = 0; j < nc; ++j)
+                        extend = extend || run[j] + d(kind, k, j) <= T;
+                    if (extend)
+                        for (uint32_t j = 0; j < nc; ++j) run[j] += d(kind, k, j);
+                }
+                if (extend) continue;
+                // close the unit a..k-1 (row 0 alone is no unit)
+                if (a > 0 || k > 1) {
+                    uint32_t j = 0;
+                    while (j + 1 < nc && run[j] > T) ++j;
+                    if (run[j] > T) return false;
+                    units.push_back({a, k - 1, j, run[j]});
+                }
+                if (k == np) break;
+                a = k;  // k becomes an anchor
+                for (uint32_t j = 0; j < nc; ++j) run[j] = d(kPassPeel, k, j);
+            }
         }
-        for (uint32_t j = 0; j < nc; ++j)
-            pass_features(g, (int32_t)k, cands[j], lv, ph, &X[((size_t)k * nc + j) * kMlpIn]);
-    }
-    const std::vector<float> y = mlp_predict(c, c->sched_mlp, X, np * nc);
-    std::vector<double> d((size_t)np * nc);
-    for (size_t i = 0; i < d.size(); ++i) d[i] = std::exp2((double)y[i]);
-    auto assign = [&](double T, std::vector<uint32_t>& sz) -> bool {
         uint32_t big = 0;
         std::vector<double> singles;
-        for (uint32_t k = k0; k < np; ++k) {
-            uint32_t j = 0;
-            while (j + 1 < nc && d[(size_t)k * nc + j] > T) ++j;
-            if (d[(size_t)k * nc + j] > T) return false;
-            sz[k] = cands[j];
-            if (cands[j] >= 2)
-                big += cands[j];
+        for (const Unit& u : units) {
+            if (cands[u.j] >= 2)
+                big += cands[u.j];
             else
-                singles.push_back(d[(size_t)k * nc]);
+                singles.push_back(u.cost);
         }
         if (big > (singles.empty() ? G : G * 3 / 4)) return false;
-        const uint32_t slots = G - big;
         if (singles.empty()) return true;
+        const uint32_t slots = G - big;
         if (!slots) return false;
         std::sort(singles.begin(), singles.end(), std::greater<double>());
         std::vector<double> load(std::min<size_t>(slots, singles.size()), 0.0);
@@ -619,25 +747,35 @@ bool mlp_team_sizes(dcx_ctx* c, const dcx_graph* g, int32_t k_first, int32_t kr,
         }
         return true;
     };
-    double lo = 0.0, hi = 0.0;
-    for (uint32_t k = k0; k < np; ++k) hi += d[(size_t)k * nc];  // one workgroup does all
-    std::vector<uint32_t> sz(np, 1);
-    if (!assign(hi, sz)) return false;
-    for (int it = 0; it < 40; ++it) {
+    // one workgroup doing every row is always within this bound
+    double hi = 0.0;
+    for (uint32_t k = 1; k < np; ++k)
+        hi += d(kPassPeel, k, 0) + d(kPassFirst, k, 0) + d(kPassDerived, k, 0);
+    hi = 2.0 * hi + 1.0;
+    std::vector<Unit> units;
+    if (!plan(hi, units)) return false;
+    double lo = 0.0;
+    for (int it = 0; it < 50; ++it) {
         const double mid = 0.5 * (lo + hi);
-        std::vector<uint32_t> t2(np, 1);
-        if (assign(mid, t2))
+        if (plan(mid, units))
             hi = mid;
         else
             lo = mid;
     }
-    assign(hi, sz);
-    want = sz;
+    plan(hi, units);
+    if (!fixed) {
+        peel.assign(peel.size(), 1);
+        std::fill(chain_last.begin(), chain_last.end(), -1);
+    }
+    want.assign(np, 1);
     wts.assign(np, 0.0);
-    for (uint32_t k = k0; k < np; ++k) {
-        uint32_t j = 0;
-        while (cands[j] != sz[k]) ++j;
-        wts[k] = d[(size_t)k * nc + j] * sz[k];  // longest (in workgroup time) placed first
+    for (const Unit& u : units) {
+        if (!fixed) {
+            for (uint32_t k = u.a + 1; k <= u.b; ++k) peel[k] = 0;
+            chain_last[u.a] = u.b > u.a ? (int32_t)u.b : -1;
+        }
+        want[u.a] = cands[u.j];
+        wts[u.a] = u.cost * cands[u.j];
     }
     return true;
 }
@@ -721,7 +859,7 @@ void compute_rows(dcx_ctx* c, dcx_graph* g, int variant, int32_t kr = -1) {
     g->rows_cleared = 0;
     int launches = 0, slots_used = 1;
     int32_t split = kr + 1;  // batched: first LDS-resident row
-    bool teams_ran = false, lds_ran = false, first_ran = false, chains_ran = false;
+    bool teams_ran = false, lds_ran = false, first_ran = false;
     std::vector<std::pair<uint32_t, int32_t>> last_teams;  // (size, first k) for diagnosis
     if (few) {
         g->rows_first_grid = kr + 1;  // no row is a team pass
@@ -821,42 +959,90 @@ void compute_rows(dcx_ctx* c, dcx_graph* g, int variant, int32_t kr = -1) {
         const bool have_row0 = g->row0_ready;
         const int32_t k_first = ksplit >= 2 || have_row0 ? 1 : 0;
         g->rows_first_grid = k_first;
-        // Chains: of the team rows, only every spacing-th (counted from row
-        // 0, which is ready before the team launch) is peeled; the rows
-        // between are derived from the row above after the team launch.
+        // Peeled and derived team rows: a row is either peeled (an anchor)
+        // or derived from the row above; the rows a+1..b derived from
+        // anchor a run on the team that peeled a, right after it (row 0,
+        // ready before the launch, anchors a chain of its own).  The plan
+        // comes from the cost model when one is attached and
+        // cfg.chain_rows is 0, else every spacing-th row from row 0 is an
+        // anchor (the fixed rule); the model then still picks the widths.
         const int32_t spacing =
             c->cfg.chain_rows == 0 ? kChainRows : std::max<int32_t>(c->cfg.chain_rows, 1);
         std::vector<char> peel(kr + 1, 1);
         std::vector<int32_t> chain_last(kr + 1, -1);
-        uint32_t npeel = 0;
-        if (spacing > 1 && k_first == 1) {
+        const bool chains_ok = k_first == 1 && ksplit >= 2;
+        if (chains_ok && spacing > 1) {
             for (int32_t a = 0; a < ksplit; a += spacing) {
                 const int32_t b = std::min(a + spacing - 1, ksplit - 1);
                 chain_last[a] = b > a ? b : -1;
                 for (int32_t k = a + 1; k <= b; ++k) peel[k] = 0;
             }
         }
-        for (int32_t k = k_first; k < ksplit; ++k) npeel += peel[k] ? 1u : 0u;
+        std::vector<uint32_t> want;
+        std::vector<double> wts;
+        bool by_model = false;
+        if (c->sched_mlp && chains_ok) {
+            const bool fixed = c->cfg.chain_rows != 0;
+            std::vector<char> p2 = peel;
+            std::vector<int32_t> c2 = chain_last;
+            // without a configured cap the model may give a unit a whole
+            // XCD (the fixed rule stops at kTeamMax)
+            const uint32_t mcap = c->cfg.team_max > 0 ? tmax : G / kXcds;
+            by_model = mlp_rows_plan(c, g, ksplit - 1, G, mcap, fixed, p2, c2, want, wts);
+            if (by_model) {
+                peel = p2;
+                chain_last = c2;
+            }
+        }
+        // units: anchors with their chains, scheduled by unit (team) weight
+        std::vector<char> unit(kr + 1, 0);
+        std::vector<double> uw(kr + 1, 0.0);
+        uint32_t nunits = 0;
+        for (int32_t a = 0; a < ksplit; ++a) {
+            const bool peeled = a >= k_first && peel[a];
+            if (!peeled && chain_last[a] < 0) continue;
+            unit[a] = 1;
+            ++nunits;
+            if (peeled) uw[a] += 2.0 * (double)g->h_ek[a] + (double)g->h_vk[a] + 64.0;
+            if (chain_last[a] >= 0)
+                uw[a] += 2.0 * (double)g->h_ek[a + 1] + (double)g->h_vk[a + 1] + 64.0;
+        }
         Sched s;
         uint32_t busy = 0;
-        if (npeel) {
-            std::vector<uint32_t> want;
-            std::vector<double> wts;
-            const bool by_model = c->sched_mlp && mlp_team_sizes(c, g, k_first, ksplit - 1, G,
-                                                                 tmax, want, wts);
-            s = by_model ? make_schedule(g, k_first, ksplit - 1, G, tmax, &want, &wts, &peel)
-                         : make_schedule(g, k_first, ksplit - 1, G, tmax, nullptr, nullptr, &peel);
+        std::vector<int32_t> pkb;  // per pass of s.ks: -1 peel, else last derived row
+        if (nunits) {
+            s = make_schedule(g, 0, ksplit - 1, G,
+                              by_model && c->cfg.team_max <= 0 ? G / kXcds : tmax,
+                              by_model ? &want : nullptr, by_model ? &wts : &uw, &unit);
             c->timing.rows_scheduler = by_model ? 1 : 0;
             busy = (uint32_t)std::count_if(s.block_map.begin(), s.block_map.begin() + G,
                                            [](uint32_t t) { return t != ~0u; });
-            for (const TeamDesc& td : s.teams)
-                last_teams.emplace_back(td.size,
-                                        td.pass_begin < td.pass_end ? s.ks[td.pass_begin] : -1);
-            for (const TeamDesc& td : s.teams)
+            // expand every unit into its passes: the anchor's peel, then its chain
+            std::vector<int32_t> pks;
+            for (TeamDesc& td : s.teams) {
+                const uint32_t b0 = (uint32_t)pks.size();
                 for (uint32_t i = td.pass_begin; i < td.pass_end; ++i) {
-                    g->h_pass_variant[s.ks[i] + 1] = DCX_VARIANT_BATCHED;
-                    g->h_pass_team[s.ks[i] + 1] = (uint16_t)td.size;
+                    const int32_t a = s.ks[i];
+                    if (a >= k_first && peel[a]) {
+                        pks.push_back(a);
+                        pkb.push_back(-1);
+                        g->h_pass_variant[a + 1] = DCX_VARIANT_BATCHED;
+                        g->h_pass_team[a + 1] = (uint16_t)td.size;
+                    }
+                    if (chain_last[a] >= 0) {
+                        pks.push_back(a);
+                        pkb.push_back(chain_last[a]);
+                        for (int32_t k = a + 1; k <= chain_last[a]; ++k) {
+                            g->h_pass_variant[k + 1] = DCX_VARIANT_DERIVED;
+                            g->h_pass_team[k + 1] = (uint16_t)td.size;
+                        }
+                    }
                 }
+                td.pass_begin = b0;
+                td.pass_end = (uint32_t)pks.size();
+                last_teams.emplace_back(td.size, td.pass_begin < td.pass_end ? pks[b0] : -1);
+            }
+            s.ks = pks;
         }
         std::vector<int32_t> lks;
         for (int32_t k = ksplit; k <= kr; ++k) {
@@ -870,19 +1056,21 @@ void compute_rows(dcx_ctx* c, dcx_graph* g, int variant, int32_t kr = -1) {
             lks.empty() ? 0u : (std::max<uint32_t>(g->h_vk[ksplit], 1) + 31) / 32 * 32;
         const uint32_t lblocks = (uint32_t)std::min<size_t>(lks.size(), (size_t)c->num_cus);
         // device copies: team table, block map, team passes, LDS passes
-        const size_t nt = s.teams.size(), nb = npeel ? G : 0, nk = s.ks.size();
-        const size_t bytes = nt * sizeof(TeamDesc) + 8 * nb + 4 * nk + 4 * lks.size() + 64;
+        const size_t nt = s.teams.size(), nb = nunits ? G : 0, nk = s.ks.size();
+        const size_t bytes = nt * sizeof(TeamDesc) + 8 * nb + 8 * nk + 4 * lks.size() + 64;
         ensure_bytes(c, &c->sched, &c->sched_bytes, bytes);
         char* base = static_cast<char*>(c->sched);
         TeamDesc* d_teams = reinterpret_cast<TeamDesc*>(base);
         uint32_t* d_bt = reinterpret_cast<uint32_t*>(base + nt * sizeof(TeamDesc));
         int32_t* d_ks = reinterpret_cast<int32_t*>(d_bt + 2 * nb);
-        int32_t* d_lks = d_ks + nk;
+        int32_t* d_kb = d_ks + nk;
+        int32_t* d_lks = d_kb + nk;
         std::vector<char> host(bytes);
         std::memcpy(host.data(), s.teams.data(), nt * sizeof(TeamDesc));
         std::memcpy(host.data() + nt * sizeof(TeamDesc), s.block_map.data(), 8 * nb);
         std::memcpy(host.data() + nt * sizeof(TeamDesc) + 8 * nb, s.ks.data(), 4 * nk);
-        std::memcpy(host.data() + nt * sizeof(TeamDesc) + 8 * nb + 4 * nk, lks.data(),
+        std::memcpy(host.data() + nt * sizeof(TeamDesc) + 8 * nb + 4 * nk, pkb.data(), 4 * nk);
+        std::memcpy(host.data() + nt * sizeof(TeamDesc) + 8 * nb + 8 * nk, lks.data(),
                     4 * lks.size());
         HIPX(c, hipMemcpyAsync(c->sched, host.data(), bytes, hipMemcpyHostToDevice, c->stream));
         const size_t pool_words = nt * ((size_t)s.sync_stride + kSlotWords);
@@ -936,74 +1124,28 @@ void compute_rows(dcx_ctx* c, dcx_graph* g, int variant, int32_t kr = -1) {
             HIPX(c, hipStreamWaitEvent(c->side, c->fork, 0));
         }
         if (nt) {
+            // row 0's chain is set up on the whole grid first (its team
+            // would spend the time of several derived rows on it)
+            uint32_t chain0 = 0;
+            for (const TeamDesc& td : s.teams)
+                for (uint32_t i = td.pass_begin; i < td.pass_end && !chain0; ++i)
+                    if (s.ks[i] == 0 && pkb[i] >= 0) {
+                        HIPX(c, launch_chain_preset(
+                                    basep, g->vk, g->rows,
+                                    static_cast<uint32_t*>(c->scratch) + td.scratch_off,
+                                    td.scratch_cap, c->pools + nt * s.sync_stride + td.slot_off,
+                                    V, c->stream));
+                        chain0 = 1;
+                    }
             HIPX(c, launch_rows_teams(basep, g->vk, g->rows, g->lmax_rows, d_teams, d_bt, d_ks,
-                                      (uint32_t)nb, c->scratch, c->pools,
+                                      d_kb, (uint32_t)nb, c->scratch, c->pools,
                                       c->pools + nt * s.sync_stride, c->timeout_ticks,
-                                      c->ctrs + kCtrErr, shrink, c->stream));
+                                      c->ctrs + kCtrErr, shrink, chain0, c->stream));
             ++launches;
             team_err = c->pools + kSyncError;
             nteam_err = (uint32_t)nt;
             team_err_stride = s.sync_stride;
             HIPX(c, hipEventRecord(c->sev[2], c->stream));
-        }
-        if (std::any_of(chain_last.begin(), chain_last.end(), [](int32_t b) { return b >= 0; })) {
-            // one team per chain (short chains share a workgroup), sized by
-            // the work of its first derived row; launched behind the peeled
-            // rows on the same stream, beside the LDS-resident rows
-            const uint32_t Gc = (uint32_t)std::min<int64_t>(
-                G, (int64_t)c->num_cus * std::max(1, rows_chains_blocks_per_cu()));
-            std::vector<char> take(kr + 1, 0);
-            std::vector<double> cw(kr + 1, 0.0);
-            for (int32_t a = 0; a <= kr; ++a)
-                if (chain_last[a] >= 0) {
-                    take[a] = 1;
-                    cw[a] = 2.0 * (double)g->h_ek[a + 1] + (double)g->h_vk[a + 1] + 64.0;
-                }
-            const Sched cs = make_schedule(g, 0, ksplit - 1, Gc, tmax, nullptr, &cw, &take);
-            std::vector<int32_t> cks;
-            for (int32_t a : cs.ks) {
-                cks.push_back(a);
-                cks.push_back(chain_last[a]);
-            }
-            const size_t cnt = cs.teams.size();
-            const size_t cbytes = cnt * sizeof(TeamDesc) + 8ull * Gc + 4 * cks.size() + 64;
-            ensure_bytes(c, &c->chain_sched, &c->chain_sched_bytes, cbytes);
-            std::vector<char> hb(cbytes);
-            std::memcpy(hb.data(), cs.teams.data(), cnt * sizeof(TeamDesc));
-            std::memcpy(hb.data() + cnt * sizeof(TeamDesc), cs.block_map.data(), 8ull * Gc);
-            std::memcpy(hb.data() + cnt * sizeof(TeamDesc) + 8ull * Gc, cks.data(),
-                        4 * cks.size());
-            HIPX(c, hipMemcpyAsync(c->chain_sched, hb.data(), cbytes, hipMemcpyHostToDevice,
-                                   c->stream));
-            char* cb = static_cast<char*>(c->chain_sched);
-            const size_t cpool = cnt * ((size_t)cs.sync_stride + kSlotWords);
-            if (c->chain_pools_words < cpool) {
-                dev_free(c->chain_pools);
-                c->chain_pools_words = 0;
-                dev_alloc(c, &c->chain_pools, cpool);
-                c->chain_pools_words = cpool;
-            }
-            HIPX(c, hipMemsetAsync(c->chain_pools, 0, 4 * cpool, c->stream));
-            ensure_bytes(c, &c->chain_scratch, &c->chain_scratch_bytes,
-                         4 * std::max<uint64_t>(cs.scratch_words, 1));
-            HIPX(c, hipEventRecord(c->cev[0], c->stream));
-            HIPX(c, launch_rows_chains(
-                        V, basep.K, basep.C, g->order, g->vk, g->rows, g->lmax_rows, g->pstats,
-                        reinterpret_cast<const TeamDesc*>(cb),
-                        reinterpret_cast<const uint32_t*>(cb + cnt * sizeof(TeamDesc)),
-                        reinterpret_cast<const int32_t*>(cb + cnt * sizeof(TeamDesc) + 8ull * Gc),
-                        Gc, c->chain_scratch, c->chain_pools,
-                        c->chain_pools + cnt * cs.sync_stride, c->timeout_ticks,
-                        c->ctrs + kCtrErr, c->stream));
-            HIPX(c, hipEventRecord(c->cev[1], c->stream));
-            ++launches;
-            chains_ran = true;
-            for (const TeamDesc& td : cs.teams)
-                for (uint32_t i = td.pass_begin; i < td.pass_end; ++i)
-                    for (int32_t k = cs.ks[i] + 1; k <= chain_last[cs.ks[i]]; ++k) {
-                        g->h_pass_variant[k + 1] = DCX_VARIANT_HINDEX;
-                        g->h_pass_team[k + 1] = (uint16_t)td.size;
-                    }
         }
         if (!lks.empty()) {
             HIPX(c, hipEventRecord(c->sev[0], c->side));
@@ -1054,11 +1196,6 @@ void compute_rows(dcx_ctx* c, dcx_graph* g, int variant, int32_t kr = -1) {
     c->timing.rows_lds_ms = 0.0f;
     c->timing.rows_first_ms = 0.0f;
     c->timing.rows_chains_ms = 0.0f;
-    if (chains_ran) {
-        float ms = 0;
-        HIPX(c, hipEventElapsedTime(&ms, c->cev[0], c->cev[1]));
-        c->timing.rows_chains_ms = ms;
-    }
     if (teams_ran) {
         float ms = 0;
         HIPX(c, hipEventElapsedTime(&ms, c->sev[3], c->sev[2]));
@@ -1081,6 +1218,7 @@ void compute_rows(dcx_ctx* c, dcx_graph* g, int variant, int32_t kr = -1) {
                       hipMemcpyDeviceToHost));
     g->rows_k = K + 1;
     g->rows_valid = true;
+    if (variant == DCX_VARIANT_BATCHED) record_row_stats(g, kr);
 }
 
 __global__ void k_ragged_gather(const int32_t* rows, const int32_t* kmaxes, const int64_t* row_ptr,
@@ -1195,6 +1333,14 @@ dcx_status dcx_device_count(int32_t* n) {
     return DCX_OK;
 }
 
+dcx_status dcx_ctx_set_rows_rule(dcx_ctx* ctx, int32_t chain_rows, int32_t team_max) {
+    if (!ctx || chain_rows < -1 || team_max < 0 || team_max > 256) return DCX_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx->cfg.chain_rows = chain_rows;
+    ctx->cfg.team_max = team_max;
+    return DCX_OK;
+}
+
 dcx_status dcx_ctx_create(const dcx_config* cfg, dcx_ctx** out) {
     if (!out) return DCX_ERR_INVALID_ARG;
     *out = nullptr;
@@ -1261,8 +1407,6 @@ dcx_status dcx_ctx_create(const dcx_config* cfg, dcx_ctx** out) {
         return DCX_ERR_HIP;
     for (auto& e : c->sev)
         if (hipEventCreate(&e) != hipSuccess) return DCX_ERR_HIP;
-    for (auto& e : c->cev)
-        if (hipEventCreate(&e) != hipSuccess) return DCX_ERR_HIP;
     // the static scratch of dcx_klist_noalloc: its address on this device
     if (noalloc_scratch_address(&c->noalloc_scratch) != hipSuccess || !c->noalloc_scratch)
         return DCX_ERR_HIP;
@@ -1313,11 +1457,6 @@ dcx_status dcx_ctx_destroy(dcx_ctx* c) {
     if (c->join) (void)hipEventDestroy(c->join);
     for (auto& e : c->sev)
         if (e) (void)hipEventDestroy(e);
-    for (auto& e : c->cev)
-        if (e) (void)hipEventDestroy(e);
-    dev_free(c->chain_sched);
-    dev_free(c->chain_pools);
-    dev_free(c->chain_scratch);
     delete c;
     return DCX_OK;
 }

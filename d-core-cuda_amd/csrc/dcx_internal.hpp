@@ -193,13 +193,27 @@ int pass_grid_blocks_per_cu();
 int rows_teams_blocks_per_cu();
 // Rows of one graph (core layout): the launch has `blocks` workgroups;
 // block_team[b] names the team of block b (~0u: the block has no work) and
-// block_team[blocks + b] its rank in the team.  Pass k of team t is
-// team_ks[..]; base describes the graph (k, nmembers, out_row set per pass).
+// block_team[blocks + b] its rank in the team.  Pass p of a team
+// (teams[t].pass_begin <= p < pass_end) peels row team_ks[p] when
+// team_kb[p] < 0, and otherwise derives rows team_ks[p]+1 .. team_kb[p] from
+// row team_ks[p] (row 0, or an anchor an earlier pass of the team peeled);
+// a team's scratch then holds 12 arrays of scratch_cap >= V_{team_ks[p]}
+// words.  base describes the graph (k, nmembers, out_row set per pass;
+// stats: row k's log words at stats + kStatWords * k).
 hipError_t launch_rows_teams(const PassSpec& base, const uint32_t* vk, int32_t* rows,
                              int32_t* lmax_rows, const TeamDesc* teams, const uint32_t* block_team,
-                             const int32_t* team_ks, uint32_t blocks, void* scratch,
-                             uint32_t* sync_pool, uint32_t* slot_pool, uint64_t timeout_ticks,
-                             uint32_t* err, uint32_t shrink_at, hipStream_t st);
+                             const int32_t* team_ks, const int32_t* team_kb, uint32_t blocks,
+                             void* scratch, uint32_t* sync_pool, uint32_t* slot_pool,
+                             uint64_t timeout_ticks, uint32_t* err, uint32_t shrink_at,
+                             uint32_t chain0_preset, hipStream_t st);
+// Sets up row 0's chain (rows 1.. derived from row 0) on the whole grid:
+// values, member prefixes and supports for row 1 in the chain state of its
+// team (team_scratch: 12 arrays of cap >= V words; slots: the team's slot
+// words, zeroed).  Row 0 must be in rows; the team launch then runs that
+// chain with chain0_preset = 1.
+hipError_t launch_chain_preset(const PassSpec& base, const uint32_t* vk, int32_t* rows,
+                               uint32_t* team_scratch, uint64_t cap, uint32_t* slots, uint32_t V,
+                               hipStream_t st);
 // Rows whose core has at most `cap` (<= kLdsMaxMembers) members, one
 // LDS-resident workgroup each, taken longest first from ks[0..nks) through
 // the counter *queue (zero at launch).  Scratch: blocks * kScratchArrays *
@@ -279,18 +293,6 @@ constexpr uint32_t kNoAllocMaxVertices = 1u << 18;
 // Address of the static scratch array on the current device.
 hipError_t noalloc_scratch_address(void** ptr);
 
-// ---- rows derived from the row above (chain_kernels.hip) -----------------
-// Chains of the batched rows: team t derives, for each pair (a, b) of
-// chain_ks[2*pass_begin .. 2*pass_end), rows a+1..b from row a (already in
-// rows).  Scratch per team: 12 arrays of scratch_cap words.  Pool words as
-// for launch_rows_teams (sync words zeroed, slots anything).
-hipError_t launch_rows_chains(uint32_t V, const Csr& K, const Csr& C, const uint32_t* order,
-                              const uint32_t* vk, int32_t* rows, int32_t* lmax_rows,
-                              uint32_t* stats, const TeamDesc* teams, const uint32_t* block_team,
-                              const int32_t* chain_ks, uint32_t blocks, void* scratch,
-                              uint32_t* sync_pool, uint32_t* slot_pool, uint64_t timeout_ticks,
-                              uint32_t* err, hipStream_t st);
-int rows_chains_blocks_per_cu();
 
 // ---- localized maintenance (local_kernels.hip) --------------------------
 enum : int32_t { kLocalKmaxDel = 1, kLocalRowDel = 2, kLocalCoreIns = 3, kLocalRowIns = 4 };

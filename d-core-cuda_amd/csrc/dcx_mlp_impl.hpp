@@ -2,7 +2,9 @@
 // dcx_api.hip).  No reference counterpart: the reference's launch shape and
 // pass order are fixed (dcore, main.cu:197-262).  The model is trained on
 // the pass log the decomposition itself records (dcx_graph_pass_log) and
-// only chooses team widths, so every schedule gives the same bits.
+// chooses, per team row, between peeling it and deriving it from the row
+// above, and the team widths; both kinds of row are exact, so every plan
+// gives the same bits.
 #pragma once
 
 extern "C" {
@@ -147,10 +149,11 @@ dcx_status dcx_set_scheduler(dcx_ctx* ctx, dcx_mlp* m) {
     return DCX_OK;
 }
 
-// Training samples from the resident decomposition's pass log: one per row
-// pass that ran in the batched launch, features of pass_features with the
-// team width it ran on and its own level/phase counts (with_log = 0: those
-// two as unknown, as for a first run), target log2(ms + 0.01).
+// Training samples from the resident decomposition's pass log: one per team
+// row of the batched launch (peeled, or derived from the row above),
+// features of pass_features with the team width and kind it ran as and the
+// work statistics its run logged (with_log = 0: those as unknown), target
+// log2(ms + 0.01).
 dcx_status dcx_graph_pass_features(dcx_ctx* ctx, const dcx_graph* g, int32_t with_log,
                                    uint32_t cap, float* X, float* T, uint32_t* n) {
     dcx_status s = check_graph(ctx, g);
@@ -162,17 +165,16 @@ dcx_status dcx_graph_pass_features(dcx_ctx* ctx, const dcx_graph* g, int32_t wit
             return DCX_OK;
         uint32_t m = 0;
         for (int32_t k = 0; k <= g->kmax; ++k) {
-            // team passes only: the LDS-resident rows (team 0) are not
-            // scheduled by width
-            if (g->h_pass_variant[k + 1] != DCX_VARIANT_BATCHED || g->h_pass_team[k + 1] == 0 ||
-                k < g->rows_first_grid)
-                continue;
+            // team rows only (peeled or derived): the LDS-resident rows
+            // (team 0) are not planned by the model
+            const int kind = last_pass_kind(g, k);
+            if (kind < 0) continue;
             const uint32_t* w = g->h_pstats.data() + (size_t)(k + 1) * kStatWords;
             const uint64_t ticks = (uint64_t)w[kStatTicksLo] | ((uint64_t)w[kStatTicksHi] << 32);
             const double ms = (double)ticks / ctx->wall_khz;
             if (m < cap) {
-                pass_features(g, k, g->h_pass_team[k + 1], with_log ? w[kStatLevels] : 0,
-                              with_log ? w[kStatPhases] : 0, X + (size_t)m * kMlpIn);
+                pass_features(g, k, g->h_pass_team[k + 1], kind, with_log != 0,
+                              X + (size_t)m * kMlpIn);
                 T[m] = (float)std::log2(ms + 0.01);
             }
             ++m;

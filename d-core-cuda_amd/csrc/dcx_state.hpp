@@ -45,7 +45,6 @@ struct dcx_ctx {
     // start and end of the team launch on `stream` (ev[5] opens the rows
     // kernels, row 0's grid pass first when the other rows use its bound)
     hipEvent_t sev[4] = {};
-    hipEvent_t cev[2] = {};  // around the chains launch
     std::mutex mu;
     std::string err;
     dcx_timing timing{};
@@ -69,13 +68,6 @@ struct dcx_ctx {
     size_t sched_bytes = 0;
     uint32_t* pools = nullptr;  // team sync words + team slots
     size_t pools_words = 0;
-    // chains launch: schedule, sync words + slots, scratch
-    void* chain_sched = nullptr;
-    size_t chain_sched_bytes = 0;
-    uint32_t* chain_pools = nullptr;
-    size_t chain_pools_words = 0;
-    void* chain_scratch = nullptr;
-    size_t chain_scratch_bytes = 0;
     // localized maintenance: per-slot worklists, task list, small words
     // ([0] task counter, [1] recorded kmax changes, [2] failure bits,
     // [3] largest kmax, [4] largest row-0 value, [5] edits not found), and
@@ -177,6 +169,12 @@ struct dcx_graph {
     uint32_t* pstats = nullptr;  // [(kmax+2) * kStatWords] device pass log: kmax pass, rows
     size_t pstats_cap = 0;
     std::vector<uint32_t> h_pstats;      // host copy of pstats after the last rows launch
+    // Work statistics of the team rows over the batched launches so far,
+    // value + 1 (0: row k never ran that way): phases of its last peel;
+    // rounds and list entries read of its last derivation from the row
+    // above (a chain's supports set-up excluded).  Cost-model features.
+    std::vector<uint32_t> h_peel_phases, h_der_rounds;
+    std::vector<uint64_t> h_der_reads;
     std::vector<int8_t> h_pass_variant;  // [kmax+2] variant that last ran each pass
     std::vector<uint16_t> h_pass_team;   // [kmax+2] workgroups that ran it
     int32_t* rows = nullptr;     // (kmax+1) x V

@@ -36,6 +36,7 @@
 
 #include <climits>
 
+#include "chain_device.hpp"
 #include "dcx_internal.hpp"
 #include "grid_sync.hpp"
 
@@ -1399,17 +1400,23 @@ __global__ void k_gather_rank(const int32_t* in, const uint32_t* order, uint32_t
 }
 
 // Persistent rows kernel: block b belongs to team block_team[b]; a team
-// runs its passes one after another, re-synchronising between them.
+// runs its passes one after another, re-synchronising between them.  A
+// pass is either a peel of row team_ks[p] (team_kb[p] < 0) or the rows
+// team_ks[p]+1 .. team_kb[p] derived from row team_ks[p] (chain_device.hpp);
+// a team runs all its peels before its chains, so every anchor's values are
+// complete when its chain starts (row 0's before the launch).
 // Two workgroups of kBlk threads must be resident per CU (the launch packs
 // two per CU and a team's members wait for each other): 4 waves per SIMD.
+static_assert(chain::kCBlk == kBlk, "chains run in the rows kernel's workgroups");
 __global__ __launch_bounds__(kBlk, 4) void k_rows_teams(PassSpec base, const uint32_t* vk,
                                                      int32_t* rows, int32_t* lmax_rows,
                                                      const TeamDesc* teams,
                                                      const uint32_t* block_team,
-                                                     const int32_t* team_ks, void* scratch,
+                                                     const int32_t* team_ks,
+                                                     const int32_t* team_kb, void* scratch,
                                                      uint32_t* sync_pool, uint32_t* slot_pool,
                                                      uint64_t timeout_ticks, uint32_t* err,
-                                                     uint32_t shrink_at) {
+                                                     uint32_t shrink_at, uint32_t chain0_preset) {
     __shared__ TileLds<kBlk> T;
     __shared__ uint32_t comb[2 * kCombRanks];
     const uint32_t team = block_team[blockIdx.x];
@@ -1417,16 +1424,21 @@ __global__ __launch_bounds__(kBlk, 4) void k_rows_teams(PassSpec base, const uin
     for (uint32_t j = threadIdx.x; j < 2 * kCombRanks; j += kBlk) comb[j] = 0;
     __syncthreads();
     const TeamDesc td = teams[team];
-    const PassScratch S =
-        scratch_at(static_cast<uint32_t*>(scratch) + td.scratch_off, td.scratch_cap);
+    uint32_t* const team_scratch = static_cast<uint32_t*>(scratch) + td.scratch_off;
+    const PassScratch S = scratch_at(team_scratch, td.scratch_cap);
     GridSync gs;
     gs.w = sync_pool + td.sync_off;
     gs.nblocks = td.size;
     gs.rank = block_team[gridDim.x + blockIdx.x];
     gs.timeout_ticks = timeout_ticks;
     gs.epoch = 0;
+    // the team's peels first, then its chains (every anchor is peeled by
+    // then); two loops keep the two kinds of pass apart in registers
+    uint32_t seq = 0;
     for (uint32_t p = td.pass_begin; p < td.pass_end; ++p) {
+        if (team_kb[p] >= 0) continue;
         const int32_t k = team_ks[p];
+        gs.nblocks = td.size;  // a peel pass may have narrowed the team
         PassSpec P = base;
         P.k = k;
         P.nmembers = vk[k];
@@ -1436,10 +1448,29 @@ __global__ __launch_bounds__(kBlk, 4) void k_rows_teams(PassSpec base, const uin
         // one code path for every team size (a team of one runs the same
         // pass with workgroup barriers), which keeps the kernel within the
         // registers of two resident workgroups per CU
-        gs.nblocks = td.size;  // the pass may have narrowed the team
         run_pass<kBlk, true, kCombByRank>(P, S, slot_pool + td.slot_off, T, lmax_rows + k, &gs,
                                           err, shrink_at, comb);
-        if (!team_resync(gs, p - td.pass_begin + 1)) return;
+        if (!team_resync(gs, ++seq)) return;
+    }
+    gs.nblocks = td.size;
+    uint32_t tag = 0;  // round tags of the team's chains
+    for (uint32_t p = td.pass_begin; p < td.pass_end; ++p) {
+        if (team_kb[p] < 0) continue;
+        chain::ChainArgs A;
+        A.V = base.V;
+        A.K = base.K;
+        A.C = base.C;
+        A.order = base.order;
+        A.vk = vk;
+        A.rows = rows;
+        A.lmax_rows = lmax_rows;
+        A.stats = base.stats;
+        A.err = err;
+        const bool preset = team_ks[p] == 0 && chain0_preset != 0;
+        if (!chain::run_chain(A, chain::chain_state(team_scratch, td.scratch_cap),
+                              slot_pool + td.slot_off, gs, team_ks[p], team_kb[p], preset, tag))
+            return;
+        if (!team_resync(gs, ++seq)) return;
     }
 }
 
@@ -1645,12 +1676,36 @@ hipError_t launch_gather_rank(const int32_t* in, const uint32_t* order, uint32_t
 
 hipError_t launch_rows_teams(const PassSpec& base, const uint32_t* vk, int32_t* rows,
                              int32_t* lmax_rows, const TeamDesc* teams, const uint32_t* block_team,
-                             const int32_t* team_ks, uint32_t blocks, void* scratch,
-                             uint32_t* sync_pool, uint32_t* slot_pool, uint64_t timeout_ticks,
-                             uint32_t* err, uint32_t shrink_at, hipStream_t st) {
+                             const int32_t* team_ks, const int32_t* team_kb, uint32_t blocks,
+                             void* scratch, uint32_t* sync_pool, uint32_t* slot_pool,
+                             uint64_t timeout_ticks, uint32_t* err, uint32_t shrink_at,
+                             uint32_t chain0_preset, hipStream_t st) {
     hipLaunchKernelGGL(k_rows_teams, dim3(blocks), dim3(kBlk), 0, st, base, vk, rows, lmax_rows,
-                       teams, block_team, team_ks, scratch, sync_pool, slot_pool, timeout_ticks,
-                       err, shrink_at);
+                       teams, block_team, team_ks, team_kb, scratch, sync_pool, slot_pool,
+                       timeout_ticks, err, shrink_at, chain0_preset);
+    return hipGetLastError();
+}
+
+hipError_t launch_chain_preset(const PassSpec& base, const uint32_t* vk, int32_t* rows,
+                               uint32_t* team_scratch, uint64_t cap, uint32_t* slots, uint32_t V,
+                               hipStream_t st) {
+    chain::ChainArgs A;
+    A.V = base.V;
+    A.K = base.K;
+    A.C = base.C;
+    A.order = base.order;
+    A.vk = vk;
+    A.rows = rows;
+    A.lmax_rows = nullptr;
+    A.stats = nullptr;
+    A.err = nullptr;
+    const chain::ChainState S = chain::chain_state(team_scratch, cap);
+    const uint32_t vb = std::max(1u, std::min(4096u, (V + 255u) / 256u));
+    hipLaunchKernelGGL(chain::k_chain_preset_values, dim3(vb), dim3(256), 0, st, A, S);
+    hipLaunchKernelGGL(chain::k_chain_preset_supports, dim3(2048), dim3(chain::kCBlk), 0, st, A,
+                       S, slots);
+    hipLaunchKernelGGL(chain::k_chain_preset_long, dim3(1024), dim3(chain::kCBlk), 0, st, A, S,
+                       slots);
     return hipGetLastError();
 }
 

@@ -14,6 +14,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libdcx.so")
 
 VARIANT_AUTO, VARIANT_PEEL, VARIANT_BATCHED, VARIANT_HINDEX = 0, 1, 2, 3
+# pass log only: a batched team row derived from the row above
+VARIANT_DERIVED = 4
 INGEST_ON_DEVICE, INGEST_DROP_SELF_LOOPS, INGEST_DEDUP = 1, 2, 4
 STATUS = {
     0: "DCX_OK", 1: "DCX_ERR_INVALID_ARG", 2: "DCX_ERR_NO_DEVICE", 3: "DCX_ERR_HIP",
@@ -334,10 +336,18 @@ class Context:
         return t
 
     def set_scheduler(self, mlp):
-        """Batched rows take team widths from the cost model (None: the
-        work-share rule)."""
+        """Batched team rows are planned by the cost model -- peeled or
+        derived from the row above, and team widths (None: the fixed rule)."""
         check(lib().dcx_set_scheduler(self.h, mlp.h if mlp is not None else None),
               "dcx_set_scheduler", self.h)
+
+    def set_rows_rule(self, chain_rows=0, team_max=0):
+        """cfg.chain_rows and cfg.team_max for the next decompositions
+        (chain_rows 0: the default rule, or the cost model's plan when one
+        is attached; 1: every team row peeled; n > 1: every n-th row from
+        row 0 peeled; team_max 0: the default widest team)."""
+        check(lib().dcx_ctx_set_rows_rule(self.h, i32(chain_rows), i32(team_max)),
+              "dcx_ctx_set_rows_rule", self.h)
 
     def last_trace(self):
         """Phase trace of the last grid peel pass (needs trace_phases=1):

@@ -134,7 +134,10 @@ typedef enum dcx_variant {
     DCX_VARIANT_AUTO = 0,    /* library choice (batched rows) */
     DCX_VARIANT_PEEL = 1,    /* plain: one k per launch, whole grid per pass */
     DCX_VARIANT_BATCHED = 2, /* all k-passes in one persistent launch */
-    DCX_VARIANT_HINDEX = 3   /* h-index fixpoint per k */
+    DCX_VARIANT_HINDEX = 3,  /* h-index fixpoint per k */
+    /* pass log only: a batched row derived from the row above by h-index
+     * refinement with per-vertex supports (not a dcx_decompose argument) */
+    DCX_VARIANT_DERIVED = 4
 } dcx_variant;
 
 typedef struct dcx_config {
@@ -196,12 +199,13 @@ typedef struct dcx_timing {
     double total_ms;       /* kmax_ms + order_ms + rows_ms */
     int32_t rows_launches; /* kernel launches spent on the rows */
     int32_t rows_slots;    /* workgroups busy in the batched launch */
-    int32_t rows_scheduler; /* batched team widths: 0 work-share rule, 1 cost model */
+    int32_t rows_scheduler; /* batched plan: 0 fixed rule, 1 cost model */
     int32_t rows_split;    /* batched: rows k >= rows_split ran LDS-resident */
     float rows_teams_ms;   /* batched: the team launch alone */
     float rows_lds_ms;     /* batched: the LDS-resident launch alone (beside it) */
     float rows_first_ms;   /* batched: row 0's grid pass, run first to bound the others */
-    float rows_chains_ms;  /* batched: the launch deriving rows from the row above */
+    float rows_chains_ms;  /* 0: rows derived from the row above run inside the
+                              team launch, after their anchor's peel */
 } dcx_timing;
 
 dcx_status dcx_config_init(dcx_config* cfg);
@@ -213,6 +217,11 @@ dcx_status dcx_ctx_create(const dcx_config* cfg, dcx_ctx** out);
 dcx_status dcx_ctx_destroy(dcx_ctx* ctx);
 /* Detail of the last failing call on this context ("" if none). */
 const char* dcx_ctx_last_error(const dcx_ctx* ctx);
+/* Sets cfg.chain_rows and cfg.team_max of a context (which batched team
+ * rows are peeled and which derived from the row above, and the widest
+ * team) for its next decompositions; results are identical.  No reference
+ * counterpart. */
+dcx_status dcx_ctx_set_rows_rule(dcx_ctx* ctx, int32_t chain_rows, int32_t team_max);
 dcx_status dcx_last_timing(const dcx_ctx* ctx, dcx_timing* t);
 
 /* Uploads a CSR-with-slack graph (layout of dcx_csr_build).  Replaces
@@ -275,8 +284,11 @@ typedef struct dcx_pass_log {
     int32_t k;         /* -1 for the kmax pass */
     int32_t variant;   /* dcx_variant that ran the pass (0 = not run) */
     uint32_t members;  /* vertices of the pass (V, or V_k of the (k,0)-core) */
-    uint32_t phases;   /* synchronised phases (peel) or sweeps (h-index) */
-    uint32_t levels;   /* level iterations (peel), 0 for h-index */
+    uint32_t phases;   /* synchronised phases (peel), sweeps (h-index) or
+                          refinement rounds (derived) */
+    uint32_t levels;   /* level iterations (peel); 0 for h-index; derived:
+                          1 for the first row of a chain (its time includes
+                          the supports' set-up from the anchor row), else 0 */
     uint32_t team;     /* workgroups that ran it */
     uint64_t edges;    /* edges of the pass (E, or E_k) */
     double ms;         /* device wall-clock time of the pass */
@@ -422,15 +434,20 @@ dcx_status dcx_state_hash(dcx_ctx* ctx, dcx_graph* g, uint64_t* hash);
 
 /* ---- learned variant scheduler ---------------------------------------
  * A small fp32 MLP (8 inputs, two hidden layers of 32 ReLU units, one
- * output) predicts log2(ms) of a row pass from its features (sizes of the
- * (k,0)-core, the vertices whose kmax is k, the level/phase counts of the
- * row's last run, and the team width it would run on); forward, backward
- * and Adam run in hand-written HIP kernels.  Attached to a context, it sets
- * each pass's team width in the batched launch (the narrowest width whose
- * predicted time fits the smallest feasible makespan); without it the width
- * follows the pass's share of the work.  Every width gives the same bits.
- * No reference counterpart (the reference's schedule is fixed,
- * main.cu:197-262). */
+ * output) predicts log2(ms) of a team row of the batched launch from its
+ * features: sizes and density of the (k,0)-core, k's place among the rows,
+ * the vertices that left the core since row k-1, the kind of the row
+ * (peeled; derived from the row above as the first row of a chain, whose
+ * time includes the supports' set-up; derived later in a chain), the
+ * phase or round count of its last run of that kind, and the team width
+ * it would run on.  Forward, backward and Adam run in hand-written HIP
+ * kernels.  Attached to a context (and cfg.chain_rows = 0), it chooses per
+ * team row between the two exact variants -- peel it, or derive it from
+ * the row above by h-index refinement -- and the team widths: the plan
+ * whose predicted makespan is smallest.  Without it every chain_rows-th row
+ * from row 0 is peeled and the widths follow each unit's share of the
+ * work.  Every plan gives the same bits.  No reference counterpart (the
+ * reference's schedule is fixed, main.cu:197-262). */
 typedef struct dcx_mlp dcx_mlp;
 uint32_t dcx_mlp_param_count(void);
 /* Parameter layout: W1[32][8], b1[32], W2[32][32], b2[32], W3[32], b3. */
@@ -448,9 +465,10 @@ dcx_status dcx_mlp_train(dcx_mlp* m, const float* X, const float* T, uint32_t n,
                          float lr, float* final_loss);
 /* Attach (or with NULL detach) the model as the batched rows scheduler. */
 dcx_status dcx_set_scheduler(dcx_ctx* ctx, dcx_mlp* m);
-/* Training samples from the resident decomposition's pass log (rows run in
- * the batched launch): features (with_log = 0: level/phase counts set to
- * unknown) and targets log2(ms + 0.01); *n = samples available. */
+/* Training samples from the resident decomposition's pass log (the team
+ * rows of the batched launch, peeled or derived): features (with_log = 0:
+ * phase/round count set to unknown) and targets log2(ms + 0.01); *n =
+ * samples available. */
 dcx_status dcx_graph_pass_features(dcx_ctx* ctx, const dcx_graph* g, int32_t with_log,
                                    uint32_t cap, float* X, float* T, uint32_t* n);
 

@@ -10,7 +10,7 @@ every maintained state are compared with the oracle's recomputation.
   h-index variants (oracle arrays are compared in full in
   test_gpu_decompose.py; here the hash of each variant).
 * R-MAT 24 seed 4 and Chung-Lu 2^24 seed 5 (learned-scheduling workload):
-  work-share and learned team widths; the graph is sampled on the device and
+  the fixed rule and the learned plan; the graph is sampled on the device and
   deduplicated there (the same graph as the host generator's).
 * 256 Chung-Lu graphs (batched workload): their disjoint union in one
   launch sequence, and each graph's own kmax and lmax.
@@ -62,12 +62,13 @@ def test_rmat20_variants_equal_oracle_hash():
 
 
 def _learned_pair(c, g, G, name):
-    """Work-share widths, then widths from a cost model trained on the GPU
-    from this graph's own pass log (from an R-MAT 16 graph's, every team row
-    peeled, when this one has no team row to log, as Chung-Lu 2^24 with
-    kmax = 2): both must give the oracle's hash."""
+    """The fixed rule, then the plan (peeled and derived team rows, widths) of
+    a cost model trained on the GPU from this graph's own pass log (from an
+    R-MAT 16 graph's, every team row peeled, when this one has no team row
+    to log, as Chung-Lu 2^24 with kmax = 2): both must give the oracle's
+    hash."""
     c.set_scheduler(None)
-    say(f"{name}: work-share schedule")
+    say(f"{name}: fixed rule")
     K, L = g.decompose(dcx.VARIANT_BATCHED)
     assert (K, L) == (G["kmax"], G["lmax"])
     assert f"{g.state_hash():016x}" == G["hash"]
@@ -85,7 +86,7 @@ def _learned_pair(c, g, G, name):
     m = dcx.MLP(c, seed=3)
     m.train(X, T, epochs=800, lr=3e-3)
     c.set_scheduler(m)
-    say(f"{name}: learned schedule")
+    say(f"{name}: learned plan")
     K, L = g.decompose(dcx.VARIANT_BATCHED)
     used = c.timing().rows_scheduler
     h = g.state_hash()
@@ -97,7 +98,7 @@ def _learned_pair(c, g, G, name):
 
 
 @pytest.mark.timeout(900)
-def test_rmat24_work_share_and_learned_equal_oracle_hash():
+def test_rmat24_fixed_rule_and_learned_equal_oracle_hash():
     G = gold("rmat24_s4")
     with dcx.Context(0) as c:
         say("rmat24 seed 4: device sampling and dedup ingest")
@@ -111,7 +112,7 @@ def test_rmat24_work_share_and_learned_equal_oracle_hash():
 
 
 @pytest.mark.timeout(900)
-def test_chunglu24_work_share_and_learned_equal_oracle_hash():
+def test_chunglu24_fixed_rule_and_learned_equal_oracle_hash():
     G = gold("chunglu24_s5")
     V = 1 << 24
     say("chunglu24 seed 5: host generation")
