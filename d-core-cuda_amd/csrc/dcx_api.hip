@@ -219,6 +219,19 @@ int32_t run_kmax_with_row0(dcx_ctx* c, dcx_graph* g) {
     return c->h_scalars[0];
 }
 
+// The E_k histogram's queue of long-list slices (E / 2048 + 64 entries:
+// a list of n > 4096 entries gives at most n / 2048 slices).
+uint64_t* ensure_ekq(dcx_ctx* c, dcx_graph* g) {
+    const size_t need = (size_t)(g->E / 2048) + 64;
+    if (g->ekq_cap < need) {
+        dev_free(g->ekq);
+        g->ekq_cap = 0;
+        dev_alloc(c, &g->ekq, need);
+        g->ekq_cap = need;
+    }
+    return g->ekq;
+}
+
 void compute_kmax(dcx_ctx* c, dcx_graph* g, int variant = DCX_VARIANT_BATCHED,
                   bool warm = false, bool clear_rows = false) {
     if (!g->kmaxes) dev_alloc(c, &g->kmaxes, g->V);
@@ -286,7 +299,7 @@ void compute_kmax(dcx_ctx* c, dcx_graph* g, int variant = DCX_VARIANT_BATCHED,
         dev_free(g->ekh);
         dev_alloc(c, &g->vk, need);
         dev_alloc(c, &g->hist, need);
-        dev_alloc(c, &g->ekh, need);
+        dev_alloc(c, &g->ekh, need + 1);
         g->vk_cap = need;
     }
     if (!g->order) dev_alloc(c, &g->order, g->V);
@@ -298,7 +311,7 @@ void compute_kmax(dcx_ctx* c, dcx_graph* g, int variant = DCX_VARIANT_BATCHED,
     HIPX(c, hipEventRecord(c->fork, c->stream));
     HIPX(c, hipStreamWaitEvent(c->side, c->fork, 0));
     HIPX(c, launch_core_edges(g->kmaxes, Csr{g->out_off, g->out_deg, g->out_nbr}, g->V, g->kmax,
-                              g->ekh, c->side));
+                              g->ekh, ensure_ekq(c, g), c->side));
     HIPX(c, hipEventRecord(c->join, c->side));
     const bool few_rows = variant == DCX_VARIANT_BATCHED && g->row0_pending && g->kmax >= 1 &&
                           g->kmax <= kFewRows;
@@ -372,13 +385,13 @@ void ensure_order(dcx_ctx* c, dcx_graph* g) {
         dev_free(g->ekh);
         dev_alloc(c, &g->vk, need);
         dev_alloc(c, &g->hist, need);
-        dev_alloc(c, &g->ekh, need);
+        dev_alloc(c, &g->ekh, need + 1);
         g->vk_cap = need;
     }
     if (!g->order) dev_alloc(c, &g->order, g->V);
     HIPX(c, launch_order_by_kmax(g->kmaxes, g->V, g->kmax, g->hist, g->vk, g->order, c->stream));
     HIPX(c, launch_core_edges(g->kmaxes, Csr{g->out_off, g->out_deg, g->out_nbr}, g->V, g->kmax,
-                              g->ekh, c->stream));
+                              g->ekh, ensure_ekq(c, g), c->stream));
     g->h_vk.assign(need, 0);
     std::vector<uint64_t> eh(need, 0);
     HIPX(c, hipMemcpyAsync(g->h_vk.data(), g->vk, need * sizeof(uint32_t), hipMemcpyDeviceToHost,
@@ -1281,6 +1294,7 @@ void free_graph_arrays(dcx_graph* g) {
     dev_free(g->vk);
     dev_free(g->hist);
     dev_free(g->ekh);
+    dev_free(g->ekq);
     dev_free(g->rank);
     dev_free(g->cout_off);
     dev_free(g->cin_off);

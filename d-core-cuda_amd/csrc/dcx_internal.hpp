@@ -228,9 +228,11 @@ uint32_t rows_lds_static_bytes();
 hipError_t launch_order_by_kmax(const int32_t* kmaxes, uint32_t V, int32_t kmax,
                                 uint32_t* hist /*[kmax+2]*/, uint32_t* vk /*[kmax+2]*/,
                                 uint32_t* order, hipStream_t st);
-// ek[k] = edges with both endpoints of kmax >= k, k = 0..kmax (hist [kmax+2]).
+// hist[m] = edges whose endpoints' smaller kmax is m, m = 0..kmax (hist has
+// kmax + 3 words: the last holds the long-list queue's length); longq: room
+// for E / 2048 + 64 slices of long lists (E = the graph's edges).
 hipError_t launch_core_edges(const int32_t* kmaxes, Csr out, uint32_t V, int32_t kmax,
-                             uint64_t* hist, hipStream_t st);
+                             uint64_t* hist, uint64_t* longq, hipStream_t st);
 
 // ---- h-index fixpoint (hindex_kernels.hip) -----------------------------
 // One row in the core layout: initialises lv from prev (the previous row in

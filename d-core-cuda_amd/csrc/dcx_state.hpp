@@ -141,7 +141,9 @@ struct dcx_graph {
     uint32_t* order = nullptr;   // [V] vertices by kmax, descending (rank -> id)
     uint32_t* vk = nullptr;      // [kmax+2] members per k
     uint32_t* hist = nullptr;    // [kmax+2]
-    uint64_t* ekh = nullptr;     // [kmax+2] edge histogram by min kmax
+    uint64_t* ekh = nullptr;     // [kmax+3] edge histogram by min kmax, queue length
+    uint64_t* ekq = nullptr;     // its queue of long-list slices
+    size_t ekq_cap = 0;
     size_t vk_cap = 0;
     std::vector<uint32_t> h_vk;  // host copy of vk
     std::vector<uint64_t> h_ek;  // E_k, k = 0..kmax

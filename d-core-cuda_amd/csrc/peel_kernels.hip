@@ -862,8 +862,11 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
                 const uint32_t ipw =
                     solo ? (uint32_t)BLK : min((uint32_t)BLK, (nF + tn - 1) / tn);
                 // one tile: up to ipw frontier vertices (`has`, v per
-                // thread, `items` of them), their lists edge-parallel
-                auto push_tile = [&](bool has, uint32_t v, uint32_t items) {
+                // thread, `items` of them), their lists edge-parallel; of
+                // the tile's edges this workgroup takes share `part` of
+                // `parts` (a tile of long lists is split over the team)
+                auto push_tile = [&](bool has, uint32_t v, uint32_t items, uint32_t part,
+                                     uint32_t parts) {
                     __syncthreads();
                     uint32_t kl = 0, cl = 0;
                     if (has) {
@@ -877,15 +880,17 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
                     // every thread's prefix is in LDS before any thread searches it
                     __syncthreads();
                     const uint32_t total = T.total;
+                    const uint32_t elo = (uint32_t)((uint64_t)total * part / parts);
+                    const uint32_t ehi = (uint32_t)((uint64_t)total * (part + 1) / parts);
                     // kUnroll independent edges per thread per step, so their
                     // loads and atomics overlap instead of forming one chain.
-                    for (uint32_t eb = 0; eb < total; eb += kUnroll * BLK) {
+                    for (uint32_t eb = elo; eb < ehi; eb += kUnroll * BLK) {
                         uint32_t x[kUnroll];
                         bool ks[kUnroll], ok[kUnroll];
 #pragma unroll
                         for (int u = 0; u < kUnroll; ++u) {
                             const uint32_t e = eb + u * BLK + tid;
-                            ok[u] = e < total;
+                            ok[u] = e < ehi;
                             x[u] = 0;
                             ks[u] = false;
                             if (ok[u]) {
@@ -902,30 +907,16 @@ __device__ void run_pass(const PassSpec& P, const PassScratch& S, uint32_t* slot
                 for (uint32_t base = tw * ipw; base < nF; base += tn * ipw) {
                     const uint32_t i = base + tid;
                     const bool has = tid < ipw && i < nF;
-                    push_tile(has, has ? ld<MULTI>(&F[i]) : 0u, min(nF - base, ipw));
+                    push_tile(has, has ? ld<MULTI>(&F[i]) : 0u, min(nF - base, ipw), 0u, 1u);
                 }
-                // long lists: every member walks its slice of each of them
-                for (uint32_t li = 0; li < nL; ++li) {
-                    const uint32_t v = ld<MULTI>(&FL[li]);
-                    uint32_t kl, cl;
-                    list_lens(P, S, v, kl, cl);
-                    const uint32_t tot = kl + cl;
-                    const uint32_t lo = (uint32_t)((uint64_t)tot * tw / tn);
-                    const uint32_t hi = (uint32_t)((uint64_t)tot * (tw + 1) / tn);
-                    const uint32_t* kn = P.K.nbr + P.K.off[v];
-                    const uint32_t* cn = P.C.nbr + P.C.off[v];
-                    for (uint32_t eb = lo; eb < hi; eb += kUnroll * BLK) {
-                        uint32_t x[kUnroll];
-                        bool ks[kUnroll], ok[kUnroll];
-#pragma unroll
-                        for (int u = 0; u < kUnroll; ++u) {
-                            const uint32_t e = eb + u * BLK + tid;
-                            ok[u] = e < hi;
-                            ks[u] = e < kl;
-                            x[u] = ok[u] ? (ks[u] ? kn[e] : cn[e - kl]) : 0u;
-                        }
-                        visit(x, ks, ok);
-                    }
+                // long lists, BLK at a time as one tile whose edges the
+                // members split evenly: the team walks many long lists at
+                // once instead of one list after another
+                for (uint32_t lb = 0; lb < nL; lb += BLK) {
+                    const uint32_t i = lb + tid;
+                    const bool has = i < nL;
+                    push_tile(has, has ? ld<MULTI>(&FL[i]) : 0u, min(nL - lb, (uint32_t)BLK), tw,
+                              tn);
                 }
             }
             if (COMB != kCombNone && !pull) {
@@ -1541,10 +1532,27 @@ __global__ void k_kmax_scatter(const int32_t* kmaxes, uint32_t V, int32_t kmax, 
     }
 }
 
-// hist[m] += edges whose endpoints' smaller kmax is m (one wave per vertex).
+// hist[m] += edges whose endpoints' smaller kmax is m.  One wave per
+// vertex; a list of more than kCoreEdgesLong entries is cut into slices of
+// that many, queued (v << 32 | first entry) for k_core_edges_long, where a
+// workgroup takes each slice.  Bins: the wave's own kmax (most of its list)
+// and bins below kCoreEdgesBallot are counted by ballot in registers, the
+// rest by 64-bit LDS atomics (per workgroup, added to hist at the end).
 constexpr int kCoreEdgesUnroll = 4;
+constexpr uint32_t kCoreEdgesLong = 4096;
+constexpr int kCoreEdgesBallot = 8;
+
+__device__ __forceinline__ void core_edges_add(unsigned long long* h, unsigned long long* hist,
+                                               bool lds, int32_t m, unsigned long long n) {
+    if (lds)
+        atomicAdd(&h[m], n);
+    else
+        atomicAdd(&hist[m], n);
+}
+
 __global__ void k_core_edges(const int32_t* kmaxes, Csr out, uint32_t V, int32_t kmax,
-                             unsigned long long* hist) {
+                             unsigned long long* hist, unsigned long long* longq,
+                             uint32_t* longn) {
     __shared__ unsigned long long h[kHistLds];
     const bool lds = kmax + 1 <= kHistLds;
     if (lds)
@@ -1552,11 +1560,24 @@ __global__ void k_core_edges(const int32_t* kmaxes, Csr out, uint32_t V, int32_t
     __syncthreads();
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / W;
     const uint32_t nw = gridDim.x * blockDim.x / W;
+    uint32_t low[kCoreEdgesBallot];  // wave-uniform counts of bins 0..7
+#pragma unroll
+    for (int b = 0; b < kCoreEdgesBallot; ++b) low[b] = 0;
     for (uint32_t v = wave; v < V; v += nw) {
         const int32_t kv = kmaxes[v];
         const uint32_t o = out.off[v], n = (uint32_t)out.deg[v];
-        // kCoreEdgesUnroll slices of a long list per round: their loads are
-        // independent, so a hub's list is a chain of n / 256 round trips
+        if (n > kCoreEdgesLong) {
+            if (lane_id() == 0) {
+                const uint32_t ns = (n + kCoreEdgesLong - 1) / kCoreEdgesLong;
+                const uint32_t at = atomicAdd(longn, ns);
+                for (uint32_t i = 0; i < ns; ++i)
+                    longq[at + i] = ((unsigned long long)v << 32) | (i * kCoreEdgesLong);
+            }
+            continue;
+        }
+        unsigned long long same_n = 0;
+        // kCoreEdgesUnroll slices of the list per round: their loads are
+        // independent
         for (uint32_t jb = 0; jb < n; jb += kCoreEdgesUnroll * W) {
             uint32_t u[kCoreEdgesUnroll];
 #pragma unroll
@@ -1571,22 +1592,53 @@ __global__ void k_core_edges(const int32_t* kmaxes, Csr out, uint32_t V, int32_t
 #pragma unroll
             for (int r = 0; r < kCoreEdgesUnroll; ++r) {
                 // most of a list's edges end at a vertex of kmax >= kv, i.e.
-                // in the same bin: one add per wavefront for those
-                const unsigned long long same = __ballot(m[r] == kv);
-                if (same && lane_id() == __ffsll(same) - 1) {
-                    if (lds)
-                        atomicAdd(&h[kv], (unsigned long long)__popcll(same));
-                    else
-                        atomicAdd(&hist[kv], (unsigned long long)__popcll(same));
-                }
-                if (m[r] >= 0 && m[r] != kv) {
-                    if (lds)
-                        atomicAdd(&h[m[r]], 1ull);
-                    else
-                        atomicAdd(&hist[m[r]], 1ull);
-                }
+                // in the wave's own bin; low bins by ballot; the rest by atomics
+                same_n += __popcll(__ballot(m[r] == kv));
+#pragma unroll
+                for (int b = 0; b < kCoreEdgesBallot; ++b)
+                    low[b] += (uint32_t)__popcll(__ballot(m[r] == b && b != kv));
+                if (m[r] >= kCoreEdgesBallot && m[r] != kv) core_edges_add(h, hist, lds, m[r], 1ull);
             }
         }
+        if (lane_id() == 0 && same_n) core_edges_add(h, hist, lds, kv, same_n);
+    }
+    if (lane_id() == 0)
+        for (int b = 0; b < kCoreEdgesBallot && b <= kmax; ++b)
+            if (low[b]) core_edges_add(h, hist, lds, b, low[b]);
+    __syncthreads();
+    if (lds)
+        for (int i = threadIdx.x; i <= kmax; i += blockDim.x)
+            if (h[i]) atomicAdd(&hist[i], h[i]);
+}
+
+// The queued slices of long lists, one workgroup per slice.
+__global__ __launch_bounds__(1024) void k_core_edges_long(const int32_t* kmaxes, Csr out,
+                                                          int32_t kmax, unsigned long long* hist,
+                                                          const unsigned long long* longq,
+                                                          const uint32_t* longn) {
+    __shared__ unsigned long long h[kHistLds];
+    const bool lds = kmax + 1 <= kHistLds;
+    if (lds)
+        for (int i = threadIdx.x; i <= kmax; i += blockDim.x) h[i] = 0;
+    __syncthreads();
+    const uint32_t nq = *longn;
+    for (uint32_t q = blockIdx.x; q < nq; q += gridDim.x) {
+        const unsigned long long e = longq[q];
+        const uint32_t v = (uint32_t)(e >> 32), j0 = (uint32_t)e;
+        const int32_t kv = kmaxes[v];
+        const uint32_t o = out.off[v], n = (uint32_t)out.deg[v];
+        const uint32_t j1 = min(n, j0 + kCoreEdgesLong);
+        unsigned long long same_n = 0;
+        for (uint32_t j = j0 + threadIdx.x; j < j1; j += blockDim.x) {
+            const int32_t m = min(kv, kmaxes[out.nbr[o + j]]);
+            if (m == kv)
+                ++same_n;
+            else
+                core_edges_add(h, hist, lds, m, 1ull);
+        }
+        // per-thread own-bin counts summed per wave
+        for (int d = W / 2; d > 0; d >>= 1) same_n += __shfl_xor(same_n, d, W);
+        if (lane_id() == 0 && same_n) core_edges_add(h, hist, lds, kv, same_n);
     }
     __syncthreads();
     if (lds)
@@ -1741,12 +1793,18 @@ hipError_t launch_order_by_kmax(const int32_t* kmaxes, uint32_t V, int32_t kmax,
 }
 
 hipError_t launch_core_edges(const int32_t* kmaxes, Csr out, uint32_t V, int32_t kmax,
-                             uint64_t* hist, hipStream_t st) {
-    hipError_t e = hipMemsetAsync(hist, 0, sizeof(uint64_t) * (size_t)(kmax + 2), st);
+                             uint64_t* hist, uint64_t* longq, hipStream_t st) {
+    // hist[kmax+2] then the queue's length word (zeroed together)
+    hipError_t e = hipMemsetAsync(hist, 0, sizeof(uint64_t) * (size_t)(kmax + 3), st);
     if (e != hipSuccess) return e;
+    uint32_t* longn = reinterpret_cast<uint32_t*>(hist + kmax + 2);
     const unsigned blocks = (unsigned)min((V + 15u) / 16u, 2048u);
     hipLaunchKernelGGL(k_core_edges, dim3(blocks ? blocks : 1), dim3(1024), 0, st, kmaxes, out, V,
-                       kmax, reinterpret_cast<unsigned long long*>(hist));
+                       kmax, reinterpret_cast<unsigned long long*>(hist),
+                       reinterpret_cast<unsigned long long*>(longq), longn);
+    hipLaunchKernelGGL(k_core_edges_long, dim3(1024), dim3(1024), 0, st, kmaxes, out, kmax,
+                       reinterpret_cast<unsigned long long*>(hist),
+                       reinterpret_cast<const unsigned long long*>(longq), longn);
     return hipGetLastError();
 }
 
