@@ -1532,15 +1532,13 @@ __global__ void k_kmax_scatter(const int32_t* kmaxes, uint32_t V, int32_t kmax, 
     }
 }
 
-// hist[m] += edges whose endpoints' smaller kmax is m.  One wave per
-// vertex; a list of more than kCoreEdgesLong entries is cut into slices of
-// that many, queued (v << 32 | first entry) for k_core_edges_long, where a
-// workgroup takes each slice.  Bins: the wave's own kmax (most of its list)
-// and bins below kCoreEdgesBallot are counted by ballot in registers, the
-// rest by 64-bit LDS atomics (per workgroup, added to hist at the end).
+// hist[m] += edges whose endpoints' smaller kmax is m, counted by 64-bit
+// LDS atomics per workgroup and added to hist at the end.  A list of more
+// than kCoreEdgesLong entries is cut into slices of that many, queued
+// (v << 32 | first entry) for k_core_edges_long, where a workgroup takes
+// each slice.
 constexpr int kCoreEdgesUnroll = 4;
 constexpr uint32_t kCoreEdgesLong = 4096;
-constexpr int kCoreEdgesBallot = 8;
 
 __device__ __forceinline__ void core_edges_add(unsigned long long* h, unsigned long long* hist,
                                                bool lds, int32_t m, unsigned long long n) {
@@ -1550,64 +1548,78 @@ __device__ __forceinline__ void core_edges_add(unsigned long long* h, unsigned l
         atomicAdd(&hist[m], n);
 }
 
-__global__ void k_core_edges(const int32_t* kmaxes, Csr out, uint32_t V, int32_t kmax,
-                             unsigned long long* hist, unsigned long long* longq,
-                             uint32_t* longn) {
+// Tiles of kCoreEdgesTile consecutive vertices: their list lengths are
+// prefix-summed in LDS and every thread takes kCoreEdgesUnroll edges of the
+// tile per step (owner found by binary search), so short lists do not leave
+// most lanes idle; lists of more than kCoreEdgesLong entries are queued.
+constexpr int kCoreEdgesTile = 1024;
+__global__ __launch_bounds__(kCoreEdgesTile) void k_core_edges(const int32_t* kmaxes, Csr out,
+                                                               uint32_t V, int32_t kmax,
+                                                               unsigned long long* hist,
+                                                               unsigned long long* longq,
+                                                               uint32_t* longn) {
     __shared__ unsigned long long h[kHistLds];
+    __shared__ uint32_t s_off[kCoreEdgesTile], s_scan[kCoreEdgesTile];
+    __shared__ int32_t s_kv[kCoreEdgesTile];
+    __shared__ uint32_t s_wsum[kCoreEdgesTile / W], s_total;
     const bool lds = kmax + 1 <= kHistLds;
+    const uint32_t tid = threadIdx.x;
     if (lds)
-        for (int i = threadIdx.x; i <= kmax; i += blockDim.x) h[i] = 0;
+        for (int i = tid; i <= kmax; i += blockDim.x) h[i] = 0;
     __syncthreads();
-    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / W;
-    const uint32_t nw = gridDim.x * blockDim.x / W;
-    uint32_t low[kCoreEdgesBallot];  // wave-uniform counts of bins 0..7
-#pragma unroll
-    for (int b = 0; b < kCoreEdgesBallot; ++b) low[b] = 0;
-    for (uint32_t v = wave; v < V; v += nw) {
-        const int32_t kv = kmaxes[v];
-        const uint32_t o = out.off[v], n = (uint32_t)out.deg[v];
-        if (n > kCoreEdgesLong) {
-            if (lane_id() == 0) {
+    for (uint32_t base = blockIdx.x * kCoreEdgesTile; base < V;
+         base += gridDim.x * kCoreEdgesTile) {
+        const uint32_t v = base + tid;
+        uint32_t n = 0, o = 0;
+        int32_t kv = 0;
+        if (v < V) {
+            n = (uint32_t)out.deg[v];
+            o = out.off[v];
+            kv = kmaxes[v];
+            if (n > kCoreEdgesLong) {
                 const uint32_t ns = (n + kCoreEdgesLong - 1) / kCoreEdgesLong;
                 const uint32_t at = atomicAdd(longn, ns);
                 for (uint32_t i = 0; i < ns; ++i)
                     longq[at + i] = ((unsigned long long)v << 32) | (i * kCoreEdgesLong);
+                n = 0;
             }
-            continue;
         }
-        unsigned long long same_n = 0;
-        // kCoreEdgesUnroll slices of the list per round: their loads are
-        // independent
-        for (uint32_t jb = 0; jb < n; jb += kCoreEdgesUnroll * W) {
+        s_off[tid] = o;
+        s_kv[tid] = kv;
+        s_scan[tid] = block_excl_scan<kCoreEdgesTile>(n, s_wsum, &s_total);
+        __syncthreads();
+        const uint32_t total = s_total;
+        const uint32_t items = min(V - base, (uint32_t)kCoreEdgesTile);
+        for (uint32_t eb = 0; eb < total; eb += kCoreEdgesUnroll * kCoreEdgesTile) {
             uint32_t u[kCoreEdgesUnroll];
+            int32_t ko[kCoreEdgesUnroll];
 #pragma unroll
             for (int r = 0; r < kCoreEdgesUnroll; ++r) {
-                const uint32_t j = jb + (uint32_t)(r * W + lane_id());
-                u[r] = j < n ? out.nbr[o + j] : 0u;
+                const uint32_t e = eb + r * kCoreEdgesTile + tid;
+                u[r] = ~0u;
+                ko[r] = 0;
+                if (e < total) {
+                    uint32_t lo = 0, hi = items - 1;
+                    while (lo < hi) {
+                        const uint32_t mid = (lo + hi + 1) >> 1;
+                        if (s_scan[mid] <= e)
+                            lo = mid;
+                        else
+                            hi = mid - 1;
+                    }
+                    u[r] = out.nbr[s_off[lo] + (e - s_scan[lo])];
+                    ko[r] = s_kv[lo];
+                }
             }
-            int32_t m[kCoreEdgesUnroll];
 #pragma unroll
             for (int r = 0; r < kCoreEdgesUnroll; ++r)
-                m[r] = jb + (uint32_t)(r * W + lane_id()) < n ? min(kv, kmaxes[u[r]]) : -1;
-#pragma unroll
-            for (int r = 0; r < kCoreEdgesUnroll; ++r) {
-                // most of a list's edges end at a vertex of kmax >= kv, i.e.
-                // in the wave's own bin; low bins by ballot; the rest by atomics
-                same_n += __popcll(__ballot(m[r] == kv));
-#pragma unroll
-                for (int b = 0; b < kCoreEdgesBallot; ++b)
-                    low[b] += (uint32_t)__popcll(__ballot(m[r] == b && b != kv));
-                if (m[r] >= kCoreEdgesBallot && m[r] != kv) core_edges_add(h, hist, lds, m[r], 1ull);
-            }
+                if (u[r] != ~0u) core_edges_add(h, hist, lds, min(ko[r], kmaxes[u[r]]), 1ull);
         }
-        if (lane_id() == 0 && same_n) core_edges_add(h, hist, lds, kv, same_n);
+        __syncthreads();
     }
-    if (lane_id() == 0)
-        for (int b = 0; b < kCoreEdgesBallot && b <= kmax; ++b)
-            if (low[b]) core_edges_add(h, hist, lds, b, low[b]);
     __syncthreads();
     if (lds)
-        for (int i = threadIdx.x; i <= kmax; i += blockDim.x)
+        for (int i = tid; i <= kmax; i += blockDim.x)
             if (h[i]) atomicAdd(&hist[i], h[i]);
 }
 
@@ -1798,7 +1810,7 @@ hipError_t launch_core_edges(const int32_t* kmaxes, Csr out, uint32_t V, int32_t
     hipError_t e = hipMemsetAsync(hist, 0, sizeof(uint64_t) * (size_t)(kmax + 3), st);
     if (e != hipSuccess) return e;
     uint32_t* longn = reinterpret_cast<uint32_t*>(hist + kmax + 2);
-    const unsigned blocks = (unsigned)min((V + 15u) / 16u, 2048u);
+    const unsigned blocks = (unsigned)min((V + 1023u) / 1024u, 4096u);
     hipLaunchKernelGGL(k_core_edges, dim3(blocks ? blocks : 1), dim3(1024), 0, st, kmaxes, out, V,
                        kmax, reinterpret_cast<unsigned long long*>(hist),
                        reinterpret_cast<unsigned long long*>(longq), longn);
