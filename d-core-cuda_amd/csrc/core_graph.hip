@@ -1,9 +1,14 @@
 // Core-ordered graph: vertices relabelled by kmax (descending), every list
-// sorted by the new ids.  Then for every k the (k,0)-core is the id prefix
-// [0, V_k), and within a member's list the neighbours inside the core form
-// a prefix too -- a k-pass touches exactly the edges of the core (E_k per
-// direction) and needs no membership test.  Built once per decomposition,
-// after the kmax pass, with one radix sort per direction.
+// ordered by its neighbours' kmax, descending, and holding new ids.  Then
+// for every k the (k,0)-core is the id prefix [0, V_k), and within a
+// member's list the neighbours inside the core form a prefix too (new id <
+// V_k exactly when kmax >= k) -- a k-pass touches exactly the edges of the
+// core (E_k per direction) and needs no membership test.  Built once per
+// decomposition, after the kmax pass: the lists are moved to their new
+// owners' places and ordered there by a counting sort on kmax (a wavefront
+// per short list, stable; a workgroup per long list, ties in no fixed
+// order).  Graphs whose kmax exceeds the LDS histogram sort every list
+// fully by new id with one radix sort per direction instead.
 #include <hip/hip_runtime.h>
 
 #include "dcx_internal.hpp"
@@ -48,6 +53,151 @@ __global__ void k_keys_to_nbr(const uint64_t* keys, uint64_t n, uint64_t mask, u
 }
 
 __global__ void k_set_u32(uint32_t* p, uint32_t v) { *p = v; }
+
+constexpr uint32_t kShortList = 64;
+constexpr uint32_t kTinyList = 16;
+constexpr int kListBins = 4096;  // kmax + 1 at most, for the counting sort
+constexpr int kLongBlk = 256;
+
+// Place of a list entry with key kk held by lane l of a group of `width`
+// lanes holding the list's n entries: #{j : key_j > kk} + #{j < l : key_j
+// == kk} (kmax descending, ties in list order).
+__device__ __forceinline__ uint32_t list_place(int32_t kk, uint32_t l, uint32_t n, int width) {
+    uint32_t pos = 0;
+    for (uint32_t j = 0; j < n; ++j) {
+        const int32_t kj = __shfl(kk, (int)j, width);
+        pos += (kj > kk || (kj == kk && j < l)) ? 1u : 0u;
+    }
+    return pos;
+}
+
+// Lists of at most kShortList entries: a wavefront takes four consecutive
+// new owners at a time, one per 16-lane quarter when all four lists have at
+// most kTinyList entries, else one list after another on all 64 lanes.
+// Longer lists are queued (new owner id) for k_lists_long.
+__global__ void k_lists_short(const uint32_t* order, const uint32_t* rank, const int32_t* kmaxes,
+                              uint32_t V, const uint32_t* off, const int32_t* deg,
+                              const uint32_t* nbr, const uint32_t* off_new, uint32_t* out,
+                              uint32_t* longq, uint32_t* longn) {
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / W;
+    const uint32_t nw = gridDim.x * blockDim.x / W;
+    const uint32_t lane = (uint32_t)lane_id();
+    for (uint32_t i0 = wave * 4; i0 < V; i0 += nw * 4) {
+        // the four owners' lists: quarter q looks up owner i0 + q
+        const uint32_t q = lane / kTinyList, l = lane % kTinyList;
+        const uint32_t iq = i0 + q;
+        const uint32_t vq = iq < V ? order[iq] : 0u;
+        const uint32_t nq = iq < V ? (uint32_t)deg[vq] : 0u;
+        const bool tiny = __all(nq <= kTinyList);
+        if (tiny) {
+            const bool has = l < nq;
+            const uint32_t u = has ? nbr[off[vq] + l] : 0u;
+            const int32_t kk = has ? kmaxes[u] : -1;
+            const uint32_t r = has ? rank[u] : 0u;
+            // every quarter runs the longest list's count (shorter lists
+            // compare padding, key -1, which sorts last and is not written)
+            const uint32_t nmax = __reduce_max_sync(~0ull, nq);
+            uint32_t pos = 0;
+            for (uint32_t j = 0; j < nmax; ++j) {
+                const int32_t kj = __shfl(kk, (int)j, (int)kTinyList);
+                pos += (j < nq && (kj > kk || (kj == kk && j < l))) ? 1u : 0u;
+            }
+            if (has) out[off_new[iq] + pos] = r;
+            continue;
+        }
+        for (uint32_t t = 0; t < 4 && i0 + t < V; ++t) {
+            const uint32_t i = i0 + t;
+            const uint32_t v = __shfl(vq, (int)(t * kTinyList), W);
+            const uint32_t n = __shfl(nq, (int)(t * kTinyList), W);
+            if (n > kShortList) {
+                if (lane == 0) longq[atomicAdd(longn, 1u)] = i;
+                continue;
+            }
+            const bool has = lane < n;
+            const uint32_t u = has ? nbr[off[v] + lane] : 0u;
+            const int32_t kk = has ? kmaxes[u] : -1;
+            const uint32_t r = has ? rank[u] : 0u;
+            const uint32_t pos = list_place(kk, lane, n, W);
+            if (has) out[off_new[i] + pos] = r;
+        }
+    }
+}
+
+// The queued long lists, one workgroup each: LDS histogram of the
+// neighbours' kmax (and its range), start of every bin of the range in
+// descending order (kLongBlk bins per step), then each entry placed at its
+// bin's cursor; the range's bins are cleared for the next list.
+__global__ __launch_bounds__(kLongBlk) void k_lists_long(const uint32_t* order, const uint32_t* rank,
+                                                         const int32_t* kmaxes,
+                                                         const uint32_t* off, const int32_t* deg,
+                                                         const uint32_t* nbr,
+                                                         const uint32_t* off_new, uint32_t* out,
+                                                         const uint32_t* longq,
+                                                         const uint32_t* longn) {
+    __shared__ uint32_t cnt[kListBins];
+    __shared__ uint32_t w[kLongBlk / W];
+    __shared__ int32_t s_lo, s_hi;
+    __shared__ uint32_t s_carry;
+    const uint32_t nq = *longn;
+    const uint32_t tid = threadIdx.x;
+    for (int b = tid; b < kListBins; b += kLongBlk) cnt[b] = 0;
+    if (tid == 0) {
+        s_lo = kListBins;
+        s_hi = -1;
+    }
+    __syncthreads();
+    for (uint32_t q = blockIdx.x; q < nq; q += gridDim.x) {
+        const uint32_t i = longq[q];
+        const uint32_t v = order[i];
+        const uint32_t n = (uint32_t)deg[v];
+        const uint32_t* l = nbr + off[v];
+        uint32_t* dst = out + off_new[i];
+        int32_t lo = kListBins, hi = -1;
+        for (uint32_t j = tid; j < n; j += kLongBlk) {
+            const int32_t k = kmaxes[l[j]];
+            atomicAdd(&cnt[k], 1u);
+            lo = min(lo, k);
+            hi = max(hi, k);
+        }
+        atomicMin(&s_lo, lo);
+        atomicMax(&s_hi, hi);
+        if (tid == 0) s_carry = 0;
+        __syncthreads();
+        lo = s_lo;
+        hi = s_hi;
+        // starts in descending kmax, kLongBlk bins per step
+        for (int top = hi; top >= lo; top -= kLongBlk) {
+            const int b = top - (int)tid;
+            const uint32_t c = b >= lo ? cnt[b] : 0u;
+            // exclusive scan over the step (wave shuffles, then waves)
+            const int ln = lane_id(), wv = (int)(tid / W);
+            uint32_t inc = c;
+            for (int d = 1; d < W; d <<= 1) {
+                const uint32_t y = __shfl_up(inc, d, W);
+                if (ln >= d) inc += y;
+            }
+            if (ln == W - 1) w[wv] = inc;
+            __syncthreads();
+            uint32_t before = s_carry;
+            for (int x = 0; x < wv; ++x) before += w[x];
+            if (b >= lo) cnt[b] = before + inc - c;
+            __syncthreads();
+            if (tid == kLongBlk - 1) s_carry = before + inc;
+            __syncthreads();
+        }
+        for (uint32_t j = tid; j < n; j += kLongBlk) {
+            const uint32_t u = l[j];
+            dst[atomicAdd(&cnt[kmaxes[u]], 1u)] = rank[u];
+        }
+        __syncthreads();
+        for (int b = lo + (int)tid; b <= hi; b += kLongBlk) cnt[b] = 0;
+        if (tid == 0) {
+            s_lo = kListBins;
+            s_hi = -1;
+        }
+        __syncthreads();
+    }
+}
 
 // out[0] += sum of deg, out[1] += entries of [0, deg) that are not vertex
 // ids (one wavefront per vertex).
@@ -99,7 +249,9 @@ int bits_for(uint32_t v) {
 
 }  // namespace
 
-size_t core_graph_tmp_bytes(uint32_t V, uint64_t E) {
+size_t core_graph_tmp_bytes(uint32_t V, uint64_t E, int32_t kmax) {
+    if (kmax + 1 <= kListBins)  // scan words + long-list queue and its length
+        return 4ull * (scan_tmp_words(V) + 1) + 4ull * ((E ? E : 1) / kShortList + 64) + 256;
     // keys + alternate keys + scan words + radix table
     return 16ull * (E ? E : 1) + 4ull * (scan_tmp_words(V) + 1) +
            4ull * (radix_tmp_words(E) + 1) + 256;
@@ -112,12 +264,39 @@ hipError_t launch_audit(uint32_t V, Csr side, unsigned long long* out, hipStream
     return hipGetLastError();
 }
 
-hipError_t build_core_graph(uint32_t V, uint64_t E, const uint32_t* order, Csr out, Csr in,
-                            uint32_t* rank, uint32_t* cout_off, int32_t* cout_deg,
-                            uint32_t* cout_nbr, uint32_t* cin_off, int32_t* cin_deg,
-                            uint32_t* cin_nbr, void* tmp, hipStream_t st) {
+hipError_t build_core_graph(uint32_t V, uint64_t E, const uint32_t* order, const int32_t* kmaxes,
+                            int32_t kmax, Csr out, Csr in, uint32_t* rank, uint32_t* cout_off,
+                            int32_t* cout_deg, uint32_t* cout_nbr, uint32_t* cin_off,
+                            int32_t* cin_deg, uint32_t* cin_nbr, void* tmp, hipStream_t st) {
     if (V == 0) return hipSuccess;
     char* p = static_cast<char*>(tmp);
+    if (kmax + 1 <= kListBins) {
+        uint32_t* scan_tmp = reinterpret_cast<uint32_t*>(p);
+        uint32_t* longn = scan_tmp + scan_tmp_words(V) + 1;
+        uint32_t* longq = longn + 64;  // 256-byte aligned from longn
+        hipLaunchKernelGGL(k_rank, dim3(grid_for(V, 256)), dim3(256), 0, st, order, V, rank);
+        const Csr sides[2] = {out, in};
+        uint32_t* offs[2] = {cout_off, cin_off};
+        int32_t* degs[2] = {cout_deg, cin_deg};
+        uint32_t* nbrs[2] = {cout_nbr, cin_nbr};
+        for (int s = 0; s < 2; ++s) {
+            uint32_t* dnew = reinterpret_cast<uint32_t*>(degs[s]);
+            hipLaunchKernelGGL(k_new_degrees, dim3(grid_for(V, 256)), dim3(256), 0, st, order, V,
+                               sides[s].deg, dnew);
+            hipError_t e = launch_excl_scan_u32(dnew, offs[s], V, scan_tmp, st);
+            if (e != hipSuccess) return e;
+            hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, st, offs[s] + V, (uint32_t)E);
+            if (E == 0) continue;
+            hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, st, longn, 0u);
+            hipLaunchKernelGGL(k_lists_short, dim3(grid_for(V, 64)), dim3(1024), 0, st, order,
+                               rank, kmaxes, V, sides[s].off, sides[s].deg, sides[s].nbr, offs[s],
+                               nbrs[s], longq, longn);
+            hipLaunchKernelGGL(k_lists_long, dim3(4096), dim3(kLongBlk), 0, st, order, rank,
+                               kmaxes, sides[s].off, sides[s].deg, sides[s].nbr, offs[s],
+                               nbrs[s], longq, longn);
+        }
+        return hipGetLastError();
+    }
     uint64_t* keys = reinterpret_cast<uint64_t*>(p);
     uint64_t* alt = keys + (E ? E : 1);
     uint32_t* scan_tmp = reinterpret_cast<uint32_t*>(alt + (E ? E : 1));

@@ -133,8 +133,9 @@ void build_core(dcx_ctx* c, dcx_graph* g) {
         dev_alloc(c, &g->cin_nbr, cap);
         g->core_e_cap = cap;
     }
-    ensure_bytes(c, &c->tmp, &c->tmp_bytes, core_graph_tmp_bytes(V, g->E));
-    HIPX(c, build_core_graph(V, g->E, g->order, Csr{g->out_off, g->out_deg, g->out_nbr},
+    ensure_bytes(c, &c->tmp, &c->tmp_bytes, core_graph_tmp_bytes(V, g->E, g->kmax));
+    HIPX(c, build_core_graph(V, g->E, g->order, g->kmaxes, g->kmax,
+                             Csr{g->out_off, g->out_deg, g->out_nbr},
                              Csr{g->in_off, g->in_deg, g->in_nbr}, g->rank, g->cout_off,
                              g->cout_deg, g->cout_nbr, g->cin_off, g->cin_deg, g->cin_nbr, c->tmp,
                              c->stream));
@@ -255,15 +256,36 @@ void compute_kmax(dcx_ctx* c, dcx_graph* g, int variant = DCX_VARIANT_BATCHED,
         // every other row): compute it beside the kmax pass
         const bool pair = variant == DCX_VARIANT_BATCHED && g->V > c->lds_cap &&
                           c->grid_blocks >= 2 * (int)kXcds;
+        // Valid rows hold 0 in every cell (k, v) with k > kmax(v): a new
+        // decomposition, which writes every cell of every core member, then
+        // only has to zero the cells of vertices whose kmax fell (and rows
+        // beyond the old row count), not the whole array.
+        const bool keep = pair && clear_rows && g->rows_valid && g->kmax_valid && g->rows;
+        const int32_t old_rows = keep ? g->rows_k : 0;
+        if (keep) {
+            if (!g->kmax_old) dev_alloc(c, &g->kmax_old, g->V);
+            HIPX(c, hipMemcpyAsync(g->kmax_old, g->kmaxes, 4ull * g->V, hipMemcpyDeviceToDevice,
+                                   c->stream));
+        }
         g->kmax = pair ? run_kmax_with_row0(c, g) : run_grid_pass(c, kmax_spec(g), 0, 1);
         if (pair && clear_rows && g->kmax >= 0) {
-            // every row follows: clear them on the second stream after row
-            // 0, beside the order kernels
             const size_t cells = ((size_t)g->kmax + 1) * g->V;
             g->rows_valid = false;
-            grow_keep(c, &g->rows, &g->rows_cap, cells, 0);
-            HIPX(c, hipMemsetAsync(g->rows, 0, cells * sizeof(int32_t), c->side2));
-            HIPX(c, hipEventRecord(c->clear_done, c->side2));
+            if (keep && g->rows_cap >= cells) {
+                HIPX(c, launch_clear_stale(g->kmaxes, g->kmax_old, g->V, old_rows, g->kmax + 1,
+                                           g->rows, c->stream));
+                if ((size_t)old_rows < (size_t)g->kmax + 1)
+                    HIPX(c, hipMemsetAsync(g->rows + (size_t)old_rows * g->V, 0,
+                                           (cells - (size_t)old_rows * g->V) * sizeof(int32_t),
+                                           c->stream));
+                HIPX(c, hipEventRecord(c->clear_done, c->stream));
+            } else {
+                // every row follows: clear them on the second stream after
+                // row 0, beside the order kernels
+                grow_keep(c, &g->rows, &g->rows_cap, cells, 0);
+                HIPX(c, hipMemsetAsync(g->rows, 0, cells * sizeof(int32_t), c->side2));
+                HIPX(c, hipEventRecord(c->clear_done, c->side2));
+            }
             g->rows_cleared = cells;
         }
     } else {

@@ -228,6 +228,10 @@ uint32_t rows_lds_static_bytes();
 hipError_t launch_order_by_kmax(const int32_t* kmaxes, uint32_t V, int32_t kmax,
                                 uint32_t* hist /*[kmax+2]*/, uint32_t* vk /*[kmax+2]*/,
                                 uint32_t* order, hipStream_t st);
+// Zeroes rows[k * V + v] for k in (knew[v], min(kold[v], rows_old - 1)]
+// (rows_new: the new row count; rows from rows_old on are the caller's).
+hipError_t launch_clear_stale(const int32_t* knew, const int32_t* kold, uint32_t V,
+                              int32_t rows_old, int32_t rows_new, int32_t* rows, hipStream_t st);
 // hist[m] = edges whose endpoints' smaller kmax is m, m = 0..kmax (hist has
 // kmax + 3 words: the last holds the long-list queue's length); longq: room
 // for E / 2048 + 64 slices of long lists (E = the graph's edges).
@@ -252,14 +256,17 @@ size_t radix_tmp_words(uint64_t n);
 hipError_t launch_radix_sort_u64(uint64_t* keys, uint64_t* alt, uint64_t n, int bits,
                                  uint32_t* table, uint64_t** result, hipStream_t st,
                                  int lo_bit = 0);
-size_t core_graph_tmp_bytes(uint32_t V, uint64_t E);
+size_t core_graph_tmp_bytes(uint32_t V, uint64_t E, int32_t kmax);
 // Adds sum(deg) to out[0] and the count of list entries in [0, deg) that
 // are not vertex ids (< V) to out[1].
 hipError_t launch_audit(uint32_t V, Csr side, unsigned long long* out, hipStream_t st);
-hipError_t build_core_graph(uint32_t V, uint64_t E, const uint32_t* order, Csr out, Csr in,
-                            uint32_t* rank, uint32_t* cout_off, int32_t* cout_deg,
-                            uint32_t* cout_nbr, uint32_t* cin_off, int32_t* cin_deg,
-                            uint32_t* cin_nbr, void* tmp, hipStream_t st);
+// Core-ordered graph (core_graph.hip): rank[v] = new id of v (position in
+// order); lists of new owner i = the lists of order[i] in new ids, ordered
+// by the neighbours' kmax, descending.
+hipError_t build_core_graph(uint32_t V, uint64_t E, const uint32_t* order, const int32_t* kmaxes,
+                            int32_t kmax, Csr out, Csr in, uint32_t* rank, uint32_t* cout_off,
+                            int32_t* cout_deg, uint32_t* cout_nbr, uint32_t* cin_off,
+                            int32_t* cin_deg, uint32_t* cin_nbr, void* tmp, hipStream_t st);
 
 // ---- graph updates (maint_kernels.hip); pairs are (u, v) interleaved ---
 hipError_t launch_find_edges(const uint32_t* off, const int32_t* deg, const uint32_t* nbr,

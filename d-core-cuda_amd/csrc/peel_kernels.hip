@@ -1658,6 +1658,16 @@ __global__ __launch_bounds__(1024) void k_core_edges_long(const int32_t* kmaxes,
             if (h[i]) atomicAdd(&hist[i], h[i]);
 }
 
+// rows[k * V + v] = 0 for k in (knew[v], min(kold[v], rows_old - 1)]: the
+// cells a fallen kmax leaves outside the cores.
+__global__ void k_clear_stale(const int32_t* knew, const int32_t* kold, uint32_t V,
+                              int32_t rows_old, int32_t* rows) {
+    for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < V; v += gridDim.x * blockDim.x) {
+        const int32_t a = knew[v], b = min(kold[v], rows_old - 1);
+        for (int32_t k = a + 1; k <= b; ++k) rows[(size_t)k * V + v] = 0;
+    }
+}
+
 __global__ void k_fill_i32(int32_t* p, int32_t v, size_t n) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
          i += (size_t)gridDim.x * blockDim.x)
@@ -1817,6 +1827,16 @@ hipError_t launch_core_edges(const int32_t* kmaxes, Csr out, uint32_t V, int32_t
     hipLaunchKernelGGL(k_core_edges_long, dim3(1024), dim3(1024), 0, st, kmaxes, out, kmax,
                        reinterpret_cast<unsigned long long*>(hist),
                        reinterpret_cast<const unsigned long long*>(longq), longn);
+    return hipGetLastError();
+}
+
+hipError_t launch_clear_stale(const int32_t* knew, const int32_t* kold, uint32_t V,
+                              int32_t rows_old, int32_t rows_new, int32_t* rows, hipStream_t st) {
+    (void)rows_new;
+    if (!V || rows_old <= 0) return hipSuccess;
+    const unsigned blocks = (unsigned)min((V + 255u) / 256u, 8192u);
+    hipLaunchKernelGGL(k_clear_stale, dim3(blocks), dim3(256), 0, st, knew, kold, V, rows_old,
+                       rows);
     return hipGetLastError();
 }
 
