@@ -1467,6 +1467,7 @@ dcx_status dcx_ctx_destroy(dcx_ctx* c) {
     dev_free(c->pools);
     dev_free(c->trace);
     dev_free(c->fail_rec);
+    dev_free(c->zeros);
     dev_free(c->ctrs);
     dev_free(c->ctrs2);
     dev_free(c->d_scalars);

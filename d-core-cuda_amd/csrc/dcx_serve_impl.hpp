@@ -73,6 +73,48 @@ uint64_t state_hash(dcx_ctx* c, dcx_graph* g) {
     return h;
 }
 
+// out[0] += cells where a and b differ, out[1] = min over them of the cell
+// index (left at ~0 when none differs).
+__global__ void k_compare_i32(const int32_t* a, const int32_t* b, uint64_t n,
+                              unsigned long long* out) {
+    unsigned long long cnt = 0, first = ~0ull;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x) {
+        if (a[i] != b[i]) {
+            ++cnt;
+            first = first < i ? first : i;
+        }
+    }
+    for (int d = 32; d > 0; d >>= 1) {
+        cnt += __shfl_xor(cnt, d, 64);
+        const unsigned long long f = __shfl_xor(first, d, 64);
+        first = first < f ? first : f;
+    }
+    if ((threadIdx.x & 63) == 0 && cnt) {
+        atomicAdd(out, cnt);
+        atomicMin(out + 1, first);
+    }
+}
+
+// Mismatching cells of a[0..n) and b[0..n) (device arrays) and the first of
+// them (~0 if none); `base` is added to the first index.
+void compare_cells(dcx_ctx* c, const int32_t* a, const int32_t* b, uint64_t n, uint64_t base,
+                   uint64_t* count, uint64_t* first) {
+    if (!n) return;
+    unsigned long long* d = reinterpret_cast<unsigned long long*>(c->d_scalars + 16);
+    HIPX(c, hipMemsetAsync(d, 0, 8, c->stream));
+    HIPX(c, hipMemsetAsync(d + 1, 0xff, 8, c->stream));
+    const unsigned blocks = (unsigned)std::min<uint64_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(k_compare_i32, dim3(blocks), dim3(256), 0, c->stream, a, b, n, d);
+    HIPX(c, hipGetLastError());
+    HIPX(c, hipMemcpyAsync(c->h_scalars + 16, d, 16, hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    uint64_t r[2];
+    std::memcpy(r, c->h_scalars + 16, 16);
+    *count += r[0];
+    if (r[0] && r[1] + base < *first) *first = r[1] + base;
+}
+
 struct ServeReq {
     uint64_t id = 0;
     std::vector<uint32_t> ins, del;
@@ -295,6 +337,58 @@ dcx_status dcx_state_hash(dcx_ctx* ctx, dcx_graph* g, uint64_t* hash) {
     if (!g->kmax_valid || !g->rows_valid) return DCX_ERR_STATE;
     return guarded(ctx, [&]() -> dcx_status {
         *hash = state_hash(ctx, g);
+        return DCX_OK;
+    });
+}
+
+dcx_status dcx_compare_i32(dcx_ctx* ctx, const int32_t* a, const int32_t* b, uint64_t n,
+                           uint64_t* mismatches, uint64_t* first) {
+    if (!ctx || !mismatches || (n && (!a || !b))) return DCX_ERR_INVALID_ARG;
+    return guarded(ctx, [&]() -> dcx_status {
+        *mismatches = 0;
+        uint64_t f = ~0ull;
+        compare_cells(ctx, a, b, n, 0, mismatches, &f);
+        if (first) *first = f;
+        return DCX_OK;
+    });
+}
+
+dcx_status dcx_compare_decompositions(dcx_ctx* ctx, dcx_graph* a, dcx_graph* b,
+                                      uint64_t* mismatches, uint64_t* first) {
+    dcx_status s = check_graph(ctx, a);
+    if (s != DCX_OK) return s;
+    s = check_graph(ctx, b);
+    if (s != DCX_OK) return s;
+    if (!mismatches) return DCX_ERR_INVALID_ARG;
+    if (a->V != b->V) {
+        set_err(ctx, "dcx_compare_decompositions: %u vertices against %u", a->V, b->V);
+        return DCX_ERR_INVALID_ARG;
+    }
+    if (!a->kmax_valid || !a->rows_valid || !b->kmax_valid || !b->rows_valid)
+        return DCX_ERR_STATE;
+    return guarded(ctx, [&]() -> dcx_status {
+        const uint64_t V = a->V;
+        uint64_t cnt = 0, f = ~0ull;
+        compare_cells(ctx, a->kmaxes, b->kmaxes, V, 0, &cnt, &f);
+        // rows 0..min(kmax): cell (k, v) is index V * (1 + k) + v
+        const int32_t kc = std::min(a->kmax, b->kmax);
+        if (kc >= 0)
+            compare_cells(ctx, a->rows, b->rows, (uint64_t)(kc + 1) * V, V, &cnt, &f);
+        // rows only one side has: every nonzero cell there differs
+        const dcx_graph* hi = a->kmax > b->kmax ? a : b;
+        for (int32_t k = kc + 1; k <= hi->kmax; ++k) {
+            if (!ctx->zeros || ctx->zeros_n < V) {
+                dev_free(ctx->zeros);
+                ctx->zeros_n = 0;
+                dev_alloc(ctx, &ctx->zeros, (size_t)V);
+                ctx->zeros_n = V;
+                HIPX(ctx, hipMemsetAsync(ctx->zeros, 0, 4 * V, ctx->stream));
+            }
+            compare_cells(ctx, hi->rows + (size_t)k * V, ctx->zeros, V, V * (1 + (uint64_t)k),
+                          &cnt, &f);
+        }
+        *mismatches = cnt;
+        if (first) *first = f;
         return DCX_OK;
     });
 }

@@ -45,6 +45,8 @@ struct dcx_ctx {
     // start and end of the team launch on `stream` (ev[5] opens the rows
     // kernels, row 0's grid pass first when the other rows use its bound)
     hipEvent_t sev[4] = {};
+    int32_t* zeros = nullptr;  // V zero words (dcx_compare_decompositions)
+    size_t zeros_n = 0;
     std::mutex mu;
     std::string err;
     dcx_timing timing{};

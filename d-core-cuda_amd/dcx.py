@@ -341,6 +341,16 @@ class Context:
         check(lib().dcx_set_scheduler(self.h, mlp.h if mlp is not None else None),
               "dcx_set_scheduler", self.h)
 
+    def compare_arrays(self, a, b):
+        """dcx_compare_i32 on two device arrays (DeviceArray) of equal size:
+        (mismatching entries, first such index or None)."""
+        if a.size != b.size:
+            raise ValueError("arrays of different sizes")
+        n, f = ctypes.c_uint64(), ctypes.c_uint64()
+        check(lib().dcx_compare_i32(self.h, a.ptr, b.ptr, ctypes.c_uint64(a.size),
+                                    ctypes.byref(n), ctypes.byref(f)), "dcx_compare_i32", self.h)
+        return n.value, (None if n.value == 0 else f.value)
+
     def set_rows_rule(self, chain_rows=0, team_max=0):
         """cfg.chain_rows and cfg.team_max for the next decompositions
         (chain_rows 0: the default rule, or the cost model's plan when one
@@ -635,6 +645,16 @@ class Graph:
         check(lib().dcx_state_hash(self.ctx.h, self.h, ctypes.byref(h)), "dcx_state_hash",
               self.ctx.h)
         return h.value
+
+    def compare(self, other):
+        """dcx_compare_decompositions against another resident graph of the
+        same V: (mismatching cells, first such cell or None); cell v is
+        kmaxes[v], cell V * (1 + k) + v is row k of v."""
+        n, f = ctypes.c_uint64(), ctypes.c_uint64()
+        check(lib().dcx_compare_decompositions(self.ctx.h, self.h, other.h, ctypes.byref(n),
+                                               ctypes.byref(f)),
+              "dcx_compare_decompositions", self.ctx.h)
+        return n.value, (None if n.value == 0 else f.value)
 
     def serve(self, **kw):
         """Starts a maintenance server on this graph (dcx_serve_start)."""

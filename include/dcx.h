@@ -431,6 +431,20 @@ dcx_status dcx_serve_stop(dcx_server* sv, dcx_serve_stats* stats);
  * finaliser).  Independent of evaluation order; no reference counterpart
  * (the reference compares whole arrays on the host, src/utils.h:13-43). */
 dcx_status dcx_state_hash(dcx_ctx* ctx, dcx_graph* g, uint64_t* hash);
+/* Device-side compare, the verified counterpart of the reference's
+ * compareArray kernel (src/cuda_utils.cuh:79-93, which counts mismatching
+ * entries of two device arrays): a and b are device pointers of n int32;
+ * *mismatches = entries that differ, *first (nullable) = the first such
+ * index (UINT64_MAX if none). */
+dcx_status dcx_compare_i32(dcx_ctx* ctx, const int32_t* a, const int32_t* b, uint64_t n,
+                           uint64_t* mismatches, uint64_t* first);
+/* Compares two resident decompositions on the device -- the reference's
+ * "maintained == recomputed" check (main.cu:1905-1924, 2194-2211) without a
+ * download: cell index v for kmaxes[v], V * (1 + k) + v for row k (a row
+ * only one side has counts its nonzero cells).  Both graphs need the same
+ * V and valid rows. */
+dcx_status dcx_compare_decompositions(dcx_ctx* ctx, dcx_graph* a, dcx_graph* b,
+                                      uint64_t* mismatches, uint64_t* first);
 
 /* ---- learned variant scheduler ---------------------------------------
  * A small fp32 MLP (8 inputs, two hidden layers of 32 ReLU units, one
