@@ -329,6 +329,7 @@ def run_headline(args):
     for _ in range(2):
         g.decompose(dcx.VARIANT_BATCHED)
         heur.append(ctx.timing().total_ms)
+    log(f"fixed rule: {min(heur):.1f} ms")
     h_hash = g.state_hash()
     model = None
     n_samples = 0
@@ -346,6 +347,8 @@ def run_headline(args):
         for spacing, tmax in ((8, 0), (8, 4)):
             ctx.set_rows_rule(spacing, tmax)
             g.decompose(dcx.VARIANT_BATCHED)
+            log(f"exploration run (every {spacing}th row peeled, team cap {tmax or 32}): "
+                f"{ctx.timing().total_ms:.1f} ms")
             collect()
         ctx.set_rows_rule(0, 0)
         model = dcx.MLP(ctx, seed=7)

@@ -233,8 +233,11 @@ uint64_t* ensure_ekq(dcx_ctx* c, dcx_graph* g) {
     return g->ekq;
 }
 
+// clear_rows: every row follows (a full decomposition); rows_zero: the rows
+// array still holds 0 in every cell (k, v) with k > kmax(v) for the kmaxes
+// being replaced (it held a valid decomposition of them).
 void compute_kmax(dcx_ctx* c, dcx_graph* g, int variant = DCX_VARIANT_BATCHED,
-                  bool warm = false, bool clear_rows = false) {
+                  bool warm = false, bool clear_rows = false, bool rows_zero = false) {
     if (!g->kmaxes) dev_alloc(c, &g->kmaxes, g->V);
     g->core_valid = false;
     g->order_valid = false;
@@ -260,7 +263,7 @@ void compute_kmax(dcx_ctx* c, dcx_graph* g, int variant = DCX_VARIANT_BATCHED,
         // decomposition, which writes every cell of every core member, then
         // only has to zero the cells of vertices whose kmax fell (and rows
         // beyond the old row count), not the whole array.
-        const bool keep = pair && clear_rows && g->rows_valid && g->kmax_valid && g->rows;
+        const bool keep = pair && clear_rows && rows_zero && g->kmax_valid && g->rows;
         const int32_t old_rows = keep ? g->rows_k : 0;
         if (keep) {
             if (!g->kmax_old) dev_alloc(c, &g->kmax_old, g->V);
@@ -1153,8 +1156,10 @@ void compute_rows(dcx_ctx* c, dcx_graph* g, int variant, int32_t kr = -1) {
         if (!lks.empty()) {
             // LDS rows on the side stream: each of its workgroups needs a
             // CU's LDS to itself and waits for nothing, so it starts where
-            // team workgroups have finished (or have not started yet) and
-            // never holds up a team for longer than one row
+            // team workgroups have finished (or have not started yet); it
+            // takes rows from its queue until the queue drains, so a team
+            // member that finds no free CU can wait until then (bounded by
+            // the barrier timeout, as every wait)
             HIPX(c, hipEventRecord(c->fork, c->stream));
             HIPX(c, hipStreamWaitEvent(c->side, c->fork, 0));
         }
@@ -1859,9 +1864,10 @@ dcx_status dcx_decompose(dcx_ctx* ctx, dcx_graph* g, int32_t variant, int32_t* k
     }
     return guarded(ctx, [&]() -> dcx_status {
         ctx->timing = dcx_timing{};
+        const bool rows_zero = g->rows_valid && g->kmax_valid;
         g->rows_valid = false;
         const auto t0 = std::chrono::steady_clock::now();
-        compute_kmax(ctx, g, variant, false, true);
+        compute_kmax(ctx, g, variant, false, true, rows_zero);
         compute_rows(ctx, g, variant);
         sync(ctx);
         g->full_ms = std::chrono::duration<double, std::milli>(
