@@ -232,12 +232,12 @@ def run_ingest(args):
     print(json.dumps(line), flush=True)
 
 
-def read_pmc(name):
-    """Calibrated HBM bytes per launch from a committed rocprofv3 PMC summary
-    under profiles/ (a 'traffic_bytes_per_launch:' line), or None."""
+def read_pmc(name, key="traffic_bytes_per_launch"):
+    """A figure of a committed rocprofv3 PMC summary under profiles/ (a
+    '<key>: value' line, e.g. calibrated HBM bytes per launch), or None."""
     try:
         for line in open(os.path.join(ROOT, "profiles", name)):
-            if line.startswith("traffic_bytes_per_launch:"):
+            if line.startswith(key + ":"):
                 return float(line.split(":")[1])
     except OSError:
         pass
@@ -421,6 +421,9 @@ def run_headline(args):
                 if dom == "k_rows_teams" else None,
                 "kernel": dom.split(" ")[0], "kernel_ms": round(ms_dom, 3), "alg_bytes": b_dom,
                 "rows": rows_dom}
+    pmc_name = f"k_rows_teams_rmat{scale}_pmc.txt"
+    fr_traffic = read_pmc(pmc_name, "fixed_rule_traffic_bytes_per_launch")
+    fr_alg = read_pmc(pmc_name, "fixed_rule_alg_bytes_per_launch")
     b_all = team_b + lds_b + b_kmax + 8.0 * E + 20.0 * V  # + row 0 beside the kmax pass
     extra = {
         "kernels": {k: {"alg_bytes": v[0], "ms": round(v[1], 3), "rows": v[2],
@@ -438,6 +441,12 @@ def run_headline(args):
                          "order": round(mean("order_ms"), 3),
                          "rows_kernel": round(mean("rows_kernel_ms"), 3),
                          "rows_total": round(mean("rows_ms"), 3)},
+        # counters of the team launch exist for the fixed rule's plan only
+        # (profiles/k_rows_teams_rmat24_pmc.txt)
+        "team_launch_traffic_fixed_rule": (
+            {"traffic": fr_traffic, "alg_bytes": fr_alg,
+             "traffic_over_alg": round(fr_traffic / fr_alg, 3)}
+            if fr_traffic and fr_alg else None),
         "schedule": {"kind": args.schedule, "model_used": used_model,
                      "training": f"{n_samples} samples: this graph's logged team rows under the "
                                  "fixed rule, two exploration runs (every 8th row peeled, team "
