@@ -286,6 +286,50 @@ def oracle_sample(g, V, threads, log):
                                               "estimate_s": est1}}
 
 
+def train_learned_plan(dcx, ctx, g, log, Xs=None, Ts=None):
+    """The cost model's training protocol on graph g (untimed): the pass log
+    of the fixed rule's last run (already decomposed by the caller), two
+    exploration runs (every 8th team row peeled, team caps 32 and 4, so the
+    samples cover narrow teams too), then two refinement rounds in which the
+    model plans a run and is retrained with that run's log.  Returns the
+    attached model, the sample count and the refinement runs' times."""
+    Xs = list(Xs or [])
+    Ts = list(Ts or [])
+
+    def collect():
+        for wl in (True, False):
+            X, T = g.pass_features(with_log=wl)
+            Xs.append(X)
+            Ts.append(T)
+
+    collect()
+    for spacing, tmax in ((8, 0), (8, 4)):
+        ctx.set_rows_rule(spacing, tmax)
+        g.decompose(dcx.VARIANT_BATCHED)
+        log(f"exploration run (every {spacing}th row peeled, team cap {tmax or 32}): "
+            f"{ctx.timing().total_ms:.1f} ms")
+        collect()
+    ctx.set_rows_rule(0, 0)
+    if sum(len(t) for t in Ts) == 0:
+        log("no team rows to learn from (every row ran whole-graph or LDS-resident): "
+            "the fixed rule stays")
+        return None, 0, []
+    model = dcx.MLP(ctx, seed=7)
+    refine_ms = []
+    n = 0
+    for rnd in range(3):
+        X, T = np.concatenate(Xs), np.concatenate(Ts)
+        loss = model.train(X, T, epochs=3000 if rnd == 0 else 1500, lr=3e-3)
+        n = len(T)
+        log(f"cost model round {rnd}: {n} samples, loss {loss:.4f}")
+        ctx.set_scheduler(model)
+        if rnd < 2:
+            g.decompose(dcx.VARIANT_BATCHED)
+            refine_ms.append(round(ctx.timing().total_ms, 2))
+            collect()
+    return model, n, refine_ms
+
+
 def run_headline(args):
     """R-MAT scale 24 (2^28 samples, seed 4), learned schedule: the
     learned-scheduling workload's R-MAT graph, the largest single-GPU
@@ -336,32 +380,7 @@ def run_headline(args):
     refine_ms = []
     if args.schedule == "learned":
         Xs, Ts = learned_training_set(dcx, log) if args.full_train else ([], [])
-
-        def collect():
-            for wl in (True, False):
-                X, T = g.pass_features(with_log=wl)
-                Xs.append(X)
-                Ts.append(T)
-
-        collect()
-        for spacing, tmax in ((8, 0), (8, 4)):
-            ctx.set_rows_rule(spacing, tmax)
-            g.decompose(dcx.VARIANT_BATCHED)
-            log(f"exploration run (every {spacing}th row peeled, team cap {tmax or 32}): "
-                f"{ctx.timing().total_ms:.1f} ms")
-            collect()
-        ctx.set_rows_rule(0, 0)
-        model = dcx.MLP(ctx, seed=7)
-        for rnd in range(3):
-            X, T = np.concatenate(Xs), np.concatenate(Ts)
-            loss = model.train(X, T, epochs=3000 if rnd == 0 else 1500, lr=3e-3)
-            n_samples = len(T)
-            log(f"cost model round {rnd}: {len(T)} samples, loss {loss:.4f}")
-            ctx.set_scheduler(model)
-            if rnd < 2:
-                g.decompose(dcx.VARIANT_BATCHED)
-                refine_ms.append(round(ctx.timing().total_ms, 2))
-                collect()
+        model, n_samples, refine_ms = train_learned_plan(dcx, ctx, g, log, Xs, Ts)
     for _ in range(args.warmup):
         g.decompose(dcx.VARIANT_BATCHED)
     tims = []
@@ -984,9 +1003,9 @@ def learned_training_set(dcx, log):
 
 def run_learned(args):
     """R-MAT scale 24 and a Chung-Lu digraph of the same size, each decomposed
-    with the work-share team widths, with team widths from the GPU-trained
-    cost model, and with the plain one-k-per-launch peel; bit-exactness
-    between the three is checked by the device state hash."""
+    with the fixed rule, with the plan of the GPU-trained cost model (trained
+    as for the headline), and with the plain one-k-per-launch peel;
+    bit-exactness between the three is checked by the device state hash."""
     import dcx
 
     def log(msg):
@@ -995,19 +1014,22 @@ def run_learned(args):
     sc = args.learned_scale
     V = 1 << sc
     t0 = time.perf_counter()
-    Xs, Ts = learned_training_set(dcx, log)
     ctx = dcx.Context(0)
-    model = dcx.MLP(ctx, seed=7)
     results = []
-    for name, gen in ((f"rmat{sc}", lambda: dcx.gen_rmat(sc, 16 << sc, 4)),
+    for name, gen in ((f"rmat{sc}", lambda: ctx.gen_rmat_device(sc, 16 << sc, 4)),
                       (f"chunglu{sc}", lambda: dcx.gen_chunglu(V, 16 << sc, 5))):
         t1 = time.perf_counter()
         s, d = gen()
-        E = int(s.size)
-        g = ctx.upload_edges(V, s, d)
+        if isinstance(s, dcx.DeviceArray):
+            g = ctx.ingest(V, s, d, 0, drop_self_loops=True, dedup=True)
+            s.free()
+            d.free()
+        else:
+            g = ctx.ingest(V, s, d, 0)
         del s, d
-        log(f"{name}: V={V} E={E} generated and uploaded in {time.perf_counter() - t1:.1f}s")
-        # work-share widths (the fixed heuristic); its logs join the training set
+        E = int(g.info()[1])
+        log(f"{name}: V={V} E={E} generated and built in {time.perf_counter() - t1:.1f}s")
+        # the fixed rule (every 32nd team row peeled, widths by work share)
         ctx.set_scheduler(None)
         heur = []
         for _ in range(3):
@@ -1015,19 +1037,8 @@ def run_learned(args):
             t = ctx.timing()
             heur.append((t.total_ms, t.rows_kernel_ms, t.kmax_ms, t.order_ms))
         h_hash = g.state_hash()
-        log(f"{name}: kmax={K} lmax={lmax} heuristic {[round(x[0], 1) for x in heur]} ms")
-        for wl in (True, False):
-            X, T = g.pass_features(with_log=wl)
-            Xs.append(X)
-            Ts.append(T)
-        X = np.concatenate(Xs)
-        T = np.concatenate(Ts)
-        t1 = time.perf_counter()
-        loss = model.train(X, T, epochs=2000, lr=3e-3)
-        corr = float(np.corrcoef(model.predict(X), T)[0, 1])
-        log(f"{name}: model trained on {len(T)} samples in {time.perf_counter() - t1:.1f}s, "
-            f"loss {loss:.4f}, fit corr {corr:.3f}")
-        ctx.set_scheduler(model)
+        log(f"{name}: kmax={K} lmax={lmax} fixed rule {[round(x[0], 1) for x in heur]} ms")
+        model, n_samples, refine_ms = train_learned_plan(dcx, ctx, g, log)
         learned = []
         for _ in range(3):
             g.decompose(dcx.VARIANT_BATCHED)
@@ -1036,6 +1047,11 @@ def run_learned(args):
                             t.rows_scheduler))
         m_hash = g.state_hash()
         ctx.set_scheduler(None)
+        X, T = g.pass_features(with_log=True)
+        corr = (float(np.corrcoef(model.predict(X), T)[0, 1])
+                if model is not None and len(T) > 2 else None)
+        if model is not None:
+            model.free()
         log(f"{name}: learned {[round(x[0], 1) for x in learned]} ms, "
             f"hash equal {m_hash == h_hash}")
         plain = None
@@ -1050,19 +1066,19 @@ def run_learned(args):
         lb = min(learned)
         results.append({
             "graph": name, "V": V, "E": E, "kmax": K, "lmax": lmax,
-            "heuristic_ms": round(hb[0], 2), "heuristic_rows_kernel_ms": round(hb[1], 2),
+            "fixed_rule_ms": round(hb[0], 2), "fixed_rule_rows_kernel_ms": round(hb[1], 2),
             "learned_ms": round(lb[0], 2), "learned_rows_kernel_ms": round(lb[1], 2),
             "learned_scheduler_used": bool(lb[4]),
             "kmax_pass_ms": round(lb[2], 2), "order_ms": round(lb[3], 2),
-            "learned_speedup_vs_heuristic": round(hb[0] / lb[0], 3),
+            "learned_speedup_vs_fixed_rule": round(hb[0] / lb[0], 3),
             "plain": plain,
             "learned_speedup_vs_plain": round(plain["total_ms"] / lb[0], 2) if plain else None,
             "state_hash": f"{h_hash:016x}",
-            "bit_exact_learned_vs_heuristic": m_hash == h_hash,
-            "model_fit_corr": round(corr, 3),
+            "bit_exact_learned_vs_fixed_rule": m_hash == h_hash,
+            "model_fit_corr": round(corr, 3) if corr is not None else None,
+            "training_samples": n_samples,
         })
         g.free()
-    model.free()
     ctx.close()
     E_tot = sum(r["E"] for r in results)
     T_tot = sum(r["learned_ms"] for r in results)
@@ -1082,7 +1098,7 @@ def run_learned(args):
         "data": "synthetic",
         "config": {"workload": f"learned: R-MAT {sc} (16 * 2^{sc} samples, seed 4) and Chung-Lu "
                                f"V = 2^{sc}, 16 * 2^{sc} samples, gamma 2.1, seed 5; MLP 8-32-32-1 "
-                               "trained on the GPU from logged passes",
+                               "trained on the GPU from each graph's logged team rows",
                    "graphs": [r["graph"] for r in results]},
         "graphs": results,
         "wall_s": round(time.perf_counter() - t0, 1),
