@@ -38,13 +38,17 @@
 // A chain runs inside the persistent rows launch (k_rows_teams), on the
 // team that peeled its anchor or, for row 0's chain, on a team of its own:
 // workgroups of one XCD with the bounded team barrier of grid_sync.hpp,
-// every shared word accessed agent-coherently.  Lists of at least
-// kLongList entries are walked by a whole workgroup, the rest by one
-// wavefront.
+// every shared word accessed agent-coherently.  A vertex's lists are walked
+// by one thread when they hold at most kTiny entries together, by one
+// wavefront below kLongList entries and by a whole workgroup from there:
+// most vertices of the low rows have a handful of member neighbours, and a
+// wavefront per vertex would leave such walks bound by the latency of one
+// vertex after another.
 #pragma once
 #include <hip/hip_runtime.h>
 
 #include <climits>
+#include <type_traits>
 
 #include "dcx_internal.hpp"
 #include "grid_sync.hpp"
@@ -56,9 +60,18 @@ constexpr int W = 64;
 constexpr int kCBlk = 512;
 constexpr int kCWaves = kCBlk / W;
 constexpr uint32_t kLongList = 2048;
+constexpr uint32_t kTiny = 16;
 
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & (W - 1); }
 __device__ __forceinline__ uint32_t wave_id() { return threadIdx.x / W; }
+
+// Walk groups: one thread (1), one wavefront (W) or the workgroup (kCBlk);
+// gme<G>() is the caller's index inside its group.
+template <int G>
+__device__ __forceinline__ uint32_t gme() {
+    static_assert(G == 1 || G == W || G == kCBlk, "walk group");
+    return G == 1 ? 0u : (G == W ? lane_id() : threadIdx.x);
+}
 
 template <typename T>
 __device__ __forceinline__ T ldc(const T* p) {
@@ -73,6 +86,28 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
 #pragma unroll
     for (int d = W / 2; d > 0; d >>= 1) x += (uint32_t)__shfl_xor(x, d, W);
     return x;
+}
+__device__ __forceinline__ uint64_t wave_sum64(uint64_t x) {
+#pragma unroll
+    for (int d = W / 2; d > 0; d >>= 1) x += (uint64_t)__shfl_xor((unsigned long long)x, d, W);
+    return x;
+}
+
+// Per lane of the calling wavefront an item (has: valid) whose lists hold
+// len < kLongList entries: f(group, item) runs on the lane itself for
+// items of at most kTiny entries, and on the whole wavefront, item after
+// item, for the others (group: std::integral_constant<int, 1 or W>).
+// Called by all lanes of the wavefront.
+template <typename F>
+__device__ __forceinline__ void by_lane_or_wave(bool has, uint32_t len, uint32_t item, F&& f) {
+    const bool tiny = has && len <= kTiny;
+    if (tiny) f(std::integral_constant<int, 1>(), item);
+    unsigned long long m = __ballot(has && !tiny);
+    while (m) {
+        const int l = __ffsll(m) - 1;
+        m &= m - 1;
+        f(std::integral_constant<int, W>(), (uint32_t)__shfl((int)item, l, W));
+    }
 }
 
 __device__ __forceinline__ uint32_t first_not_below(const uint32_t* a, uint32_t n, uint32_t bound) {
@@ -95,12 +130,14 @@ struct ChainState {
     uint32_t* pk;    // member prefix length of the K (out) list
     uint32_t* pc;    // member prefix length of the C (in) list
     uint32_t* mark;  // last round tag the vertex was queued for
-    uint32_t* qs0;   // short-list queues (ping-pong by round parity)
-    uint32_t* qs1;
+    uint32_t* qs0;   // short-list queues (ping-pong by round parity): vertices
+    uint32_t* qs1;   //   of at most kTiny entries from the front, the others
+                     //   from the back (qmed)
     uint32_t* ql0;   // long-list queues
     uint32_t* ql1;
-    int32_t* oqs;    // old value of each short-queue entry
+    int32_t* oqs;    // old value of each short-queue entry (same positions)
     int32_t* oql;    // old value of each long-queue entry
+    uint32_t cap;    // words per array
 };
 
 __host__ __device__ __forceinline__ ChainState chain_state(uint32_t* p, uint64_t cap) {
@@ -117,6 +154,7 @@ __host__ __device__ __forceinline__ ChainState chain_state(uint32_t* p, uint64_t
     s.ql1 = p + 9 * cap;
     s.oqs = reinterpret_cast<int32_t*>(p + 10 * cap);
     s.oql = reinterpret_cast<int32_t*>(p + 11 * cap);
+    s.cap = (uint32_t)cap;
     return s;
 }
 static_assert(kScratchArrays >= 12, "a chain needs 12 scratch arrays");
@@ -126,12 +164,16 @@ __device__ __forceinline__ uint32_t* qshort(const ChainState& S, uint32_t par) {
 __device__ __forceinline__ uint32_t* qlong(const ChainState& S, uint32_t par) {
     return par ? S.ql1 : S.ql0;
 }
+// Position of the j-th medium entry in a short queue (and in oqs): the
+// tiny and medium entries of one round are distinct members, so the two
+// ends never meet.
+__device__ __forceinline__ uint32_t qmed(const ChainState& S, uint32_t j) { return S.cap - 1 - j; }
 
-// Slot words of a chain team (kSlotWords per team): [0..1] short-queue
-// lengths, [2..3] long-queue lengths (ping-pong by round parity), [4] row
-// maximum, [5] rounds of the current row, [8..9] list entries the row's
-// walks read (uint64).
-enum : uint32_t { kCqShort = 0, kCqLong = 2, kCqMax = 4, kCqRounds = 5, kCqVisited = 8 };
+// Slot words of a chain team (kSlotWords per team), queue lengths ping-pong
+// by round parity: [0..1] tiny entries, [2..3] long entries, [4] row
+// maximum, [6..7] medium entries, [8..9] list entries the row's walks read
+// (uint64).
+enum : uint32_t { kCqTiny = 0, kCqLong = 2, kCqMax = 4, kCqMed = 6, kCqVisited = 8 };
 __device__ __forceinline__ unsigned long long* visited_word(uint32_t* slots) {
     return reinterpret_cast<unsigned long long*>(slots + kCqVisited);
 }
@@ -155,31 +197,34 @@ __device__ __forceinline__ void enqueue(const ChainState& S, uint32_t* slots, ui
     if (len >= kLongList) {
         const uint32_t at = atomicAdd(&slots[kCqLong + par], 1u);
         stc(&qlong(S, par)[at], y);
+    } else if (len > kTiny) {
+        const uint32_t at = atomicAdd(&slots[kCqMed + par], 1u);
+        stc(&qshort(S, par)[qmed(S, at)], y);
     } else {
-        const uint32_t at = atomicAdd(&slots[kCqShort + par], 1u);
+        const uint32_t at = atomicAdd(&slots[kCqTiny + par], 1u);
         stc(&qshort(S, par)[at], y);
     }
 }
 
 // Counts of v's member neighbours with value >= l (K side, C side), summed
-// over the calling group: a wavefront (lane 0..63 of its lists) or the
-// whole workgroup (BLOCK; red: LDS of kCWaves * 2 words).
-template <bool BLOCK>
+// over the calling group G (kCBlk: red is LDS of kCWaves * 2 words).
+template <int G>
 __device__ __forceinline__ void count_ge(const ChainArgs& A, const ChainState& S, uint32_t v,
                                          int32_t l, uint32_t* red, uint32_t& ck, uint32_t& cc,
                                          uint64_t& vis) {
     const uint32_t nk = ldc(&S.pk[v]), nc = ldc(&S.pc[v]);
-    if (!BLOCK || threadIdx.x == 0) vis += nk + nc;  // flushed by lane 0 of each wave
+    const uint32_t me = gme<G>();
+    if (me == 0) vis += nk + nc;  // summed over the wavefront at the row's end
     const uint32_t* kn = A.K.nbr + A.K.off[v];
     const uint32_t* cn = A.C.nbr + A.C.off[v];
-    const uint32_t step = BLOCK ? kCBlk : W;
-    const uint32_t me = BLOCK ? threadIdx.x : lane_id();
     uint32_t a = 0, b = 0;
-    for (uint32_t j = me; j < nk; j += step) a += ldc(&S.x[kn[j]]) >= l ? 1u : 0u;
-    for (uint32_t j = me; j < nc; j += step) b += ldc(&S.x[cn[j]]) >= l ? 1u : 0u;
-    a = wave_sum(a);
-    b = wave_sum(b);
-    if (BLOCK) {
+    for (uint32_t j = me; j < nk; j += G) a += ldc(&S.x[kn[j]]) >= l ? 1u : 0u;
+    for (uint32_t j = me; j < nc; j += G) b += ldc(&S.x[cn[j]]) >= l ? 1u : 0u;
+    if (G >= W) {
+        a = wave_sum(a);
+        b = wave_sum(b);
+    }
+    if (G == kCBlk) {
         __syncthreads();
         if (lane_id() == 0) {
             red[2 * wave_id()] = a;
@@ -201,41 +246,45 @@ constexpr int kProbe = 4;
 
 // For thresholds t[0..kProbe) (decreasing; an entry <= `below` is unused):
 // counts of v's member neighbours with value >= t[j], K side ck[j] and C
-// side cc[j], summed over the calling group (BLOCK: red holds
+// side cc[j], summed over the calling group G (kCBlk: red holds
 // 2 * kProbe * kCWaves words).
-template <bool BLOCK>
+template <int G>
 __device__ __forceinline__ void count_multi(const ChainArgs& A, const ChainState& S, uint32_t v,
                                             const int32_t (&t)[kProbe], uint32_t* red,
                                             uint32_t (&ck)[kProbe], uint32_t (&cc)[kProbe],
                                             uint64_t& vis) {
     const uint32_t nk = ldc(&S.pk[v]), nc = ldc(&S.pc[v]);
-    if (!BLOCK || threadIdx.x == 0) vis += nk + nc;  // flushed by lane 0 of each wave
+    const uint32_t me = gme<G>();
+    if (me == 0) vis += nk + nc;  // summed over the wavefront at the row's end
     const uint32_t* kn = A.K.nbr + A.K.off[v];
     const uint32_t* cn = A.C.nbr + A.C.off[v];
-    const uint32_t step = BLOCK ? kCBlk : W;
-    const uint32_t me = BLOCK ? threadIdx.x : lane_id();
     // per-lane counts of both sides packed in one word (K low, C high half:
-    // a lane sees at most 2^16 - 1 entries of one list, lists being shorter
-    // than 2^16 times the group's width), unpacked before the group sums
+    // a lane sees at most 2^16 - 1 entries of one list -- lists walked by
+    // a wavefront or workgroup are shorter than 2^16 times its width, by a
+    // thread at most kTiny long), unpacked before the group sums
     uint32_t acc[kProbe];
 #pragma unroll
     for (int j = 0; j < kProbe; ++j) acc[j] = 0;
-    for (uint32_t i = me; i < nk; i += step) {
+    for (uint32_t i = me; i < nk; i += G) {
         const int32_t xv = ldc(&S.x[kn[i]]);
 #pragma unroll
         for (int j = 0; j < kProbe; ++j) acc[j] += xv >= t[j] ? 1u : 0u;
     }
-    for (uint32_t i = me; i < nc; i += step) {
+    for (uint32_t i = me; i < nc; i += G) {
         const int32_t xv = ldc(&S.x[cn[i]]);
 #pragma unroll
         for (int j = 0; j < kProbe; ++j) acc[j] += xv >= t[j] ? 0x10000u : 0u;
     }
 #pragma unroll
     for (int j = 0; j < kProbe; ++j) {
-        ck[j] = wave_sum(acc[j] & 0xffffu);
-        cc[j] = wave_sum(acc[j] >> 16);
+        ck[j] = acc[j] & 0xffffu;
+        cc[j] = acc[j] >> 16;
+        if (G >= W) {
+            ck[j] = wave_sum(ck[j]);
+            cc[j] = wave_sum(cc[j]);
+        }
     }
-    if (BLOCK) {
+    if (G == kCBlk) {
         __syncthreads();
         if (lane_id() == 0) {
 #pragma unroll
@@ -269,7 +318,7 @@ __device__ __forceinline__ void count_multi(const ChainArgs& A, const ChainState
 // the counts of the pass that found nv feasible: a fresh count could read
 // neighbours that fell since and leave a support below its threshold that
 // no decrement crosses.  Returns nv; the caller's group has the same view.
-template <bool BLOCK>
+template <int G>
 __device__ __forceinline__ int32_t evaluate(const ChainArgs& A, const ChainState& S, uint32_t v,
                                             int32_t old, int32_t k, uint32_t* red, uint32_t& os,
                                             uint32_t& cs, uint64_t& vis) {
@@ -289,7 +338,7 @@ __device__ __forceinline__ int32_t evaluate(const ChainArgs& A, const ChainState
         }
         first = false;
         uint32_t ck[kProbe], cc[kProbe];
-        count_multi<BLOCK>(A, S, v, t, red, ck, cc, vis);
+        count_multi<G>(A, S, v, t, red, ck, cc, vis);
         // the largest tested threshold above lo that is feasible
         int jf = -1;
         int32_t below = hi + 1;  // the smallest tested value above lo, if none is feasible
@@ -326,18 +375,17 @@ __device__ __forceinline__ int32_t evaluate(const ChainArgs& A, const ChainState
 
 // v fell from old to nv: neighbours y with nv < x(y) <= old lose v from a
 // support; a support that crosses below its threshold queues y.
-template <bool BLOCK>
+template <int G>
 __device__ __forceinline__ void notify(const ChainArgs& A, const ChainState& S, uint32_t* slots,
                                        uint32_t par, uint32_t v, int32_t old, int32_t nv,
                                        int32_t k, uint32_t tag, uint64_t& vis) {
     const uint32_t nk = ldc(&S.pk[v]), nc = ldc(&S.pc[v]);
-    if (!BLOCK || threadIdx.x == 0) vis += nk + nc;  // flushed by lane 0 of each wave
+    const uint32_t me = gme<G>();
+    if (me == 0) vis += nk + nc;  // summed over the wavefront at the row's end
     const uint32_t* kn = A.K.nbr + A.K.off[v];
     const uint32_t* cn = A.C.nbr + A.C.off[v];
-    const uint32_t step = BLOCK ? kCBlk : W;
-    const uint32_t me = BLOCK ? threadIdx.x : lane_id();
     // u in C-list of v: v is in u's K (out) list -> os(u)
-    for (uint32_t j = me; j < nc; j += step) {
+    for (uint32_t j = me; j < nc; j += G) {
         const uint32_t u = cn[j];
         const int32_t xu = ldc(&S.x[u]);
         if (xu > nv && xu <= old) {
@@ -346,7 +394,7 @@ __device__ __forceinline__ void notify(const ChainArgs& A, const ChainState& S, 
         }
     }
     // w in K-list of v: v is in w's C (in) list -> cs(w)
-    for (uint32_t j = me; j < nk; j += step) {
+    for (uint32_t j = me; j < nk; j += G) {
         const uint32_t w = kn[j];
         const int32_t xw = ldc(&S.x[w]);
         if (xw > nv && xw <= old) {
@@ -358,23 +406,22 @@ __device__ __forceinline__ void notify(const ChainArgs& A, const ChainState& S, 
 
 // r (value xr) leaves the core at row k (new member bound V): its member
 // neighbours drop it from their prefixes and supports.
-template <bool BLOCK>
+template <int G>
 __device__ __forceinline__ void leave(const ChainArgs& A, const ChainState& S, uint32_t r,
                                       uint32_t V, uint64_t& vis) {
     const int32_t xr = ldc(&S.x[r]);
     const uint32_t nk = ldc(&S.pk[r]), nc = ldc(&S.pc[r]);
-    if (!BLOCK || threadIdx.x == 0) vis += nk + nc;  // flushed by lane 0 of each wave
+    const uint32_t me = gme<G>();
+    if (me == 0) vis += nk + nc;  // summed over the wavefront at the row's end
     const uint32_t* kn = A.K.nbr + A.K.off[r];
     const uint32_t* cn = A.C.nbr + A.C.off[r];
-    const uint32_t step = BLOCK ? kCBlk : W;
-    const uint32_t me = BLOCK ? threadIdx.x : lane_id();
-    for (uint32_t j = me; j < nc; j += step) {
+    for (uint32_t j = me; j < nc; j += G) {
         const uint32_t u = cn[j];
         if (u >= V) continue;
         consume(atomicSub(&S.pk[u], 1u));
         if (xr >= ldc(&S.x[u])) consume((uint32_t)atomicSub(&S.os[u], 1));
     }
-    for (uint32_t j = me; j < nk; j += step) {
+    for (uint32_t j = me; j < nk; j += G) {
         const uint32_t w = kn[j];
         if (w >= V) continue;
         consume(atomicSub(&S.pc[w], 1u));
@@ -401,24 +448,28 @@ __global__ __launch_bounds__(256) void k_chain_preset_values(ChainArgs A, ChainS
     }
 }
 
-// Supports at own level for ranks [0, V_1): one wavefront per vertex;
-// vertices with long lists are queued for k_chain_preset_long.
+// Supports at own level for ranks [0, V_1): one thread or wavefront per
+// vertex; vertices with long lists are queued for k_chain_preset_long.
 __global__ __launch_bounds__(kCBlk) void k_chain_preset_supports(ChainArgs A, ChainState S,
                                                                  uint32_t* slots) {
     const uint32_t V1 = A.vk[1];
-    const uint32_t nw = gridDim.x * kCWaves;
     uint64_t vis = 0;  // (not logged: row 1's log counts its own walks only)
-    for (uint32_t r = blockIdx.x * kCWaves + wave_id(); r < V1; r += nw) {
-        if (S.pk[r] + S.pc[r] >= kLongList) {
-            if (lane_id() == 0) S.ql0[atomicAdd(&slots[kCqLong], 1u)] = r;
-            continue;
-        }
-        uint32_t ck, cc;
-        count_ge<false>(A, S, r, S.x[r], nullptr, ck, cc, vis);
-        if (lane_id() == 0) {
-            S.os[r] = (int32_t)ck;
-            S.cs[r] = (int32_t)cc;
-        }
+    for (uint32_t base = (blockIdx.x * kCWaves + wave_id()) * W; base < V1;
+         base += gridDim.x * kCBlk) {
+        const uint32_t r = base + lane_id();
+        const bool has = r < V1;
+        const uint32_t len = has ? S.pk[r] + S.pc[r] : 0u;
+        const bool lng = len >= kLongList;
+        if (lng) S.ql0[atomicAdd(&slots[kCqLong], 1u)] = r;
+        by_lane_or_wave(has && !lng, len, r, [&](auto g, uint32_t rr) {
+            constexpr int G = decltype(g)::value;
+            uint32_t ck, cc;
+            count_ge<G>(A, S, rr, S.x[rr], nullptr, ck, cc, vis);
+            if (gme<G>() == 0) {
+                S.os[rr] = (int32_t)ck;
+                S.cs[rr] = (int32_t)cc;
+            }
+        });
     }
 }
 
@@ -431,7 +482,7 @@ __global__ __launch_bounds__(kCBlk) void k_chain_preset_long(ChainArgs A, ChainS
     for (uint32_t i = blockIdx.x; i < nl; i += gridDim.x) {
         const uint32_t r = S.ql0[i];
         uint32_t ck, cc;
-        count_ge<true>(A, S, r, S.x[r], red, ck, cc, vis);
+        count_ge<kCBlk>(A, S, r, S.x[r], red, ck, cc, vis);
         if (threadIdx.x == 0) {
             S.os[r] = (int32_t)ck;
             S.cs[r] = (int32_t)cc;
@@ -451,10 +502,11 @@ __global__ __launch_bounds__(kCBlk) void k_chain_preset_long(ChainArgs A, ChainS
 __device__ inline bool run_chain(const ChainArgs& A, const ChainState& S, uint32_t* slots,
                                  GridSync& gs, int32_t a, int32_t b, bool preset, uint32_t& tag) {
     __shared__ uint32_t red[2 * kProbe * kCWaves];
-    __shared__ uint32_t s_n[2];
+    __shared__ uint32_t s_n[3];
     const uint32_t tw = gs.rank, tn = gs.nblocks;
     const uint32_t tid = threadIdx.x;
-    const uint32_t gw = tw * kCWaves + wave_id(), nw = tn * kCWaves;
+    const uint32_t gw = tw * kCWaves + wave_id(), nw = tn * kCWaves;  // wavefronts
+    const uint32_t gt = tw * kCBlk + tid, nth = tn * kCBlk;            // threads
     bool ok = true;
     auto barrier = [&]() {
         if (!ok) return;
@@ -463,19 +515,40 @@ __device__ inline bool run_chain(const ChainArgs& A, const ChainState& S, uint32
         else
             __syncthreads();
     };
+    // the queue lengths of parity p (tiny, medium, long), read by every
+    // workgroup after the barrier that ends their filling
+    auto counts = [&](uint32_t p, uint32_t& nt, uint32_t& nm, uint32_t& nl) {
+        if (tid == 0) {
+            s_n[0] = coherent_read(&slots[kCqTiny + p]);
+            s_n[1] = coherent_read(&slots[kCqMed + p]);
+            s_n[2] = coherent_read(&slots[kCqLong + p]);
+        }
+        __syncthreads();
+        nt = s_n[0];
+        nm = s_n[1];
+        nl = s_n[2];
+        __syncthreads();
+    };
+    // a rank whose walk does not fit one thread goes to a queue of parity p
+    auto defer = [&](uint32_t p, uint32_t r, uint32_t len) {
+        if (len < kLongList)
+            stc(&qshort(S, p)[qmed(S, atomicAdd(&slots[kCqMed + p], 1u))], r);
+        else
+            stc(&qlong(S, p)[atomicAdd(&slots[kCqLong + p], 1u)], r);
+    };
     const uint64_t t_chain = wall_clock64();
-    // list entries read by this thread's group (counted by every lane,
-    // added once per wavefront or workgroup at the end of each row; the
+    // list entries read by this thread's walks (a walk is counted by its
+    // first thread; summed per wavefront at the end of each row; the
     // supports' set-up counts towards the first row)
     uint64_t vis = 0;
     const uint32_t Va = A.vk[a];
     uint32_t V = A.vk[a + 1];
     if (preset) {
-        if (tw == 0 && tid < 6) coherent_write(slots + tid, 0u);
+        if (tw == 0 && tid < 8) coherent_write(slots + tid, 0u);
         barrier();
     } else {
         // ---- start from row a: values, prefixes and supports for row a+1
-        for (uint32_t r = tw * kCBlk + tid; r < Va; r += tn * kCBlk) {
+        for (uint32_t r = gt; r < Va; r += nth) {
             stc(&S.x[r], A.rows[(size_t)a * A.V + A.order[r]]);
             stc(&S.mark[r], 0u);
             if (r < V) {
@@ -483,23 +556,21 @@ __device__ inline bool run_chain(const ChainArgs& A, const ChainState& S, uint32
                 stc(&S.pc[r], first_not_below(A.C.nbr + A.C.off[r], (uint32_t)A.C.deg[r], V));
             }
         }
-        if (tw == 0 && tid < 6) coherent_write(slots + tid, 0u);
+        if (tw == 0 && tid < 8) coherent_write(slots + tid, 0u);
         if (tw == 0 && tid == 0) consume((uint32_t)atomicExch(visited_word(slots), 0ull));
         barrier();
-        // supports: one wavefront per vertex; long lists are collected and
-        // counted by whole workgroups
-        for (uint32_t r = gw; r < V && ok; r += nw) {
+        // supports: tiny lists counted by their own thread, the others
+        // queued (parity 0) for wavefronts and workgroups
+        for (uint32_t r = gt; r < V && ok; r += nth) {
             const uint32_t len = ldc(&S.pk[r]) + ldc(&S.pc[r]);
-            if (len >= kLongList) {
-                if (lane_id() == 0) stc(&qlong(S, 0)[atomicAdd(&slots[kCqLong], 1u)], r);
+            if (len > kTiny) {
+                defer(0, r, len);
                 continue;
             }
             uint32_t ck, cc;
-            count_ge<false>(A, S, r, ldc(&S.x[r]), red, ck, cc, vis);
-            if (lane_id() == 0) {
-                stc(&S.os[r], (int32_t)ck);
-                stc(&S.cs[r], (int32_t)cc);
-            }
+            count_ge<1>(A, S, r, ldc(&S.x[r]), red, ck, cc, vis);
+            stc(&S.os[r], (int32_t)ck);
+            stc(&S.cs[r], (int32_t)cc);
         }
         barrier();
         if (!ok) {
@@ -507,64 +578,79 @@ __device__ inline bool run_chain(const ChainArgs& A, const ChainState& S, uint32
             return false;
         }
         {
-            if (tid == 0) s_n[0] = coherent_read(&slots[kCqLong]);
-            __syncthreads();
-            const uint32_t nl = s_n[0];
-            __syncthreads();
+            uint32_t nt, nm, nl;
+            counts(0, nt, nm, nl);
             for (uint32_t i = tw; i < nl; i += tn) {
                 const uint32_t r = ldc(&qlong(S, 0)[i]);
                 uint32_t ck, cc;
-                count_ge<true>(A, S, r, ldc(&S.x[r]), red, ck, cc, vis);
+                count_ge<kCBlk>(A, S, r, ldc(&S.x[r]), red, ck, cc, vis);
                 if (tid == 0) {
                     stc(&S.os[r], (int32_t)ck);
                     stc(&S.cs[r], (int32_t)cc);
                 }
             }
+            for (uint32_t i = gw; i < nm; i += nw) {
+                const uint32_t r = ldc(&qshort(S, 0)[qmed(S, i)]);
+                uint32_t ck, cc;
+                count_ge<W>(A, S, r, ldc(&S.x[r]), red, ck, cc, vis);
+                if (lane_id() == 0) {
+                    stc(&S.os[r], (int32_t)ck);
+                    stc(&S.cs[r], (int32_t)cc);
+                }
+            }
         }
-        // (its counter is cleared by the first round, after the barriers
-        // that every member passes only once it has read it)
+        // (parity 0's counters are cleared before their next use, after
+        // the barriers that every member passes only once it has read them)
         barrier();
     }
     // ---- rows a+1 .. b
     for (int32_t k = a + 1; k <= b && ok; ++k) {
         const uint64_t t_row = k == a + 1 ? t_chain : wall_clock64();
         V = A.vk[k];
-        if (k > a + 1) {
-            const uint32_t Vp = A.vk[k - 1];
-            // 1. leavers: long lists by workgroups, short by wavefronts
-            for (uint32_t r = V + tw; r < Vp; r += tn) {
-                if (ldc(&S.pk[r]) + ldc(&S.pc[r]) >= kLongList) leave<true>(A, S, r, V, vis);
+        const uint32_t Vp = k > a + 1 ? A.vk[k - 1] : V;
+        if (Vp > V) {
+            // 1. leavers [V, Vp): tiny lists by their own thread, the others
+            // queued for wavefronts and workgroups.  The queues of parity
+            // tag & 1 are free and their counters zero: the previous row's
+            // rounds ended on them.
+            const uint32_t lp = tag & 1u;
+            for (uint32_t r = V + gt; r < Vp; r += nth) {
+                const uint32_t len = ldc(&S.pk[r]) + ldc(&S.pc[r]);
+                if (len > kTiny)
+                    defer(lp, r, len);
+                else
+                    leave<1>(A, S, r, V, vis);
             }
-            for (uint32_t r = V + gw; r < Vp; r += nw) {
-                if (ldc(&S.pk[r]) + ldc(&S.pc[r]) < kLongList) leave<false>(A, S, r, V, vis);
-            }
+            barrier();
+            uint32_t nt, nm, nl;
+            counts(lp, nt, nm, nl);
+            for (uint32_t i = tw; i < nl; i += tn) leave<kCBlk>(A, S, ldc(&qlong(S, lp)[i]), V, vis);
+            for (uint32_t i = gw; i < nm; i += nw)
+                leave<W>(A, S, ldc(&qshort(S, lp)[qmed(S, i)]), V, vis);
             barrier();
         }
         // 2. violators of row k
         ++tag;
         uint32_t par = tag & 1u;
         if (tw == 0 && tid == 0) {
-            coherent_write(&slots[kCqShort + par], 0u);
+            coherent_write(&slots[kCqTiny + par], 0u);
+            coherent_write(&slots[kCqMed + par], 0u);
             coherent_write(&slots[kCqLong + par], 0u);
             coherent_write(&slots[kCqMax], 0u);
         }
         barrier();
-        for (uint32_t r = tw * kCBlk + tid; r < V; r += tn * kCBlk) {
+        for (uint32_t r = gt; r < V; r += nth) {
             const int32_t xr = ldc(&S.x[r]);
             if (ldc(&S.os[r]) < xr || ldc(&S.cs[r]) < k) enqueue(S, slots, par, r, tag);
         }
         barrier();
+        // 3. rounds
         uint32_t rounds = 0;
         for (;;) {
             if (!ok) break;
-            if (tid == 0) {
-                s_n[0] = coherent_read(&slots[kCqShort + par]);
-                s_n[1] = coherent_read(&slots[kCqLong + par]);
-            }
-            __syncthreads();
-            const uint32_t ns = s_n[0], nl = s_n[1];
-            __syncthreads();
-            if (ns + nl == 0) break;
+            uint32_t nt, nm, nl;
+            counts(par, nt, nm, nl);
+            if (nt + nm + nl == 0) break;
             // every round lowers a value (spurious queueing aside, which
             // one exact re-count ends): more rounds than twice the
             // members means the supports are wrong
@@ -575,15 +661,17 @@ __device__ inline bool run_chain(const ChainArgs& A, const ChainState& S, uint32
             }
             const uint32_t ntag = tag + 1, npar = ntag & 1u;
             if (tw == 0 && tid == 0) {
-                coherent_write(&slots[kCqShort + npar], 0u);
+                coherent_write(&slots[kCqTiny + npar], 0u);
+                coherent_write(&slots[kCqMed + npar], 0u);
                 coherent_write(&slots[kCqLong + npar], 0u);
             }
-            // A. evaluate
+            // A. evaluate (long lists by workgroups, medium by wavefronts,
+            // tiny by threads)
             for (uint32_t i = tw; i < nl; i += tn) {
                 const uint32_t v = ldc(&qlong(S, par)[i]);
                 const int32_t old = ldc(&S.x[v]);
                 uint32_t os, cs;
-                const int32_t nv = evaluate<true>(A, S, v, old, k, red, os, cs, vis);
+                const int32_t nv = evaluate<kCBlk>(A, S, v, old, k, red, os, cs, vis);
                 if (tid == 0) {
                     stc(&S.os[v], (int32_t)os);
                     stc(&S.cs[v], (int32_t)cs);
@@ -592,29 +680,46 @@ __device__ inline bool run_chain(const ChainArgs& A, const ChainState& S, uint32
                 }
                 __syncthreads();
             }
-            for (uint32_t i = gw; i < ns; i += nw) {
-                const uint32_t v = ldc(&qshort(S, par)[i]);
+            for (uint32_t i = gw; i < nm; i += nw) {
+                const uint32_t at = qmed(S, i);
+                const uint32_t v = ldc(&qshort(S, par)[at]);
                 const int32_t old = ldc(&S.x[v]);
                 uint32_t os, cs;
-                const int32_t nv = evaluate<false>(A, S, v, old, k, red, os, cs, vis);
+                const int32_t nv = evaluate<W>(A, S, v, old, k, red, os, cs, vis);
                 if (lane_id() == 0) {
                     stc(&S.os[v], (int32_t)os);
                     stc(&S.cs[v], (int32_t)cs);
-                    stc(&S.oqs[i], old);
+                    stc(&S.oqs[at], old);
                     stc(&S.x[v], nv);
                 }
+            }
+            for (uint32_t i = gt; i < nt; i += nth) {
+                const uint32_t v = ldc(&qshort(S, par)[i]);
+                const int32_t old = ldc(&S.x[v]);
+                uint32_t os, cs;
+                const int32_t nv = evaluate<1>(A, S, v, old, k, red, os, cs, vis);
+                stc(&S.os[v], (int32_t)os);
+                stc(&S.cs[v], (int32_t)cs);
+                stc(&S.oqs[i], old);
+                stc(&S.x[v], nv);
             }
             barrier();
             // B. notify
             for (uint32_t i = tw; i < nl; i += tn) {
                 const uint32_t v = ldc(&qlong(S, par)[i]);
                 const int32_t old = ldc(&S.oql[i]), nv = ldc(&S.x[v]);
-                if (nv < old) notify<true>(A, S, slots, npar, v, old, nv, k, ntag, vis);
+                if (nv < old) notify<kCBlk>(A, S, slots, npar, v, old, nv, k, ntag, vis);
             }
-            for (uint32_t i = gw; i < ns; i += nw) {
+            for (uint32_t i = gw; i < nm; i += nw) {
+                const uint32_t at = qmed(S, i);
+                const uint32_t v = ldc(&qshort(S, par)[at]);
+                const int32_t old = ldc(&S.oqs[at]), nv = ldc(&S.x[v]);
+                if (nv < old) notify<W>(A, S, slots, npar, v, old, nv, k, ntag, vis);
+            }
+            for (uint32_t i = gt; i < nt; i += nth) {
                 const uint32_t v = ldc(&qshort(S, par)[i]);
                 const int32_t old = ldc(&S.oqs[i]), nv = ldc(&S.x[v]);
-                if (nv < old) notify<false>(A, S, slots, npar, v, old, nv, k, ntag, vis);
+                if (nv < old) notify<1>(A, S, slots, npar, v, old, nv, k, ntag, vis);
             }
             barrier();
             tag = ntag;
@@ -623,7 +728,7 @@ __device__ inline bool run_chain(const ChainArgs& A, const ChainState& S, uint32
         if (!ok) break;
         // 4. the row, and its maximum
         int32_t mx = 0;
-        for (uint32_t r = tw * kCBlk + tid; r < V; r += tn * kCBlk) {
+        for (uint32_t r = gt; r < V; r += nth) {
             const int32_t xr = ldc(&S.x[r]);
             A.rows[(size_t)k * A.V + A.order[r]] = xr;
             mx = max(mx, xr);
@@ -631,7 +736,7 @@ __device__ inline bool run_chain(const ChainArgs& A, const ChainState& S, uint32
 #pragma unroll
         for (int d = W / 2; d > 0; d >>= 1) mx = max(mx, __shfl_xor(mx, d, W));
         if (lane_id() == 0 && mx > 0) consume((uint32_t)atomicMax(&slots[kCqMax], (uint32_t)mx));
-        // wavefront walks counted on lane 0, workgroup walks on thread 0
+        vis = wave_sum64(vis);
         if (lane_id() == 0 && vis) consume((uint32_t)atomicAdd(visited_word(slots), vis));
         vis = 0;
         barrier();
