@@ -288,11 +288,12 @@ def oracle_sample(g, V, threads, log):
 
 def train_learned_plan(dcx, ctx, g, log, Xs=None, Ts=None):
     """The cost model's training protocol on graph g (untimed): the pass log
-    of the fixed rule's last run (already decomposed by the caller), two
-    exploration runs (every 8th team row peeled, team caps 32 and 4, so the
-    samples cover narrow teams too), then two refinement rounds in which the
-    model plans a run and is retrained with that run's log.  Returns the
-    attached model, the sample count and the refinement runs' times."""
+    of the fixed rule's last run (already decomposed by the caller), three
+    exploration runs (every 8th team row an anchor: peeled with team caps 32
+    and 4, so the samples cover narrow teams too, and derived from row 0),
+    then two refinement rounds in which the model plans a run and is
+    retrained with that run's log.  Returns the attached model, the sample
+    count and the refinement runs' times."""
     Xs = list(Xs or [])
     Ts = list(Ts or [])
 
@@ -303,13 +304,14 @@ def train_learned_plan(dcx, ctx, g, log, Xs=None, Ts=None):
             Ts.append(T)
 
     collect()
-    for spacing, tmax in ((8, 0), (8, 4)):
-        ctx.set_rows_rule(spacing, tmax)
+    for spacing, tmax, far in ((8, 0, 0), (8, 4, 0), (8, 0, 1)):
+        ctx.set_rows_rule(spacing, tmax, far)
         g.decompose(dcx.VARIANT_BATCHED)
-        log(f"exploration run (every {spacing}th row peeled, team cap {tmax or 32}): "
+        kind = "derived from row 0" if far else "peeled"
+        log(f"exploration run (every {spacing}th row an anchor, {kind}, team cap {tmax or 32}): "
             f"{ctx.timing().total_ms:.1f} ms")
         collect()
-    ctx.set_rows_rule(0, 0)
+    ctx.set_rows_rule(0, 0, 0)
     if sum(len(t) for t in Ts) == 0:
         log("no team rows to learn from (every row ran whole-graph or LDS-resident): "
             "the fixed rule stays")
@@ -363,10 +365,11 @@ def run_headline(args):
 
     # the fixed rule (every 32nd team row peeled, the rest derived from the
     # row above; widths by work share), then the cost model trained on the
-    # GPU from what the decompositions logged: the fixed rule's runs, two
-    # exploration runs (every 8th row peeled, teams of up to 32 and of up
-    # to 4 workgroups), and two refinement rounds in which the model plans
-    # a run and is retrained with that run's log as well
+    # GPU from what the decompositions logged: the fixed rule's runs, three
+    # exploration runs (every 8th row an anchor: peeled on teams of up to 32
+    # and of up to 4 workgroups, and derived from row 0), and two refinement
+    # rounds in which the model plans a run and is retrained with that run's
+    # log as well
     ctx.set_scheduler(None)
     g.decompose(dcx.VARIANT_BATCHED)
     heur = []
@@ -468,8 +471,9 @@ def run_headline(args):
             if fr_traffic and fr_alg else None),
         "schedule": {"kind": args.schedule, "model_used": used_model,
                      "training": f"{n_samples} samples: this graph's logged team rows under the "
-                                 "fixed rule, two exploration runs (every 8th row peeled, team "
-                                 "caps 32 and 4) and two refinement rounds planned by the model" + (
+                                 "fixed rule, three exploration runs (every 8th row an anchor: "
+                                 "peeled with team caps 32 and 4, derived from row 0) and two "
+                                 "refinement rounds planned by the model" + (
                                      ", and smaller graphs' at several team caps"
                                      if args.full_train else ""),
                      "refinement_rounds_ms": refine_ms,

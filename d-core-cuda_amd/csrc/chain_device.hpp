@@ -434,17 +434,16 @@ __device__ __forceinline__ void leave(const ChainArgs& A, const ChainState& S, u
 // (row 0 is ready by then; a team would do the same work on a sixteenth of
 // the chip).  S is the chain's team scratch, slots its team slots (zeroed).
 
-// x, marks and member prefixes for row 1 (one thread per rank).
+// x, marks and member prefixes for row 1 (one thread per member rank).
 __global__ __launch_bounds__(256) void k_chain_preset_values(ChainArgs A, ChainState S) {
-    const uint32_t V0 = A.vk[0], V1 = A.vk[1];
-    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < V0;
+    const uint32_t V1 = A.vk[1];
+    for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < V1;
          r += gridDim.x * blockDim.x) {
-        S.x[r] = A.rows[A.order[r]];
+        const uint32_t pk = first_not_below(A.K.nbr + A.K.off[r], (uint32_t)A.K.deg[r], V1);
+        S.x[r] = min(A.rows[A.order[r]], (int32_t)pk);  // (both bound row 1)
         S.mark[r] = 0u;
-        if (r < V1) {
-            S.pk[r] = first_not_below(A.K.nbr + A.K.off[r], (uint32_t)A.K.deg[r], V1);
-            S.pc[r] = first_not_below(A.C.nbr + A.C.off[r], (uint32_t)A.C.deg[r], V1);
-        }
+        S.pk[r] = pk;
+        S.pc[r] = first_not_below(A.C.nbr + A.C.off[r], (uint32_t)A.C.deg[r], V1);
     }
 }
 
@@ -490,17 +489,23 @@ __global__ __launch_bounds__(kCBlk) void k_chain_preset_long(ChainArgs A, ChainS
     }
 }
 
-// Derives rows a+1..b from row a on the calling team (gs: its barrier
-// words, rank and size).  With `preset`, the chain's values, prefixes and
+// Derives rows a+1..b on the calling team (gs: its barrier words, rank and
+// size), starting from row src <= a restricted to the (a+1,0)-core (an upper
+// bound of row a+1, since D_{a+1,l} lies inside D_{src,l}; normally src = a,
+// the row above).  With `preset`, the chain's values, prefixes and
 // supports for row a+1 are already in S (row 0's chain, set up by the
 // k_chain_preset_* kernels).  `tag` carries the round tags from chain to
 // chain of the team (they only grow).  Returns false when the team stopped
 // on a timeout or on inconsistent supports (reported through A.err).  The
 // pass-log entry of each derived row gets its rounds as phases, levels = 1
 // for the first row of a chain whose time includes the supports' set-up
-// (not preset) and 0 for the others.
+// from the row above (not preset), 2 for such a row started from an earlier
+// row (src < a), and 0 for the others.  Every start value is also capped by
+// the vertex's member out-degree (its prefix pk): v keeps at least l_{a+1}(v)
+// member out-neighbours in D_{a+1,l}, so that is an upper bound too.
 __device__ inline bool run_chain(const ChainArgs& A, const ChainState& S, uint32_t* slots,
-                                 GridSync& gs, int32_t a, int32_t b, bool preset, uint32_t& tag) {
+                                 GridSync& gs, int32_t a, int32_t b, int32_t src, bool preset,
+                                 uint32_t& tag) {
     __shared__ uint32_t red[2 * kProbe * kCWaves];
     __shared__ uint32_t s_n[3];
     const uint32_t tw = gs.rank, tn = gs.nblocks;
@@ -541,20 +546,19 @@ __device__ inline bool run_chain(const ChainArgs& A, const ChainState& S, uint32
     // first thread; summed per wavefront at the end of each row; the
     // supports' set-up counts towards the first row)
     uint64_t vis = 0;
-    const uint32_t Va = A.vk[a];
     uint32_t V = A.vk[a + 1];
     if (preset) {
         if (tw == 0 && tid < 8) coherent_write(slots + tid, 0u);
         barrier();
     } else {
-        // ---- start from row a: values, prefixes and supports for row a+1
-        for (uint32_t r = gt; r < Va; r += nth) {
-            stc(&S.x[r], A.rows[(size_t)a * A.V + A.order[r]]);
+        // ---- start from row src: values, prefixes and supports for row a+1
+        // (members of row a+1 only: the scratch holds V_{a+1} words at least)
+        for (uint32_t r = gt; r < V; r += nth) {
+            const uint32_t pk = first_not_below(A.K.nbr + A.K.off[r], (uint32_t)A.K.deg[r], V);
+            stc(&S.x[r], min(A.rows[(size_t)src * A.V + A.order[r]], (int32_t)pk));
             stc(&S.mark[r], 0u);
-            if (r < V) {
-                stc(&S.pk[r], first_not_below(A.K.nbr + A.K.off[r], (uint32_t)A.K.deg[r], V));
-                stc(&S.pc[r], first_not_below(A.C.nbr + A.C.off[r], (uint32_t)A.C.deg[r], V));
-            }
+            stc(&S.pk[r], pk);
+            stc(&S.pc[r], first_not_below(A.C.nbr + A.C.off[r], (uint32_t)A.C.deg[r], V));
         }
         if (tw == 0 && tid < 8) coherent_write(slots + tid, 0u);
         if (tw == 0 && tid == 0) consume((uint32_t)atomicExch(visited_word(slots), 0ull));
@@ -748,7 +752,7 @@ __device__ inline bool run_chain(const ChainArgs& A, const ChainState& S, uint32
                 uint32_t* st = A.stats + (size_t)kStatWords * k;
                 const uint64_t dt = wall_clock64() - t_row;
                 st[kStatPhases] = rounds;
-                st[kStatLevels] = k == a + 1 && !preset ? 1u : 0u;
+                st[kStatLevels] = k == a + 1 && !preset ? (src < a ? 2u : 1u) : 0u;
                 st[kStatTicksLo] = (uint32_t)dt;
                 st[kStatTicksHi] = (uint32_t)(dt >> 32);
                 st[kStatFailed] = 0;

@@ -5,10 +5,13 @@
 // V_k exactly when kmax >= k) -- a k-pass touches exactly the edges of the
 // core (E_k per direction) and needs no membership test.  Built once per
 // decomposition, after the kmax pass: the lists are moved to their new
-// owners' places and ordered there by a counting sort on kmax (a wavefront
-// per short list, stable; a workgroup per long list, ties in no fixed
-// order).  Graphs whose kmax exceeds the LDS histogram sort every list
-// fully by new id with one radix sort per direction instead.
+// owners' places and ordered there.  A list of at most 64 entries is
+// sorted by new id on one wavefront (new-id order is kmax order, so the
+// neighbour's kmax need not be read); a longer one by a counting sort on
+// kmax in one workgroup (ties in no fixed order), from one 8-byte gather
+// per entry of a packed (kmax, new id) word.  Graphs whose kmax exceeds the
+// LDS histogram sort every list fully by new id with one radix sort per
+// direction instead.
 #include <hip/hip_runtime.h>
 
 #include "dcx_internal.hpp"
@@ -23,6 +26,16 @@ __device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (W - 1)); 
 __global__ void k_rank(const uint32_t* order, uint32_t V, uint32_t* rank) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < V; i += gridDim.x * blockDim.x)
         rank[order[i]] = i;
+}
+
+// rank[v] = new id of v and kr[v] = (kmax(v) << 32) | new id.
+__global__ void k_rank_packed(const uint32_t* order, const int32_t* kmaxes, uint32_t V,
+                              uint32_t* rank, uint64_t* kr) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < V; i += gridDim.x * blockDim.x) {
+        const uint32_t v = order[i];
+        rank[v] = i;
+        kr[v] = ((uint64_t)(uint32_t)kmaxes[v] << 32) | i;
+    }
 }
 
 __global__ void k_new_degrees(const uint32_t* order, uint32_t V, const int32_t* deg,
@@ -59,14 +72,14 @@ constexpr uint32_t kTinyList = 16;
 constexpr int kListBins = 4096;  // kmax + 1 at most, for the counting sort
 constexpr int kLongBlk = 256;
 
-// Place of a list entry with key kk held by lane l of a group of `width`
-// lanes holding the list's n entries: #{j : key_j > kk} + #{j < l : key_j
-// == kk} (kmax descending, ties in list order).
-__device__ __forceinline__ uint32_t list_place(int32_t kk, uint32_t l, uint32_t n, int width) {
+// Place of a list entry with new id r held by lane l of a wavefront
+// holding the list's n entries: #{j : r_j < r} + #{j < l : r_j == r}
+// (ascending, repeated entries in list order).
+__device__ __forceinline__ uint32_t list_place(uint32_t r, uint32_t l, uint32_t n) {
     uint32_t pos = 0;
     for (uint32_t j = 0; j < n; ++j) {
-        const int32_t kj = __shfl(kk, (int)j, width);
-        pos += (kj > kk || (kj == kk && j < l)) ? 1u : 0u;
+        const uint32_t rj = (uint32_t)__shfl((int)r, (int)j, W);
+        pos += (rj < r || (rj == r && j < l)) ? 1u : 0u;
     }
     return pos;
 }
@@ -75,7 +88,7 @@ __device__ __forceinline__ uint32_t list_place(int32_t kk, uint32_t l, uint32_t 
 // new owners at a time, one per 16-lane quarter when all four lists have at
 // most kTinyList entries, else one list after another on all 64 lanes.
 // Longer lists are queued (new owner id) for k_lists_long.
-__global__ void k_lists_short(const uint32_t* order, const uint32_t* rank, const int32_t* kmaxes,
+__global__ void k_lists_short(const uint32_t* order, const uint32_t* rank,
                               uint32_t V, const uint32_t* off, const int32_t* deg,
                               const uint32_t* nbr, const uint32_t* off_new, uint32_t* out,
                               uint32_t* longq, uint32_t* longn) {
@@ -92,15 +105,14 @@ __global__ void k_lists_short(const uint32_t* order, const uint32_t* rank, const
         if (tiny) {
             const bool has = l < nq;
             const uint32_t u = has ? nbr[off[vq] + l] : 0u;
-            const int32_t kk = has ? kmaxes[u] : -1;
             const uint32_t r = has ? rank[u] : 0u;
-            // every quarter runs the longest list's count (shorter lists
-            // compare padding, key -1, which sorts last and is not written)
+            // every quarter runs the longest list's count (entries past a
+            // shorter list's end are not counted, nor written)
             const uint32_t nmax = __reduce_max_sync(~0ull, nq);
             uint32_t pos = 0;
             for (uint32_t j = 0; j < nmax; ++j) {
-                const int32_t kj = __shfl(kk, (int)j, (int)kTinyList);
-                pos += (j < nq && (kj > kk || (kj == kk && j < l))) ? 1u : 0u;
+                const uint32_t rj = (uint32_t)__shfl((int)r, (int)j, (int)kTinyList);
+                pos += (j < nq && (rj < r || (rj == r && j < l))) ? 1u : 0u;
             }
             if (has) out[off_new[iq] + pos] = r;
             continue;
@@ -115,20 +127,23 @@ __global__ void k_lists_short(const uint32_t* order, const uint32_t* rank, const
             }
             const bool has = lane < n;
             const uint32_t u = has ? nbr[off[v] + lane] : 0u;
-            const int32_t kk = has ? kmaxes[u] : -1;
             const uint32_t r = has ? rank[u] : 0u;
-            const uint32_t pos = list_place(kk, lane, n, W);
+            const uint32_t pos = list_place(r, lane, n);
             if (has) out[off_new[i] + pos] = r;
         }
     }
 }
 
+// Entries of a long list each thread keeps in registers between the
+// histogram and the placement (longer lists gather the rest twice).
+constexpr int kLongRegs = 8;
+
 // The queued long lists, one workgroup each: LDS histogram of the
 // neighbours' kmax (and its range), start of every bin of the range in
 // descending order (kLongBlk bins per step), then each entry placed at its
-// bin's cursor; the range's bins are cleared for the next list.
-__global__ __launch_bounds__(kLongBlk) void k_lists_long(const uint32_t* order, const uint32_t* rank,
-                                                         const int32_t* kmaxes,
+// bin's cursor; the range's bins are cleared for the next list.  kr: the
+// packed (kmax, new id) word of every vertex.
+__global__ __launch_bounds__(kLongBlk) void k_lists_long(const uint32_t* order, const uint64_t* kr,
                                                          const uint32_t* off, const int32_t* deg,
                                                          const uint32_t* nbr,
                                                          const uint32_t* off_new, uint32_t* out,
@@ -153,8 +168,22 @@ __global__ __launch_bounds__(kLongBlk) void k_lists_long(const uint32_t* order, 
         const uint32_t* l = nbr + off[v];
         uint32_t* dst = out + off_new[i];
         int32_t lo = kListBins, hi = -1;
-        for (uint32_t j = tid; j < n; j += kLongBlk) {
-            const int32_t k = kmaxes[l[j]];
+        uint64_t held[kLongRegs];
+#pragma unroll
+        for (int h = 0; h < kLongRegs; ++h) {
+            const uint32_t j = tid + (uint32_t)h * kLongBlk;
+            held[h] = j < n ? kr[l[j]] : 0ull;
+        }
+#pragma unroll
+        for (int h = 0; h < kLongRegs; ++h) {
+            if (tid + (uint32_t)h * kLongBlk >= n) break;
+            const int32_t k = (int32_t)(held[h] >> 32);
+            atomicAdd(&cnt[k], 1u);
+            lo = min(lo, k);
+            hi = max(hi, k);
+        }
+        for (uint32_t j = tid + kLongRegs * kLongBlk; j < n; j += kLongBlk) {
+            const int32_t k = (int32_t)(kr[l[j]] >> 32);
             atomicAdd(&cnt[k], 1u);
             lo = min(lo, k);
             hi = max(hi, k);
@@ -185,9 +214,14 @@ __global__ __launch_bounds__(kLongBlk) void k_lists_long(const uint32_t* order, 
             if (tid == kLongBlk - 1) s_carry = before + inc;
             __syncthreads();
         }
-        for (uint32_t j = tid; j < n; j += kLongBlk) {
-            const uint32_t u = l[j];
-            dst[atomicAdd(&cnt[kmaxes[u]], 1u)] = rank[u];
+#pragma unroll
+        for (int h = 0; h < kLongRegs; ++h) {
+            if (tid + (uint32_t)h * kLongBlk >= n) break;
+            dst[atomicAdd(&cnt[(int32_t)(held[h] >> 32)], 1u)] = (uint32_t)held[h];
+        }
+        for (uint32_t j = tid + kLongRegs * kLongBlk; j < n; j += kLongBlk) {
+            const uint64_t w = kr[l[j]];
+            dst[atomicAdd(&cnt[(int32_t)(w >> 32)], 1u)] = (uint32_t)w;
         }
         __syncthreads();
         for (int b = lo + (int)tid; b <= hi; b += kLongBlk) cnt[b] = 0;
@@ -250,8 +284,9 @@ int bits_for(uint32_t v) {
 }  // namespace
 
 size_t core_graph_tmp_bytes(uint32_t V, uint64_t E, int32_t kmax) {
-    if (kmax + 1 <= kListBins)  // scan words + long-list queue and its length
-        return 4ull * (scan_tmp_words(V) + 1) + 4ull * ((E ? E : 1) / kShortList + 64) + 256;
+    if (kmax + 1 <= kListBins)  // packed words + scan words + long-list queue and its length
+        return 8ull * V + 4ull * (scan_tmp_words(V) + 1) + 4ull * ((E ? E : 1) / kShortList + 64) +
+               256;
     // keys + alternate keys + scan words + radix table
     return 16ull * (E ? E : 1) + 4ull * (scan_tmp_words(V) + 1) +
            4ull * (radix_tmp_words(E) + 1) + 256;
@@ -271,10 +306,12 @@ hipError_t build_core_graph(uint32_t V, uint64_t E, const uint32_t* order, const
     if (V == 0) return hipSuccess;
     char* p = static_cast<char*>(tmp);
     if (kmax + 1 <= kListBins) {
-        uint32_t* scan_tmp = reinterpret_cast<uint32_t*>(p);
+        uint64_t* kr = reinterpret_cast<uint64_t*>(p);
+        uint32_t* scan_tmp = reinterpret_cast<uint32_t*>(kr + V);
         uint32_t* longn = scan_tmp + scan_tmp_words(V) + 1;
         uint32_t* longq = longn + 64;  // 256-byte aligned from longn
-        hipLaunchKernelGGL(k_rank, dim3(grid_for(V, 256)), dim3(256), 0, st, order, V, rank);
+        hipLaunchKernelGGL(k_rank_packed, dim3(grid_for(V, 256)), dim3(256), 0, st, order, kmaxes,
+                           V, rank, kr);
         const Csr sides[2] = {out, in};
         uint32_t* offs[2] = {cout_off, cin_off};
         int32_t* degs[2] = {cout_deg, cin_deg};
@@ -289,11 +326,11 @@ hipError_t build_core_graph(uint32_t V, uint64_t E, const uint32_t* order, const
             if (E == 0) continue;
             hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, st, longn, 0u);
             hipLaunchKernelGGL(k_lists_short, dim3(grid_for(V, 64)), dim3(1024), 0, st, order,
-                               rank, kmaxes, V, sides[s].off, sides[s].deg, sides[s].nbr, offs[s],
+                               rank, V, sides[s].off, sides[s].deg, sides[s].nbr, offs[s],
                                nbrs[s], longq, longn);
-            hipLaunchKernelGGL(k_lists_long, dim3(4096), dim3(kLongBlk), 0, st, order, rank,
-                               kmaxes, sides[s].off, sides[s].deg, sides[s].nbr, offs[s],
-                               nbrs[s], longq, longn);
+            hipLaunchKernelGGL(k_lists_long, dim3(4096), dim3(kLongBlk), 0, st, order, kr,
+                               sides[s].off, sides[s].deg, sides[s].nbr, offs[s], nbrs[s], longq,
+                               longn);
         }
         return hipGetLastError();
     }

@@ -565,7 +565,11 @@ Sched make_schedule(const dcx_graph* g, int32_t k_first, int32_t kr, uint32_t G,
     return s;
 }
 
-enum : int { kPassPeel = 0, kPassFirst = 1, kPassDerived = 2 };
+// Kinds of team row: peeled; derived from the row above as the first row
+// of a chain (its time includes the supports' set-up); derived later in a
+// chain; derived from row 0 as the first row of a chain (an anchor that is
+// not peeled).
+enum : int { kPassPeel = 0, kPassFirst = 1, kPassDerived = 2, kPassFar = 3, kPassKinds = 4 };
 
 // Kind of row k's last run in the batched launch (-1: none, or another
 // variant), from the pass log.
@@ -574,7 +578,8 @@ int last_pass_kind(const dcx_graph* g, int32_t k) {
     if (g->h_pass_variant[k + 1] == DCX_VARIANT_BATCHED) return k >= g->rows_first_grid ? 0 : -1;
     if (g->h_pass_variant[k + 1] != DCX_VARIANT_DERIVED) return -1;
     if (g->h_pstats.size() < ((size_t)k + 2) * kStatWords) return -1;
-    return g->h_pstats[(size_t)(k + 1) * kStatWords + kStatLevels] ? kPassFirst : kPassDerived;
+    const uint32_t lv = g->h_pstats[(size_t)(k + 1) * kStatWords + kStatLevels];
+    return lv == 2 ? kPassFar : lv ? kPassFirst : kPassDerived;
 }
 
 // Folds the team rows of the last batched launch (rows 0..kr) into the
@@ -584,6 +589,8 @@ void record_row_stats(dcx_graph* g, int32_t kr) {
     g->h_peel_phases.resize(n, 0);
     g->h_der_rounds.resize(n, 0);
     g->h_der_reads.resize(n, 0);
+    g->h_far_rounds.resize(n, 0);
+    g->h_far_reads.resize(n, 0);
     for (int32_t k = 0; k <= kr && (size_t)k < n; ++k) {
         const int kind = last_pass_kind(g, k);
         if (kind < 0) continue;
@@ -594,9 +601,14 @@ void record_row_stats(dcx_graph* g, int32_t kr) {
             // a first row's walks include the supports' set-up: one pass
             // over both member prefixes of every member, 2 E_k entries
             uint64_t reads = w[kStatAlive];
-            if (kind == kPassFirst) reads -= std::min<uint64_t>(reads, 2 * g->h_ek[k]);
-            g->h_der_rounds[k] = w[kStatPhases] + 1;
-            g->h_der_reads[k] = reads + 1;
+            if (kind != kPassDerived) reads -= std::min<uint64_t>(reads, 2 * g->h_ek[k]);
+            if (kind == kPassFar) {
+                g->h_far_rounds[k] = w[kStatPhases] + 1;
+                g->h_far_reads[k] = reads + 1;
+            } else {
+                g->h_der_rounds[k] = w[kStatPhases] + 1;
+                g->h_der_reads[k] = reads + 1;
+            }
         }
     }
 }
@@ -615,13 +627,14 @@ T row_stat(const std::vector<T>& v, int32_t k) {
 
 // Scheduler features of row k run as `kind` (kPassPeel: peeled; kPassFirst:
 // derived from the row above as the first row of a chain, its time
-// including the supports' set-up; kPassDerived: derived, later in a chain)
-// on a team of `team` workgroups (kMlpIn floats, about [0, 1]): sizes of the
-// (k,0)-core and the vertices that left it since row k-1, the kind, the
-// width, and the work row k took when it last ran that way (phases of a
-// peel; rounds and list entries read of a derivation, plus the set-up's
-// 2 E_k entries for a first row), from this graph's batched launches (or
-// the nearest row's; with_stats = false: those two as unknown).
+// including the supports' set-up; kPassDerived: derived, later in a chain;
+// kPassFar: derived from row 0 as the first row of a chain) on a team of
+// `team` workgroups (kMlpIn floats, about [0, 1]): sizes of the (k,0)-core
+// and the vertices that left it since row k-1, the kind, the width, and the
+// work row k took when it last ran that way (phases of a peel; rounds and
+// list entries read of a derivation, plus the set-up's 2 E_k entries for a
+// first row), from this graph's batched launches (or the nearest row's;
+// with_stats = false: those two as unknown).
 void pass_features(const dcx_graph* g, int32_t k, uint32_t team, int kind, bool with_stats,
                    float* f) {
     const double Vk = g->h_vk[k], Ek = (double)g->h_ek[k];
@@ -633,17 +646,18 @@ void pass_features(const dcx_graph* g, int32_t k, uint32_t team, int kind, bool 
             steps = ph ? ph - 1.0 : 0.0;
             reads = 2.0 * Ek;
         } else {
-            const uint32_t r = row_stat(g->h_der_rounds, k);
-            const uint64_t e = row_stat(g->h_der_reads, k);
+            const bool far = kind == kPassFar;
+            const uint32_t r = row_stat(far ? g->h_far_rounds : g->h_der_rounds, k);
+            const uint64_t e = row_stat(far ? g->h_far_reads : g->h_der_reads, k);
             steps = r ? r - 1.0 : 0.0;
             reads = e ? (double)(e - 1) : 0.0;
-            if (kind == kPassFirst) reads += 2.0 * Ek;
+            if (kind != kPassDerived) reads += 2.0 * Ek;
         }
     }
     f[0] = (float)(std::log2(1.0 + Vk) / 24.0);
     f[1] = (float)(std::log2(1.0 + Ek) / 32.0);
     f[2] = (float)(std::log2(1.0 + Vp - Vk) / 24.0);
-    f[3] = (float)kind * 0.5f;
+    f[3] = (float)kind / 3.0f;
     f[4] = (float)(std::log2((double)std::max(team, 1u)) / 8.0);
     f[5] = (float)(std::log2(1.0 + steps) / 12.0);
     f[6] = (float)(std::log2(1.0 + reads) / 36.0);
@@ -681,23 +695,28 @@ std::vector<float> mlp_predict(dcx_ctx* c, dcx_mlp* m, const std::vector<float>&
 }
 
 // Plan of the team rows 1..kr (row 0 is ready before the launch) from the
-// cost model.  A unit is an anchor a (row 0, or a row peeled by the unit's
-// team) with the chain a+1..b derived from it on the same team; its
-// predicted time at width t is peel(a, t) (0 for row 0) + first(a+1, t) +
-// sum over k = a+2..b of derived(k, t).  For a makespan T the rows are cut
-// into units from the top, each extended while some width keeps it within
-// T, and each unit takes the narrowest width that does; T is the smallest
-// value (binary search) for which the wide units fit 3/4 of the grid (all
-// of it when no unit is left for a single workgroup) and the single-
-// workgroup units, packed longest first, fit the rest.  With `fixed` set,
-// peel/chain_last are given and only the widths are chosen.  Outputs:
-// peel[k] (row k is an anchor), chain_last[a] (last row of a's chain, -1:
-// none), and per anchor the width want[a] and the weight wts[a] (predicted
-// workgroup time, which places the units longest first).  Returns false
-// if no plan fits.
+// cost model.  A unit is an anchor a with the chain a+1..b derived from it
+// on the same team: row 0 (ready before the launch), a row the team peels,
+// or a row the team derives from row 0 (far[a]).  Its predicted time at
+// width t is, for a peeled anchor, peel(a, t) + first(a+1, t) + the sum
+// over k = a+2..b of derived(k, t); for an anchor derived from row 0,
+// far(a, t) + the sum over k = a+1..b of derived(k, t); for row 0 the
+// derived rows alone.  For a makespan T the rows are cut into units from
+// the top, each extended while some width and anchor kind keep it within
+// T, and each unit takes the narrowest width that does, with the cheaper
+// kind there; T is the smallest value (binary search) for which the wide
+// units fit 3/4 of the grid (all of it when no unit is left for a single
+// workgroup) and the single-workgroup units, packed longest first, fit the
+// rest.  With `fixed` set, the anchors, their kinds and chains are given
+// and only the widths are chosen.  Outputs: peel[k] (row k is an anchor),
+// far[k] (the anchor k is derived from row 0), chain_last[a] (last row of
+// a's chain, -1: none), and per anchor the width want[a] and the weight
+// wts[a] (predicted workgroup time, which places the units longest first).
+// Returns false if no plan fits.
 bool mlp_rows_plan(dcx_ctx* c, const dcx_graph* g, int32_t kr, uint32_t G, uint32_t team_max,
-                   bool fixed, std::vector<char>& peel, std::vector<int32_t>& chain_last,
-                   std::vector<uint32_t>& want, std::vector<double>& wts) {
+                   bool fixed, std::vector<char>& peel, std::vector<char>& far,
+                   std::vector<int32_t>& chain_last, std::vector<uint32_t>& want,
+                   std::vector<double>& wts) {
     const uint32_t np = (uint32_t)kr + 1;
     const uint32_t cap = std::min(team_max, G / kXcds);
     std::vector<uint32_t> cands{1};
@@ -705,27 +724,29 @@ bool mlp_rows_plan(dcx_ctx* c, const dcx_graph* g, int32_t kr, uint32_t G, uint3
     const uint32_t nc = (uint32_t)cands.size();
     if (!have_row_stats(g)) return false;  // nothing logged yet: the fixed rule
     // predictions: [kind][k][width]
-    std::vector<float> X((size_t)3 * np * nc * kMlpIn);
-    for (int kind = 0; kind < 3; ++kind)
+    std::vector<float> X((size_t)kPassKinds * np * nc * kMlpIn);
+    for (int kind = 0; kind < kPassKinds; ++kind)
         for (uint32_t k = 0; k < np; ++k)
             for (uint32_t j = 0; j < nc; ++j)
                 pass_features(g, (int32_t)k, cands[j], kind, true,
                               &X[(((size_t)kind * np + k) * nc + j) * kMlpIn]);
-    const std::vector<float> y = mlp_predict(c, c->sched_mlp, X, 3 * np * nc);
+    const std::vector<float> y = mlp_predict(c, c->sched_mlp, X, kPassKinds * np * nc);
     auto d = [&](int kind, uint32_t k, uint32_t j) {
         return std::exp2((double)y[((size_t)kind * np + k) * nc + j]);
     };
-    // unit cost at every width, for the unit a..b
-    auto unit_cost = [&](uint32_t a, uint32_t b, uint32_t j) {
-        double t = a > 0 ? d(kPassPeel, a, j) : 0.0;
-        // (row 0's chain is set up before the launch: its first row is an
-        // ordinary derived row)
-        for (uint32_t k = a + 1; k <= b; ++k)
-            t += d(k == a + 1 && a > 0 ? kPassFirst : kPassDerived, k, j);
+    // kind of row k in the unit anchored at a (alt 1: a derived from row 0)
+    auto kind_of = [&](uint32_t a, int alt, uint32_t k) {
+        if (k == a) return alt ? kPassFar : kPassPeel;
+        return k == a + 1 && a > 0 && !alt ? kPassFirst : kPassDerived;
+    };
+    auto unit_cost = [&](uint32_t a, int alt, uint32_t b, uint32_t j) {
+        double t = 0.0;
+        for (uint32_t k = a > 0 ? a : 1; k <= b; ++k) t += d(kind_of(a, alt, k), k, j);
         return t;
     };
     struct Unit {
         uint32_t a, b, j;
+        int alt;
         double cost;
     };
     auto plan = [&](double T, std::vector<Unit>& units) -> bool {
@@ -734,34 +755,43 @@ bool mlp_rows_plan(dcx_ctx* c, const dcx_graph* g, int32_t kr, uint32_t G, uint3
             for (uint32_t a = 0; a < np; ++a) {
                 if (!(a == 0 ? chain_last[0] >= 0 : peel[a] != 0)) continue;
                 const uint32_t b = chain_last[a] >= 0 ? (uint32_t)chain_last[a] : a;
+                const int alt = a > 0 && far[a] ? 1 : 0;
                 uint32_t j = 0;
-                while (j + 1 < nc && unit_cost(a, b, j) > T) ++j;
-                if (unit_cost(a, b, j) > T) return false;
-                units.push_back({a, b, j, unit_cost(a, b, j)});
+                while (j + 1 < nc && unit_cost(a, alt, b, j) > T) ++j;
+                if (unit_cost(a, alt, b, j) > T) return false;
+                units.push_back({a, b, j, alt, unit_cost(a, alt, b, j)});
             }
         } else {
             uint32_t a = 0;
-            std::vector<double> run(nc, 0.0);  // cost of a..k at each width
+            // cost of a..k at each width, per anchor kind (row 0: both 0)
+            std::vector<double> run[2] = {std::vector<double>(nc, 0.0),
+                                          std::vector<double>(nc, 0.0)};
             for (uint32_t k = 1; k <= np; ++k) {
                 bool extend = false;
                 if (k < np) {
-                    const int kind = k == a + 1 && a > 0 ? kPassFirst : kPassDerived;
-                    for (uint32_t j = 0; j < nc; ++j)
-                        extend = extend || run[j] + d(kind, k, j) <= T;
+                    for (int alt = 0; alt < 2; ++alt)
+                        for (uint32_t j = 0; j < nc; ++j)
+                            extend = extend || run[alt][j] + d(kind_of(a, alt, k), k, j) <= T;
                     if (extend)
-                        for (uint32_t j = 0; j < nc; ++j) run[j] += d(kind, k, j);
+                        for (int alt = 0; alt < 2; ++alt)
+                            for (uint32_t j = 0; j < nc; ++j)
+                                run[alt][j] += d(kind_of(a, alt, k), k, j);
                 }
                 if (extend) continue;
                 // close the unit a..k-1 (row 0 alone is no unit)
                 if (a > 0 || k > 1) {
                     uint32_t j = 0;
-                    while (j + 1 < nc && run[j] > T) ++j;
-                    if (run[j] > T) return false;
-                    units.push_back({a, k - 1, j, run[j]});
+                    while (j + 1 < nc && std::min(run[0][j], run[1][j]) > T) ++j;
+                    const int alt = a > 0 && run[1][j] < run[0][j] ? 1 : 0;
+                    if (run[alt][j] > T) return false;
+                    units.push_back({a, k - 1, j, alt, run[alt][j]});
                 }
                 if (k == np) break;
                 a = k;  // k becomes an anchor
-                for (uint32_t j = 0; j < nc; ++j) run[j] = d(kPassPeel, k, j);
+                for (uint32_t j = 0; j < nc; ++j) {
+                    run[0][j] = d(kPassPeel, k, j);
+                    run[1][j] = d(kPassFar, k, j);
+                }
             }
         }
         uint32_t big = 0;
@@ -785,10 +815,10 @@ bool mlp_rows_plan(dcx_ctx* c, const dcx_graph* g, int32_t kr, uint32_t G, uint3
         }
         return true;
     };
-    // one workgroup doing every row is always within this bound
+    // one workgroup doing every row, each of every kind, is within this bound
     double hi = 0.0;
     for (uint32_t k = 1; k < np; ++k)
-        hi += d(kPassPeel, k, 0) + d(kPassFirst, k, 0) + d(kPassDerived, k, 0);
+        for (int kind = 0; kind < kPassKinds; ++kind) hi += d(kind, k, 0);
     hi = 2.0 * hi + 1.0;
     std::vector<Unit> units;
     if (!plan(hi, units)) return false;
@@ -803,6 +833,7 @@ bool mlp_rows_plan(dcx_ctx* c, const dcx_graph* g, int32_t kr, uint32_t G, uint3
     plan(hi, units);
     if (!fixed) {
         peel.assign(peel.size(), 1);
+        far.assign(far.size(), 0);
         std::fill(chain_last.begin(), chain_last.end(), -1);
     }
     want.assign(np, 1);
@@ -811,6 +842,7 @@ bool mlp_rows_plan(dcx_ctx* c, const dcx_graph* g, int32_t kr, uint32_t G, uint3
         if (!fixed) {
             for (uint32_t k = u.a + 1; k <= u.b; ++k) peel[k] = 0;
             chain_last[u.a] = u.b > u.a ? (int32_t)u.b : -1;
+            far[u.a] = (char)u.alt;
         }
         want[u.a] = cands[u.j];
         wts[u.a] = u.cost * cands[u.j];
@@ -997,16 +1029,18 @@ void compute_rows(dcx_ctx* c, dcx_graph* g, int variant, int32_t kr = -1) {
         const bool have_row0 = g->row0_ready;
         const int32_t k_first = ksplit >= 2 || have_row0 ? 1 : 0;
         g->rows_first_grid = k_first;
-        // Peeled and derived team rows: a row is either peeled (an anchor)
-        // or derived from the row above; the rows a+1..b derived from
-        // anchor a run on the team that peeled a, right after it (row 0,
-        // ready before the launch, anchors a chain of its own).  The plan
-        // comes from the cost model when one is attached and
-        // cfg.chain_rows is 0, else every spacing-th row from row 0 is an
-        // anchor (the fixed rule); the model then still picks the widths.
+        // Anchors and derived team rows: a row is either an anchor --
+        // peeled, or derived from row 0 (far) -- or derived from the row
+        // above; the rows a+1..b derived from anchor a run on the team
+        // that computed a, right after it (row 0, ready before the launch,
+        // anchors a chain of its own).  The plan comes from the cost model
+        // when one is attached and cfg.chain_rows is 0, else every
+        // spacing-th row from row 0 is an anchor, of the kind cfg.anchors
+        // names (the fixed rule); the model then still picks the widths.
         const int32_t spacing =
             c->cfg.chain_rows == 0 ? kChainRows : std::max<int32_t>(c->cfg.chain_rows, 1);
         std::vector<char> peel(kr + 1, 1);
+        std::vector<char> far(kr + 1, c->cfg.anchors == 1 ? 1 : 0);
         std::vector<int32_t> chain_last(kr + 1, -1);
         const bool chains_ok = k_first == 1 && ksplit >= 2;
         if (chains_ok && spacing > 1) {
@@ -1021,14 +1055,15 @@ void compute_rows(dcx_ctx* c, dcx_graph* g, int variant, int32_t kr = -1) {
         bool by_model = false;
         if (c->sched_mlp && chains_ok) {
             const bool fixed = c->cfg.chain_rows != 0;
-            std::vector<char> p2 = peel;
+            std::vector<char> p2 = peel, f2 = far;
             std::vector<int32_t> c2 = chain_last;
             // without a configured cap the model may give a unit a whole
             // XCD (the fixed rule stops at kTeamMax)
             const uint32_t mcap = c->cfg.team_max > 0 ? tmax : G / kXcds;
-            by_model = mlp_rows_plan(c, g, ksplit - 1, G, mcap, fixed, p2, c2, want, wts);
+            by_model = mlp_rows_plan(c, g, ksplit - 1, G, mcap, fixed, p2, f2, c2, want, wts);
             if (by_model) {
                 peel = p2;
+                far = f2;
                 chain_last = c2;
             }
         }
@@ -1056,11 +1091,24 @@ void compute_rows(dcx_ctx* c, dcx_graph* g, int variant, int32_t kr = -1) {
             busy = (uint32_t)std::count_if(s.block_map.begin(), s.block_map.begin() + G,
                                            [](uint32_t t) { return t != ~0u; });
             // expand every unit into its passes: the anchor's peel, then its chain
+            // (an anchor derived from row 0 is one chain a-1 -> b that
+            // starts from row 0's values: only when row 0 is ready before
+            // the launch)
             std::vector<int32_t> pks;
             for (TeamDesc& td : s.teams) {
                 const uint32_t b0 = (uint32_t)pks.size();
                 for (uint32_t i = td.pass_begin; i < td.pass_end; ++i) {
                     const int32_t a = s.ks[i];
+                    if (k_first == 1 && a >= 1 && peel[a] && far[a]) {
+                        const int32_t b = chain_last[a] >= 0 ? chain_last[a] : a;
+                        pks.push_back(a - 1);
+                        pkb.push_back(b | kChainFromRow0);
+                        for (int32_t k = a; k <= b; ++k) {
+                            g->h_pass_variant[k + 1] = DCX_VARIANT_DERIVED;
+                            g->h_pass_team[k + 1] = (uint16_t)td.size;
+                        }
+                        continue;
+                    }
                     if (a >= k_first && peel[a]) {
                         pks.push_back(a);
                         pkb.push_back(-1);
@@ -1374,11 +1422,14 @@ dcx_status dcx_device_count(int32_t* n) {
     return DCX_OK;
 }
 
-dcx_status dcx_ctx_set_rows_rule(dcx_ctx* ctx, int32_t chain_rows, int32_t team_max) {
-    if (!ctx || chain_rows < -1 || team_max < 0 || team_max > 256) return DCX_ERR_INVALID_ARG;
+dcx_status dcx_ctx_set_rows_rule(dcx_ctx* ctx, int32_t chain_rows, int32_t team_max,
+                                 int32_t anchors) {
+    if (!ctx || chain_rows < -1 || team_max < 0 || team_max > 256 || anchors < 0 || anchors > 1)
+        return DCX_ERR_INVALID_ARG;
     std::lock_guard<std::mutex> lk(ctx->mu);
     ctx->cfg.chain_rows = chain_rows;
     ctx->cfg.team_max = team_max;
+    ctx->cfg.anchors = anchors;
     return DCX_OK;
 }
 
@@ -1397,7 +1448,7 @@ dcx_status dcx_ctx_create(const dcx_config* cfg, dcx_ctx** out) {
         cfg->team_max > 256 || cfg->shrink_at < 0)
         return DCX_ERR_INVALID_ARG;
     if (cfg->maintenance < 0 || cfg->maintenance > 2 || cfg->local_budget < 0 ||
-        cfg->chain_rows < -1)
+        cfg->chain_rows < -1 || cfg->anchors < 0 || cfg->anchors > 1)
         return DCX_ERR_INVALID_ARG;
     std::unique_ptr<dcx_ctx> c(new (std::nothrow) dcx_ctx);
     if (!c) return DCX_ERR_OUT_OF_MEMORY;

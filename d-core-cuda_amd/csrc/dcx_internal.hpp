@@ -196,10 +196,12 @@ int rows_teams_blocks_per_cu();
 // block_team[blocks + b] its rank in the team.  Pass p of a team
 // (teams[t].pass_begin <= p < pass_end) peels row team_ks[p] when
 // team_kb[p] < 0, and otherwise derives rows team_ks[p]+1 .. team_kb[p] from
-// row team_ks[p] (row 0, or an anchor an earlier pass of the team peeled);
-// a team's scratch then holds 12 arrays of scratch_cap >= V_{team_ks[p]}
-// words.  base describes the graph (k, nmembers, out_row set per pass;
+// row team_ks[p] (row 0, or an anchor an earlier pass of the team peeled),
+// or, with kChainFromRow0 set in team_kb[p], starting from row 0's values
+// (an upper bound of every row); a team's scratch then holds 12 arrays of
+// scratch_cap >= V_{team_ks[p]} words.  base describes the graph (k, nmembers, out_row set per pass;
 // stats: row k's log words at stats + kStatWords * k).
+constexpr int32_t kChainFromRow0 = 1 << 30;
 hipError_t launch_rows_teams(const PassSpec& base, const uint32_t* vk, int32_t* rows,
                              int32_t* lmax_rows, const TeamDesc* teams, const uint32_t* block_team,
                              const int32_t* team_ks, const int32_t* team_kb, uint32_t blocks,

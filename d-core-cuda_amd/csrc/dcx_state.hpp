@@ -176,9 +176,10 @@ struct dcx_graph {
     // Work statistics of the team rows over the batched launches so far,
     // value + 1 (0: row k never ran that way): phases of its last peel;
     // rounds and list entries read of its last derivation from the row
-    // above (a chain's supports set-up excluded).  Cost-model features.
-    std::vector<uint32_t> h_peel_phases, h_der_rounds;
-    std::vector<uint64_t> h_der_reads;
+    // above, and of its last derivation from row 0 (a chain's supports
+    // set-up excluded).  Cost-model features.
+    std::vector<uint32_t> h_peel_phases, h_der_rounds, h_far_rounds;
+    std::vector<uint64_t> h_der_reads, h_far_reads;
     std::vector<int8_t> h_pass_variant;  // [kmax+2] variant that last ran each pass
     std::vector<uint16_t> h_pass_team;   // [kmax+2] workgroups that ran it
     int32_t* rows = nullptr;     // (kmax+1) x V

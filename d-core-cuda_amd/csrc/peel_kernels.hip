@@ -1457,9 +1457,11 @@ __global__ __launch_bounds__(kBlk, 4) void k_rows_teams(PassSpec base, const uin
         A.lmax_rows = lmax_rows;
         A.stats = base.stats;
         A.err = err;
-        const bool preset = team_ks[p] == 0 && chain0_preset != 0;
+        const int32_t a = team_ks[p], b = team_kb[p] & ~kChainFromRow0;
+        const int32_t src = (team_kb[p] & kChainFromRow0) ? 0 : a;
+        const bool preset = a == 0 && chain0_preset != 0;
         if (!chain::run_chain(A, chain::chain_state(team_scratch, td.scratch_cap),
-                              slot_pool + td.slot_off, gs, team_ks[p], team_kb[p], preset, tag))
+                              slot_pool + td.slot_off, gs, a, b, src, preset, tag))
             return;
         if (!team_resync(gs, ++seq)) return;
     }

@@ -38,7 +38,7 @@ class Config(ctypes.Structure):
                 ("scratch_budget", ctypes.c_uint64), ("pull_ratio", ctypes.c_int32),
                 ("lds_members", ctypes.c_int32), ("maintenance", ctypes.c_int32),
                 ("local_budget", ctypes.c_int32), ("chain_rows", ctypes.c_int32),
-                ("reserved32", ctypes.c_int32), ("reserved", ctypes.c_uint64 * 1)]
+                ("anchors", ctypes.c_int32), ("reserved", ctypes.c_uint64 * 1)]
 
 
 class Timing(ctypes.Structure):
@@ -286,7 +286,7 @@ class Context:
 
     def __init__(self, device=0, variant=VARIANT_AUTO, rows_slots=0, scratch_budget=0,
                  team_max=0, shrink_at=0, trace_phases=0, pull_ratio=0, lds_members=0,
-                 maintenance=0, local_budget=0, chain_rows=0):
+                 maintenance=0, local_budget=0, chain_rows=0, anchors=0):
         cfg = Config()
         check(lib().dcx_config_init(ctypes.byref(cfg)), "dcx_config_init")
         cfg.device = device
@@ -301,6 +301,7 @@ class Context:
         cfg.maintenance = maintenance
         cfg.local_budget = local_budget
         cfg.chain_rows = chain_rows
+        cfg.anchors = anchors
         h = ctypes.c_void_p()
         check(lib().dcx_ctx_create(ctypes.byref(cfg), ctypes.byref(h)), "dcx_ctx_create")
         self.h = h
@@ -351,12 +352,13 @@ class Context:
                                     ctypes.byref(n), ctypes.byref(f)), "dcx_compare_i32", self.h)
         return n.value, (None if n.value == 0 else f.value)
 
-    def set_rows_rule(self, chain_rows=0, team_max=0):
-        """cfg.chain_rows and cfg.team_max for the next decompositions
-        (chain_rows 0: the default rule, or the cost model's plan when one
-        is attached; 1: every team row peeled; n > 1: every n-th row from
-        row 0 peeled; team_max 0: the default widest team)."""
-        check(lib().dcx_ctx_set_rows_rule(self.h, i32(chain_rows), i32(team_max)),
+    def set_rows_rule(self, chain_rows=0, team_max=0, anchors=0):
+        """cfg.chain_rows, cfg.team_max and cfg.anchors for the next
+        decompositions (chain_rows 0: the default rule, or the cost model's
+        plan when one is attached; 1: every team row an anchor; n > 1: every
+        n-th row from row 0 an anchor; team_max 0: the default widest team;
+        anchors 0: peeled, 1: derived from row 0)."""
+        check(lib().dcx_ctx_set_rows_rule(self.h, i32(chain_rows), i32(team_max), i32(anchors)),
               "dcx_ctx_set_rows_rule", self.h)
 
     def last_trace(self):

@@ -184,7 +184,11 @@ typedef struct dcx_config {
      * visited); 0 = the default spacing, 1 or -1 = every row peeled.
      * Results are identical. */
     int32_t chain_rows;
-    int32_t reserved32;
+    /* Batched rows, fixed rule: how an anchor row (one not derived from the
+     * row above) is computed: 0 = peeled (default); 1 = derived from row
+     * 0 (h-index refinement from row 0's values, which bound every row).
+     * The cost model chooses per anchor.  Results are identical. */
+    int32_t anchors;
     uint64_t reserved[1];
 } dcx_config;
 
@@ -217,11 +221,13 @@ dcx_status dcx_ctx_create(const dcx_config* cfg, dcx_ctx** out);
 dcx_status dcx_ctx_destroy(dcx_ctx* ctx);
 /* Detail of the last failing call on this context ("" if none). */
 const char* dcx_ctx_last_error(const dcx_ctx* ctx);
-/* Sets cfg.chain_rows and cfg.team_max of a context (which batched team
- * rows are peeled and which derived from the row above, and the widest
- * team) for its next decompositions; results are identical.  No reference
+/* Sets cfg.chain_rows, cfg.team_max and cfg.anchors of a context (which
+ * batched team rows are anchors and which derived from the row above, the
+ * widest team, and whether anchors are peeled or derived from row 0) for
+ * its next decompositions; results are identical.  No reference
  * counterpart. */
-dcx_status dcx_ctx_set_rows_rule(dcx_ctx* ctx, int32_t chain_rows, int32_t team_max);
+dcx_status dcx_ctx_set_rows_rule(dcx_ctx* ctx, int32_t chain_rows, int32_t team_max,
+                                 int32_t anchors);
 dcx_status dcx_last_timing(const dcx_ctx* ctx, dcx_timing* t);
 
 /* Uploads a CSR-with-slack graph (layout of dcx_csr_build).  Replaces
@@ -288,7 +294,9 @@ typedef struct dcx_pass_log {
                           refinement rounds (derived) */
     uint32_t levels;   /* level iterations (peel); 0 for h-index; derived:
                           1 for the first row of a chain (its time includes
-                          the supports' set-up from the anchor row), else 0 */
+                          the supports' set-up from the anchor row), 2 for
+                          an anchor derived from row 0 (set-up included),
+                          else 0 */
     uint32_t team;     /* workgroups that ran it */
     uint64_t edges;    /* edges of the pass (E, or E_k) */
     double ms;         /* device wall-clock time of the pass */

@@ -1,11 +1,13 @@
 """Rows derived from the row above ("chains", dcx_config.chain_rows): only
-every chain_rows-th team row is peeled and the rows between are refined
-from the row above with per-vertex supports.  Every row must equal the CPU
-oracle's, bit for bit, for chain spacings 2, 3, 5 and 16 and with chains
-off (1); graphs where every row is a team row (lds_members = -1) and the
-default split; the derived rows must show in the pass log (variant
-DERIVED, the first row of each chain flagged, its list reads counted).  At
-R-MAT 20 the state hash must equal the oracle's
+every chain_rows-th team row is an anchor and the rows between are refined
+from the row above with per-vertex supports; an anchor is peeled or
+(dcx_config.anchors = 1) refined from row 0's values.  Every row must equal
+the CPU oracle's, bit for bit, for chain spacings 2, 3, 5 and 16 and with
+chains off (1), with both kinds of anchor; graphs where every row is a team
+row (lds_members = -1) and the default split; the derived rows must show in
+the pass log (variant DERIVED, the first row of each chain flagged -- 1
+after a peeled anchor, 2 for an anchor derived from row 0 -- its list
+reads counted).  At R-MAT 20 the state hash must equal the oracle's
 (tests/golden/hashes.json)."""
 import json
 import os
@@ -54,13 +56,14 @@ CASES = {
 
 
 @pytest.mark.parametrize("name", list(CASES))
-@pytest.mark.parametrize("spacing", [2, 3, 5, 16, 1])
-def test_chains_equal_oracle(name, spacing):
+@pytest.mark.parametrize("spacing,anchors", [(2, 0), (3, 0), (5, 0), (16, 0), (1, 0), (2, 1),
+                                             (3, 1), (5, 1), (16, 1), (1, 1)])
+def test_chains_equal_oracle(name, spacing, anchors):
     V, s, d = CASES[name]()
     s, d = np.asarray(s, np.uint32), np.asarray(d, np.uint32)
     ref = oracle.decompose(oracle.csr_from_edges(V, s, d))
-    with dcx.Context(0, lds_members=-1, chain_rows=spacing) as c:
-        say(f"chains {name} V={V} E={len(s)} spacing={spacing}")
+    with dcx.Context(0, lds_members=-1, chain_rows=spacing, anchors=anchors) as c:
+        say(f"chains {name} V={V} E={len(s)} spacing={spacing} anchors={anchors}")
         g = c.upload_edges(V, s, d)
         K, L = g.decompose(dcx.VARIANT_BATCHED)
         t = c.timing()
@@ -69,7 +72,18 @@ def test_chains_equal_oracle(name, spacing):
         assert np.array_equal(g.rows(), ref[3])
         log = g.pass_log()
         derived = [p for p in log[1:] if p["variant"] == dcx.VARIANT_DERIVED]
-        if spacing > 1 and K >= 2:
+        if anchors == 1 and K >= 2:
+            # every team row but row 0 is derived: anchors from row 0
+            # (flag 2), the rows between from the row above; row 1 is row
+            # 0's chain, set up before the launch (flag 0)
+            assert len(derived) == K, derived
+            for p in derived:
+                anchor = p["k"] % spacing == 0 or spacing == 1
+                want = 2 if anchor and p["k"] >= 2 else 0
+                assert p["levels"] == want, p
+                if want:
+                    assert p["alive"] > 0, p
+        elif spacing > 1 and K >= 2:
             assert derived and all(p["k"] % spacing != 0 for p in derived), derived
             for p in derived:
                 # first row of a chain from a peeled anchor (row 0's chain is
@@ -78,9 +92,9 @@ def test_chains_equal_oracle(name, spacing):
                 assert p["levels"] == (1 if first and p["k"] > 1 else 0), p
                 if first and p["k"] > 1:
                     assert p["alive"] > 0, p  # the supports' set-up read lists
-            assert t.rows_scheduler == 0
         else:
             assert not derived
+        assert t.rows_scheduler == 0
         # a second decomposition on the same context (reused scratch)
         K2, L2 = g.decompose(dcx.VARIANT_BATCHED)
         assert (K2, L2) == (K, L)
