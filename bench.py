@@ -407,7 +407,7 @@ def run_headline(args):
     split = int(tims[-1].rows_split)
     first = 1 if mean("rows_first_ms") > 0 else 0
     team_b = lds_b = peel_equiv = 0.0
-    n_peeled = n_derived = 0
+    n_peeled = n_derived = n_far = 0
     read_entries = core_entries = 0
     for p in log_rows[1:]:
         k, vk_, ek_ = p["k"], float(p["members"]), float(p["edges"])
@@ -415,7 +415,10 @@ def run_headline(args):
         peel_equiv += full
         if p["variant"] == dcx.VARIANT_DERIVED:
             b = 8.0 * p["alive"] + 20 * vk_
-            n_derived += 1
+            if p["levels"] == 2:
+                n_far += 1  # an anchor derived from row 0
+            else:
+                n_derived += 1
             read_entries += p["alive"]
         else:
             b = full
@@ -430,6 +433,7 @@ def run_headline(args):
     teams_ms, lds_ms = mean("rows_teams_ms"), mean("rows_lds_ms")
     kernels = {
         "k_rows_teams": (team_b, teams_ms, f"{first}..{split - 1} ({n_peeled} peeled, "
+                                           f"{n_far} derived from row 0, "
                                            f"{n_derived} derived from the row above)"),
         "k_rows_lds": (lds_b, lds_ms, f"{split}..{K}"),
         "k_pass_grid (kmax pass)": (b_kmax, mean("kmax_ms"), "kmax pass"),

@@ -72,6 +72,41 @@ constexpr uint32_t kTinyList = 16;
 constexpr int kListBins = 4096;  // kmax + 1 at most, for the counting sort
 constexpr int kLongBlk = 256;
 
+// kmax of the vertex with new id r: the largest k with V_k > r (svk: V_k
+// for k = 0..kmax, non-increasing, V_0 = V > r; kmax is non-increasing in
+// the new id, so min(kmax of ids i and r) is the kmax of id max(i, r)).
+__device__ __forceinline__ int32_t kmax_of_id(const uint32_t* svk, int32_t kmax, uint32_t r) {
+    int32_t lo = 0, hi = kmax;
+    while (lo < hi) {
+        const int32_t mid = (lo + hi + 1) >> 1;
+        if (svk[mid] > r)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+
+// Loads V_k into LDS and clears the block's E_k bins (EK only).
+template <bool EK, typename H>
+__device__ __forceinline__ void ek_begin(const uint32_t* vk, int32_t kmax, uint32_t* svk, H* seh) {
+    if (!EK) return;
+    for (int32_t b = threadIdx.x; b <= kmax; b += blockDim.x) {
+        svk[b] = vk[b];
+        seh[b] = 0;
+    }
+    __syncthreads();
+}
+
+// Adds the block's E_k bins to ekh (EK only).
+template <bool EK, typename H>
+__device__ __forceinline__ void ek_end(int32_t kmax, const H* seh, unsigned long long* ekh) {
+    if (!EK) return;
+    __syncthreads();
+    for (int32_t b = threadIdx.x; b <= kmax; b += blockDim.x)
+        if (seh[b]) atomicAdd(&ekh[b], (unsigned long long)seh[b]);
+}
+
 // Place of a list entry with new id r held by lane l of a wavefront
 // holding the list's n entries: #{j : r_j < r} + #{j < l : r_j == r}
 // (ascending, repeated entries in list order).
@@ -87,11 +122,20 @@ __device__ __forceinline__ uint32_t list_place(uint32_t r, uint32_t l, uint32_t 
 // Lists of at most kShortList entries: a wavefront takes four consecutive
 // new owners at a time, one per 16-lane quarter when all four lists have at
 // most kTinyList entries, else one list after another on all 64 lanes.
-// Longer lists are queued (new owner id) for k_lists_long.
-__global__ void k_lists_short(const uint32_t* order, const uint32_t* rank,
-                              uint32_t V, const uint32_t* off, const int32_t* deg,
-                              const uint32_t* nbr, const uint32_t* off_new, uint32_t* out,
-                              uint32_t* longq, uint32_t* longn) {
+// Longer lists are queued (new owner id) for k_lists_long.  EK (out side):
+// every entry also adds 1 to ekh[min of its endpoints' kmax], the E_k
+// histogram (E_k = edges inside the (k,0)-core = its suffix sums).
+template <bool EK>
+__global__ __launch_bounds__(1024) void k_lists_short(const uint32_t* order, const uint32_t* rank,
+                                                      uint32_t V, const uint32_t* off,
+                                                      const int32_t* deg, const uint32_t* nbr,
+                                                      const uint32_t* off_new, uint32_t* out,
+                                                      uint32_t* longq, uint32_t* longn,
+                                                      const uint32_t* vk, int32_t kmax,
+                                                      unsigned long long* ekh) {
+    __shared__ uint32_t svk[EK ? kListBins : 1];
+    __shared__ unsigned long long seh[EK ? kListBins : 1];
+    ek_begin<EK>(vk, kmax, svk, seh);
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / W;
     const uint32_t nw = gridDim.x * blockDim.x / W;
     const uint32_t lane = (uint32_t)lane_id();
@@ -114,7 +158,10 @@ __global__ void k_lists_short(const uint32_t* order, const uint32_t* rank,
                 const uint32_t rj = (uint32_t)__shfl((int)r, (int)j, (int)kTinyList);
                 pos += (j < nq && (rj < r || (rj == r && j < l))) ? 1u : 0u;
             }
-            if (has) out[off_new[iq] + pos] = r;
+            if (has) {
+                out[off_new[iq] + pos] = r;
+                if (EK) atomicAdd(&seh[kmax_of_id(svk, kmax, max(iq, r))], 1ull);
+            }
             continue;
         }
         for (uint32_t t = 0; t < 4 && i0 + t < V; ++t) {
@@ -129,9 +176,13 @@ __global__ void k_lists_short(const uint32_t* order, const uint32_t* rank,
             const uint32_t u = has ? nbr[off[v] + lane] : 0u;
             const uint32_t r = has ? rank[u] : 0u;
             const uint32_t pos = list_place(r, lane, n);
-            if (has) out[off_new[i] + pos] = r;
+            if (has) {
+                out[off_new[i] + pos] = r;
+                if (EK) atomicAdd(&seh[kmax_of_id(svk, kmax, max(i, r))], 1ull);
+            }
         }
     }
+    ek_end<EK>(kmax, seh, ekh);
 }
 
 // Entries of a long list each thread keeps in registers between the
@@ -142,14 +193,21 @@ constexpr int kLongRegs = 8;
 // neighbours' kmax (and its range), start of every bin of the range in
 // descending order (kLongBlk bins per step), then each entry placed at its
 // bin's cursor; the range's bins are cleared for the next list.  kr: the
-// packed (kmax, new id) word of every vertex.
+// packed (kmax, new id) word of every vertex.  EK (out side): the list's
+// entries are also added to the E_k histogram ekh, from the kmax bins:
+// bin b < kmax(v) gets the entries of kmax b, bin kmax(v) the rest.
+template <bool EK>
 __global__ __launch_bounds__(kLongBlk) void k_lists_long(const uint32_t* order, const uint64_t* kr,
                                                          const uint32_t* off, const int32_t* deg,
                                                          const uint32_t* nbr,
                                                          const uint32_t* off_new, uint32_t* out,
                                                          const uint32_t* longq,
-                                                         const uint32_t* longn) {
+                                                         const uint32_t* longn, int32_t kmax,
+                                                         unsigned long long* ekh) {
     __shared__ uint32_t cnt[kListBins];
+    __shared__ uint32_t seh[EK ? kListBins : 1];
+    if (EK)
+        for (int b = threadIdx.x; b <= kmax; b += kLongBlk) seh[b] = 0;
     __shared__ uint32_t w[kLongBlk / W];
     __shared__ int32_t s_lo, s_hi;
     __shared__ uint32_t s_carry;
@@ -194,6 +252,20 @@ __global__ __launch_bounds__(kLongBlk) void k_lists_long(const uint32_t* order, 
         __syncthreads();
         lo = s_lo;
         hi = s_hi;
+        if (EK) {
+            // (read before the scan below turns the counts into cursors)
+            const int32_t kv = (int32_t)(kr[v] >> 32);
+            uint32_t above = 0;
+            for (int b = lo + (int)tid; b <= hi; b += kLongBlk) {
+                const uint32_t c = cnt[b];
+                if (b < kv)
+                    seh[b] += c;
+                else
+                    above += c;
+            }
+            if (above) atomicAdd(&seh[kv], above);
+            __syncthreads();
+        }
         // starts in descending kmax, kLongBlk bins per step
         for (int top = hi; top >= lo; top -= kLongBlk) {
             const int b = top - (int)tid;
@@ -231,6 +303,7 @@ __global__ __launch_bounds__(kLongBlk) void k_lists_long(const uint32_t* order, 
         }
         __syncthreads();
     }
+    ek_end<EK>(kmax, seh, ekh);
 }
 
 // out[0] += sum of deg, out[1] += entries of [0, deg) that are not vertex
@@ -302,7 +375,8 @@ hipError_t launch_audit(uint32_t V, Csr side, unsigned long long* out, hipStream
 hipError_t build_core_graph(uint32_t V, uint64_t E, const uint32_t* order, const int32_t* kmaxes,
                             int32_t kmax, Csr out, Csr in, uint32_t* rank, uint32_t* cout_off,
                             int32_t* cout_deg, uint32_t* cout_nbr, uint32_t* cin_off,
-                            int32_t* cin_deg, uint32_t* cin_nbr, void* tmp, hipStream_t st) {
+                            int32_t* cin_deg, uint32_t* cin_nbr, void* tmp, const uint32_t* vk,
+                            unsigned long long* ekh, hipStream_t st) {
     if (V == 0) return hipSuccess;
     char* p = static_cast<char*>(tmp);
     if (kmax + 1 <= kListBins) {
@@ -312,6 +386,11 @@ hipError_t build_core_graph(uint32_t V, uint64_t E, const uint32_t* order, const
         uint32_t* longq = longn + 64;  // 256-byte aligned from longn
         hipLaunchKernelGGL(k_rank_packed, dim3(grid_for(V, 256)), dim3(256), 0, st, order, kmaxes,
                            V, rank, kr);
+        if (ekh) {
+            const hipError_t me =
+                hipMemsetAsync(ekh, 0, ((size_t)kmax + 1) * sizeof(unsigned long long), st);
+            if (me != hipSuccess) return me;
+        }
         const Csr sides[2] = {out, in};
         uint32_t* offs[2] = {cout_off, cin_off};
         int32_t* degs[2] = {cout_deg, cin_deg};
@@ -325,12 +404,21 @@ hipError_t build_core_graph(uint32_t V, uint64_t E, const uint32_t* order, const
             hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, st, offs[s] + V, (uint32_t)E);
             if (E == 0) continue;
             hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, st, longn, 0u);
-            hipLaunchKernelGGL(k_lists_short, dim3(grid_for(V, 64)), dim3(1024), 0, st, order,
-                               rank, V, sides[s].off, sides[s].deg, sides[s].nbr, offs[s],
-                               nbrs[s], longq, longn);
-            hipLaunchKernelGGL(k_lists_long, dim3(4096), dim3(kLongBlk), 0, st, order, kr,
-                               sides[s].off, sides[s].deg, sides[s].nbr, offs[s], nbrs[s], longq,
-                               longn);
+            if (s == 0 && ekh) {
+                hipLaunchKernelGGL(k_lists_short<true>, dim3(grid_for(V, 64, 2048)), dim3(1024),
+                                   0, st, order, rank, V, sides[s].off, sides[s].deg,
+                                   sides[s].nbr, offs[s], nbrs[s], longq, longn, vk, kmax, ekh);
+                hipLaunchKernelGGL(k_lists_long<true>, dim3(4096), dim3(kLongBlk), 0, st, order,
+                                   kr, sides[s].off, sides[s].deg, sides[s].nbr, offs[s], nbrs[s],
+                                   longq, longn, kmax, ekh);
+            } else {
+                hipLaunchKernelGGL(k_lists_short<false>, dim3(grid_for(V, 64)), dim3(1024), 0, st,
+                                   order, rank, V, sides[s].off, sides[s].deg, sides[s].nbr,
+                                   offs[s], nbrs[s], longq, longn, vk, kmax, nullptr);
+                hipLaunchKernelGGL(k_lists_long<false>, dim3(4096), dim3(kLongBlk), 0, st, order,
+                                   kr, sides[s].off, sides[s].deg, sides[s].nbr, offs[s], nbrs[s],
+                                   longq, longn, kmax, nullptr);
+            }
         }
         return hipGetLastError();
     }

@@ -114,8 +114,9 @@ void audit_adjacency(dcx_ctx* c, dcx_graph* g) {
     g->audited = true;
 }
 
-// Relabels by kmax and sorts every list (see core_graph.hip).
-void build_core(dcx_ctx* c, dcx_graph* g) {
+// Relabels by kmax and sorts every list (see core_graph.hip); with_ek:
+// the E_k histogram (g->ekh) comes out of the same pass.
+void build_core(dcx_ctx* c, dcx_graph* g, bool with_ek = false) {
     const uint32_t V = g->V;
     if (!g->audited) audit_adjacency(c, g);  // once per edit, not per rebuild
     if (!g->rank) {
@@ -138,6 +139,7 @@ void build_core(dcx_ctx* c, dcx_graph* g) {
                              Csr{g->out_off, g->out_deg, g->out_nbr},
                              Csr{g->in_off, g->in_deg, g->in_nbr}, g->rank, g->cout_off,
                              g->cout_deg, g->cout_nbr, g->cin_off, g->cin_deg, g->cin_nbr, c->tmp,
+                             g->vk, with_ek ? reinterpret_cast<unsigned long long*>(g->ekh) : nullptr,
                              c->stream));
     g->core_valid = true;
 }
@@ -329,19 +331,24 @@ void compute_kmax(dcx_ctx* c, dcx_graph* g, int variant = DCX_VARIANT_BATCHED,
     }
     if (!g->order) dev_alloc(c, &g->order, g->V);
     HIPX(c, launch_order_by_kmax(g->kmaxes, g->V, g->kmax, g->hist, g->vk, g->order, c->stream));
-    // The E_k histogram runs on the side stream beside the core-ordered
-    // graph's build (both need only the kmaxes and the order).  The core is
-    // built here unless the rows will be whole-graph passes (few rows after
-    // row 0); otherwise when a pass first needs it (ensure_core).
-    HIPX(c, hipEventRecord(c->fork, c->stream));
-    HIPX(c, hipStreamWaitEvent(c->side, c->fork, 0));
-    HIPX(c, launch_core_edges(g->kmaxes, Csr{g->out_off, g->out_deg, g->out_nbr}, g->V, g->kmax,
-                              g->ekh, ensure_ekq(c, g), c->side));
-    HIPX(c, hipEventRecord(c->join, c->side));
+    // The core is built here unless the rows will be whole-graph passes
+    // (few rows after row 0); otherwise when a pass first needs it
+    // (ensure_core).  Its out-side list pass also counts the E_k histogram
+    // (it reads every list entry's endpoints anyway); without a build, or
+    // when the build cannot count it, the histogram runs on its own, on
+    // the side stream beside the build.
     const bool few_rows = variant == DCX_VARIANT_BATCHED && g->row0_pending && g->kmax >= 1 &&
                           g->kmax <= kFewRows;
-    if (!few_rows) build_core(c, g);
-    HIPX(c, hipStreamWaitEvent(c->stream, c->join, 0));
+    const bool ek_in_build = !few_rows && core_graph_writes_ek(g->kmax);
+    if (!ek_in_build) {
+        HIPX(c, hipEventRecord(c->fork, c->stream));
+        HIPX(c, hipStreamWaitEvent(c->side, c->fork, 0));
+        HIPX(c, launch_core_edges(g->kmaxes, Csr{g->out_off, g->out_deg, g->out_nbr}, g->V,
+                                  g->kmax, g->ekh, ensure_ekq(c, g), c->side));
+        HIPX(c, hipEventRecord(c->join, c->side));
+    }
+    if (!few_rows) build_core(c, g, ek_in_build);
+    if (!ek_in_build) HIPX(c, hipStreamWaitEvent(c->stream, c->join, 0));
     if (g->row0_pending) {
         HIPX(c, hipEventRecord(c->order_done, c->stream));
         HIPX(c, hipStreamWaitEvent(c->stream, c->row0_done, 0));
@@ -583,9 +590,13 @@ int last_pass_kind(const dcx_graph* g, int32_t k) {
 }
 
 // Folds the team rows of the last batched launch (rows 0..kr) into the
-// per-row work statistics.
-void record_row_stats(dcx_graph* g, int32_t kr) {
+// per-row work statistics and measured times (wall_khz: device clock).
+void record_row_stats(dcx_graph* g, int32_t kr, double wall_khz) {
     const size_t n = (size_t)g->kmax + 1;
+    for (int kind = 0; kind < kPassKinds; ++kind) {
+        g->h_meas_ms[kind].resize(n, 0.0f);
+        g->h_meas_team[kind].resize(n, 0);
+    }
     g->h_peel_phases.resize(n, 0);
     g->h_der_rounds.resize(n, 0);
     g->h_der_reads.resize(n, 0);
@@ -595,6 +606,9 @@ void record_row_stats(dcx_graph* g, int32_t kr) {
         const int kind = last_pass_kind(g, k);
         if (kind < 0) continue;
         const uint32_t* w = g->h_pstats.data() + (size_t)(k + 1) * kStatWords;
+        const uint64_t ticks = (uint64_t)w[kStatTicksLo] | ((uint64_t)w[kStatTicksHi] << 32);
+        g->h_meas_ms[kind][k] = (float)std::max(ticks / wall_khz, 1e-3);
+        g->h_meas_team[kind][k] = g->h_pass_team[k + 1];
         if (kind == kPassPeel) {
             g->h_peel_phases[k] = w[kStatPhases] + 1;
         } else {
@@ -701,7 +715,9 @@ std::vector<float> mlp_predict(dcx_ctx* c, dcx_mlp* m, const std::vector<float>&
 // width t is, for a peeled anchor, peel(a, t) + first(a+1, t) + the sum
 // over k = a+2..b of derived(k, t); for an anchor derived from row 0,
 // far(a, t) + the sum over k = a+1..b of derived(k, t); for row 0 the
-// derived rows alone.  For a makespan T the rows are cut into units from
+// derived rows alone (each prediction scaled by the ratio of measured to
+// predicted time of the row's last run of that kind).  For a makespan T
+// the rows are cut into units from
 // the top, each extended while some width and anchor kind keep it within
 // T, and each unit takes the narrowest width that does, with the cheaper
 // kind there; T is the smallest value (binary search) for which the wide
@@ -731,8 +747,25 @@ bool mlp_rows_plan(dcx_ctx* c, const dcx_graph* g, int32_t kr, uint32_t G, uint3
                 pass_features(g, (int32_t)k, cands[j], kind, true,
                               &X[(((size_t)kind * np + k) * nc + j) * kMlpIn]);
     const std::vector<float> y = mlp_predict(c, c->sched_mlp, X, kPassKinds * np * nc);
+    // calibration: where row k last ran as `kind` on a candidate width, the
+    // predictions for row k and that kind are scaled so that this width's
+    // prediction equals the measured time (the model keeps the shape over
+    // widths, the measurement fixes the level)
+    std::vector<double> cal((size_t)kPassKinds * np, 1.0);
+    for (int kind = 0; kind < kPassKinds; ++kind) {
+        const auto& ms = g->h_meas_ms[kind];
+        const auto& tm = g->h_meas_team[kind];
+        for (uint32_t k = 0; k < np && k < ms.size(); ++k) {
+            if (ms[k] <= 0.0f) continue;
+            for (uint32_t j = 0; j < nc; ++j)
+                if (cands[j] == tm[k]) {
+                    const double p = std::exp2((double)y[((size_t)kind * np + k) * nc + j]);
+                    cal[(size_t)kind * np + k] = (double)ms[k] / std::max(p, 1e-6);
+                }
+        }
+    }
     auto d = [&](int kind, uint32_t k, uint32_t j) {
-        return std::exp2((double)y[((size_t)kind * np + k) * nc + j]);
+        return cal[(size_t)kind * np + k] * std::exp2((double)y[((size_t)kind * np + k) * nc + j]);
     };
     // kind of row k in the unit anchored at a (alt 1: a derived from row 0)
     auto kind_of = [&](uint32_t a, int alt, uint32_t k) {
@@ -1306,7 +1339,7 @@ void compute_rows(dcx_ctx* c, dcx_graph* g, int variant, int32_t kr = -1) {
                       hipMemcpyDeviceToHost));
     g->rows_k = K + 1;
     g->rows_valid = true;
-    if (variant == DCX_VARIANT_BATCHED) record_row_stats(g, kr);
+    if (variant == DCX_VARIANT_BATCHED) record_row_stats(g, kr, c->wall_khz);
 }
 
 __global__ void k_ragged_gather(const int32_t* rows, const int32_t* kmaxes, const int64_t* row_ptr,

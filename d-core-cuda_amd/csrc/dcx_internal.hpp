@@ -264,11 +264,16 @@ size_t core_graph_tmp_bytes(uint32_t V, uint64_t E, int32_t kmax);
 hipError_t launch_audit(uint32_t V, Csr side, unsigned long long* out, hipStream_t st);
 // Core-ordered graph (core_graph.hip): rank[v] = new id of v (position in
 // order); lists of new owner i = the lists of order[i] in new ids, ordered
-// by the neighbours' kmax, descending.
+// by the neighbours' kmax, descending.  With ekh (and vk, the V_k), the
+// build also writes the E_k histogram: ekh[m] = out-edges whose endpoints'
+// smaller kmax is m, m = 0..kmax (kmax < 4096; else ekh is left alone).
 hipError_t build_core_graph(uint32_t V, uint64_t E, const uint32_t* order, const int32_t* kmaxes,
                             int32_t kmax, Csr out, Csr in, uint32_t* rank, uint32_t* cout_off,
                             int32_t* cout_deg, uint32_t* cout_nbr, uint32_t* cin_off,
-                            int32_t* cin_deg, uint32_t* cin_nbr, void* tmp, hipStream_t st);
+                            int32_t* cin_deg, uint32_t* cin_nbr, void* tmp, const uint32_t* vk,
+                            unsigned long long* ekh, hipStream_t st);
+// True when build_core_graph can write the E_k histogram for this kmax.
+inline bool core_graph_writes_ek(int32_t kmax) { return kmax + 1 <= 4096; }
 
 // ---- graph updates (maint_kernels.hip); pairs are (u, v) interleaved ---
 hipError_t launch_find_edges(const uint32_t* off, const int32_t* deg, const uint32_t* nbr,

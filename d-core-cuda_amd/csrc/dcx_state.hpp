@@ -180,6 +180,11 @@ struct dcx_graph {
     // set-up excluded).  Cost-model features.
     std::vector<uint32_t> h_peel_phases, h_der_rounds, h_far_rounds;
     std::vector<uint64_t> h_der_reads, h_far_reads;
+    // Time (ms, 0: none) and team width of row k's last run of each kind of
+    // team row (peeled, first derived, derived, derived from row 0): the
+    // cost model's predictions for row k and that kind are scaled to it.
+    std::vector<float> h_meas_ms[4];
+    std::vector<uint16_t> h_meas_team[4];
     std::vector<int8_t> h_pass_variant;  // [kmax+2] variant that last ran each pass
     std::vector<uint16_t> h_pass_team;   // [kmax+2] workgroups that ran it
     int32_t* rows = nullptr;     // (kmax+1) x V

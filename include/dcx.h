@@ -457,18 +457,21 @@ dcx_status dcx_compare_decompositions(dcx_ctx* ctx, dcx_graph* a, dcx_graph* b,
 /* ---- learned variant scheduler ---------------------------------------
  * A small fp32 MLP (8 inputs, two hidden layers of 32 ReLU units, one
  * output) predicts log2(ms) of a team row of the batched launch from its
- * features: sizes and density of the (k,0)-core, k's place among the rows,
- * the vertices that left the core since row k-1, the kind of the row
- * (peeled; derived from the row above as the first row of a chain, whose
- * time includes the supports' set-up; derived later in a chain), the
- * phase or round count of its last run of that kind, and the team width
- * it would run on.  Forward, backward and Adam run in hand-written HIP
- * kernels.  Attached to a context (and cfg.chain_rows = 0), it chooses per
- * team row between the two exact variants -- peel it, or derive it from
- * the row above by h-index refinement -- and the team widths: the plan
- * whose predicted makespan is smallest.  Without it every chain_rows-th row
- * from row 0 is peeled and the widths follow each unit's share of the
- * work.  Every plan gives the same bits.  No reference counterpart (the
+ * features: sizes and density of the (k,0)-core, the vertices that left
+ * the core since row k-1, the kind of the row (peeled; derived from the
+ * row above as the first row of a chain, whose time includes the
+ * supports' set-up; derived later in a chain; derived from row 0 as an
+ * anchor), the phase or round count and list reads of its last run of
+ * that kind, and the team width it would run on.  Forward, backward and
+ * Adam run in hand-written HIP kernels.  Attached to a context (and
+ * cfg.chain_rows = 0), it chooses per team row among the exact variants --
+ * peel it, derive it from row 0, or derive it from the row above by
+ * h-index refinement -- and the team widths: the plan whose predicted
+ * makespan is smallest, each prediction scaled by the ratio of measured to
+ * predicted time of the row's last run of that kind on this graph.
+ * Without it every chain_rows-th row from row 0 is an anchor of the kind
+ * cfg.anchors names and the widths follow each unit's share of the work.
+ * Every plan gives the same bits.  No reference counterpart (the
  * reference's schedule is fixed, main.cu:197-262). */
 typedef struct dcx_mlp dcx_mlp;
 uint32_t dcx_mlp_param_count(void);
