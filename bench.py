@@ -292,8 +292,10 @@ def train_learned_plan(dcx, ctx, g, log, Xs=None, Ts=None):
     exploration runs (every 8th team row an anchor: peeled with team caps 32
     and 4, so the samples cover narrow teams too, and derived from row 0),
     then two refinement rounds in which the model plans a run and is
-    retrained with that run's log.  Returns the attached model, the sample
-    count and the refinement runs' times."""
+    retrained with that run's log, and three settling runs of the final
+    model's plan (each run's measured row times calibrate the next plan's
+    predictions).  Returns the attached model, the sample count and the
+    refinement and settling runs' times."""
     Xs = list(Xs or [])
     Ts = list(Ts or [])
 
@@ -329,6 +331,10 @@ def train_learned_plan(dcx, ctx, g, log, Xs=None, Ts=None):
             g.decompose(dcx.VARIANT_BATCHED)
             refine_ms.append(round(ctx.timing().total_ms, 2))
             collect()
+    for _ in range(3):
+        g.decompose(dcx.VARIANT_BATCHED)
+        refine_ms.append(round(ctx.timing().total_ms, 2))
+    log(f"refinement and settling runs: {refine_ms} ms")
     return model, n, refine_ms
 
 
@@ -476,11 +482,12 @@ def run_headline(args):
         "schedule": {"kind": args.schedule, "model_used": used_model,
                      "training": f"{n_samples} samples: this graph's logged team rows under the "
                                  "fixed rule, three exploration runs (every 8th row an anchor: "
-                                 "peeled with team caps 32 and 4, derived from row 0) and two "
-                                 "refinement rounds planned by the model" + (
+                                 "peeled with team caps 32 and 4, derived from row 0), two "
+                                 "refinement rounds planned by the model and three settling "
+                                 "runs (measured row times calibrate the predictions)" + (
                                      ", and smaller graphs' at several team caps"
                                      if args.full_train else ""),
-                     "refinement_rounds_ms": refine_ms,
+                     "refinement_and_settling_runs_ms": refine_ms,
                      "fixed_rule_ms": round(min(heur), 2),
                      "learned_speedup_vs_fixed_rule": round(min(heur) / ms_per_step, 3),
                      "bit_exact_vs_fixed_rule": state_hash == h_hash},

@@ -356,10 +356,15 @@ int bits_for(uint32_t v) {
 
 }  // namespace
 
+// Scratch words of one side's list build: scan words, the long-list
+// queue's length (64 words, keeping the queue 256-byte aligned) and queue.
+size_t side_tmp_words(uint32_t V, uint64_t E) {
+    return scan_tmp_words(V) + 1 + 64 + ((E ? E : 1) / kShortList + 64);
+}
+
 size_t core_graph_tmp_bytes(uint32_t V, uint64_t E, int32_t kmax) {
-    if (kmax + 1 <= kListBins)  // packed words + scan words + long-list queue and its length
-        return 8ull * V + 4ull * (scan_tmp_words(V) + 1) + 4ull * ((E ? E : 1) / kShortList + 64) +
-               256;
+    if (kmax + 1 <= kListBins)  // packed words; per side scan words, long-list queue, its length
+        return 8ull * V + 2 * 4ull * (side_tmp_words(V, E)) + 256;
     // keys + alternate keys + scan words + radix table
     return 16ull * (E ? E : 1) + 4ull * (scan_tmp_words(V) + 1) +
            4ull * (radix_tmp_words(E) + 1) + 256;
@@ -376,14 +381,12 @@ hipError_t build_core_graph(uint32_t V, uint64_t E, const uint32_t* order, const
                             int32_t kmax, Csr out, Csr in, uint32_t* rank, uint32_t* cout_off,
                             int32_t* cout_deg, uint32_t* cout_nbr, uint32_t* cin_off,
                             int32_t* cin_deg, uint32_t* cin_nbr, void* tmp, const uint32_t* vk,
-                            unsigned long long* ekh, hipStream_t st) {
+                            unsigned long long* ekh, hipStream_t st, hipStream_t st2,
+                            hipEvent_t fork, hipEvent_t join) {
     if (V == 0) return hipSuccess;
     char* p = static_cast<char*>(tmp);
     if (kmax + 1 <= kListBins) {
         uint64_t* kr = reinterpret_cast<uint64_t*>(p);
-        uint32_t* scan_tmp = reinterpret_cast<uint32_t*>(kr + V);
-        uint32_t* longn = scan_tmp + scan_tmp_words(V) + 1;
-        uint32_t* longq = longn + 64;  // 256-byte aligned from longn
         hipLaunchKernelGGL(k_rank_packed, dim3(grid_for(V, 256)), dim3(256), 0, st, order, kmaxes,
                            V, rank, kr);
         if (ekh) {
@@ -391,34 +394,52 @@ hipError_t build_core_graph(uint32_t V, uint64_t E, const uint32_t* order, const
                 hipMemsetAsync(ekh, 0, ((size_t)kmax + 1) * sizeof(unsigned long long), st);
             if (me != hipSuccess) return me;
         }
+        // the in side on st2 (when given) beside the out side: both are
+        // bound by their random gathers, and each side's long lists end
+        // on a few workgroups that the other side's kernels fill in beside
+        const bool two = st2 && fork && join;
+        if (two) {
+            hipError_t e = hipEventRecord(fork, st);
+            if (e == hipSuccess) e = hipStreamWaitEvent(st2, fork, 0);
+            if (e != hipSuccess) return e;
+        }
         const Csr sides[2] = {out, in};
         uint32_t* offs[2] = {cout_off, cin_off};
         int32_t* degs[2] = {cout_deg, cin_deg};
         uint32_t* nbrs[2] = {cout_nbr, cin_nbr};
         for (int s = 0; s < 2; ++s) {
+            const hipStream_t ss = s == 1 && two ? st2 : st;
+            uint32_t* scan_tmp = reinterpret_cast<uint32_t*>(kr + V) + s * side_tmp_words(V, E);
+            uint32_t* longn = scan_tmp + scan_tmp_words(V) + 1;
+            uint32_t* longq = longn + 64;  // 256-byte aligned from longn
             uint32_t* dnew = reinterpret_cast<uint32_t*>(degs[s]);
-            hipLaunchKernelGGL(k_new_degrees, dim3(grid_for(V, 256)), dim3(256), 0, st, order, V,
+            hipLaunchKernelGGL(k_new_degrees, dim3(grid_for(V, 256)), dim3(256), 0, ss, order, V,
                                sides[s].deg, dnew);
-            hipError_t e = launch_excl_scan_u32(dnew, offs[s], V, scan_tmp, st);
+            hipError_t e = launch_excl_scan_u32(dnew, offs[s], V, scan_tmp, ss);
             if (e != hipSuccess) return e;
-            hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, st, offs[s] + V, (uint32_t)E);
+            hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, ss, offs[s] + V, (uint32_t)E);
             if (E == 0) continue;
-            hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, st, longn, 0u);
+            hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(1), 0, ss, longn, 0u);
             if (s == 0 && ekh) {
                 hipLaunchKernelGGL(k_lists_short<true>, dim3(grid_for(V, 64, 2048)), dim3(1024),
-                                   0, st, order, rank, V, sides[s].off, sides[s].deg,
+                                   0, ss, order, rank, V, sides[s].off, sides[s].deg,
                                    sides[s].nbr, offs[s], nbrs[s], longq, longn, vk, kmax, ekh);
-                hipLaunchKernelGGL(k_lists_long<true>, dim3(4096), dim3(kLongBlk), 0, st, order,
+                hipLaunchKernelGGL(k_lists_long<true>, dim3(4096), dim3(kLongBlk), 0, ss, order,
                                    kr, sides[s].off, sides[s].deg, sides[s].nbr, offs[s], nbrs[s],
                                    longq, longn, kmax, ekh);
             } else {
-                hipLaunchKernelGGL(k_lists_short<false>, dim3(grid_for(V, 64)), dim3(1024), 0, st,
+                hipLaunchKernelGGL(k_lists_short<false>, dim3(grid_for(V, 64)), dim3(1024), 0, ss,
                                    order, rank, V, sides[s].off, sides[s].deg, sides[s].nbr,
                                    offs[s], nbrs[s], longq, longn, vk, kmax, nullptr);
-                hipLaunchKernelGGL(k_lists_long<false>, dim3(4096), dim3(kLongBlk), 0, st, order,
+                hipLaunchKernelGGL(k_lists_long<false>, dim3(4096), dim3(kLongBlk), 0, ss, order,
                                    kr, sides[s].off, sides[s].deg, sides[s].nbr, offs[s], nbrs[s],
                                    longq, longn, kmax, nullptr);
             }
+        }
+        if (two) {
+            hipError_t e = hipEventRecord(join, st2);
+            if (e == hipSuccess) e = hipStreamWaitEvent(st, join, 0);
+            if (e != hipSuccess) return e;
         }
         return hipGetLastError();
     }

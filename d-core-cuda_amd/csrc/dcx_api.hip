@@ -115,7 +115,8 @@ void audit_adjacency(dcx_ctx* c, dcx_graph* g) {
 }
 
 // Relabels by kmax and sorts every list (see core_graph.hip); with_ek:
-// the E_k histogram (g->ekh) comes out of the same pass.
+// the E_k histogram (g->ekh) comes out of the same pass, and the in side is
+// built on the side stream beside the out side.
 void build_core(dcx_ctx* c, dcx_graph* g, bool with_ek = false) {
     const uint32_t V = g->V;
     if (!g->audited) audit_adjacency(c, g);  // once per edit, not per rebuild
@@ -140,7 +141,7 @@ void build_core(dcx_ctx* c, dcx_graph* g, bool with_ek = false) {
                              Csr{g->in_off, g->in_deg, g->in_nbr}, g->rank, g->cout_off,
                              g->cout_deg, g->cout_nbr, g->cin_off, g->cin_deg, g->cin_nbr, c->tmp,
                              g->vk, with_ek ? reinterpret_cast<unsigned long long*>(g->ekh) : nullptr,
-                             c->stream));
+                             c->stream, with_ek ? c->side : nullptr, c->fork, c->join));
     g->core_valid = true;
 }
 
@@ -184,7 +185,7 @@ void grow_keep(dcx_ctx* c, T** p, size_t* cap, size_t n, size_t keep) {
 // and lmax in g->row0_stats; returns kmax.
 int32_t run_kmax_with_row0(dcx_ctx* c, dcx_graph* g) {
     const uint32_t V = g->V;
-    const int G1 = std::max(1, c->grid_blocks * g->row0_eighths / 8), G0 = c->grid_blocks - G1;
+    const int G1 = std::max(1, c->grid_blocks * g->row0_16ths / 16), G0 = c->grid_blocks - G1;
     ensure_scratch(c, 2ull * kScratchArrays * V * 4);
     if (!c->ctrs2) {
         dev_alloc(c, &c->ctrs2, kCtrWords);
@@ -388,10 +389,10 @@ void compute_kmax(dcx_ctx* c, dcx_graph* g, int variant = DCX_VARIANT_BATCHED,
         float r_end = 0, o_end = 0;
         HIPX(c, hipEventElapsedTime(&r_end, c->ev[0], c->row0_done));
         HIPX(c, hipEventElapsedTime(&o_end, c->ev[0], c->order_done));
-        if (r_end > o_end && g->row0_eighths < 4)
-            ++g->row0_eighths;  // the library stream waited for row 0
-        else if (r_end < c->timing.kmax_ms && g->row0_eighths > 1)
-            --g->row0_eighths;  // row 0 ended before the kmax pass
+        if (r_end > o_end && g->row0_16ths < 8)
+            ++g->row0_16ths;  // the library stream waited for row 0
+        else if (r_end < c->timing.kmax_ms && g->row0_16ths > 2)
+            --g->row0_16ths;  // row 0 ended before the kmax pass
     }
     g->kmax_valid = true;
     g->order_valid = true;

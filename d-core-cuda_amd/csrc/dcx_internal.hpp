@@ -267,11 +267,14 @@ hipError_t launch_audit(uint32_t V, Csr side, unsigned long long* out, hipStream
 // by the neighbours' kmax, descending.  With ekh (and vk, the V_k), the
 // build also writes the E_k histogram: ekh[m] = out-edges whose endpoints'
 // smaller kmax is m, m = 0..kmax (kmax < 4096; else ekh is left alone).
+// With st2 and two events, the in side is built on st2 beside the out side
+// (st waits for it before returning).
 hipError_t build_core_graph(uint32_t V, uint64_t E, const uint32_t* order, const int32_t* kmaxes,
                             int32_t kmax, Csr out, Csr in, uint32_t* rank, uint32_t* cout_off,
                             int32_t* cout_deg, uint32_t* cout_nbr, uint32_t* cin_off,
                             int32_t* cin_deg, uint32_t* cin_nbr, void* tmp, const uint32_t* vk,
-                            unsigned long long* ekh, hipStream_t st);
+                            unsigned long long* ekh, hipStream_t st, hipStream_t st2 = nullptr,
+                            hipEvent_t fork = nullptr, hipEvent_t join = nullptr);
 // True when build_core_graph can write the E_k histogram for this kmax.
 inline bool core_graph_writes_ek(int32_t kmax) { return kmax + 1 <= 4096; }
 

@@ -164,11 +164,11 @@ struct dcx_graph {
     uint32_t* row0_stats = nullptr;
     bool row0_ready = false;
     bool row0_pending = false;  // row 0's pass queued on side2, not yet joined
-    // Eighths of the grid row 0's pass takes beside the kmax pass, adapted
-    // after every such run: both passes are chains of dependent phases on
-    // power-law graphs (few workgroups suffice) but throughput-bound on
-    // graphs with few levels (half each).
-    int32_t row0_eighths = 2;
+    // Sixteenths of the grid row 0's pass takes beside the kmax pass (2..8),
+    // adapted after every such run: both passes are chains of dependent
+    // phases on power-law graphs (few workgroups suffice) but
+    // throughput-bound on graphs with few levels (half each).
+    int32_t row0_16ths = 4;
     size_t rows_cleared = 0;  // cells of `rows` zeroed on side2 for the next rows launch
     uint32_t* pstats = nullptr;  // [(kmax+2) * kStatWords] device pass log: kmax pass, rows
     size_t pstats_cap = 0;
