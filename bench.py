@@ -100,6 +100,9 @@ def main():
     ap.add_argument("--serve-batches", type=int, default=10000)
     ap.add_argument("--serve-loads", default="0.5,0.9",
                     help="serve: Poisson loads as fractions of the measured capacity")
+    ap.add_argument("--serve-schedule", default="learned", choices=["learned", "workshare"],
+                    help="serve: the recomputations' rows planned by a cost model trained on "
+                         "the base graph (untimed), or the fixed rule")
     ap.add_argument("--scale", type=int, default=None,
                     help="R-MAT scale (headline: 24, rmat: 20)")
     ap.add_argument("--variant", default="batched", choices=["batched", "peel", "hindex"])
@@ -864,7 +867,10 @@ def run_serve(args):
     run in four segments with the maintained state checked against a full
     GPU recomputation (dcx_state_hash) after each.  Then Poisson arrivals at
     the given fractions of that capacity on a fresh copy of the base graph;
-    latency = completion - scheduled arrival."""
+    latency = completion - scheduled arrival.  With the learned schedule a
+    cost model is trained on the base graph first (untimed, the headline's
+    protocol) and stays attached: the recomputations a group falls back to
+    plan their rows with it."""
     import dcx
 
     scale, nb = args.serve_scale, args.serve_batches
@@ -883,6 +889,14 @@ def run_serve(args):
         g = ctx.upload_edges(V, src, dst, slack=4)
         g.decompose()
         return g
+
+    model = None
+    if args.serve_schedule == "learned":
+        def log(msg):
+            print(f"# {msg}", file=sys.stderr, flush=True)
+        g = fresh()
+        model, n_samples, _ = train_learned_plan(dcx, ctx, g, log)
+        g.free()
 
     def collect(sv, res, want, label):
         last = time.perf_counter()
@@ -964,6 +978,9 @@ def run_serve(args):
                       "final_state_equals_recomputed": bool(ok),
                       "failed_requests": sum(1 for r in res.values() if r["status"] != 0)})
         print(f"# load {frac}: {loads[-1]}", file=sys.stderr, flush=True)
+    ctx.set_scheduler(None)
+    if model is not None:
+        model.free()
     ctx.close()
     head = loads[-1] if loads else {}
     line = {
@@ -983,7 +1000,9 @@ def run_serve(args):
                                "(log-uniform 1..4096 edits, 70% insert / 30% delete, seed 3), "
                                "Poisson arrivals",
                    "V": V, "E": int(src.size), "edits": edits,
-                   "load_of_headline": head.get("load")},
+                   "load_of_headline": head.get("load"),
+                   "schedule": ("learned (cost model trained on the base graph)"
+                                if model is not None else "fixed rule")},
         "capacity_req_s": round(capacity, 2),
         "capacity_edits_s": round(edits / t_busy, 1),
         "loads": loads,

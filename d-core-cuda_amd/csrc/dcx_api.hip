@@ -729,11 +729,15 @@ std::vector<float> mlp_predict(dcx_ctx* c, dcx_mlp* m, const std::vector<float>&
 // far[k] (the anchor k is derived from row 0), chain_last[a] (last row of
 // a's chain, -1: none), and per anchor the width want[a] and the weight
 // wts[a] (predicted workgroup time, which places the units longest first).
-// Returns false if no plan fits.
+// With `best` (a plan measured on this graph), the new plan is taken only
+// if its predicted makespan beats best's, predicted the same way, by 3 %;
+// else best's plan is returned (*used_best).  Returns false if no plan fits.
 bool mlp_rows_plan(dcx_ctx* c, const dcx_graph* g, int32_t kr, uint32_t G, uint32_t team_max,
                    bool fixed, std::vector<char>& peel, std::vector<char>& far,
                    std::vector<int32_t>& chain_last, std::vector<uint32_t>& want,
-                   std::vector<double>& wts) {
+                   std::vector<double>& wts, const dcx_graph::RowsPlan* best = nullptr,
+                   bool* used_best = nullptr) {
+    if (used_best) *used_best = false;
     const uint32_t np = (uint32_t)kr + 1;
     const uint32_t cap = std::min(team_max, G / kXcds);
     std::vector<uint32_t> cands{1};
@@ -865,6 +869,33 @@ bool mlp_rows_plan(dcx_ctx* c, const dcx_graph* g, int32_t kr, uint32_t G, uint3
             lo = mid;
     }
     plan(hi, units);
+    if (!fixed && best && best->valid && best->want.size() == np &&
+        best->peel.size() == peel.size()) {
+        // best's predicted makespan: its units at their widths
+        double tb = 0.0;
+        bool ok = true;
+        for (uint32_t a = 0; a < np && ok; ++a) {
+            if (!(a == 0 ? best->chain_last[0] >= 0 : best->peel[a] != 0)) continue;
+            const uint32_t b = best->chain_last[a] >= 0 ? (uint32_t)best->chain_last[a] : a;
+            const int alt = a > 0 && best->far[a] ? 1 : 0;
+            uint32_t j = 0;
+            while (j < nc && cands[j] != best->want[a]) ++j;
+            if (j == nc) {
+                ok = false;
+                break;
+            }
+            tb = std::max(tb, unit_cost(a, alt, b, j));
+        }
+        if (ok && tb <= 1.03 * hi) {
+            peel = best->peel;
+            far = best->far;
+            chain_last = best->chain_last;
+            want = best->want;
+            wts = best->wts;
+            if (used_best) *used_best = true;
+            return true;
+        }
+    }
     if (!fixed) {
         peel.assign(peel.size(), 1);
         far.assign(far.size(), 0);
@@ -1087,6 +1118,7 @@ void compute_rows(dcx_ctx* c, dcx_graph* g, int variant, int32_t kr = -1) {
         std::vector<uint32_t> want;
         std::vector<double> wts;
         bool by_model = false;
+        g->last_plan.valid = false;
         if (c->sched_mlp && chains_ok) {
             const bool fixed = c->cfg.chain_rows != 0;
             std::vector<char> p2 = peel, f2 = far;
@@ -1094,11 +1126,29 @@ void compute_rows(dcx_ctx* c, dcx_graph* g, int variant, int32_t kr = -1) {
             // without a configured cap the model may give a unit a whole
             // XCD (the fixed rule stops at kTeamMax)
             const uint32_t mcap = c->cfg.team_max > 0 ? tmax : G / kXcds;
-            by_model = mlp_rows_plan(c, g, ksplit - 1, G, mcap, fixed, p2, f2, c2, want, wts);
+            // the fastest plan measured since the last edit, for the same rows
+            const dcx_graph::RowsPlan* best =
+                !fixed && g->best_plan.valid && g->best_plan.edits == g->edits &&
+                        g->best_plan.ksplit == ksplit
+                    ? &g->best_plan
+                    : nullptr;
+            by_model = mlp_rows_plan(c, g, ksplit - 1, G, mcap, fixed, p2, f2, c2, want, wts,
+                                     best);
             if (by_model) {
                 peel = p2;
                 far = f2;
                 chain_last = c2;
+                if (!fixed) {
+                    dcx_graph::RowsPlan& lp = g->last_plan;
+                    lp.valid = true;
+                    lp.ksplit = ksplit;
+                    lp.edits = g->edits;
+                    lp.peel = peel;
+                    lp.far = far;
+                    lp.chain_last = chain_last;
+                    lp.want = want;
+                    lp.wts = wts;
+                }
             }
         }
         // units: anchors with their chains, scheduled by unit (team) weight
@@ -1322,6 +1372,20 @@ void compute_rows(dcx_ctx* c, dcx_graph* g, int variant, int32_t kr = -1) {
         float ms = 0;
         HIPX(c, hipEventElapsedTime(&ms, c->sev[3], c->sev[2]));
         c->timing.rows_teams_ms = ms;
+        // the cost model's plan, measured: it becomes the best plan if it
+        // beat it (or is it, measured again)
+        dcx_graph::RowsPlan& lp = g->last_plan;
+        dcx_graph::RowsPlan& bp = g->best_plan;
+        if (lp.valid) {
+            lp.teams_ms = ms;
+            const bool same = bp.valid && bp.edits == lp.edits && bp.ksplit == lp.ksplit &&
+                              bp.peel == lp.peel && bp.far == lp.far &&
+                              bp.chain_last == lp.chain_last && bp.want == lp.want;
+            if (!bp.valid || bp.edits != lp.edits || bp.ksplit != lp.ksplit || same ||
+                ms < bp.teams_ms)
+                bp = lp;
+            lp.valid = false;
+        }
     }
     if (first_ran) {
         float ms = 0;

@@ -125,6 +125,7 @@ struct dcx_graph {
     bool served = false;  // owned by a running dcx_server: API calls refused
     bool broken = false;  // an edit failed half-way: the adjacency is not trusted
     bool audited = false; // the adjacency passed the audit and was not edited since
+    uint64_t edits = 0;   // edit batches applied (plans measured before one are stale)
     uint32_t V = 0;
     uint64_t E = 0;
     // Adjacency arrays: lists start at off[v] with room for capv[v] entries;
@@ -185,6 +186,21 @@ struct dcx_graph {
     // cost model's predictions for row k and that kind are scaled to it.
     std::vector<float> h_meas_ms[4];
     std::vector<uint16_t> h_meas_team[4];
+    // A plan of the cost model for the team rows 1..ksplit-1 (anchors,
+    // their kinds and chains, widths and weights) with the team launch's
+    // measured time: the fastest one measured since the last edit, and the
+    // one of the last launch until its time is known.
+    struct RowsPlan {
+        bool valid = false;
+        int32_t ksplit = -1;
+        uint64_t edits = 0;
+        std::vector<char> peel, far;
+        std::vector<int32_t> chain_last;
+        std::vector<uint32_t> want;
+        std::vector<double> wts;
+        float teams_ms = 0.0f;
+    };
+    RowsPlan best_plan, last_plan;
     std::vector<int8_t> h_pass_variant;  // [kmax+2] variant that last ran each pass
     std::vector<uint16_t> h_pass_team;   // [kmax+2] workgroups that ran it
     int32_t* rows = nullptr;     // (kmax+1) x V

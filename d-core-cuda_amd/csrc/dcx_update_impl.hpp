@@ -262,6 +262,7 @@ dcx_status edit_batch(dcx_ctx* c, dcx_graph* g, const Staged& L, uint32_t* H, ui
                       bool uploaded, const uint32_t* ins, const uint32_t* del, int32_t* m_del,
                       uint32_t* relocated) {
     g->audited = false;  // the lists change below
+    ++g->edits;
     g->row0_ready = false;
     const uint32_t n_ins = L.n_ins, n_del = L.n_del;
     auto dptr = [&](size_t idx) { return D + L.at[idx]; };
