@@ -448,9 +448,14 @@ __global__ __launch_bounds__(256) void k_chain_preset_values(ChainArgs A, ChainS
 }
 
 // Supports at own level for ranks [0, V_1): one thread or wavefront per
-// vertex; vertices with long lists are queued for k_chain_preset_long.
+// vertex; the lists of a vertex with long lists are cut into slices of
+// `slice` entries (both lists in a row), queued as (rank, first entry) for
+// k_chain_preset_long, its supports zeroed for the slices' counts.  The
+// queue's two words live in ql0..ql1 and qs0..qs1, each 2 cap words long
+// (contiguous in the chain state): the host picks slice >= 4 E_1 / cap, so
+// the slices (at most V_1 + 4 E_1 / slice) fit.
 __global__ __launch_bounds__(kCBlk) void k_chain_preset_supports(ChainArgs A, ChainState S,
-                                                                 uint32_t* slots) {
+                                                                 uint32_t* slots, uint32_t slice) {
     const uint32_t V1 = A.vk[1];
     uint64_t vis = 0;  // (not logged: row 1's log counts its own walks only)
     for (uint32_t base = (blockIdx.x * kCWaves + wave_id()) * W; base < V1;
@@ -459,7 +464,16 @@ __global__ __launch_bounds__(kCBlk) void k_chain_preset_supports(ChainArgs A, Ch
         const bool has = r < V1;
         const uint32_t len = has ? S.pk[r] + S.pc[r] : 0u;
         const bool lng = len >= kLongList;
-        if (lng) S.ql0[atomicAdd(&slots[kCqLong], 1u)] = r;
+        if (lng) {
+            S.os[r] = 0;
+            S.cs[r] = 0;
+            const uint32_t ns = (len + slice - 1) / slice;
+            const uint32_t at = atomicAdd(&slots[kCqLong], ns);
+            for (uint32_t i = 0; i < ns; ++i) {
+                S.ql0[at + i] = r;
+                S.qs0[at + i] = i * slice;
+            }
+        }
         by_lane_or_wave(has && !lng, len, r, [&](auto g, uint32_t rr) {
             constexpr int G = decltype(g)::value;
             uint32_t ck, cc;
@@ -472,20 +486,43 @@ __global__ __launch_bounds__(kCBlk) void k_chain_preset_supports(ChainArgs A, Ch
     }
 }
 
-// Supports of the queued long-list vertices: one workgroup each.
+// The queued slices of long lists, one workgroup per slice: counts of the
+// slice's member neighbours with value >= x(r), added to r's supports.
 __global__ __launch_bounds__(kCBlk) void k_chain_preset_long(ChainArgs A, ChainState S,
-                                                             uint32_t* slots) {
+                                                             uint32_t* slots, uint32_t slice) {
     __shared__ uint32_t red[2 * kCWaves];
-    const uint32_t nl = slots[kCqLong];
-    uint64_t vis = 0;  // (not logged)
-    for (uint32_t i = blockIdx.x; i < nl; i += gridDim.x) {
-        const uint32_t r = S.ql0[i];
-        uint32_t ck, cc;
-        count_ge<kCBlk>(A, S, r, S.x[r], red, ck, cc, vis);
-        if (threadIdx.x == 0) {
-            S.os[r] = (int32_t)ck;
-            S.cs[r] = (int32_t)cc;
+    const uint32_t nq = slots[kCqLong];
+    for (uint32_t q = blockIdx.x; q < nq; q += gridDim.x) {
+        const uint32_t r = S.ql0[q], j0 = S.qs0[q];
+        const uint32_t nk = S.pk[r], nc = S.pc[r];
+        const uint32_t j1 = min(nk + nc, j0 + slice);
+        const int32_t l = S.x[r];
+        const uint32_t* kn = A.K.nbr + A.K.off[r];
+        const uint32_t* cn = A.C.nbr + A.C.off[r];
+        uint32_t a = 0, b = 0;
+        for (uint32_t j = j0 + threadIdx.x; j < j1; j += kCBlk) {
+            if (j < nk)
+                a += S.x[kn[j]] >= l ? 1u : 0u;
+            else
+                b += S.x[cn[j - nk]] >= l ? 1u : 0u;
         }
+        a = wave_sum(a);
+        b = wave_sum(b);
+        if (lane_id() == 0) {
+            red[2 * wave_id()] = a;
+            red[2 * wave_id() + 1] = b;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t ta = 0, tb = 0;
+            for (int w = 0; w < kCWaves; ++w) {
+                ta += red[2 * w];
+                tb += red[2 * w + 1];
+            }
+            if (ta) atomicAdd(&S.os[r], (int32_t)ta);
+            if (tb) atomicAdd(&S.cs[r], (int32_t)tb);
+        }
+        __syncthreads();
     }
 }
 

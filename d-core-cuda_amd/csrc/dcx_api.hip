@@ -1306,7 +1306,7 @@ void compute_rows(dcx_ctx* c, dcx_graph* g, int variant, int32_t kr = -1) {
                                     basep, g->vk, g->rows,
                                     static_cast<uint32_t*>(c->scratch) + td.scratch_off,
                                     td.scratch_cap, c->pools + nt * s.sync_stride + td.slot_off,
-                                    V, c->stream));
+                                    V, g->h_ek.size() > 1 ? g->h_ek[1] : 0, c->stream));
                         chain0 = 1;
                     }
             HIPX(c, launch_rows_teams(basep, g->vk, g->rows, g->lmax_rows, d_teams, d_bt, d_ks,

@@ -211,11 +211,12 @@ hipError_t launch_rows_teams(const PassSpec& base, const uint32_t* vk, int32_t* 
 // Sets up row 0's chain (rows 1.. derived from row 0) on the whole grid:
 // values, member prefixes and supports for row 1 in the chain state of its
 // team (team_scratch: 12 arrays of cap >= V words; slots: the team's slot
-// words, zeroed).  Row 0 must be in rows; the team launch then runs that
-// chain with chain0_preset = 1.
+// words, zeroed; e1: E_1, which sizes the slices of long lists).  Row 0
+// must be in rows; the team launch then runs that chain with
+// chain0_preset = 1.
 hipError_t launch_chain_preset(const PassSpec& base, const uint32_t* vk, int32_t* rows,
                                uint32_t* team_scratch, uint64_t cap, uint32_t* slots, uint32_t V,
-                               hipStream_t st);
+                               uint64_t e1, hipStream_t st);
 // Rows whose core has at most `cap` (<= kLdsMaxMembers) members, one
 // LDS-resident workgroup each, taken longest first from ks[0..nks) through
 // the counter *queue (zero at launch).  Scratch: blocks * kScratchArrays *

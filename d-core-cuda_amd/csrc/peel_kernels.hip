@@ -1764,7 +1764,7 @@ hipError_t launch_rows_teams(const PassSpec& base, const uint32_t* vk, int32_t* 
 
 hipError_t launch_chain_preset(const PassSpec& base, const uint32_t* vk, int32_t* rows,
                                uint32_t* team_scratch, uint64_t cap, uint32_t* slots, uint32_t V,
-                               hipStream_t st) {
+                               uint64_t e1, hipStream_t st) {
     chain::ChainArgs A;
     A.V = base.V;
     A.K = base.K;
@@ -1778,10 +1778,15 @@ hipError_t launch_chain_preset(const PassSpec& base, const uint32_t* vk, int32_t
     const chain::ChainState S = chain::chain_state(team_scratch, cap);
     const uint32_t vb = std::max(1u, std::min(4096u, (V + 255u) / 256u));
     hipLaunchKernelGGL(chain::k_chain_preset_values, dim3(vb), dim3(256), 0, st, A, S);
+    // slices of the long lists: at least 4096 entries, and few enough that
+    // V_1 + 4 E_1 / slice entries fit the queue's 2 cap words
+    const uint64_t want = (4 * e1 + cap - 1) / std::max<uint64_t>(cap, 1);
+    const uint32_t slice = (uint32_t)std::min<uint64_t>(
+        std::max<uint64_t>(4096, (want + 63) / 64 * 64), 1u << 30);
     hipLaunchKernelGGL(chain::k_chain_preset_supports, dim3(2048), dim3(chain::kCBlk), 0, st, A,
-                       S, slots);
-    hipLaunchKernelGGL(chain::k_chain_preset_long, dim3(1024), dim3(chain::kCBlk), 0, st, A, S,
-                       slots);
+                       S, slots, slice);
+    hipLaunchKernelGGL(chain::k_chain_preset_long, dim3(2048), dim3(chain::kCBlk), 0, st, A, S,
+                       slots, slice);
     return hipGetLastError();
 }
 
