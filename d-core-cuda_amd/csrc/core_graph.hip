@@ -307,30 +307,69 @@ __global__ __launch_bounds__(kLongBlk) void k_lists_long(const uint32_t* order, 
 }
 
 // out[0] += sum of deg, out[1] += entries of [0, deg) that are not vertex
-// ids (one wavefront per vertex).
-__global__ void k_audit(uint32_t V, const uint32_t* off, const int32_t* deg, const uint32_t* nbr,
-                        unsigned long long* out) {
-    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / W;
-    const uint32_t nw = gridDim.x * blockDim.x / W;
+// ids, plus the negative degrees.  Tiles of kAuditTile consecutive vertices:
+// their list lengths are prefix-summed in LDS and the tile's entries are
+// dealt over all threads (owner found by binary search), so a tile of short
+// lists keeps every lane busy and a hub's list is read by the whole block.
+constexpr int kAuditTile = 1024;
+__global__ __launch_bounds__(kAuditTile) void k_audit(uint32_t V, const uint32_t* off,
+                                                      const int32_t* deg, const uint32_t* nbr,
+                                                      unsigned long long* out) {
+    __shared__ uint32_t s_off[kAuditTile], s_scan[kAuditTile];
+    __shared__ uint32_t s_wsum[kAuditTile / W];
+    __shared__ unsigned long long s_sum, s_bad;
+    const uint32_t tid = threadIdx.x;
+    if (tid == 0) s_sum = s_bad = 0;
+    __syncthreads();
     uint64_t sum = 0, bad = 0;
-    for (uint32_t v = wave; v < V; v += nw) {
-        const int32_t d = deg[v];
-        if (lane_id() == 0) sum += d < 0 ? 0 : (uint64_t)d;
-        if (d < 0) {
-            if (lane_id() == 0) ++bad;
-            continue;
+    for (uint32_t base = blockIdx.x * kAuditTile; base < V; base += gridDim.x * kAuditTile) {
+        const uint32_t v = base + tid;
+        uint32_t n = 0;
+        if (v < V) {
+            const int32_t d = deg[v];
+            if (d < 0)
+                ++bad;
+            else
+                n = (uint32_t)d;
+            sum += n;
+            s_off[tid] = off[v];
         }
-        const uint32_t* l = nbr + off[v];
-        for (uint32_t j = lane_id(); j < (uint32_t)d; j += W) bad += l[j] >= V;
+        // block-wide exclusive scan of n (lengths < 2^32 per tile: a tile's
+        // lists fit the 32-bit list offsets)
+        uint32_t inc = n;
+        for (int d = 1; d < W; d <<= 1) {
+            const uint32_t y = __shfl_up(inc, d, W);
+            if (lane_id() >= d) inc += y;
+        }
+        if (lane_id() == W - 1) s_wsum[tid / W] = inc;
+        __syncthreads();
+        uint32_t before = 0, total = 0;
+        for (int w = 0; w < kAuditTile / W; ++w) {
+            const uint32_t x = s_wsum[w];
+            if (w < (int)(tid / W)) before += x;
+            total += x;
+        }
+        s_scan[tid] = before + inc - n;
+        __syncthreads();
+        const uint32_t items = min(V - base, (uint32_t)kAuditTile);
+        for (uint32_t e = tid; e < total; e += kAuditTile) {
+            uint32_t lo = 0, hi = items - 1;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi + 1) >> 1;
+                if (s_scan[mid] <= e)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            bad += nbr[s_off[lo] + (e - s_scan[lo])] >= V ? 1u : 0u;
+        }
+        __syncthreads();
     }
     for (int dd = W / 2; dd > 0; dd >>= 1) {
         sum += __shfl_xor(sum, dd, W);
         bad += __shfl_xor(bad, dd, W);
     }
     // one pair of atomics per block (same-address atomics serialise)
-    __shared__ unsigned long long s_sum, s_bad;
-    if (threadIdx.x == 0) s_sum = s_bad = 0;
-    __syncthreads();
     if (lane_id() == 0) {
         atomicAdd(&s_sum, (unsigned long long)sum);
         atomicAdd(&s_bad, (unsigned long long)bad);
@@ -372,8 +411,8 @@ size_t core_graph_tmp_bytes(uint32_t V, uint64_t E, int32_t kmax) {
 
 hipError_t launch_audit(uint32_t V, Csr side, unsigned long long* out, hipStream_t st) {
     if (V == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_audit, dim3(grid_for(V, 16, 2048)), dim3(1024), 0, st, V, side.off,
-                       side.deg, side.nbr, out);
+    hipLaunchKernelGGL(k_audit, dim3(grid_for(V, kAuditTile, 2048)), dim3(kAuditTile), 0, st, V,
+                       side.off, side.deg, side.nbr, out);
     return hipGetLastError();
 }
 
