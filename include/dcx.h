@@ -488,7 +488,9 @@ dcx_status dcx_mlp_gradient(dcx_mlp* m, const float* X, const float* T, uint32_t
 /* `epochs` full-batch Adam steps (beta 0.9 / 0.999) on the mean of the above. */
 dcx_status dcx_mlp_train(dcx_mlp* m, const float* X, const float* T, uint32_t n, uint32_t epochs,
                          float lr, float* final_loss);
-/* Attach (or with NULL detach) the model as the batched rows scheduler. */
+/* Attach (or with NULL detach) the model as the batched rows scheduler.
+ * Per graph, the fastest plan measured since the graph's last edit is kept
+ * and reused unless the model predicts a new plan at least 3 % faster. */
 dcx_status dcx_set_scheduler(dcx_ctx* ctx, dcx_mlp* m);
 /* Training samples from the resident decomposition's pass log (the team
  * rows of the batched launch, peeled or derived): features (with_log = 0:
