@@ -291,9 +291,10 @@ def oracle_sample(g, V, threads, log):
 
 def train_learned_plan(dcx, ctx, g, log, Xs=None, Ts=None):
     """The cost model's training protocol on graph g (untimed): the pass log
-    of the fixed rule's last run (already decomposed by the caller), three
-    exploration runs (every 8th team row an anchor: peeled with team caps 32
-    and 4, so the samples cover narrow teams too, and derived from row 0),
+    of the fixed rule's last run (already decomposed by the caller), four
+    exploration runs (every 8th team row an anchor: peeled with team caps
+    32, 4 and 64, so the samples cover narrow and whole-XCD teams, and
+    derived from row 0 with team cap 64),
     then two refinement rounds in which the model plans a run and is
     retrained with that run's log, and three settling runs of the final
     model's plan (each run's measured row times calibrate the next plan's
@@ -309,7 +310,7 @@ def train_learned_plan(dcx, ctx, g, log, Xs=None, Ts=None):
             Ts.append(T)
 
     collect()
-    for spacing, tmax, far in ((8, 0, 0), (8, 4, 0), (8, 0, 1)):
+    for spacing, tmax, far in ((8, 0, 0), (8, 4, 0), (8, 64, 0), (8, 64, 1)):
         ctx.set_rows_rule(spacing, tmax, far)
         g.decompose(dcx.VARIANT_BATCHED)
         kind = "derived from row 0" if far else "peeled"
@@ -374,9 +375,9 @@ def run_headline(args):
 
     # the fixed rule (every 32nd team row peeled, the rest derived from the
     # row above; widths by work share), then the cost model trained on the
-    # GPU from what the decompositions logged: the fixed rule's runs, three
-    # exploration runs (every 8th row an anchor: peeled on teams of up to 32
-    # and of up to 4 workgroups, and derived from row 0), and two refinement
+    # GPU from what the decompositions logged: the fixed rule's runs, four
+    # exploration runs (every 8th row an anchor: peeled on teams of up to
+    # 32, 4 and 64 workgroups, and derived from row 0), and two refinement
     # rounds in which the model plans a run and is retrained with that run's
     # log as well
     ctx.set_scheduler(None)
@@ -484,10 +485,10 @@ def run_headline(args):
             if fr_traffic and fr_alg else None),
         "schedule": {"kind": args.schedule, "model_used": used_model,
                      "training": f"{n_samples} samples: this graph's logged team rows under the "
-                                 "fixed rule, three exploration runs (every 8th row an anchor: "
-                                 "peeled with team caps 32 and 4, derived from row 0), two "
-                                 "refinement rounds planned by the model and three settling "
-                                 "runs (measured row times calibrate the predictions)" + (
+                                 "fixed rule, four exploration runs (every 8th row an anchor: "
+                                 "peeled with team caps 32, 4 and 64, derived from row 0 with "
+                                 "cap 64), two refinement rounds planned by the model and three "
+                                 "settling runs (measured row times calibrate the predictions)" + (
                                      ", and smaller graphs' at several team caps"
                                      if args.full_train else ""),
                      "refinement_and_settling_runs_ms": refine_ms,

@@ -68,9 +68,9 @@ def test_training_fits_smooth_target():
 
 def _train_on_logs(c, g, model, rounds=2):
     """The training protocol of the learned plan: the logs of the fixed rule,
-    of three exploration runs (every 8th team row an anchor: peeled with
-    team caps 32 and 4, derived from row 0) and of refinement rounds in
-    which the model plans a run itself."""
+    of four exploration runs (every 8th team row an anchor: peeled with
+    team caps 32, 4 and 64, derived from row 0 with cap 64) and of
+    refinement rounds in which the model plans a run itself."""
     Xs, Ts = [], []
 
     def collect():
@@ -83,7 +83,7 @@ def _train_on_logs(c, g, model, rounds=2):
     c.set_rows_rule(0, 0)
     g.decompose(dcx.VARIANT_BATCHED)
     collect()
-    for spacing, tmax, far in ((8, 0, 0), (8, 4, 0), (8, 0, 1)):
+    for spacing, tmax, far in ((8, 0, 0), (8, 4, 0), (8, 64, 0), (8, 64, 1)):
         c.set_rows_rule(spacing, tmax, far)
         g.decompose(dcx.VARIANT_BATCHED)
         collect()
