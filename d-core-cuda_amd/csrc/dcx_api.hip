@@ -861,7 +861,9 @@ bool mlp_rows_plan(dcx_ctx* c, const dcx_graph* g, int32_t kr, uint32_t G, uint3
     std::vector<Unit> units;
     if (!plan(hi, units)) return false;
     double lo = 0.0;
-    for (int it = 0; it < 50; ++it) {
+    // to 0.1 % of the makespan (each step evaluates a whole plan, and this
+    // runs on the host between the order kernels and the rows launch)
+    for (int it = 0; it < 60 && hi - lo > 1e-3 * hi; ++it) {
         const double mid = 0.5 * (lo + hi);
         if (plan(mid, units))
             hi = mid;
