@@ -310,11 +310,22 @@ __global__ __launch_bounds__(kLongBlk) void k_lists_long(const uint32_t* order, 
 // ids, plus the negative degrees.  Tiles of kAuditTile consecutive vertices:
 // their list lengths are prefix-summed in LDS and the tile's entries are
 // dealt over all threads (owner found by binary search), so a tile of short
-// lists keeps every lane busy and a hub's list is read by the whole block.
+// lists keeps every lane busy.  Lists of more than kAuditLong entries are
+// cut into slices of kAuditSlice, queued (v << 32 | first entry) for
+// k_audit_long, one wavefront per slice: R-MAT's hubs and dense vertices
+// have the lowest ids, and a tile of them would leave one block with a
+// large share of all edges.
+// A list that would reach past the nbr array (nlen entries), or whose
+// slices do not fit the queue (qcap), is counted bad and not read: the
+// audit itself stays in bounds whatever the degrees hold.
 constexpr int kAuditTile = 1024;
+constexpr uint32_t kAuditLong = 256;
+constexpr uint32_t kAuditSlice = 2048;
 __global__ __launch_bounds__(kAuditTile) void k_audit(uint32_t V, const uint32_t* off,
                                                       const int32_t* deg, const uint32_t* nbr,
-                                                      unsigned long long* out) {
+                                                      uint64_t nlen, unsigned long long* out,
+                                                      unsigned long long* longq, uint32_t* longn,
+                                                      uint32_t qcap) {
     __shared__ uint32_t s_off[kAuditTile], s_scan[kAuditTile];
     __shared__ uint32_t s_wsum[kAuditTile / W];
     __shared__ unsigned long long s_sum, s_bad;
@@ -327,12 +338,28 @@ __global__ __launch_bounds__(kAuditTile) void k_audit(uint32_t V, const uint32_t
         uint32_t n = 0;
         if (v < V) {
             const int32_t d = deg[v];
-            if (d < 0)
+            const uint32_t o = off[v];
+            if (d < 0) {
                 ++bad;
-            else
-                n = (uint32_t)d;
-            sum += n;
-            s_off[tid] = off[v];
+            } else {
+                sum += (uint32_t)d;
+                if ((uint64_t)o + (uint32_t)d > nlen)
+                    ++bad;  // not read
+                else
+                    n = (uint32_t)d;
+            }
+            s_off[tid] = o;
+            if (n > kAuditLong) {
+                const uint32_t ns = (n + kAuditSlice - 1) / kAuditSlice;
+                const uint32_t at = atomicAdd(longn, ns);
+                if ((uint64_t)at + ns <= qcap) {
+                    for (uint32_t i = 0; i < ns; ++i)
+                        longq[at + i] = ((unsigned long long)v << 32) | (i * kAuditSlice);
+                } else {
+                    ++bad;  // not read
+                }
+                n = 0;
+            }
         }
         // block-wide exclusive scan of n (lengths < 2^32 per tile: a tile's
         // lists fit the 32-bit list offsets)
@@ -381,6 +408,32 @@ __global__ __launch_bounds__(kAuditTile) void k_audit(uint32_t V, const uint32_t
     }
 }
 
+// The queued slices of long lists, one wavefront each: out[1] += their
+// entries >= V.
+__global__ __launch_bounds__(kAuditTile) void k_audit_long(uint32_t V, const uint32_t* off,
+                                                           const int32_t* deg, const uint32_t* nbr,
+                                                           unsigned long long* out,
+                                                           const unsigned long long* longq,
+                                                           const uint32_t* longn, uint32_t qcap) {
+    __shared__ unsigned long long s_bad;
+    if (threadIdx.x == 0) s_bad = 0;
+    __syncthreads();
+    const uint32_t nq = min(*longn, qcap);  // entries past qcap were not written
+    uint64_t bad = 0;
+    const uint32_t nw = gridDim.x * (kAuditTile / W);
+    for (uint32_t q = blockIdx.x * (kAuditTile / W) + threadIdx.x / W; q < nq; q += nw) {
+        const unsigned long long e = longq[q];
+        const uint32_t v = (uint32_t)(e >> 32), j0 = (uint32_t)e;
+        const uint32_t* l = nbr + off[v];
+        const uint32_t j1 = min((uint32_t)deg[v], j0 + kAuditSlice);
+        for (uint32_t j = j0 + lane_id(); j < j1; j += W) bad += l[j] >= V ? 1u : 0u;
+    }
+    for (int dd = W / 2; dd > 0; dd >>= 1) bad += __shfl_xor(bad, dd, W);
+    if (lane_id() == 0 && bad) atomicAdd(&s_bad, (unsigned long long)bad);
+    __syncthreads();
+    if (threadIdx.x == 0 && s_bad) atomicAdd(out + 1, s_bad);
+}
+
 unsigned grid_for(uint64_t n, unsigned per, unsigned cap = 8192) {
     uint64_t b = (n + per - 1) / per;
     if (!b) b = 1;
@@ -409,10 +462,26 @@ size_t core_graph_tmp_bytes(uint32_t V, uint64_t E, int32_t kmax) {
            4ull * (radix_tmp_words(E) + 1) + 256;
 }
 
-hipError_t launch_audit(uint32_t V, Csr side, unsigned long long* out, hipStream_t st) {
+// Slices of a valid side: at most E / kAuditLong lists are longer than
+// kAuditLong, each cut into at most 1 + n / kAuditSlice slices.
+static uint32_t audit_qcap(uint64_t E) {
+    return (uint32_t)std::min<uint64_t>(E / kAuditLong + E / kAuditSlice + 64, 0xffffffffu);
+}
+
+size_t audit_tmp_bytes(uint64_t E) { return 8ull * audit_qcap(E) + 256; }
+
+hipError_t launch_audit(uint32_t V, Csr side, uint64_t nlen, uint64_t E, unsigned long long* out,
+                        void* tmp, hipStream_t st) {
     if (V == 0) return hipSuccess;
+    uint32_t* longn = static_cast<uint32_t*>(tmp);
+    unsigned long long* longq = reinterpret_cast<unsigned long long*>(static_cast<char*>(tmp) + 256);
+    const uint32_t qcap = audit_qcap(E);
+    hipError_t e = hipMemsetAsync(longn, 0, 4, st);
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_audit, dim3(grid_for(V, kAuditTile, 2048)), dim3(kAuditTile), 0, st, V,
-                       side.off, side.deg, side.nbr, out);
+                       side.off, side.deg, side.nbr, nlen, out, longq, longn, qcap);
+    hipLaunchKernelGGL(k_audit_long, dim3(2048), dim3(kAuditTile), 0, st, V, side.off, side.deg,
+                       side.nbr, out, longq, longn, qcap);
     return hipGetLastError();
 }
 

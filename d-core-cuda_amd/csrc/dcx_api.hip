@@ -97,8 +97,13 @@ int32_t run_grid_pass(dcx_ctx* c, const PassSpec& p0, int a = -1, int b = -1) {
 void audit_adjacency(dcx_ctx* c, dcx_graph* g) {
     unsigned long long* d = reinterpret_cast<unsigned long long*>(c->d_scalars + 24);
     HIPX(c, hipMemsetAsync(d, 0, 32, c->stream));
-    HIPX(c, launch_audit(g->V, Csr{g->out_off, g->out_deg, g->out_nbr}, d, c->stream));
-    HIPX(c, launch_audit(g->V, Csr{g->in_off, g->in_deg, g->in_nbr}, d + 2, c->stream));
+    // (the queue of long-list slices is sized for a valid side of E
+    // entries; slices that do not fit count as bad entries)
+    ensure_bytes(c, &c->tmp, &c->tmp_bytes, audit_tmp_bytes(g->E));
+    HIPX(c, launch_audit(g->V, Csr{g->out_off, g->out_deg, g->out_nbr}, g->out_alloc, g->E, d,
+                         c->tmp, c->stream));
+    HIPX(c, launch_audit(g->V, Csr{g->in_off, g->in_deg, g->in_nbr}, g->in_alloc, g->E, d + 2,
+                         c->tmp, c->stream));
     HIPX(c, hipMemcpyAsync(c->h_scalars + 24, d, 32, hipMemcpyDeviceToHost, c->stream));
     sync(c);
     uint64_t w[4];

@@ -262,7 +262,11 @@ hipError_t launch_radix_sort_u64(uint64_t* keys, uint64_t* alt, uint64_t n, int 
 size_t core_graph_tmp_bytes(uint32_t V, uint64_t E, int32_t kmax);
 // Adds sum(deg) to out[0] and the count of list entries in [0, deg) that
 // are not vertex ids (< V) to out[1].
-hipError_t launch_audit(uint32_t V, Csr side, unsigned long long* out, hipStream_t st);
+// nlen: entries allocated for side.nbr (lists reaching past it are counted
+// bad, not read); E: the graph's edge count; tmp: audit_tmp_bytes(E) bytes.
+size_t audit_tmp_bytes(uint64_t E);
+hipError_t launch_audit(uint32_t V, Csr side, uint64_t nlen, uint64_t E, unsigned long long* out,
+                        void* tmp, hipStream_t st);
 // Core-ordered graph (core_graph.hip): rank[v] = new id of v (position in
 // order); lists of new owner i = the lists of order[i] in new ids, ordered
 // by the neighbours' kmax, descending.  With ekh (and vk, the V_k), the
