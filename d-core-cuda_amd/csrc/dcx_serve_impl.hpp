@@ -261,35 +261,90 @@ struct dcx_server {
             // requests complete with that status and touch nothing
             const dcx_status gs = prior != DCX_OK ? prior : guarded(c, [&]() -> dcx_status {
                 U.kmax_before = g->kmax_valid ? g->kmax : -1;
-                int32_t m_del = -1;
-                std::vector<uint32_t> ins_all;
-                bool changed = false, any_ins = false;
-                EditLog log;  // the group's net effect, for local maintenance
+                // 1. which of the group's pairs the graph holds before the
+                //    group: one find pass per request, one wait for all
                 for (size_t i = 0; i < grp.size(); ++i) {
                     ServeReq* r = grp[i];
+                    const Staged& L = r->L;
+                    uint32_t* D = dpool + r->off;
                     HIPX(c, hipStreamWaitEvent(c->stream, r->up, 0));
-                    int32_t md = -1;
+                    if (L.words > L.in_words)
+                        HIPX(c, hipMemsetAsync(D + L.in_words, 0, 4 * (L.words - L.in_words),
+                                               c->stream));
+                    HIPX(c, launch_find_edges(g->out_off, g->out_deg, g->out_nbr, D + L.at[L.i_del],
+                                              L.n_del, D + L.at[L.i_fdel], c->stream));
+                    HIPX(c, launch_find_edges(g->out_off, g->out_deg, g->out_nbr, D + L.at[L.i_ins],
+                                              L.n_ins, D + L.at[L.i_fins], c->stream));
+                    if (L.words > L.in_words)
+                        HIPX(c, hipMemcpyAsync(hpool + r->off + L.in_words, D + L.in_words,
+                                               4 * (L.words - L.in_words), hipMemcpyDeviceToHost,
+                                               c->stream));
+                }
+                sync(c);
+                // 2. each request in order against the graph as the requests
+                //    before it left it (as edit_batch would check it alone):
+                //    its deletions must be present, its insertions absent
+                //    unless it deletes them first; a request that fails
+                //    changes nothing
+                std::unordered_map<uint64_t, bool> now;  // pairs the accepted requests touched
+                auto present = [&](uint64_t key, uint32_t before) {
+                    const auto it = now.find(key);
+                    return it != now.end() ? it->second : before != 0;
+                };
+                EditLog log;  // the accepted requests' edits in order
+                for (size_t i = 0; i < grp.size(); ++i) {
+                    ServeReq* r = grp[i];
+                    const Staged& L = r->L;
+                    const uint32_t* H = hpool + r->off;
+                    const uint32_t* del = r->del.data();
+                    const uint32_t* ins = r->ins.data();
+                    dcx_status st = DCX_OK;
+                    for (uint32_t j = 0; j < L.n_del && st == DCX_OK; ++j)
+                        if (!present((uint64_t)del[2 * j] << 32 | del[2 * j + 1],
+                                     H[L.at[L.i_fdel] + j]))
+                            st = DCX_ERR_NOT_FOUND;
+                    if (st == DCX_OK && L.n_ins) {
+                        std::unordered_set<uint64_t> deleted;
+                        for (uint32_t j = 0; j < L.n_del; ++j)
+                            deleted.insert((uint64_t)del[2 * j] << 32 | del[2 * j + 1]);
+                        for (uint32_t j = 0; j < L.n_ins && st == DCX_OK; ++j) {
+                            const uint64_t key = (uint64_t)ins[2 * j] << 32 | ins[2 * j + 1];
+                            if (present(key, H[L.at[L.i_fins] + j]) && !deleted.count(key))
+                                st = DCX_ERR_INVALID_ARG;
+                        }
+                    }
+                    sts[i] = st;
+                    if (st != DCX_OK) continue;
+                    for (uint32_t j = 0; j < L.n_del; ++j)
+                        now[(uint64_t)del[2 * j] << 32 | del[2 * j + 1]] = false;
+                    for (uint32_t j = 0; j < L.n_ins; ++j)
+                        now[(uint64_t)ins[2 * j] << 32 | ins[2 * j + 1]] = true;
+                    log.add(del, L.n_del, false);
+                    log.add(ins, L.n_ins, true);
+                }
+                // 3. their net effect as one edit (valid by construction)
+                const NetEdits N = log.net();
+                const uint32_t n_ins = (uint32_t)(N.ins.size() / 2);
+                const uint32_t n_del = (uint32_t)(N.del.size() / 2);
+                const bool changed = n_ins + n_del > 0;
+                int32_t m_del = -1;
+                if (changed) {
+                    const Staged L = layout_batch(N.ins.data(), n_ins, N.del.data(), n_del);
+                    ensure_staging(c, L.words);
+                    fill_batch(L, c->upd_host, N.ins.data(), N.del.data());
                     uint32_t rel = 0;
-                    sts[i] = edit_batch(c, g, r->L, hpool + r->off, dpool + r->off, true,
-                                        r->ins.data(), r->del.data(), &md, &rel);
-                    if (sts[i] != DCX_OK) continue;
-                    m_del = std::max(m_del, md);
-                    log.add(r->del.data(), (uint32_t)(r->del.size() / 2), false);
-                    log.add(r->ins.data(), (uint32_t)(r->ins.size() / 2), true);
-                    ins_all.insert(ins_all.end(), r->ins.begin(), r->ins.end());
-                    changed = changed || !r->ins.empty() || !r->del.empty();
-                    any_ins = any_ins || !r->ins.empty();
+                    const dcx_status es = edit_batch(c, g, L, c->upd_host, c->upd_dev, false,
+                                                     N.ins.data(), N.del.data(), &m_del, &rel);
+                    if (es != DCX_OK) throw Fail{es};
                     U.relocated += rel;
                 }
+                U.net_deleted = (int32_t)n_del;
+                U.net_inserted = (int32_t)n_ins;
                 if (g->kmax_valid && changed && local_enabled(c, g)) {
-                    const NetEdits N = log.net();
-                    U.net_deleted = (int32_t)(N.del.size() / 2);
-                    U.net_inserted = (int32_t)(N.ins.size() / 2);
                     U.path = local_maintain(c, g, N, &U);
                     local_result(c, g, U.path);
                 } else if (g->kmax_valid) {
-                    maintain(c, g, ins_all.data(), (uint32_t)(ins_all.size() / 2), m_del, changed,
-                             !any_ins, &U);
+                    maintain(c, g, N.ins.data(), n_ins, m_del, changed, n_ins == 0, &U);
                 }
                 U.kmax_after = g->kmax_valid ? g->kmax : -1;
                 if (cfg.hash_states) hash = state_hash(c, g);
