@@ -132,3 +132,45 @@ def test_serve_statuses_and_budget_admission():
     assert fin_m == fin_a
     gids = [r["group"] for r in mix if r["group"]]
     assert gids == sorted(gids)  # groups complete in arrival order
+
+
+def test_serve_requests_that_depend_on_each_other_in_one_group():
+    """Requests of one group that touch the same edges: a delete, a
+    re-insert, a second delete, an insert of a new edge, a second insert of
+    it (must fail: the edge exists after the first), a delete of an edge that
+    was never there (must fail), a delete plus re-insert in one request.
+    Statuses and the state after each request equal the one-request-per-
+    group run, and the final state equals the oracle's recomputation."""
+    V, src, dst, reqs = stream(nb=6, seed=11)
+    # pairs the base graph or any request of the stream holds
+    seen = set(zip(src.tolist(), dst.tolist()))
+    for ins, dels in reqs:
+        seen.update(map(tuple, dels.tolist()))
+        seen.update(map(tuple, ins.tolist()))
+    e = (int(src[1]), int(dst[1]))
+    new = next((u, V - 1 - u) for u in range(V) if u != V - 1 - u and (u, V - 1 - u) not in seen)
+    gone = next((V - 1, v) for v in range(V - 1) if (V - 1, v) not in seen and (V - 1, v) != new)
+    none = np.zeros((0, 2), np.uint32)
+    one = lambda p: np.array([p], np.uint32)
+    dep = [(none, one(e)), (one(e), none), (none, one(e)), (one(new), none), (one(new), none),
+           (none, one(gone)), (one(new), one(new))]
+    # the first request must find e present: keep it out of the stream's edits
+    base = [(ins, dels) for ins, dels in reqs
+            if e not in set(map(tuple, dels.tolist())) | set(map(tuple, ins.tolist()))]
+    mixed = base[:3] + dep + base[3:]
+    say("dependent requests alone")
+    alone, _, fin_a = serve(V, src, dst, mixed, burst=False, max_group=1)
+    st = [r["status"] for r in alone]
+    k = len(base[:3])
+    assert st[k:k + 7] == [0, 0, 0, 0, 1, 8, 0], st
+    say("dependent requests in one group")
+    mix, stats, fin_m = serve(V, src, dst, mixed, burst=True, max_group=64)
+    assert [r["status"] for r in mix] == st
+    assert len({r["group"] for r in mix[k:k + 7]}) < 7  # grouped
+    for i, r in enumerate(mix):
+        last_of_group = i + 1 == len(mix) or mix[i + 1]["group"] != r["group"]
+        if last_of_group:
+            assert r["state_hash"] == alone[i]["state_hash"], i
+    assert fin_m == fin_a
+    accepted = [q for q, s in zip(mixed, st) if s == 0]
+    assert fin_a == oracle_hashes(V, src, dst, accepted)[-1]
