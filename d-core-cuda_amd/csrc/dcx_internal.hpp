@@ -284,8 +284,9 @@ hipError_t build_core_graph(uint32_t V, uint64_t E, const uint32_t* order, const
 inline bool core_graph_writes_ek(int32_t kmax) { return kmax + 1 <= 4096; }
 
 // ---- graph updates (maint_kernels.hip); pairs are (u, v) interleaved ---
-hipError_t launch_find_edges(const uint32_t* off, const int32_t* deg, const uint32_t* nbr,
-                             const uint32_t* pairs, uint32_t n, uint32_t* found, hipStream_t st);
+// found[i] = 1 if pair i is an edge (looked up on the shorter side).
+hipError_t launch_find_edges(Csr out, Csr in, const uint32_t* pairs, uint32_t n, uint32_t* found,
+                             hipStream_t st);
 hipError_t launch_check_room(const int32_t* deg, const uint32_t* capv, const uint32_t* gkey,
                              const uint32_t* gcnt, uint32_t ngroups, uint32_t* overflow,
                              hipStream_t st);

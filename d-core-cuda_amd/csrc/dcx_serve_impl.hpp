@@ -271,10 +271,12 @@ struct dcx_server {
                     if (L.words > L.in_words)
                         HIPX(c, hipMemsetAsync(D + L.in_words, 0, 4 * (L.words - L.in_words),
                                                c->stream));
-                    HIPX(c, launch_find_edges(g->out_off, g->out_deg, g->out_nbr, D + L.at[L.i_del],
-                                              L.n_del, D + L.at[L.i_fdel], c->stream));
-                    HIPX(c, launch_find_edges(g->out_off, g->out_deg, g->out_nbr, D + L.at[L.i_ins],
-                                              L.n_ins, D + L.at[L.i_fins], c->stream));
+                    const Csr so{g->out_off, g->out_deg, g->out_nbr};
+                    const Csr si{g->in_off, g->in_deg, g->in_nbr};
+                    HIPX(c, launch_find_edges(so, si, D + L.at[L.i_del], L.n_del,
+                                              D + L.at[L.i_fdel], c->stream));
+                    HIPX(c, launch_find_edges(so, si, D + L.at[L.i_ins], L.n_ins,
+                                              D + L.at[L.i_fins], c->stream));
                     if (L.words > L.in_words)
                         HIPX(c, hipMemcpyAsync(hpool + r->off + L.in_words, D + L.in_words,
                                                4 * (L.words - L.in_words), hipMemcpyDeviceToHost,

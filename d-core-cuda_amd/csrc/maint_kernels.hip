@@ -42,14 +42,18 @@ __device__ __forceinline__ int64_t wave_find(const uint32_t* list, uint32_t n, u
     return -1;
 }
 
-// found[i] = 1 if pair i (u = pairs[2i], v = pairs[2i+1]) is an edge.
-__global__ void k_find_edges(const uint32_t* off, const int32_t* deg, const uint32_t* nbr,
-                             const uint32_t* pairs, uint32_t n, uint32_t* found) {
+// found[i] = 1 if pair i (u = pairs[2i], v = pairs[2i+1]) is an edge,
+// looked up in the shorter of u's out-list and v's in-list (both hold it;
+// an R-MAT edge usually has a hub at one end).
+__global__ void k_find_edges(Csr out, Csr in, const uint32_t* pairs, uint32_t n,
+                             uint32_t* found) {
     const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / W;
     const uint32_t nw = gridDim.x * blockDim.x / W;
     for (uint32_t i = wave; i < n; i += nw) {
         const uint32_t u = pairs[2 * i], v = pairs[2 * i + 1];
-        const int64_t p = wave_find(nbr + off[u], (uint32_t)deg[u], v);
+        const uint32_t du = (uint32_t)out.deg[u], dv = (uint32_t)in.deg[v];
+        const int64_t p = du <= dv ? wave_find(out.nbr + out.off[u], du, v)
+                                   : wave_find(in.nbr + in.off[v], dv, u);
         if (lane_id() == 0) found[i] = p >= 0;
     }
 }
@@ -152,11 +156,11 @@ unsigned blocks_for(uint64_t n, unsigned per_block, unsigned cap = 4096) {
 
 }  // namespace
 
-hipError_t launch_find_edges(const uint32_t* off, const int32_t* deg, const uint32_t* nbr,
-                             const uint32_t* pairs, uint32_t n, uint32_t* found, hipStream_t st) {
+hipError_t launch_find_edges(Csr out, Csr in, const uint32_t* pairs, uint32_t n, uint32_t* found,
+                             hipStream_t st) {
     if (!n) return hipSuccess;
-    hipLaunchKernelGGL(k_find_edges, dim3(blocks_for(n, 4)), dim3(256), 0, st, off, deg, nbr,
-                       pairs, n, found);
+    hipLaunchKernelGGL(k_find_edges, dim3(blocks_for(n, 4)), dim3(256), 0, st, out, in, pairs, n,
+                       found);
     return hipGetLastError();
 }
 
