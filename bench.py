@@ -289,6 +289,12 @@ def oracle_sample(g, V, threads, log):
                                               "estimate_s": est1}}
 
 
+# Settling runs at the end of the training protocol: each costs one
+# decomposition (about 0.2 s at R-MAT 24), and more of them give the plan
+# memory more measured plans to keep the fastest of.
+SETTLING_RUNS = 8
+
+
 def train_learned_plan(dcx, ctx, g, log, Xs=None, Ts=None):
     """The cost model's training protocol on graph g (untimed): the pass log
     of the fixed rule's last run (already decomposed by the caller), four
@@ -296,9 +302,10 @@ def train_learned_plan(dcx, ctx, g, log, Xs=None, Ts=None):
     32, 4 and 64, so the samples cover narrow and whole-XCD teams, and
     derived from row 0 with team cap 64),
     then two refinement rounds in which the model plans a run and is
-    retrained with that run's log, and three settling runs of the final
-    model's plan (each run's measured row times calibrate the next plan's
-    predictions).  Returns the attached model, the sample count and the
+    retrained with that run's log, and SETTLING_RUNS settling runs of the
+    final model (each run's measured row times calibrate the next plan's
+    predictions; the graph keeps the fastest plan measured so far and tries
+    another only when it is predicted at least 3 % faster).  Returns the attached model, the sample count and the
     refinement and settling runs' times."""
     Xs = list(Xs or [])
     Ts = list(Ts or [])
@@ -335,7 +342,7 @@ def train_learned_plan(dcx, ctx, g, log, Xs=None, Ts=None):
             g.decompose(dcx.VARIANT_BATCHED)
             refine_ms.append(round(ctx.timing().total_ms, 2))
             collect()
-    for _ in range(3):
+    for _ in range(SETTLING_RUNS):
         g.decompose(dcx.VARIANT_BATCHED)
         refine_ms.append(round(ctx.timing().total_ms, 2))
     log(f"refinement and settling runs: {refine_ms} ms")
@@ -487,8 +494,9 @@ def run_headline(args):
                      "training": f"{n_samples} samples: this graph's logged team rows under the "
                                  "fixed rule, four exploration runs (every 8th row an anchor: "
                                  "peeled with team caps 32, 4 and 64, derived from row 0 with "
-                                 "cap 64), two refinement rounds planned by the model and three "
-                                 "settling runs (measured row times calibrate the predictions)" + (
+                                 "cap 64), two refinement rounds planned by the model and "
+                                 f"{SETTLING_RUNS} settling runs (measured row times calibrate "
+                                 "the predictions)" + (
                                      ", and smaller graphs' at several team caps"
                                      if args.full_train else ""),
                      "refinement_and_settling_runs_ms": refine_ms,
